@@ -1,0 +1,149 @@
+/*
+ * oracle.h -- FP64 CPU restatement of nrCPS's auxiliary-field propagator and
+ * Slater-determinant correlator path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product path.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library never links or calls it.
+ *
+ * Conventions (all follow the reference):
+ *   - complex data is interleaved (re, im) doubles;
+ *   - site index i = z + Z*(y + Y*x), z fastest     (src/utils/vector3.C:175-187)
+ *   - sign -1 = FFTW_FORWARD ("k -> x"), sign +1 = FFTW_BACKWARD ("x -> k"),
+ *     both unnormalised                              (src/fourier/noarch/fourier.C:29-35)
+ *   - enum values match include/enum.h.
+ *
+ * Parity pins for every function are listed in tests/test_oracle.py.
+ */
+#ifndef NRCPS_ORACLE_H
+#define NRCPS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums (values as in include/enum.h of the reference) ---- */
+enum { ORC_DISP_STANDARD = 0, ORC_DISP_QUADRATIC = 1, ORC_DISP_PERFECT = 2 };
+enum { ORC_CUTOFF_NONE = 0, ORC_CUTOFF_HARD = 1 };
+enum { ORC_INTER_NONE = 0, ORC_INTER_XISQMINUSONE = 1, ORC_INTER_ONEMINUSXIINVSQ = 2 };
+
+/* ---- FFT ---- */
+/* In-place unnormalised 3D C2C DFT of `batch` contiguous volumes [x][y][z]. */
+int orc_fft3d(double* buf, int lx, int ly, int lz, int batch, int sign);
+/* Strided 1D DFT of `count` transforms of length n (for tests). */
+int orc_fft1d(double* buf, int n, int count, int sign);
+
+/* ---- tables ---- */
+/* Dispersion(type, mass, cutoff) (src/alg/dispersion.C:11-73). bc: 0 = PRD, 1 = APRD per axis. */
+void orc_dispersion(double* out, int lx, int ly, int lz, const int bc[3], int type,
+                    double mass, int cutoff_type, double cutoff_sq);
+/* Kinetic xi(k) = 1 + Dispersion(type, mass/dt, HARD)  (src/hamiltonian/hamiltonian.C:260-281). */
+void orc_kinetic_xi(double* out, int lx, int ly, int lz, const int bc[3], int type,
+                    double mass, double dt, double cutoff_sq);
+/* exp(-tau*eps(k)), eps = (1/2M) sum_i 4 sin^2(k_i/2), periodic lattice momenta. */
+void orc_kinetic_exp(double* out, int lx, int ly, int lz, double tau, double mass);
+/* Interaction form factor f(k) (src/hamiltonian/hamiltonian.C:77-147).  Returns 0, or -1 if
+ * the Taylor series is negative somewhere (the reference aborts there). */
+int orc_interaction_table(double* out, int lx, int ly, int lz, const int bc[3], double mass,
+                          int itype, const double* couplings, int n_couplings, double dt,
+                          double cutoff_sq);
+/* Interaction::Initialize: fld = (1 + F_-[f * F_+[lat]]) / V, lat interleaved complex
+ * (src/hamiltonian/noarch/hamiltonian_initialize.C:14-47). */
+int orc_interaction_field(double* fld, const double* lat, const double* f, int lx, int ly, int lz);
+
+/* ---- momenta ---- */
+/* Momenta(arg) (src/alg/momenta.C:12-127): momenta with log(1+eps) <= fermi_energy, sorted
+ * by energy with the reference's insertion sort.  Writes up to max_count triples into
+ * moms (count x 3), the opposite-parity index of each into parity (may be NULL) and the
+ * energies into omega (may be NULL).  Returns the count, or -1 if an opposite-parity
+ * partner is missing, or -2 if nothing is below the Fermi level. */
+int orc_momenta(int lx, int ly, int lz, const int bc[3], int disp_type, double mass,
+                double fermi_energy, double cutoff_sq, int* moms, int* parity, double* omega,
+                int max_count);
+/* The N lowest periodic lattice momenta by |n|^2 (n folded into (-L/2, L/2]); ties keep
+ * the enumeration order n1, n2, n3 in [0, L).  Writes N triples in [0,L). */
+int orc_lowest_momenta(int L, int n, int* moms);
+
+/* ---- random numbers ---- */
+typedef struct {
+  int x[96];      /* 96 24-bit numbers: pair k, half h, lane l at 8k + 4h + l */
+  int carry[4];
+  int pr, prm, ir, jr, is, is_old, level;
+} orc_ranlxs;
+/* Luscher's ranlxs (luxury level 0/1/2) as in src/random/ranlxs.C:49-206. */
+int   orc_ranlxs_init(orc_ranlxs* r, int level, long seed);
+float orc_ranlxs_next(orc_ranlxs* r);
+void  orc_ranlxs_get_state(const orc_ranlxs* r, long state[105]);
+int   orc_ranlxs_set_state(orc_ranlxs* r, const long state[105]);
+/* Lattice::Refresh() for FIELD_TYPE_ZTWO: phi_i = 2*floor(2u)-1 for i = 0..V-1 in site
+ * order (src/alg/lattice.C:81-86; src/random/random.C:129-132).  Output int8. */
+void  orc_ranlxs_z2_field(orc_ranlxs* r, int vol, int8_t* phi);
+
+/* Philox4x32-10 (Salmon et al., SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Counter-based Z2 field of one (config, slice): packed bits, word w of 32 sites
+ * (site 32w + j <-> bit j), from Philox block g = w/4 with counter (g, t, config, stream)
+ * and key (seed lo, seed hi); word w = out[w % 4].  phi = +1 where the bit is set. */
+void orc_z2_bits(uint64_t seed, uint32_t config, uint32_t t, uint32_t stream, int vol,
+                 uint32_t* words);
+void orc_bits_to_phi(const uint32_t* words, int vol, int8_t* phi);
+
+/* ---- linear algebra ---- */
+/* Crout LU with implicit scaled partial pivoting, last maximal row wins (src/utils/ludcmp.C:9-65),
+ * on contiguous row-major complex storage.  Returns 0, or -1 for an all-zero row. */
+int orc_ludcmp(double* a, int n, int* indx, double* d);
+/* From an LU: log|det| = sum log|u_ii|; phase = d * prod u_ii/|u_ii| (unit complex);
+ * argsum = arg(d) + sum arg(u_ii) (the imaginary part of the reference's DET_TYPE_LOG);
+ * det = d * prod u_ii (DET_TYPE_STANDARD, src/slater_det/slater_det.C:93-109). */
+void orc_lu_det(const double* lu, int n, double d, double* logabs, double phase[2],
+                double* argsum, double det[2]);
+/* M[s][t] = sum_i conj(phi_s[i]) * psi_t[i] (src/one_body/noarch/one_body_project.C:14-27;
+ * src/slater_det/slater_det.C:33-37).  phi: ns x vol, psi: nt x vol, M: ns x nt. */
+void orc_overlap(const double* phi, const double* psi, double* M, int ns, int nt, long vol);
+
+/* ---- the correlator path ---- */
+enum { ORC_SPLIT_REFERENCE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
+       ORC_SPLIT_BASELINE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */
+enum { ORC_FIELD_PHILOX = 0, ORC_FIELD_RANLXS = 1, ORC_FIELD_ARRAY = 2 };
+
+typedef struct {
+  int L;               /* cubic lattice side */
+  int N;               /* orbitals (= sinks = sources) per spin */
+  int T;               /* time slices */
+  int split;           /* ORC_SPLIT_* */
+  int minors;          /* 0: det of the full N x N matrix; 1: all leading minors n = 1..N */
+  const int* moms;     /* N momenta (n1, n2, n3) in [0, L): sources and sinks */
+  const double* kin;   /* V real: 1/xi_half (reference) or exp(-tau eps) (baseline) */
+  double A;            /* field strength: multiplier (1 + A phi) */
+  const double* smear; /* optional V real form factor f(k); when set, the reference field is
+                          Interaction::Initialize's (1 + F-[f F+ phi])/V and A is unused */
+  int field_kind;      /* ORC_FIELD_* */
+  uint64_t seed;       /* PHILOX */
+  uint32_t config;     /* PHILOX config index */
+  orc_ranlxs* rng;     /* RANLXS: consumed sequentially, T*V draws */
+  const int8_t* phi;   /* ARRAY: T x V values in {-1, +1} */
+} orc_run_desc;
+
+/* Runs one configuration through T slices, measuring before evolving at each t
+ * (production/slater_det/main.C:185-220).  Outputs, for each t and each measured minor m
+ * (m = 0 only, or m = 0..N-1 for minor size m+1), at [t*nm + m]:
+ *   logabs, phase (2 doubles), argsum.  M_out (optional): T x N x N complex overlaps.
+ * psi_out (optional): final N x V orbitals in storage space. */
+int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double* argsum,
+                   double* M_out, double* psi_out);
+
+/* Exact E[S(t)^2] for a single k=0 orbital with iid Z2 fields:
+ * g_0 = 1/V, g <- F^-1[kb^2 F[(1 + A^2 delta_r0) F^-1[ka^2 F g]]], C(t) = sum_r g_t(r).
+ * ka / kb are the kinetic multipliers before / after the field (V real, even in k). */
+int orc_exact_c1(int L, int T, const double* ka, const double* kb, double A, double* C);
+
+/* Wall time of `nthreads` independent configuration streams, each running `slices` slices
+ * of configuration `d` (config index offset per thread), in seconds. */
+double orc_time_threads(const orc_run_desc* d, int nthreads, int slices);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
