@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Throughput of the auxiliary-field correlator path on one MI355X.
+
+Workload: L=32 lattice (V = 32768), N=33 plane-wave orbitals per spin (the lowest 33
+momenta), T=128 slices, tau=0.05, M=1, A=0.4, uniform Z2 fields from Philox, FP32
+evolution (FFT + kinetic + field multiplies), FP64 overlaps and LU log-determinants.
+A step is one pass of the production slice loop (measure, then evolve; see
+production/slater_det/main.C:185-220) over a batch of configurations, i.e. `configs x T`
+configuration-slices.  Inputs are generated on the device; the timed region includes
+returning the per-configuration log|det| and phase to the host.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def _tables(L, tau, M):
+    """xi_half = 1 + dt/2 * eps(k), eps = (1/2M) sum 4 sin^2(k/2): the reference's Kinetic
+    (hamiltonian.C:260-281) with mass M/tau and dt = 1/2, STANDARD dispersion."""
+    n = np.arange(L)
+    s = np.sin(np.pi * n / L) ** 2
+    eps = 2.0 * (s[:, None, None] + s[None, :, None] + s[None, None, :]) / M
+    return (1.0 + 0.5 * tau * eps).reshape(-1)
+
+
+def _lowest(L, N):
+    f = np.where(np.arange(L) > L // 2, np.arange(L) - L, np.arange(L))
+    pts = [(a, b, c) for a in range(L) for b in range(L) for c in range(L)]
+    q = [f[a] ** 2 + f[b] ** 2 + f[c] ** 2 for a, b, c in pts]
+    order = sorted(range(len(pts)), key=lambda i: q[i])  # stable: ties keep enumeration order
+    return np.array([pts[i] for i in order[:N]], dtype=np.int32)
+
+
+def cpu_baseline(L, N, tau, A, slices, threads):
+    """The FP64 oracle (CPU restatement of the reference algorithm: batched FFTs, N^2
+    full-length overlaps, NR LU) on `threads` host cores, one configuration stream each."""
+    import oracle
+
+    oracle.lib()
+    moms = _lowest(L, N)
+    kin = _tables(L, tau, 1.0)
+    secs = oracle.time_threads(L, N, slices, moms, kin, A, nthreads=threads, config=1 << 20)
+    if secs <= 0:
+        return None
+    return {"value": threads * slices / secs, "unit": "configs*slices/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{threads} threads x 1 config x {slices} slices at L={L}, N={N}, FP64; "
+                      f"{secs:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--configs", type=int, default=1024)
+    ap.add_argument("--T", type=int, default=128)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--N", type=int, default=33)
+    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nrcps_amd as na
+
+    L, N, T, C = args.L, args.N, args.T, args.configs
+    tau, A = 0.05, 0.4
+    V = L ** 3
+    ctx = na.Context(0)
+    moms = _lowest(L, N)
+    kin = _tables(L, tau, 1.0)
+    ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_REFERENCE)
+    seed = 7012
+
+    for w in range(args.warmup):
+        ens.run(seed, (1 << 24) + w * C, T)
+        print(f"warmup step {w} done", file=sys.stderr, flush=True)
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        la, ph, _ = ens.run(seed, k * C, T)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    cs_per_s = C * T * args.steps / dt
+    assert np.all(np.isfinite(la)), "non-finite log|det|"
+    print(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel durations from HIP events on the ensemble's stream (one extra, untimed step)
+    ens.profile(True)
+    ens.run(seed, 1 << 23, T)
+    tx, nx = ens.kernel_time(0)
+    tyz, nyz = ens.kernel_time(1)
+    tlu, nlu = ens.kernel_time(2)
+    ens.profile(False)
+    # both passes together evolve one slice of every configuration; algorithmic bytes per
+    # configuration-slice: read + write the N FP32 orbitals once plus phi as one byte/site
+    b_evo = N * 16 * V + V
+    slice_ms = tx / nx + tyz / nyz
+    achieved = b_evo * C / (slice_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "xpass+yzpass (one slice)", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "avg_ms": {"xpass": tx / nx, "yzpass": tyz / nyz,
+                                            "lu": tlu / max(nlu, 1)},
+                "bytes_per_cs": b_evo}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(L, N, tau, A, args.cpu_slices, threads)
+
+    out = {
+        "metric": "aux-field configs*time-slices/sec at L=32,N=33",
+        "value": cs_per_s, "unit": "configs*slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"correlator path, config 3 shape: L={L}, N={N}, T={T}, "
+                               f"tau={tau}, A={A}, {C} configs per step, FP32 evolution, "
+                               f"FP64 LU", "L": L, "N": N, "T": T, "configs": C,
+                   "parallelism": "single GPU, batched configurations"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = cs_per_s / cpu["value"]
+    print(json.dumps(out))
+    ens.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/*
+ * nrcps.h -- C ABI of the MI355X (gfx950) implementation of nrCPS's auxiliary-field
+ * propagator and Slater-determinant correlator path.
+ *
+ * Every entry point returns an int status: NRC_OK (0) or a negative NRC_ERR_* code, and
+ * never exits the process (the reference's ERR.General terminates; the C++ class layer in
+ * include/nrcps/ turns non-zero codes back into that behaviour).  Pointers named *_dev are
+ * device memory obtained from nrc_malloc; everything else is host memory.
+ *
+ * Data conventions (the reference's): complex values are interleaved (re, im); a volume
+ * is [x][y][z] with z fastest (src/utils/vector3.C:175-187); a batch of volumes is
+ * contiguous with stride V = L^3 (src/fourier/fourier_common.C:16); sign -1 is
+ * FFTW_FORWARD and +1 FFTW_BACKWARD, both unnormalised (src/fourier/noarch/fourier.C:29-35).
+ */
+#ifndef NRCPS_H
+#define NRCPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRC_OK 0
+#define NRC_ERR_ARG (-1)         /* invalid argument */
+#define NRC_ERR_UNSUPPORTED (-2) /* shape or mode this build does not implement */
+#define NRC_ERR_HIP (-3)         /* HIP runtime failure */
+#define NRC_ERR_NOMEM (-4)       /* device allocation failed */
+#define NRC_ERR_SINGULAR (-5)    /* an all-zero matrix row (ludcmp.C:25-28 exits there) */
+
+#define NRC_PREC_F32 0
+#define NRC_PREC_F64 1
+
+#define NRC_SPLIT_REFERENCE 0 /* k-space storage; per slice K(1/2) I K(1/2), field (1+A phi)/V */
+#define NRC_SPLIT_BASELINE 1  /* x-space storage; psi <- F+[kappa F-[psi]]/V, then (1 + A phi) */
+
+#define NRC_OVERLAP_F64 0    /* dense overlaps in FP64 */
+#define NRC_OVERLAP_F32 1    /* dense overlaps of FP32 data, FP64 accumulation */
+#define NRC_OVERLAP_BF16X3 2 /* split-BF16x3 MFMA products, FP32 accumulation */
+
+typedef struct nrc_context nrc_context;
+typedef struct nrc_fft nrc_fft;
+typedef struct nrc_ensemble nrc_ensemble;
+
+/* ---- context, memory: Cuda::Malloc / Free / MemCopy / DeviceQ (include/cuda_utils.h:10-15) */
+int nrc_init(int device, nrc_context** ctx);
+int nrc_finalize(nrc_context* ctx);
+const char* nrc_error_string(int code);
+int nrc_device_name(nrc_context* ctx, char* buf, int len, int* n_cu);
+int nrc_malloc(nrc_context* ctx, void** p_dev, size_t bytes);
+int nrc_free(nrc_context* ctx, void* p_dev);
+int nrc_memcpy_h2d(nrc_context* ctx, void* dst_dev, const void* src, size_t bytes);
+int nrc_memcpy_d2h(nrc_context* ctx, void* dst, const void* src_dev, size_t bytes);
+int nrc_memcpy_d2d(nrc_context* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+int nrc_memset(nrc_context* ctx, void* p_dev, int byte, size_t bytes);
+int nrc_sync(nrc_context* ctx);
+
+/* ---- FFT: Fourier(batch) / Forward / Backward (include/fourier.h:7-45;
+ *      src/fourier/cuda/fourier.cu:9-79).  In-place batched 3D C2C on cubic lattices
+ *      L in {4, 6, 8, 12, 16, 24, 32}. */
+int nrc_fft_plan(nrc_context* ctx, int lx, int ly, int lz, int batch, int prec, nrc_fft** plan);
+int nrc_fft_exec(nrc_fft* plan, void* buf_dev, int sign);
+int nrc_fft_destroy(nrc_fft* plan);
+
+/* ---- element-wise operators
+ * nrc_scale_real: buf[b][i] *= table[i] (op 0) or /= table[i] (op 1) -- Kinetic::Evolve
+ *   (src/hamiltonian/cuda/hamiltonian_evolve.cu:90-104, hamiltonian_kernels.cu:17-42).
+ * nrc_mul_complex: buf[b][i] *= field[i] -- the multiply inside Interaction::Evolve
+ *   (hamiltonian_evolve.cu:42-66, hamiltonian_kernels.cu:44-57). */
+int nrc_scale_real(nrc_context* ctx, void* buf_dev, const void* table_dev, int batch, long vol,
+                   int op, int prec);
+int nrc_mul_complex(nrc_context* ctx, void* buf_dev, const void* field_dev, int batch, long vol,
+                    int prec);
+
+/* ---- overlaps: OneBody::Project, Cuda::InnerProduct and the SlaterDet constructor
+ *      (src/one_body/cuda/one_body_project.cu:17-34; src/utils/cuda/cuda_utils.cu:88-127;
+ *      src/slater_det/slater_det.C:25-39).  M_dev[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i]
+ *      for ncfg batches of nt orbitals (stride vol), complex FP64 output. */
+int nrc_overlap(nrc_context* ctx, const void* phi_dev, const void* psi_dev, double* M_dev,
+                int ns, int nt, long vol, int ncfg, int mode);
+
+/* ---- batched LU log-determinant: ludcmp (include/ludcmp.h:5; src/utils/ludcmp.C:9-65) and
+ *      SlaterDet::Run's det / log det (src/slater_det/slater_det.C:59-115).  For each of
+ *      ncfg row-major n x n complex matrices (row stride ldm) and, if minors, each leading
+ *      minor m = 1..n: log|det|, det/|det| (2 doubles) and the reference's DET_TYPE_LOG
+ *      imaginary part.  Outputs [c][m] on the device; status_dev[c][m] = 1 for a zero row. */
+int nrc_lu_det(nrc_context* ctx, const double* M_dev, int n, int ldm, int ncfg, int minors,
+               double* logabs_dev, double* phase_dev, double* argsum_dev, int* status_dev);
+
+/* ---- ludcmp itself, batched (include/ludcmp.h:5; src/utils/ludcmp.C:9-65): a_dev holds ncfg
+ *      contiguous row-major n x n complex matrices, factored in place; indx_dev[c][n] and
+ *      d_dev[c] as the reference returns them.  The arithmetic is the reference's, operation
+ *      for operation, so pivots and factors are bit-identical to the host build. */
+int nrc_ludcmp(nrc_context* ctx, double* a_dev, int n, int ncfg, int* indx_dev, double* d_dev,
+               int* status_dev);
+/* |z| of n complex values as std::abs(complex<double>) computes it on the host (the modulus
+ * ludcmp pivots with, ludcmp.C:24,43). */
+int nrc_ref_abs(nrc_context* ctx, const double* z_dev, double* out_dev, long n);
+
+/* ---- auxiliary field: Lattice::Refresh with FIELD_TYPE_ZTWO (src/alg/lattice.C:60-109),
+ *      counter-based.  words_dev[c][w], w < ceil(vol/32): Philox4x32-10 block w/4 with
+ *      counter (w/4, t, config0 + c, stream), key (seed lo, seed hi); site 32w+j is +1 where
+ *      bit j is set, else -1. */
+int nrc_z2_bits(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                uint32_t stream, int vol, uint32_t* words_dev);
+
+/* ---- ensemble engine: the production slice loop (production/slater_det/main.C:185-220)
+ *      for a batch of configurations at once: measure (overlaps of plane-wave sinks, LU),
+ *      then evolve one slice (Interaction::Initialize + Propagator::Run with [K, I, K]).
+ *      Orbitals, fields and overlap matrices stay on the device. */
+typedef struct nrc_ensemble_desc {
+  int L;             /* cubic lattice side */
+  int n_orb;         /* N orbitals per configuration; sources = sinks (<= 64) */
+  int n_configs;     /* configurations evolved together */
+  int prec;          /* NRC_PREC_F32 / NRC_PREC_F64 for orbitals and FFTs */
+  int split;         /* NRC_SPLIT_REFERENCE / NRC_SPLIT_BASELINE */
+  int minors;        /* 0: det of the full N x N matrix; 1: every leading minor */
+  const int* moms;   /* n_orb x 3 plane-wave momenta (n1, n2, n3) in [0, L) */
+  const double* kin; /* V: reference split xi_half (psi /= xi); baseline exp(-tau eps) */
+  double A;          /* field multiplier (1 + A phi) */
+  const double* smear; /* optional V form factor f(k) (reference split): the field becomes
+                          Interaction::Initialize's (1 + F-[f F+ phi])/V */
+} nrc_ensemble_desc;
+
+int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* desc, nrc_ensemble** ens);
+/* Fields from Philox(seed) for configs config0 .. config0 + n_configs - 1.  Host outputs
+ * (any may be NULL), indexed [t][c][m]: logabs, phase (2 doubles), argsum. */
+int nrc_ensemble_run(nrc_ensemble* ens, uint64_t seed, uint32_t config0, int T, double* logabs,
+                     double* phase, double* argsum);
+/* Same with caller-supplied fields: words[t][c][ceil(V/32)] (host), e.g. from ranlxs. */
+int nrc_ensemble_run_fields(nrc_ensemble* ens, const uint32_t* words, int T, double* logabs,
+                            double* phase, double* argsum);
+/* Orbitals of configuration c after the last run, N x V complex in the ensemble precision,
+ * in storage space (k-space for the reference split). */
+int nrc_ensemble_orbitals(nrc_ensemble* ens, int c, void* out);
+/* Per-kernel timing with HIP events on the ensemble's stream (enable before a run).
+ * kind: 0 X pass, 1 YZ pass, 2 LU, 3 field generation.  Returns total ms and launches. */
+int nrc_ensemble_profile(nrc_ensemble* ens, int enable);
+int nrc_ensemble_kernel_time(nrc_ensemble* ens, int kind, double* total_ms, long* launches);
+int nrc_ensemble_destroy(nrc_ensemble* ens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

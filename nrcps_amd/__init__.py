@@ -1,0 +1,275 @@
+"""nrcps_amd -- MI355X (gfx950) implementation of nrCPS's auxiliary-field propagator and
+Slater-determinant correlator path.
+
+This module is a thin ctypes binding of the C ABI in include/nrcps.h (libnrcps_hip.so).
+It never falls back to a CPU path: if the HIP library is missing or no GPU is present,
+`Context()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libnrcps_hip.so")
+
+OK = 0
+PREC_F32, PREC_F64 = 0, 1
+SPLIT_REFERENCE, SPLIT_BASELINE = 0, 1
+OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3 = 0, 1, 2
+SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
+
+# every symbol include/nrcps.h declares
+EXPORTS = [
+    "nrc_init", "nrc_finalize", "nrc_error_string", "nrc_device_name", "nrc_malloc", "nrc_free",
+    "nrc_memcpy_h2d", "nrc_memcpy_d2h", "nrc_memcpy_d2d", "nrc_memset", "nrc_sync",
+    "nrc_fft_plan", "nrc_fft_exec", "nrc_fft_destroy", "nrc_scale_real", "nrc_mul_complex",
+    "nrc_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
+    "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_profile",
+    "nrc_ensemble_kernel_time", "nrc_ensemble_destroy",
+]
+
+
+class NrcError(RuntimeError):
+    pass
+
+
+class EnsembleDesc(C.Structure):
+    _fields_ = [("L", C.c_int), ("n_orb", C.c_int), ("n_configs", C.c_int), ("prec", C.c_int),
+                ("split", C.c_int), ("minors", C.c_int), ("moms", C.c_void_p),
+                ("kin", C.c_void_p), ("A", C.c_double), ("smear", C.c_void_p)]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libnrcps_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NrcError(f"{LIB_PATH} is missing: run `python -m nrcps_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i, u32, u64, dp = C.c_void_p, C.c_size_t, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
+    sig = {
+        "nrc_init": [i, C.POINTER(vp)], "nrc_finalize": [vp], "nrc_error_string": [i],
+        "nrc_device_name": [vp, C.c_char_p, i, C.POINTER(i)],
+        "nrc_malloc": [vp, C.POINTER(vp), sz], "nrc_free": [vp, vp],
+        "nrc_memcpy_h2d": [vp, vp, vp, sz], "nrc_memcpy_d2h": [vp, vp, vp, sz],
+        "nrc_memcpy_d2d": [vp, vp, vp, sz], "nrc_memset": [vp, vp, i, sz], "nrc_sync": [vp],
+        "nrc_fft_plan": [vp, i, i, i, i, i, C.POINTER(vp)], "nrc_fft_exec": [vp, vp, i],
+        "nrc_fft_destroy": [vp],
+        "nrc_scale_real": [vp, vp, vp, i, C.c_long, i, i],
+        "nrc_mul_complex": [vp, vp, vp, i, C.c_long, i],
+        "nrc_overlap": [vp, vp, vp, vp, i, i, C.c_long, i, i],
+        "nrc_lu_det": [vp, vp, i, i, i, i, vp, vp, vp, vp],
+        "nrc_ludcmp": [vp, vp, i, i, vp, vp, vp], "nrc_ref_abs": [vp, vp, vp, C.c_long],
+        "nrc_z2_bits": [vp, u64, u32, i, u32, u32, i, vp],
+        "nrc_ensemble_create": [vp, C.POINTER(EnsembleDesc), C.POINTER(vp)],
+        "nrc_ensemble_run": [vp, u64, u32, i, dp, dp, dp],
+        "nrc_ensemble_run_fields": [vp, vp, i, dp, dp, dp],
+        "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_profile": [vp, i],
+        "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
+        "nrc_ensemble_destroy": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_char_p if name == "nrc_error_string" else C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != OK:
+        msg = load_library().nrc_error_string(rc).decode()
+        raise NrcError(f"{what} failed: {msg} ({rc})")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """A device allocation with a shape/dtype, owned by a Context."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(ctx.lib.nrc_malloc(ctx.h, C.byref(p), self.nbytes), "nrc_malloc")
+        self.ptr = p
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        _check(self.ctx.lib.nrc_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes), "h2d")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx.lib.nrc_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr and getattr(self.ctx, "h", None):
+            self.ctx.lib.nrc_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU: a HIP stream and the twiddle tables (nrc_init)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.nrc_init(device, C.byref(h)), "nrc_init")
+        self.h = h
+        buf = C.create_string_buffer(256)
+        ncu = C.c_int()
+        self.lib.nrc_device_name(self.h, buf, 256, C.byref(ncu))
+        self.arch, self.n_cu = buf.value.decode(), ncu.value
+
+    def array(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def sync(self):
+        _check(self.lib.nrc_sync(self.h), "nrc_sync")
+
+    # ---------------------------------------------------------------- operators
+    def fft3d(self, buf: DeviceArray, L: int, batch: int, sign: int, prec: int):
+        plan = C.c_void_p()
+        _check(self.lib.nrc_fft_plan(self.h, L, L, L, batch, prec, C.byref(plan)), "fft_plan")
+        try:
+            _check(self.lib.nrc_fft_exec(plan, buf.ptr, sign), "fft_exec")
+        finally:
+            self.lib.nrc_fft_destroy(plan)
+
+    def scale_real(self, buf, table, batch, vol, op, prec):
+        _check(self.lib.nrc_scale_real(self.h, buf.ptr, table.ptr, batch, vol, op, prec),
+               "scale_real")
+
+    def mul_complex(self, buf, field, batch, vol, prec):
+        _check(self.lib.nrc_mul_complex(self.h, buf.ptr, field.ptr, batch, vol, prec),
+               "mul_complex")
+
+    def overlap(self, phi, psi, M, ns, nt, vol, ncfg, mode):
+        _check(self.lib.nrc_overlap(self.h, phi.ptr, psi.ptr, M.ptr, ns, nt, vol, ncfg, mode),
+               "overlap")
+
+    def lu_det(self, M, n, ncfg, minors=False, ldm=None):
+        nm = n if minors else 1
+        la = self.array((ncfg, nm), np.float64)
+        ph = self.array((ncfg, nm), np.complex128)
+        ag = self.array((ncfg, nm), np.float64)
+        st = self.array((ncfg, nm), np.int32)
+        _check(self.lib.nrc_lu_det(self.h, M.ptr, n, ldm or n, ncfg, int(minors), la.ptr,
+                                   ph.ptr, ag.ptr, st.ptr), "lu_det")
+        return la.download(), ph.download(), ag.download(), st.download()
+
+    def ludcmp(self, a: np.ndarray):
+        """Batched ludcmp of contiguous complex matrices a[c][n][n]: (lu, indx, d, status)."""
+        ncfg, n, _ = a.shape
+        d_a = self.to_device(np.ascontiguousarray(a, np.complex128))
+        indx = self.array((ncfg, n), np.int32)
+        d = self.array((ncfg,), np.float64)
+        st = self.array((ncfg,), np.int32)
+        _check(self.lib.nrc_ludcmp(self.h, d_a.ptr, n, ncfg, indx.ptr, d.ptr, st.ptr), "ludcmp")
+        return d_a.download(), indx.download(), d.download(), st.download()
+
+    def ref_abs(self, z: np.ndarray) -> np.ndarray:
+        z = np.ascontiguousarray(z, np.complex128).reshape(-1)
+        out = self.array(z.shape, np.float64)
+        _check(self.lib.nrc_ref_abs(self.h, self.to_device(z).ptr, out.ptr, z.size), "ref_abs")
+        return out.download()
+
+    def z2_bits(self, seed, config0, nconf, t, vol, stream=0) -> np.ndarray:
+        nw = (vol + 31) // 32
+        w = self.array((nconf, nw), np.uint32)
+        _check(self.lib.nrc_z2_bits(self.h, seed, config0, nconf, t, stream, vol, w.ptr),
+               "z2_bits")
+        return w.download()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nrc_finalize(self.h)
+            self.h = None
+
+
+class Ensemble:
+    """The batched production slice loop (nrc_ensemble_*)."""
+
+    def __init__(self, ctx: Context, L, moms, kin, A, n_configs, prec=PREC_F32,
+                 split=SPLIT_REFERENCE, minors=False, smear=None):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.L, self.N, self.C, self.prec = L, len(moms), n_configs, prec
+        self.nm = self.N if minors else 1
+        self._moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)
+        self._kin = np.ascontiguousarray(kin, dtype=np.float64)
+        d = EnsembleDesc()
+        d.L, d.n_orb, d.n_configs, d.prec, d.split, d.minors = (L, self.N, n_configs, prec, split,
+                                                               int(minors))
+        d.moms, d.kin, d.A = self._moms.ctypes.data, self._kin.ctypes.data, A
+        self._smear = None
+        if smear is not None:
+            self._smear = np.ascontiguousarray(smear, dtype=np.float64)
+            d.smear = self._smear.ctypes.data
+        h = C.c_void_p()
+        _check(self.lib.nrc_ensemble_create(ctx.h, C.byref(d), C.byref(h)), "ensemble_create")
+        self.h = h
+
+    def _outs(self, T):
+        return (np.zeros((T, self.C, self.nm)), np.zeros((T, self.C, self.nm), np.complex128),
+                np.zeros((T, self.C, self.nm)))
+
+    def run(self, seed: int, config0: int, T: int):
+        la, ph, ag = self._outs(T)
+        _check(self.lib.nrc_ensemble_run(self.h, seed, config0, T, _ptr(la), _ptr(ph), _ptr(ag)),
+               "ensemble_run")
+        return la, ph, ag
+
+    def run_fields(self, words: np.ndarray, T: int):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        la, ph, ag = self._outs(T)
+        _check(self.lib.nrc_ensemble_run_fields(self.h, _ptr(w), T, _ptr(la), _ptr(ph),
+                                                _ptr(ag)), "ensemble_run_fields")
+        return la, ph, ag
+
+    def orbitals(self, c: int) -> np.ndarray:
+        dt = np.complex64 if self.prec == PREC_F32 else np.complex128
+        out = np.zeros((self.N, self.L ** 3), dtype=dt)
+        _check(self.lib.nrc_ensemble_orbitals(self.h, c, _ptr(out)), "ensemble_orbitals")
+        return out
+
+    def profile(self, enable: bool):
+        _check(self.lib.nrc_ensemble_profile(self.h, int(enable)), "ensemble_profile")
+
+    def kernel_time(self, kind: int):
+        ms, n = C.c_double(), C.c_long()
+        _check(self.lib.nrc_ensemble_kernel_time(self.h, kind, C.byref(ms), C.byref(n)),
+               "kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        # an ensemble lives on its context's device; after nrc_finalize it is gone with it
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.lib.nrc_ensemble_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

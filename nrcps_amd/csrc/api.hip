@@ -1,0 +1,684 @@
+// api.hip -- the extern "C" entry points of libnrcps_hip.so (declared in include/nrcps.h).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "nrc_device.h"
+#include "nrc_internal.h"
+
+struct nrc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char name[256] = {0};
+  int n_cu = 0;
+};
+
+struct nrc_fft {
+  nrc_context* ctx;
+  int L, batch, prec;
+};
+
+#define NRC_HIP(expr)                                  \
+  do {                                                 \
+    hipError_t e_ = (expr);                            \
+    if (e_ != hipSuccess) return NRC_ERR_HIP;          \
+  } while (0)
+
+#define NRC_TRY(expr)                                  \
+  do {                                                 \
+    int r_ = (expr);                                   \
+    if (r_ != NRC_OK) return r_;                       \
+  } while (0)
+
+static size_t elem_bytes(int prec) { return prec == NRC_PREC_F64 ? 16 : 8; }
+static bool prec_ok(int prec) { return prec == NRC_PREC_F32 || prec == NRC_PREC_F64; }
+
+extern "C" {
+
+const char* nrc_error_string(int code) {
+  switch (code) {
+    case NRC_OK: return "ok";
+    case NRC_ERR_ARG: return "invalid argument";
+    case NRC_ERR_UNSUPPORTED: return "unsupported shape or mode";
+    case NRC_ERR_HIP: return "HIP runtime error";
+    case NRC_ERR_NOMEM: return "device allocation failed";
+    case NRC_ERR_SINGULAR: return "singular matrix (all-zero row)";
+    default: return "unknown error";
+  }
+}
+
+int nrc_init(int device, nrc_context** ctx) {
+  if (!ctx) return NRC_ERR_ARG;
+  int n = 0;
+  NRC_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return NRC_ERR_ARG;
+  NRC_HIP(hipSetDevice(device));
+  nrc_context* c = new nrc_context();
+  c->device = device;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return NRC_ERR_HIP; }
+  strncpy(c->name, p.gcnArchName, sizeof(c->name) - 1);
+  c->n_cu = p.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return NRC_ERR_HIP;
+  }
+  if (nrc::upload_twiddles() != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return NRC_ERR_HIP;
+  }
+  *ctx = c;
+  return NRC_OK;
+}
+
+int nrc_finalize(nrc_context* ctx) {
+  if (!ctx) return NRC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return NRC_OK;
+}
+
+int nrc_device_name(nrc_context* ctx, char* buf, int len, int* n_cu) {
+  if (!ctx || !buf || len < 1) return NRC_ERR_ARG;
+  strncpy(buf, ctx->name, (size_t)len - 1);
+  buf[len - 1] = 0;
+  if (n_cu) *n_cu = ctx->n_cu;
+  return NRC_OK;
+}
+
+int nrc_malloc(nrc_context* ctx, void** p, size_t bytes) {
+  if (!ctx || !p) return NRC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return NRC_ERR_NOMEM;
+  return NRC_OK;
+}
+
+int nrc_free(nrc_context* ctx, void* p) {
+  if (!ctx) return NRC_ERR_ARG;
+  if (!p) return NRC_OK;
+  NRC_HIP(hipStreamSynchronize(ctx->stream));
+  NRC_HIP(hipFree(p));
+  return NRC_OK;
+}
+
+int nrc_memcpy_h2d(nrc_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return NRC_ERR_ARG;
+  NRC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  NRC_HIP(hipStreamSynchronize(ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_memcpy_d2h(nrc_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return NRC_ERR_ARG;
+  NRC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  NRC_HIP(hipStreamSynchronize(ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_memcpy_d2d(nrc_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return NRC_ERR_ARG;
+  NRC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_memset(nrc_context* ctx, void* p, int byte, size_t bytes) {
+  if (!ctx || (!p && bytes)) return NRC_ERR_ARG;
+  NRC_HIP(nrc::launch_fill(p, (uint8_t)byte, bytes, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_sync(nrc_context* ctx) {
+  if (!ctx) return NRC_ERR_ARG;
+  NRC_HIP(hipStreamSynchronize(ctx->stream));
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ FFT
+int nrc_fft_plan(nrc_context* ctx, int lx, int ly, int lz, int batch, int prec, nrc_fft** plan) {
+  if (!ctx || !plan || batch < 0 || !prec_ok(prec)) return NRC_ERR_ARG;
+  if (lx != ly || ly != lz || !nrc::pass_supported(lx)) return NRC_ERR_UNSUPPORTED;
+  *plan = new nrc_fft{ctx, lx, batch, prec};
+  return NRC_OK;
+}
+
+int nrc_fft_exec(nrc_fft* p, void* buf, int sign) {
+  if (!p || !buf || (sign != -1 && sign != 1)) return NRC_ERR_ARG;
+  // X pass: fft_a is the sign +1 x-transform, fft_b the sign -1 one; YZ pass: fft_a is the
+  // sign -1 yz-transform, fft_b the sign +1 one.
+  nrc::PassArgs x, yz;
+  x.psi = yz.psi = buf;
+  x.nvol = yz.nvol = p->batch;
+  x.L = yz.L = p->L;
+  if (sign < 0) { x.fft_b = 1; yz.fft_a = 1; } else { x.fft_a = 1; yz.fft_b = 1; }
+  NRC_HIP(nrc::launch_xpass(p->prec, x, p->ctx->stream));
+  NRC_HIP(nrc::launch_yzpass(p->prec, yz, p->ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_fft_destroy(nrc_fft* p) {
+  delete p;
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ element-wise
+int nrc_scale_real(nrc_context* ctx, void* buf, const void* table, int batch, long vol, int op,
+                   int prec) {
+  if (!ctx || !buf || !table || batch < 0 || vol < 1 || (op != 0 && op != 1) || !prec_ok(prec))
+    return NRC_ERR_ARG;
+  NRC_HIP(nrc::launch_scale_real(prec, buf, table, batch, vol, op, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_mul_complex(nrc_context* ctx, void* buf, const void* field, int batch, long vol,
+                    int prec) {
+  if (!ctx || !buf || !field || batch < 0 || vol < 1 || !prec_ok(prec)) return NRC_ERR_ARG;
+  NRC_HIP(nrc::launch_mul_complex(prec, buf, field, batch, vol, ctx->stream));
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ overlaps, LU, fields
+int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns, int nt,
+                long vol, int ncfg, int mode) {
+  if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
+  if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {
+    const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
+    NRC_HIP(nrc::launch_overlap_dense(prec, phi, psi, (double2*)M, ns, nt, vol, ncfg,
+                                      ctx->stream));
+    return NRC_OK;
+  }
+  return NRC_ERR_UNSUPPORTED;
+}
+
+int nrc_lu_det(nrc_context* ctx, const double* M, int n, int ldm, int ncfg, int minors,
+               double* logabs, double* phase, double* argsum, int* status) {
+  if (!ctx || !M || !logabs || !phase || !argsum || n < 1 || ldm < n || ncfg < 0)
+    return NRC_ERR_ARG;
+  if (n > 64) return NRC_ERR_UNSUPPORTED;
+  if (ncfg == 0) return NRC_OK;
+  NRC_HIP(nrc::launch_lu_det((const double2*)M, n, ldm, ncfg, minors, logabs, (double2*)phase,
+                             argsum, status, nullptr, nullptr, nullptr, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_ludcmp(nrc_context* ctx, double* a, int n, int ncfg, int* indx, double* d, int* status) {
+  if (!ctx || !a || !indx || !d || n < 1 || ncfg < 0) return NRC_ERR_ARG;
+  if (n > 64) return NRC_ERR_UNSUPPORTED;
+  if (ncfg == 0) return NRC_OK;
+  NRC_HIP(nrc::launch_lu_det((const double2*)a, n, n, ncfg, 0, nullptr, nullptr, nullptr, status,
+                             (double2*)a, indx, d, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_ref_abs(nrc_context* ctx, const double* z, double* out, long n) {
+  if (!ctx || !z || !out || n < 0) return NRC_ERR_ARG;
+  if (n == 0) return NRC_OK;
+  NRC_HIP(nrc::launch_ref_abs((const double2*)z, out, n, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_z2_bits(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                uint32_t stream, int vol, uint32_t* words) {
+  if (!ctx || !words || nconf < 0 || vol < 1) return NRC_ERR_ARG;
+  if (nconf == 0) return NRC_OK;
+  NRC_HIP(nrc::launch_z2_bits(seed, config0, nconf, t, stream, vol, words, ctx->stream));
+  return NRC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ensemble engine
+struct nrc_ensemble {
+  nrc_context* ctx = nullptr;
+  nrc_ensemble_desc d{};
+  long V = 0;
+  int nw = 0, nm = 1;
+  std::vector<int> moms;
+  void* psi = nullptr;       // [C][N][V] complex
+  void* tab_x = nullptr;     // reference: 1/xi_half (X pass)
+  void* tab_yz = nullptr;    // baseline: kappa/V (YZ pass)
+  void* tab_x_lo = nullptr;  // FP32 remainders of the tables above
+  void* tab_yz_lo = nullptr;
+  void* smear = nullptr;     // reference + smear: f(k)
+  void* work = nullptr;      // reference + smear: [C][V] complex scratch
+  void* rfield = nullptr;    // reference + smear: [C][V] real field
+  uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
+  size_t words_cap = 0;
+  double2* M = nullptr;      // [C][N][N]
+  double* o_la = nullptr;    // [T][C][nm]
+  double2* o_ph = nullptr;
+  double* o_as = nullptr;
+  int* o_st = nullptr;
+  size_t out_cap = 0;        // in [T][C][nm] entries
+  int* g_ptr = nullptr;
+  int* g_pos = nullptr;
+  int* g_sink = nullptr;
+  int* site_of_orb = nullptr;
+  int* moms_dev = nullptr;
+  bool profile = false;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long prof_n[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_kind;
+};
+
+static void ens_free(nrc_ensemble* e) {
+  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->tab_x_lo, e->tab_yz_lo, e->smear, e->work,
+                  e->rfield, e->words, e->M,
+                  e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
+                  e->site_of_orb, e->moms_dev};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  for (hipEvent_t x : e->ev) hipEventDestroy(x);
+  delete e;
+}
+
+template <typename T>
+static int upload_table(nrc_ensemble* e, void** dst, const std::vector<double>& h) {
+  std::vector<T> t(h.begin(), h.end());
+  if (hipMalloc(dst, t.size() * sizeof(T)) != hipSuccess) return NRC_ERR_NOMEM;
+  NRC_HIP(hipMemcpy(*dst, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NRC_OK;
+}
+
+static int ens_upload(nrc_ensemble* e, void** dst, const std::vector<double>& h) {
+  return e->d.prec == NRC_PREC_F64 ? upload_table<double>(e, dst, h)
+                                   : upload_table<float>(e, dst, h);
+}
+
+// a multiplier table: FP64 as is; FP32 as the nearest floats plus their remainders
+static int ens_upload_pair(nrc_ensemble* e, void** hi, void** lo, const std::vector<double>& h) {
+  NRC_TRY(ens_upload(e, hi, h));
+  if (e->d.prec == NRC_PREC_F64) return NRC_OK;
+  std::vector<double> r(h.size());
+  for (size_t i = 0; i < h.size(); ++i) r[i] = h[i] - (double)(float)h[i];
+  return upload_table<float>(e, lo, r);
+}
+
+template <typename X>
+static int dev_alloc(X** p, size_t n) {
+  if (hipMalloc((void**)p, (n ? n : 1) * sizeof(X)) != hipSuccess) return NRC_ERR_NOMEM;
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
+                                   nrc_ensemble** out) {
+  if (!ctx || !d || !out || !d->moms || !d->kin) return NRC_ERR_ARG;
+  if (d->n_orb < 1 || d->n_configs < 1 || !prec_ok(d->prec)) return NRC_ERR_ARG;
+  if (d->split != NRC_SPLIT_REFERENCE && d->split != NRC_SPLIT_BASELINE) return NRC_ERR_ARG;
+  if (!nrc::pass_supported(d->L) || d->n_orb > 64) return NRC_ERR_UNSUPPORTED;
+  if (d->smear && d->split != NRC_SPLIT_REFERENCE) return NRC_ERR_ARG;
+  const int L = d->L, N = d->n_orb;
+  for (int i = 0; i < 3 * N; ++i)
+    if (d->moms[i] < 0 || d->moms[i] >= L) return NRC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  nrc_ensemble* e = new nrc_ensemble();
+  e->ctx = ctx;
+  e->d = *d;
+  e->V = (long)L * L * L;
+  e->nw = (int)((e->V + 31) / 32);
+  e->nm = d->minors ? N : 1;
+  e->moms.assign(d->moms, d->moms + 3 * N);
+  e->d.moms = e->moms.data();
+  e->d.kin = nullptr;
+  e->d.smear = nullptr;
+  const long V = e->V;
+  const long C = d->n_configs;
+  int rc = NRC_OK;
+#define ENS_TRY(x)                 \
+  do {                             \
+    rc = (x);                      \
+    if (rc != NRC_OK) {            \
+      ens_free(e);                 \
+      return rc;                   \
+    }                              \
+  } while (0)
+  ENS_TRY(hipMalloc(&e->psi, (size_t)C * N * V * elem_bytes(d->prec)) == hipSuccess
+              ? NRC_OK : NRC_ERR_NOMEM);
+  std::vector<double> h(V);
+  if (d->split == NRC_SPLIT_REFERENCE) {
+    for (long i = 0; i < V; ++i) h[i] = 1.0 / d->kin[i];
+    ENS_TRY(ens_upload_pair(e, &e->tab_x, &e->tab_x_lo, h));
+    if (d->smear) {
+      h.assign(d->smear, d->smear + V);
+      ENS_TRY(ens_upload(e, &e->smear, h));
+      ENS_TRY(hipMalloc(&e->work, (size_t)C * V * elem_bytes(d->prec)) == hipSuccess
+                  ? NRC_OK : NRC_ERR_NOMEM);
+      ENS_TRY(hipMalloc(&e->rfield, (size_t)C * V * elem_bytes(d->prec) / 2) == hipSuccess
+                  ? NRC_OK : NRC_ERR_NOMEM);
+    }
+  } else {
+    for (long i = 0; i < V; ++i) h[i] = d->kin[i] / (double)V;
+    ENS_TRY(ens_upload_pair(e, &e->tab_yz, &e->tab_yz_lo, h));
+  }
+  ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
+  // sink gather lists: reference split in the X pass (pencil (k1,k2), position k0);
+  // baseline split in the YZ pass column stage (pencil (k0,k2), position k1)
+  const int npen = L * L;
+  std::vector<int> cnt(npen + 1, 0), pos(N), snk(N), site(N);
+  std::vector<std::pair<int, int>> ent(N);
+  for (int s = 0; s < N; ++s) {
+    const int k0 = d->moms[3 * s], k1 = d->moms[3 * s + 1], k2 = d->moms[3 * s + 2];
+    site[s] = (k0 * L + k1) * L + k2;
+    const int pen = d->split == NRC_SPLIT_REFERENCE ? k1 * L + k2 : k0 * L + k2;
+    const int p = d->split == NRC_SPLIT_REFERENCE ? k0 : k1;
+    ent[s] = {pen, s};
+    pos[s] = p;
+    cnt[pen + 1]++;
+  }
+  for (int p = 0; p < npen; ++p) cnt[p + 1] += cnt[p];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1), gpos(N), gsink(N);
+  for (int s = 0; s < N; ++s) {
+    const int slot = fill[ent[s].first]++;
+    gpos[slot] = pos[s];
+    gsink[slot] = s;
+  }
+  ENS_TRY(dev_alloc(&e->g_ptr, npen + 1));
+  ENS_TRY(dev_alloc(&e->g_pos, N));
+  ENS_TRY(dev_alloc(&e->g_sink, N));
+  ENS_TRY(dev_alloc(&e->site_of_orb, N));
+  ENS_TRY(dev_alloc(&e->moms_dev, 3 * N));
+  hipMemcpy(e->g_ptr, cnt.data(), (npen + 1) * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(e->g_pos, gpos.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(e->g_sink, gsink.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(e->site_of_orb, site.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    ens_free(e);
+    return NRC_ERR_HIP;
+  }
+#undef ENS_TRY
+  *out = e;
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_destroy(nrc_ensemble* e) {
+  if (!e) return NRC_ERR_ARG;
+  hipSetDevice(e->ctx->device);
+  hipStreamSynchronize(e->ctx->stream);
+  ens_free(e);
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_profile(nrc_ensemble* e, int enable) {
+  if (!e) return NRC_ERR_ARG;
+  e->profile = enable != 0;
+  for (int k = 0; k < 4; ++k) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_kernel_time(nrc_ensemble* e, int kind, double* ms, long* n) {
+  if (!e || kind < 0 || kind > 3) return NRC_ERR_ARG;
+  if (ms) *ms = e->prof_ms[kind];
+  if (n) *n = e->prof_n[kind];
+  return NRC_OK;
+}
+
+// events are recorded on the ensemble's stream around each launch when profiling
+struct EvScope {
+  nrc_ensemble* e;
+  int kind;
+  size_t idx;
+  EvScope(nrc_ensemble* e_, int k) : e(e_), kind(k), idx(0) {
+    if (!e->profile) return;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    idx = e->ev.size();
+    e->ev.push_back(a);
+    e->ev.push_back(b);
+    e->ev_kind.push_back(kind);
+    hipEventRecord(a, e->ctx->stream);
+  }
+  ~EvScope() {
+    if (e->profile) hipEventRecord(e->ev[idx + 1], e->ctx->stream);
+  }
+};
+
+static int ens_collect_profile(nrc_ensemble* e) {
+  if (!e->profile) return NRC_OK;
+  NRC_HIP(hipStreamSynchronize(e->ctx->stream));
+  for (size_t i = 0; i < e->ev_kind.size(); ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e->ev[2 * i], e->ev[2 * i + 1]) == hipSuccess) {
+      e->prof_ms[e->ev_kind[i]] += ms;
+      e->prof_n[e->ev_kind[i]] += 1;
+    }
+  }
+  for (hipEvent_t x : e->ev) hipEventDestroy(x);
+  e->ev.clear();
+  e->ev_kind.clear();
+  return NRC_OK;
+}
+
+static nrc::MulSpec mul_table(const void* t, const void* lo = nullptr) {
+  nrc::MulSpec m;
+  m.kind = nrc::MUL_TABLE;
+  m.table = t;
+  m.table_lo = lo;
+  return m;
+}
+
+// the reference field (1 + F-[f F+ phi])/V, computed as F+[f F-[phi]] (equal for the even,
+// real f of periodic lattices), real part kept: src/hamiltonian/noarch/hamiltonian_initialize.C
+template <typename T>
+__global__ void k_bits_to_complex(nrc::C2<T>* out, const uint32_t* words, int nw, long V,
+                                  long C) {
+  const long n = C * V;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / V, s = i - c * V;
+    const uint32_t w = words[c * nw + (s >> 5)];
+    out[i] = {((w >> (s & 31)) & 1u) ? T(1) : T(-1), T(0)};
+  }
+}
+
+template <typename T>
+__global__ void k_complex_to_field(const nrc::C2<T>* in, T* out, long n, double invV) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = (T)(((double)in[i].x + 1.0) * invV);
+}
+
+static int ens_smear_field(nrc_ensemble* e, const uint32_t* words) {
+  const long n = (long)e->d.n_configs * e->V;
+  long blocks = std::min<long>(65536, (n + 255) / 256);
+  hipStream_t s = e->ctx->stream;
+  if (e->d.prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_bits_to_complex<double>, dim3(blocks), dim3(256), 0, s,
+                       (nrc::C2<double>*)e->work, words, e->nw, e->V, (long)e->d.n_configs);
+  else
+    hipLaunchKernelGGL(k_bits_to_complex<float>, dim3(blocks), dim3(256), 0, s,
+                       (nrc::C2<float>*)e->work, words, e->nw, e->V, (long)e->d.n_configs);
+  NRC_HIP(hipGetLastError());
+  nrc::PassArgs a;
+  a.psi = e->work;
+  a.nvol = e->d.n_configs;
+  a.L = e->d.L;
+  a.fft_b = 1;
+  NRC_HIP(nrc::launch_xpass(e->d.prec, a, s));
+  a.fft_a = 1;
+  a.fft_b = 1;
+  a.m1 = mul_table(e->smear);
+  NRC_HIP(nrc::launch_yzpass(e->d.prec, a, s));
+  nrc::PassArgs b;
+  b.psi = e->work;
+  b.nvol = e->d.n_configs;
+  b.L = e->d.L;
+  b.fft_a = 1;
+  NRC_HIP(nrc::launch_xpass(e->d.prec, b, s));
+  if (e->d.prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_complex_to_field<double>, dim3(blocks), dim3(256), 0, s,
+                       (const nrc::C2<double>*)e->work, (double*)e->rfield, n,
+                       1.0 / (double)e->V);
+  else
+    hipLaunchKernelGGL(k_complex_to_field<float>, dim3(blocks), dim3(256), 0, s,
+                       (const nrc::C2<float>*)e->work, (float*)e->rfield, n,
+                       1.0 / (double)e->V);
+  NRC_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+static int ens_reserve_outputs(nrc_ensemble* e, int T) {
+  const size_t need = (size_t)T * e->d.n_configs * e->nm;
+  if (need <= e->out_cap) return NRC_OK;
+  hipFree(e->o_la); hipFree(e->o_ph); hipFree(e->o_as); hipFree(e->o_st);
+  e->o_la = nullptr; e->o_ph = nullptr; e->o_as = nullptr; e->o_st = nullptr;
+  e->out_cap = 0;
+  NRC_TRY(dev_alloc(&e->o_la, need));
+  NRC_TRY(dev_alloc(&e->o_ph, need));
+  NRC_TRY(dev_alloc(&e->o_as, need));
+  NRC_TRY(dev_alloc(&e->o_st, need));
+  e->out_cap = need;
+  return NRC_OK;
+}
+
+// the slice loop; fields come from Philox (words_host == nullptr) or the caller
+static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
+                   int T, double* logabs, double* phase, double* argsum) {
+  if (T < 1) return NRC_ERR_ARG;
+  hipSetDevice(e->ctx->device);
+  hipStream_t s = e->ctx->stream;
+  const int N = e->d.n_orb, L = e->d.L, prec = e->d.prec;
+  const long C = e->d.n_configs, V = e->V;
+  const bool ref = e->d.split == NRC_SPLIT_REFERENCE;
+  const size_t slice_words = (size_t)C * e->nw;
+  const size_t need_words = words_host ? slice_words * T : slice_words;
+  if (need_words > e->words_cap) {
+    hipFree(e->words);
+    e->words = nullptr;
+    e->words_cap = 0;
+    NRC_TRY(dev_alloc(&e->words, need_words));
+    e->words_cap = need_words;
+  }
+  if (words_host)
+    NRC_HIP(hipMemcpyAsync(e->words, words_host, need_words * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+  NRC_TRY(ens_reserve_outputs(e, T));
+  const size_t per_t = (size_t)C * e->nm;
+
+  // Propagator::Set(one_bodies[n]->Get(), n): plane-wave sources
+  if (ref)
+    NRC_HIP(nrc::launch_set_unit_sources(prec, e->psi, e->site_of_orb, C * N, N, V, s));
+  else
+    NRC_HIP(nrc::launch_set_plane_waves(prec, e->psi, e->moms_dev, C * N, N, L, s));
+
+  nrc::GatherSpec g;
+  g.ptr = e->g_ptr;
+  g.pos = e->g_pos;
+  g.sink = e->g_sink;
+  g.M = e->M;
+  g.nsink = N;
+  g.scale = ref ? 1.0 : 1.0 / sqrt((double)V);
+
+  auto field_of = [&](int t) -> const uint32_t* {
+    return words_host ? e->words + (size_t)t * slice_words : e->words;
+  };
+  auto lu = [&](int t) -> int {
+    EvScope sc(e, 2);
+    NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
+                               e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
+                               nullptr, nullptr, nullptr, s));
+    return NRC_OK;
+  };
+  nrc::MulSpec fieldspec;
+  fieldspec.kind = nrc::MUL_FIELD;
+  fieldspec.nw = e->nw;
+
+  for (int t = 0; t <= T; ++t) {
+    const bool last = t == T;
+    // X pass: finish slice t-1, measure t (reference split), start slice t
+    {
+      nrc::PassArgs a;
+      a.psi = e->psi;
+      a.nvol = C * N;
+      a.norb = N;
+      a.L = L;
+      a.fft_a = t > 0;
+      a.fft_b = !last;
+      if (ref) {
+        if (t > 0) a.m1 = mul_table(e->tab_x, e->tab_x_lo);
+        if (!last) {
+          a.gather = 1;
+          a.g = g;
+          a.m2 = mul_table(e->tab_x, e->tab_x_lo);
+        }
+      } else if (t > 0) {
+        a.m1 = fieldspec;
+        a.m1.bits = field_of(t - 1);
+        a.m1.a0 = 1.0;
+        a.m1.a1 = e->d.A;
+      }
+      EvScope sc(e, 0);
+      NRC_HIP(nrc::launch_xpass(prec, a, s));
+    }
+    if (last) break;
+    if (ref) NRC_TRY(lu(t));
+    // Lattice::Refresh for slice t
+    if (!words_host) {
+      EvScope sc(e, 3);
+      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
+    }
+    if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
+    // YZ pass: the field (reference) or kinetic (baseline) multiply between the FFTs
+    {
+      nrc::PassArgs a;
+      a.psi = e->psi;
+      a.nvol = C * N;
+      a.norb = N;
+      a.L = L;
+      a.fft_a = 1;
+      a.fft_b = 1;
+      if (ref) {
+        if (e->smear) {
+          a.m1.kind = nrc::MUL_RFIELD;
+          a.m1.table = e->rfield;
+        } else {
+          a.m1 = fieldspec;
+          a.m1.bits = field_of(t);
+          a.m1.a0 = 1.0 / (double)V;
+          a.m1.a1 = e->d.A / (double)V;
+        }
+      } else {
+        a.gather = 1;
+        a.g = g;
+        a.m1 = mul_table(e->tab_yz, e->tab_yz_lo);
+      }
+      EvScope sc(e, 1);
+      NRC_HIP(nrc::launch_yzpass(prec, a, s));
+    }
+    if (!ref) NRC_TRY(lu(t));
+  }
+  const size_t tot = per_t * T;
+  if (logabs) NRC_HIP(hipMemcpyAsync(logabs, e->o_la, tot * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (phase) NRC_HIP(hipMemcpyAsync(phase, e->o_ph, tot * sizeof(double2), hipMemcpyDeviceToHost, s));
+  if (argsum) NRC_HIP(hipMemcpyAsync(argsum, e->o_as, tot * sizeof(double), hipMemcpyDeviceToHost, s));
+  NRC_HIP(hipStreamSynchronize(s));
+  NRC_TRY(ens_collect_profile(e));
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, int T,
+                                double* logabs, double* phase, double* argsum) {
+  if (!e) return NRC_ERR_ARG;
+  return ens_run(e, seed, config0, nullptr, T, logabs, phase, argsum);
+}
+
+extern "C" int nrc_ensemble_run_fields(nrc_ensemble* e, const uint32_t* words, int T,
+                                       double* logabs, double* phase, double* argsum) {
+  if (!e || !words) return NRC_ERR_ARG;
+  return ens_run(e, 0, 0, words, T, logabs, phase, argsum);
+}
+
+extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
+  if (!e || !out || c < 0 || c >= e->d.n_configs) return NRC_ERR_ARG;
+  const size_t bytes = (size_t)e->d.n_orb * e->V * elem_bytes(e->d.prec);
+  NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * bytes, bytes, hipMemcpyDeviceToHost,
+                         e->ctx->stream));
+  NRC_HIP(hipStreamSynchronize(e->ctx->stream));
+  return NRC_OK;
+}

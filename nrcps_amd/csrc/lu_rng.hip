@@ -1,0 +1,311 @@
+// lu_rng.hip -- batched FP64 LU log-determinants, dense overlaps, Philox Z2 fields.
+#include "nrc_device.h"
+#include "nrc_internal.h"
+
+namespace nrc {
+
+// ------------------------------------------------------------------ Philox Z2 fields
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Word w of configuration c's field at
+// slice t is output (w % 4) of the block with counter (w / 4, t, c, stream) and key
+// (seed lo, seed hi); bit j of word w is site 32 w + j, phi = +1 where set.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+__global__ void k_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                          uint32_t stream, int nw, uint32_t* words) {
+  const int ng = (nw + 3) / 4;
+  const long n = (long)nconf * ng;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(i / ng), g = (int)(i - (long)c * ng);
+    uint32_t ctr[4] = {(uint32_t)g, t, config0 + (uint32_t)c, stream};
+    philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t* out = words + (long)c * nw + 4 * g;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (4 * g + q < nw) out[q] = ctr[q];
+  }
+}
+
+hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                          uint32_t stream, int vol, uint32_t* words, hipStream_t s) {
+  const int nw = (vol + 31) / 32;
+  const long n = (long)nconf * ((nw + 3) / 4);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_z2_bits, dim3((unsigned)blocks), dim3(256), 0, s, seed, config0, nconf,
+                     t, stream, nw, words);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batched LU
+// One wavefront per matrix, lane i owns row i, the matrix sits in LDS column-major so that
+// lanes touch consecutive addresses.  The factorisation is right-looking: at step k the
+// column below the pivot is scaled by the reciprocal pivot and the trailing block gets
+// a_ij -= l_ik u_kj.  Every a_ij therefore receives its updates in the order k = 0, 1, ...,
+// exactly the order of the Crout dot products in src/utils/ludcmp.C:32-40, and the pivot
+// is chosen as there: implicit row scaling vv_i = 1 / max_j |a_ij| and the LAST row with the
+// largest vv_i |a_ik| (the reference compares with >=, ludcmp.C:43).  A zero pivot is
+// replaced by 1e-20 (ludcmp.C:57).  An all-zero row sets status = 1 (the reference exits).
+//
+// Every floating-point operation is the one the reference executes, in the same order and
+// without fused multiply-adds, so that pivot ties break the same way and the factors are
+// bit-identical: |z| is the corrected hypot of C. F. Borges ("An improved algorithm for
+// hypot(a,b)", 2019; the non-FMA kernel with the 2^-600 / 2^511 / 2^-511 / 2^-54 scaling
+// thresholds of the host C library's hypot, which std::abs(complex<double>) calls), the
+// reciprocal pivot is Smith's complex division (the host compiler's complex division), and
+// complex products are (ac - bd, ad + bc).
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double hypot_kernel(double ax, double ay) {
+  double h = sqrt(ax * ax + ay * ay);
+  double t1, t2;
+  if (h <= 2.0 * ay) {
+    const double delta = h - ay;
+    t1 = ax * (2.0 * delta - ax);
+    t2 = (delta - 2.0 * (ax - ay)) * delta;
+  } else {
+    const double delta = h - ax;
+    t1 = 2.0 * delta * (ax - 2.0 * ay);
+    t2 = (4.0 * delta - ay) * ay + delta * delta;
+  }
+  h -= (t1 + t2) / (2.0 * h);
+  return h;
+}
+
+__device__ double ref_abs(double x, double y) {
+  if (isinf(x) || isinf(y)) return INFINITY;
+  if (isnan(x) || isnan(y)) return x + y;
+  x = fabs(x);
+  y = fabs(y);
+  double ax = x < y ? y : x;
+  const double ay = x < y ? x : y;
+  const double SCALE = 0x1p-600, LARGE = 0x1p+511, TINY = 0x1p-511, EPS = 0x1p-54;
+  if (ax > LARGE) {
+    if (ay <= ax * EPS) return ax + ay;
+    return hypot_kernel(ax * SCALE, ay * SCALE) / SCALE;
+  }
+  if (ay < TINY) {
+    if (ax >= ay / EPS) return ax + ay;
+    ax = hypot_kernel(ax / SCALE, ay / SCALE) * SCALE;
+    return ax;
+  }
+  if (ay <= ax * EPS) return ax + ay;
+  return hypot_kernel(ax, ay);
+}
+
+__device__ __forceinline__ double cabs2(double2 a) { return ref_abs(a.x, a.y); }
+
+// (1 + 0i) / z by Smith's method
+__device__ __forceinline__ double2 crecip(double2 z) {
+  const double a = 1.0, b = 0.0, c = z.x, d = z.y;
+  double ratio, denom, x, y;
+  if (fabs(c) < fabs(d)) {
+    ratio = c / d;
+    denom = (c * ratio) + d;
+    x = ((a * ratio) + b) / denom;
+    y = ((b * ratio) - a) / denom;
+  } else {
+    ratio = d / c;
+    denom = (d * ratio) + c;
+    x = ((b * ratio) + a) / denom;
+    y = (b - (a * ratio)) / denom;
+  }
+  return make_double2(x, y);
+}
+
+__device__ __forceinline__ double2 cmuld(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, int n, int ldm,
+                                               int ncfg, int minors, double* logabs,
+                                               double2* phase, double* argsum, int* status,
+                                               double2* lu_out, int* indx_out, double* d_out) {
+  extern __shared__ double2 a[];
+  const int nm = minors ? n : 1;
+  const long blk = blockIdx.x;
+  const long cfg = blk / nm;
+  const int m = (int)(blk - cfg * nm);
+  const int sz = minors ? m + 1 : n;
+  const int i = threadIdx.x;
+  const double2* Mc = M + cfg * (long)n * ldm;
+  if (i < sz)
+    for (int j = 0; j < sz; ++j) a[j * sz + i] = Mc[(long)i * ldm + j];
+  __syncthreads();
+  double vv = 0.0;
+  int bad = 0;
+  if (i < sz) {
+    double big = 0.0;
+    for (int j = 0; j < sz; ++j) {
+      const double t = cabs2(a[j * sz + i]);
+      if (t > big) big = t;
+    }
+    if (big == 0.0) bad = 1;
+    else vv = 1.0 / big;
+  }
+  bad = __any(bad);
+  double dsign = 1.0, la = 0.0, as = 0.0;
+  double2 ph = make_double2(1.0, 0.0);
+  int my_indx = 0;
+  if (!bad) {
+    for (int k = 0; k < sz; ++k) {
+      double bv = (i >= k && i < sz) ? vv * cabs2(a[k * sz + i]) : -1.0;
+      int bi = i;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
+      }
+      const int imax = bi;
+      if (i == k) my_indx = imax;
+      if (imax != k) {
+        if (i < sz) {
+          const double2 t = a[i * sz + k];
+          a[i * sz + k] = a[i * sz + imax];
+          a[i * sz + imax] = t;
+        }
+        const double vk = __shfl(vv, k);
+        if (i == imax) vv = vk;
+        dsign = -dsign;
+      }
+      __syncthreads();
+      double2 piv = a[k * sz + k];
+      double mag = cabs2(piv);
+      if (mag == 0.0) {
+        piv = make_double2(1.0e-20, 0.0);
+        mag = 1.0e-20;
+        if (i == 0) a[k * sz + k] = piv;
+      }
+      la += log(mag);
+      ph = cmuld(ph, make_double2(piv.x / mag, piv.y / mag));
+      as += atan2(piv.y, piv.x);
+      const double2 r = crecip(piv);
+      if (i > k && i < sz) {
+        const double2 l = cmuld(a[k * sz + i], r);
+        a[k * sz + i] = l;
+        for (int j = k + 1; j < sz; ++j) {
+          const double2 p = cmuld(l, a[j * sz + k]);
+          double2 x = a[j * sz + i];
+          x.x = x.x - p.x;
+          x.y = x.y - p.y;
+          a[j * sz + i] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const long o = cfg * nm + m;
+  if (lu_out && i < sz) {
+    double2* out = lu_out + cfg * (long)n * ldm;
+    for (int j = 0; j < sz; ++j) out[(long)i * ldm + j] = a[j * sz + i];
+    indx_out[cfg * (long)n + i] = my_indx;
+    if (i == 0) d_out[cfg] = dsign;
+  }
+  if (i == 0) {
+    if (status) status[o] = bad ? 1 : 0;
+    if (logabs) logabs[o] = bad ? -INFINITY : la;
+    if (phase) phase[o] = bad ? make_double2(0.0, 0.0) : make_double2(dsign * ph.x, dsign * ph.y);
+    if (argsum) argsum[o] = bad ? 0.0 : as + (dsign < 0.0 ? 3.14159265358979323846 : 0.0);
+  }
+}
+
+__global__ void k_ref_abs(const double2* z, double* out, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = cabs2(z[i]);
+}
+
+#pragma clang fp contract(on)
+
+hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_ref_abs, dim3((unsigned)blocks), dim3(256), 0, s, z, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
+                         double* logabs, double2* phase, double* argsum, int* status,
+                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s) {
+  if (n < 1 || n > 64) return hipErrorInvalidValue;
+  const long blocks = (long)ncfg * (minors ? n : 1);
+  const size_t shm = (size_t)n * n * sizeof(double2);
+  hipLaunchKernelGGL(k_lu_det, dim3((unsigned)blocks), dim3(64), shm, s, M, n, ldm, ncfg,
+                     minors, logabs, phase, argsum, status, lu_out, indx_out, d_out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ dense overlaps
+// M[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i] (src/one_body/noarch/one_body_project.C:
+// 14-27 for every (s, t), src/slater_det/slater_det.C:33-37).  One workgroup per (c, t)
+// streams its orbital once and accumulates 16 sinks at a time in FP64; the sinks are shared
+// by every workgroup and stay cache resident.
+template <typename T>
+__global__ __launch_bounds__(256) void k_overlap_dense(const C2<T>* __restrict__ phi,
+                                                       const C2<T>* __restrict__ psi,
+                                                       double2* M, int ns, int nt, long vol) {
+  constexpr int CH = 16;
+  __shared__ double red[CH][2][4];
+  const long ct = blockIdx.x;  // c * nt + t
+  const long c = ct / nt;
+  const int t = (int)(ct - c * nt);
+  const int s0 = blockIdx.y * CH;
+  const C2<T>* p = psi + ct * vol;
+  double ar[CH], ai[CH];
+#pragma unroll
+  for (int q = 0; q < CH; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
+  for (long i = threadIdx.x; i < vol; i += 256) {
+    const C2<T> b = p[i];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      if (s0 + q < ns) {
+        const C2<T> f = phi[(long)(s0 + q) * vol + i];
+        ar[q] += (double)f.x * b.x + (double)f.y * b.y;
+        ai[q] += (double)f.x * b.y - (double)f.y * b.x;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    double x = ar[q], y = ai[q];
+    for (int off = 32; off >= 1; off >>= 1) {
+      x += __shfl_xor(x, off);
+      y += __shfl_xor(y, off);
+    }
+    if (lane == 0) { red[q][0][w] = x; red[q][1][w] = y; }
+  }
+  __syncthreads();
+  if (threadIdx.x < CH && s0 + (int)threadIdx.x < ns) {
+    const int q = threadIdx.x;
+    const double x = red[q][0][0] + red[q][0][1] + red[q][0][2] + red[q][0][3];
+    const double y = red[q][1][0] + red[q][1][1] + red[q][1][2] + red[q][1][3];
+    M[(c * ns + s0 + q) * nt + t] = make_double2(x, y);
+  }
+}
+
+hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,
+                                int ns, int nt, long vol, int ncfg, hipStream_t s) {
+  dim3 grid((unsigned)((long)ncfg * nt), (unsigned)((ns + 15) / 16));
+  if (prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_overlap_dense<double>, grid, dim3(256), 0, s,
+                       (const C2<double>*)phi, (const C2<double>*)psi, M, ns, nt, vol);
+  else
+    hipLaunchKernelGGL(k_overlap_dense<float>, grid, dim3(256), 0, s, (const C2<float>*)phi,
+                       (const C2<float>*)psi, M, ns, nt, vol);
+  return hipGetLastError();
+}
+
+}  // namespace nrc

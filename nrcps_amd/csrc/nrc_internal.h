@@ -1,0 +1,90 @@
+// nrc_internal.h -- host-side declarations shared by the nrCPS HIP library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nrcps.h"
+
+namespace nrc {
+
+enum MulKind { MUL_NONE = 0, MUL_TABLE = 1, MUL_FIELD = 2, MUL_RFIELD = 3 };
+
+// A real per-site multiplier applied inside an FFT pass.
+//   MUL_TABLE : table[site]                       (same for every volume)
+//   MUL_FIELD : a0 + a1 * phi(site), phi = +-1 from the configuration's packed bits
+//   MUL_RFIELD: table[config * V + site]          (per-configuration real field)
+//
+// In FP32 passes a multiplier is applied as v*hi + v*lo, with hi the float nearest to the
+// FP64 value and lo the float nearest to the remainder, so the factor carries ~48 bits and
+// the rounding of the factor itself does not bias the evolution slice after slice.
+struct MulSpec {
+  int kind = MUL_NONE;
+  const void* table = nullptr;     // device, float or double (the pass precision)
+  const void* table_lo = nullptr;  // device, FP32 remainders of a MUL_TABLE table (or null)
+  const uint32_t* bits = nullptr;  // device, [config][nw]
+  int nw = 0;
+  double a0 = 0.0, a1 = 0.0;
+};
+
+// Plane-wave sink measurement folded into a pass: for each pencil p of a volume, the
+// entries ptr[p] .. ptr[p+1] name (position within the pencil, sink index); the value at
+// that position (times `scale`) is written to M[(config * nsink + sink) * norb + orbital].
+struct GatherSpec {
+  const int* ptr = nullptr;
+  const int* pos = nullptr;
+  const int* sink = nullptr;
+  double2* M = nullptr;
+  int nsink = 0;
+  double scale = 1.0;
+};
+
+struct PassArgs {
+  void* psi = nullptr;  // volumes [nvol][L][L][L] complex (float2 / double2)
+  long nvol = 0;        // configurations * orbitals
+  int norb = 1;         // orbitals per configuration (volume v belongs to config v / norb)
+  int L = 0;
+  // X pass: fft_a = sign +1 x-transform first, fft_b = sign -1 x-transform last.
+  // YZ pass: fft_a = sign -1 yz-transform first, fft_b = sign +1 yz-transform last.
+  int fft_a = 0, fft_b = 0;
+  MulSpec m1, m2;  // X pass: m1 before the gather, m2 after.  YZ pass: m1 between the FFTs.
+  int gather = 0;
+  GatherSpec g;
+};
+
+hipError_t launch_xpass(int prec, const PassArgs& a, hipStream_t s);
+hipError_t launch_yzpass(int prec, const PassArgs& a, hipStream_t s);
+bool pass_supported(int L);
+
+// element-wise helpers (the reference's separate kernels, used by the class layer)
+hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch, long vol,
+                             int op, hipStream_t s);
+hipError_t launch_mul_complex(int prec, void* buf, const void* field, long batch, long vol,
+                              hipStream_t s);
+hipError_t launch_fill(void* p, uint8_t v, size_t bytes, hipStream_t s);
+hipError_t launch_set_unit_sources(int prec, void* psi, const int* site_of_orb, long nvol,
+                                   int norb, long vol, hipStream_t s);
+hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvol, int norb,
+                                  int L, hipStream_t s);
+
+// Philox Z2 field bits: words[c][w] for configs config0 .. config0 + nconf - 1
+hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                          uint32_t stream, int vol, uint32_t* words, hipStream_t s);
+
+// batched LU log-determinant of M[c][n][n] (leading minors when minors != 0)
+hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
+                         double* logabs, double2* phase, double* argsum, int* status,
+                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s);
+// |z| computed exactly as the host C library's hypot (the LU's pivot metric)
+hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s);
+
+// dense overlaps M[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i]
+hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,
+                                int ns, int nt, long vol, int ncfg, hipStream_t s);
+hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
+                                 int nt, long vol, int ncfg, float* partial, hipStream_t s);
+hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
+
+hipError_t upload_twiddles();
+
+}  // namespace nrc

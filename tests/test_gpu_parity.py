@@ -1,0 +1,344 @@
+"""Parity of the HIP product path (through the C ABI) with the FP64 oracle.
+
+Tolerances: integer and index work (fields, pivots) bit-exact; FP64 paths 1e-12 relative;
+FP32 paths as stated per test (FP32 evolution error, measured against FP64)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import nrcps_amd as na
+
+    c = na.Context(0)
+    print(f"\n[gpu] device {c.arch} with {c.n_cu} CUs", flush=True)
+    assert c.arch.startswith("gfx950"), c.arch
+    yield c
+    c.close()
+
+
+def _rand_c(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ------------------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("L", [4, 6, 8, 12, 16, 24, 32])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fft3d_vs_oracle(ctx, orc, L, prec):
+    import nrcps_amd as na
+
+    print(f"[gpu] fft L={L} prec={prec}", flush=True)
+    rng = np.random.default_rng(L * 10 + prec)
+    batch = 3
+    a = _rand_c(rng, (batch, L, L, L))
+    dt = np.complex64 if prec == na.PREC_F32 else np.complex128
+    tol = 2e-6 if prec == na.PREC_F32 else 1e-13
+    for sign in (-1, 1):
+        d = ctx.to_device(a.astype(dt))
+        ctx.fft3d(d, L, batch, sign, prec)
+        got = d.download()
+        ref = orc.fft3d(a.astype(dt).astype(np.complex128), sign)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < tol, (sign, err)
+
+
+def test_fft3d_plane_wave_delta_f32(ctx):
+    L = 32
+    x = np.arange(L)
+    k = (3, 30, 17)
+    pw = np.exp(2j * np.pi * (k[0] * x[:, None, None] + k[1] * x[None, :, None]
+                              + k[2] * x[None, None, :]) / L).astype(np.complex64)
+    d = ctx.to_device(pw[None])
+    ctx.fft3d(d, L, 1, -1, 0)
+    got = d.download()[0]
+    assert abs(got[k] - L ** 3) < 0.2
+    got[k] = 0
+    assert np.abs(got).max() < 0.2
+
+
+# ------------------------------------------------------------------------------ element-wise
+@pytest.mark.parametrize("prec", [0, 1])
+def test_scale_and_mul_complex(ctx, prec):
+    rng = np.random.default_rng(7)
+    dt, rt = (np.complex64, np.float32) if prec == 0 else (np.complex128, np.float64)
+    V, B = 1728, 4
+    a = _rand_c(rng, (B, V)).astype(dt)
+    t = (1.0 + rng.random(V)).astype(rt)
+    f = _rand_c(rng, V).astype(dt)
+    d = ctx.to_device(a)
+    ctx.scale_real(d, ctx.to_device(t), B, V, 1, prec)
+    np.testing.assert_allclose(d.download(), a / t, rtol=2e-7 if prec == 0 else 1e-15)
+    d = ctx.to_device(a)
+    ctx.scale_real(d, ctx.to_device(t), B, V, 0, prec)
+    np.testing.assert_allclose(d.download(), a * t, rtol=2e-7 if prec == 0 else 1e-15)
+    d = ctx.to_device(a)
+    ctx.mul_complex(d, ctx.to_device(f), B, V, prec)
+    np.testing.assert_allclose(d.download(), a * f, rtol=1e-6 if prec == 0 else 1e-15)
+
+
+# ------------------------------------------------------------------------------ fields
+@pytest.mark.parametrize("vol", [216, 1728, 4096, 13824, 32768])
+def test_z2_bits_bit_exact(ctx, orc, vol):
+    seed = 0xDEADBEEF12345
+    w = ctx.z2_bits(seed, 17, 5, 9, vol)
+    for c in range(5):
+        assert np.array_equal(w[c], orc.z2_bits(seed, 17 + c, 9, vol))
+
+
+# ------------------------------------------------------------------------------ LU
+@pytest.mark.parametrize("n", [1, 2, 7, 14, 33, 64])
+@pytest.mark.parametrize("minors", [False, True])
+def test_lu_det_vs_oracle(ctx, orc, n, minors):
+    print(f"[gpu] lu n={n} minors={minors}", flush=True)
+    rng = np.random.default_rng(n)
+    ncfg = 6
+    Ms = _rand_c(rng, (ncfg, n, n))
+    la, ph, ag, st = ctx.lu_det(ctx.to_device(Ms), n, ncfg, minors)
+    assert np.all(st == 0)
+    for c in range(ncfg):
+        for m in range(n if minors else 1):
+            sz = m + 1 if minors else n
+            lu, indx, d = orc.ludcmp(Ms[c, :sz, :sz])
+            rla, rph, rag, _ = orc.lu_det(lu, d)
+            assert abs(la[c, m] - rla) < 1e-12 * max(1.0, abs(rla))
+            assert abs(ph[c, m] - rph) < 1e-12
+            assert abs(ag[c, m] - rag) < 1e-11
+
+
+def test_lu_known_answer_and_ties(ctx, orc):
+    n = 10
+    a = np.array([[1.0 / (s + t + 2.0) for t in range(n)] for s in range(n)], dtype=complex)
+    eye = np.eye(n, dtype=complex)
+    tie = np.zeros((n, n), complex)
+    tie[:3, :3] = [[1, 2, 0], [1, 0, 2], [1, 1, 1]]
+    tie[3:, 3:] = np.eye(n - 3)
+    Ms = np.stack([a, eye, tie])
+    la, ph, ag, st = ctx.lu_det(ctx.to_device(Ms), n, 3)
+    for c in range(3):
+        lu, indx, d = orc.ludcmp(Ms[c])
+        rla, rph, rag, _ = orc.lu_det(lu, d)
+        # ill-conditioned known-answer matrix: same pivots, rounding within cond * eps
+        assert abs(la[c, 0] - rla) < (1e-9 if c == 0 else 1e-15)
+        assert abs(ph[c, 0] - rph) < 1e-12 and abs(ag[c, 0] - rag) < 1e-12
+    assert la[1, 0] == 0.0 and ph[1, 0] == 1
+
+
+def test_ref_abs_bit_identical_to_libm(ctx):
+    """|z| as std::abs(complex<double>) computes it (glibc hypot), across scaling regimes."""
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    libm.hypot.restype = ctypes.c_double
+    libm.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(12)
+    n = 200000
+    sx = 2.0 ** rng.choice([0, 0, 0, -3, 7, -300, 300, -530, 520, -1070], n)
+    sy = 2.0 ** rng.choice([0, 0, -2, 3, -40, 200, -520], n)
+    z = rng.standard_normal(n) * sx + 1j * rng.standard_normal(n) * sy
+    z[:4] = [0, 3 + 4j, 1e-320 + 0j, -0.0 + 2j]
+    got = ctx.ref_abs(z)
+    ref = np.array([libm.hypot(a.real, a.imag) for a in z[:20000]])
+    assert np.array_equal(got[:20000], ref)
+    ref2 = np.array([libm.hypot(a.real, a.imag) for a in z[20000:]])
+    assert np.array_equal(got[20000:], ref2)
+
+
+@pytest.mark.parametrize("n", [2, 3, 10, 33, 64])
+def test_ludcmp_bit_exact_vs_reference(ctx, orc, n):
+    """Batched ludcmp reproduces the reference's factors, pivots and parity bit for bit,
+    including the row-maximum ties that are common for small matrices."""
+    rng = np.random.default_rng(100 + n)
+    ncfg = 64
+    Ms = _rand_c(rng, (ncfg, n, n))
+    if n == 10:
+        Ms[0] = np.array([[1.0 / (s + t + 2.0) for t in range(n)] for s in range(n)])
+        Ms[1] = np.eye(n)
+    lu, indx, d, st = ctx.ludcmp(Ms)
+    assert np.all(st == 0)
+    for c in range(ncfg):
+        rlu, rindx, rd = orc.ludcmp(Ms[c])
+        assert np.array_equal(indx[c], rindx), c
+        assert d[c] == rd
+        assert np.array_equal(lu[c], rlu), (c, np.abs(lu[c] - rlu).max())
+        ref = orc.ref_ludcmp(Ms[c])
+        if ref is not None:
+            assert np.array_equal(ref[0], lu[c]) and np.array_equal(ref[1], indx[c])
+
+
+def test_lu_singular_row_status(ctx):
+    M = np.eye(4, dtype=complex)[None].repeat(2, 0)
+    M[1, 2, :] = 0
+    la, ph, ag, st = ctx.lu_det(ctx.to_device(M), 4, 2)
+    assert st[0, 0] == 0 and st[1, 0] == 1 and np.isneginf(la[1, 0])
+
+
+# ------------------------------------------------------------------------------ overlaps
+@pytest.mark.parametrize("prec,mode", [(1, 0), (0, 1)])
+def test_overlap_dense_vs_oracle(ctx, orc, prec, mode):
+    rng = np.random.default_rng(3)
+    ns, nt, V, ncfg = 19, 19, 4096, 3
+    dt = np.complex64 if prec == 0 else np.complex128
+    phi = _rand_c(rng, (ns, V)).astype(dt)
+    psi = _rand_c(rng, (ncfg, nt, V)).astype(dt)
+    M = ctx.array((ncfg, ns, nt), np.complex128)
+    ctx.overlap(ctx.to_device(phi), ctx.to_device(psi), M, ns, nt, V, ncfg, mode)
+    got = M.download()
+    for c in range(ncfg):
+        ref = orc.overlap(phi.astype(np.complex128), psi[c].astype(np.complex128))
+        np.testing.assert_allclose(got[c], ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+# ------------------------------------------------------------------------------ the path
+def _tables(orc, L, split, tau=0.05):
+    if split == 0:
+        return orc.kinetic_xi(L, orc.DISP_STANDARD, 1.0 / tau, 0.5)
+    return orc.kinetic_exp(L, tau, 1.0)
+
+
+def _compare(gpu, ref, tol_la, tol_ph):
+    la, ph = gpu
+    dla = np.abs(la - ref["logabs"]).max()
+    dph = np.abs(ph - ref["phase"]).max()
+    assert dla < tol_la and dph < tol_ph, (dla, dph)
+    return dla, dph
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("minors", [False, True])
+def test_ensemble_f64_vs_oracle_config2_shape(ctx, orc, split, minors):
+    """L=16, N=7 plane waves, tau=0.05, A=0.4, Z2 fields from Philox, FP64 throughout."""
+    import nrcps_amd as na
+
+    L, N, T, C, seed = 16, 7, 64, 4, 7012
+    print(f"[gpu] ensemble f64 split={split} minors={minors}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, split)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F64, split=split, minors=minors)
+    la, ph, _ = ens.run(seed, 100, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, minors=minors, seed=seed,
+                             config=100 + c)
+        _compare((la[:, c], ph[:, c]), ref, 1e-12, 1e-12)
+    ens.close()
+
+
+def test_ensemble_f64_correlator_relative_error(ctx, orc):
+    """C(t) = <det^2> over configurations agrees with the oracle to 1e-12 relative."""
+    import nrcps_amd as na
+
+    L, N, T, C = 16, 7, 64, 16
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, 0)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F64)
+    la, ph, _ = ens.run(11, 0, T)
+    g = (np.exp(2 * la[:, :, 0]) * ph[:, :, 0] ** 2).mean(1)
+    r = np.zeros(T, complex)
+    for c in range(C):
+        o = orc.run_config(L, N, T, moms, kin, A=0.4, seed=11, config=c)
+        r += np.exp(2 * o["logabs"][:, 0]) * o["phase"][:, 0] ** 2
+    r /= C
+    rel = np.abs(g - r) / np.abs(r)
+    print(f"[gpu] FP64 correlator max rel err {rel.max():.3e}", flush=True)
+    assert rel.max() < 1e-12
+    ens.close()
+
+
+def test_ensemble_reference_fields_ranlxs(ctx, orc):
+    """Reference-faithful fields: one ranlxs lux-2 stream seeded 7012, consumed site by site
+    and slice by slice (lattice.C:29,81-86; production/slater_det/main.C:218)."""
+    import nrcps_amd as na
+
+    L, N, T, C = 8, 7, 12, 3
+    V = L ** 3
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, 0)
+    rng = orc.ranlxs(2, 7012)
+    phis = np.zeros((C, T, V), np.int8)
+    import ctypes
+
+    for c in range(C):
+        for t in range(T):
+            orc.lib().orc_ranlxs_z2_field(ctypes.byref(rng), V, phis[c, t])
+    words = np.zeros((T, C, (V + 31) // 32), np.uint32)
+    bits = (phis > 0).astype(np.uint64)
+    for c in range(C):
+        for t in range(T):
+            for w in range(words.shape[2]):
+                chunk = bits[c, t, 32 * w: 32 * w + 32]
+                words[t, c, w] = int(sum(int(b) << j for j, b in enumerate(chunk)))
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F64)
+    la, ph, _ = ens.run_fields(words, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, field="array", phi=phis[c])
+        _compare((la[:, c], ph[:, c]), ref, 1e-12, 1e-12)
+
+
+def test_ensemble_smeared_interaction(ctx, orc):
+    """Interaction with the production multi-coupling form factor (hamiltonian.C:77-147)."""
+    import nrcps_amd as na
+
+    L, N, T, C = 12, 7, 10, 2
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, 0)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0)
+    assert rc == 0
+    f = f * 0.05  # a weaker coupling keeps (1 + smeared field) positive on this lattice
+    ens = na.Ensemble(ctx, L, moms, kin, 0.0, C, prec=na.PREC_F64, smear=f)
+    la, ph, _ = ens.run(5, 0, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, smear=f, seed=5, config=c)
+        _compare((la[:, c], ph[:, c]), ref, 1e-11, 1e-11)
+
+
+def test_ensemble_orbitals_vs_oracle(ctx, orc):
+    import nrcps_amd as na
+
+    L, N, T = 12, 7, 6
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, 0)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, 2, prec=na.PREC_F64)
+    ens.run(3, 40, T)
+    ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=3, config=41, want_psi=True)["psi"]
+    got = ens.orbitals(1)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("split", [0, 1])
+def test_ensemble_f32_config3_shape(ctx, orc, split):
+    """L=32, N=33, FP32 evolution: per-config det^2 relative error against FP64."""
+    import nrcps_amd as na
+
+    L, N, T, C = 32, 33, 16, 2
+    print(f"[gpu] ensemble f32 L=32 N=33 split={split}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, split)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32, split=split)
+    la, ph, _ = ens.run(21, 0, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, seed=21, config=c)
+        d2 = np.exp(2 * la[:, c, 0]) * ph[:, c, 0] ** 2
+        r2 = np.exp(2 * ref["logabs"][:, 0]) * ref["phase"][:, 0] ** 2
+        rel = np.abs(d2 - r2) / np.abs(r2)
+        print(f"[gpu] FP32 det^2 rel err by t (config {c}): max {rel.max():.2e}", flush=True)
+        assert rel.max() < 1e-3
+
+
+def test_ensemble_full_size_free_gas(ctx, orc):
+    """A = 0 at L=32, N=33, T=128: det M(t) = prod_n (1/xi_half(k_n))^(2t) exactly."""
+    import nrcps_amd as na
+
+    L, N, T, C = 32, 33, 128, 8
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, 0)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.0, C, prec=na.PREC_F32)
+    la, ph, _ = ens.run(1, 0, T)
+    idx = (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]
+    exact = -2 * np.arange(T)[:, None] * np.log(kin[idx]).sum()
+    err = np.abs(la[:, :, 0] - exact).max()
+    print(f"[gpu] FP32 free gas log|det| max abs error over T=128: {err:.2e}", flush=True)
+    assert err < 3e-5
+    assert np.abs(ph[:, :, 0] - 1).max() < 1e-5
+    assert np.all(la[0] == 0.0)
