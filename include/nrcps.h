@@ -72,6 +72,9 @@ int nrc_scale_real(nrc_context* ctx, void* buf_dev, const void* table_dev, int b
                    int op, int prec);
 int nrc_mul_complex(nrc_context* ctx, void* buf_dev, const void* field_dev, int batch, long vol,
                     int prec);
+/* field = (field + 1)/V in place (re), field/V (im): the end of Interaction::Initialize
+ * (src/hamiltonian/cuda/hamiltonian_initialize.cu:30-35, hamiltonian_kernels.cu:59-71). */
+int nrc_plus_one_div_vol(nrc_context* ctx, void* buf_dev, long vol, int prec);
 
 /* ---- overlaps: OneBody::Project, Cuda::InnerProduct and the SlaterDet constructor
  *      (src/one_body/cuda/one_body_project.cu:17-34; src/utils/cuda/cuda_utils.cu:88-127;
@@ -134,8 +137,16 @@ int nrc_ensemble_run_fields(nrc_ensemble* ens, const uint32_t* words, int T, dou
 /* Orbitals of configuration c after the last run, N x V complex in the ensemble precision,
  * in storage space (k-space for the reference split). */
 int nrc_ensemble_orbitals(nrc_ensemble* ens, int c, void* out);
+/* Kernel path for the evolution: NRC_PATH_SINGLE_PASS evolves each slice in one kernel that
+ * reads and writes every orbital once (FP32 L <= 32, FP64 L <= 16); NRC_PATH_TWO_PASS uses an
+ * X pass and a YZ pass per slice (every supported shape); NRC_PATH_AUTO picks per shape. */
+#define NRC_PATH_AUTO 0
+#define NRC_PATH_TWO_PASS 1
+#define NRC_PATH_SINGLE_PASS 2
+int nrc_ensemble_set_path(nrc_ensemble* ens, int path);
 /* Per-kernel timing with HIP events on the ensemble's stream (enable before a run).
- * kind: 0 X pass, 1 YZ pass, 2 LU, 3 field generation.  Returns total ms and launches. */
+ * kind: 0 X pass or single-pass slice kernel, 1 YZ pass, 2 LU, 3 field generation,
+ * 4 sink gather.  Returns total ms and launches. */
 int nrc_ensemble_profile(nrc_ensemble* ens, int enable);
 int nrc_ensemble_kernel_time(nrc_ensemble* ens, int kind, double* total_ms, long* launches);
 int nrc_ensemble_destroy(nrc_ensemble* ens);

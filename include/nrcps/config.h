@@ -1,0 +1,3 @@
+// config.h -- the reference header of this name (include/config.h), served by the MI355X class
+// layer declared in nrcps_host.h.
+#include "nrcps_host.h"

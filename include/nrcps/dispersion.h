@@ -1,0 +1,3 @@
+// dispersion.h -- the reference header of this name (include/dispersion.h), served by the MI355X class
+// layer declared in nrcps_host.h.
+#include "nrcps_host.h"

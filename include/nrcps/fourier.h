@@ -1,0 +1,3 @@
+// fourier.h -- the reference header of this name (include/fourier.h), served by the MI355X class
+// layer declared in nrcps_host.h.
+#include "nrcps_host.h"

@@ -1,0 +1,3 @@
+// uniform_deviates.h -- the reference header of this name (include/uniform_deviates.h), served by the MI355X class
+// layer declared in nrcps_host.h.
+#include "nrcps_host.h"

@@ -1,0 +1,3 @@
+// vector3.h -- the reference header of this name (include/vector3.h), served by the MI355X class
+// layer declared in nrcps_host.h.
+#include "nrcps_host.h"

@@ -19,6 +19,7 @@ OK = 0
 PREC_F32, PREC_F64 = 0, 1
 SPLIT_REFERENCE, SPLIT_BASELINE = 0, 1
 OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3 = 0, 1, 2
+PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
 
 # every symbol include/nrcps.h declares
@@ -26,8 +27,10 @@ EXPORTS = [
     "nrc_init", "nrc_finalize", "nrc_error_string", "nrc_device_name", "nrc_malloc", "nrc_free",
     "nrc_memcpy_h2d", "nrc_memcpy_d2h", "nrc_memcpy_d2d", "nrc_memset", "nrc_sync",
     "nrc_fft_plan", "nrc_fft_exec", "nrc_fft_destroy", "nrc_scale_real", "nrc_mul_complex",
+    "nrc_plus_one_div_vol",
     "nrc_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
-    "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_profile",
+    "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
+    "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_destroy",
 ]
 
@@ -64,6 +67,7 @@ def load_library() -> C.CDLL:
         "nrc_fft_destroy": [vp],
         "nrc_scale_real": [vp, vp, vp, i, C.c_long, i, i],
         "nrc_mul_complex": [vp, vp, vp, i, C.c_long, i],
+        "nrc_plus_one_div_vol": [vp, vp, C.c_long, i],
         "nrc_overlap": [vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_lu_det": [vp, vp, i, i, i, i, vp, vp, vp, vp],
         "nrc_ludcmp": [vp, vp, i, i, vp, vp, vp], "nrc_ref_abs": [vp, vp, vp, C.c_long],
@@ -71,7 +75,8 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_create": [vp, C.POINTER(EnsembleDesc), C.POINTER(vp)],
         "nrc_ensemble_run": [vp, u64, u32, i, dp, dp, dp],
         "nrc_ensemble_run_fields": [vp, vp, i, dp, dp, dp],
-        "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_profile": [vp, i],
+        "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
+        "nrc_ensemble_profile": [vp, i],
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "nrc_ensemble_destroy": [vp],
     }
@@ -252,6 +257,10 @@ class Ensemble:
         out = np.zeros((self.N, self.L ** 3), dtype=dt)
         _check(self.lib.nrc_ensemble_orbitals(self.h, c, _ptr(out)), "ensemble_orbitals")
         return out
+
+    def set_path(self, path: int):
+        """PATH_AUTO, PATH_TWO_PASS or PATH_SINGLE_PASS (nrc_ensemble_set_path)."""
+        _check(self.lib.nrc_ensemble_set_path(self.h, path), "ensemble_set_path")
 
     def profile(self, enable: bool):
         _check(self.lib.nrc_ensemble_profile(self.h, int(enable)), "ensemble_profile")

@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libnrcps_hip.so")
-SOURCES = ["fft_passes.hip", "lu_rng.hip", "overlap_mfma.hip", "api.hip"]
+SOURCES = ["fft_passes.hip", "slice_fused.hip", "lu_rng.hip", "overlap_mfma.hip", "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRC_OFFLOAD_ARCH", "gfx950")
 
@@ -33,8 +33,17 @@ def _run(cmd):
     return r
 
 
+HOST_LIB = os.path.join(PKG, "libnrcps.so")
+HOST_SOURCES = ["host/util.cpp", "host/physics.cpp"]
+DRIVER = os.path.join(PKG, "drivers", "slater_det")
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+            f"-I{os.path.join(ROOT, 'include')}"]
+
+
 def build(jobs: int = 4) -> str:
-    """Compiles every source to an object (in parallel) and links the shared library."""
+    """Compiles every HIP source to an object (in parallel) and links libnrcps_hip.so, then
+    the host class layer libnrcps.so and the slater_det driver on top of it."""
     objs, procs = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
@@ -48,7 +57,21 @@ def build(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, *FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+def build_host() -> None:
+    """The C++ class layer (reference class API over the C ABI) and its driver."""
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX, *CXXFLAGS, "-shared", "-o", tmp, *[os.path.join(PKG, s) for s in HOST_SOURCES],
+          f"-L{PKG}", "-lnrcps_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    tmp = DRIVER + ".tmp"
+    _run([CXX, *CXXFLAGS, f"-I{os.path.join(ROOT, 'include', 'nrcps')}", "-o", tmp,
+          os.path.join(PKG, "drivers", "slater_det.cpp"), f"-L{PKG}", "-lnrcps", "-lnrcps_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, DRIVER)
 
 
 def _wait(procs):

@@ -65,11 +65,6 @@ int nrc_init(int device, nrc_context** ctx) {
     delete c;
     return NRC_ERR_HIP;
   }
-  if (nrc::upload_twiddles() != hipSuccess) {
-    hipStreamDestroy(c->stream);
-    delete c;
-    return NRC_ERR_HIP;
-  }
   *ctx = c;
   return NRC_OK;
 }
@@ -181,6 +176,12 @@ int nrc_mul_complex(nrc_context* ctx, void* buf, const void* field, int batch, l
   return NRC_OK;
 }
 
+int nrc_plus_one_div_vol(nrc_context* ctx, void* buf, long vol, int prec) {
+  if (!ctx || !buf || vol < 1 || !prec_ok(prec)) return NRC_ERR_ARG;
+  NRC_HIP(nrc::launch_plus_one_div_vol(prec, buf, vol, ctx->stream));
+  return NRC_OK;
+}
+
 // ------------------------------------------------------------------ overlaps, LU, fields
 int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns, int nt,
                 long vol, int ncfg, int mode) {
@@ -259,9 +260,10 @@ struct nrc_ensemble {
   int* g_sink = nullptr;
   int* site_of_orb = nullptr;
   int* moms_dev = nullptr;
+  int path = NRC_PATH_AUTO;
   bool profile = false;
-  double prof_ms[4] = {0, 0, 0, 0};
-  long prof_n[4] = {0, 0, 0, 0};
+  double prof_ms[5] = {0, 0, 0, 0, 0};
+  long prof_n[5] = {0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev;
   std::vector<int> ev_kind;
 };
@@ -404,15 +406,23 @@ extern "C" int nrc_ensemble_destroy(nrc_ensemble* e) {
   return NRC_OK;
 }
 
+extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
+  if (!e || path < NRC_PATH_AUTO || path > NRC_PATH_SINGLE_PASS) return NRC_ERR_ARG;
+  if (path == NRC_PATH_SINGLE_PASS && !nrc::slice1_supported(e->d.prec, e->d.L))
+    return NRC_ERR_UNSUPPORTED;
+  e->path = path;
+  return NRC_OK;
+}
+
 extern "C" int nrc_ensemble_profile(nrc_ensemble* e, int enable) {
   if (!e) return NRC_ERR_ARG;
   e->profile = enable != 0;
-  for (int k = 0; k < 4; ++k) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
+  for (int k = 0; k < 5; ++k) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
   return NRC_OK;
 }
 
 extern "C" int nrc_ensemble_kernel_time(nrc_ensemble* e, int kind, double* ms, long* n) {
-  if (!e || kind < 0 || kind > 3) return NRC_ERR_ARG;
+  if (!e || kind < 0 || kind > 4) return NRC_ERR_ARG;
   if (ms) *ms = e->prof_ms[kind];
   if (n) *n = e->prof_n[kind];
   return NRC_OK;
@@ -589,7 +599,58 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   fieldspec.kind = nrc::MUL_FIELD;
   fieldspec.nw = e->nw;
 
-  for (int t = 0; t <= T; ++t) {
+  const bool single = e->path == NRC_PATH_SINGLE_PASS ||
+                      (e->path == NRC_PATH_AUTO && nrc::slice1_supported(prec, L));
+  // single pass: every slice is one kernel over the orbitals (slice_fused.hip)
+  for (int t = 0; single && t < T; ++t) {
+    if (ref) {
+      {
+        EvScope sc(e, 4);
+        NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
+      }
+      NRC_TRY(lu(t));
+    }
+    if (!words_host) {
+      EvScope sc(e, 3);
+      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
+    }
+    if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
+    nrc::SliceArgs a;
+    a.psi = e->psi;
+    a.nvol = C * N;
+    a.norb = N;
+    a.L = L;
+    if (ref) {
+      a.pre = mul_table(e->tab_x, e->tab_x_lo);
+      a.post = a.pre;
+      if (e->smear) {
+        a.mid.kind = nrc::MUL_RFIELD;
+        a.mid.table = e->rfield;
+      } else {
+        a.mid = fieldspec;
+        a.mid.bits = field_of(t);
+        a.mid.a0 = 1.0 / (double)V;
+        a.mid.a1 = e->d.A / (double)V;
+      }
+    } else {
+      a.mid = mul_table(e->tab_yz, e->tab_yz_lo);
+      a.gather_mid = 1;
+      a.g = g;
+      a.post = fieldspec;
+      a.post.bits = field_of(t);
+      a.post.a0 = 1.0;
+      a.post.a1 = e->d.A;
+    }
+    {
+      EvScope sc(e, 0);
+      NRC_HIP(nrc::launch_slice1(prec, a, s));
+    }
+    if (!ref) NRC_TRY(lu(t));
+  }
+
+  // two passes per slice: an X pass that finishes slice t-1, measures t and starts slice t,
+  // then a YZ pass
+  for (int t = 0; !single && t <= T; ++t) {
     const bool last = t == T;
     // X pass: finish slice t-1, measure t (reference split), start slice t
     {

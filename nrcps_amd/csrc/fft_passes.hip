@@ -14,82 +14,9 @@
 //
 // Both passes only stream: there is no data reuse between workgroups, so blocks map to
 // XCDs in launch order without remapping.
-#include "nrc_device.h"
-#include "nrc_internal.h"
-
-__constant__ nrc::C2<double> c_tw64[nrc::TW_TOTAL];
-__constant__ nrc::C2<float> c_tw32[nrc::TW_TOTAL];
-__constant__ nrc::C2<float> c_tw32_lo[nrc::TW_TOTAL];
+#include "nrc_ops.h"
 
 namespace nrc {
-
-hipError_t upload_twiddles() {
-  static const int Ls[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
-  C2<double> h64[TW_TOTAL];
-  C2<float> h32[TW_TOTAL], l32[TW_TOTAL];
-  for (int L : Ls)
-    for (int e = 0; e < L; ++e) {
-      const double a = 2.0 * 3.14159265358979323846 * (double)e / (double)L;
-      const double c = cos(a), s = sin(a);
-      h64[tw_offset(L) + e] = {c, s};
-      h32[tw_offset(L) + e] = {(float)c, (float)s};
-      l32[tw_offset(L) + e] = {(float)(c - (double)(float)c), (float)(s - (double)(float)s)};
-    }
-  hipError_t err = hipMemcpyToSymbol(HIP_SYMBOL(c_tw64), h64, sizeof(h64));
-  if (err != hipSuccess) return err;
-  err = hipMemcpyToSymbol(HIP_SYMBOL(c_tw32_lo), l32, sizeof(l32));
-  if (err != hipSuccess) return err;
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_tw32), h32, sizeof(h32));
-}
-
-// ------------------------------------------------------------------ per-site multipliers
-// v * (hi + lo): hi carries the factor, lo its FP32 remainder (zero in FP64 passes)
-template <typename T>
-__device__ __forceinline__ C2<T> scale2(C2<T> v, T hi, T lo) {
-  if constexpr (sizeof(T) == 4) {
-    return {fmaf(v.x, lo, v.x * hi), fmaf(v.y, lo, v.y * hi)};
-  } else {
-    return {v.x * hi, v.y * hi};
-  }
-}
-
-// v[i] *= multiplier at site(i) = i * stride + off
-template <typename T, int L>
-__device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, long off,
-                                          long stride, long V) {
-  if (m.kind == MUL_NONE) return;
-  if (m.kind == MUL_FIELD) {
-    const double dp = m.a0 + m.a1, dm = m.a0 - m.a1;
-    const T ph = T(dp), mh = T(dm);
-    const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
-    const uint32_t* bits = m.bits + cfg * m.nw;
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-      const long site = i * stride + off;
-      const bool up = (bits[site >> 5] >> (site & 31)) & 1u;
-      v[i] = scale2<T>(v[i], up ? ph : mh, up ? pl : ml);
-    }
-    return;
-  }
-  const T* tab = reinterpret_cast<const T*>(m.table) + (m.kind == MUL_RFIELD ? cfg * V : 0);
-  const T* lo = reinterpret_cast<const T*>(m.table_lo);
-#pragma unroll
-  for (int i = 0; i < L; ++i) {
-    const long site = i * stride + off;
-    v[i] = scale2<T>(v[i], tab[site], lo ? lo[site] : T(0));
-  }
-}
-
-template <typename T, int L>
-__device__ __forceinline__ void gather_pencil(const C2<T>* v, const GatherSpec& g, int pencil,
-                                              long cfg, int orb, int norb) {
-  const int e0 = g.ptr[pencil], e1 = g.ptr[pencil + 1];
-  for (int e = e0; e < e1; ++e) {
-    const C2<T> val = reg_select<T, L>(v, g.pos[e]);
-    g.M[(cfg * g.nsink + g.sink[e]) * norb + orb] =
-        make_double2((double)val.x * g.scale, (double)val.y * g.scale);
-  }
-}
 
 // ------------------------------------------------------------------ X pass
 template <typename T, int L>
@@ -275,6 +202,28 @@ hipError_t launch_mul_complex(int prec, void* buf, const void* field, long batch
   else
     hipLaunchKernelGGL(k_mul_complex<float>, dim3(grid_for(n)), dim3(256), 0, s,
                        (C2<float>*)buf, (const C2<float>*)field, n, vol);
+  return hipGetLastError();
+}
+
+// field = (field + 1) / V: the last step of Interaction::Initialize
+// (src/hamiltonian/cuda/hamiltonian_kernels.cu:59-71)
+template <typename T>
+__global__ void k_plus_one_div_vol(C2<T>* f, long vol) {
+  const T inv = T(1) / (T)vol;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < vol;
+       i += (long)gridDim.x * blockDim.x) {
+    const C2<T> v = f[i];
+    f[i] = {(v.x + T(1)) / (T)vol, v.y * inv};
+  }
+}
+
+hipError_t launch_plus_one_div_vol(int prec, void* buf, long vol, hipStream_t s) {
+  if (prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_plus_one_div_vol<double>, dim3(grid_for(vol)), dim3(256), 0, s,
+                       (C2<double>*)buf, vol);
+  else
+    hipLaunchKernelGGL(k_plus_one_div_vol<float>, dim3(grid_for(vol)), dim3(256), 0, s,
+                       (C2<float>*)buf, vol);
   return hipGetLastError();
 }
 

@@ -2,9 +2,8 @@
 //
 //   * C2<T>            interleaved complex value (re, im) of float or double;
 //   * fft_reg<T,L,S>   fully unrolled in-register Stockham DFT of length L, sign S;
-//   * twiddle tables   e^{2 pi i e / L} per supported L in __constant__ memory (host fills
-//                      them in FP64 with libm at context creation; the FP32 table holds the
-//                      correctly rounded FP64 values).
+//   * twiddles         e^{2 pi i e / L} evaluated at compile time (kTw<L>), folded into the
+//                      unrolled transforms as literals.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,24 +55,55 @@ __host__ __device__ constexpr int fft_ns_before(int L, int s) {
   for (int i = 0; i < s; ++i) n *= fft_radix(L, i);
   return n;
 }
-// offsets of each length's table inside the twiddle arrays
-__host__ __device__ constexpr int tw_offset(int L) {
-  return L == 4 ? 0 : L == 6 ? 4 : L == 8 ? 10 : L == 12 ? 18 : L == 16 ? 30 : L == 24 ? 46
-       : L == 32 ? 70 : L == 48 ? 102 : L == 64 ? 150 : -1;
+
+// ------------------------------------------------------------------ compile-time twiddles
+// cos / sin of 2 pi e / L evaluated at compile time: exact integer reduction to the first
+// octant, then Taylor series (|error| < 2e-16).  Inside the unrolled transforms e is a
+// constant, so every twiddle folds into instruction literals: no tables, no registers held.
+constexpr double CX_PI_2 = 1.5707963267948966192313216916398;
+__host__ __device__ constexpr double cx_sin_poly(double x) {
+  double x2 = x * x, t = x, s = x;
+  for (int k = 1; k < 16; ++k) { t *= -x2 / ((2.0 * k) * (2.0 * k + 1.0)); s += t; }
+  return s;
 }
-constexpr int TW_TOTAL = 214;  // 4+6+8+12+16+24+32+48+64
-
-}  // namespace nrc
-
-// cos, sin of 2 pi e / L for e < L, per supported L at tw_offset(L).  The FP32 table holds the
-// nearest floats and c_tw32_lo the floats nearest to the remainders: an FP32 twiddle is
-// applied as a*(hi + lo), so |w| = 1 to ~2^-48 and the forward/backward pair of every slice
-// does not shrink or grow the orbitals systematically.
-extern __constant__ nrc::C2<double> c_tw64[nrc::TW_TOTAL];
-extern __constant__ nrc::C2<float> c_tw32[nrc::TW_TOTAL];
-extern __constant__ nrc::C2<float> c_tw32_lo[nrc::TW_TOTAL];
-
-namespace nrc {
+__host__ __device__ constexpr double cx_cos_poly(double x) {
+  double x2 = x * x, t = 1.0, s = 1.0;
+  for (int k = 1; k < 16; ++k) { t *= -x2 / ((2.0 * k - 1.0) * (2.0 * k)); s += t; }
+  return s;
+}
+struct CxCS { double c, s; };
+__host__ __device__ constexpr CxCS cx_sincos(int e, int L) {
+  e = ((e % L) + L) % L;
+  const int n = 4 * e, q = n / L, r = n - q * L;  // angle = (pi/2) (q + r/L)
+  double cb = 0.0, sb = 0.0;
+  if (2 * r <= L) {
+    const double th = CX_PI_2 * r / L;
+    cb = cx_cos_poly(th);
+    sb = cx_sin_poly(th);
+  } else {
+    const double th = CX_PI_2 * (L - r) / L;
+    cb = cx_sin_poly(th);
+    sb = cx_cos_poly(th);
+  }
+  switch (q & 3) {
+    case 0: return {cb, sb};
+    case 1: return {-sb, cb};
+    case 2: return {-cb, -sb};
+    default: return {sb, -cb};
+  }
+}
+template <int L>
+struct TwTable {
+  CxCS w[L > 0 ? L : 1];
+};
+template <int L>
+__host__ __device__ constexpr TwTable<L> make_tw() {
+  TwTable<L> t{};
+  for (int e = 0; e < L; ++e) t.w[e] = cx_sincos(e, L);
+  return t;
+}
+template <int L>
+inline constexpr TwTable<L> kTw = make_tw<L>();
 
 // x * (hi + lo) for an FP32 constant split into its nearest float and remainder
 template <typename T>
@@ -82,23 +112,31 @@ __device__ __forceinline__ T mulk(T x, T hi, T lo) {
   else return x * hi;
 }
 
-// a * e^{S 2 pi i e / L}; e is a compile-time constant once the caller is unrolled
-template <typename T, int L, int S>
-__device__ __forceinline__ C2<T> twiddle(C2<T> a, int e) {
-  if (e == 0) return a;
-  if (2 * e == L) return {-a.x, -a.y};
-  if (4 * e == L) return cmul_i<T, S>(a);
-  if (4 * e == 3 * L) return cmul_i<T, -S>(a);
-  if constexpr (sizeof(T) == 4) {
-    C2<float> h = c_tw32[tw_offset(L) + e], l = c_tw32_lo[tw_offset(L) + e];
-    if (S < 0) { h.y = -h.y; l.y = -l.y; }
-    const float re = fmaf(a.x, l.x, fmaf(-a.y, l.y, fmaf(a.x, h.x, -a.y * h.y)));
-    const float im = fmaf(a.x, l.y, fmaf(a.y, l.x, fmaf(a.x, h.y, a.y * h.x)));
-    return {re, im};
+// a * e^{S 2 pi i e / L}; e is a compile-time constant once the caller is unrolled.
+// In FP32 the twiddle is applied as a*(hi + lo) with lo the FP32 remainder of the FP64 value,
+// so |w| = 1 to ~2^-48: the forward/backward pair of every slice then does not shrink or grow
+// the orbitals systematically, which rounded twiddles alone do by ~1e-7 per slice.
+template <typename T, int L, int S, int E>
+__device__ __forceinline__ C2<T> twiddle_c(C2<T> a) {
+  if constexpr (E == 0) {
+    return a;
+  } else if constexpr (2 * E == L) {
+    return {-a.x, -a.y};
+  } else if constexpr (4 * E == L) {
+    return cmul_i<T, S>(a);
+  } else if constexpr (4 * E == 3 * L) {
+    return cmul_i<T, -S>(a);
   } else {
-    C2<double> w = c_tw64[tw_offset(L) + e];
-    w.y = S > 0 ? w.y : -w.y;
-    return cmul(a, w);
+    constexpr double wc = kTw<L>.w[E].c, ws = (S > 0 ? 1.0 : -1.0) * kTw<L>.w[E].s;
+    if constexpr (sizeof(T) == 4) {
+      constexpr float hx = (float)wc, hy = (float)ws;
+      constexpr float lx = (float)(wc - (double)hx), ly = (float)(ws - (double)hy);
+      const float re = fmaf(a.x, lx, fmaf(-a.y, ly, fmaf(a.x, hx, -a.y * hy)));
+      const float im = fmaf(a.x, ly, fmaf(a.y, lx, fmaf(a.x, hy, a.y * hx)));
+      return {re, im};
+    } else {
+      return cmul(a, C2<double>{wc, ws});
+    }
   }
 }
 
@@ -169,23 +207,34 @@ struct Dft<T, 8, S> {
 };
 
 // ------------------------------------------------------------------ Stockham stage in registers
-template <typename T, int L, int S, int STAGE>
-__device__ __forceinline__ void stockham_stage(const C2<T>* in, C2<T>* out) {
+// The loops over butterflies (J) and their inputs (RR) are template recursions, so every
+// twiddle exponent is a constant expression and its value a literal.
+template <typename T, int L, int S, int NR, int K, int TSTEP, int J, int RR, int R>
+__device__ __forceinline__ void stockham_load(const C2<T>* in, C2<T>* v) {
+  constexpr int E = (RR * K * TSTEP) % L;
+  v[RR] = twiddle_c<T, L, S, E>(in[J + RR * NR]);
+  if constexpr (RR + 1 < R) stockham_load<T, L, S, NR, K, TSTEP, J, RR + 1, R>(in, v);
+}
+
+template <typename T, int L, int S, int STAGE, int J>
+__device__ __forceinline__ void stockham_butterfly(const C2<T>* in, C2<T>* out) {
   constexpr int R = fft_radix(L, STAGE);
   constexpr int NS = fft_ns_before(L, STAGE);
   constexpr int NR = L / R;
   constexpr int TSTEP = L / (NS * R);
+  constexpr int K = J % NS;
+  C2<T> v[R];
+  stockham_load<T, L, S, NR, K, TSTEP, J, 0, R>(in, v);
+  Dft<T, R, S>::run(v);
+  constexpr int D = (J / NS) * NS * R + K;
 #pragma unroll
-  for (int j = 0; j < NR; ++j) {
-    const int k = j % NS;
-    C2<T> v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = twiddle<T, L, S>(in[j + r * NR], (r * k * TSTEP) % L);
-    Dft<T, R, S>::run(v);
-    const int d = (j / NS) * NS * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) out[d + r * NS] = v[r];
-  }
+  for (int r = 0; r < R; ++r) out[D + r * NS] = v[r];
+  if constexpr (J + 1 < NR) stockham_butterfly<T, L, S, STAGE, J + 1>(in, out);
+}
+
+template <typename T, int L, int S, int STAGE>
+__device__ __forceinline__ void stockham_stage(const C2<T>* in, C2<T>* out) {
+  stockham_butterfly<T, L, S, STAGE, 0>(in, out);
 }
 
 // In-place (by value semantics) unnormalised DFT of v[0..L-1]:

@@ -56,12 +56,32 @@ hipError_t launch_xpass(int prec, const PassArgs& a, hipStream_t s);
 hipError_t launch_yzpass(int prec, const PassArgs& a, hipStream_t s);
 bool pass_supported(int L);
 
+// One whole slice per orbital volume in a single HBM pass (slice_fused.hip):
+// pre multiplier, x/y/z DFT (-1), mid multiplier, DFT (+1), post multiplier, with the sink
+// gather at the start (gather_pre, x-pencils) or between the transforms (gather_mid,
+// (x, z) columns).
+struct SliceArgs {
+  void* psi = nullptr;
+  long nvol = 0;
+  int norb = 1;
+  int L = 0;
+  MulSpec pre, mid, post;
+  int gather_pre = 0, gather_mid = 0;
+  GatherSpec g;
+};
+bool slice1_supported(int prec, int L);
+hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);
+// M[c][s][n] = psi[c][n][site_of_sink[s]] (plane-wave sinks on k-space orbitals)
+hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink, double2* M,
+                               long C, int N, long V, hipStream_t s);
+
 // element-wise helpers (the reference's separate kernels, used by the class layer)
 hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch, long vol,
                              int op, hipStream_t s);
 hipError_t launch_mul_complex(int prec, void* buf, const void* field, long batch, long vol,
                               hipStream_t s);
 hipError_t launch_fill(void* p, uint8_t v, size_t bytes, hipStream_t s);
+hipError_t launch_plus_one_div_vol(int prec, void* buf, long vol, hipStream_t s);
 hipError_t launch_set_unit_sources(int prec, void* psi, const int* site_of_orb, long nvol,
                                    int norb, long vol, hipStream_t s);
 hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvol, int norb,
@@ -84,7 +104,5 @@ hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, doub
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
                                  int nt, long vol, int ncfg, float* partial, hipStream_t s);
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
-
-hipError_t upload_twiddles();
 
 }  // namespace nrc

@@ -1,0 +1,707 @@
+// physics.cpp -- Random/Ranlxs, Lattice, Fourier, Dispersion, Kinetic, Interaction,
+// Potential, Propagator, OneBody, SlaterDet, ludcmp and Momenta of the host class layer.
+// Device work goes through the C ABI (nrcps.h); every status is checked with ERR.Check,
+// which terminates like the reference's fatal errors.
+//
+// Reference behaviour: random.C:12-181, ranlxs.C:49-265, lattice.C:15-109, fourier.C:6-101
+// (FFTW conventions) / fourier.cu:9-79, dispersion.C:11-93, hamiltonian.C:77-281,
+// hamiltonian_evolve.cu:18-104, hamiltonian_initialize.cu:17-39, propagator.C:18-136,
+// one_body.C:21-316, one_body_project.cu:17-34, slater_det.C:12-128, ludcmp.C:9-65,
+// momenta.C:12-190.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <limits>
+
+#include "../../include/nrcps/nrcps_host.h"
+
+static nrc_context* ctx() { return nrc_host_context(); }
+
+template <typename X>
+static X* dev_alloc(size_t n, const char* fname) {
+  void* p = nullptr;
+  ERR.Check(fname, nrc_malloc(ctx(), &p, n * sizeof(X)));
+  return (X*)p;
+}
+
+template <typename X>
+static void upload(X* dst, const std::vector<X>& src, const char* fname) {
+  ERR.Check(fname, nrc_memcpy_h2d(ctx(), dst, src.data(), src.size() * sizeof(X)));
+}
+
+// ------------------------------------------------------------------ Ranlxs
+// Luscher's ranlxs: four interleaved 24-bit subtract-with-borrow lanes x_n = x_{n-10} -
+// x_{n-24} - c, with pr pair-steps per refill of 96 outputs (pr = 109, 202, 397 for luxury
+// 0, 1, 2).  State layout and seeding as the reference so that sequences and the 105-long
+// state vector agree.
+static const int RLX_BASE = 0x1000000, RLX_MASK = 0xffffff;
+
+Ranlxs::Ranlxs(int lv) : pr(0), prm(0), ir(0), jr(7), is(95), is_old(0), level(lv) {
+  if (lv < 0 || lv > 2) ERR.General("Ranlxs::Ranlxs(int)", "Bad luxury level (0, 1 or 2).");
+  Init(1);
+}
+
+void Ranlxs::Init(long seed) {
+  if (seed <= 0 || seed > 2147483647L)
+    ERR.General("Ranlxs::Init(long)", "Bad choice of seed (should be between 1 and 2^31-1).");
+  pr = level == 0 ? 109 : level == 1 ? 202 : 397;
+  int bits[31];
+  long s = seed;
+  for (int k = 0; k < 31; ++k) {
+    bits[k] = (int)(s % 2);
+    s /= 2;
+  }
+  int ib = 0, jb = 18;
+  for (int lane = 0; lane < 4; ++lane)
+    for (int k = 0; k < 24; ++k) {
+      int v = 0;
+      for (int l = 0; l < 24; ++l) {
+        v = 2 * v + bits[ib];
+        bits[ib] = (bits[ib] + bits[jb]) % 2;
+        ib = (ib + 1) % 31;
+        jb = (jb + 1) % 31;
+      }
+      num[4 * k + lane] = (k % 4 == lane) ? RLX_MASK - v : v;
+    }
+  for (int l = 0; l < 4; ++l) carry[l] = 0;
+  ir = 0;
+  jr = 7;
+  is = 95;
+  is_old = 0;
+  prm = pr % 12;
+}
+
+void Ranlxs::Update() {
+  int i = ir, j = jr;
+  for (int step = 0; step < pr; ++step) {
+    int* xi = num + 8 * i;
+    const int* xj = num + 8 * j;
+    for (int l = 0; l < 4; ++l) {
+      int d = xj[l] - xi[l] - carry[l];
+      if (d < 0) {
+        xi[4 + l] += 1;
+        d += RLX_BASE;
+      }
+      xi[l] = d & RLX_MASK;
+    }
+    for (int l = 0; l < 4; ++l) {
+      const int d = xj[4 + l] - xi[4 + l];
+      carry[l] = d < 0;
+      xi[4 + l] = (d + RLX_BASE) & RLX_MASK;
+    }
+    i = (i + 1) % 12;
+    j = (j + 1) % 12;
+  }
+  ir = (ir + prm) % 12;
+  jr = (jr + prm) % 12;
+  is = 8 * ir;
+  is_old = is;
+}
+
+Float Ranlxs::Run() {
+  is = (is + 1) % 96;
+  if (is == is_old) Update();
+  return (Float)((float)std::ldexp(1.0, -24) * (float)num[is]);
+}
+
+void Ranlxs::GetState(long* st) {
+  st[0] = 105;
+  for (int k = 0; k < 96; ++k) st[k + 1] = num[k];
+  for (int l = 0; l < 4; ++l) st[97 + l] = carry[l];
+  st[101] = pr;
+  st[102] = ir;
+  st[103] = jr;
+  st[104] = is;
+}
+
+void Ranlxs::SetState(long* st) {
+  const char* fname = "Ranlxs::SetState(long*)";
+  if (st[0] != 105) ERR.General(fname, "Unexpected input data.");
+  for (int k = 0; k < 96; ++k)
+    if (st[k + 1] < 0 || st[k + 1] > RLX_MASK) ERR.General(fname, "Unexpected input data.");
+  for (int l = 0; l < 4; ++l)
+    if (st[97 + l] != 0 && st[97 + l] != 1) ERR.General(fname, "Unexpected input data.");
+  if ((st[101] != 109 && st[101] != 202 && st[101] != 397) || st[102] < 0 || st[102] > 11 ||
+      st[103] != (st[102] + 7) % 12 || st[104] < 0 || st[104] > 95)
+    ERR.General(fname, "Unexpected input data.");
+  for (int k = 0; k < 96; ++k) num[k] = (int)st[k + 1];
+  for (int l = 0; l < 4; ++l) carry[l] = (int)st[97 + l];
+  pr = (int)st[101];
+  ir = (int)st[102];
+  jr = (int)st[103];
+  is = (int)st[104];
+  is_old = 8 * ir;
+  prm = pr % 12;
+  level = pr == 109 ? 0 : pr == 202 ? 1 : 2;
+}
+
+RandomType Ranlxs::TypeQ() {
+  return level == 0 ? RANDOM_TYPE_RANLXS0 : level == 1 ? RANDOM_TYPE_RANLXS1 : RANDOM_TYPE_RANLXS2;
+}
+
+// ------------------------------------------------------------------ Random
+Random::Random(const RandomArg& a) : r(nullptr), random_type(a.random_type) {
+  const char* fname = "Random::Random(const RandomArg &)";
+  switch (a.random_type) {
+    case RANDOM_TYPE_RANLXS0: r = new Ranlxs(0); break;
+    case RANDOM_TYPE_RANLXS1: r = new Ranlxs(1); break;
+    case RANDOM_TYPE_RANLXS2: r = new Ranlxs(2); break;
+    default: ERR.NotImplemented(fname, "random_type (only RANLXS0/1/2 are built).");
+  }
+  long seed = 1;
+  switch (a.seed_type) {
+    case SEED_TYPE_DEFAULT: seed = 1; break;
+    case SEED_TYPE_INPUT:
+      seed = a.seed;
+      if (seed <= 0) ERR.General(fname, "Input seed must be positive.");
+      break;
+    case SEED_TYPE_TIME: seed = (long)std::time(nullptr); break;
+    case SEED_TYPE_FILE: break;
+  }
+  if (a.seed_type == SEED_TYPE_FILE) {
+    ReadState(a.file_stem.c_str());
+  } else {
+    seed += 491 * Comms::Rank();  // random.C:88
+    r->Init(seed);
+  }
+}
+
+Random::~Random() { delete r; }
+Float Random::Uniform() { return r->Run(); }
+
+Float Random::Gauss(Float sigma) {  // polar Box-Muller (random.C:99-111)
+  Float x, y, r2;
+  do {
+    x = 2 * Uniform() - 1;
+    y = 2 * Uniform() - 1;
+    r2 = x * x + y * y;
+  } while (r2 > 1.0 || r2 == 0);
+  return sigma * y * std::sqrt(-2.0 * std::log(r2) / r2);
+}
+
+int Random::Z(int p) { return (int)std::floor(p * Uniform()); }
+
+// <stem>.<rank>: RandomType (int) then long[StateSize] (random.C:134-181)
+void Random::WriteState(const char* stem) {
+  const char* fname = "Random::WriteState(const char*)";
+  std::vector<long> st(r->StateSize());
+  r->GetState(st.data());
+  FILE* f = Fopen(IO_TYPE_ALL, stem, "wb");
+  if (Fwrite(&random_type, sizeof(RandomType), 1, f) != 1)
+    ERR.General(fname, "Cannot write seed type to file.");
+  if ((int)Fwrite(st.data(), sizeof(long), st.size(), f) != (int)st.size())
+    ERR.General(fname, "Cannot write state to file.");
+  Fclose(f);
+}
+
+void Random::ReadState(const char* stem) {
+  const char* fname = "Random::ReadState(const char*)";
+  std::vector<long> st(r->StateSize());
+  RandomType t;
+  FILE* f = Fopen(IO_TYPE_ALL, stem, "rb");
+  if (Fread(&t, sizeof(RandomType), 1, f) != 1) ERR.General(fname, "Cannot read seed type.");
+  if (t != random_type) ERR.General(fname, "Random type of file does not match.");
+  if ((int)Fread(st.data(), sizeof(long), st.size(), f) != (int)st.size())
+    ERR.General(fname, "File may be corrupted.");
+  Fclose(f);
+  r->SetState(st.data());
+}
+
+// ------------------------------------------------------------------ Lattice
+Lattice::Lattice(const LatticeArg& a, Random* rng)
+    : rng_p(rng), field_type(a.field_type), field_size(2 * GJP.Vol()), field(field_size),
+      dev_field(dev_alloc<Float>(field_size, "Lattice::Lattice")) {
+  Refresh();
+}
+
+Lattice::~Lattice() { nrc_free(ctx(), dev_field); }
+
+// lattice.C:60-109: host draws in site order, then one copy to the device
+void Lattice::Refresh() {
+  if (field_type == FIELD_TYPE_GAUSS) {
+    for (int i = 0; i < field_size; i += 2) {
+      field[i] = rng_p->Gauss(1.0);
+      field[i + 1] = 0.0;
+    }
+  } else if (field_type == FIELD_TYPE_COMPLEXGAUSS) {
+    for (int i = 0; i < field_size; i += 2) {
+      field[i] = rng_p->Gauss(1.0);
+      field[i + 1] = rng_p->Gauss(1.0);
+    }
+  } else if (field_type == FIELD_TYPE_ZTWO) {
+    for (int i = 0; i < field_size; i += 2) {
+      field[i] = 2.0 * rng_p->Z(2) - 1.0;
+      field[i + 1] = 0.0;
+    }
+  } else {
+    for (int i = 0; i < field_size; i += 2) {
+      const int z = rng_p->Z(3);
+      field[i] = z == 0 ? 1.0 : -0.5;
+      field[i + 1] = z == 0 ? 0.0 : (z == 1 ? 0.8660254037844388 : -0.8660254037844388);
+    }
+  }
+  upload(dev_field, field, "void Lattice::Refresh()");
+}
+
+// shifted fields for importance sampling (lattice.C:111-165; the reference's GPU build
+// rejects these, here the shift is applied before the copy to the device)
+void Lattice::Refresh(Float shift) { Refresh(std::complex<Float>(shift, 0.0)); }
+
+void Lattice::Refresh(std::complex<Float> shift) {
+  std::vector<Float> s((size_t)field_size);
+  for (int i = 0; i < field_size; i += 2) {
+    s[i] = shift.real();
+    s[i + 1] = shift.imag();
+  }
+  Refresh(s.data());
+}
+
+void Lattice::Refresh(Float* shift) {
+  Refresh();
+  for (int i = 0; i < field_size; ++i) field[i] += shift[i];
+  upload(dev_field, field, "void Lattice::Refresh(Float*)");
+}
+
+Float* Lattice::Get() { return dev_field; }
+
+std::complex<Float> Lattice::Mean() {
+  Float re = 0.0, im = 0.0;
+  const int vol = field_size / 2;
+  for (int i = 0; i < vol; ++i) {
+    re += field[2 * i];
+    im += field[2 * i + 1];
+  }
+  return std::complex<Float>(re / vol, im / vol);
+}
+
+Float Lattice::Variance() {
+  const std::complex<Float> m = Mean();
+  Float v = 0.0;
+  for (int i = 0; i < field_size; ++i) v += field[i] * field[i];
+  return v / (field_size / 2) - std::norm(m);
+}
+
+// ------------------------------------------------------------------ Fourier
+Fourier::Fourier(int batch) : batch_size(batch), b(nullptr), plan(nullptr) {
+  const char* fname = "Fourier::Fourier(int)";
+  if (GJP.Xsites() != GJP.Ysites() || GJP.Ysites() != GJP.Zsites())
+    ERR.NotImplemented(fname, "non-cubic lattices.");
+  b = dev_alloc<fftComplex>((size_t)batch * GJP.Vol(), fname);
+  ERR.Check(fname, nrc_fft_plan(ctx(), GJP.Xsites(), GJP.Ysites(), GJP.Zsites(), batch,
+                                NRC_FLOAT_PREC, &plan));
+}
+
+Fourier::~Fourier() {
+  nrc_fft_destroy(plan);
+  nrc_free(ctx(), b);
+}
+
+void Fourier::Forward() { ERR.Check("void Fourier::Forward()", nrc_fft_exec(plan, b, -1)); }
+void Fourier::Backward() { ERR.Check("void Fourier::Backward()", nrc_fft_exec(plan, b, +1)); }
+
+fftComplex* Fourier::Get(int n) {
+  const char* fname = "fftComplex* Fourier::Get(int)";
+  if (n < 0) ERR.General(fname, "Argument must be non-negative.");
+  if (n >= batch_size) ERR.General(fname, "Argument must be less than batch_size.");
+  return b + (size_t)n * GJP.Vol();
+}
+
+// ------------------------------------------------------------------ Dispersion
+Dispersion::Dispersion(DispersionType type, Float mass, CutoffType cutoff) : dispersion(GJP.Vol()) {
+  Float bc[3];
+  GJP.GetBcShift(bc);
+  const int X = GJP.Xsites(), Y = GJP.Ysites(), Z = GJP.Zsites();
+  const int bx = GJP.Xbc(), by = GJP.Ybc(), bz = GJP.Zbc();
+  for (int i = 0; i < GJP.Vol(); ++i) {
+    int x = i / (Y * Z), y = (i / Z) % Y, z = i % Z;
+    if (type == DISPERSION_TYPE_STANDARD) {
+      // 2 sum_j sin^2(p_j / 2) / mass
+      const Float sx = std::sin((x + bc[0]) * PI / X);
+      const Float sy = std::sin((y + bc[1]) * PI / Y);
+      const Float sz = std::sin((z + bc[2]) * PI / Z);
+      Float e = 2.0 * (sx * sx + sy * sy + sz * sz);
+      e /= mass;
+      dispersion[i] = e;
+    } else {
+      if (x >= X / 2) x = X - x - bx;
+      if (y >= Y / 2) y = Y - y - by;
+      if (z >= Z / 2) z = Z - z - bz;
+      const Float sx = (x + bc[0]) / (Float)X, sy = (y + bc[1]) / (Float)Y, sz = (z + bc[2]) / (Float)Z;
+      const Float psq = (sx * sx + sy * sy + sz * sz) * 4.0 * PISQ;
+      if (cutoff == CUTOFF_TYPE_HARD && psq > GJP.CutoffSq())
+        dispersion[i] = std::numeric_limits<Float>::infinity();
+      else if (type == DISPERSION_TYPE_QUADRATIC)
+        dispersion[i] = psq / (2.0 * mass);
+      else
+        dispersion[i] = std::exp(psq / (2.0 * mass)) - 1.0;
+    }
+  }
+}
+
+Float Dispersion::Get(int n1, int n2, int n3) {
+  return dispersion[n3 + n2 * GJP.Zsites() + n1 * GJP.Ysites() * GJP.Zsites()];
+}
+
+// ------------------------------------------------------------------ Kinetic
+Kinetic::Kinetic(const KineticArg& a, Float dt) : xi(GJP.Vol()) {
+  Dispersion d(a.dispersion_type, a.mass / dt, CUTOFF_TYPE_HARD);
+  for (int i = 0; i < GJP.Vol(); ++i) xi[i] = 1.0 + d.Get(i);
+  dev_xi = dev_alloc<Float>(xi.size(), "Kinetic::Kinetic");
+  upload(dev_xi, xi, "Kinetic::Kinetic");
+}
+
+Kinetic::~Kinetic() { nrc_free(ctx(), dev_xi); }
+
+void Kinetic::Evolve(Fourier* b) {  // psi(k) /= xi(k) for every batch member
+  ERR.Check("void Kinetic::Evolve(Fourier*)",
+            nrc_scale_real(ctx(), b->Get(0), dev_xi, b->BatchSize(), GJP.Vol(), 1, NRC_FLOAT_PREC));
+}
+
+// ------------------------------------------------------------------ Interaction
+Interaction::Interaction(Lattice* lat, const InteractionArg& a, Float dt)
+    : lattice(lat), field(nullptr), interaction(GJP.Vol()) {
+  const char* fname = "Interaction::Interaction(Lattice*, const InteractionArg &, Float)";
+  const int vol = GJP.Vol();
+  const Float mass = a.mass;
+  Dispersion d(DISPERSION_TYPE_PERFECT, mass, CUTOFF_TYPE_NONE);
+  const Float XI = std::exp(GJP.CutoffSq() / (2.0 * mass));
+  for (int i = 0; i < vol; ++i) {
+    Float xi = 1.0 + d.Get(i);
+    if (xi > XI) xi = XI;
+    Float psq = 0.0;
+    if (a.interaction_type == INTERACTION_TYPE_ONEMINUSXIINVSQ) psq = 1.0 - 1.0 / (xi * xi);
+    else if (a.interaction_type == INTERACTION_TYPE_XISQMINUSONE) psq = xi * xi - 1.0;
+    else ERR.NotImplemented(fname, "Interaction type.");
+    psq *= mass;
+    Float v = 0.0;  // Taylor series in p^2
+    for (int j = 0; j < a.num_couplings; ++j) v += a.couplings[j] * std::pow(psq, j);
+    if (v < 0.0) ERR.General(fname, "Interaction is less the zero; cannot take square root.");
+    v *= 2.0 * TWOPI / mass;
+    v = std::sqrt(v);
+    v /= vol;  // the FFTs are unnormalised
+    v *= dt;
+    interaction[i] = v;
+  }
+  dev_interaction = dev_alloc<Float>(vol, fname);
+  upload(dev_interaction, interaction, fname);
+  field = new Fourier(1);
+}
+
+Interaction::~Interaction() {
+  delete field;
+  nrc_free(ctx(), dev_interaction);
+}
+
+// field = (1 + F-[f F+[phi]]) / V (hamiltonian_initialize.cu:17-35)
+void Interaction::Initialize() {
+  const char* fname = "void Interaction::Initialize()";
+  const int vol = GJP.Vol();
+  ERR.Check(fname, nrc_memcpy_d2d(ctx(), field->Get(0), lattice->Get(), 2 * vol * sizeof(Float)));
+  field->Backward();
+  ERR.Check(fname, nrc_scale_real(ctx(), field->Get(0), dev_interaction, 1, vol, 0, NRC_FLOAT_PREC));
+  field->Forward();
+  ERR.Check(fname, nrc_plus_one_div_vol(ctx(), field->Get(0), vol, NRC_FLOAT_PREC));
+}
+
+void Interaction::Evolve(Fourier* b) {
+  const char* fname = "void Interaction::Evolve(Fourier*)";
+  if (GJP.APBCQuery()) ERR.NotImplemented(fname, "APBCs no allowed.");
+  b->Forward();  // to position space
+  ERR.Check(fname, nrc_mul_complex(ctx(), b->Get(0), field->Get(0), b->BatchSize(), GJP.Vol(),
+                                   NRC_FLOAT_PREC));
+  b->Backward();  // to momentum space
+}
+
+void Interactions::Initialize() {
+  for (size_t n = 0; n < interactions.size(); ++n) interactions[n]->Initialize();
+}
+
+// ------------------------------------------------------------------ Potential
+Potential::Potential(const PotentialArg& a, Float dt) : potential(GJP.Vol()) {
+  const char* fname = "Potential::Potential(const PotentialArg &, Float)";
+  if (GJP.APBCQuery()) ERR.NotImplemented(fname, "Must use PBCs when a potential is present.");
+  const int X = GJP.Xsites(), Y = GJP.Ysites(), Z = GJP.Zsites(), vol = GJP.Vol();
+  for (int i = 0; i < vol; ++i) {
+    int z = i % Z, y = (i / Z) % Y, x = i / (Y * Z);
+    if (x >= X / 2) x -= X;
+    if (y >= Y / 2) y -= Y;
+    if (z >= Z / 2) z -= Z;
+    Float v = 0.0;
+    if (a.potential_form == POTENTIAL_FORM_HARMONIC) {
+      v = a.spring_constant1 * x * x;
+      v += a.spring_constant2 * y * y;
+      v += a.spring_constant3 * z * z;
+      v /= 2.0;
+    } else if (a.potential_form != POTENTIAL_FORM_NONE) {
+      ERR.NotImplemented(fname, "Potential form not supported.");
+    }
+    Float p = a.potential_type == POTENTIAL_TYPE_LIN ? 1.0 - dt * v : std::exp(-dt * v);
+    p /= vol;
+    potential[i] = p;
+  }
+  dev_potential = dev_alloc<Float>(vol, fname);
+  upload(dev_potential, potential, fname);
+}
+
+Potential::~Potential() { nrc_free(ctx(), dev_potential); }
+
+void Potential::Evolve(Fourier* b) {
+  const char* fname = "void Potential::Evolve(Fourier*)";
+  b->Forward();
+  ERR.Check(fname, nrc_scale_real(ctx(), b->Get(0), dev_potential, b->BatchSize(), GJP.Vol(), 0,
+                                  NRC_FLOAT_PREC));
+  b->Backward();
+}
+
+// ------------------------------------------------------------------ Propagator
+Propagator::Propagator(const PropagatorArg& a) : b(new Fourier(a.n_fermions)), vol(GJP.Vol()) {}
+Propagator::~Propagator() { delete b; }
+
+void Propagator::Run(std::vector<Hamiltonian*> hs) {
+  for (size_t i = 0; i < hs.size(); ++i) hs[i]->Evolve(b);
+}
+
+void Propagator::Set(Float* psi, int n) {
+  ERR.Check("void Propagator::Set(Float*, int)",
+            nrc_memcpy_d2d(ctx(), Get(n), psi, 2 * (size_t)vol * sizeof(Float)));
+}
+
+Float* Propagator::Get(int n) { return (Float*)b->Get(n); }
+
+void Propagator::Normalize() { ERR.NotImplemented("void Propagator::Normalize()", "Sorry."); }
+void Propagator::Normalize(Float) {
+  ERR.NotImplemented("void Propagator::Normalize(Float)", "Sorry.");
+}
+
+// ------------------------------------------------------------------ OneBody
+OneBody::OneBody(const OneBodyArg& a)
+    : source_type(a.source_type), psi(2 * GJP.Vol(), 0.0), vol(GJP.Vol()) {
+  dev_psi = dev_alloc<Float>(psi.size(), "OneBody::OneBody");
+  dev_m = dev_alloc<double>(2, "OneBody::OneBody");
+}
+
+OneBody::~OneBody() {
+  nrc_free(ctx(), dev_psi);
+  nrc_free(ctx(), dev_m);
+}
+
+void OneBody::Upload() { upload(dev_psi, psi, "OneBody::Set"); }
+
+void OneBody::Set() {
+  if (source_type == SOURCE_TYPE_SHO)
+    ERR.NotImplemented("void OneBody::Set()", "SOURCE_TYPE_SHO sources.");
+  Upload();
+}
+
+// one_body.C:172-212: coefficients of momentum basis vectors, one "k1 k2 k3 re im" per line
+void OneBody::Set(std::string name) {
+  const char* fname = "void OneBody::Set(string)";
+  std::fill(psi.begin(), psi.end(), 0.0);
+  std::ifstream file(name.c_str());
+  if (!file) ERR.FileR(fname, name.c_str());
+  int k1, k2, k3;
+  Float re, im;
+  while (file >> k1 >> k2 >> k3 >> re >> im) {
+    const int j = 2 * Vector3ToIndex(Vector3(k1, k2, k3));
+    psi[j] = re;
+    psi[j + 1] = im;
+  }
+  Set();
+}
+
+void OneBody::Set(int k1, int k2, int k3) {
+  std::fill(psi.begin(), psi.end(), 0.0);
+  psi[2 * Vector3ToIndex(Vector3(k1, k2, k3))] = 1.0;
+  Upload();
+}
+
+void OneBody::Set(int k1, int k2, int k3, int parity) {
+  std::fill(psi.begin(), psi.end(), 0.0);
+  psi[2 * Vector3ToIndex(Vector3(k1, k2, k3))] = 1.0;
+  psi[2 * Vector3ToIndex(-1 * Vector3(k1, k2, k3))] = parity ? 1.0 : -1.0;
+  Upload();
+}
+
+void OneBody::Set(const Vector3& v) { Set(v.GetX(), v.GetY(), v.GetZ()); }
+
+void OneBody::Set(Float* src) {
+  for (int i = 0; i < 2 * vol; ++i) psi[i] = src[i];
+  Upload();
+}
+
+Float* OneBody::Get() { return dev_psi; }
+
+std::complex<Float> OneBody::Project(Float* phi) {
+  const char* fname = "complex<Float> OneBody::Project(Float*)";
+  const int mode = NRC_FLOAT_PREC == NRC_PREC_F64 ? NRC_OVERLAP_F64 : NRC_OVERLAP_F32;
+  ERR.Check(fname, nrc_overlap(ctx(), dev_psi, phi, dev_m, 1, 1, vol, 1, mode));
+  double m[2];
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), m, dev_m, sizeof(m)));
+  return std::complex<Float>((Float)m[0], (Float)m[1]);
+}
+
+// ------------------------------------------------------------------ SlaterDet
+// M[t + n s] = <sink_s | orbital_t> (slater_det.C:25-39).  When the orbitals are one
+// propagator's contiguous batch, each sink row is one batched overlap launch and the matrix
+// crosses to the host once; LU and determinants run on the device (nrc_lu_det) on
+// contiguous storage.
+SlaterDet::SlaterDet(Float** props, OneBody** one_body, int n)
+    : slater_mat((size_t)n * n), mat_size(n), det_type(DET_TYPE_STANDARD) {
+  const char* fname = "SlaterDet::SlaterDet(Float**, OneBody**, int)";
+  const long vol = GJP.Vol();
+  bool contiguous = true;
+  for (int t = 1; t < n; ++t) contiguous = contiguous && props[t] == props[0] + 2 * vol * t;
+  const int mode = NRC_FLOAT_PREC == NRC_PREC_F64 ? NRC_OVERLAP_F64 : NRC_OVERLAP_F32;
+  double* dev_M = dev_alloc<double>(2 * (size_t)n * n, fname);
+  for (int s = 0; s < n; ++s) {
+    if (contiguous) {
+      ERR.Check(fname, nrc_overlap(ctx(), one_body[s]->Get(), props[0], dev_M + 2 * (size_t)s * n,
+                                   1, n, vol, 1, mode));
+    } else {
+      for (int t = 0; t < n; ++t)
+        ERR.Check(fname, nrc_overlap(ctx(), one_body[s]->Get(), props[t],
+                                     dev_M + 2 * ((size_t)s * n + t), 1, 1, vol, 1, mode));
+    }
+  }
+  std::vector<double> h(2 * (size_t)n * n);
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), h.data(), dev_M, h.size() * sizeof(double)));
+  nrc_free(ctx(), dev_M);
+  for (size_t i = 0; i < (size_t)n * n; ++i)
+    slater_mat[i] = std::complex<Float>((Float)h[2 * i], (Float)h[2 * i + 1]);
+}
+
+SlaterDet::~SlaterDet() {}
+
+// det (STANDARD) or log det (LOG) of the leading n x n minor (slater_det.C:59-115)
+std::complex<Float> SlaterDet::Run(int n) {
+  const char* fname = "complex<Float> SlaterDet::Run(int)";
+  if (n > mat_size) ERR.General(fname, "Submatrix exceeds size of matrix.");
+  if (n < 1) {
+    VRB.Warn(fname, "n < 1, returning unity.");
+    return 1.0;
+  }
+  std::vector<double> h(2 * (size_t)n * n);
+  for (int s = 0; s < n; ++s)
+    for (int t = 0; t < n; ++t) {
+      h[2 * ((size_t)s * n + t)] = slater_mat[t + mat_size * s].real();
+      h[2 * ((size_t)s * n + t) + 1] = slater_mat[t + mat_size * s].imag();
+    }
+  double* dev = dev_alloc<double>(h.size() + 4 + 1, fname);
+  int* dev_st = dev_alloc<int>(1, fname);
+  ERR.Check(fname, nrc_memcpy_h2d(ctx(), dev, h.data(), h.size() * sizeof(double)));
+  double* out = dev + h.size();  // logabs, phase (2), argsum
+  ERR.Check(fname, nrc_lu_det(ctx(), dev, n, n, 1, 0, out, out + 1, out + 3, dev_st));
+  double r[4];
+  int st = 0;
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), r, out, sizeof(r)));
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), &st, dev_st, sizeof(int)));
+  nrc_free(ctx(), dev);
+  nrc_free(ctx(), dev_st);
+  if (st) {
+    std::printf("ludcmp(complex<Float>*, int, int*, Float*): singular matrix\n");
+    std::exit(EXIT_FAILURE);
+  }
+  if (det_type == DET_TYPE_LOG) return std::complex<Float>((Float)r[0], (Float)r[3]);
+  const double m = std::exp(r[0]);
+  return std::complex<Float>((Float)(m * r[1]), (Float)(m * r[2]));
+}
+
+// ------------------------------------------------------------------ ludcmp
+// The reference's host routine, run by the device kernel that reproduces it bit for bit.
+void ludcmp(std::complex<Float>* a, int n, int* indx, Float* d) {
+  const char* fname = "ludcmp(complex<Float>*, int, int*, Float*)";
+  std::vector<double> h(2 * (size_t)n * n);
+  for (size_t i = 0; i < (size_t)n * n; ++i) {
+    h[2 * i] = a[i].real();
+    h[2 * i + 1] = a[i].imag();
+  }
+  double* dev = dev_alloc<double>(h.size() + 1, fname);
+  int* di = dev_alloc<int>((size_t)n + 1, fname);
+  ERR.Check(fname, nrc_memcpy_h2d(ctx(), dev, h.data(), h.size() * sizeof(double)));
+  ERR.Check(fname, nrc_ludcmp(ctx(), dev, n, 1, di, dev + h.size(), di + n));
+  std::vector<int> hi((size_t)n + 1);
+  double dd = 1.0;
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), h.data(), dev, h.size() * sizeof(double)));
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), hi.data(), di, hi.size() * sizeof(int)));
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), &dd, dev + h.size(), sizeof(double)));
+  nrc_free(ctx(), dev);
+  nrc_free(ctx(), di);
+  if (hi[n]) {
+    std::printf("ludcmp(complex<Float>*, int, int*, Float*): singular matrix\n");
+    std::exit(EXIT_FAILURE);
+  }
+  for (size_t i = 0; i < (size_t)n * n; ++i)
+    a[i] = std::complex<Float>((Float)h[2 * i], (Float)h[2 * i + 1]);
+  for (int i = 0; i < n; ++i) indx[i] = hi[i];
+  *d = (Float)dd;
+}
+
+// ------------------------------------------------------------------ Momenta
+Momenta::Momenta(const MomentaArg& a) : momenta_count(0) {
+  const char* fname = "Momenta::Momenta(const MomentaArg &)";
+  const int X = GJP.Xsites(), Y = GJP.Ysites(), Z = GJP.Zsites();
+  Dispersion d(a.dispersion_type, a.mass, CUTOFF_TYPE_HARD);
+  for (int n1 = 0; n1 < X; ++n1)
+    for (int n2 = 0; n2 < Y; ++n2)
+      for (int n3 = 0; n3 < Z; ++n3) {
+        const Float e = std::log(1 + d.Get(n1, n2, n3));
+        if (e <= a.fermi_energy) {
+          omega.push_back(e);
+          momentum.push_back(std::vector<int>{n1, n2, n3});
+        }
+      }
+  momenta_count = (int)omega.size();
+  if (!momenta_count) ERR.General(fname, "No energy levels exceed the specified fermi level.");
+  // insertion sort by energy, with the reference's lower bound j > 0 (momenta.C:62-73)
+  for (int i = 1; i < momenta_count; ++i) {
+    const Float e = omega[i];
+    const std::vector<int> tmp = momentum[i];
+    int j = i - 1;
+    while (j > 0 && omega[j] > e) {
+      omega[j + 1] = omega[j];
+      momentum[j + 1] = momentum[j];
+      --j;
+    }
+    omega[j + 1] = e;
+    momentum[j + 1] = tmp;
+  }
+  shell_count.push_back(1);
+  for (int i = 1; i < momenta_count; ++i) {
+    if (omega[i] == omega[i - 1]) shell_count.back()++;
+    else shell_count.push_back(1);
+  }
+  parity_list.assign(momenta_count, 0);
+  const int dims[3] = {X, Y, Z};
+  const int bcs[3] = {GJP.Xbc(), GJP.Ybc(), GJP.Zbc()};
+  for (int i = 0; i < momenta_count; ++i) {
+    std::vector<int> p(3);
+    for (int ax = 0; ax < 3; ++ax) p[ax] = (dims[ax] - momentum[i][ax] - bcs[ax]) % dims[ax];
+    bool found = false;
+    for (int j = 0; j < momenta_count; ++j)
+      if (p == momentum[j]) {
+        parity_list[i] = j;
+        found = true;
+      }
+    if (!found) ERR.General(fname, "Opposite parity vector is missing; try adjusting the fermi energy.");
+  }
+}
+
+std::vector<int> Momenta::GetMomentum(int n) {
+  if (n >= momenta_count || n < 0)
+    ERR.General("vector<int> Momenta::GetMomentum(int)", "Specified momentum exceeds momentum count.");
+  return momentum[n];
+}
+
+int Momenta::ShellCount(int n) {
+  if (n < 0 || n >= NumShells()) ERR.General("Momenta::ShellCount(int)", "Invalid argument.");
+  return shell_count[n];
+}
+
+int Momenta::AccShellCount(int n) {
+  if (n < 0) return 0;
+  if (n >= NumShells()) ERR.General("Momenta::AccShellCount(int)", "Invalid argument.");
+  int acc = 0;
+  for (int s = 0; s <= n; ++s) acc += shell_count[s];
+  return acc;
+}

@@ -1,0 +1,69 @@
+"""The class layer end to end on the GPU: the slater_det driver (the production loop written
+against the reference's class API) on a small production-style argument set, checked
+against the oracle fed with the same ranlxs stream, kinetic table, smeared interaction and
+momenta.  FP64 build: agreement to 1e-10 relative per (n, t)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det")
+PI = 3.1415926535897932
+
+ARGS = {
+    "verbose.arg": "0\n0\n0\n0\n0\n",
+    "do.arg": "6\n8\n8\n8\nBOUNDARY_TYPE_PRD\nBOUNDARY_TYPE_PRD\nBOUNDARY_TYPE_PRD\n1.0\n",
+    "evo.arg": "0\n1\n2\n",
+    "lattice.arg": "FIELD_TYPE_ZTWO\n",
+    "random.arg": "RANDOM_TYPE_RANLXS2\nSEED_TYPE_INPUT\n7012\nrngs/ranlxs2\n",
+    "interaction.arg": "5.0\nINTERACTION_TYPE_ONEMINUSXIINVSQ\n9.82275\n2.06919\n0.45723\n"
+                       "0.00223528\n0.0451682\n0.00709158\n0.000244729\n",
+    "kinetic.arg": "DISPERSION_TYPE_PERFECT\n1.18343\n",
+    "propagator.arg": "1\nresults/prop\n",
+    "one_body.arg": "SOURCE_TYPE_MOM\n1.0\n1.0\n1.0\n",
+    "momenta.arg": "DISPERSION_TYPE_PERFECT\n0.3\n1.18343\n",
+}
+
+
+def test_slater_det_driver_matches_oracle(orc):
+    assert os.path.exists(DRIVER), "the slater_det driver is not built"
+    L, T, ncfg = 8, 6, 2
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in ARGS.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        print("[gpu] running the slater_det driver (L=8, T=6, 2 configurations)", flush=True)
+        r = subprocess.run([DRIVER], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        cut2 = (1.0 * PI) * (1.0 * PI)
+        n, moms, par, om = orc.momenta(L, orc.DISP_PERFECT, 1.18343, 0.3, cutoff_sq=cut2)
+        assert n == 7
+        got = []
+        for k in range(1, n + 1):
+            rows = [list(map(float, line.split()))
+                    for line in open(os.path.join(tmp, "results", f"slater.{k}")) if line.strip()]
+            got.append(np.array(rows).reshape(ncfg, T, 2))
+        got = np.stack(got)  # [n][block][t][re, im]
+        state = open(os.path.join(tmp, "rngs", "ranlxs2.0"), "rb").read()
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    assert rc == 0
+    rng = orc.ranlxs(2, 7012)
+    for j in range(ncfg):
+        ref = orc.run_config(L, n, T, moms, kin, smear=f, minors=True, field="ranlxs", rng=rng)
+        d2 = np.exp(2 * ref["logabs"])  # |det|^2 [t][n]
+        for k in range(n):
+            np.testing.assert_allclose(got[k, j, :, 0], d2[:, k], rtol=1e-10)
+            assert np.all(got[k, j, :, 1] == 0.0)
+    # the RNG state written before the last block: RandomType then long[105]
+    assert len(state) == 4 + 8 * 105
+    st = np.frombuffer(state[4:], dtype=np.int64)
+    assert st[0] == 105 and st[101] == 397

@@ -205,17 +205,22 @@ def _compare(gpu, ref, tol_la, tol_ph):
     return dla, dph
 
 
+PATHS = [1, 2]  # two-pass, single-pass
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("minors", [False, True])
-def test_ensemble_f64_vs_oracle_config2_shape(ctx, orc, split, minors):
+def test_ensemble_f64_vs_oracle_config2_shape(ctx, orc, split, minors, path):
     """L=16, N=7 plane waves, tau=0.05, A=0.4, Z2 fields from Philox, FP64 throughout."""
     import nrcps_amd as na
 
     L, N, T, C, seed = 16, 7, 64, 4, 7012
-    print(f"[gpu] ensemble f64 split={split} minors={minors}", flush=True)
+    print(f"[gpu] ensemble f64 split={split} minors={minors} path={path}", flush=True)
     moms = orc.lowest_momenta(L, N)
     kin = _tables(orc, L, split)
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F64, split=split, minors=minors)
+    ens.set_path(path)
     la, ph, _ = ens.run(seed, 100, T)
     for c in range(C):
         ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, minors=minors, seed=seed,
@@ -275,7 +280,8 @@ def test_ensemble_reference_fields_ranlxs(ctx, orc):
         _compare((la[:, c], ph[:, c]), ref, 1e-12, 1e-12)
 
 
-def test_ensemble_smeared_interaction(ctx, orc):
+@pytest.mark.parametrize("path", PATHS)
+def test_ensemble_smeared_interaction(ctx, orc, path):
     """Interaction with the production multi-coupling form factor (hamiltonian.C:77-147)."""
     import nrcps_amd as na
 
@@ -287,35 +293,40 @@ def test_ensemble_smeared_interaction(ctx, orc):
     assert rc == 0
     f = f * 0.05  # a weaker coupling keeps (1 + smeared field) positive on this lattice
     ens = na.Ensemble(ctx, L, moms, kin, 0.0, C, prec=na.PREC_F64, smear=f)
+    ens.set_path(path)
     la, ph, _ = ens.run(5, 0, T)
     for c in range(C):
         ref = orc.run_config(L, N, T, moms, kin, smear=f, seed=5, config=c)
         _compare((la[:, c], ph[:, c]), ref, 1e-11, 1e-11)
 
 
-def test_ensemble_orbitals_vs_oracle(ctx, orc):
+@pytest.mark.parametrize("path", PATHS)
+def test_ensemble_orbitals_vs_oracle(ctx, orc, path):
     import nrcps_amd as na
 
     L, N, T = 12, 7, 6
     moms = orc.lowest_momenta(L, N)
     kin = _tables(orc, L, 0)
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, 2, prec=na.PREC_F64)
+    ens.set_path(path)
     ens.run(3, 40, T)
     ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=3, config=41, want_psi=True)["psi"]
     got = ens.orbitals(1)
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("split", [0, 1])
-def test_ensemble_f32_config3_shape(ctx, orc, split):
+def test_ensemble_f32_config3_shape(ctx, orc, split, path):
     """L=32, N=33, FP32 evolution: per-config det^2 relative error against FP64."""
     import nrcps_amd as na
 
     L, N, T, C = 32, 33, 16, 2
-    print(f"[gpu] ensemble f32 L=32 N=33 split={split}", flush=True)
+    print(f"[gpu] ensemble f32 L=32 N=33 split={split} path={path}", flush=True)
     moms = orc.lowest_momenta(L, N)
     kin = _tables(orc, L, split)
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32, split=split)
+    ens.set_path(path)
     la, ph, _ = ens.run(21, 0, T)
     for c in range(C):
         ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, seed=21, config=c)
@@ -323,10 +334,11 @@ def test_ensemble_f32_config3_shape(ctx, orc, split):
         r2 = np.exp(2 * ref["logabs"][:, 0]) * ref["phase"][:, 0] ** 2
         rel = np.abs(d2 - r2) / np.abs(r2)
         print(f"[gpu] FP32 det^2 rel err by t (config {c}): max {rel.max():.2e}", flush=True)
-        assert rel.max() < 1e-3
+        assert rel.max() < 1e-4
 
 
-def test_ensemble_full_size_free_gas(ctx, orc):
+@pytest.mark.parametrize("path", PATHS)
+def test_ensemble_full_size_free_gas(ctx, orc, path):
     """A = 0 at L=32, N=33, T=128: det M(t) = prod_n (1/xi_half(k_n))^(2t) exactly."""
     import nrcps_amd as na
 
@@ -334,6 +346,7 @@ def test_ensemble_full_size_free_gas(ctx, orc):
     moms = orc.lowest_momenta(L, N)
     kin = _tables(orc, L, 0)
     ens = na.Ensemble(ctx, L, moms, kin, 0.0, C, prec=na.PREC_F32)
+    ens.set_path(path)
     la, ph, _ = ens.run(1, 0, T)
     idx = (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]
     exact = -2 * np.arange(T)[:, None] * np.log(kin[idx]).sum()
