@@ -71,9 +71,11 @@ def main():
     ap.add_argument("--T", type=int, default=128)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--N", type=int, default=33)
-    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", type=int, default=0,
+                    help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -89,7 +91,8 @@ def main():
     ctx = na.Context(0)
     moms = _lowest(L, N)
     kin = _tables(L, tau, 1.0)
-    ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_REFERENCE)
+    ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+    ens.set_path(args.path)
     seed = 7012
 
     for w in range(args.warmup):
@@ -113,16 +116,24 @@ def main():
     tx, nx = ens.kernel_time(0)
     tyz, nyz = ens.kernel_time(1)
     tlu, nlu = ens.kernel_time(2)
+    tf, nf = ens.kernel_time(3)
+    tg, ng = ens.kernel_time(4)
     ens.profile(False)
-    # both passes together evolve one slice of every configuration; algorithmic bytes per
-    # configuration-slice: read + write the N FP32 orbitals once plus phi as one byte/site
+    # the evolution kernels of one slice of every configuration (one single-pass launch, or an
+    # X pass and a YZ pass); algorithmic bytes per configuration-slice: read + write the N FP32
+    # orbitals once plus phi as one byte per site
     b_evo = N * 16 * V + V
-    slice_ms = tx / nx + tyz / nyz
+    slice_ms = (tx + tyz) / T
     achieved = b_evo * C / (slice_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "xpass+yzpass (one slice)", "achieved": achieved,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "avg_ms": {"xpass": tx / nx, "yzpass": tyz / nyz,
-                                            "lu": tlu / max(nlu, 1)},
+    single = nyz == 0
+    roofline = {"bound": "hbm",
+                "kernel": "k_slice1 (one launch per slice)" if single
+                else "k_xpass + k_yzpass (one slice)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_ms": {"evolution_per_slice": slice_ms, "xpass_or_slice": tx / max(nx, 1),
+                           "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
+                           "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
                 "bytes_per_cs": b_evo}
 
     cpu = None
@@ -135,7 +146,7 @@ def main():
         "value": cs_per_s, "unit": "configs*slices/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"correlator path, config 3 shape: L={L}, N={N}, T={T}, "
+        "config": {"workload": f"Slater-determinant correlator over aux-field configurations: L={L}, N={N}, T={T}, "
                                f"tau={tau}, A={A}, {C} configs per step, FP32 evolution, "
                                f"FP64 LU", "L": L, "N": N, "T": T, "configs": C,
                    "parallelism": "single GPU, batched configurations"},

@@ -32,8 +32,8 @@ extern "C" {
 #define NRC_PREC_F32 0
 #define NRC_PREC_F64 1
 
-#define NRC_SPLIT_REFERENCE 0 /* k-space storage; per slice K(1/2) I K(1/2), field (1+A phi)/V */
-#define NRC_SPLIT_BASELINE 1  /* x-space storage; psi <- F+[kappa F-[psi]]/V, then (1 + A phi) */
+#define NRC_SPLIT_KSPACE 0 /* k-space storage; per slice K(1/2) I K(1/2), field (1+A phi)/V */
+#define NRC_SPLIT_XSPACE 1  /* x-space storage; psi <- F+[kappa F-[psi]]/V, then (1 + A phi) */
 
 #define NRC_OVERLAP_F64 0    /* dense overlaps in FP64 */
 #define NRC_OVERLAP_F32 1    /* dense overlaps of FP32 data, FP64 accumulation */
@@ -117,13 +117,18 @@ typedef struct nrc_ensemble_desc {
   int n_orb;         /* N orbitals per configuration; sources = sinks (<= 64) */
   int n_configs;     /* configurations evolved together */
   int prec;          /* NRC_PREC_F32 / NRC_PREC_F64 for orbitals and FFTs */
-  int split;         /* NRC_SPLIT_REFERENCE / NRC_SPLIT_BASELINE */
+  int split;         /* NRC_SPLIT_KSPACE / NRC_SPLIT_XSPACE */
   int minors;        /* 0: det of the full N x N matrix; 1: every leading minor */
   const int* moms;   /* n_orb x 3 plane-wave momenta (n1, n2, n3) in [0, L) */
-  const double* kin; /* V: reference split xi_half (psi /= xi); baseline exp(-tau eps) */
+  const double* kin; /* V: k-space split xi_half (psi /= xi); x-space split exp(-tau eps) */
   double A;          /* field multiplier (1 + A phi) */
-  const double* smear; /* optional V form factor f(k) (reference split): the field becomes
+  const double* smear; /* optional V form factor f(k) (k-space split): the field becomes
                           Interaction::Initialize's (1 + F-[f F+ phi])/V */
+  const double* sinks; /* optional n_orb x V complex sink vectors in storage space (k-space
+                          for the k-space split, x-space for the x-space split).  When set,
+                          overlaps are dense products (split-BF16x3 on the matrix cores for
+                          FP32, FP64 otherwise) instead of the plane-wave gather; single-pass
+                          path only. */
 } nrc_ensemble_desc;
 
 int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* desc, nrc_ensemble** ens);
@@ -135,7 +140,7 @@ int nrc_ensemble_run(nrc_ensemble* ens, uint64_t seed, uint32_t config0, int T, 
 int nrc_ensemble_run_fields(nrc_ensemble* ens, const uint32_t* words, int T, double* logabs,
                             double* phase, double* argsum);
 /* Orbitals of configuration c after the last run, N x V complex in the ensemble precision,
- * in storage space (k-space for the reference split). */
+ * in storage space (k-space for the k-space split). */
 int nrc_ensemble_orbitals(nrc_ensemble* ens, int c, void* out);
 /* Kernel path for the evolution: NRC_PATH_SINGLE_PASS evolves each slice in one kernel that
  * reads and writes every orbital once (FP32 L <= 32, FP64 L <= 16); NRC_PATH_TWO_PASS uses an

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG, "libnrcps_hip.so")
 
 OK = 0
 PREC_F32, PREC_F64 = 0, 1
-SPLIT_REFERENCE, SPLIT_BASELINE = 0, 1
+SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3 = 0, 1, 2
 PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
@@ -42,7 +42,8 @@ class NrcError(RuntimeError):
 class EnsembleDesc(C.Structure):
     _fields_ = [("L", C.c_int), ("n_orb", C.c_int), ("n_configs", C.c_int), ("prec", C.c_int),
                 ("split", C.c_int), ("minors", C.c_int), ("moms", C.c_void_p),
-                ("kin", C.c_void_p), ("A", C.c_double), ("smear", C.c_void_p)]
+                ("kin", C.c_void_p), ("A", C.c_double), ("smear", C.c_void_p),
+                ("sinks", C.c_void_p)]
 
 
 _lib = None
@@ -217,7 +218,7 @@ class Ensemble:
     """The batched production slice loop (nrc_ensemble_*)."""
 
     def __init__(self, ctx: Context, L, moms, kin, A, n_configs, prec=PREC_F32,
-                 split=SPLIT_REFERENCE, minors=False, smear=None):
+                 split=SPLIT_KSPACE, minors=False, smear=None, sinks=None):
         self.ctx, self.lib = ctx, ctx.lib
         self.L, self.N, self.C, self.prec = L, len(moms), n_configs, prec
         self.nm = self.N if minors else 1
@@ -231,6 +232,10 @@ class Ensemble:
         if smear is not None:
             self._smear = np.ascontiguousarray(smear, dtype=np.float64)
             d.smear = self._smear.ctypes.data
+        self._sinks = None
+        if sinks is not None:
+            self._sinks = np.ascontiguousarray(sinks, dtype=np.complex128).reshape(-1)
+            d.sinks = self._sinks.ctypes.data
         h = C.c_void_p()
         _check(self.lib.nrc_ensemble_create(ctx.h, C.byref(d), C.byref(h)), "ensemble_create")
         self.h = h

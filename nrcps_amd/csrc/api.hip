@@ -13,7 +13,21 @@ struct nrc_context {
   hipStream_t stream = nullptr;
   char name[256] = {0};
   int n_cu = 0;
+  void* work = nullptr;  // grow-only workspace of the library's own calls
+  size_t work_bytes = 0;
 };
+
+// the context workspace, grown to at least `bytes` (synchronises before a reallocation)
+static void* ctx_workspace(nrc_context* c, size_t bytes) {
+  if (bytes <= c->work_bytes) return c->work;
+  hipStreamSynchronize(c->stream);
+  if (c->work) hipFree(c->work);
+  c->work = nullptr;
+  c->work_bytes = 0;
+  if (hipMalloc(&c->work, bytes) != hipSuccess) return nullptr;
+  c->work_bytes = bytes;
+  return c->work;
+}
 
 struct nrc_fft {
   nrc_context* ctx;
@@ -73,6 +87,7 @@ int nrc_finalize(nrc_context* ctx) {
   if (!ctx) return NRC_ERR_ARG;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  if (ctx->work) hipFree(ctx->work);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return NRC_OK;
@@ -192,7 +207,17 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
                                       ctx->stream));
     return NRC_OK;
   }
-  return NRC_ERR_UNSUPPORTED;
+  if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
+  if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
+  if (ncfg == 0) return NRC_OK;
+  const size_t split = ((size_t)4 * ns * vol * 2 + 255) / 256 * 256;  // hi + lo bf16
+  const size_t part = nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg) * sizeof(float);
+  char* w = (char*)ctx_workspace(ctx, split + part);
+  if (!w) return NRC_ERR_NOMEM;
+  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, ctx->stream));
+  NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
+                                     (float*)(w + split), ctx->stream));
+  return NRC_OK;
 }
 
 int nrc_lu_det(nrc_context* ctx, const double* M, int n, int ldm, int ncfg, int minors,
@@ -241,10 +266,11 @@ struct nrc_ensemble {
   std::vector<int> moms;
   void* psi = nullptr;       // [C][N][V] complex
   void* tab_x = nullptr;     // reference: 1/xi_half (X pass)
-  void* tab_yz = nullptr;    // baseline: kappa/V (YZ pass)
+  void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass)
   void* tab_x_lo = nullptr;  // FP32 remainders of the tables above
   void* tab_yz_lo = nullptr;
   void* smear = nullptr;     // reference + smear: f(k)
+  void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
@@ -269,7 +295,8 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->tab_x_lo, e->tab_yz_lo, e->smear, e->work,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->tab_x_lo, e->tab_yz_lo, e->smear, e->sinks,
+                  e->work,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev};
@@ -311,9 +338,9 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
                                    nrc_ensemble** out) {
   if (!ctx || !d || !out || !d->moms || !d->kin) return NRC_ERR_ARG;
   if (d->n_orb < 1 || d->n_configs < 1 || !prec_ok(d->prec)) return NRC_ERR_ARG;
-  if (d->split != NRC_SPLIT_REFERENCE && d->split != NRC_SPLIT_BASELINE) return NRC_ERR_ARG;
+  if (d->split != NRC_SPLIT_KSPACE && d->split != NRC_SPLIT_XSPACE) return NRC_ERR_ARG;
   if (!nrc::pass_supported(d->L) || d->n_orb > 64) return NRC_ERR_UNSUPPORTED;
-  if (d->smear && d->split != NRC_SPLIT_REFERENCE) return NRC_ERR_ARG;
+  if (d->smear && d->split != NRC_SPLIT_KSPACE) return NRC_ERR_ARG;
   const int L = d->L, N = d->n_orb;
   for (int i = 0; i < 3 * N; ++i)
     if (d->moms[i] < 0 || d->moms[i] >= L) return NRC_ERR_ARG;
@@ -328,6 +355,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   e->d.moms = e->moms.data();
   e->d.kin = nullptr;
   e->d.smear = nullptr;
+  e->d.sinks = nullptr;
   const long V = e->V;
   const long C = d->n_configs;
   int rc = NRC_OK;
@@ -342,7 +370,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   ENS_TRY(hipMalloc(&e->psi, (size_t)C * N * V * elem_bytes(d->prec)) == hipSuccess
               ? NRC_OK : NRC_ERR_NOMEM);
   std::vector<double> h(V);
-  if (d->split == NRC_SPLIT_REFERENCE) {
+  if (d->split == NRC_SPLIT_KSPACE) {
     for (long i = 0; i < V; ++i) h[i] = 1.0 / d->kin[i];
     ENS_TRY(ens_upload_pair(e, &e->tab_x, &e->tab_x_lo, h));
     if (d->smear) {
@@ -358,16 +386,29 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     ENS_TRY(ens_upload_pair(e, &e->tab_yz, &e->tab_yz_lo, h));
   }
   ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
-  // sink gather lists: reference split in the X pass (pencil (k1,k2), position k0);
-  // baseline split in the YZ pass column stage (pencil (k0,k2), position k1)
+  if (d->sinks) {  // dense sinks in the ensemble precision
+    if (d->prec == NRC_PREC_F64) {
+      ENS_TRY(hipMalloc(&e->sinks, (size_t)N * V * 16) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+      ENS_TRY(hipMemcpy(e->sinks, d->sinks, (size_t)N * V * 16, hipMemcpyHostToDevice) ==
+                      hipSuccess ? NRC_OK : NRC_ERR_HIP);
+    } else {
+      std::vector<float> f((size_t)2 * N * V);
+      for (size_t i = 0; i < f.size(); ++i) f[i] = (float)d->sinks[i];
+      ENS_TRY(hipMalloc(&e->sinks, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+      ENS_TRY(hipMemcpy(e->sinks, f.data(), f.size() * 4, hipMemcpyHostToDevice) == hipSuccess
+                  ? NRC_OK : NRC_ERR_HIP);
+    }
+  }
+  // sink gather lists: k-space split in the X pass (pencil (k1,k2), position k0);
+  // x-space split in the YZ pass column stage (pencil (k0,k2), position k1)
   const int npen = L * L;
   std::vector<int> cnt(npen + 1, 0), pos(N), snk(N), site(N);
   std::vector<std::pair<int, int>> ent(N);
   for (int s = 0; s < N; ++s) {
     const int k0 = d->moms[3 * s], k1 = d->moms[3 * s + 1], k2 = d->moms[3 * s + 2];
     site[s] = (k0 * L + k1) * L + k2;
-    const int pen = d->split == NRC_SPLIT_REFERENCE ? k1 * L + k2 : k0 * L + k2;
-    const int p = d->split == NRC_SPLIT_REFERENCE ? k0 : k1;
+    const int pen = d->split == NRC_SPLIT_KSPACE ? k1 * L + k2 : k0 * L + k2;
+    const int p = d->split == NRC_SPLIT_KSPACE ? k0 : k1;
     ent[s] = {pen, s};
     pos[s] = p;
     cnt[pen + 1]++;
@@ -555,7 +596,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   hipStream_t s = e->ctx->stream;
   const int N = e->d.n_orb, L = e->d.L, prec = e->d.prec;
   const long C = e->d.n_configs, V = e->V;
-  const bool ref = e->d.split == NRC_SPLIT_REFERENCE;
+  const bool ref = e->d.split == NRC_SPLIT_KSPACE;
   const size_t slice_words = (size_t)C * e->nw;
   const size_t need_words = words_host ? slice_words * T : slice_words;
   if (need_words > e->words_cap) {
@@ -602,11 +643,17 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   const bool single = e->path == NRC_PATH_SINGLE_PASS ||
                       (e->path == NRC_PATH_AUTO && nrc::slice1_supported(prec, L));
   // single pass: every slice is one kernel over the orbitals (slice_fused.hip)
+  if (e->sinks && !single) return NRC_ERR_UNSUPPORTED;  // dense sinks read the stored state
+  const bool measure_first = ref || e->sinks;
   for (int t = 0; single && t < T; ++t) {
-    if (ref) {
+    if (measure_first) {
       {
         EvScope sc(e, 4);
-        NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
+        if (e->sinks)
+          NRC_TRY(nrc_overlap(e->ctx, e->sinks, e->psi, (double*)e->M, N, N, V, (int)C,
+                              prec == NRC_PREC_F64 ? NRC_OVERLAP_F64 : NRC_OVERLAP_BF16X3));
+        else
+          NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
       }
       NRC_TRY(lu(t));
     }
@@ -634,8 +681,10 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
       }
     } else {
       a.mid = mul_table(e->tab_yz, e->tab_yz_lo);
-      a.gather_mid = 1;
-      a.g = g;
+      if (!e->sinks) {
+        a.gather_mid = 1;
+        a.g = g;
+      }
       a.post = fieldspec;
       a.post.bits = field_of(t);
       a.post.a0 = 1.0;
@@ -645,14 +694,14 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
       EvScope sc(e, 0);
       NRC_HIP(nrc::launch_slice1(prec, a, s));
     }
-    if (!ref) NRC_TRY(lu(t));
+    if (!measure_first) NRC_TRY(lu(t));
   }
 
   // two passes per slice: an X pass that finishes slice t-1, measures t and starts slice t,
   // then a YZ pass
   for (int t = 0; !single && t <= T; ++t) {
     const bool last = t == T;
-    // X pass: finish slice t-1, measure t (reference split), start slice t
+    // X pass: finish slice t-1, measure t (k-space split), start slice t
     {
       nrc::PassArgs a;
       a.psi = e->psi;
@@ -685,7 +734,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
       NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
     }
     if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
-    // YZ pass: the field (reference) or kinetic (baseline) multiply between the FFTs
+    // YZ pass: the field (k-space split) or kinetic (x-space split) multiply between the FFTs
     {
       nrc::PassArgs a;
       a.psi = e->psi;

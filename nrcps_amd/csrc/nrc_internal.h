@@ -103,6 +103,8 @@ hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, doub
                                 int ns, int nt, long vol, int ncfg, hipStream_t s);
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
                                  int nt, long vol, int ncfg, float* partial, hipStream_t s);
+// phi_split: bf16 hi[ns][2V] followed by lo[ns][2V]
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
+size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 
 }  // namespace nrc

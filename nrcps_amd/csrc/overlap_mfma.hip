@@ -1,3 +1,234 @@
-// overlap_mfma.hip -- split-BF16x3 overlap products on MFMA (placeholder unit).
+// overlap_mfma.hip -- dense overlaps M[c][s][t] = sum_i conj(phi_s(i)) psi_{c,t}(i) on the
+// matrix cores, as split-BF16x3 products with FP32 accumulation (general sinks: the
+// reference's OneBody::Project / Cuda::InnerProduct, one_body_project.cu:17-34,
+// cuda_utils.cu:88-127, for any sink vectors, not only plane waves).
+//
+// Complex products as one real GEMM over K = 2V interleaved (re, im) values:
+//   Re M[s][t] = phi'_s . psi'_t,  Im M[s][t] = phi'_s . psi''_t,
+// with phi' = (re, im) interleaved, psi' = (re, im) and psi'' = (im, -re).  The B matrix has
+// two columns per orbital (2t: psi', 2t+1: psi''), built in registers from one load.
+//
+// Split-BF16x3: a = a_hi + a_lo with a_hi = bf16(a), a_lo = bf16(a - a_hi); a.b is taken as
+// a_hi.b_hi + a_hi.b_lo + a_lo.b_hi (the dropped a_lo.b_lo is ~2^-18 relative).  Sinks are
+// split once (they are shared by every configuration); orbitals are split as they are read.
+//
+// v_mfma_f32_16x16x32_bf16: lane l supplies A row (l % 16) and B column (l % 16), eight K
+// values each for the lane group l / 16; D row (l / 16) * 4 + r, column l % 16.  A and B use
+// the same lane/slot -> k assignment, so the sum over k is the full dot product.
+//
+// Each workgroup (4 waves) takes one configuration and one chunk of K; every wave
+// accumulates all (sink tile, column tile) pairs for its K-steps, the waves are summed in
+// LDS and the chunk's partial tile written out; a second kernel sums the chunks in FP64.
 #include "nrc_device.h"
 #include "nrc_internal.h"
+
+namespace nrc {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int OV_MAXT = 4;    // sink tiles of 16 (ns <= 64)
+constexpr int OV_MAXN = 8;    // column tiles of 16 (2 nt <= 128)
+constexpr int OV_KSTEP = 32;  // reals per MFMA
+constexpr int OV_WAVES = 4;
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// split 8 floats into bf16 hi and lo parts
+__device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const __bf16 h = to_bf16(v[q]);
+    hi[q] = h;
+    lo[q] = to_bf16(v[q] - (float)h);
+  }
+}
+
+// sinks phi[s][V] complex FP32 -> hi[s][2V], lo[s][2V] bf16 (interleaved re, im)
+__global__ void k_split_bf16(const float2* phi, __bf16* hi, __bf16* lo, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 v = phi[i];
+    const __bf16 hr = to_bf16(v.x), hi_ = to_bf16(v.y);
+    hi[2 * i] = hr;
+    hi[2 * i + 1] = hi_;
+    lo[2 * i] = to_bf16(v.x - (float)hr);
+    lo[2 * i + 1] = to_bf16(v.y - (float)hi_);
+  }
+}
+
+hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s) {
+  const long n = (long)ns * vol;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  __bf16* hi = (__bf16*)out;
+  __bf16* lo = hi + 2 * n;
+  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)blocks), dim3(256), 0, s, phi, hi, lo, n);
+  return hipGetLastError();
+}
+
+// grid (kchunks, ncfg); partial[c][chunk][ns_pad][nc_pad] FP32
+template <int MT, int NCT>
+__global__ __launch_bounds__(256) void k_overlap_bf16x3(const __bf16* __restrict__ phi_hi,
+                                                        const __bf16* __restrict__ phi_lo,
+                                                        const float2* __restrict__ psi,
+                                                        float* __restrict__ partial, int ns,
+                                                        int nt, long vol, int mt, int nc_t,
+                                                        long kchunk) {
+  __shared__ float red[OV_WAVES - 1][4][64];
+  const int chunk = blockIdx.x, nchunks = gridDim.x;
+  const long c = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, grp = lane >> 4;
+  const long K = 2 * vol;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  f32x4 acc[MT][NCT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const float2* psic = psi + c * nt * vol;
+  for (long kb = k0 + (long)wave * OV_KSTEP; kb < k1; kb += OV_WAVES * OV_KSTEP) {
+    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements ka/2 .. ka/2 + 3
+    bf16x8 ah[MT], al[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+      if (a < mt) {
+        const int s = a * 16 + row;
+        if (s < ns && ka < K) {
+          ah[a] = *reinterpret_cast<const bf16x8*>(phi_hi + (long)s * K + ka);
+          al[a] = *reinterpret_cast<const bf16x8*>(phi_lo + (long)s * K + ka);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) { ah[a][q] = to_bf16(0.f); al[a][q] = to_bf16(0.f); }
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) {
+      if (b < nc_t) {
+        const int col = b * 16 + row, t = col >> 1, part = col & 1;
+        float v[8];
+        if (t < nt && ka < K) {
+          const float4* p = reinterpret_cast<const float4*>(psic + (long)t * vol + ka / 2);
+          const float4 u0 = p[0], u1 = p[1];
+          const float raw[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            v[2 * q] = part ? raw[2 * q + 1] : raw[2 * q];
+            v[2 * q + 1] = part ? -raw[2 * q] : raw[2 * q + 1];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = 0.f;
+        }
+        bf16x8 bh, bl;
+        split8(v, bh, bl);
+#pragma unroll
+        for (int a = 0; a < MT; ++a) {
+          if (a < mt) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  // sum the waves tile by tile: waves 1..3 park a tile in LDS, wave 0 adds and stores it
+  const int ldp = nc_t * 16;
+  float* out = partial + (c * nchunks + chunk) * (long)(mt * 16) * ldp;
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) {
+      if (a < mt && b < nc_t) {  // uniform over the workgroup
+        if (wave > 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = acc[a][b][r];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float x = acc[a][b][r];
+#pragma unroll
+            for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
+            out[(long)(a * 16 + grp * 4 + r) * ldp + b * 16 + row] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+}
+
+// M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
+__global__ void k_overlap_reduce(const float* partial, double2* M, int ns, int nt, int nchunks,
+                                 int mt, int nc_t, long ncfg) {
+  const long n = ncfg * ns * nt;
+  const int ldp = nc_t * 16;
+  const long tile = (long)(mt * 16) * ldp;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / ((long)ns * nt);
+    const int r = (int)(i - c * ns * nt), s = r / nt, t = r - (r / nt) * nt;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nchunks; ++k) {
+      const float* p = partial + (c * nchunks + k) * tile + (long)s * ldp;
+      re += (double)p[2 * t];
+      im += (double)p[2 * t + 1];
+    }
+    M[i] = make_double2(re, im);
+  }
+}
+
+// chunk of K per workgroup: a multiple of the 4-wave step, about 16 chunks per configuration
+long overlap_bf16x3_kchunk(long vol) {
+  const long K = 2 * vol, step = OV_WAVES * OV_KSTEP;
+  long kc = (K / 16 + step - 1) / step * step;
+  return kc < step ? step : kc;
+}
+
+size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg) {
+  const int mt = (ns + 15) / 16, nc_t = (2 * nt + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const long nchunks = (K + kc - 1) / kc;
+  return (size_t)ncfg * nchunks * (mt * 16) * (nc_t * 16);
+}
+
+hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
+                                 int nt, long vol, int ncfg, float* partial, hipStream_t s) {
+  if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || 2 * nt > 16 * OV_MAXN || (vol & 3))
+    return hipErrorInvalidValue;
+  const int mt = (ns + 15) / 16, nc_t = (2 * nt + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const int nchunks = (int)((K + kc - 1) / kc);
+  const __bf16* hi = (const __bf16*)phi_split;
+  const __bf16* lo = hi + 2 * (long)ns * vol;
+  const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
+  // one instantiation per (sink tiles, column tiles), so only the used accumulators exist
+#define NRC_OV_CASE(A, B)                                                                    \
+  if (mt == A && nc_t == B) {                                                                \
+    hipLaunchKernelGGL((k_overlap_bf16x3<A, B>), grid, block, 0, s, hi, lo, psi, partial, ns, \
+                       nt, vol, mt, nc_t, kc);                                               \
+  }
+#define NRC_OV_ROW(A) \
+  NRC_OV_CASE(A, 1) NRC_OV_CASE(A, 2) NRC_OV_CASE(A, 3) NRC_OV_CASE(A, 4) NRC_OV_CASE(A, 5) \
+  NRC_OV_CASE(A, 6) NRC_OV_CASE(A, 7) NRC_OV_CASE(A, 8)
+  NRC_OV_ROW(1) NRC_OV_ROW(2) NRC_OV_ROW(3) NRC_OV_ROW(4)
+#undef NRC_OV_ROW
+#undef NRC_OV_CASE
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long n = (long)ncfg * ns * nt;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, ns,
+                     nt, nchunks, mt, nc_t, (long)ncfg);
+  return hipGetLastError();
+}
+
+}  // namespace nrc

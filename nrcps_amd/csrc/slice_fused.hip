@@ -9,9 +9,9 @@
 //       y-DFT (+1); z-DFT (+1) per row; LDS -> registers
 //   x-DFT (+1, registers) -> post multiplier -> store
 //
-// Reference split (k-space storage): pre = post = 1/xi_half, mid = (1 + A phi)/V, gather
+// K-space split (k-space storage): pre = post = 1/xi_half, mid = (1 + A phi)/V, gather
 // before the slice (the measurement at t precedes the evolution, production/slater_det/
-// main.C:206-215).  Baseline split (x-space storage): mid = kappa/V, gather at mid (full
+// main.C:206-215).  X-space split (x-space storage): mid = kappa/V, gather at mid (full
 // forward transform), post = 1 + A phi.
 //
 // Thread t owns the x-pencils (y, z) = t + q*NT, q < PPT, so every global load and store is
@@ -144,7 +144,7 @@ hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sin
 template <typename T, int L, int PPT>
 static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nvol), block(L * L / PPT);
-  // reference split: 1/xi_half, field, 1/xi_half, measure first; baseline split: kappa/V in
+  // k-space split: 1/xi_half, field, 1/xi_half, measure first; x-space split: kappa/V in
   // the middle with the measurement there, the field last
   if (a.gather_pre) return hipErrorInvalidValue;  // measured by k_gather_sites beforehand
   if (a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD && a.post.kind == MUL_TABLE &&
@@ -159,6 +159,10 @@ static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
            !a.gather_pre && a.gather_mid)
     hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_NONE, MUL_TABLE, MUL_FIELD, false, true>), grid,
                        block, 0, s, a);
+  else if (a.pre.kind == MUL_NONE && a.mid.kind == MUL_TABLE && a.post.kind == MUL_FIELD &&
+           !a.gather_pre && !a.gather_mid)  // x-space split measured by dense overlaps
+    hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_NONE, MUL_TABLE, MUL_FIELD, false, false>),
+                       grid, block, 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -18,7 +18,7 @@ REF_LUDCMP_PATH = os.path.join(HERE, "_ref", "libref_ludcmp.so")
 DISP_STANDARD, DISP_QUADRATIC, DISP_PERFECT = 0, 1, 2
 CUTOFF_NONE, CUTOFF_HARD = 0, 1
 INTER_NONE, INTER_XISQMINUSONE, INTER_ONEMINUSXIINVSQ = 0, 1, 2
-SPLIT_REFERENCE, SPLIT_BASELINE = 0, 1
+SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 FIELD_PHILOX, FIELD_RANLXS, FIELD_ARRAY = 0, 1, 2
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -258,7 +258,7 @@ def overlap(phi: np.ndarray, psi: np.ndarray) -> np.ndarray:
     return M.view(np.complex128).reshape(ns, nt)
 
 
-def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_REFERENCE, minors=False, smear=None,
+def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
                want_psi=False):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
@@ -311,7 +311,7 @@ def exact_c1(L, T, ka, kb, A):
     return C_
 
 
-def time_threads(L, N, slices, moms, kin, A=0.4, split=SPLIT_REFERENCE, nthreads=1,
+def time_threads(L, N, slices, moms, kin, A=0.4, split=SPLIT_KSPACE, nthreads=1,
                  seed=7012, config=0):
     """Wall seconds for `nthreads` independent config streams of `slices` slices each."""
     moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)

@@ -673,7 +673,7 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
                    double* M_out, double* psi_out) {
   const int L = d->L, N = d->N, T = d->T;
   const long V = (long)L * L * L;
-  const int xspace = d->split == ORC_SPLIT_BASELINE;
+  const int xspace = d->split == ORC_SPLIT_XSPACE;
   const int nm = d->minors ? N : 1;
   double* psi = (double*)malloc(2 * (size_t)N * V * sizeof(double));
   double* sink = (double*)malloc(2 * (size_t)N * V * sizeof(double));

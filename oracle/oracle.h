@@ -104,8 +104,8 @@ void orc_lu_det(const double* lu, int n, double d, double* logabs, double phase[
 void orc_overlap(const double* phi, const double* psi, double* M, int ns, int nt, long vol);
 
 /* ---- the correlator path ---- */
-enum { ORC_SPLIT_REFERENCE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
-       ORC_SPLIT_BASELINE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */
+enum { ORC_SPLIT_KSPACE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
+       ORC_SPLIT_XSPACE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */
 enum { ORC_FIELD_PHILOX = 0, ORC_FIELD_RANLXS = 1, ORC_FIELD_ARRAY = 2 };
 
 typedef struct {
@@ -115,7 +115,7 @@ typedef struct {
   int split;           /* ORC_SPLIT_* */
   int minors;          /* 0: det of the full N x N matrix; 1: all leading minors n = 1..N */
   const int* moms;     /* N momenta (n1, n2, n3) in [0, L): sources and sinks */
-  const double* kin;   /* V real: 1/xi_half (reference) or exp(-tau eps) (baseline) */
+  const double* kin;   /* V real: 1/xi_half (k-space split) or exp(-tau eps) (x-space split) */
   double A;            /* field strength: multiplier (1 + A phi) */
   const double* smear; /* optional V real form factor f(k); when set, the reference field is
                           Interaction::Initialize's (1 + F-[f F+ phi])/V and A is unused */

@@ -331,7 +331,7 @@ def test_overlap_is_conjugate_dot_and_unit_vector_gather(orc):
 
 # ------------------------------------------------------------------------------ the path
 def _ref_tables(L, tau=0.05, M=1.0):
-    """Reference split with BASELINE's physics: Kinetic(mass M/tau, dt=1/2), STANDARD."""
+    """K-space split at tau=0.05, M=1: Kinetic(mass M/tau, dt=1/2), STANDARD."""
     return __import__("oracle").kinetic_xi(L, 0, M / tau, 0.5)
 
 
@@ -340,7 +340,7 @@ def test_free_gas_closed_form(orc, split):
     # A = 0: plane waves are eigenvectors, det M(t) = prod_n kappa(k_n)^t (cf. tests/free_gas)
     L, N, T = 8, 7, 6
     moms = orc.lowest_momenta(L, N)
-    if split == orc.SPLIT_REFERENCE:
+    if split == orc.SPLIT_KSPACE:
         kin = _ref_tables(L)
         kap = 1 / kin ** 2
     else:
@@ -385,7 +385,7 @@ def test_exact_c1_matches_bruteforce_L2(orc, split):
     L, T, A = 2, 3, 0.4
     V = L ** 3
     moms = np.zeros((1, 3), dtype=np.int32)
-    if split == orc.SPLIT_REFERENCE:
+    if split == orc.SPLIT_KSPACE:
         kin = _ref_tables(L, tau=0.3)
         ka = kb = 1 / kin
     else:
@@ -404,13 +404,13 @@ def test_exact_c1_matches_bruteforce_L2(orc, split):
 
 
 def test_exact_c1_vs_monte_carlo_config1(orc):
-    # L=12, T=32, tau=0.05, M=1, A=0.4, k=0 orbital, 500 configurations (BASELINE wording:
+    # L=12, T=32, tau=0.05, M=1, A=0.4, k=0 orbital, 500 configurations (x-space split:
     # x-space storage, psi <- IFFT[e^{-tau eps} FFT psi] then (1 + A phi))
     L, T, A, ncfg = 12, 32, 0.4, 500
     kin = orc.kinetic_exp(L, 0.05, 1.0)
     exact = orc.exact_c1(L, T, kin, np.ones(L ** 3), A)
     moms = np.zeros((1, 3), dtype=np.int32)
-    s2 = np.array([_s2(orc.run_config(L, 1, T, moms, kin, A=A, split=orc.SPLIT_BASELINE,
+    s2 = np.array([_s2(orc.run_config(L, 1, T, moms, kin, A=A, split=orc.SPLIT_XSPACE,
                                       seed=7012, config=c)) for c in range(ncfg)])
     mean, err = s2.mean(0), s2.std(0, ddof=1) / np.sqrt(ncfg)
     assert np.all(np.abs(mean[1:] - exact[1:]) < 4 * err[1:] + 1e-12)
