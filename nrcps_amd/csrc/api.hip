@@ -265,11 +265,9 @@ struct nrc_ensemble {
   int nw = 0, nm = 1;
   std::vector<int> moms;
   void* psi = nullptr;       // [C][N][V] complex
-  void* tab_x = nullptr;     // reference: 1/xi_half (X pass)
-  void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass)
-  void* tab_x_lo = nullptr;  // FP32 remainders of the tables above
-  void* tab_yz_lo = nullptr;
-  void* smear = nullptr;     // reference + smear: f(k)
+  void* tab_x = nullptr;     // k-space split: 1/xi_half (X pass); MUL_TABLE format
+  void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass); MUL_TABLE format
+  void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
   void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
@@ -295,8 +293,7 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->tab_x_lo, e->tab_yz_lo, e->smear, e->sinks,
-                  e->work,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->smear, e->sinks, e->work,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev};
@@ -319,13 +316,16 @@ static int ens_upload(nrc_ensemble* e, void** dst, const std::vector<double>& h)
                                    : upload_table<float>(e, dst, h);
 }
 
-// a multiplier table: FP64 as is; FP32 as the nearest floats plus their remainders
-static int ens_upload_pair(nrc_ensemble* e, void** hi, void** lo, const std::vector<double>& h) {
-  NRC_TRY(ens_upload(e, hi, h));
-  if (e->d.prec == NRC_PREC_F64) return NRC_OK;
-  std::vector<double> r(h.size());
-  for (size_t i = 0; i < h.size(); ++i) r[i] = h[i] - (double)(float)h[i];
-  return upload_table<float>(e, lo, r);
+// a MUL_TABLE multiplier table: FP64 as is; FP32 as interleaved (nearest float, float nearest
+// to the remainder) pairs
+static int ens_upload_pair(nrc_ensemble* e, void** dst, const std::vector<double>& h) {
+  if (e->d.prec == NRC_PREC_F64) return ens_upload(e, dst, h);
+  std::vector<double> r(2 * h.size());
+  for (size_t i = 0; i < h.size(); ++i) {
+    r[2 * i] = (double)(float)h[i];
+    r[2 * i + 1] = h[i] - r[2 * i];
+  }
+  return upload_table<float>(e, dst, r);
 }
 
 template <typename X>
@@ -372,10 +372,10 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   std::vector<double> h(V);
   if (d->split == NRC_SPLIT_KSPACE) {
     for (long i = 0; i < V; ++i) h[i] = 1.0 / d->kin[i];
-    ENS_TRY(ens_upload_pair(e, &e->tab_x, &e->tab_x_lo, h));
+    ENS_TRY(ens_upload_pair(e, &e->tab_x, h));
     if (d->smear) {
       h.assign(d->smear, d->smear + V);
-      ENS_TRY(ens_upload(e, &e->smear, h));
+      ENS_TRY(ens_upload_pair(e, &e->smear, h));
       ENS_TRY(hipMalloc(&e->work, (size_t)C * V * elem_bytes(d->prec)) == hipSuccess
                   ? NRC_OK : NRC_ERR_NOMEM);
       ENS_TRY(hipMalloc(&e->rfield, (size_t)C * V * elem_bytes(d->prec) / 2) == hipSuccess
@@ -383,7 +383,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     }
   } else {
     for (long i = 0; i < V; ++i) h[i] = d->kin[i] / (double)V;
-    ENS_TRY(ens_upload_pair(e, &e->tab_yz, &e->tab_yz_lo, h));
+    ENS_TRY(ens_upload_pair(e, &e->tab_yz, h));
   }
   ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
   if (d->sinks) {  // dense sinks in the ensemble precision
@@ -506,11 +506,10 @@ static int ens_collect_profile(nrc_ensemble* e) {
   return NRC_OK;
 }
 
-static nrc::MulSpec mul_table(const void* t, const void* lo = nullptr) {
+static nrc::MulSpec mul_table(const void* t) {
   nrc::MulSpec m;
   m.kind = nrc::MUL_TABLE;
   m.table = t;
-  m.table_lo = lo;
   return m;
 }
 
@@ -668,7 +667,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
     a.norb = N;
     a.L = L;
     if (ref) {
-      a.pre = mul_table(e->tab_x, e->tab_x_lo);
+      a.pre = mul_table(e->tab_x);
       a.post = a.pre;
       if (e->smear) {
         a.mid.kind = nrc::MUL_RFIELD;
@@ -680,7 +679,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
         a.mid.a1 = e->d.A / (double)V;
       }
     } else {
-      a.mid = mul_table(e->tab_yz, e->tab_yz_lo);
+      a.mid = mul_table(e->tab_yz);
       if (!e->sinks) {
         a.gather_mid = 1;
         a.g = g;
@@ -711,11 +710,11 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
       a.fft_a = t > 0;
       a.fft_b = !last;
       if (ref) {
-        if (t > 0) a.m1 = mul_table(e->tab_x, e->tab_x_lo);
+        if (t > 0) a.m1 = mul_table(e->tab_x);
         if (!last) {
           a.gather = 1;
           a.g = g;
-          a.m2 = mul_table(e->tab_x, e->tab_x_lo);
+          a.m2 = mul_table(e->tab_x);
         }
       } else if (t > 0) {
         a.m1 = fieldspec;
@@ -756,7 +755,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
       } else {
         a.gather = 1;
         a.g = g;
-        a.m1 = mul_table(e->tab_yz, e->tab_yz_lo);
+        a.m1 = mul_table(e->tab_yz);
       }
       EvScope sc(e, 1);
       NRC_HIP(nrc::launch_yzpass(prec, a, s));

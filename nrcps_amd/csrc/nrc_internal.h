@@ -15,13 +15,14 @@ enum MulKind { MUL_NONE = 0, MUL_TABLE = 1, MUL_FIELD = 2, MUL_RFIELD = 3 };
 //   MUL_FIELD : a0 + a1 * phi(site), phi = +-1 from the configuration's packed bits
 //   MUL_RFIELD: table[config * V + site]          (per-configuration real field)
 //
-// In FP32 passes a multiplier is applied as v*hi + v*lo, with hi the float nearest to the
-// FP64 value and lo the float nearest to the remainder, so the factor carries ~48 bits and
-// the rounding of the factor itself does not bias the evolution slice after slice.
+// In FP32 passes a MUL_TABLE entry is an interleaved (hi, lo) float pair applied as
+// v*hi + v*lo, with hi the float nearest to the FP64 value and lo the float nearest to the
+// remainder, so the factor carries ~48 bits and its rounding does not bias the evolution
+// slice after slice; FP64 passes hold one double per site.  MUL_FIELD factors are split the
+// same way in registers.
 struct MulSpec {
   int kind = MUL_NONE;
-  const void* table = nullptr;     // device, float or double (the pass precision)
-  const void* table_lo = nullptr;  // device, FP32 remainders of a MUL_TABLE table (or null)
+  const void* table = nullptr;     // device: MUL_TABLE entries, or the MUL_RFIELD field
   const uint32_t* bits = nullptr;  // device, [config][nw]
   int nw = 0;
   double a0 = 0.0, a1 = 0.0;

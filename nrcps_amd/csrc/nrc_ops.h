@@ -18,39 +18,119 @@ __device__ __forceinline__ C2<T> scale2(C2<T> v, T hi, T lo) {
   }
 }
 
-// v[i] *= multiplier at site(i) = i * stride + off, the kind fixed at compile time
-template <typename T, int L, int KIND>
-__device__ __forceinline__ void apply_mul_k(C2<T>* v, const MulSpec& m, long cfg, long off,
-                                            long stride, long V) {
+// A MUL_TABLE table holds one TabEntry per site: (hi, lo) float pairs in FP32 passes, the
+// double itself in FP64 passes.
+template <typename T>
+struct TabEntryT {
+  using type = float2;
+};
+template <>
+struct TabEntryT<double> {
+  using type = double;
+};
+template <typename T>
+using TabEntry = typename TabEntryT<T>::type;
+
+template <typename T>
+__device__ __forceinline__ C2<T> scale_tab(C2<T> v, TabEntry<T> e) {
+  if constexpr (sizeof(T) == 4) {
+    return scale2<T>(v, e.x, e.y);
+  } else {
+    return scale2<T>(v, e, T(0));
+  }
+}
+
+// ------------------------------------------------------------------ buffer addressing
+// Global accesses of the slice kernels go through buffer resources: the uniform base lives in
+// SGPRs, each lane adds one 32-bit byte offset and each access a constant byte offset held in
+// an SGPR, so a whole pencil of loads needs a single address VGPR.  Accesses past `bytes`
+// read zero and are dropped on store.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kBufWord3 = 0x00020000;  // raw 32-bit accesses (gfx9 buffer descriptor word 3)
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kBufWord3);
+}
+
+template <typename U>
+__device__ __forceinline__ U bload(rsrc_t r, uint32_t voff, uint32_t soff) {
+  static_assert(sizeof(U) == 4 || sizeof(U) == 8 || sizeof(U) == 16, "buffer load size");
+  if constexpr (sizeof(U) == 4) {
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (sizeof(U) == 8) {
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+}
+
+template <typename U>
+__device__ __forceinline__ void bstore(U val, rsrc_t r, uint32_t voff, uint32_t soff) {
+  static_assert(sizeof(U) == 8 || sizeof(U) == 16, "buffer store size");
+  if constexpr (sizeof(U) == 8) {
+    using V2 = __attribute__((__vector_size__(2 * sizeof(uint32_t)))) uint32_t;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, val), r, voff, soff, 0);
+  } else {
+    using V4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, val), r, voff, soff, 0);
+  }
+}
+
+// v[i] *= multiplier at site(i) = i * STRIDE + off for i < L, the kind fixed at compile time.
+// Sites of one call differ by the compile-time STRIDE only, so table and field reads take one
+// lane offset plus constants.  V is the volume of one configuration.
+template <typename T, int L, long STRIDE, int KIND>
+__device__ __forceinline__ void apply_mul_k(C2<T>* v, const MulSpec& m, long cfg, uint32_t off,
+                                            long V) {
   if constexpr (KIND == MUL_FIELD) {
     const double dp = m.a0 + m.a1, dm = m.a0 - m.a1;
     const T ph = T(dp), mh = T(dm);
     const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
-    const uint32_t* bits = m.bits + cfg * m.nw;
+    const rsrc_t rb = make_rsrc(m.bits + cfg * m.nw, (uint32_t)m.nw * 4u);
 #pragma unroll
     for (int i = 0; i < L; ++i) {
-      const long site = i * stride + off;
-      const bool up = (bits[site >> 5] >> (site & 31)) & 1u;
+      uint32_t w, bit;
+      if constexpr (STRIDE % 32 == 0) {  // word = i * STRIDE / 32 + off / 32
+        w = bload<uint32_t>(rb, (off >> 5) << 2, (uint32_t)(i * (STRIDE / 32)) * 4u);
+        bit = off & 31u;
+      } else {
+        const uint32_t site = (uint32_t)(i * STRIDE) + off;
+        w = bload<uint32_t>(rb, (site >> 5) << 2, 0u);
+        bit = site & 31u;
+      }
+      const bool up = (w >> bit) & 1u;
       v[i] = scale2<T>(v[i], up ? ph : mh, up ? pl : ml);
     }
-  } else if constexpr (KIND == MUL_TABLE || KIND == MUL_RFIELD) {
-    const T* tab = reinterpret_cast<const T*>(m.table) + (KIND == MUL_RFIELD ? cfg * V : 0);
-    const T* lo = reinterpret_cast<const T*>(m.table_lo);
+  } else if constexpr (KIND == MUL_TABLE) {
+    using E = TabEntry<T>;
+    const rsrc_t rt = make_rsrc(m.table, (uint32_t)(V * sizeof(E)));
     // chunks of 8 sites with scheduling barriers between them: the table loads of later
-    // chunks are not hoisted, so at most 16 table registers are live at a time
+    // chunks are not hoisted, so few table registers are live at a time
+#pragma unroll
+    for (int i0 = 0; i0 < L; i0 += 8) {
+#pragma unroll
+      for (int i = i0; i < i0 + 8 && i < L; ++i)
+        v[i] = scale_tab<T>(v[i], bload<E>(rt, off * (uint32_t)sizeof(E),
+                                           (uint32_t)(i * STRIDE * (long)sizeof(E))));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else if constexpr (KIND == MUL_RFIELD) {
+    const rsrc_t rt = make_rsrc(reinterpret_cast<const T*>(m.table) + cfg * V,
+                                (uint32_t)(V * sizeof(T)));
 #pragma unroll
     for (int i0 = 0; i0 < L; i0 += 8) {
 #pragma unroll
       for (int i = i0; i < i0 + 8 && i < L; ++i) {
-        const long site = i * stride + off;
-        v[i] = scale2<T>(v[i], tab[site], (KIND == MUL_TABLE && lo) ? lo[site] : T(0));
+        const T f = bload<T>(rt, off * (uint32_t)sizeof(T), (uint32_t)(i * STRIDE * (long)sizeof(T)));
+        v[i] = scale2<T>(v[i], f, T(0));
       }
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
 
-// v[i] *= multiplier at site(i) = i * stride + off
+// v[i] *= multiplier at site(i) = i * stride + off (the kind chosen at run time)
 template <typename T, int L>
 __device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, long off,
                                           long stride, long V) {
@@ -68,13 +148,15 @@ __device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, 
     }
     return;
   }
-  const T* tab = reinterpret_cast<const T*>(m.table) + (m.kind == MUL_RFIELD ? cfg * V : 0);
-  const T* lo = reinterpret_cast<const T*>(m.table_lo);
+  if (m.kind == MUL_RFIELD) {
+    const T* tab = reinterpret_cast<const T*>(m.table) + cfg * V;
 #pragma unroll
-  for (int i = 0; i < L; ++i) {
-    const long site = i * stride + off;
-    v[i] = scale2<T>(v[i], tab[site], lo ? lo[site] : T(0));
+    for (int i = 0; i < L; ++i) v[i] = scale2<T>(v[i], tab[i * stride + off], T(0));
+    return;
   }
+  const TabEntry<T>* tab = reinterpret_cast<const TabEntry<T>*>(m.table);
+#pragma unroll
+  for (int i = 0; i < L; ++i) v[i] = scale_tab<T>(v[i], tab[i * stride + off]);
 }
 
 template <typename T, int L>

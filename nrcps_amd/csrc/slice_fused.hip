@@ -33,7 +33,8 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
   const long cfg = vol / a.norb;
   const int orb = (int)(vol - cfg * a.norb);
   const int tid = threadIdx.x;
-  C* base = reinterpret_cast<C*>(a.psi) + vol * V;
+  // this volume through a buffer resource: one lane offset per pencil, x * P in the constant
+  const rsrc_t rv = make_rsrc(reinterpret_cast<C*>(a.psi) + vol * V, (uint32_t)(V * sizeof(C)));
   C v[PPT][L];
   // pencil by pencil (scheduling barriers keep the next pencil's loads and table reads from
   // being hoisted over this pencil's transform, which would not fit the register file)
@@ -41,9 +42,10 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
   for (int q = 0; q < PPT; ++q) {
     const int yz = tid + q * NT;
 #pragma unroll
-    for (int x = 0; x < L; ++x) v[q][x] = base[(long)x * P + yz];
+    for (int x = 0; x < L; ++x)
+      v[q][x] = bload<C>(rv, (uint32_t)yz * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
     if constexpr (GPRE) gather_pencil<T, L>(v[q], a.g, yz, cfg, orb, a.norb);
-    apply_mul_k<T, L, KPRE>(v[q], a.pre, cfg, yz, P, V);
+    apply_mul_k<T, L, P, KPRE>(v[q], a.pre, cfg, (uint32_t)yz, V);
     fft_reg<T, L, -1>(v[q]);
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
       fft_reg<T, L, -1>(w);
       const int x = h * H + rx;
       if constexpr (GMID) gather_pencil<T, L>(w, a.g, x * L + rq, cfg, orb, a.norb);
-      apply_mul_k<T, L, KMID>(w, a.mid, cfg, (long)x * P + rq, L, V);
+      apply_mul_k<T, L, L, KMID>(w, a.mid, cfg, (uint32_t)(x * P + rq), V);
       fft_reg<T, L, +1>(w);
 #pragma unroll
       for (int y = 0; y < L; ++y) col[y * LP] = w[y];
@@ -103,9 +105,10 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
   for (int q = 0; q < PPT; ++q) {
     const int yz = tid + q * NT;
     fft_reg<T, L, +1>(v[q]);
-    apply_mul_k<T, L, KPOST>(v[q], a.post, cfg, yz, P, V);
+    apply_mul_k<T, L, P, KPOST>(v[q], a.post, cfg, (uint32_t)yz, V);
 #pragma unroll
-    for (int x = 0; x < L; ++x) base[(long)x * P + yz] = v[q][x];
+    for (int x = 0; x < L; ++x)
+      bstore<C>(v[q][x], rv, (uint32_t)yz * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
     __builtin_amdgcn_sched_barrier(0);
   }
 }
