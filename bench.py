@@ -62,6 +62,37 @@ def cpu_baseline(L, N, tau, A, slices, threads):
                       f"{secs:.2f} s wall"}
 
 
+def correlator(la, ph):
+    """C(t) = < det(up) det(down) > = < det^2 > over configurations (up and down share the
+    field and the orbitals), from log|det| and det/|det|; returned as (log scale, C / scale)."""
+    la2 = 2.0 * la[:, :, 0]
+    m = la2.max(axis=1, keepdims=True)
+    c = (np.exp(la2 - m) * ph[:, :, 0] ** 2).mean(axis=1)
+    return m[:, 0], c
+
+
+def precision_report(ctx, na, L, N, T, tau, A, nconf, seed):
+    """Per-t relative error of C(t) from the FP32 path against the FP64 path on the same
+    configurations (same Philox fields), and the largest log|det| difference."""
+    moms = _lowest(L, N)
+    kin = _tables(L, tau, 1.0)
+    out = {}
+    for prec in (na.PREC_F32, na.PREC_F64):
+        e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=prec, split=na.SPLIT_KSPACE)
+        out[prec] = e.run(seed, 0, T)
+        e.close()
+    (la32, ph32, _), (la64, ph64, _) = out[na.PREC_F32], out[na.PREC_F64]
+    m32, c32 = correlator(la32, ph32)
+    m64, c64 = correlator(la64, ph64)
+    rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+    return {"configs": nconf, "T": T, "max_rel_err_C": float(rel.max()),
+            "median_rel_err_C": float(np.median(rel)),
+            "rel_err_C_at_t": {str(t): float(rel[t]) for t in (0, T // 4, T // 2, 3 * T // 4,
+                                                                T - 1)},
+            "max_abs_err_logdet": float(np.abs(la32 - la64).max()),
+            "reference": "FP64 evolution (FFT, multipliers, overlaps) on the same fields"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +105,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision-configs", type=int, default=64,
+                    help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
@@ -136,6 +169,12 @@ def main():
                            "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
                 "bytes_per_cs": b_evo}
 
+    precision = None
+    if args.precision_configs > 0:
+        precision = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed)
+        print(f"precision: max rel err of C(t) {precision['max_rel_err_C']:.2e}",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -152,6 +191,7 @@ def main():
                    "parallelism": "single GPU, batched configurations"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "precision": precision,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]

@@ -105,17 +105,22 @@ __host__ __device__ constexpr TwTable<L> make_tw() {
 template <int L>
 inline constexpr TwTable<L> kTw = make_tw<L>();
 
-// x * (hi + lo) for an FP32 constant split into its nearest float and remainder
+// x * (hi + lo) for an FP32 constant split into its nearest float and remainder.  The small
+// product comes first and the large one last, inside one fma, so the remainder reaches the
+// result before its single final rounding (added after that rounding it would fall below
+// half an ulp and be lost).
 template <typename T>
 __device__ __forceinline__ T mulk(T x, T hi, T lo) {
-  if constexpr (sizeof(T) == 4) return fmaf(x, lo, x * hi);
+  if constexpr (sizeof(T) == 4) return fmaf(x, hi, x * lo);
   else return x * hi;
 }
 
 // a * e^{S 2 pi i e / L}; e is a compile-time constant once the caller is unrolled.
 // In FP32 the twiddle is applied as a*(hi + lo) with lo the FP32 remainder of the FP64 value,
-// so |w| = 1 to ~2^-48: the forward/backward pair of every slice then does not shrink or grow
-// the orbitals systematically, which rounded twiddles alone do by ~1e-7 per slice.
+// the remainder terms summed first and the hi products last (see mulk).  Twiddles rounded to
+// FP32 are biased in magnitude: every forward/backward pair would then shrink the orbitals by
+// ~4e-8 (L = 32), coherently slice after slice; with the remainders folded in before the
+// final rounding the pair is unbiased to ~1e-11.
 template <typename T, int L, int S, int E>
 __device__ __forceinline__ C2<T> twiddle_c(C2<T> a) {
   if constexpr (E == 0) {
@@ -131,8 +136,8 @@ __device__ __forceinline__ C2<T> twiddle_c(C2<T> a) {
     if constexpr (sizeof(T) == 4) {
       constexpr float hx = (float)wc, hy = (float)ws;
       constexpr float lx = (float)(wc - (double)hx), ly = (float)(ws - (double)hy);
-      const float re = fmaf(a.x, lx, fmaf(-a.y, ly, fmaf(a.x, hx, -a.y * hy)));
-      const float im = fmaf(a.x, ly, fmaf(a.y, lx, fmaf(a.x, hy, a.y * hx)));
+      const float re = fmaf(a.x, hx, fmaf(-a.y, hy, fmaf(a.x, lx, -a.y * ly)));
+      const float im = fmaf(a.x, hy, fmaf(a.y, hx, fmaf(a.x, ly, a.y * lx)));
       return {re, im};
     } else {
       return cmul(a, C2<double>{wc, ws});

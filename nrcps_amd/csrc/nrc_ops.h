@@ -8,11 +8,12 @@
 namespace nrc {
 
 // ------------------------------------------------------------------ per-site multipliers
-// v * (hi + lo): hi carries the factor, lo its FP32 remainder (zero in FP64 passes)
+// v * (hi + lo): hi carries the factor, lo its FP32 remainder (zero in FP64 passes); the
+// remainder product enters the fma of the main product, before its one rounding
 template <typename T>
 __device__ __forceinline__ C2<T> scale2(C2<T> v, T hi, T lo) {
   if constexpr (sizeof(T) == 4) {
-    return {fmaf(v.x, lo, v.x * hi), fmaf(v.y, lo, v.y * hi)};
+    return {fmaf(v.x, hi, v.x * lo), fmaf(v.y, hi, v.y * lo)};
   } else {
     return {v.x * hi, v.y * hi};
   }
