@@ -93,6 +93,59 @@ def precision_report(ctx, na, L, N, T, tau, A, nconf, seed):
             "reference": "FP64 evolution (FFT, multipliers, overlaps) on the same fields"}
 
 
+def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2):
+    """4096 configurations of uneven shape: L uniform in {16, 24, 32}, N uniform in
+    {1, 7, 19, 27, 33} (closed shells), drawn with a fixed seed and grouped by shape; every
+    group is autotuned, then all groups run together, T slices captured into one HIP graph.
+    Returns aggregate and per-shape configs*slices/s."""
+    rng = np.random.default_rng(4096)
+    Ls = rng.choice([16, 24, 32], nconf)
+    Ns = rng.choice([1, 7, 19, 27, 33], nconf)
+    shapes = sorted({(int(l), int(n)) for l, n in zip(Ls, Ns)})
+    groups, per_shape = [], []
+    for L, N in shapes:
+        cnt = int(np.sum((Ls == L) & (Ns == N)))
+        e = na.Ensemble(ctx, L, _lowest(L, N), _tables(L, tau, 1.0), A, cnt, prec=na.PREC_F32,
+                        split=na.SPLIT_KSPACE)
+        path, ms = e.tune(3)
+        groups.append(e)
+        per_shape.append({"L": L, "N": N, "configs": cnt,
+                          "path": "single-pass" if path == na.PATH_SINGLE_PASS else "two-pass",
+                          "tuned_ms_per_slice": ms})
+    # each shape alone
+    for e, rec in zip(groups, per_shape):
+        e.run(seed, 0, T)
+        ctx.sync()
+        t0 = time.perf_counter()
+        e.run(seed, 1 << 20, T)
+        rec["cs_per_s_alone"] = rec["configs"] * T / (time.perf_counter() - t0)
+    b = na.Batch(ctx, groups)
+    out = {}
+    for graph in (True, False):
+        c0 = [0] * len(groups)
+        b.run(seed, c0, T, graph=graph)  # warm-up (and the capture)
+        ts = []
+        for r in range(runs):
+            c0 = [(r + 1) << 20] * len(groups)
+            t0 = time.perf_counter()
+            b.run(seed, c0, T, graph=graph)
+            ts.append(time.perf_counter() - t0)
+        out[graph] = (min(ts), b.capture_ms() if graph else 0.0)
+    b.close()
+    for e in groups:
+        e.close()
+    (tg, cap), (tn, _) = out[True], out[False]
+    return {"workload": f"{nconf} configurations, L uniform in {{16,24,32}}, N uniform in "
+                        f"{{1,7,19,27,33}}, T={T}, tau={tau}, A={A}, FP32 evolution, FP64 LU",
+            "configs": nconf, "T": T, "shape_groups": len(groups),
+            "value": nconf * T / tg, "unit": "configs*slices/s",
+            "ms_per_run": 1e3 * tg, "graph": True,
+            "capture_and_instantiate_ms": cap,
+            "value_without_graph": nconf * T / tn,
+            "timing": "one graph launch per run, capture included only in the warm-up run",
+            "per_shape": per_shape}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +158,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched uneven-shape measurement")
     ap.add_argument("--precision-configs", type=int, default=64,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
     ap.add_argument("--path", type=int, default=0,
@@ -169,6 +224,13 @@ def main():
                            "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
                 "bytes_per_cs": b_evo}
 
+    batched = None
+    if not args.no_batched:
+        ens.close()
+        batched = batched_uneven(ctx, na)
+        print(f"batched uneven shapes: {batched['value']:.0f} configs*slices/s",
+              file=sys.stderr, flush=True)
+
     precision = None
     if args.precision_configs > 0:
         precision = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed)
@@ -192,6 +254,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "precision": precision,
+        "batched_uneven": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]

@@ -154,7 +154,30 @@ int nrc_ensemble_set_path(nrc_ensemble* ens, int path);
  * 4 sink gather.  Returns total ms and launches. */
 int nrc_ensemble_profile(nrc_ensemble* ens, int enable);
 int nrc_ensemble_kernel_time(nrc_ensemble* ens, int kind, double* total_ms, long* launches);
+/* The per-slice results of the ensemble's last run (nrc_ensemble_run, _run_fields or a batch
+ * run), [t][c][m] as nrc_ensemble_run returns them; any output may be NULL. */
+int nrc_ensemble_results(nrc_ensemble* ens, double* logabs, double* phase, double* argsum);
+/* Autotuner: times every evolution path the ensemble's shape supports (two-pass FFT passes,
+ * single-pass slice kernel) over T_probe slices and keeps the fastest for later runs. */
+int nrc_ensemble_tune(nrc_ensemble* ens, int T_probe, int* path, double* ms_per_slice);
 int nrc_ensemble_destroy(nrc_ensemble* ens);
+
+/* ---- batches of uneven shapes: many configurations of different (L, N) evolved together,
+ *      one ensemble per shape group.  No counterpart in the reference, which runs one shape
+ *      per process (production/slater_det/main.C:185-247); each group is that loop.
+ * nrc_batch_create: the groups (ensembles of this context, in no other batch) get their own
+ *   streams; destroy the batch before its ensembles.
+ * nrc_batch_run: T slices of every group, group i on configurations config0[i] ..
+ *   (Philox(seed) fields).  graph != 0 captures the kernels of all groups and slices into one
+ *   HIP graph (fork/join over the group streams) and launches it as one unit; the graph is
+ *   reused while T, seed and config0 are unchanged.  Results: nrc_ensemble_results per group.
+ * nrc_batch_capture_ms: host time of the last capture and instantiation. */
+typedef struct nrc_batch nrc_batch;
+int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, int n_groups,
+                     nrc_batch** batch);
+int nrc_batch_run(nrc_batch* batch, uint64_t seed, const uint32_t* config0, int T, int graph);
+int nrc_batch_capture_ms(nrc_batch* batch, double* ms);
+int nrc_batch_destroy(nrc_batch* batch);
 
 #ifdef __cplusplus
 }

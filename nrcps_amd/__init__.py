@@ -31,7 +31,9 @@ EXPORTS = [
     "nrc_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_profile",
-    "nrc_ensemble_kernel_time", "nrc_ensemble_destroy",
+    "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
+    "nrc_ensemble_destroy", "nrc_batch_create", "nrc_batch_run", "nrc_batch_capture_ms",
+    "nrc_batch_destroy",
 ]
 
 
@@ -79,7 +81,13 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
         "nrc_ensemble_profile": [vp, i],
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
+        "nrc_ensemble_results": [vp, dp, dp, dp],
+        "nrc_ensemble_tune": [vp, i, C.POINTER(i), C.POINTER(C.c_double)],
         "nrc_ensemble_destroy": [vp],
+        "nrc_batch_create": [vp, vp, i, C.POINTER(vp)],
+        "nrc_batch_run": [vp, u64, vp, i, i],
+        "nrc_batch_capture_ms": [vp, C.POINTER(C.c_double)],
+        "nrc_batch_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -276,10 +284,57 @@ class Ensemble:
                "kernel_time")
         return ms.value, n.value
 
+    def results(self, T: int):
+        """The last run's [t][c][m] outputs (also after a batch run)."""
+        la, ph, ag = self._outs(T)
+        _check(self.lib.nrc_ensemble_results(self.h, _ptr(la), _ptr(ph), _ptr(ag)),
+               "ensemble_results")
+        return la, ph, ag
+
+    def tune(self, T_probe: int = 4):
+        """Autotuner (nrc_ensemble_tune): returns (chosen path, ms per slice)."""
+        p, ms = C.c_int(), C.c_double()
+        _check(self.lib.nrc_ensemble_tune(self.h, T_probe, C.byref(p), C.byref(ms)),
+               "ensemble_tune")
+        return p.value, ms.value
+
     def close(self):
         # an ensemble lives on its context's device; after nrc_finalize it is gone with it
         if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.lib.nrc_ensemble_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Shape groups evolved together (nrc_batch_*): one Ensemble per (L, N) group."""
+
+    def __init__(self, ctx: Context, groups):
+        self.ctx, self.lib, self.groups = ctx, ctx.lib, list(groups)
+        arr = (C.c_void_p * len(self.groups))(*[g.h for g in self.groups])
+        h = C.c_void_p()
+        _check(self.lib.nrc_batch_create(ctx.h, arr, len(self.groups), C.byref(h)),
+               "batch_create")
+        self.h = h
+
+    def run(self, seed: int, config0, T: int, graph: bool = True):
+        c0 = np.ascontiguousarray(config0, dtype=np.uint32)
+        assert c0.size == len(self.groups)
+        _check(self.lib.nrc_batch_run(self.h, seed, _ptr(c0), T, int(graph)), "batch_run")
+
+    def capture_ms(self) -> float:
+        ms = C.c_double()
+        _check(self.lib.nrc_batch_capture_ms(self.h, C.byref(ms)), "batch_capture_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.lib.nrc_batch_destroy(self.h)
         self.h = None
 
     def __del__(self):

@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "nrc_device.h"
@@ -198,6 +199,29 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf, long vol, int prec) {
 }
 
 // ------------------------------------------------------------------ overlaps, LU, fields
+}  // extern "C"
+
+// workspace of one split-BF16x3 overlap call: the split sinks (hi and lo bf16) and the
+// per-chunk FP32 partial tiles
+static size_t overlap_split_bytes(int ns, long vol) {
+  return ((size_t)4 * ns * vol * 2 + 255) / 256 * 256;
+}
+static size_t overlap_bf16x3_workspace(int ns, int nt, long vol, int ncfg) {
+  return overlap_split_bytes(ns, vol) +
+         nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg) * sizeof(float);
+}
+static int overlap_bf16x3(const void* phi, const void* psi, double* M, int ns, int nt, long vol,
+                          int ncfg, void* work, hipStream_t s) {
+  char* w = (char*)work;
+  const size_t split = overlap_split_bytes(ns, vol);
+  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s));
+  NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
+                                     (float*)(w + split), s));
+  return NRC_OK;
+}
+
+extern "C" {
+
 int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns, int nt,
                 long vol, int ncfg, int mode) {
   if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
@@ -210,14 +234,10 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
   if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
   if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
   if (ncfg == 0) return NRC_OK;
-  const size_t split = ((size_t)4 * ns * vol * 2 + 255) / 256 * 256;  // hi + lo bf16
-  const size_t part = nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg) * sizeof(float);
-  char* w = (char*)ctx_workspace(ctx, split + part);
+  const size_t bytes = overlap_bf16x3_workspace(ns, nt, vol, ncfg);
+  void* w = ctx_workspace(ctx, bytes);
   if (!w) return NRC_ERR_NOMEM;
-  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, ctx->stream));
-  NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
-                                     (float*)(w + split), ctx->stream));
-  return NRC_OK;
+  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream);
 }
 
 int nrc_lu_det(nrc_context* ctx, const double* M, int n, int ldm, int ncfg, int minors,
@@ -260,6 +280,7 @@ int nrc_z2_bits(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, ui
 // ------------------------------------------------------------------ ensemble engine
 struct nrc_ensemble {
   nrc_context* ctx = nullptr;
+  hipStream_t stream = nullptr;  // the context's stream, or a batch stream while batched
   nrc_ensemble_desc d{};
   long V = 0;
   int nw = 0, nm = 1;
@@ -269,6 +290,7 @@ struct nrc_ensemble {
   void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass); MUL_TABLE format
   void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
   void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
+  void* ovl_work = nullptr;  // workspace of the split-BF16x3 overlaps of dense FP32 sinks
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
@@ -285,6 +307,7 @@ struct nrc_ensemble {
   int* site_of_orb = nullptr;
   int* moms_dev = nullptr;
   int path = NRC_PATH_AUTO;
+  int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
   bool profile = false;
   double prof_ms[5] = {0, 0, 0, 0, 0};
   long prof_n[5] = {0, 0, 0, 0, 0};
@@ -293,7 +316,7 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->smear, e->sinks, e->work,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->smear, e->sinks, e->ovl_work, e->work,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev};
@@ -347,6 +370,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   hipSetDevice(ctx->device);
   nrc_ensemble* e = new nrc_ensemble();
   e->ctx = ctx;
+  e->stream = ctx->stream;
   e->d = *d;
   e->V = (long)L * L * L;
   e->nw = (int)((e->V + 31) / 32);
@@ -392,11 +416,17 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
       ENS_TRY(hipMemcpy(e->sinks, d->sinks, (size_t)N * V * 16, hipMemcpyHostToDevice) ==
                       hipSuccess ? NRC_OK : NRC_ERR_HIP);
     } else {
+      if ((V & 3) != 0) {
+        ens_free(e);
+        return NRC_ERR_UNSUPPORTED;
+      }
       std::vector<float> f((size_t)2 * N * V);
       for (size_t i = 0; i < f.size(); ++i) f[i] = (float)d->sinks[i];
       ENS_TRY(hipMalloc(&e->sinks, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
       ENS_TRY(hipMemcpy(e->sinks, f.data(), f.size() * 4, hipMemcpyHostToDevice) == hipSuccess
                   ? NRC_OK : NRC_ERR_HIP);
+      ENS_TRY(hipMalloc(&e->ovl_work, overlap_bf16x3_workspace(N, N, V, (int)C)) == hipSuccess
+                  ? NRC_OK : NRC_ERR_NOMEM);
     }
   }
   // sink gather lists: k-space split in the X pass (pencil (k1,k2), position k0);
@@ -442,7 +472,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
 extern "C" int nrc_ensemble_destroy(nrc_ensemble* e) {
   if (!e) return NRC_ERR_ARG;
   hipSetDevice(e->ctx->device);
-  hipStreamSynchronize(e->ctx->stream);
+  hipStreamSynchronize(e->stream);
   ens_free(e);
   return NRC_OK;
 }
@@ -483,16 +513,16 @@ struct EvScope {
     e->ev.push_back(a);
     e->ev.push_back(b);
     e->ev_kind.push_back(kind);
-    hipEventRecord(a, e->ctx->stream);
+    hipEventRecord(a, e->stream);
   }
   ~EvScope() {
-    if (e->profile) hipEventRecord(e->ev[idx + 1], e->ctx->stream);
+    if (e->profile) hipEventRecord(e->ev[idx + 1], e->stream);
   }
 };
 
 static int ens_collect_profile(nrc_ensemble* e) {
   if (!e->profile) return NRC_OK;
-  NRC_HIP(hipStreamSynchronize(e->ctx->stream));
+  NRC_HIP(hipStreamSynchronize(e->stream));
   for (size_t i = 0; i < e->ev_kind.size(); ++i) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, e->ev[2 * i], e->ev[2 * i + 1]) == hipSuccess) {
@@ -537,7 +567,7 @@ __global__ void k_complex_to_field(const nrc::C2<T>* in, T* out, long n, double 
 static int ens_smear_field(nrc_ensemble* e, const uint32_t* words) {
   const long n = (long)e->d.n_configs * e->V;
   long blocks = std::min<long>(65536, (n + 255) / 256);
-  hipStream_t s = e->ctx->stream;
+  hipStream_t s = e->stream;
   if (e->d.prec == NRC_PREC_F64)
     hipLaunchKernelGGL(k_bits_to_complex<double>, dim3(blocks), dim3(256), 0, s,
                        (nrc::C2<double>*)e->work, words, e->nw, e->V, (long)e->d.n_configs);
@@ -587,18 +617,14 @@ static int ens_reserve_outputs(nrc_ensemble* e, int T) {
   return NRC_OK;
 }
 
-// the slice loop; fields come from Philox (words_host == nullptr) or the caller
-static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
-                   int T, double* logabs, double* phase, double* argsum) {
+// allocations and host-to-device copies of a run of T slices (caller fields: words_host)
+static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (T < 1) return NRC_ERR_ARG;
   hipSetDevice(e->ctx->device);
-  hipStream_t s = e->ctx->stream;
-  const int N = e->d.n_orb, L = e->d.L, prec = e->d.prec;
-  const long C = e->d.n_configs, V = e->V;
-  const bool ref = e->d.split == NRC_SPLIT_KSPACE;
-  const size_t slice_words = (size_t)C * e->nw;
+  const size_t slice_words = (size_t)e->d.n_configs * e->nw;
   const size_t need_words = words_host ? slice_words * T : slice_words;
   if (need_words > e->words_cap) {
+    NRC_HIP(hipStreamSynchronize(e->stream));
     hipFree(e->words);
     e->words = nullptr;
     e->words_cap = 0;
@@ -607,8 +633,18 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   }
   if (words_host)
     NRC_HIP(hipMemcpyAsync(e->words, words_host, need_words * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, s));
-  NRC_TRY(ens_reserve_outputs(e, T));
+                           hipMemcpyHostToDevice, e->stream));
+  return ens_reserve_outputs(e, T);
+}
+
+// the kernels of a run of T slices on the ensemble's stream; nothing here allocates, copies
+// to or from the host or synchronises, so a batch can capture it into a graph
+static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool host_words, int T) {
+  hipStream_t s = e->stream;
+  const int N = e->d.n_orb, L = e->d.L, prec = e->d.prec;
+  const long C = e->d.n_configs, V = e->V;
+  const bool ref = e->d.split == NRC_SPLIT_KSPACE;
+  const size_t slice_words = (size_t)C * e->nw;
   const size_t per_t = (size_t)C * e->nm;
 
   // Propagator::Set(one_bodies[n]->Get(), n): plane-wave sources
@@ -626,7 +662,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   g.scale = ref ? 1.0 : 1.0 / sqrt((double)V);
 
   auto field_of = [&](int t) -> const uint32_t* {
-    return words_host ? e->words + (size_t)t * slice_words : e->words;
+    return host_words ? e->words + (size_t)t * slice_words : e->words;
   };
   auto lu = [&](int t) -> int {
     EvScope sc(e, 2);
@@ -648,15 +684,17 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
     if (measure_first) {
       {
         EvScope sc(e, 4);
-        if (e->sinks)
-          NRC_TRY(nrc_overlap(e->ctx, e->sinks, e->psi, (double*)e->M, N, N, V, (int)C,
-                              prec == NRC_PREC_F64 ? NRC_OVERLAP_F64 : NRC_OVERLAP_BF16X3));
+        if (e->sinks && prec == NRC_PREC_F64)
+          NRC_HIP(nrc::launch_overlap_dense(prec, e->sinks, e->psi, e->M, N, N, V, (int)C, s));
+        else if (e->sinks)
+          NRC_TRY(overlap_bf16x3(e->sinks, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work,
+                                 s));
         else
           NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
       }
       NRC_TRY(lu(t));
     }
-    if (!words_host) {
+    if (!host_words) {
       EvScope sc(e, 3);
       NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
     }
@@ -728,7 +766,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
     if (last) break;
     if (ref) NRC_TRY(lu(t));
     // Lattice::Refresh for slice t
-    if (!words_host) {
+    if (!host_words) {
       EvScope sc(e, 3);
       NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
     }
@@ -762,13 +800,26 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
     }
     if (!ref) NRC_TRY(lu(t));
   }
-  const size_t tot = per_t * T;
+  return NRC_OK;
+}
+
+// the per-slice results of the last run, [t][c][m], copied to the host (any may be null)
+static int ens_collect(nrc_ensemble* e, int T, double* logabs, double* phase, double* argsum) {
+  hipStream_t s = e->stream;
+  const size_t tot = (size_t)e->d.n_configs * e->nm * T;
   if (logabs) NRC_HIP(hipMemcpyAsync(logabs, e->o_la, tot * sizeof(double), hipMemcpyDeviceToHost, s));
   if (phase) NRC_HIP(hipMemcpyAsync(phase, e->o_ph, tot * sizeof(double2), hipMemcpyDeviceToHost, s));
   if (argsum) NRC_HIP(hipMemcpyAsync(argsum, e->o_as, tot * sizeof(double), hipMemcpyDeviceToHost, s));
   NRC_HIP(hipStreamSynchronize(s));
-  NRC_TRY(ens_collect_profile(e));
-  return NRC_OK;
+  return ens_collect_profile(e);
+}
+
+static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
+                   int T, double* logabs, double* phase, double* argsum) {
+  NRC_TRY(ens_prepare(e, words_host, T));
+  NRC_TRY(ens_enqueue(e, seed, config0, words_host != nullptr, T));
+  e->last_T = T;
+  return ens_collect(e, T, logabs, phase, argsum);
 }
 
 extern "C" int nrc_ensemble_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, int T,
@@ -787,7 +838,206 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
   if (!e || !out || c < 0 || c >= e->d.n_configs) return NRC_ERR_ARG;
   const size_t bytes = (size_t)e->d.n_orb * e->V * elem_bytes(e->d.prec);
   NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * bytes, bytes, hipMemcpyDeviceToHost,
-                         e->ctx->stream));
-  NRC_HIP(hipStreamSynchronize(e->ctx->stream));
+                         e->stream));
+  NRC_HIP(hipStreamSynchronize(e->stream));
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_results(nrc_ensemble* e, double* logabs, double* phase,
+                                    double* argsum) {
+  if (!e || e->last_T < 1) return NRC_ERR_ARG;
+  return ens_collect(e, e->last_T, logabs, phase, argsum);
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times every evolution path the ensemble's shape supports over a few slices (HIP events on
+// the ensemble's stream, after one untimed slice) and keeps the fastest.
+extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
+                                 double* ms_per_slice) {
+  if (!e || T_probe < 1) return NRC_ERR_ARG;
+  std::vector<int> cand = {NRC_PATH_TWO_PASS};
+  if (nrc::slice1_supported(e->d.prec, e->d.L)) cand.push_back(NRC_PATH_SINGLE_PASS);
+  if (e->sinks) cand = {NRC_PATH_SINGLE_PASS};
+  hipEvent_t a, b;
+  NRC_HIP(hipEventCreate(&a));
+  NRC_HIP(hipEventCreate(&b));
+  int best = cand[0];
+  float best_ms = -1.f;
+  int rc = NRC_OK;
+  for (int p : cand) {
+    e->path = p;
+    rc = ens_run(e, 0x5eedULL, 0xfffff000u, nullptr, 1, nullptr, nullptr, nullptr);
+    if (rc != NRC_OK) break;
+    rc = ens_prepare(e, nullptr, T_probe);
+    if (rc != NRC_OK) break;
+    hipEventRecord(a, e->stream);
+    rc = ens_enqueue(e, 0x5eedULL, 0xfffff000u, false, T_probe);
+    hipEventRecord(b, e->stream);
+    if (rc != NRC_OK) break;
+    if (hipEventSynchronize(b) != hipSuccess) { rc = NRC_ERR_HIP; break; }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    if (best_ms < 0.f || ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  e->last_T = 0;
+  if (rc != NRC_OK) return rc;
+  e->path = best;
+  if (path_out) *path_out = best;
+  if (ms_per_slice) *ms_per_slice = best_ms / T_probe;
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ batches of shape groups
+struct nrc_batch {
+  nrc_context* ctx = nullptr;
+  std::vector<nrc_ensemble*> g;
+  std::vector<hipStream_t> streams;  // one per group, default priority
+  std::vector<hipEvent_t> done;      // one per group (fork/join)
+  hipEvent_t fork = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int exec_T = 0;
+  uint64_t exec_seed = 0;
+  std::vector<uint32_t> exec_cfg;
+  double capture_ms = 0.0;
+};
+
+static void batch_drop_graph(nrc_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  b->exec_T = 0;
+}
+
+extern "C" int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, int n_groups,
+                                nrc_batch** out) {
+  if (!ctx || !groups || n_groups < 1 || !out) return NRC_ERR_ARG;
+  for (int i = 0; i < n_groups; ++i) {
+    if (!groups[i] || groups[i]->ctx != ctx || groups[i]->stream != ctx->stream)
+      return NRC_ERR_ARG;  // an ensemble belongs to one batch at a time
+    for (int j = 0; j < i; ++j)
+      if (groups[j] == groups[i]) return NRC_ERR_ARG;
+  }
+  hipSetDevice(ctx->device);
+  nrc_batch* b = new nrc_batch();
+  b->ctx = ctx;
+  int rc = NRC_OK;
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = NRC_ERR_HIP;
+  for (int i = 0; rc == NRC_OK && i < n_groups; ++i) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { rc = NRC_ERR_HIP; break; }
+    b->streams.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) { rc = NRC_ERR_HIP; break; }
+    b->done.push_back(ev);
+  }
+  if (rc != NRC_OK) {
+    for (hipStream_t st : b->streams) hipStreamDestroy(st);
+    for (hipEvent_t ev : b->done) hipEventDestroy(ev);
+    if (b->fork) hipEventDestroy(b->fork);
+    delete b;
+    return rc;
+  }
+  hipStreamSynchronize(ctx->stream);
+  for (int i = 0; i < n_groups; ++i) {
+    b->g.push_back(groups[i]);
+    groups[i]->stream = b->streams[i];
+  }
+  *out = b;
+  return NRC_OK;
+}
+
+// Runs T slices of every group: configurations config0[i] .. of group i, fields from
+// Philox(seed).  graph != 0: the groups' kernels are captured once into one HIP graph (the
+// context stream forks to the group streams and joins them again) and launched as a whole;
+// the graph is kept while T, seed and config0 stay the same.  graph == 0: the same kernels
+// are launched directly on the group streams.
+extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config0, int T,
+                             int graph) {
+  if (!b || !config0 || T < 1) return NRC_ERR_ARG;
+  hipSetDevice(b->ctx->device);
+  const int n = (int)b->g.size();
+  for (nrc_ensemble* e : b->g) {
+    e->profile = false;
+    NRC_TRY(ens_prepare(e, nullptr, T));
+    e->last_T = T;
+  }
+  for (hipStream_t st : b->streams) NRC_HIP(hipStreamSynchronize(st));
+  hipStream_t main = b->ctx->stream;
+  if (!graph) {
+    NRC_HIP(hipEventRecord(b->fork, main));
+    for (int i = 0; i < n; ++i) {
+      NRC_HIP(hipStreamWaitEvent(b->streams[i], b->fork, 0));
+      NRC_TRY(ens_enqueue(b->g[i], seed, config0[i], false, T));
+      NRC_HIP(hipEventRecord(b->done[i], b->streams[i]));
+      NRC_HIP(hipStreamWaitEvent(main, b->done[i], 0));
+    }
+    NRC_HIP(hipStreamSynchronize(main));
+    return NRC_OK;
+  }
+  const bool same = b->exec && b->exec_T == T && b->exec_seed == seed &&
+                    std::equal(b->exec_cfg.begin(), b->exec_cfg.end(), config0);
+  if (!same) {
+    batch_drop_graph(b);
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipEventRecord(t0, main);
+    hipEventSynchronize(t0);
+    const auto wall0 = std::chrono::steady_clock::now();
+    NRC_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+    int rc = NRC_OK;
+    if (hipEventRecord(b->fork, main) != hipSuccess) rc = NRC_ERR_HIP;
+    for (int i = 0; rc == NRC_OK && i < n; ++i) {
+      if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) { rc = NRC_ERR_HIP; break; }
+      rc = ens_enqueue(b->g[i], seed, config0[i], false, T);
+      if (rc != NRC_OK) break;
+      if (hipEventRecord(b->done[i], b->streams[i]) != hipSuccess ||
+          hipStreamWaitEvent(main, b->done[i], 0) != hipSuccess)
+        rc = NRC_ERR_HIP;
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(main, &gr);
+    if (rc == NRC_OK && ce != hipSuccess) rc = NRC_ERR_HIP;
+    if (rc == NRC_OK && hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0) != hipSuccess)
+      rc = NRC_ERR_HIP;
+    if (gr) hipGraphDestroy(gr);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    if (rc != NRC_OK) {
+      batch_drop_graph(b);
+      return rc;
+    }
+    b->capture_ms = std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - wall0).count();
+    b->exec_T = T;
+    b->exec_seed = seed;
+    b->exec_cfg.assign(config0, config0 + n);
+  }
+  NRC_HIP(hipGraphLaunch(b->exec, main));
+  NRC_HIP(hipStreamSynchronize(main));
+  return NRC_OK;
+}
+
+extern "C" int nrc_batch_capture_ms(nrc_batch* b, double* ms) {
+  if (!b || !ms) return NRC_ERR_ARG;
+  *ms = b->capture_ms;
+  return NRC_OK;
+}
+
+extern "C" int nrc_batch_destroy(nrc_batch* b) {
+  if (!b) return NRC_ERR_ARG;
+  hipSetDevice(b->ctx->device);
+  for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+  hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graph(b);
+  for (nrc_ensemble* e : b->g) e->stream = b->ctx->stream;
+  for (hipStream_t st : b->streams) hipStreamDestroy(st);
+  for (hipEvent_t ev : b->done) hipEventDestroy(ev);
+  hipEventDestroy(b->fork);
+  delete b;
   return NRC_OK;
 }
