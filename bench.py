@@ -146,6 +146,81 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
             "per_shape": per_shape}
 
 
+def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, n_plain=65536,
+                     n_learned=16384, steps=300, batch=64, lr=2e-3, seed=7012):
+    """Effective samples per second of the learned proposal against the uniform sampler.
+    The uniform sampler generates n_train configurations; the network is trained on them
+    (weighted maximum likelihood, det^2 weights); then n_plain uniform and n_learned proposed
+    configurations estimate C(t) = <det(t)^2>.  Per t in {T/4, T/2, 3T/4}: the relative
+    standard error of each estimate, the per-sample variance v = n rel_se^2 and the cost per
+    sample tau give the effective samples per second 1 / (v tau) at equal error."""
+    from nrcps_amd import proposal as pr
+
+    moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
+    ts = [T // 4, T // 2, 3 * T // 4]
+    chunk = min(n_train, n_learned, 16384)
+    ens = na.Ensemble(ctx, L, moms, kin, A, chunk, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+    ens.run(seed, 0, 2)  # warm-up
+    # uniform sampler: the training sample, then the comparison sample
+    t0 = time.perf_counter()
+    la_train = np.concatenate([ens.run(seed, k, T)[0] for k in range(0, n_train, chunk)], axis=1)
+    t_train_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    la_plain = np.concatenate([ens.run(seed, (1 << 22) + k, T)[0]
+                               for k in range(0, n_plain, chunk)], axis=1)
+    t_plain = time.perf_counter() - t0
+    # training
+    prop = pr.Proposal(ctx, L, max(batch, chunk))
+    t0 = time.perf_counter()
+    losses, mu, sd = pr.train(ctx, prop, la_train, seed, 0, steps, batch, lr, ts,
+                              np.random.default_rng(1))
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    # proposed configurations
+    sampler = pr.LearnedSampler(ctx, prop, ens, mu, sd)
+    sampler.run(seed + 1, 1 << 23, 2)  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    parts = [sampler.run(seed + 1, (1 << 24) + k, T) for k in range(0, n_learned, chunk)]
+    t_learned = time.perf_counter() - t0
+    la_l = np.concatenate([p[0] for p in parts], axis=1)
+    logr = np.concatenate([p[2] for p in parts], axis=1)
+    sp = pr.estimator_stats(la_plain, None, ts)
+    sl = pr.estimator_stats(la_l, logr, ts)
+    tau_p, tau_l = t_plain / n_plain, t_learned / n_learned
+    tau_l_amort = (t_learned + t_train + t_train_gen) / n_learned
+    per_t = {}
+    for t in ts:
+        vp = sp[t]["rel_se"] ** 2 * n_plain
+        vl = sl[t]["rel_se"] ** 2 * n_learned
+        sig = float(np.hypot(sp[t]["rel_se"], sl[t]["rel_se"]))
+        per_t[str(t)] = {
+            "log_C_plain": sp[t]["log_C"], "log_C_learned": sl[t]["log_C"],
+            "rel_se_plain": sp[t]["rel_se"], "rel_se_learned": sl[t]["rel_se"],
+            "bias_within_2sigma": bool(abs(sl[t]["log_C"] - sp[t]["log_C"]) <= 2 * sig),
+            "ess_plain": sp[t]["ess"], "ess_learned": sl[t]["ess"],
+            "eff_samples_per_s_plain": 1.0 / (vp * tau_p),
+            "eff_samples_per_s_learned": 1.0 / (vl * tau_l),
+            "gain": (vp * tau_p) / (vl * tau_l),
+            "gain_with_training_amortised": (vp * tau_p) / (vl * tau_l_amort)}
+    ps = prop.get_params()
+    prop.close()
+    ens.close()
+    w = np.exp(logr[ts[-1]] - logr[ts[-1]].max())  # the ratios r = prod 0.5 / q alone
+    return {"workload": f"L={L}, N={N}, T={T}, tau={tau}, A={A}; {n_train} uniform "
+                        f"configurations for training, {n_plain} uniform and {n_learned} "
+                        f"proposed configurations for C(t)",
+            "metric": "effective samples/s at equal statistical error of C(t)",
+            "gain_min_over_t": min(v["gain"] for v in per_t.values()),
+            "per_t": per_t,
+            "proposal_weight_ess_fraction_at_3T/4": float(w.sum() ** 2 / (w ** 2).sum() / w.size),
+            "seconds": {"uniform_training_sample": t_train_gen, "training": t_train,
+                        "uniform_comparison_sample": t_plain, "proposed_sample": t_learned},
+            "training": {"steps": steps, "batch": batch, "lr": lr,
+                         "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
+                         "output_bias": float(ps[8240])}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +235,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven-shape measurement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-proposal measurement")
     ap.add_argument("--precision-configs", type=int, default=64,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
     ap.add_argument("--path", type=int, default=0,
@@ -231,6 +308,13 @@ def main():
         print(f"batched uneven shapes: {batched['value']:.0f} configs*slices/s",
               file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned:
+        ens.close()
+        learned = learned_proposal(ctx, na)
+        print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
+              file=sys.stderr, flush=True)
+
     precision = None
     if args.precision_configs > 0:
         precision = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed)
@@ -255,6 +339,7 @@ def main():
         "cpu_baseline": cpu,
         "precision": precision,
         "batched_uneven": batched,
+        "learned_proposal": learned,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]

@@ -179,6 +179,60 @@ int nrc_batch_run(nrc_batch* batch, uint64_t seed, const uint32_t* config0, int 
 int nrc_batch_capture_ms(nrc_batch* batch, double* ms);
 int nrc_batch_destroy(nrc_batch* batch);
 
+/* ---- slice-by-slice stepping: the loop of nrc_ensemble_run (production/slater_det/
+ *      main.C:185-220) split so that the caller chooses each slice's field after seeing the
+ *      measurement at t (Lattice::Refresh, lattice.C:60-109, replaced by the caller's
+ *      words_dev[c][ceil(V/32)], device memory).  k-space split, single-pass path, on the
+ *      context stream (not while the ensemble is in a batch).
+ * nrc_ensemble_begin: plane-wave sources and result rows for T slices.
+ * nrc_ensemble_measure: overlaps and LU at t into result row t.
+ * nrc_ensemble_evolve: slice t with the given fields.
+ * nrc_ensemble_results_dev: device pointers of the [T][C][m] log|det| and phase rows. */
+int nrc_ensemble_begin(nrc_ensemble* ens, int T);
+int nrc_ensemble_measure(nrc_ensemble* ens, int t);
+int nrc_ensemble_evolve(nrc_ensemble* ens, int t, const uint32_t* words_dev);
+int nrc_ensemble_results_dev(nrc_ensemble* ens, const double** logabs_dev,
+                             const double** phase_dev);
+
+/* ---- learned field proposal.  No counterpart in the reference, whose fields are always
+ *      uniform Z2 (lattice.C:81-86, random.C:129-132); the uniform sampler stays the fallback
+ *      and the judge of the proposal's bias.  A 3-layer periodic 3D convolutional network
+ *      (3x3x3 kernels, 16 hidden channels, ReLU; BF16 weights and activations, FP32
+ *      accumulation) maps phi(t-1) and the configuration's normalised log-weight u to
+ *      per-site logits l of P(phi(t) = +1).  Parameters: P = nrc_proposal_param_count()
+ *      floats, W1[16][27][2] b1[16] W2[16][27][16] b2[16] W3[27][16] b3, taps 9(dx+1) +
+ *      3(dy+1) + (dz+1), out[s] = sum W[tap] in[s + d(tap)] with periodic wrap.
+ * nrc_proposal_forward: logits_dev[B][V] of prev_dev[B][nw] (NULL: no previous slice) and
+ *   u_dev[B] (all device).
+ * nrc_proposal_sample: phi(t) for configurations config0.. into words_dev[B][nw]: phi = +1
+ *   where the Philox4x32-10 word (counter (site/4, t, config, stream), key seed; output
+ *   site%4) lies below T = ceil(sigmoid(l) 2^32), so q(+1) = T/2^32 exactly, and
+ *   logr_dev[c] += sum over sites of log(0.5 / q(phi)) in FP64 (the exact reweighting ratio).
+ * nrc_proposal_train_step: gradient of sum_b w_b sum_s -log q(target_b(s)) / (V wsum) into
+ *   grad_out (host, may be NULL) and the loss; lr > 0 then takes an Adam step (0.9, 0.999,
+ *   1e-8) on the FP32 master parameters.
+ * nrc_norm_logw: u_dev[c] = (2 logabs_dev[c*stride] - mu) / sd.
+ * nrc_z2_bits_list: the Philox Z2 words of (configs_dev[i], ts_dev[i]) pairs (as nrc_z2_bits). */
+typedef struct nrc_proposal nrc_proposal;
+int nrc_proposal_create(nrc_context* ctx, int L, int max_batch, nrc_proposal** prop);
+int nrc_proposal_destroy(nrc_proposal* prop);
+int nrc_proposal_param_count(void);
+int nrc_proposal_set_params(nrc_proposal* prop, const float* params);
+int nrc_proposal_get_params(nrc_proposal* prop, float* params);
+int nrc_proposal_forward(nrc_proposal* prop, const uint32_t* prev_dev, const float* u_dev, int B,
+                         float* logits_dev);
+int nrc_proposal_sample(nrc_proposal* prop, const uint32_t* prev_dev, const float* u_dev, int B,
+                        uint64_t seed, uint32_t config0, uint32_t t, uint32_t stream,
+                        uint32_t* words_dev, double* logr_dev);
+int nrc_proposal_train_step(nrc_proposal* prop, const uint32_t* prev_dev,
+                            const uint32_t* target_dev, const float* u_dev, const float* w_dev,
+                            int B, double wsum, float lr, float* grad_out, double* loss_out);
+int nrc_norm_logw(nrc_context* ctx, const double* logabs_dev, int stride, int B, double mu,
+                  double sd, float* u_dev);
+int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t* configs_dev,
+                     const uint32_t* ts_dev, int n, uint32_t stream, int vol,
+                     uint32_t* words_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,11 @@ EXPORTS = [
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
     "nrc_ensemble_destroy", "nrc_batch_create", "nrc_batch_run", "nrc_batch_capture_ms",
-    "nrc_batch_destroy",
+    "nrc_batch_destroy", "nrc_ensemble_begin", "nrc_ensemble_measure", "nrc_ensemble_evolve",
+    "nrc_ensemble_results_dev", "nrc_proposal_create", "nrc_proposal_destroy",
+    "nrc_proposal_param_count", "nrc_proposal_set_params", "nrc_proposal_get_params",
+    "nrc_proposal_forward", "nrc_proposal_sample", "nrc_proposal_train_step", "nrc_norm_logw",
+    "nrc_z2_bits_list",
 ]
 
 
@@ -88,6 +92,17 @@ def load_library() -> C.CDLL:
         "nrc_batch_run": [vp, u64, vp, i, i],
         "nrc_batch_capture_ms": [vp, C.POINTER(C.c_double)],
         "nrc_batch_destroy": [vp],
+        "nrc_ensemble_begin": [vp, i], "nrc_ensemble_measure": [vp, i],
+        "nrc_ensemble_evolve": [vp, i, vp],
+        "nrc_ensemble_results_dev": [vp, C.POINTER(vp), C.POINTER(vp)],
+        "nrc_proposal_create": [vp, i, i, C.POINTER(vp)], "nrc_proposal_destroy": [vp],
+        "nrc_proposal_param_count": [], "nrc_proposal_set_params": [vp, vp],
+        "nrc_proposal_get_params": [vp, vp], "nrc_proposal_forward": [vp, vp, vp, i, vp],
+        "nrc_proposal_sample": [vp, vp, vp, i, u64, u32, u32, u32, vp, vp],
+        "nrc_proposal_train_step": [vp, vp, vp, vp, vp, i, C.c_double, C.c_float, vp,
+                                    C.POINTER(C.c_double)],
+        "nrc_norm_logw": [vp, vp, i, i, C.c_double, C.c_double, vp],
+        "nrc_z2_bits_list": [vp, u64, vp, vp, i, u32, i, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -257,6 +272,23 @@ class Ensemble:
         _check(self.lib.nrc_ensemble_run(self.h, seed, config0, T, _ptr(la), _ptr(ph), _ptr(ag)),
                "ensemble_run")
         return la, ph, ag
+
+    # slice-by-slice stepping (nrc_ensemble_begin / measure / evolve)
+    def begin(self, T: int):
+        _check(self.lib.nrc_ensemble_begin(self.h, T), "ensemble_begin")
+
+    def measure(self, t: int):
+        _check(self.lib.nrc_ensemble_measure(self.h, t), "ensemble_measure")
+
+    def evolve(self, t: int, words: "DeviceArray"):
+        _check(self.lib.nrc_ensemble_evolve(self.h, t, words.ptr), "ensemble_evolve")
+
+    def results_dev(self):
+        """Device addresses (ints) of the [T][C][m] log|det| and phase result rows."""
+        la, ph = C.c_void_p(), C.c_void_p()
+        _check(self.lib.nrc_ensemble_results_dev(self.h, C.byref(la), C.byref(ph)),
+               "ensemble_results_dev")
+        return la.value, ph.value
 
     def run_fields(self, words: np.ndarray, T: int):
         w = np.ascontiguousarray(words, dtype=np.uint32)

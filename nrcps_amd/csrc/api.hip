@@ -30,6 +30,11 @@ static void* ctx_workspace(nrc_context* c, size_t bytes) {
   return c->work;
 }
 
+namespace nrc {
+hipStream_t ctx_stream(nrc_context* c) { return c->stream; }
+int ctx_device(nrc_context* c) { return c->device; }
+}  // namespace nrc
+
 struct nrc_fft {
   nrc_context* ctx;
   int L, batch, prec;
@@ -617,6 +622,108 @@ static int ens_reserve_outputs(nrc_ensemble* e, int T) {
   return NRC_OK;
 }
 
+static nrc::GatherSpec ens_gather_spec(const nrc_ensemble* e) {
+  nrc::GatherSpec g;
+  g.ptr = e->g_ptr;
+  g.pos = e->g_pos;
+  g.sink = e->g_sink;
+  g.M = e->M;
+  g.nsink = e->d.n_orb;
+  g.scale = e->d.split == NRC_SPLIT_KSPACE ? 1.0 : 1.0 / sqrt((double)e->V);
+  return g;
+}
+
+// Propagator::Set(one_bodies[n]->Get(), n): plane-wave sources in every configuration
+static int ens_set_sources(nrc_ensemble* e) {
+  const int N = e->d.n_orb;
+  const long C = e->d.n_configs;
+  if (e->d.split == NRC_SPLIT_KSPACE)
+    NRC_HIP(nrc::launch_set_unit_sources(e->d.prec, e->psi, e->site_of_orb, C * N, N, e->V,
+                                         e->stream));
+  else
+    NRC_HIP(nrc::launch_set_plane_waves(e->d.prec, e->psi, e->moms_dev, C * N, N, e->d.L,
+                                        e->stream));
+  return NRC_OK;
+}
+
+// ---- one slice of the single-pass path, in two halves (the loop of ens_enqueue, and the
+// step entry points that let a caller choose each slice's field after seeing the measurement)
+
+// measurement at t into the results row t: overlaps (gathered plane-wave sinks or dense
+// sinks) and the LU log-det (production/slater_det/main.C:206-215)
+static int ens_measure(nrc_ensemble* e, int t) {
+  hipStream_t s = e->stream;
+  const int N = e->d.n_orb, prec = e->d.prec;
+  const long C = e->d.n_configs, V = e->V;
+  const size_t per_t = (size_t)C * e->nm;
+  {
+    EvScope sc(e, 4);
+    if (e->sinks && prec == NRC_PREC_F64)
+      NRC_HIP(nrc::launch_overlap_dense(prec, e->sinks, e->psi, e->M, N, N, V, (int)C, s));
+    else if (e->sinks)
+      NRC_TRY(overlap_bf16x3(e->sinks, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s));
+    else
+      NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
+  }
+  EvScope sc(e, 2);
+  NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
+                             e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
+                             nullptr, nullptr, nullptr, s));
+  return NRC_OK;
+}
+
+// evolution of slice t with the fields words[c][nw] (device), single-pass kernel; for the
+// x-space split with gathered sinks the measurement of t happens inside the slice
+static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
+  hipStream_t s = e->stream;
+  const int N = e->d.n_orb, L = e->d.L, prec = e->d.prec;
+  const long C = e->d.n_configs, V = e->V;
+  const bool ref = e->d.split == NRC_SPLIT_KSPACE;
+  const size_t per_t = (size_t)C * e->nm;
+  if (ref && e->smear) NRC_TRY(ens_smear_field(e, words));
+  nrc::MulSpec fieldspec;
+  fieldspec.kind = nrc::MUL_FIELD;
+  fieldspec.nw = e->nw;
+  fieldspec.bits = words;
+  nrc::SliceArgs a;
+  a.psi = e->psi;
+  a.nvol = C * N;
+  a.norb = N;
+  a.L = L;
+  if (ref) {
+    a.pre = mul_table(e->tab_x);
+    a.post = a.pre;
+    if (e->smear) {
+      a.mid.kind = nrc::MUL_RFIELD;
+      a.mid.table = e->rfield;
+    } else {
+      a.mid = fieldspec;
+      a.mid.a0 = 1.0 / (double)V;
+      a.mid.a1 = e->d.A / (double)V;
+    }
+  } else {
+    a.mid = mul_table(e->tab_yz);
+    if (!e->sinks) {
+      a.gather_mid = 1;
+      a.g = ens_gather_spec(e);
+    }
+    a.post = fieldspec;
+    a.post.a0 = 1.0;
+    a.post.a1 = e->d.A;
+  }
+  {
+    EvScope sc(e, 0);
+    NRC_HIP(nrc::launch_slice1(prec, a, s));
+  }
+  if (!ref && !e->sinks) {  // measured inside the slice
+    EvScope sc(e, 2);
+    NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
+                               e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
+                               nullptr, nullptr, nullptr, s));
+  }
+  return NRC_OK;
+}
+
 // allocations and host-to-device copies of a run of T slices (caller fields: words_host)
 static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (T < 1) return NRC_ERR_ARG;
@@ -647,19 +754,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   const size_t slice_words = (size_t)C * e->nw;
   const size_t per_t = (size_t)C * e->nm;
 
-  // Propagator::Set(one_bodies[n]->Get(), n): plane-wave sources
-  if (ref)
-    NRC_HIP(nrc::launch_set_unit_sources(prec, e->psi, e->site_of_orb, C * N, N, V, s));
-  else
-    NRC_HIP(nrc::launch_set_plane_waves(prec, e->psi, e->moms_dev, C * N, N, L, s));
-
-  nrc::GatherSpec g;
-  g.ptr = e->g_ptr;
-  g.pos = e->g_pos;
-  g.sink = e->g_sink;
-  g.M = e->M;
-  g.nsink = N;
-  g.scale = ref ? 1.0 : 1.0 / sqrt((double)V);
+  NRC_TRY(ens_set_sources(e));
+  const nrc::GatherSpec g = ens_gather_spec(e);
 
   auto field_of = [&](int t) -> const uint32_t* {
     return host_words ? e->words + (size_t)t * slice_words : e->words;
@@ -681,57 +777,12 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   if (e->sinks && !single) return NRC_ERR_UNSUPPORTED;  // dense sinks read the stored state
   const bool measure_first = ref || e->sinks;
   for (int t = 0; single && t < T; ++t) {
-    if (measure_first) {
-      {
-        EvScope sc(e, 4);
-        if (e->sinks && prec == NRC_PREC_F64)
-          NRC_HIP(nrc::launch_overlap_dense(prec, e->sinks, e->psi, e->M, N, N, V, (int)C, s));
-        else if (e->sinks)
-          NRC_TRY(overlap_bf16x3(e->sinks, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work,
-                                 s));
-        else
-          NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
-      }
-      NRC_TRY(lu(t));
-    }
+    if (measure_first) NRC_TRY(ens_measure(e, t));
     if (!host_words) {
       EvScope sc(e, 3);
       NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
     }
-    if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
-    nrc::SliceArgs a;
-    a.psi = e->psi;
-    a.nvol = C * N;
-    a.norb = N;
-    a.L = L;
-    if (ref) {
-      a.pre = mul_table(e->tab_x);
-      a.post = a.pre;
-      if (e->smear) {
-        a.mid.kind = nrc::MUL_RFIELD;
-        a.mid.table = e->rfield;
-      } else {
-        a.mid = fieldspec;
-        a.mid.bits = field_of(t);
-        a.mid.a0 = 1.0 / (double)V;
-        a.mid.a1 = e->d.A / (double)V;
-      }
-    } else {
-      a.mid = mul_table(e->tab_yz);
-      if (!e->sinks) {
-        a.gather_mid = 1;
-        a.g = g;
-      }
-      a.post = fieldspec;
-      a.post.bits = field_of(t);
-      a.post.a0 = 1.0;
-      a.post.a1 = e->d.A;
-    }
-    {
-      EvScope sc(e, 0);
-      NRC_HIP(nrc::launch_slice1(prec, a, s));
-    }
-    if (!measure_first) NRC_TRY(lu(t));
+    NRC_TRY(ens_evolve(e, t, field_of(t)));
   }
 
   // two passes per slice: an X pass that finishes slice t-1, measures t and starts slice t,
@@ -1039,5 +1090,40 @@ extern "C" int nrc_batch_destroy(nrc_batch* b) {
   for (hipEvent_t ev : b->done) hipEventDestroy(ev);
   hipEventDestroy(b->fork);
   delete b;
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ slice-by-slice stepping
+static bool ens_step_ok(const nrc_ensemble* e) {
+  return e->stream == e->ctx->stream && e->d.split == NRC_SPLIT_KSPACE &&
+         (e->path == NRC_PATH_SINGLE_PASS ||
+          (e->path == NRC_PATH_AUTO && nrc::slice1_supported(e->d.prec, e->d.L)));
+}
+
+extern "C" int nrc_ensemble_begin(nrc_ensemble* e, int T) {
+  if (!e || T < 1) return NRC_ERR_ARG;
+  if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  NRC_TRY(ens_prepare(e, nullptr, T));
+  e->last_T = T;
+  return ens_set_sources(e);
+}
+
+extern "C" int nrc_ensemble_measure(nrc_ensemble* e, int t) {
+  if (!e || t < 0 || t >= e->last_T) return NRC_ERR_ARG;
+  if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  return ens_measure(e, t);
+}
+
+extern "C" int nrc_ensemble_evolve(nrc_ensemble* e, int t, const uint32_t* words_dev) {
+  if (!e || !words_dev || t < 0 || t >= e->last_T) return NRC_ERR_ARG;
+  if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  return ens_evolve(e, t, words_dev);
+}
+
+extern "C" int nrc_ensemble_results_dev(nrc_ensemble* e, const double** logabs_dev,
+                                        const double** phase_dev) {
+  if (!e || e->last_T < 1) return NRC_ERR_ARG;
+  if (logabs_dev) *logabs_dev = e->o_la;
+  if (phase_dev) *phase_dev = (const double*)e->o_ph;
   return NRC_OK;
 }

@@ -5,20 +5,9 @@
 namespace nrc {
 
 // ------------------------------------------------------------------ Philox Z2 fields
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Word w of configuration c's field at
-// slice t is output (w % 4) of the block with counter (w / 4, t, c, stream) and key
-// (seed lo, seed hi); bit j of word w is site 32 w + j, phi = +1 where set.
-__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
-    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
-  }
-}
-
+// Word w of configuration c's field at slice t is output (w % 4) of the Philox4x32-10 block
+// with counter (w / 4, t, c, stream) and key (seed lo, seed hi); bit j of word w is site
+// 32 w + j, phi = +1 where set.
 __global__ void k_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
                           uint32_t stream, int nw, uint32_t* words) {
   const int ng = (nw + 3) / 4;

@@ -108,4 +108,8 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 
+// the context's stream and device (api.hip), for entry points defined in other files
+hipStream_t ctx_stream(nrc_context* c);
+int ctx_device(nrc_context* c);
+
 }  // namespace nrc

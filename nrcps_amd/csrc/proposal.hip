@@ -1,0 +1,704 @@
+// proposal.hip -- the learned auxiliary-field proposal: a 3-layer periodic 3D convolutional
+// network (3x3x3 kernels, 16 hidden channels, BF16 weights and activations, FP32
+// accumulation) that maps the previous slice's field phi(t-1) and the configuration's current
+// normalised log-weight u to per-site Bernoulli logits for phi(t), its sampler with exact
+// reweighting ratios, and its training kernels (weighted maximum likelihood, Adam).
+//
+// Lattice sites are the reference's order z + L (y + L x) (src/utils/vector3.C:175-187); a
+// convolution is out[s] = sum_tap W[tap] in[s + d(tap)] with periodic wrap, tap =
+// 9 (dx+1) + 3 (dy+1) + (dz+1).  Activations are [config][site][16] bf16 (32 B per site).
+//
+// Parameters (FP32 master copy, P_N floats):
+//   W1 [16][27][2] (input 0: phi(t-1) in {-1, 0, +1}, input 1: u), b1 [16]   -> ReLU
+//   W2 [16][27][16], b2 [16]                                                   -> ReLU
+//   W3 [27][16], b3                                                            -> logit
+// The 16 -> 16 layer, forward and backward-data, runs on the matrix cores
+// (v_mfma_f32_16x16x32_bf16); the 2 -> 16 and 16 -> 1 layers and the weight gradients are
+// VALU kernels.
+#include "nrc_device.h"
+#include "nrc_internal.h"
+
+namespace nrc {
+
+constexpr int PC = 16;
+constexpr int NTAP = 27;
+constexpr int P_W1 = 0;
+constexpr int P_B1 = P_W1 + PC * NTAP * 2;
+constexpr int P_W2 = P_B1 + PC;
+constexpr int P_B2 = P_W2 + PC * NTAP * PC;
+constexpr int P_W3 = P_B2 + PC;
+constexpr int P_B3 = P_W3 + NTAP * PC;
+constexpr int P_N = P_B3 + 1;
+static_assert(P_N == 8241, "parameter count");
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct Lat {
+  int L;
+  long V;
+};
+
+// neighbour of site s at tap (periodic)
+__device__ __forceinline__ long nbr_site(const Lat& g, int x, int y, int z, int tap) {
+  const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+  int xx = x + dx, yy = y + dy, zz = z + dz;
+  xx += (xx < 0) ? g.L : 0;
+  xx -= (xx >= g.L) ? g.L : 0;
+  yy += (yy < 0) ? g.L : 0;
+  yy -= (yy >= g.L) ? g.L : 0;
+  zz += (zz < 0) ? g.L : 0;
+  zz -= (zz >= g.L) ? g.L : 0;
+  return ((long)xx * g.L + yy) * g.L + zz;
+}
+
+__device__ __forceinline__ void site_xyz(const Lat& g, long s, int& x, int& y, int& z) {
+  z = (int)(s % g.L);
+  const long r = s / g.L;
+  y = (int)(r % g.L);
+  x = (int)(r / g.L);
+}
+
+__device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
+
+// ------------------------------------------------------------------ parameters
+__global__ void k_fill_zero_f(float* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.f;
+}
+__global__ void k_fill_zero_d(double* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// bf16 copies of the master weights, and W2 flipped and transposed for the backward-data
+// convolution: w2t[ci][tap][co] = W2[co][26 - tap][ci]
+__global__ void k_refresh(const float* p, __bf16* wb, __bf16* w2t) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P_N; i += gridDim.x * blockDim.x) {
+    wb[i] = (__bf16)p[i];
+    if (i >= P_W2 && i < P_B2) {
+      const int j = i - P_W2, co = j / (NTAP * PC), tap = (j / PC) % NTAP, ci = j % PC;
+      w2t[(ci * NTAP + (NTAP - 1 - tap)) * PC + co] = (__bf16)p[i];
+    }
+  }
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; step is 1-based
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1,
+                       float b2, float eps, int step) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// ------------------------------------------------------------------ layer 1 (2 -> 16)
+// prev: [B][nw] field words of slice t-1 (null: no previous slice, phi = 0); u: [B]
+__global__ __launch_bounds__(256) void k_conv1(const uint32_t* __restrict__ prev, int nw,
+                                               const float* __restrict__ u,
+                                               const __bf16* __restrict__ wb,
+                                               __bf16* __restrict__ A1, Lat g, long B) {
+  const long n = B * g.V;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / g.V, s = i - c * g.V;
+    int x, y, z;
+    site_xyz(g, s, x, y, z);
+    float acc[PC];
+    const float uc = u[c];
+#pragma unroll
+    for (int co = 0; co < PC; ++co) {
+      float su = 0.f;
+      for (int tap = 0; tap < NTAP; ++tap) su += bf(wb[P_W1 + (co * NTAP + tap) * 2 + 1]);
+      acc[co] = bf(wb[P_B1 + co]) + uc * su;
+    }
+    if (prev) {
+      const uint32_t* w = prev + c * nw;
+      for (int tap = 0; tap < NTAP; ++tap) {
+        const long q = nbr_site(g, x, y, z, tap);
+        const float f = ((w[q >> 5] >> (q & 31)) & 1u) ? 1.f : -1.f;
+#pragma unroll
+        for (int co = 0; co < PC; ++co) acc[co] += f * bf(wb[P_W1 + (co * NTAP + tap) * 2]);
+      }
+    }
+    bf16x8 o0, o1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      o0[k] = (__bf16)fmaxf(acc[k], 0.f);
+      o1[k] = (__bf16)fmaxf(acc[8 + k], 0.f);
+    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(A1 + i * PC);
+    dst[0] = o0;
+    dst[1] = o1;
+  }
+}
+
+// ------------------------------------------------------------------ 16 -> 16 on the matrix cores
+// out[c][s][co] = sum_tap sum_ci w[co][tap][ci] in[c][s + d(tap)][ci] as a GEMM with M = 16
+// output channels, N = 16 sites per tile and K = 27 taps x 16 channels (14 steps of 32, the
+// last half zero).  v_mfma_f32_16x16x32_bf16: lane l supplies A row l % 16 and B column
+// l % 16 with the eight K values 8 (l / 16) .. +7 of the step, i.e. tap 2 step + (l / 16) / 2
+// and channels 8 ((l / 16) % 2) .. +7; the weights of a lane stay in registers.
+// FWD: out = bf16(relu(acc + bias)); otherwise out = bf16(acc) where mask > 0, else 0.
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_conv16(const __bf16* __restrict__ in,
+                                                const __bf16* __restrict__ w,
+                                                const __bf16* __restrict__ bias,
+                                                const __bf16* __restrict__ mask,
+                                                __bf16* __restrict__ out, Lat g, long B) {
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, grp = lane >> 4;
+  const int ci0 = 8 * (grp & 1);
+  bf16x8 wa[14];
+#pragma unroll
+  for (int st = 0; st < 14; ++st) {
+    const int tap = 2 * st + (grp >> 1);
+    if (tap < NTAP) {
+      wa[st] = *reinterpret_cast<const bf16x8*>(w + (row * NTAP + tap) * PC + ci0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) wa[st][k] = (__bf16)0.f;
+    }
+  }
+  float bco[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) bco[r] = FWD ? bf(bias[grp * 4 + r]) : 0.f;
+  const long ntiles = B * g.V / 16;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nwave = (long)gridDim.x * (blockDim.x >> 6);
+  for (long tile = wave; tile < ntiles; tile += nwave) {
+    const long i0 = tile * 16;
+    const long c = i0 / g.V;
+    const long s = i0 - c * g.V + row;
+    int x, y, z;
+    site_xyz(g, s, x, y, z);
+    const __bf16* inc = in + c * g.V * PC;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 14; ++st) {
+      const int tap = 2 * st + (grp >> 1);
+      bf16x8 b;
+      if (tap < NTAP) {
+        const long q = nbr_site(g, x, y, z, tap);
+        b = *reinterpret_cast<const bf16x8*>(inc + q * PC + ci0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[st], b, acc, 0, 0, 0);
+    }
+    // D: lane holds output channels 4 grp .. 4 grp + 3 of site s
+    const long o = (c * g.V + s) * PC + grp * 4;
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+    bf16x4 res;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float v = acc[r];
+      if constexpr (FWD) {
+        v = fmaxf(v + bco[r], 0.f);
+      } else {
+        v = bf(mask[o + r]) > 0.f ? v : 0.f;
+      }
+      res[r] = (__bf16)v;
+    }
+    *reinterpret_cast<bf16x4*>(out + o) = res;
+  }
+}
+
+// ------------------------------------------------------------------ layer 3 (16 -> 1)
+__device__ __forceinline__ float conv3_logit(const __bf16* __restrict__ A2c, const Lat& g,
+                                             int x, int y, int z, const float* w3, float b3) {
+  float acc = b3;
+  for (int tap = 0; tap < NTAP; ++tap) {
+    const long q = nbr_site(g, x, y, z, tap);
+    const bf16x8* p = reinterpret_cast<const bf16x8*>(A2c + q * PC);
+    const bf16x8 a = p[0], b = p[1];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      acc = fmaf(w3[tap * PC + k], bf(a[k]), acc);
+      acc = fmaf(w3[tap * PC + 8 + k], bf(b[k]), acc);
+    }
+  }
+  return acc;
+}
+
+// sum over a wavefront in FP64
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// MODE 0: logits out.  MODE 1: sample phi(t) from Philox(seed; site / 4, t, config, stream)
+// word site % 4: phi = +1 iff the word is below T = ceil(sigmoid(logit) 2^32) (clamped to
+// [1, 2^32 - 1]), so q(+1) = T / 2^32 exactly; the log ratio log(q_uniform / q) of every site
+// is summed in FP64 into logr[c].  MODE 2: training, dlogit = scale w[c] (q(+1) - phi01) for
+// the target field and the weighted negative log-likelihood summed into loss[0].
+// Sites of a wavefront are consecutive and of one configuration (V % 64 == 0).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ A2,
+                                               const __bf16* __restrict__ wb, Lat g, long B,
+                                               float* __restrict__ logits, uint64_t seed,
+                                               uint32_t config0, uint32_t t, uint32_t stream,
+                                               uint32_t* __restrict__ words,
+                                               double* __restrict__ logr,
+                                               const uint32_t* __restrict__ target,
+                                               const float* __restrict__ w, float scale,
+                                               double* __restrict__ loss) {
+  __shared__ float w3[NTAP * PC];
+  for (int i = threadIdx.x; i < NTAP * PC; i += blockDim.x) w3[i] = bf(wb[P_W3 + i]);
+  __syncthreads();
+  const float b3 = bf(wb[P_B3]);
+  const long n = B * g.V;
+  const int nw = (int)(g.V / 32);
+  // every thread of a wavefront runs the same number of iterations (n % 64 == 0)
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / g.V, s = i - c * g.V;
+    int x, y, z;
+    site_xyz(g, s, x, y, z);
+    const float lg = conv3_logit(A2 + c * g.V * PC, g, x, y, z, w3, b3);
+    if constexpr (MODE == 0) {
+      logits[i] = lg;
+    } else if constexpr (MODE == 1) {
+      uint32_t ctr[4] = {(uint32_t)(s >> 2), t, config0 + (uint32_t)c, stream};
+      philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const uint32_t r = ctr[s & 3];
+      const double p = 1.0 / (1.0 + exp(-(double)lg));
+      double th = ceil(p * 4294967296.0);
+      th = th < 1.0 ? 1.0 : (th > 4294967295.0 ? 4294967295.0 : th);
+      const bool up = (double)r < th;
+      // log(q_uniform / q(phi)) = log(2^31 / T) or log(2^31 / (2^32 - T))
+      const double d = (th - 2147483648.0) / 2147483648.0;
+      const double lr = up ? -log1p(d) : -log1p(-d);
+      const uint64_t m = __ballot(up);
+      const int lane = threadIdx.x & 63;
+      if (lane == 0) words[c * nw + (s >> 5)] = (uint32_t)m;
+      if (lane == 32) words[c * nw + (s >> 5)] = (uint32_t)(m >> 32);
+      const double tot = wave_sum(lr);
+      if (lane == 0) atomicAdd(logr + c, tot);
+    } else {
+      const uint32_t wd = target[c * nw + (s >> 5)];
+      const float y1 = ((wd >> (s & 31)) & 1u) ? 1.f : 0.f;
+      const float q = 1.f / (1.f + expf(-lg));
+      logits[i] = scale * w[c] * (q - y1);
+      // -log q(target) = softplus(-lg) for phi = +1, softplus(lg) for phi = -1
+      const float a = y1 > 0.f ? -lg : lg;
+      const double nll = (double)(fmaxf(a, 0.f) + log1pf(expf(-fabsf(a))));
+      const double tot = wave_sum((double)w[c] * nll);
+      if ((threadIdx.x & 63) == 0) atomicAdd(loss, tot);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ backward
+// dZ2[c][s][ci] = (A2 > 0) * sum_tap W3[tap][ci] dlogit[c][s - d(tap)]
+__global__ __launch_bounds__(256) void k_conv3_bwd(const float* __restrict__ dlogit,
+                                                   const __bf16* __restrict__ A2,
+                                                   const __bf16* __restrict__ wb,
+                                                   __bf16* __restrict__ dZ2, Lat g, long B) {
+  __shared__ float w3[NTAP * PC];
+  for (int i = threadIdx.x; i < NTAP * PC; i += blockDim.x) w3[i] = bf(wb[P_W3 + i]);
+  __syncthreads();
+  const long n = B * g.V;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / g.V, s = i - c * g.V;
+    int x, y, z;
+    site_xyz(g, s, x, y, z);
+    float acc[PC];
+#pragma unroll
+    for (int k = 0; k < PC; ++k) acc[k] = 0.f;
+    for (int tap = 0; tap < NTAP; ++tap) {
+      const long q = nbr_site(g, x, y, z, NTAP - 1 - tap);  // s - d(tap)
+      const float d = dlogit[c * g.V + q];
+#pragma unroll
+      for (int k = 0; k < PC; ++k) acc[k] = fmaf(w3[tap * PC + k], d, acc[k]);
+    }
+    const bf16x8* a = reinterpret_cast<const bf16x8*>(A2 + i * PC);
+    const bf16x8 a0 = a[0], a1 = a[1];
+    bf16x8 o0, o1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      o0[k] = (__bf16)(bf(a0[k]) > 0.f ? acc[k] : 0.f);
+      o1[k] = (__bf16)(bf(a1[k]) > 0.f ? acc[8 + k] : 0.f);
+    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(dZ2 + i * PC);
+    dst[0] = o0;
+    dst[1] = o1;
+  }
+}
+
+// Weight gradients: every workgroup sums its share of the (config, site) range into one row
+// of `partial`, a second kernel adds the rows.  Sites are staged in chunks of 64 together
+// with their 27 neighbour indices (LDS), so the threads, one per gradient entry, only load
+// the operand values.
+constexpr int WG_CHUNK = 64;
+
+// LAYER 3: entries (tap, ci) = 432 and the bias: sum dlogit[s] A2[s + d(tap)][ci]
+// LAYER 2: entries (co, tap, ci) = 6912 handled as 256 threads (co, ci) x 27 taps, bias by
+//          the threads with ci == 0: sum dZ2[s][co] A1[s + d(tap)][ci]
+// LAYER 1: entries (co, tap) = 432 for input 0, plus per co the u-weighted sum (input 1,
+//          the same for all taps) and the bias: sum dZ1[s][co] phi(s + d(tap)), u dZ1, dZ1
+template <int LAYER>
+__global__ __launch_bounds__(LAYER == 2 ? 256 : 448) void k_wgrad(
+    const float* __restrict__ dlogit, const __bf16* __restrict__ dZ, const __bf16* __restrict__ act,
+    const uint32_t* __restrict__ prev, int nw, const float* __restrict__ u, Lat g, long B,
+    long per_block, float* __restrict__ partial, int row_len) {
+  __shared__ long nb[WG_CHUNK][NTAP];
+  __shared__ float dv[WG_CHUNK][PC];
+  __shared__ long cfg_of[WG_CHUNK];
+  const int tid = threadIdx.x;
+  const long n = B * g.V;
+  const long i0 = (long)blockIdx.x * per_block;
+  const long i1 = i0 + per_block < n ? i0 + per_block : n;
+  float acc[NTAP];
+#pragma unroll
+  for (int k = 0; k < NTAP; ++k) acc[k] = 0.f;
+  float accb = 0.f, accu = 0.f;
+  for (long base = i0; base < i1; base += WG_CHUNK) {
+    const int cnt = (int)(i1 - base < WG_CHUNK ? i1 - base : WG_CHUNK);
+    __syncthreads();
+    for (int j = tid; j < cnt * NTAP; j += blockDim.x) {
+      const int k = j / NTAP, tap = j - k * NTAP;
+      const long i = base + k, c = i / g.V, s = i - c * g.V;
+      int x, y, z;
+      site_xyz(g, s, x, y, z);
+      nb[k][tap] = c * g.V + nbr_site(g, x, y, z, tap);
+      if (tap == 0) cfg_of[k] = c;
+    }
+    for (int j = tid; j < cnt * PC; j += blockDim.x) {
+      const int k = j / PC, ch = j - k * PC;
+      const long i = base + k;
+      if constexpr (LAYER == 3) {
+        dv[k][ch] = ch == 0 ? dlogit[i] : 0.f;
+      } else {
+        dv[k][ch] = bf(dZ[i * PC + ch]);
+      }
+    }
+    __syncthreads();
+    if constexpr (LAYER == 3) {
+      if (tid < NTAP * PC) {
+        const int tap = tid / PC, ci = tid - (tid / PC) * PC;
+        for (int k = 0; k < cnt; ++k) acc[0] = fmaf(dv[k][0], bf(act[nb[k][tap] * PC + ci]), acc[0]);
+      } else if (tid == NTAP * PC) {
+        for (int k = 0; k < cnt; ++k) accb += dv[k][0];
+      }
+    } else if constexpr (LAYER == 2) {
+      const int co = tid / PC, ci = tid - (tid / PC) * PC;
+      for (int k = 0; k < cnt; ++k) {
+        const float d = dv[k][co];
+#pragma unroll
+        for (int tap = 0; tap < NTAP; ++tap)
+          acc[tap] = fmaf(d, bf(act[nb[k][tap] * PC + ci]), acc[tap]);
+        accb += d;
+      }
+    } else {
+      if (tid < PC * NTAP) {
+        const int co = tid / NTAP, tap = tid - (tid / NTAP) * NTAP;
+        if (prev) {
+          for (int k = 0; k < cnt; ++k) {
+            const long q = nb[k][tap], c = cfg_of[k], sq = q - c * g.V;
+            const float f = ((prev[c * nw + (sq >> 5)] >> (sq & 31)) & 1u) ? 1.f : -1.f;
+            acc[0] = fmaf(dv[k][co], f, acc[0]);
+          }
+        }
+      } else if (tid < PC * NTAP + PC) {
+        const int co = tid - PC * NTAP;
+        for (int k = 0; k < cnt; ++k) {
+          accb += dv[k][co];
+          accu = fmaf(dv[k][co], u[cfg_of[k]], accu);
+        }
+      }
+    }
+  }
+  float* out = partial + (long)blockIdx.x * row_len;
+  if constexpr (LAYER == 3) {
+    if (tid < NTAP * PC) out[tid] = acc[0];
+    else if (tid == NTAP * PC) out[NTAP * PC] = accb;
+  } else if constexpr (LAYER == 2) {
+    const int co = tid / PC, ci = tid - (tid / PC) * PC;
+#pragma unroll
+    for (int tap = 0; tap < NTAP; ++tap) out[(co * NTAP + tap) * PC + ci] = acc[tap];
+    if (ci == 0) out[NTAP * PC * PC + co] = accb;
+  } else {
+    if (tid < PC * NTAP) {
+      const int co = tid / NTAP, tap = tid - (tid / NTAP) * NTAP;
+      out[(co * NTAP + tap) * 2] = acc[0];
+    } else if (tid < PC * NTAP + PC) {
+      const int co = tid - PC * NTAP;
+      for (int tap = 0; tap < NTAP; ++tap) out[(co * NTAP + tap) * 2 + 1] = accu;
+      out[PC * NTAP * 2 + co] = accb;
+    }
+  }
+}
+
+// grad[off + j] = sum over rows of partial[row][j]
+__global__ void k_sum_rows(const float* partial, int rows, int row_len, float* grad) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < row_len; j += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < rows; ++r) s += (double)partial[(long)r * row_len + j];
+    grad[j] = (float)s;
+  }
+}
+
+// u[c] = (2 logabs[c * stride] - mu) / sd
+__global__ void k_norm_logw(const double* logabs, int stride, int B, double mu, double sd,
+                            float* u) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < B; c += gridDim.x * blockDim.x)
+    u[c] = (float)((2.0 * logabs[(long)c * stride] - mu) / sd);
+}
+
+// field words of (config[i], t[i]) pairs, the plain sampler's Philox fields
+__global__ void k_z2_bits_list(uint64_t seed, const uint32_t* cfg, const uint32_t* ts, int n,
+                               uint32_t stream, int nw, uint32_t* words) {
+  const int ng = (nw + 3) / 4;
+  const long tot = (long)n * ng;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i / ng), gq = (int)(i - (long)k * ng);
+    uint32_t ctr[4] = {(uint32_t)gq, ts[k], cfg[k], stream};
+    philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t* out = words + (long)k * nw + 4 * gq;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (4 * gq + q < nw) out[q] = ctr[q];
+  }
+}
+
+}  // namespace nrc
+
+// ------------------------------------------------------------------ C ABI
+using namespace nrc;
+
+struct nrc_proposal {
+  nrc_context* ctx = nullptr;
+  Lat g{};
+  long max_batch = 0;
+  float* p = nullptr;  // master parameters
+  float* grad = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  __bf16* wb = nullptr;
+  __bf16* w2t = nullptr;
+  __bf16* A1 = nullptr;  // [max_batch][V][16]
+  __bf16* A2 = nullptr;
+  __bf16* dZ1 = nullptr;  // training only, allocated on the first training call
+  __bf16* dZ2 = nullptr;
+  float* logits = nullptr;  // [max_batch][V]
+  float* partial = nullptr;
+  int partial_rows = 0;
+  double* loss = nullptr;
+  int step = 0;
+};
+
+#define PR_HIP(x)                              \
+  do {                                         \
+    if ((x) != hipSuccess) return NRC_ERR_HIP; \
+  } while (0)
+
+static int blocks_for(long n, int threads = 256, long cap = 8192) {
+  long b = (n + threads - 1) / threads;
+  if (b > cap) b = cap;
+  return (int)(b < 1 ? 1 : b);
+}
+
+static void pr_free(nrc_proposal* p) {
+  void* bufs[] = {p->p, p->grad, p->m, p->v, p->wb, p->w2t, p->A1, p->A2, p->dZ1, p->dZ2,
+                  p->logits, p->partial, p->loss};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  delete p;
+}
+
+extern "C" int nrc_proposal_create(nrc_context* ctx, int L, int max_batch, nrc_proposal** out) {
+  if (!ctx || !out || max_batch < 1 || L < 4 || L > 64) return NRC_ERR_ARG;
+  const long V = (long)L * L * L;
+  if (V % 64 != 0) return NRC_ERR_UNSUPPORTED;
+  hipSetDevice(ctx_device(ctx));
+  nrc_proposal* p = new nrc_proposal();
+  p->ctx = ctx;
+  p->g = Lat{L, V};
+  p->max_batch = max_batch;
+  const size_t act = (size_t)max_batch * V * PC * sizeof(__bf16);
+  bool ok = hipMalloc(&p->p, P_N * 4) == hipSuccess && hipMalloc(&p->grad, P_N * 4) == hipSuccess &&
+            hipMalloc(&p->m, P_N * 4) == hipSuccess && hipMalloc(&p->v, P_N * 4) == hipSuccess &&
+            hipMalloc(&p->wb, P_N * 2) == hipSuccess &&
+            hipMalloc(&p->w2t, PC * NTAP * PC * 2) == hipSuccess &&
+            hipMalloc(&p->A1, act) == hipSuccess && hipMalloc(&p->A2, act) == hipSuccess &&
+            hipMalloc(&p->logits, (size_t)max_batch * V * 4) == hipSuccess &&
+            hipMalloc(&p->loss, sizeof(double)) == hipSuccess;
+  if (!ok) {
+    pr_free(p);
+    return NRC_ERR_NOMEM;
+  }
+  *out = p;
+  return NRC_OK;
+}
+
+extern "C" int nrc_proposal_destroy(nrc_proposal* p) {
+  if (!p) return NRC_ERR_ARG;
+  hipStreamSynchronize(ctx_stream(p->ctx));
+  pr_free(p);
+  return NRC_OK;
+}
+
+extern "C" int nrc_proposal_param_count(void) { return P_N; }
+
+extern "C" int nrc_proposal_set_params(nrc_proposal* p, const float* params) {
+  if (!p || !params) return NRC_ERR_ARG;
+  hipStream_t s = ctx_stream(p->ctx);
+  PR_HIP(hipMemcpyAsync(p->p, params, P_N * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fill_zero_f, dim3(blocks_for(P_N)), dim3(256), 0, s, p->m, P_N);
+  hipLaunchKernelGGL(k_fill_zero_f, dim3(blocks_for(P_N)), dim3(256), 0, s, p->v, P_N);
+  hipLaunchKernelGGL(k_refresh, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->wb, p->w2t);
+  PR_HIP(hipGetLastError());
+  p->step = 0;
+  PR_HIP(hipStreamSynchronize(s));
+  return NRC_OK;
+}
+
+extern "C" int nrc_proposal_get_params(nrc_proposal* p, float* params) {
+  if (!p || !params) return NRC_ERR_ARG;
+  hipStream_t s = ctx_stream(p->ctx);
+  PR_HIP(hipMemcpyAsync(params, p->p, P_N * 4, hipMemcpyDeviceToHost, s));
+  PR_HIP(hipStreamSynchronize(s));
+  return NRC_OK;
+}
+
+static int pr_forward_hidden(nrc_proposal* p, const uint32_t* prev, const float* u, long B) {
+  hipStream_t s = ctx_stream(p->ctx);
+  const long n = B * p->g.V;
+  const int nw = (int)(p->g.V / 32);
+  hipLaunchKernelGGL(k_conv1, dim3(blocks_for(n)), dim3(256), 0, s, prev, nw, u, p->wb, p->A1,
+                     p->g, B);
+  const long tiles = n / 16;
+  hipLaunchKernelGGL((k_conv16<true>), dim3(blocks_for(tiles * 64, 256, 4096)), dim3(256), 0, s,
+                     p->A1, p->wb + P_W2, p->wb + P_B2, (const __bf16*)nullptr, p->A2, p->g, B);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+// logits[B][V] (device) of the fields prev[B][nw] (device; null: no previous slice) and the
+// normalised log-weights u[B] (device)
+extern "C" int nrc_proposal_forward(nrc_proposal* p, const uint32_t* prev_dev, const float* u_dev,
+                                    int B, float* logits_dev) {
+  if (!p || !u_dev || !logits_dev || B < 1 || B > p->max_batch) return NRC_ERR_ARG;
+  if (pr_forward_hidden(p, prev_dev, u_dev, B) != NRC_OK) return NRC_ERR_HIP;
+  hipStream_t s = ctx_stream(p->ctx);
+  hipLaunchKernelGGL((k_conv3<0>), dim3(blocks_for((long)B * p->g.V)), dim3(256), 0, s, p->A2,
+                     p->wb, p->g, (long)B, logits_dev, 0ull, 0u, 0u, 0u, (uint32_t*)nullptr,
+                     (double*)nullptr, (const uint32_t*)nullptr, (const float*)nullptr, 0.f,
+                     (double*)nullptr);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+// phi(t) for configurations config0 .. config0 + B - 1 into words_dev[B][nw], and
+// logr_dev[c] += sum over sites of log(q_uniform / q(phi)) (FP64, device)
+extern "C" int nrc_proposal_sample(nrc_proposal* p, const uint32_t* prev_dev, const float* u_dev,
+                                   int B, uint64_t seed, uint32_t config0, uint32_t t,
+                                   uint32_t stream, uint32_t* words_dev, double* logr_dev) {
+  if (!p || !u_dev || !words_dev || !logr_dev || B < 1 || B > p->max_batch) return NRC_ERR_ARG;
+  if (pr_forward_hidden(p, prev_dev, u_dev, B) != NRC_OK) return NRC_ERR_HIP;
+  hipStream_t s = ctx_stream(p->ctx);
+  hipLaunchKernelGGL((k_conv3<1>), dim3(blocks_for((long)B * p->g.V)), dim3(256), 0, s, p->A2,
+                     p->wb, p->g, (long)B, (float*)nullptr, seed, config0, t, stream, words_dev,
+                     logr_dev, (const uint32_t*)nullptr, (const float*)nullptr, 0.f,
+                     (double*)nullptr);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+// Gradient of the weighted negative log-likelihood
+//   loss = sum_b w[b] sum_s -log q(target_b(s) | prev_b, u_b) / (V sum_b w[b])
+// into grad_out (host, P_N floats; may be null) and loss_out; with lr > 0 an Adam step
+// (beta 0.9, 0.999, eps 1e-8) follows.  All inputs are device arrays of B samples.
+extern "C" int nrc_proposal_train_step(nrc_proposal* p, const uint32_t* prev_dev,
+                                       const uint32_t* target_dev, const float* u_dev,
+                                       const float* w_dev, int B, double wsum, float lr,
+                                       float* grad_out, double* loss_out) {
+  if (!p || !target_dev || !u_dev || !w_dev || B < 1 || B > p->max_batch || !(wsum > 0.0))
+    return NRC_ERR_ARG;
+  hipStream_t s = ctx_stream(p->ctx);
+  const long n = (long)B * p->g.V;
+  const int nw = (int)(p->g.V / 32);
+  const size_t act = (size_t)p->max_batch * p->g.V * PC * sizeof(__bf16);
+  if (!p->dZ1) {
+    if (hipMalloc(&p->dZ1, act) != hipSuccess || hipMalloc(&p->dZ2, act) != hipSuccess)
+      return NRC_ERR_NOMEM;
+  }
+  const int rows = 512;
+  if (!p->partial) {
+    if (hipMalloc(&p->partial, (size_t)rows * (NTAP * PC * PC + PC) * 4) != hipSuccess)
+      return NRC_ERR_NOMEM;
+    p->partial_rows = rows;
+  }
+  if (pr_forward_hidden(p, prev_dev, u_dev, B) != NRC_OK) return NRC_ERR_HIP;
+  hipLaunchKernelGGL(k_fill_zero_d, dim3(1), dim3(64), 0, s, p->loss, 1);
+  const float scale = (float)(1.0 / (wsum * (double)p->g.V));
+  hipLaunchKernelGGL((k_conv3<2>), dim3(blocks_for(n)), dim3(256), 0, s, p->A2, p->wb, p->g,
+                     (long)B, p->logits, 0ull, 0u, 0u, 0u, (uint32_t*)nullptr, (double*)nullptr,
+                     target_dev, w_dev, scale, p->loss);
+  hipLaunchKernelGGL(k_conv3_bwd, dim3(blocks_for(n)), dim3(256), 0, s, p->logits, p->A2, p->wb,
+                     p->dZ2, p->g, (long)B);
+  hipLaunchKernelGGL((k_conv16<false>), dim3(blocks_for(n / 16 * 64, 256, 4096)), dim3(256), 0,
+                     s, p->dZ2, p->w2t, (const __bf16*)nullptr, p->A1, p->dZ1, p->g, (long)B);
+  const long per_block = (n + rows - 1) / rows;
+  // layer 3
+  hipLaunchKernelGGL((k_wgrad<3>), dim3(rows), dim3(448), 0, s, p->logits, (const __bf16*)nullptr,
+                     p->A2, (const uint32_t*)nullptr, nw, (const float*)nullptr, p->g, (long)B,
+                     per_block, p->partial, NTAP * PC + 1);
+  hipLaunchKernelGGL(k_sum_rows, dim3(blocks_for(NTAP * PC + 1)), dim3(256), 0, s, p->partial,
+                     rows, NTAP * PC + 1, p->grad + P_W3);
+  // layer 2
+  hipLaunchKernelGGL((k_wgrad<2>), dim3(rows), dim3(256), 0, s, (const float*)nullptr, p->dZ2,
+                     p->A1, (const uint32_t*)nullptr, nw, (const float*)nullptr, p->g, (long)B,
+                     per_block, p->partial, NTAP * PC * PC + PC);
+  hipLaunchKernelGGL(k_sum_rows, dim3(blocks_for(NTAP * PC * PC + PC)), dim3(256), 0, s,
+                     p->partial, rows, NTAP * PC * PC + PC, p->grad + P_W2);
+  // layer 1
+  hipLaunchKernelGGL((k_wgrad<1>), dim3(rows), dim3(448), 0, s, (const float*)nullptr, p->dZ1,
+                     (const __bf16*)nullptr, prev_dev, nw, u_dev, p->g, (long)B, per_block,
+                     p->partial, PC * NTAP * 2 + PC);
+  hipLaunchKernelGGL(k_sum_rows, dim3(blocks_for(PC * NTAP * 2 + PC)), dim3(256), 0, s,
+                     p->partial, rows, PC * NTAP * 2 + PC, p->grad + P_W1);
+  PR_HIP(hipGetLastError());
+  if (lr > 0.f) {
+    ++p->step;
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->grad, p->m, p->v,
+                       P_N, lr, 0.9f, 0.999f, 1e-8f, p->step);
+    hipLaunchKernelGGL(k_refresh, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->wb, p->w2t);
+    PR_HIP(hipGetLastError());
+  }
+  if (grad_out) PR_HIP(hipMemcpyAsync(grad_out, p->grad, P_N * 4, hipMemcpyDeviceToHost, s));
+  double l = 0.0;
+  PR_HIP(hipMemcpyAsync(&l, p->loss, sizeof(double), hipMemcpyDeviceToHost, s));
+  PR_HIP(hipStreamSynchronize(s));
+  if (loss_out) *loss_out = l * (double)scale;
+  return NRC_OK;
+}
+
+extern "C" int nrc_norm_logw(nrc_context* ctx, const double* logabs_dev, int stride, int B,
+                             double mu, double sd, float* u_dev) {
+  if (!ctx || !logabs_dev || !u_dev || B < 1 || stride < 1 || !(sd > 0.0)) return NRC_ERR_ARG;
+  hipLaunchKernelGGL(k_norm_logw, dim3(blocks_for(B)), dim3(256), 0, ctx_stream(ctx), logabs_dev,
+                     stride, B, mu, sd, u_dev);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+extern "C" int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t* configs_dev,
+                                const uint32_t* ts_dev, int n, uint32_t stream, int vol,
+                                uint32_t* words_dev) {
+  if (!ctx || !configs_dev || !ts_dev || !words_dev || n < 0 || vol < 1) return NRC_ERR_ARG;
+  if (n == 0) return NRC_OK;
+  const int nw = (vol + 31) / 32;
+  hipLaunchKernelGGL(k_z2_bits_list, dim3(blocks_for((long)n * ((nw + 3) / 4))), dim3(256), 0,
+                     ctx_stream(ctx), seed, configs_dev, ts_dev, n, stream, nw, words_dev);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}

@@ -1,0 +1,225 @@
+"""The learned auxiliary-field proposal (nrc_proposal_* in libnrcps_hip.so) and the sampler and
+trainer built on it.
+
+The reference draws every field uniformly (src/alg/lattice.C:81-86).  Here a small 3D
+convolutional network proposes phi(t) given phi(t-1) and the configuration's normalised
+log-weight at t; every site's exact proposal probability enters the importance ratio
+r_t = prod over slices < t and sites of (1/2) / q(phi), so that mean(det(t)^2 r_t) is an
+unbiased estimate of the uniform-field correlator C(t) = <det(t)^2>.  The network is trained
+by weighted maximum likelihood on fields of the uniform sampler, weighted by det^2.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import Context, DeviceArray, Ensemble, NrcError, _check, _ptr
+
+STREAM_UNIFORM = 0  # Philox stream of the uniform sampler (nrc_ensemble_run)
+STREAM_PROPOSAL = 1  # Philox stream of the proposal's site draws
+
+N_HIDDEN, N_TAP = 16, 27
+
+
+def param_layout():
+    """(name, offset, shape) of the parameter vector."""
+    out, off = [], 0
+    for name, shape in (("W1", (16, 27, 2)), ("b1", (16,)), ("W2", (16, 27, 16)), ("b2", (16,)),
+                        ("W3", (27, 16)), ("b3", (1,))):
+        n = int(np.prod(shape))
+        out.append((name, off, shape))
+        off += n
+    return out
+
+
+def unpack(params: np.ndarray) -> dict:
+    return {name: params[off:off + int(np.prod(shape))].reshape(shape)
+            for name, off, shape in param_layout()}
+
+
+def init_params(rng: np.random.Generator, out_scale: float = 1e-3) -> np.ndarray:
+    """He-scaled hidden layers; the output layer starts small, so q starts near uniform."""
+    p = np.zeros(8241, np.float32)
+    v = unpack(p)
+    v["W1"][:] = rng.standard_normal(v["W1"].shape) * np.sqrt(2.0 / (27 * 2))
+    v["W2"][:] = rng.standard_normal(v["W2"].shape) * np.sqrt(2.0 / (27 * 16))
+    v["W3"][:] = rng.standard_normal(v["W3"].shape) * out_scale
+    v["b1"][:] = 0.1
+    v["b2"][:] = 0.1
+    return p
+
+
+class Proposal:
+    """nrc_proposal_*: parameters on the device, forward, sampling and training steps."""
+
+    def __init__(self, ctx: Context, L: int, max_batch: int, params: np.ndarray | None = None):
+        self.ctx, self.lib, self.L, self.max_batch = ctx, ctx.lib, L, max_batch
+        self.V, self.nw = L ** 3, L ** 3 // 32
+        self.n_params = self.lib.nrc_proposal_param_count()
+        h = C.c_void_p()
+        _check(self.lib.nrc_proposal_create(ctx.h, L, max_batch, C.byref(h)), "proposal_create")
+        self.h = h
+        self.set_params(init_params(np.random.default_rng(0)) if params is None else params)
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.n_params
+        _check(self.lib.nrc_proposal_set_params(self.h, _ptr(p)), "proposal_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(self.lib.nrc_proposal_get_params(self.h, _ptr(p)), "proposal_get_params")
+        return p
+
+    def forward(self, prev: DeviceArray | None, u: DeviceArray, B: int, logits: DeviceArray):
+        _check(self.lib.nrc_proposal_forward(self.h, prev.ptr if prev else None, u.ptr, B,
+                                             logits.ptr), "proposal_forward")
+
+    def sample(self, prev, u, B, seed, config0, t, words: DeviceArray, logr: DeviceArray,
+               stream=STREAM_PROPOSAL):
+        _check(self.lib.nrc_proposal_sample(self.h, prev.ptr if prev else None, u.ptr, B, seed,
+                                            config0, t, stream, words.ptr, logr.ptr),
+               "proposal_sample")
+
+    def train_step(self, prev, target, u, w, B, wsum, lr, want_grad=False):
+        g = np.zeros(self.n_params, np.float32) if want_grad else None
+        loss = C.c_double()
+        _check(self.lib.nrc_proposal_train_step(self.h, prev.ptr if prev else None, target.ptr,
+                                                u.ptr, w.ptr, B, wsum, lr,
+                                                _ptr(g) if want_grad else None, C.byref(loss)),
+               "proposal_train_step")
+        return loss.value, g
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.lib.nrc_proposal_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def norm_logw(ctx: Context, logabs_dev: int, stride: int, B: int, mu: float, sd: float,
+              u: DeviceArray):
+    _check(ctx.lib.nrc_norm_logw(ctx.h, C.c_void_p(logabs_dev), stride, B, mu, sd, u.ptr),
+           "norm_logw")
+
+
+def z2_bits_list(ctx: Context, seed: int, configs: DeviceArray, ts: DeviceArray, n: int, vol: int,
+                 words: DeviceArray, stream=STREAM_UNIFORM):
+    _check(ctx.lib.nrc_z2_bits_list(ctx.h, seed, configs.ptr, ts.ptr, n, stream, vol, words.ptr),
+           "z2_bits_list")
+
+
+def logw_stats(la: np.ndarray, floor: float = 1e-3):
+    """Per-slice mean and spread of 2 log|det| over configurations (la [T][C] or [T][C][1])."""
+    x = 2.0 * (la[:, :, 0] if la.ndim == 3 else la)
+    return x.mean(axis=1), np.maximum(x.std(axis=1), floor)
+
+
+def det2_weights(la: np.ndarray, ts) -> np.ndarray:
+    """Training weights of the uniform sample: the mean over the slices ts of det(t)^2 /
+    sum_c det(t)^2."""
+    x = la[:, :, 0] if la.ndim == 3 else la
+    w = np.zeros(x.shape[1])
+    for t in ts:
+        e = np.exp(2.0 * (x[t] - x[t].max()))
+        w += e / e.sum()
+    return w / len(ts)
+
+
+class LearnedSampler:
+    """Configurations sampled slice by slice from the proposal: measure at t, normalise the
+    log-weight, draw phi(t), evolve slice t (nrc_ensemble_begin / measure / evolve)."""
+
+    def __init__(self, ctx: Context, prop: Proposal, ens: Ensemble, mu: np.ndarray,
+                 sd: np.ndarray):
+        assert ens.C <= prop.max_batch and ens.L == prop.L
+        self.ctx, self.prop, self.ens, self.mu, self.sd = ctx, prop, ens, mu, sd
+        B = ens.C
+        self.words = [ctx.array((B, prop.nw), np.uint32), ctx.array((B, prop.nw), np.uint32)]
+        self.u = ctx.array((B,), np.float32)
+        self.logr = ctx.array((B,), np.float64)
+
+    def run(self, seed: int, config0: int, T: int):
+        """Returns log|det| [T][C][m], phase [T][C][m] and log r_t [T][C] (the ratio of the
+        slices before t)."""
+        e, B = self.ens, self.ens.C
+        e.begin(T)
+        la_dev, _ = e.results_dev()
+        _check(self.ctx.lib.nrc_memset(self.ctx.h, self.logr.ptr, 0, self.logr.nbytes), "memset")
+        logr_hist = self.ctx.array((T, B), np.float64)
+        prev = None
+        for t in range(T):
+            e.measure(t)
+            _check(self.ctx.lib.nrc_memcpy_d2d(self.ctx.h, C.c_void_p(logr_hist.ptr.value + t * B * 8),
+                                               self.logr.ptr, B * 8), "d2d")
+            norm_logw(self.ctx, la_dev + t * B * e.nm * 8, e.nm, B, float(self.mu[t]),
+                      float(self.sd[t]), self.u)
+            cur = self.words[t & 1]
+            self.prop.sample(prev, self.u, B, seed, config0, t, cur, self.logr)
+            e.evolve(t, cur)
+            prev = cur
+        la, ph, _ = e.results(T)
+        return la, ph, logr_hist.download()
+
+
+def train(ctx: Context, prop: Proposal, la: np.ndarray, seed: int, config0: int, steps: int,
+          batch: int, lr: float, ts_weight, rng: np.random.Generator, log_every: int = 0):
+    """Weighted maximum likelihood on the uniform sampler's fields: configurations c are drawn
+    in proportion to their det^2 weights (det2_weights over ts_weight), slices t uniformly;
+    the sample is (phi_c(t-1), u_c(t)) -> phi_c(t), fields regenerated from Philox(seed).
+    Returns the loss history."""
+    T, Cn = la.shape[0], la.shape[1]
+    mu, sd = logw_stats(la)
+    w = det2_weights(la, ts_weight)
+    x2 = 2.0 * (la[:, :, 0] if la.ndim == 3 else la)
+    V = prop.V
+    cfg_d = ctx.array((batch,), np.uint32)
+    t_d = ctx.array((batch,), np.uint32)
+    tm1_d = ctx.array((batch,), np.uint32)
+    prev_d = ctx.array((batch, prop.nw), np.uint32)
+    tgt_d = ctx.array((batch, prop.nw), np.uint32)
+    u_d = ctx.array((batch,), np.float32)
+    w_d = ctx.array((batch,), np.float32).upload(np.ones(batch, np.float32))
+    losses = []
+    for step in range(steps):
+        c = rng.choice(Cn, size=batch, p=w)
+        first = step % T == 0  # a batch of first slices (no previous field) now and then
+        t = np.zeros(batch, np.int64) if first else rng.integers(1, T, size=batch)
+        u = ((x2[t, c] - mu[t]) / sd[t]).astype(np.float32)
+        cfg_d.upload((config0 + c).astype(np.uint32))
+        t_d.upload(t.astype(np.uint32))
+        z2_bits_list(ctx, seed, cfg_d, t_d, batch, V, tgt_d)
+        if not first:
+            tm1_d.upload((t - 1).astype(np.uint32))
+            z2_bits_list(ctx, seed, cfg_d, tm1_d, batch, V, prev_d)
+        u_d.upload(u)
+        loss, _ = prop.train_step(None if first else prev_d, tgt_d, u_d, w_d, batch,
+                                  float(batch), lr)
+        losses.append(loss)
+        if log_every and step % log_every == 0:
+            print(f"  train step {step}: loss {loss:.8f}", flush=True)
+    return np.array(losses), mu, sd
+
+
+def estimator_stats(la: np.ndarray, logr: np.ndarray | None, ts):
+    """For each t in ts: log C(t) estimate, relative standard error and the effective sample
+    size (sum x)^2 / sum x^2 of x = det(t)^2 r_t (r = 1 for the uniform sampler)."""
+    x = la[:, :, 0] if la.ndim == 3 else la
+    out = {}
+    for t in ts:
+        lx = 2.0 * x[t] + (logr[t] if logr is not None else 0.0)
+        m = lx.max()
+        v = np.exp(lx - m)
+        mean = v.mean()
+        n = v.size
+        rel_se = v.std(ddof=1) / np.sqrt(n) / mean
+        out[int(t)] = {"log_C": float(np.log(mean) + m), "rel_se": float(rel_se),
+                       "ess": float(v.sum() ** 2 / (v ** 2).sum()), "n": int(n)}
+    return out

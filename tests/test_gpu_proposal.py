@@ -1,0 +1,187 @@
+"""The learned field proposal through the C ABI (nrc_proposal_*, nrc_ensemble_begin/measure/
+evolve) against its FP64 restatement (oracle/convnet.py, bf16 rounding emulated where the
+kernels store bf16): logits, gradients, the exact sampling ratio, training, and the unbiased
+reweighted correlator against the uniform sampler."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import nrcps_amd as na
+
+    c = na.Context(0)
+    yield c
+    c.close()
+
+
+def _rand_params(seed, scale3=0.05):
+    from nrcps_amd import proposal as pr
+
+    p = pr.init_params(np.random.default_rng(seed), out_scale=scale3)
+    p[8240] = 0.02
+    return p
+
+
+def _inputs(ctx, orc, L, B, seed, t):
+    V = L ** 3
+    prev = np.stack([orc.z2_bits(seed, c, t, V) for c in range(B)])
+    tgt = np.stack([orc.z2_bits(seed, c, t + 1, V) for c in range(B)])
+    u = np.linspace(-1.5, 2.0, B).astype(np.float32)
+    return prev, tgt, u
+
+
+@pytest.mark.parametrize("first", [False, True])
+def test_forward_matches_oracle(ctx, orc, first):
+    from nrcps_amd import proposal as pr
+    from oracle import convnet as cn
+
+    L, B = 8, 3
+    print(f"[gpu] proposal forward L={L} B={B} first={first}", flush=True)
+    p = _rand_params(1)
+    prop = pr.Proposal(ctx, L, B, p)
+    prev, _, u = _inputs(ctx, orc, L, B, 4, 2)
+    out = ctx.array((B, L ** 3), np.float32)
+    prop.forward(None if first else ctx.to_device(prev), ctx.to_device(u), B, out)
+    got = out.download().reshape(B, L, L, L)
+    ref, _ = cn.forward_L(p, None if first else cn.bits_to_phi(prev, L), u, L)
+    err = np.abs(got - ref).max()
+    print(f"[gpu] logits max abs err {err:.2e} (scale {np.abs(ref).max():.2e})", flush=True)
+    assert err < 2e-3 * np.abs(ref).max() + 1e-6
+    prop.close()
+
+
+def test_gradient_matches_oracle(ctx, orc):
+    from nrcps_amd import proposal as pr
+    from oracle import convnet as cn
+
+    L, B = 8, 3
+    print(f"[gpu] proposal gradient L={L} B={B}", flush=True)
+    p = _rand_params(2, scale3=0.3)
+    prop = pr.Proposal(ctx, L, B, p)
+    prev, tgt, u = _inputs(ctx, orc, L, B, 9, 5)
+    w = np.array([1.0, 0.5, 2.0], np.float32)
+    loss, g = prop.train_step(ctx.to_device(prev), ctx.to_device(tgt), ctx.to_device(u),
+                              ctx.to_device(w), B, float(w.sum()), 0.0, want_grad=True)
+    rloss, rg, _ = cn.loss_and_grad(p, cn.bits_to_phi(prev, L), cn.bits_to_phi(tgt, L), u, w, L)
+    assert loss == pytest.approx(rloss, rel=1e-4)
+    for name, lo, hi in (("W1", 0, 864), ("b1", 864, 880), ("W2", 880, 7792), ("b2", 7792, 7808),
+                         ("W3", 7808, 8240), ("b3", 8240, 8241)):
+        e = np.abs(g[lo:hi] - rg[lo:hi]).max() / np.abs(rg[lo:hi]).max()
+        print(f"[gpu] grad {name}: max rel err {e:.2e}", flush=True)
+        assert e < 3e-2, name
+    # the parameters do not move with lr = 0
+    assert np.array_equal(prop.get_params(), p.astype(np.float32))
+    prop.close()
+
+
+def test_sampling_ratio_is_exact(ctx, orc):
+    from nrcps_amd import proposal as pr
+    from oracle import convnet as cn
+
+    L, B, t, seed, c0 = 8, 2, 3, 99, 40
+    V = L ** 3
+    p = _rand_params(3, scale3=0.5)
+    prop = pr.Proposal(ctx, L, B, p)
+    prev, _, u = _inputs(ctx, orc, L, B, 1, 0)
+    prev_d, u_d = ctx.to_device(prev), ctx.to_device(u)
+    logits = ctx.array((B, V), np.float32)
+    prop.forward(prev_d, u_d, B, logits)
+    lg = logits.download()
+    words = ctx.array((B, V // 32), np.uint32)
+    logr = ctx.to_device(np.full(B, 0.25))
+    prop.sample(prev_d, u_d, B, seed, c0, t, words, logr)
+    bits = cn.bits_to_phi(words.download(), L).reshape(B, V) > 0
+    for c in range(B):
+        draws = orc.z2_bits(seed, c0 + c, t, 32 * V, stream=pr.STREAM_PROPOSAL)
+        r, T = cn.sample_ratio(lg[c], bits[c])
+        assert np.array_equal(bits[c], draws.astype(np.float64) < T)
+        assert logr.download()[c] == pytest.approx(0.25 + r.sum(), rel=1e-12, abs=1e-12)
+    prop.close()
+
+
+def test_training_learns_a_bias(ctx, orc):
+    """Targets drawn with P(+1) = 0.7, independent of the inputs: the trained network's logits
+    approach log(0.7 / 0.3)."""
+    from nrcps_amd import proposal as pr
+
+    L, B = 8, 16
+    V = L ** 3
+    prop = pr.Proposal(ctx, L, B)
+    rng = np.random.default_rng(5)
+    prev_d = ctx.array((B, V // 32), np.uint32)
+    tgt_d = ctx.array((B, V // 32), np.uint32)
+    u_d = ctx.to_device(np.zeros(B, np.float32))
+    w_d = ctx.to_device(np.ones(B, np.float32))
+    losses = []
+    for step in range(150):
+        prev = rng.integers(0, 2 ** 32, (B, V // 32), dtype=np.uint64).astype(np.uint32)
+        bits = (rng.random((B, V)) < 0.7).reshape(B, V // 32, 32)
+        tgt = (bits.astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(-1).astype(np.uint32)
+        prev_d.upload(prev)
+        tgt_d.upload(tgt)
+        loss, _ = prop.train_step(prev_d, tgt_d, u_d, w_d, B, float(B), 3e-3)
+        losses.append(loss)
+    out = ctx.array((B, V), np.float32)
+    prop.forward(prev_d, u_d, B, out)
+    lg = out.download()
+    print(f"[gpu] trained logit mean {lg.mean():.3f} (target {np.log(7 / 3):.3f}); "
+          f"loss {losses[0]:.4f} -> {np.mean(losses[-10:]):.4f}", flush=True)
+    assert abs(lg.mean() - np.log(7 / 3)) < 0.1
+    assert np.mean(losses[-10:]) < losses[0]
+    prop.close()
+
+
+def test_stepping_reproduces_run(ctx, orc):
+    """nrc_ensemble_begin / measure / evolve with the uniform sampler's Philox words gives
+    exactly nrc_ensemble_run."""
+    import nrcps_amd as na
+
+    L, N, C, T, seed = 8, 7, 3, 6, 21
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    a = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    la, ph, _ = a.run(seed, 0, T)
+    b = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    b.begin(T)
+    words = ctx.array((C, L ** 3 // 32), np.uint32)
+    for t in range(T):
+        b.measure(t)
+        words.upload(np.stack([orc.z2_bits(seed, c, t, L ** 3) for c in range(C)]))
+        b.evolve(t, words)
+    lb, pb, _ = b.results(T)
+    assert np.array_equal(la, lb) and np.array_equal(ph, pb)
+
+
+def test_learned_sampler_is_unbiased(ctx, orc):
+    """A proposal with random (untrained) weights and a visible bias still estimates the
+    uniform-field C(t): the reweighted mean agrees with the uniform sampler within errors."""
+    import nrcps_amd as na
+    from nrcps_amd import proposal as pr
+
+    L, N, T, C = 8, 7, 12, 2048
+    print(f"[gpu] learned sampler L={L} N={N} T={T} C={C}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    plain = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    la_p, _, _ = plain.run(11, 0, T)
+    mu, sd = pr.logw_stats(la_p)
+    p = _rand_params(7, scale3=0.002)
+    p[8240] = 0.01
+    prop = pr.Proposal(ctx, L, C, p)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    s = pr.LearnedSampler(ctx, prop, ens, mu, sd)
+    la_l, _, logr = s.run(12, 0, T)
+    assert np.all(np.isfinite(logr)) and np.all(logr[0] == 0)
+    ts = [3, 6, 9, 11]
+    sp = pr.estimator_stats(la_p, None, ts)
+    sl = pr.estimator_stats(la_l, logr, ts)
+    for t in ts:
+        d = abs(sl[t]["log_C"] - sp[t]["log_C"])
+        sig = np.hypot(sl[t]["rel_se"], sp[t]["rel_se"])
+        print(f"[gpu] t={t}: log C plain {sp[t]['log_C']:.4f} learned {sl[t]['log_C']:.4f} "
+              f"(|d| {d:.4f}, sigma {sig:.4f}); ess {sp[t]['ess']:.0f} vs {sl[t]['ess']:.0f}",
+              flush=True)
+        assert d < 4 * sig
