@@ -291,7 +291,12 @@ struct nrc_ensemble {
   int nw = 0, nm = 1;
   std::vector<int> moms;
   void* psi = nullptr;       // [C][N][V] complex
-  void* tab_x = nullptr;     // k-space split: 1/xi_half (X pass); MUL_TABLE format
+  void* tab_x = nullptr;     // k-space split: K(1/2) = 1/xi_half; MUL_TABLE format
+  void* tab_k = nullptr;     // k-space split: K = 1/xi_half^2 (two folded halves)
+  double* sink_k = nullptr;  // k-space split: K(1/2)(k_s) per plane-wave sink
+  void* sinks_k = nullptr;   // k-space split, dense sinks: K(1/2) phi_s
+  std::vector<double> khalf; // k-space split: K(1/2) per site (host)
+  bool folded = false;       // the stored orbitals are xi_half psi (k-space split, t >= 1)
   void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass); MUL_TABLE format
   void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
   void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
@@ -321,7 +326,8 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_yz, e->smear, e->sinks, e->ovl_work, e->work,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->sinks_k, e->tab_yz, e->smear,
+                  e->sinks, e->ovl_work, e->work,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev};
@@ -400,8 +406,11 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
               ? NRC_OK : NRC_ERR_NOMEM);
   std::vector<double> h(V);
   if (d->split == NRC_SPLIT_KSPACE) {
-    for (long i = 0; i < V; ++i) h[i] = 1.0 / d->kin[i];
-    ENS_TRY(ens_upload_pair(e, &e->tab_x, h));
+    e->khalf.resize(V);
+    for (long i = 0; i < V; ++i) e->khalf[i] = 1.0 / d->kin[i];
+    ENS_TRY(ens_upload_pair(e, &e->tab_x, e->khalf));
+    for (long i = 0; i < V; ++i) h[i] = e->khalf[i] * e->khalf[i];
+    ENS_TRY(ens_upload_pair(e, &e->tab_k, h));
     if (d->smear) {
       h.assign(d->smear, d->smear + V);
       ENS_TRY(ens_upload_pair(e, &e->smear, h));
@@ -464,6 +473,37 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   hipMemcpy(e->g_pos, gpos.data(), N * sizeof(int), hipMemcpyHostToDevice);
   hipMemcpy(e->g_sink, gsink.data(), N * sizeof(int), hipMemcpyHostToDevice);
   hipMemcpy(e->site_of_orb, site.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  if (d->split == NRC_SPLIT_KSPACE) {  // K(1/2) at the sink momenta, and K(1/2)-scaled sinks
+    std::vector<double> sk(N);
+    for (int s2 = 0; s2 < N; ++s2) sk[s2] = e->khalf[site[s2]];
+    ENS_TRY(dev_alloc(&e->sink_k, N));
+    ENS_TRY(hipMemcpy(e->sink_k, sk.data(), N * sizeof(double), hipMemcpyHostToDevice) ==
+                    hipSuccess ? NRC_OK : NRC_ERR_HIP);
+    if (d->sinks) {
+      const size_t n = (size_t)N * V;
+      if (d->prec == NRC_PREC_F64) {
+        std::vector<double> f(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+          const double k = e->khalf[i % V];
+          f[2 * i] = d->sinks[2 * i] * k;
+          f[2 * i + 1] = d->sinks[2 * i + 1] * k;
+        }
+        ENS_TRY(hipMalloc(&e->sinks_k, f.size() * 8) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+        ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 8, hipMemcpyHostToDevice) ==
+                        hipSuccess ? NRC_OK : NRC_ERR_HIP);
+      } else {
+        std::vector<float> f(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+          const double k = e->khalf[i % V];
+          f[2 * i] = (float)(d->sinks[2 * i] * k);
+          f[2 * i + 1] = (float)(d->sinks[2 * i + 1] * k);
+        }
+        ENS_TRY(hipMalloc(&e->sinks_k, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+        ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 4, hipMemcpyHostToDevice) ==
+                        hipSuccess ? NRC_OK : NRC_ERR_HIP);
+      }
+    }
+  }
   if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
       hipSuccess) {
     ens_free(e);
@@ -637,6 +677,7 @@ static nrc::GatherSpec ens_gather_spec(const nrc_ensemble* e) {
 static int ens_set_sources(nrc_ensemble* e) {
   const int N = e->d.n_orb;
   const long C = e->d.n_configs;
+  e->folded = false;
   if (e->d.split == NRC_SPLIT_KSPACE)
     NRC_HIP(nrc::launch_set_unit_sources(e->d.prec, e->psi, e->site_of_orb, C * N, N, e->V,
                                          e->stream));
@@ -658,12 +699,15 @@ static int ens_measure(nrc_ensemble* e, int t) {
   const size_t per_t = (size_t)C * e->nm;
   {
     EvScope sc(e, 4);
+    // folded k-space state xi_half psi: sinks carry the missing K(1/2)
+    const void* sk = e->folded ? e->sinks_k : e->sinks;
     if (e->sinks && prec == NRC_PREC_F64)
-      NRC_HIP(nrc::launch_overlap_dense(prec, e->sinks, e->psi, e->M, N, N, V, (int)C, s));
+      NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks)
-      NRC_TRY(overlap_bf16x3(e->sinks, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s));
+      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s));
     else
-      NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb, e->M, C, N, V, s));
+      NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb,
+                                       e->folded ? e->sink_k : nullptr, e->M, C, N, V, s));
   }
   EvScope sc(e, 2);
   NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
@@ -690,9 +734,9 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
   a.nvol = C * N;
   a.norb = N;
   a.L = L;
+  a.grid = e->ctx->n_cu;  // persistent kernel: workgroups per CU x CUs
   if (ref) {
-    a.pre = mul_table(e->tab_x);
-    a.post = a.pre;
+    a.pre = mul_table(e->folded ? e->tab_k : e->tab_x);  // post: folded into the next pre
     if (e->smear) {
       a.mid.kind = nrc::MUL_RFIELD;
       a.mid.table = e->rfield;
@@ -715,6 +759,7 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
     EvScope sc(e, 0);
     NRC_HIP(nrc::launch_slice1(prec, a, s));
   }
+  e->folded = ref;
   if (!ref && !e->sinks) {  // measured inside the slice
     EvScope sc(e, 2);
     NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
@@ -798,12 +843,12 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
       a.L = L;
       a.fft_a = t > 0;
       a.fft_b = !last;
-      if (ref) {
-        if (t > 0) a.m1 = mul_table(e->tab_x);
+      if (ref) {  // the trailing K(1/2) of slice t-1 is folded into the next factor
         if (!last) {
           a.gather = 1;
           a.g = g;
-          a.m2 = mul_table(e->tab_x);
+          if (t > 0) a.g.sink_scale = e->sink_k;
+          a.m2 = mul_table(t > 0 ? e->tab_k : e->tab_x);
         }
       } else if (t > 0) {
         a.m1 = fieldspec;
@@ -851,6 +896,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     }
     if (!ref) NRC_TRY(lu(t));
   }
+  if (!single) e->folded = ref;
   return NRC_OK;
 }
 
@@ -891,6 +937,22 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
   NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * bytes, bytes, hipMemcpyDeviceToHost,
                          e->stream));
   NRC_HIP(hipStreamSynchronize(e->stream));
+  if (e->folded) {  // stored xi_half psi: apply the slice's trailing K(1/2)
+    const size_t n = (size_t)e->d.n_orb * e->V;
+    if (e->d.prec == NRC_PREC_F64) {
+      double* o = (double*)out;
+      for (size_t i = 0; i < n; ++i) {
+        o[2 * i] *= e->khalf[i % e->V];
+        o[2 * i + 1] *= e->khalf[i % e->V];
+      }
+    } else {
+      float* o = (float*)out;
+      for (size_t i = 0; i < n; ++i) {
+        o[2 * i] = (float)((double)o[2 * i] * e->khalf[i % e->V]);
+        o[2 * i + 1] = (float)((double)o[2 * i + 1] * e->khalf[i % e->V]);
+      }
+    }
+  }
   return NRC_OK;
 }
 

@@ -38,6 +38,7 @@ struct GatherSpec {
   double2* M = nullptr;
   int nsink = 0;
   double scale = 1.0;
+  const double* sink_scale = nullptr;  // per-sink factor (null: 1)
 };
 
 struct PassArgs {
@@ -69,12 +70,14 @@ struct SliceArgs {
   MulSpec pre, mid, post;
   int gather_pre = 0, gather_mid = 0;
   GatherSpec g;
+  int grid = 0;  // persistent kernel: the device's CU count (0: one workgroup per volume)
 };
 bool slice1_supported(int prec, int L);
 hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);
 // M[c][s][n] = psi[c][n][site_of_sink[s]] (plane-wave sinks on k-space orbitals)
-hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink, double2* M,
-                               long C, int N, long V, hipStream_t s);
+hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink,
+                               const double* sink_scale, double2* M, long C, int N, long V,
+                               hipStream_t s);
 
 // element-wise helpers (the reference's separate kernels, used by the class layer)
 hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch, long vol,

@@ -166,8 +166,9 @@ __device__ __forceinline__ void gather_pencil(const C2<T>* v, const GatherSpec& 
   const int e0 = g.ptr[pencil], e1 = g.ptr[pencil + 1];
   for (int e = e0; e < e1; ++e) {
     const C2<T> val = reg_select<T, L>(v, g.pos[e]);
+    const double f = g.sink_scale ? g.scale * g.sink_scale[g.sink[e]] : g.scale;
     g.M[(cfg * g.nsink + g.sink[e]) * norb + orb] =
-        make_double2((double)val.x * g.scale, (double)val.y * g.scale);
+        make_double2((double)val.x * f, (double)val.y * f);
   }
 }
 

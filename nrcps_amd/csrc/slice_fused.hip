@@ -9,10 +9,12 @@
 //       y-DFT (+1); z-DFT (+1) per row; LDS -> registers
 //   x-DFT (+1, registers) -> post multiplier -> store
 //
-// K-space split (k-space storage): pre = post = 1/xi_half, mid = (1 + A phi)/V, gather
-// before the slice (the measurement at t precedes the evolution, production/slater_det/
-// main.C:206-215).  X-space split (x-space storage): mid = kappa/V, gather at mid (full
-// forward transform), post = 1 + A phi.
+// K-space split (k-space storage): a slice is K(1/2) F+[(1 + A phi)/V F-[K(1/2) psi]]; the
+// trailing K(1/2) of slice t and the leading one of slice t+1 are applied as one factor
+// K = 1/xi_half^2 (pre; the first slice uses K(1/2)), no post multiplier, and the
+// measurement, which precedes the evolution (production/slater_det/main.C:206-215), scales
+// each gathered sink value by K(1/2)(k_s).  X-space split (x-space storage): mid = kappa/V,
+// gather at mid (full forward transform), post = 1 + A phi.
 //
 // Thread t owns the x-pencils (y, z) = t + q*NT, q < PPT, so every global load and store is
 // a coalesced run of consecutive (y, z).  With PPT = NH each thread also owns exactly one row
@@ -117,46 +119,250 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
 // stored in k-space (OneBody::Project of a unit vector, src/one_body/noarch/
 // one_body_project.C:14-27), read straight from memory before the slice overwrites them.
 template <typename T>
-__global__ void k_gather_sites(const C2<T>* psi, const int* site_of_sink, double2* M, long C,
-                               int N, long V) {
+__global__ void k_gather_sites(const C2<T>* psi, const int* site_of_sink, const double* scale,
+                               double2* M, long C, int N, long V) {
   const long n = C * N * N;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
     const long c = i / ((long)N * N);
     const int r = (int)(i - c * N * N), s = r / N, o = r - (r / N) * N;
     const C2<T> val = psi[(c * N + o) * V + site_of_sink[s]];
-    M[i] = make_double2((double)val.x, (double)val.y);
+    const double f = scale ? scale[s] : 1.0;
+    M[i] = make_double2((double)val.x * f, (double)val.y * f);
   }
 }
 
-hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink, double2* M,
-                               long C, int N, long V, hipStream_t s) {
+// M[c][s][n] = sink_scale[s] psi[c][n][site_of_sink[s]] (sink_scale null: 1)
+hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink,
+                               const double* sink_scale, double2* M, long C, int N, long V,
+                               hipStream_t s) {
   const long n = C * N * N;
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
   if (prec == NRC_PREC_F64)
     hipLaunchKernelGGL(k_gather_sites<double>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const C2<double>*)psi, site_of_sink, M, C, N, V);
+                       (const C2<double>*)psi, site_of_sink, sink_scale, M, C, N, V);
   else
     hipLaunchKernelGGL(k_gather_sites<float>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const C2<float>*)psi, site_of_sink, M, C, N, V);
+                       (const C2<float>*)psi, site_of_sink, sink_scale, M, C, N, V);
+  return hipGetLastError();
+}
+
+// w[y] *= (a0 + a1 phi(site)), site = off + y L, phi from the configuration's field words in
+// LDS; (ph, pl) and (mh, ml) are the compensated factors for phi = +1 and -1
+template <typename T, int L>
+__device__ __forceinline__ void mul_field_lds(C2<T>* w, const uint32_t* bl, uint32_t off, T ph,
+                                              T pl, T mh, T ml) {
+#pragma unroll
+  for (int y = 0; y < L; ++y) {
+    const uint32_t site = off + (uint32_t)(y * L);
+    const bool up = (bl[site >> 5] >> (site & 31u)) & 1u;
+    w[y] = scale2<T>(w[y], up ? ph : mh, up ? pl : ml);
+  }
+}
+
+// Persistent, software-pipelined k_slice1 for the k-space split (kinetic factor, field, no
+// trailing factor): one workgroup per CU walks the volumes blockIdx.x, + gridDim.x, ...  A
+// volume is loaded, transformed and stored exactly as in k_slice1, but the loads of the next
+// volume's first pencil are issued before this volume's yz rounds (PF = 1), before its final
+// x-transforms (PF = 0) or once its first pencil is stored (PF = 2), so they land while the
+// CU computes.  Loads complete in order,
+// so everything this volume still waits for is issued first: the configuration's field words
+// (staged in LDS, read by the yz rounds from there) go out before the loads of the other
+// pencils, and the prefetched pencil is multiplied by its kinetic factors at the end of the
+// previous volume.
+template <typename T, int L, int PPT, int PF>
+__global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
+  using C = C2<T>;
+  using E = TabEntry<T>;
+  constexpr int P = L * L, NT = P / PPT, NH = PPT, H = L / NH, LP = L + 1;
+  constexpr long V = (long)L * P;
+  constexpr int NW = (int)(V / 32), WPT = (NW + NT - 1) / NT;
+  static_assert(H * NH == L && NT * PPT == P && V % 32 == 0, "shape");
+  __shared__ C sm[H * L * LP];
+  __shared__ uint32_t bl[NW];
+  const int tid = threadIdx.x;
+  const int rx = tid / L, rq = tid - (tid / L) * L;
+  const double dp = a.mid.a0 + a.mid.a1, dm = a.mid.a0 - a.mid.a1;
+  const T ph = T(dp), mh = T(dm);
+  const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
+  const rsrc_t rt = make_rsrc(a.pre.table, (uint32_t)(V * sizeof(E)));
+  C* psi = reinterpret_cast<C*>(a.psi);
+  long vol = blockIdx.x;
+  if (vol >= a.nvol) return;
+  C nx[L];  // the first pencil of the volume about to be processed
+  // Loads of the first pencil of volume w; past the last volume the resource has no records
+  // and the loads return zeros without touching memory, so the load is unconditional (a
+  // conditional load would keep the previous values alive across the whole volume).
+  auto prefetch = [&](long w) {
+    const bool ok = w < a.nvol;
+    const rsrc_t rn = make_rsrc(psi + (ok ? w : 0) * V, ok ? (uint32_t)(V * sizeof(C)) : 0u);
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      nx[x] = bload<C>(rn, (uint32_t)tid * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
+  };
+  // the first slice factor of that pencil (the table is the same for every volume); issued
+  // once the pencil's loads are out, so that waiting for it waits for nothing later
+  auto factor = [&]() {
+    E tab0[L];
+#pragma unroll
+    for (int x = 0; x < L; ++x)
+      tab0[x] = bload<E>(rt, (uint32_t)tid * (uint32_t)sizeof(E), (uint32_t)(x * P * sizeof(E)));
+#pragma unroll
+    for (int x = 0; x < L; ++x) nx[x] = scale_tab<T>(nx[x], tab0[x]);
+  };
+  prefetch(vol);
+  factor();
+  for (; vol < a.nvol; vol += gridDim.x) {
+    const long cfg = vol / a.norb;
+    const rsrc_t rv = make_rsrc(psi + vol * V, (uint32_t)(V * sizeof(C)));
+    const long nv = vol + gridDim.x;
+    C v[PPT][L];
+    // field words first, then the other pencils
+    uint32_t wb[WPT];
+    {
+      const rsrc_t rb = make_rsrc(a.mid.bits + cfg * a.mid.nw, (uint32_t)NW * 4u);
+#pragma unroll
+      for (int k = 0; k < WPT; ++k) wb[k] = bload<uint32_t>(rb, (uint32_t)(tid + k * NT) * 4u, 0u);
+    }
+#pragma unroll
+    for (int q = 1; q < PPT; ++q) {
+      const uint32_t yz = (uint32_t)(tid + q * NT);
+#pragma unroll
+      for (int x = 0; x < L; ++x)
+        v[q][x] = bload<C>(rv, yz * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int x = 0; x < L; ++x) v[0][x] = nx[x];
+    fft_reg<T, L, -1>(v[0]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 1; q < PPT; ++q) {
+      apply_mul_k<T, L, P, MUL_TABLE>(v[q], a.pre, cfg, (uint32_t)(tid + q * NT), V);
+      fft_reg<T, L, -1>(v[q]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // stage the words; the barrier also orders the previous volume's last LDS reads before
+    // this volume's first plane writes
+#pragma unroll
+    for (int k = 0; k < WPT; ++k)
+      if (tid + k * NT < NW) bl[tid + k * NT] = wb[k];
+    __syncthreads();
+    if constexpr (PF == 1) {
+      prefetch(nv);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
+#pragma unroll
+        for (int xl = 0; xl < H; ++xl) sm[(xl * L + y) * LP + z] = v[q][h * H + xl];
+      }
+      __syncthreads();
+      {  // rows along z
+        C w[L];
+        C* row = sm + (rx * L + rq) * LP;
+#pragma unroll
+        for (int z = 0; z < L; ++z) w[z] = row[z];
+        fft_reg<T, L, -1>(w);
+#pragma unroll
+        for (int z = 0; z < L; ++z) row[z] = w[z];
+      }
+      __syncthreads();
+      {  // columns along y with the field
+        C w[L];
+        C* col = sm + rx * L * LP + rq;
+#pragma unroll
+        for (int y = 0; y < L; ++y) w[y] = col[y * LP];
+        fft_reg<T, L, -1>(w);
+        const int x = h * H + rx;
+        mul_field_lds<T, L>(w, bl, (uint32_t)(x * P + rq), ph, pl, mh, ml);
+        fft_reg<T, L, +1>(w);
+#pragma unroll
+        for (int y = 0; y < L; ++y) col[y * LP] = w[y];
+      }
+      __syncthreads();
+      {  // rows back along z
+        C w[L];
+        C* row = sm + (rx * L + rq) * LP;
+#pragma unroll
+        for (int z = 0; z < L; ++z) w[z] = row[z];
+        fft_reg<T, L, +1>(w);
+#pragma unroll
+        for (int z = 0; z < L; ++z) row[z] = w[z];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
+#pragma unroll
+        for (int xl = 0; xl < H; ++xl) v[q][h * H + xl] = sm[(xl * L + y) * LP + z];
+      }
+      if (h + 1 < NH) __syncthreads();
+    }
+    if constexpr (PF == 0) {
+      prefetch(nv);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const uint32_t yz = (uint32_t)(tid + q * NT);
+      fft_reg<T, L, +1>(v[q]);
+#pragma unroll
+      for (int x = 0; x < L; ++x)
+        bstore<C>(v[q][x], rv, yz * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (PF == 2) {  // after the first pencil is stored, into its registers
+        if (q == 0) prefetch(nv);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    factor();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// a.grid = CUs; the grid is as many workgroups as are resident at once
+template <typename T, int L, int PPT, int PF>
+static hipError_t launch_s1p(const SliceArgs& a, hipStream_t s) {
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slice1p<T, L, PPT, PF>, L * L / PPT,
+                                                     0) != hipSuccess || nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  long grid = (long)a.grid * per_cu;
+  if (grid > a.nvol) grid = a.nvol;
+  hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF>), dim3((unsigned)grid), dim3(L * L / PPT), 0, s,
+                     a);
   return hipGetLastError();
 }
 
 template <typename T, int L, int PPT>
 static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nvol), block(L * L / PPT);
-  // k-space split: 1/xi_half, field, 1/xi_half, measure first; x-space split: kappa/V in
-  // the middle with the measurement there, the field last
+  // the k-space split with the plain field: the persistent pipelined kernel when the caller
+  // gives a workgroup count (prefetch point: before the yz rounds, or after the first pencil's
+  // store where the registers are short)
+  if (a.grid > 0 && a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD &&
+      a.post.kind == MUL_NONE && !a.gather_mid)
+    return launch_s1p<T, L, PPT, (sizeof(T) == 4 && L == 24) ? 2 : 1>(a, s);
+  // k-space split: kinetic factor, field (or smeared field), measured beforehand; x-space
+  // split: kappa/V in the middle with the measurement there, the field last
   if (a.gather_pre) return hipErrorInvalidValue;  // measured by k_gather_sites beforehand
-  if (a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD && a.post.kind == MUL_TABLE &&
+  if (a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD && a.post.kind == MUL_NONE &&
       !a.gather_mid)
-    hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_TABLE, MUL_FIELD, MUL_TABLE, false, false>),
+    hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_TABLE, MUL_FIELD, MUL_NONE, false, false>),
                        grid, block, 0, s, a);
-  else if (a.pre.kind == MUL_TABLE && a.mid.kind == MUL_RFIELD && a.post.kind == MUL_TABLE &&
+  else if (a.pre.kind == MUL_TABLE && a.mid.kind == MUL_RFIELD && a.post.kind == MUL_NONE &&
            !a.gather_mid)
-    hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_TABLE, MUL_RFIELD, MUL_TABLE, false, false>),
+    hipLaunchKernelGGL((k_slice1<T, L, PPT, MUL_TABLE, MUL_RFIELD, MUL_NONE, false, false>),
                        grid, block, 0, s, a);
   else if (a.pre.kind == MUL_NONE && a.mid.kind == MUL_TABLE && a.post.kind == MUL_FIELD &&
            !a.gather_pre && a.gather_mid)
