@@ -62,6 +62,22 @@ def cpu_baseline(L, N, tau, A, slices, threads):
                       f"{secs:.2f} s wall"}
 
 
+def measured_traffic(L, N, C, per_launch):
+    """HBM bytes per launch of the slice kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or None if it was taken on
+    another shape."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("L"), t.get("N"), t.get("configs"), t.get("slices_per_launch")) != (L, N, C,
+                                                                                 per_launch):
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
 def correlator(la, ph):
     """C(t) = < det(up) det(down) > = < det^2 > over configurations (up and down share the
     field and the orbitals), from log|det| and det/|det|; returned as (log scale, C / scale)."""
@@ -291,15 +307,20 @@ def main():
     slice_ms = (tx + tyz) / T
     achieved = b_evo * C / (slice_ms * 1e-3) / 1e9
     single = nyz == 0
-    roofline = {"bound": "hbm",
-                "kernel": "k_slice1 (one launch per slice)" if single
-                else "k_xpass + k_yzpass (one slice)",
+    per_launch = T / max(nx, 1)  # slices per launch of the evolution kernel
+    kernel = (f"k_slice1p ({per_launch:g} slices per launch)" if single and per_launch > 1
+              else "k_slice1 (one launch per slice)" if single
+              else "k_xpass + k_yzpass (one slice)")
+    roofline = {"bound": "hbm", "kernel": kernel,
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "avg_ms": {"evolution_per_slice": slice_ms, "xpass_or_slice": tx / max(nx, 1),
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(L, N, C, per_launch) if single else None,
+                "algorithmic_bytes_per_launch": b_evo * C * per_launch,
+                "avg_ms": {"evolution_per_slice": slice_ms, "evolution_launch": tx / max(nx, 1),
                            "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
                            "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
-                "bytes_per_cs": b_evo}
+                "bytes_per_cs": b_evo,
+                "timing": "HIP events on the ensemble's stream around every evolution launch"}
 
     batched = None
     if not args.no_batched:

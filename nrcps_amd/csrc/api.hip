@@ -306,6 +306,7 @@ struct nrc_ensemble {
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
   size_t words_cap = 0;
   double2* M = nullptr;      // [C][N][N]
+  double2* Mx = nullptr;     // [SLICE_FUSE_MAX - 1][C][N][N]: sinks gathered between fused slices
   double* o_la = nullptr;    // [T][C][nm]
   double2* o_ph = nullptr;
   double* o_as = nullptr;
@@ -327,7 +328,7 @@ struct nrc_ensemble {
 
 static void ens_free(nrc_ensemble* e) {
   void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->sinks_k, e->tab_yz, e->smear,
-                  e->sinks, e->ovl_work, e->work,
+                  e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev};
@@ -424,6 +425,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     ENS_TRY(ens_upload_pair(e, &e->tab_yz, h));
   }
   ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
+  ENS_TRY(dev_alloc(&e->Mx, (size_t)(nrc::SLICE_FUSE_MAX - 1) * C * N * N));
   if (d->sinks) {  // dense sinks in the ensemble precision
     if (d->prec == NRC_PREC_F64) {
       ENS_TRY(hipMalloc(&e->sinks, (size_t)N * V * 16) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
@@ -769,12 +771,55 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
   return NRC_OK;
 }
 
+// slices t .. t + S - 1 in one launch of the persistent kernel (k-space split, plain field):
+// words[k * stride ..] are slice t + k's fields; the sinks gathered after slice t + k
+// (k < S - 1) are factored into result row t + k + 1
+static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words, size_t stride) {
+  hipStream_t s = e->stream;
+  const int N = e->d.n_orb;
+  const long C = e->d.n_configs, V = e->V;
+  const size_t per_t = (size_t)C * e->nm;
+  const long mstride = C * N * N;
+  nrc::SliceArgs a;
+  a.psi = e->psi;
+  a.nvol = C * N;
+  a.norb = N;
+  a.L = e->d.L;
+  a.grid = e->ctx->n_cu;
+  a.pre = mul_table(e->folded ? e->tab_k : e->tab_x);
+  a.mid.kind = nrc::MUL_FIELD;
+  a.mid.nw = e->nw;
+  a.mid.bits = words;
+  a.mid.a0 = 1.0 / (double)V;
+  a.mid.a1 = e->d.A / (double)V;
+  a.nslices = S;
+  a.pre_next = e->tab_k;
+  a.bits_stride = (long)stride;
+  a.m_stride = mstride;
+  a.g = ens_gather_spec(e);
+  a.g.M = e->Mx;
+  a.g.sink_scale = e->sink_k;  // the state after each slice is the folded one
+  {
+    EvScope sc(e, 0);
+    NRC_HIP(nrc::launch_slice1(e->d.prec, a, s));
+  }
+  e->folded = true;
+  for (int k = 1; k < S; ++k) {
+    EvScope sc(e, 2);
+    const size_t row = (size_t)(t + k) * per_t;
+    NRC_HIP(nrc::launch_lu_det(e->Mx + (size_t)(k - 1) * mstride, N, N, (int)C, e->d.minors,
+                               e->o_la + row, e->o_ph + row, e->o_as + row, e->o_st + row,
+                               nullptr, nullptr, nullptr, s));
+  }
+  return NRC_OK;
+}
+
 // allocations and host-to-device copies of a run of T slices (caller fields: words_host)
 static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (T < 1) return NRC_ERR_ARG;
   hipSetDevice(e->ctx->device);
   const size_t slice_words = (size_t)e->d.n_configs * e->nw;
-  const size_t need_words = words_host ? slice_words * T : slice_words;
+  const size_t need_words = words_host ? slice_words * T : slice_words * nrc::SLICE_FUSE_MAX;
   if (need_words > e->words_cap) {
     NRC_HIP(hipStreamSynchronize(e->stream));
     hipFree(e->words);
@@ -821,13 +866,23 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   // single pass: every slice is one kernel over the orbitals (slice_fused.hip)
   if (e->sinks && !single) return NRC_ERR_UNSUPPORTED;  // dense sinks read the stored state
   const bool measure_first = ref || e->sinks;
-  for (int t = 0; single && t < T; ++t) {
+  // k-space split with the plain field: up to SLICE_FUSE_MAX slices per launch of the
+  // persistent kernel, the measurements between them gathered inside it
+  const bool fuse = single && ref && !e->smear && !e->sinks;
+  for (int t = 0; single && t < T;) {
+    const int S = fuse ? std::min(nrc::SLICE_FUSE_MAX, T - t) : 1;
     if (measure_first) NRC_TRY(ens_measure(e, t));
     if (!host_words) {
       EvScope sc(e, 3);
-      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
+      for (int k = 0; k < S; ++k)
+        NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
+                                    e->words + (size_t)k * slice_words, s));
     }
-    NRC_TRY(ens_evolve(e, t, field_of(t)));
+    if (S > 1)
+      NRC_TRY(ens_evolve_fused(e, t, S, host_words ? field_of(t) : e->words, slice_words));
+    else
+      NRC_TRY(ens_evolve(e, t, field_of(t)));
+    t += S;
   }
 
   // two passes per slice: an X pass that finishes slice t-1, measures t and starts slice t,

@@ -71,7 +71,15 @@ struct SliceArgs {
   int gather_pre = 0, gather_mid = 0;
   GatherSpec g;
   int grid = 0;  // persistent kernel: the device's CU count (0: one workgroup per volume)
+  // persistent kernel, k-space split: nslices consecutive slices per launch; slice s > 0
+  // starts with pre_next, reads its field words at mid.bits + s * bits_stride and the sinks
+  // gathered after slice s go to g.M + s * m_stride
+  int nslices = 1;
+  const void* pre_next = nullptr;
+  long bits_stride = 0;
+  long m_stride = 0;
 };
+constexpr int SLICE_FUSE_MAX = 4;  // slices per launch of the persistent slice kernel
 bool slice1_supported(int prec, int L);
 hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);
 // M[c][s][n] = psi[c][n][site_of_sink[s]] (plane-wave sinks on k-space orbitals)

@@ -20,6 +20,8 @@
 // a coalesced run of consecutive (y, z).  With PPT = NH each thread also owns exactly one row
 // and one column of every LDS round.  L = 32 FP32: 512 threads, two pencils per thread, two
 // rounds of 16 planes in 135 KB of LDS; L <= 24 FP32 and L <= 16 FP64: one round.
+#include <type_traits>
+
 #include "nrc_ops.h"
 
 namespace nrc {
@@ -164,15 +166,18 @@ __device__ __forceinline__ void mul_field_lds(C2<T>* w, const uint32_t* bl, uint
 
 // Persistent, software-pipelined k_slice1 for the k-space split (kinetic factor, field, no
 // trailing factor): one workgroup per CU walks the volumes blockIdx.x, + gridDim.x, ...  A
-// volume is loaded, transformed and stored exactly as in k_slice1, but the loads of the next
-// volume's first pencil are issued before this volume's yz rounds (PF = 1), before its final
-// x-transforms (PF = 0) or once its first pencil is stored (PF = 2), so they land while the
-// CU computes.  Loads complete in order,
-// so everything this volume still waits for is issued first: the configuration's field words
-// (staged in LDS, read by the yz rounds from there) go out before the loads of the other
-// pencils, and the prefetched pencil is multiplied by its kinetic factors at the end of the
-// previous volume.
-template <typename T, int L, int PPT, int PF>
+// volume is loaded, transformed and stored as in k_slice1, but
+//  * the loads of the next volume's first pencil are issued before this volume's last yz
+//    rounds (PF = 1), before its final x-transforms (PF = 0) or once its first pencil is stored
+//    (PF = 2), so they land while the CU computes; loads complete in order, so everything this
+//    volume still waits for is issued first: the configurations' field words (staged in LDS)
+//    go out before the other pencils, and the prefetched pencil is multiplied by its first
+//    kinetic factor at the end of the previous volume;
+//  * a.nslices (<= SMAX) consecutive slices run on the volume while it stays on chip: between
+//    two of them the plane-wave sinks are gathered from registers into the next slice's
+//    overlap matrix (a.g, M advanced by a.m_stride per slice), and slice s > 0 starts with
+//    the factor a.pre_next.  HBM traffic per slice drops by that factor.
+template <typename T, int L, int PPT, int PF, int SMAX>
 __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   using C = C2<T>;
   using E = TabEntry<T>;
@@ -181,13 +186,16 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   constexpr int NW = (int)(V / 32), WPT = (NW + NT - 1) / NT;
   static_assert(H * NH == L && NT * PPT == P && V % 32 == 0, "shape");
   __shared__ C sm[H * L * LP];
-  __shared__ uint32_t bl[NW];
+  __shared__ uint32_t bl[SMAX][NW];
   const int tid = threadIdx.x;
   const int rx = tid / L, rq = tid - (tid / L) * L;
+  const int ns = a.nslices < SMAX ? a.nslices : SMAX;
   const double dp = a.mid.a0 + a.mid.a1, dm = a.mid.a0 - a.mid.a1;
   const T ph = T(dp), mh = T(dm);
   const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
   const rsrc_t rt = make_rsrc(a.pre.table, (uint32_t)(V * sizeof(E)));
+  MulSpec knext = a.pre;
+  knext.table = a.pre_next;
   C* psi = reinterpret_cast<C*>(a.psi);
   long vol = blockIdx.x;
   if (vol >= a.nvol) return;
@@ -216,15 +224,20 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   factor();
   for (; vol < a.nvol; vol += gridDim.x) {
     const long cfg = vol / a.norb;
+    const int orb = (int)(vol - cfg * a.norb);
     const rsrc_t rv = make_rsrc(psi + vol * V, (uint32_t)(V * sizeof(C)));
     const long nv = vol + gridDim.x;
     C v[PPT][L];
-    // field words first, then the other pencils
-    uint32_t wb[WPT];
-    {
-      const rsrc_t rb = make_rsrc(a.mid.bits + cfg * a.mid.nw, (uint32_t)NW * 4u);
+    // field words of every slice first, then the other pencils
+    uint32_t wb[SMAX][WPT];
 #pragma unroll
-      for (int k = 0; k < WPT; ++k) wb[k] = bload<uint32_t>(rb, (uint32_t)(tid + k * NT) * 4u, 0u);
+    for (int sl = 0; sl < SMAX; ++sl) {
+      const bool ok = sl < ns;
+      const rsrc_t rb = make_rsrc(a.mid.bits + (ok ? sl : 0) * a.bits_stride + cfg * a.mid.nw,
+                                  ok ? (uint32_t)NW * 4u : 0u);
+#pragma unroll
+      for (int k = 0; k < WPT; ++k)
+        wb[sl][k] = bload<uint32_t>(rb, (uint32_t)(tid + k * NT) * 4u, 0u);
     }
 #pragma unroll
     for (int q = 1; q < PPT; ++q) {
@@ -236,78 +249,107 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int x = 0; x < L; ++x) v[0][x] = nx[x];
-    fft_reg<T, L, -1>(v[0]);
-    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 1; q < PPT; ++q) {
       apply_mul_k<T, L, P, MUL_TABLE>(v[q], a.pre, cfg, (uint32_t)(tid + q * NT), V);
-      fft_reg<T, L, -1>(v[q]);
       __builtin_amdgcn_sched_barrier(0);
     }
     // stage the words; the barrier also orders the previous volume's last LDS reads before
     // this volume's first plane writes
 #pragma unroll
-    for (int k = 0; k < WPT; ++k)
-      if (tid + k * NT < NW) bl[tid + k * NT] = wb[k];
+    for (int sl = 0; sl < SMAX; ++sl)
+#pragma unroll
+      for (int k = 0; k < WPT; ++k)
+        if (tid + k * NT < NW) bl[sl][tid + k * NT] = wb[sl][k];
     __syncthreads();
-    if constexpr (PF == 1) {
-      prefetch(nv);
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    // one slice on the volume in registers and LDS; LAST: the volume's final slice (the
+    // prefetch point of PF = 1), otherwise followed by the gather of the next measurement
+    auto slice = [&](int sl, auto last_tag) {
+      constexpr bool LAST = decltype(last_tag)::value;
+      if (sl > 0) {
 #pragma unroll
-    for (int h = 0; h < NH; ++h) {
+        for (int q = 0; q < PPT; ++q) {
+          apply_mul_k<T, L, P, MUL_TABLE>(v[q], knext, cfg, (uint32_t)(tid + q * NT), V);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
 #pragma unroll
       for (int q = 0; q < PPT; ++q) {
-        const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
-#pragma unroll
-        for (int xl = 0; xl < H; ++xl) sm[(xl * L + y) * LP + z] = v[q][h * H + xl];
+        fft_reg<T, L, -1>(v[q]);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
-      {  // rows along z
-        C w[L];
-        C* row = sm + (rx * L + rq) * LP;
-#pragma unroll
-        for (int z = 0; z < L; ++z) w[z] = row[z];
-        fft_reg<T, L, -1>(w);
-#pragma unroll
-        for (int z = 0; z < L; ++z) row[z] = w[z];
+      if constexpr (LAST && PF == 1) {
+        prefetch(nv);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
-      {  // columns along y with the field
-        C w[L];
-        C* col = sm + rx * L * LP + rq;
+      const uint32_t* b = bl[sl];
 #pragma unroll
-        for (int y = 0; y < L; ++y) w[y] = col[y * LP];
-        fft_reg<T, L, -1>(w);
-        const int x = h * H + rx;
-        mul_field_lds<T, L>(w, bl, (uint32_t)(x * P + rq), ph, pl, mh, ml);
-        fft_reg<T, L, +1>(w);
+      for (int h = 0; h < NH; ++h) {
 #pragma unroll
-        for (int y = 0; y < L; ++y) col[y * LP] = w[y];
+        for (int q = 0; q < PPT; ++q) {
+          const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
+#pragma unroll
+          for (int xl = 0; xl < H; ++xl) sm[(xl * L + y) * LP + z] = v[q][h * H + xl];
+        }
+        __syncthreads();
+        {  // rows along z
+          C w[L];
+          C* row = sm + (rx * L + rq) * LP;
+#pragma unroll
+          for (int z = 0; z < L; ++z) w[z] = row[z];
+          fft_reg<T, L, -1>(w);
+#pragma unroll
+          for (int z = 0; z < L; ++z) row[z] = w[z];
+        }
+        __syncthreads();
+        {  // columns along y with the field
+          C w[L];
+          C* col = sm + rx * L * LP + rq;
+#pragma unroll
+          for (int y = 0; y < L; ++y) w[y] = col[y * LP];
+          fft_reg<T, L, -1>(w);
+          const int x = h * H + rx;
+          mul_field_lds<T, L>(w, b, (uint32_t)(x * P + rq), ph, pl, mh, ml);
+          fft_reg<T, L, +1>(w);
+#pragma unroll
+          for (int y = 0; y < L; ++y) col[y * LP] = w[y];
+        }
+        __syncthreads();
+        {  // rows back along z
+          C w[L];
+          C* row = sm + (rx * L + rq) * LP;
+#pragma unroll
+          for (int z = 0; z < L; ++z) w[z] = row[z];
+          fft_reg<T, L, +1>(w);
+#pragma unroll
+          for (int z = 0; z < L; ++z) row[z] = w[z];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+          const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
+#pragma unroll
+          for (int xl = 0; xl < H; ++xl) v[q][h * H + xl] = sm[(xl * L + y) * LP + z];
+        }
+        __syncthreads();  // the planes are free for the next round, slice or volume
       }
-      __syncthreads();
-      {  // rows back along z
-        C w[L];
-        C* row = sm + (rx * L + rq) * LP;
-#pragma unroll
-        for (int z = 0; z < L; ++z) w[z] = row[z];
-        fft_reg<T, L, +1>(w);
-#pragma unroll
-        for (int z = 0; z < L; ++z) row[z] = w[z];
+      if constexpr (LAST && PF == 0) {
+        prefetch(nv);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
+      if constexpr (!LAST) {
 #pragma unroll
-      for (int q = 0; q < PPT; ++q) {
-        const int yz = tid + q * NT, y = yz / L, z = yz - (yz / L) * L;
-#pragma unroll
-        for (int xl = 0; xl < H; ++xl) v[q][h * H + xl] = sm[(xl * L + y) * LP + z];
+        for (int q = 0; q < PPT; ++q) {
+          fft_reg<T, L, +1>(v[q]);
+          GatherSpec g = a.g;
+          g.M = a.g.M + (long)sl * a.m_stride;
+          gather_pencil<T, L>(v[q], g, tid + q * NT, cfg, orb, a.norb);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
-      if (h + 1 < NH) __syncthreads();
-    }
-    if constexpr (PF == 0) {
-      prefetch(nv);
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    };
+    for (int sl = 0; sl + 1 < ns; ++sl) slice(sl, std::integral_constant<bool, false>{});
+    slice(ns - 1, std::integral_constant<bool, true>{});
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       const uint32_t yz = (uint32_t)(tid + q * NT);
@@ -329,18 +371,20 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
 // a.grid = CUs; the grid is as many workgroups as are resident at once
 template <typename T, int L, int PPT, int PF>
 static hipError_t launch_s1p(const SliceArgs& a, hipStream_t s) {
+  constexpr int SMAX = SLICE_FUSE_MAX;
+  if (a.nslices < 1 || a.nslices > SMAX) return hipErrorInvalidValue;
   static int per_cu = 0;
   if (per_cu == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slice1p<T, L, PPT, PF>, L * L / PPT,
-                                                     0) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slice1p<T, L, PPT, PF, SMAX>,
+                                                     L * L / PPT, 0) != hipSuccess || nb < 1)
       nb = 1;
     per_cu = nb;
   }
   long grid = (long)a.grid * per_cu;
   if (grid > a.nvol) grid = a.nvol;
-  hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF>), dim3((unsigned)grid), dim3(L * L / PPT), 0, s,
-                     a);
+  hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF, SMAX>), dim3((unsigned)grid), dim3(L * L / PPT),
+                     0, s, a);
   return hipGetLastError();
 }
 
@@ -353,6 +397,7 @@ static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
   if (a.grid > 0 && a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD &&
       a.post.kind == MUL_NONE && !a.gather_mid)
     return launch_s1p<T, L, PPT, (sizeof(T) == 4 && L == 24) ? 2 : 1>(a, s);
+  if (a.nslices != 1) return hipErrorInvalidValue;  // fused slices: persistent kernel only
   // k-space split: kinetic factor, field (or smeared field), measured beforehand; x-space
   // split: kappa/V in the middle with the measurement there, the field last
   if (a.gather_pre) return hipErrorInvalidValue;  // measured by k_gather_sites beforehand

@@ -408,12 +408,10 @@ def test_ensemble_full_size_free_gas(ctx, orc, path):
     exact = -2 * np.arange(T)[:, None] * np.log(kin[idx]).sum()
     err = np.abs(la[:, :, 0] - exact).max()
     print(f"[gpu] FP32 free gas log|det| max abs error over T=128: {err:.2e}", flush=True)
-    # Free plane waves are the worst case for FP32: every slice transforms the same values,
-    # so the rounding of each transform pair repeats coherently (about 8e-8 per orbital per
-    # slice; FP32 rounding of the stored values alone loses one ulp per slice on some modes).
-    assert err < 5e-4
-    # the drift is linear in t (coherent), not a random walk
-    d = np.abs(la[1:, 0, 0] - exact[1:, 0])
-    assert d[-1] < 200 * d[0] + 1e-6
+    # Free plane waves are the hardest case for FP32: every slice transforms the same values.
+    # With the compensated (hi, lo) twiddles and factors the pair of transforms is unbiased, so
+    # the error stays near 1e-5 after 128 slices (rounded twiddles alone shrink the orbitals
+    # coherently and reach ~3e-4).
+    assert err < 5e-5
     assert np.abs(ph[:, :, 0] - 1).max() < 1e-5
     assert np.all(la[0] == 0.0)
