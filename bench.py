@@ -163,7 +163,8 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
 
 
 def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, n_plain=65536,
-                     n_learned=16384, steps=300, batch=64, lr=2e-3, seed=7012):
+                     n_learned=16384, steps=300, batch=64, lr=2e-3, seed=7012, optimizer=1,
+                     out_scale=0.0):
     """Effective samples per second of the learned proposal against the uniform sampler.
     The uniform sampler generates n_train configurations; the network is trained on them
     (weighted maximum likelihood, det^2 weights); then n_plain uniform and n_learned proposed
@@ -186,10 +187,11 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                                for k in range(0, n_plain, chunk)], axis=1)
     t_plain = time.perf_counter() - t0
     # training
-    prop = pr.Proposal(ctx, L, max(batch, chunk))
+    prop = pr.Proposal(ctx, L, max(batch, chunk),
+                       pr.init_params(np.random.default_rng(0), out_scale=out_scale))
     t0 = time.perf_counter()
     losses, mu, sd = pr.train(ctx, prop, la_train, seed, 0, steps, batch, lr, ts,
-                              np.random.default_rng(1))
+                              np.random.default_rng(1), optimizer=optimizer)
     ctx.sync()
     t_train = time.perf_counter() - t0
     # proposed configurations
@@ -233,6 +235,7 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "seconds": {"uniform_training_sample": t_train_gen, "training": t_train,
                         "uniform_comparison_sample": t_plain, "proposed_sample": t_learned},
             "training": {"steps": steps, "batch": batch, "lr": lr,
+                         "optimizer": "sgd-momentum" if optimizer == 1 else "adam",
                          "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
                          "output_bias": float(ps[8240])}}
 

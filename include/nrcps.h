@@ -209,8 +209,8 @@ int nrc_ensemble_results_dev(nrc_ensemble* ens, const double** logabs_dev,
  *   site%4) lies below T = ceil(sigmoid(l) 2^32), so q(+1) = T/2^32 exactly, and
  *   logr_dev[c] += sum over sites of log(0.5 / q(phi)) in FP64 (the exact reweighting ratio).
  * nrc_proposal_train_step: gradient of sum_b w_b sum_s -log q(target_b(s)) / (V wsum) into
- *   grad_out (host, may be NULL) and the loss; lr > 0 then takes an Adam step (0.9, 0.999,
- *   1e-8) on the FP32 master parameters.
+ *   grad_out (host, may be NULL) and the loss; lr > 0 then takes a step on the FP32 master
+ *   parameters: Adam (optimizer 0; 0.9, 0.999, 1e-8) or SGD with momentum 0.9 (optimizer 1).
  * nrc_norm_logw: u_dev[c] = (2 logabs_dev[c*stride] - mu) / sd.
  * nrc_z2_bits_list: the Philox Z2 words of (configs_dev[i], ts_dev[i]) pairs (as nrc_z2_bits). */
 typedef struct nrc_proposal nrc_proposal;
@@ -226,7 +226,8 @@ int nrc_proposal_sample(nrc_proposal* prop, const uint32_t* prev_dev, const floa
                         uint32_t* words_dev, double* logr_dev);
 int nrc_proposal_train_step(nrc_proposal* prop, const uint32_t* prev_dev,
                             const uint32_t* target_dev, const float* u_dev, const float* w_dev,
-                            int B, double wsum, float lr, float* grad_out, double* loss_out);
+                            int B, double wsum, float lr, int optimizer, float* grad_out,
+                            double* loss_out);
 int nrc_norm_logw(nrc_context* ctx, const double* logabs_dev, int stride, int B, double mu,
                   double sd, float* u_dev);
 int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t* configs_dev,

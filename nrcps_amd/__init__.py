@@ -99,7 +99,7 @@ def load_library() -> C.CDLL:
         "nrc_proposal_param_count": [], "nrc_proposal_set_params": [vp, vp],
         "nrc_proposal_get_params": [vp, vp], "nrc_proposal_forward": [vp, vp, vp, i, vp],
         "nrc_proposal_sample": [vp, vp, vp, i, u64, u32, u32, u32, vp, vp],
-        "nrc_proposal_train_step": [vp, vp, vp, vp, vp, i, C.c_double, C.c_float, vp,
+        "nrc_proposal_train_step": [vp, vp, vp, vp, vp, i, C.c_double, C.c_float, i, vp,
                                     C.POINTER(C.c_double)],
         "nrc_norm_logw": [vp, vp, i, i, C.c_double, C.c_double, vp],
         "nrc_z2_bits_list": [vp, u64, vp, vp, i, u32, i, vp],

@@ -15,6 +15,8 @@
 // The 16 -> 16 layer, forward and backward-data, runs on the matrix cores
 // (v_mfma_f32_16x16x32_bf16); the 2 -> 16 and 16 -> 1 layers and the weight gradients are
 // VALU kernels.
+#include <type_traits>
+
 #include "nrc_device.h"
 #include "nrc_internal.h"
 
@@ -61,6 +63,35 @@ __device__ __forceinline__ void site_xyz(const Lat& g, long s, int& x, int& y, i
 
 __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 
+// The 27 neighbours of a site as sums of three per-axis terms (wrapped once per site):
+// neighbour(tap) = ox[tap / 9] + oy[(tap / 3) % 3] + oz[tap % 3].  With tap a compile-time
+// constant this is two additions per neighbour.
+struct NbrOff {
+  long ox[3];
+  int oy[3], oz[3];
+  __device__ __forceinline__ void init(const Lat& g, int x, int y, int z) {
+    const int xm = x == 0 ? g.L - 1 : x - 1, xp = x == g.L - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? g.L - 1 : y - 1, yp = y == g.L - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? g.L - 1 : z - 1, zp = z == g.L - 1 ? 0 : z + 1;
+    const long LL = (long)g.L * g.L;
+    ox[0] = xm * LL; ox[1] = x * LL; ox[2] = xp * LL;
+    oy[0] = ym * g.L; oy[1] = y * g.L; oy[2] = yp * g.L;
+    oz[0] = zm; oz[1] = z; oz[2] = zp;
+  }
+  template <int TAP>
+  __device__ __forceinline__ long at() const {
+    return ox[TAP / 9] + oy[(TAP / 3) % 3] + oz[TAP % 3];
+  }
+};
+
+template <int TAP, typename F>
+__device__ __forceinline__ void for_taps(F&& f) {
+  if constexpr (TAP < 27) {
+    f(std::integral_constant<int, TAP>{});
+    for_taps<TAP + 1>(f);
+  }
+}
+
 // ------------------------------------------------------------------ parameters
 __global__ void k_fill_zero_f(float* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.f;
@@ -81,6 +112,17 @@ __global__ void k_refresh(const float* p, __bf16* wb, __bf16* w2t) {
   }
 }
 
+// SGD with momentum 0.9: m = 0.9 m + g, p -= lr m.  Parameters whose gradient is pure noise
+// move in proportion to it (Adam would move them by ~lr every step), which keeps the
+// proposal's logits from picking up site-to-site noise.
+__global__ void k_sgdm(float* p, const float* g, float* m, int n, float lr) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float mi = 0.9f * m[i] + g[i];
+    m[i] = mi;
+    p[i] -= lr * mi;
+  }
+}
+
 // Adam (Kingma & Ba 2015) with bias correction; step is 1-based
 __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1,
                        float b2, float eps, int step) {
@@ -96,43 +138,76 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, floa
 }
 
 // ------------------------------------------------------------------ layer 1 (2 -> 16)
-// prev: [B][nw] field words of slice t-1 (null: no previous slice, phi = 0); u: [B]
+// The phi(t-1) channel is a GEMM with M = 16 output channels, N = 16 sites and K = 27 taps
+// (padded to 32): one v_mfma_f32_16x16x32_bf16 per tile, lane l supplying the weights of
+// channel l % 16 and the eight neighbours 8 (l / 16) .. +7 of site l % 16 as +-1 (0: no
+// previous slice, or a padded tap).  The u channel is constant over the lattice, so its
+// contribution is u * sum_tap W1[co][tap][1]; bias, ReLU, bf16.
+// prev: [B][nw] field words of slice t-1 (null: no previous slice); u: [B]
 __global__ __launch_bounds__(256) void k_conv1(const uint32_t* __restrict__ prev, int nw,
                                                const float* __restrict__ u,
                                                const __bf16* __restrict__ wb,
                                                __bf16* __restrict__ A1, Lat g, long B) {
-  const long n = B * g.V;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const long c = i / g.V, s = i - c * g.V;
-    int x, y, z;
-    site_xyz(g, s, x, y, z);
-    float acc[PC];
-    const float uc = u[c];
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, grp = lane >> 4;
+  bf16x8 wa;
 #pragma unroll
-    for (int co = 0; co < PC; ++co) {
-      float su = 0.f;
-      for (int tap = 0; tap < NTAP; ++tap) su += bf(wb[P_W1 + (co * NTAP + tap) * 2 + 1]);
-      acc[co] = bf(wb[P_B1 + co]) + uc * su;
-    }
+  for (int k = 0; k < 8; ++k) {
+    const int tap = 8 * grp + k;
+    wa[k] = tap < NTAP ? wb[P_W1 + (row * NTAP + tap) * 2] : (__bf16)0.f;
+  }
+  float bco[4], sco[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int co = grp * 4 + r;
+    float su = 0.f;
+    for (int tap = 0; tap < NTAP; ++tap) su += bf(wb[P_W1 + (co * NTAP + tap) * 2 + 1]);
+    sco[r] = su;
+    bco[r] = bf(wb[P_B1 + co]);
+  }
+  const long ntiles = B * g.V / 16;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nwave = (long)gridDim.x * (blockDim.x >> 6);
+  for (long tile = wave; tile < ntiles; tile += nwave) {
+    const long i0 = tile * 16;
+    const long c = i0 / g.V;
+    const long s = i0 - c * g.V + row;
+    bf16x8 b;
     if (prev) {
+      int x, y, z;
+      site_xyz(g, s, x, y, z);
+      NbrOff nb;
+      nb.init(g, x, y, z);
       const uint32_t* w = prev + c * nw;
-      for (int tap = 0; tap < NTAP; ++tap) {
-        const long q = nbr_site(g, x, y, z, tap);
-        const float f = ((w[q >> 5] >> (q & 31)) & 1u) ? 1.f : -1.f;
+      // taps 8 grp + k: the group picks one of four compile-time sets of eight taps
+      auto fill = [&](auto base_tag) {
+        constexpr int T0 = decltype(base_tag)::value;
 #pragma unroll
-        for (int co = 0; co < PC; ++co) acc[co] += f * bf(wb[P_W1 + (co * NTAP + tap) * 2]);
-      }
-    }
-    bf16x8 o0, o1;
+        for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+        for_taps<0>([&](auto tap_tag) {
+          constexpr int TAP = decltype(tap_tag)::value;
+          if constexpr (TAP >= T0 && TAP < T0 + 8) {
+            const long q = nb.at<TAP>();
+            b[TAP - T0] = ((w[q >> 5] >> (q & 31)) & 1u) ? (__bf16)1.f : (__bf16)-1.f;
+          }
+        });
+      };
+      if (grp == 0) fill(std::integral_constant<int, 0>{});
+      else if (grp == 1) fill(std::integral_constant<int, 8>{});
+      else if (grp == 2) fill(std::integral_constant<int, 16>{});
+      else fill(std::integral_constant<int, 24>{});
+    } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      o0[k] = (__bf16)fmaxf(acc[k], 0.f);
-      o1[k] = (__bf16)fmaxf(acc[8 + k], 0.f);
+      for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
     }
-    bf16x8* dst = reinterpret_cast<bf16x8*>(A1 + i * PC);
-    dst[0] = o0;
-    dst[1] = o1;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, acc, 0, 0, 0);
+    const float uc = u[c];
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+    bf16x4 res;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) res[r] = (__bf16)fmaxf(acc[r] + bco[r] + uc * sco[r], 0.f);
+    *reinterpret_cast<bf16x4*>(A1 + (c * g.V + s) * PC + grp * 4) = res;
   }
 }
 
@@ -175,21 +250,41 @@ __global__ __launch_bounds__(256) void k_conv16(const __bf16* __restrict__ in,
     const long s = i0 - c * g.V + row;
     int x, y, z;
     site_xyz(g, s, x, y, z);
-    const __bf16* inc = in + c * g.V * PC;
+    NbrOff nb;
+    nb.init(g, x, y, z);
+    const __bf16* inc = in + c * g.V * PC + ci0;
+    const bool odd = (grp >> 1) != 0;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int st = 0; st < 14; ++st) {
-      const int tap = 2 * st + (grp >> 1);
+    auto step = [&](auto st_tag) {
+      constexpr int ST = decltype(st_tag)::value;
       bf16x8 b;
-      if (tap < NTAP) {
-        const long q = nbr_site(g, x, y, z, tap);
-        b = *reinterpret_cast<const bf16x8*>(inc + q * PC + ci0);
-      } else {
+      if constexpr (2 * ST + 1 < NTAP) {
+        const long q = odd ? nb.at<2 * ST + 1>() : nb.at<2 * ST>();
+        b = *reinterpret_cast<const bf16x8*>(inc + q * PC);
+      } else {  // tap 26 and the zero padding tap 27
+        if (!odd) {
+          b = *reinterpret_cast<const bf16x8*>(inc + nb.at<2 * ST>() * PC);
+        } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+          for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+        }
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[st], b, acc, 0, 0, 0);
-    }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[ST], b, acc, 0, 0, 0);
+    };
+    step(std::integral_constant<int, 0>{});
+    step(std::integral_constant<int, 1>{});
+    step(std::integral_constant<int, 2>{});
+    step(std::integral_constant<int, 3>{});
+    step(std::integral_constant<int, 4>{});
+    step(std::integral_constant<int, 5>{});
+    step(std::integral_constant<int, 6>{});
+    step(std::integral_constant<int, 7>{});
+    step(std::integral_constant<int, 8>{});
+    step(std::integral_constant<int, 9>{});
+    step(std::integral_constant<int, 10>{});
+    step(std::integral_constant<int, 11>{});
+    step(std::integral_constant<int, 12>{});
+    step(std::integral_constant<int, 13>{});
     // D: lane holds output channels 4 grp .. 4 grp + 3 of site s
     const long o = (c * g.V + s) * PC + grp * 4;
     typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
@@ -209,19 +304,27 @@ __global__ __launch_bounds__(256) void k_conv16(const __bf16* __restrict__ in,
 }
 
 // ------------------------------------------------------------------ layer 3 (16 -> 1)
-__device__ __forceinline__ float conv3_logit(const __bf16* __restrict__ A2c, const Lat& g,
-                                             int x, int y, int z, const float* w3, float b3) {
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+
+// logit of a site: b3 + sum_tap sum_ci W3[tap][ci] A2[s + d(tap)][ci], bf16 pairs through
+// v_dot2 with FP32 accumulation; w3 holds the weights as 216 bf16 pairs
+__device__ __forceinline__ float conv3_logit(const __bf16* __restrict__ A2c, const NbrOff& nb,
+                                             const bf16x2* w3, float b3) {
   float acc = b3;
-  for (int tap = 0; tap < NTAP; ++tap) {
-    const long q = nbr_site(g, x, y, z, tap);
-    const bf16x8* p = reinterpret_cast<const bf16x8*>(A2c + q * PC);
-    const bf16x8 a = p[0], b = p[1];
+  for_taps<0>([&](auto tap_tag) {
+    constexpr int TAP = decltype(tap_tag)::value;
+    const bf16x2* p = reinterpret_cast<const bf16x2*>(A2c + nb.at<TAP>() * PC);
+    typedef __attribute__((__vector_size__(8 * sizeof(uint32_t)))) uint32_t u32x8;
+    const u32x8 raw = *reinterpret_cast<const u32x8*>(p);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      acc = fmaf(w3[tap * PC + k], bf(a[k]), acc);
-      acc = fmaf(w3[tap * PC + 8 + k], bf(b[k]), acc);
+      const uint32_t word = raw[k];  // a scalar copy: bit_cast of a vector element reads element 0
+      acc = __builtin_amdgcn_fdot2_f32_bf16(w3[TAP * 8 + k], __builtin_bit_cast(bf16x2, word),
+                                            acc, false);
     }
-  }
+    // groups of nine taps: the loads of later groups are not hoisted over this one
+    if constexpr (TAP % 9 == 8) __builtin_amdgcn_sched_barrier(0);
+  });
   return acc;
 }
 
@@ -248,8 +351,13 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ A2,
                                                const uint32_t* __restrict__ target,
                                                const float* __restrict__ w, float scale,
                                                double* __restrict__ loss) {
-  __shared__ float w3[NTAP * PC];
-  for (int i = threadIdx.x; i < NTAP * PC; i += blockDim.x) w3[i] = bf(wb[P_W3 + i]);
+  __shared__ bf16x2 w3[NTAP * PC / 2];
+  for (int i = threadIdx.x; i < NTAP * PC / 2; i += blockDim.x) {
+    bf16x2 pr;
+    pr[0] = wb[P_W3 + 2 * i];
+    pr[1] = wb[P_W3 + 2 * i + 1];
+    w3[i] = pr;
+  }
   __syncthreads();
   const float b3 = bf(wb[P_B3]);
   const long n = B * g.V;
@@ -260,7 +368,9 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ A2,
     const long c = i / g.V, s = i - c * g.V;
     int x, y, z;
     site_xyz(g, s, x, y, z);
-    const float lg = conv3_logit(A2 + c * g.V * PC, g, x, y, z, w3, b3);
+    NbrOff nb;
+    nb.init(g, x, y, z);
+    const float lg = conv3_logit(A2 + c * g.V * PC, nb, w3, b3);
     if constexpr (MODE == 0) {
       logits[i] = lg;
     } else if constexpr (MODE == 1) {
@@ -615,13 +725,15 @@ extern "C" int nrc_proposal_sample(nrc_proposal* p, const uint32_t* prev_dev, co
 
 // Gradient of the weighted negative log-likelihood
 //   loss = sum_b w[b] sum_s -log q(target_b(s) | prev_b, u_b) / (V sum_b w[b])
-// into grad_out (host, P_N floats; may be null) and loss_out; with lr > 0 an Adam step
-// (beta 0.9, 0.999, eps 1e-8) follows.  All inputs are device arrays of B samples.
+// into grad_out (host, P_N floats; may be null) and loss_out; with lr > 0 an optimizer step
+// follows: Adam (optimizer 0; beta 0.9, 0.999, eps 1e-8) or SGD with momentum 0.9
+// (optimizer 1).  All inputs are device arrays of B samples.
 extern "C" int nrc_proposal_train_step(nrc_proposal* p, const uint32_t* prev_dev,
                                        const uint32_t* target_dev, const float* u_dev,
                                        const float* w_dev, int B, double wsum, float lr,
-                                       float* grad_out, double* loss_out) {
-  if (!p || !target_dev || !u_dev || !w_dev || B < 1 || B > p->max_batch || !(wsum > 0.0))
+                                       int optimizer, float* grad_out, double* loss_out) {
+  if (!p || !target_dev || !u_dev || !w_dev || B < 1 || B > p->max_batch || !(wsum > 0.0) ||
+      (optimizer != 0 && optimizer != 1))
     return NRC_ERR_ARG;
   hipStream_t s = ctx_stream(p->ctx);
   const long n = (long)B * p->g.V;
@@ -669,8 +781,12 @@ extern "C" int nrc_proposal_train_step(nrc_proposal* p, const uint32_t* prev_dev
   PR_HIP(hipGetLastError());
   if (lr > 0.f) {
     ++p->step;
-    hipLaunchKernelGGL(k_adam, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->grad, p->m, p->v,
-                       P_N, lr, 0.9f, 0.999f, 1e-8f, p->step);
+    if (optimizer == 1)
+      hipLaunchKernelGGL(k_sgdm, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->grad, p->m,
+                         P_N, lr);
+    else
+      hipLaunchKernelGGL(k_adam, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->grad, p->m,
+                         p->v, P_N, lr, 0.9f, 0.999f, 1e-8f, p->step);
     hipLaunchKernelGGL(k_refresh, dim3(blocks_for(P_N)), dim3(256), 0, s, p->p, p->wb, p->w2t);
     PR_HIP(hipGetLastError());
   }

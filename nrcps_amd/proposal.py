@@ -83,11 +83,12 @@ class Proposal:
                                             config0, t, stream, words.ptr, logr.ptr),
                "proposal_sample")
 
-    def train_step(self, prev, target, u, w, B, wsum, lr, want_grad=False):
+    def train_step(self, prev, target, u, w, B, wsum, lr, want_grad=False, optimizer=0):
+        """optimizer 0: Adam, 1: SGD with momentum 0.9."""
         g = np.zeros(self.n_params, np.float32) if want_grad else None
         loss = C.c_double()
         _check(self.lib.nrc_proposal_train_step(self.h, prev.ptr if prev else None, target.ptr,
-                                                u.ptr, w.ptr, B, wsum, lr,
+                                                u.ptr, w.ptr, B, wsum, lr, optimizer,
                                                 _ptr(g) if want_grad else None, C.byref(loss)),
                "proposal_train_step")
         return loss.value, g
@@ -170,10 +171,12 @@ class LearnedSampler:
 
 
 def train(ctx: Context, prop: Proposal, la: np.ndarray, seed: int, config0: int, steps: int,
-          batch: int, lr: float, ts_weight, rng: np.random.Generator, log_every: int = 0):
+          batch: int, lr: float, ts_weight, rng: np.random.Generator, log_every: int = 0,
+          optimizer: int = 1):
     """Weighted maximum likelihood on the uniform sampler's fields: configurations c are drawn
     in proportion to their det^2 weights (det2_weights over ts_weight), slices t uniformly;
     the sample is (phi_c(t-1), u_c(t)) -> phi_c(t), fields regenerated from Philox(seed).
+    The learning rate decays to zero on a cosine; SGD with momentum (optimizer 1) by default.
     Returns the loss history."""
     T, Cn = la.shape[0], la.shape[1]
     mu, sd = logw_stats(la)
@@ -200,8 +203,9 @@ def train(ctx: Context, prop: Proposal, la: np.ndarray, seed: int, config0: int,
             tm1_d.upload((t - 1).astype(np.uint32))
             z2_bits_list(ctx, seed, cfg_d, tm1_d, batch, V, prev_d)
         u_d.upload(u)
+        lr_t = 0.5 * lr * (1.0 + np.cos(np.pi * step / steps))  # cosine decay to zero
         loss, _ = prop.train_step(None if first else prev_d, tgt_d, u_d, w_d, batch,
-                                  float(batch), lr)
+                                  float(batch), lr_t, optimizer=optimizer)
         losses.append(loss)
         if log_every and step % log_every == 0:
             print(f"  train step {step}: loss {loss:.8f}", flush=True)
