@@ -33,6 +33,7 @@ static void* ctx_workspace(nrc_context* c, size_t bytes) {
 namespace nrc {
 hipStream_t ctx_stream(nrc_context* c) { return c->stream; }
 int ctx_device(nrc_context* c) { return c->device; }
+int ctx_n_cu(nrc_context* c) { return c->n_cu; }
 }  // namespace nrc
 
 struct nrc_fft {

@@ -122,5 +122,6 @@ size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 // the context's stream and device (api.hip), for entry points defined in other files
 hipStream_t ctx_stream(nrc_context* c);
 int ctx_device(nrc_context* c);
+int ctx_n_cu(nrc_context* c);
 
 }  // namespace nrc

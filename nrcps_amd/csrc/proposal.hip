@@ -404,6 +404,287 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ A2,
   }
 }
 
+// ------------------------------------------------------------------ fused inference
+// The whole network for one configuration inside one workgroup, no activation leaving the CU.
+// The lattice is swept plane by plane along x; the 2 -> 16 activations of three consecutive
+// planes sit in an LDS ring.  Iteration i (0 .. L + 3) computes layer 1 on plane i - 2, layer 2
+// on plane i - 3 and completes the logits of plane i - 4 (planes mod L); the sweep is unrolled
+// by three so that ring slots are compile-time offsets.
+// Layer 1 reads phi(t-1) from a copy with a periodic halo of one site (one byte per site, the
+// high byte of the bf16 +-1), so every tap is the lane's site plus a constant.
+// The 16 -> 1 layer is split by the x offset of its taps: for the layer-2 plane p,
+//   P[dyz][s][dx] = sum_ci W3[9 dx + dyz][ci] A2[p][s][ci]     (dyz = 3 (dy + 1) + dz + 1)
+// comes out of the matrix cores (v_mfma_f32_16x16x16_bf16, whose B operand is exactly the
+// layer-2 accumulator layout; A rows ordered (dyz, dx), three rows used of every four), is
+// staged in LDS for that plane, and every site gathers R_dx(p)[s] = sum_dyz P[dyz][s +
+// d(dyz)][dx].  logit(x) = R_-1(x - 1) + R_0(x) + R_+1(x + 1) + b3 accumulates in registers,
+// in that order.  Layers 1 and 2 round to bf16 exactly as k_conv1 / k_conv16 do.
+// MODE 0: logits out; MODE 1: sample as k_conv3<1> (same Philox words, thresholds and ratio).
+constexpr int FUSED_NT = 512;
+
+template <int L>
+struct FusedLds {
+  static constexpr int L2 = L * L, V = L2 * L, NW = V / 32, LP = L + 2;
+  uint32_t in[NW];
+  uint32_t out[NW];
+  bf16x8 a1[3][2][L2];  // ring slot, channel half, in-plane site
+  f32x4 p[9][L2];
+  double red[FUSED_NT / 64];
+  uint8_t phi[LP * LP * LP];
+};
+
+template <int L, int MODE>
+__global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__ prev,
+                                                    const float* __restrict__ u,
+                                                    const __bf16* __restrict__ wb, int B,
+                                                    float* __restrict__ logits, uint64_t seed,
+                                                    uint32_t config0, uint32_t t,
+                                                    uint32_t stream, uint32_t* __restrict__ words,
+                                                    double* __restrict__ logr) {
+  constexpr int L2 = L * L, V = L2 * L, NW = V / 32, TP = L2 / 16;
+  constexpr int LP = L + 2, LP2 = LP * LP, LP3 = LP2 * LP;
+  constexpr int NWAVE = FUSED_NT / 64;
+  constexpr int TPW = (TP + NWAVE - 1) / NWAVE;    // layer tiles per wavefront
+  constexpr int SPT = (L2 + FUSED_NT - 1) / FUSED_NT;  // gathered sites per thread
+  constexpr int SLOT = 2 * L2 * 16, HALF = L2 * 16;   // bytes of a ring slot, a channel half
+  static_assert(L % 4 == 0, "tiles of 16 in-plane sites and 32-site field words");
+  static_assert(sizeof(FusedLds<L>) <= 160 * 1024, "LDS of one CU");
+  typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+  __shared__ FusedLds<L> sm;
+  char* const a1b = reinterpret_cast<char*>(&sm.a1[0][0][0]);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = lane & 15, grp = lane >> 4;
+  const int half = grp & 1;
+  const bool odd = (grp >> 1) != 0;
+
+  // layer 1: A row = channel `row`, K = taps 8 grp .. +7 of the phi(t-1) channel
+  bf16x8 wa1;
+  int d1[8];  // the taps as offsets in the haloed field (taps past 26: the site itself)
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int tap = 8 * grp + k;
+    wa1[k] = tap < NTAP ? wb[P_W1 + (row * NTAP + tap) * 2] : (__bf16)0.f;
+    d1[k] = tap < NTAP ? (tap / 9 - 1) * LP2 + ((tap / 3) % 3 - 1) * LP + (tap % 3 - 1) : 0;
+  }
+  float b1c[4], s1c[4], b2c[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int co = grp * 4 + r;
+    float su = 0.f;
+    for (int tap = 0; tap < NTAP; ++tap) su += bf(wb[P_W1 + (co * NTAP + tap) * 2 + 1]);
+    s1c[r] = su;
+    b1c[r] = bf(wb[P_B1 + co]);
+    b2c[r] = bf(wb[P_B2 + co]);
+  }
+  // layer 2: as k_conv16
+  bf16x8 wa2[14];
+#pragma unroll
+  for (int st = 0; st < 14; ++st) {
+    const int tap = 2 * st + (grp >> 1);
+    if (tap < NTAP) {
+      wa2[st] = *reinterpret_cast<const bf16x8*>(wb + P_W2 + (row * NTAP + tap) * PC + 8 * half);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) wa2[st][k] = (__bf16)0.f;
+    }
+  }
+  // layer 3 rows: MFMA m, row `row` = 4 j + dx (dx = 3 unused), dyz = 4 m + j; K = channels
+  // 4 grp .. +3
+  bf16x4 wa3[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int dyz = 4 * m + (row >> 2), dxi = row & 3;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      wa3[m][k] = (dxi < 3 && dyz < 9) ? wb[P_W3 + (9 * dxi + dyz) * PC + 4 * grp + k]
+                                       : (__bf16)0.f;
+  }
+  const float b3 = bf(wb[P_B3]);
+
+  for (int c = blockIdx.x; c < B; c += gridDim.x) {
+    if (prev)
+      for (int w = tid; w < NW; w += FUSED_NT) sm.in[w] = prev[(long)c * NW + w];
+    if constexpr (MODE == 1)
+      for (int w = tid; w < NW; w += FUSED_NT) sm.out[w] = 0u;
+    __syncthreads();
+    if (prev) {
+      for (int h = tid; h < LP3; h += FUSED_NT) {
+        const int xp = h / LP2, yp = (h / LP) % LP, zp = h % LP;
+        const int x = xp == 0 ? L - 1 : (xp == LP - 1 ? 0 : xp - 1);
+        const int y = yp == 0 ? L - 1 : (yp == LP - 1 ? 0 : yp - 1);
+        const int z = zp == 0 ? L - 1 : (zp == LP - 1 ? 0 : zp - 1);
+        const int q = (x * L + y) * L + z;
+        sm.phi[h] = ((sm.in[q >> 5] >> (q & 31)) & 1u) ? 0x3f : 0xbf;
+      }
+    }
+    const float uc = u[c];
+    // logits of planes i - 3 and i - 2 in progress, of in-plane sites tid + j FUSED_NT
+    float acc_a[SPT], acc_b[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) acc_a[j] = acc_b[j] = 0.f;
+    double lsum = 0.0;
+    __syncthreads();
+
+    auto iteration = [&](const int i, auto ph_tag) {
+      constexpr int PH = decltype(ph_tag)::value;  // i % 3
+      // ---- layer 1 on plane i - 2 into ring slot i % 3
+      {
+        const int pa = (i - 2 + L) % L;
+#pragma unroll 1
+        for (int tt = 0; tt < TPW; ++tt) {
+          const int tile = wave + tt * NWAVE;
+          if (TP % NWAVE != 0 && tile >= TP) break;
+          const int yz = tile * 16 + row;
+          bf16x8 b;
+          if (prev) {
+            const int y = yz / L, z = yz % L;
+            const int pb = (pa + 1) * LP2 + (y + 1) * LP + (z + 1);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const uint16_t hb = (uint16_t)((sm.phi[pb + d1[k]] << 8) | 0x80);
+              b[k] = __builtin_bit_cast(__bf16, hb);
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+          }
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, b, acc, 0, 0, 0);
+          bf16x4 res;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) res[r] = (__bf16)fmaxf(acc[r] + b1c[r] + uc * s1c[r], 0.f);
+          *reinterpret_cast<bf16x4*>(a1b + PH * SLOT + (grp >> 1) * HALF + yz * 16 +
+                                     (grp & 1) * 8) = res;
+        }
+      }
+      __syncthreads();
+      // ---- layer 2 on plane i - 3 (A1 planes i - 4, i - 3, i - 2 in slots i + 1, i + 2, i)
+      if (i >= 2) {
+        constexpr int SL[3] = {((PH + 1) % 3) * SLOT, ((PH + 2) % 3) * SLOT, PH * SLOT};
+#pragma unroll 1
+        for (int tt = 0; tt < TPW; ++tt) {
+          const int tile = wave + tt * NWAVE;
+          if (TP % NWAVE != 0 && tile >= TP) break;
+          const int yz = tile * 16 + row;
+          const int y = yz / L, z = yz % L;
+          const int oy[3] = {((y + L - 1) % L) * L * 16, y * L * 16, ((y + 1) % L) * L * 16};
+          const int hb = half * HALF;
+          const int oz[3] = {((z + L - 1) % L) * 16 + hb, z * 16 + hb, ((z + 1) % L) * 16 + hb};
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+          auto step = [&](auto st_tag) {
+            constexpr int ST = decltype(st_tag)::value;
+            constexpr int TE = 2 * ST, TO = 2 * ST + 1;
+            const int qe = oy[(TE / 3) % 3] + oz[TE % 3];
+            bf16x8 b;
+            if constexpr (TO < NTAP) {
+              const int qo = oy[(TO / 3) % 3] + oz[TO % 3];
+              if constexpr (SL[TE / 9] == SL[TO / 9]) {
+                b = *reinterpret_cast<const bf16x8*>(a1b + SL[TE / 9] + (odd ? qo : qe));
+              } else {
+                b = *reinterpret_cast<const bf16x8*>(a1b + (odd ? qo + SL[TO / 9]
+                                                                 : qe + SL[TE / 9]));
+              }
+            } else {  // tap 26 and the zero padding tap 27
+              b = *reinterpret_cast<const bf16x8*>(a1b + SL[TE / 9] + qe);
+              if (odd) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) b[k] = (__bf16)0.f;
+              }
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2[ST], b, acc, 0, 0, 0);
+          };
+          step(std::integral_constant<int, 0>{});
+          step(std::integral_constant<int, 1>{});
+          step(std::integral_constant<int, 2>{});
+          step(std::integral_constant<int, 3>{});
+          step(std::integral_constant<int, 4>{});
+          step(std::integral_constant<int, 5>{});
+          step(std::integral_constant<int, 6>{});
+          step(std::integral_constant<int, 7>{});
+          step(std::integral_constant<int, 8>{});
+          step(std::integral_constant<int, 9>{});
+          step(std::integral_constant<int, 10>{});
+          step(std::integral_constant<int, 11>{});
+          step(std::integral_constant<int, 12>{});
+          step(std::integral_constant<int, 13>{});
+          bf16x4 a2;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a2[r] = (__bf16)fmaxf(acc[r] + b2c[r], 0.f);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            f32x4 pm = {0.f, 0.f, 0.f, 0.f};
+            pm = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa3[m], a2, pm, 0, 0, 0);
+            const int dyz = 4 * m + grp;
+            if (dyz < 9) sm.p[dyz][yz] = pm;
+          }
+        }
+      }
+      __syncthreads();
+      // ---- gather the layer-3 partial sums of plane i - 3; logits of plane i - 4 complete
+      if (i >= 2) {
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+          const int yz = tid + j * FUSED_NT;
+          if (SPT * FUSED_NT != L2 && yz >= L2) break;
+          const int y = yz / L, z = yz % L;
+          const int oy[3] = {((y + L - 1) % L) * L, y * L, ((y + 1) % L) * L};
+          const int oz[3] = {(z + L - 1) % L, z, (z + 1) % L};
+          float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+#pragma unroll
+          for (int d = 0; d < 9; ++d) {
+            const f32x4 v = sm.p[d][oy[d / 3] + oz[d % 3]];
+            r0 += v[0];
+            r1 += v[1];
+            r2 += v[2];
+          }
+          const float done = acc_a[j] + r2;
+          acc_a[j] = acc_b[j] + r1;
+          acc_b[j] = r0;
+          if (i >= 4) {
+            const int s = (i - 4) * L2 + yz;
+            const float lg = done + b3;
+            if constexpr (MODE == 0) {
+              logits[(long)c * V + s] = lg;
+            } else {
+              uint32_t ctr[4] = {(uint32_t)(s >> 2), t, config0 + (uint32_t)c, stream};
+              philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+              const uint32_t r = ctr[s & 3];
+              const double p = 1.0 / (1.0 + exp(-(double)lg));
+              double th = ceil(p * 4294967296.0);
+              th = th < 1.0 ? 1.0 : (th > 4294967295.0 ? 4294967295.0 : th);
+              const bool up = (double)r < th;
+              const double d = (th - 2147483648.0) / 2147483648.0;
+              lsum += up ? -log1p(d) : -log1p(-d);
+              if (up) atomicOr(&sm.out[s >> 5], 1u << (s & 31));
+            }
+          }
+        }
+      }
+    };
+    for (int i0 = 0; i0 < L + 4; i0 += 3) {
+      iteration(i0, std::integral_constant<int, 0>{});
+      if (i0 + 1 < L + 4) iteration(i0 + 1, std::integral_constant<int, 1>{});
+      if (i0 + 2 < L + 4) iteration(i0 + 2, std::integral_constant<int, 2>{});
+    }
+    if constexpr (MODE == 1) {
+      const double ws = wave_sum(lsum);
+      if (lane == 0) sm.red[wave] = ws;
+    }
+    __syncthreads();
+    if constexpr (MODE == 1) {
+      for (int w = tid; w < NW; w += FUSED_NT) words[(long)c * NW + w] = sm.out[w];
+      if (tid == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int k = 0; k < NWAVE; ++k) tot += sm.red[k];
+        logr[c] += tot;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ backward
 // dZ2[c][s][ci] = (A2 > 0) * sum_tap W3[tap][ci] dlogit[c][s - d(tap)]
 __global__ __launch_bounds__(256) void k_conv3_bwd(const float* __restrict__ dlogit,
@@ -692,11 +973,54 @@ static int pr_forward_hidden(nrc_proposal* p, const uint32_t* prev, const float*
   return NRC_OK;
 }
 
+// The fused inference kernel where the lattice fits its LDS ring (L <= 24); returns
+// NRC_ERR_UNSUPPORTED for other L, which take the layer-by-layer kernels.
+template <int L, int MODE>
+static int pr_infer_L(nrc_proposal* p, const uint32_t* prev, const float* u, int B, float* lg,
+                      uint64_t seed, uint32_t config0, uint32_t t, uint32_t stream,
+                      uint32_t* words, double* logr) {
+  static int occ[64] = {0};  // resident workgroups per CU, per device
+  const int dev = ctx_device(p->ctx);
+  int& o = occ[dev & 63];
+  if (o == 0) {
+    PR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_infer<L, MODE>, FUSED_NT, 0));
+    if (o < 1) o = 1;
+  }
+  long grid = (long)o * ctx_n_cu(p->ctx);
+  if (grid > B) grid = B;
+  hipLaunchKernelGGL((k_infer<L, MODE>), dim3((unsigned)grid), dim3(FUSED_NT), 0,
+                     ctx_stream(p->ctx), prev, u, p->wb, B, lg, seed, config0, t, stream, words,
+                     logr);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+template <int MODE>
+static int pr_infer(nrc_proposal* p, const uint32_t* prev, const float* u, int B, float* lg,
+                    uint64_t seed, uint32_t config0, uint32_t t, uint32_t stream,
+                    uint32_t* words, double* logr) {
+  switch (p->g.L) {
+    case 4: return pr_infer_L<4, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    case 8: return pr_infer_L<8, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    case 12:
+      return pr_infer_L<12, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    case 16:
+      return pr_infer_L<16, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    case 20:
+      return pr_infer_L<20, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    case 24:
+      return pr_infer_L<24, MODE>(p, prev, u, B, lg, seed, config0, t, stream, words, logr);
+    default: return NRC_ERR_UNSUPPORTED;
+  }
+}
+
 // logits[B][V] (device) of the fields prev[B][nw] (device; null: no previous slice) and the
 // normalised log-weights u[B] (device)
 extern "C" int nrc_proposal_forward(nrc_proposal* p, const uint32_t* prev_dev, const float* u_dev,
                                     int B, float* logits_dev) {
   if (!p || !u_dev || !logits_dev || B < 1 || B > p->max_batch) return NRC_ERR_ARG;
+  const int fr = pr_infer<0>(p, prev_dev, u_dev, B, logits_dev, 0ull, 0u, 0u, 0u, nullptr, nullptr);
+  if (fr != NRC_ERR_UNSUPPORTED) return fr;
   if (pr_forward_hidden(p, prev_dev, u_dev, B) != NRC_OK) return NRC_ERR_HIP;
   hipStream_t s = ctx_stream(p->ctx);
   hipLaunchKernelGGL((k_conv3<0>), dim3(blocks_for((long)B * p->g.V)), dim3(256), 0, s, p->A2,
@@ -713,6 +1037,9 @@ extern "C" int nrc_proposal_sample(nrc_proposal* p, const uint32_t* prev_dev, co
                                    int B, uint64_t seed, uint32_t config0, uint32_t t,
                                    uint32_t stream, uint32_t* words_dev, double* logr_dev) {
   if (!p || !u_dev || !words_dev || !logr_dev || B < 1 || B > p->max_batch) return NRC_ERR_ARG;
+  const int fr = pr_infer<1>(p, prev_dev, u_dev, B, nullptr, seed, config0, t, stream, words_dev,
+                             logr_dev);
+  if (fr != NRC_ERR_UNSUPPORTED) return fr;
   if (pr_forward_hidden(p, prev_dev, u_dev, B) != NRC_OK) return NRC_ERR_HIP;
   hipStream_t s = ctx_stream(p->ctx);
   hipLaunchKernelGGL((k_conv3<1>), dim3(blocks_for((long)B * p->g.V)), dim3(256), 0, s, p->A2,

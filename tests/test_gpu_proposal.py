@@ -33,12 +33,13 @@ def _inputs(ctx, orc, L, B, seed, t):
     return prev, tgt, u
 
 
-@pytest.mark.parametrize("first", [False, True])
-def test_forward_matches_oracle(ctx, orc, first):
+@pytest.mark.parametrize("L,first", [(8, False), (8, True), (24, False), (32, False)])
+def test_forward_matches_oracle(ctx, orc, L, first):
+    """L <= 24: the fused one-workgroup-per-configuration kernel; L = 32: layer by layer."""
     from nrcps_amd import proposal as pr
     from oracle import convnet as cn
 
-    L, B = 8, 3
+    B = 3
     print(f"[gpu] proposal forward L={L} B={B} first={first}", flush=True)
     p = _rand_params(1)
     prop = pr.Proposal(ctx, L, B, p)
@@ -77,11 +78,13 @@ def test_gradient_matches_oracle(ctx, orc):
     prop.close()
 
 
-def test_sampling_ratio_is_exact(ctx, orc):
+@pytest.mark.parametrize("L", [8, 24, 32])
+def test_sampling_ratio_is_exact(ctx, orc, L):
     from nrcps_amd import proposal as pr
     from oracle import convnet as cn
 
-    L, B, t, seed, c0 = 8, 2, 3, 99, 40
+    B, t, seed, c0 = 2, 3, 99, 40
+    print(f"[gpu] proposal sampling L={L} B={B}", flush=True)
     V = L ** 3
     p = _rand_params(3, scale3=0.5)
     prop = pr.Proposal(ctx, L, B, p)
