@@ -292,4 +292,44 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
+// ------------------------------------------------------------------ proposal threshold
+// q(+1) = T / 2^32 of a proposal logit l: T = ceil(s 2^32) clamped to [1, 2^32 - 1], where
+// s = 1 / (1 + e) and e = exp(-clamp(l, -30, 30)) by Cody and Waite's reduction
+// (n = rint(x log2(e)), r = x - n ln2_hi - n ln2_lo) and the degree-6 Taylor polynomial of
+// e^r (|r| <= ln2 / 2, relative error below 2e-7).  Every step is one correctly rounded FP32
+// operation with no contraction into fused multiply-adds, so T is reproducible bit for bit
+// on any IEEE machine (oracle/convnet.py restates it in numpy float32).
+__device__ inline uint32_t proposal_threshold(float l) {
+#pragma clang fp contract(off)
+  const float x = -fminf(fmaxf(l, -30.f), 30.f);
+  const float n = rintf(x * 1.44269504f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.42860677e-6f;
+  float p = 1.38888889e-3f;
+  p = p * r + 8.33333333e-3f;
+  p = p * r + 4.16666667e-2f;
+  p = p * r + 0.166666667f;
+  p = p * r + 0.5f;
+  p = p * r + 1.f;
+  p = p * r + 1.f;
+  const float e = ldexpf(p, (int)n);
+  const float sg = 1.f / (1.f + e);
+  const float T = ceilf(sg * 4294967296.f);
+  return T < 1.f ? 1u : (T >= 4294967296.f ? 4294967295u : (uint32_t)T);
+}
+
+// Sum over sites of log(q_uniform / q) = log(2^31 / f), f = T or 2^32 - T, as a running FP64
+// product of the f in [1/2, 1) mantissa / exponent form: one log per accumulator at the end.
+struct RatioAcc {
+  double m = 1.0;
+  int e = 0;  // sum of exponents minus 31 per site
+  __device__ __forceinline__ void add(uint32_t f) {
+    int k;
+    m = frexp(m * (double)f, &k);
+    e += k - 31;
+  }
+  __device__ __forceinline__ double value() const {
+    return -((double)e * 0.69314718055994531 + log(m));
+  }
+};
+
 }  // namespace nrc

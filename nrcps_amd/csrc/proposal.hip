@@ -336,9 +336,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // MODE 0: logits out.  MODE 1: sample phi(t) from Philox(seed; site / 4, t, config, stream)
-// word site % 4: phi = +1 iff the word is below T = ceil(sigmoid(logit) 2^32) (clamped to
-// [1, 2^32 - 1]), so q(+1) = T / 2^32 exactly; the log ratio log(q_uniform / q) of every site
-// is summed in FP64 into logr[c].  MODE 2: training, dlogit = scale w[c] (q(+1) - phi01) for
+// word site % 4: phi = +1 iff the word is below T = proposal_threshold(logit) (about
+// sigmoid(logit) 2^32), so q(+1) = T / 2^32 exactly; the log ratio log(q_uniform / q) of every
+// site is summed in FP64 into logr[c].  MODE 2: training, dlogit = scale w[c] (q(+1) - phi01) for
 // the target field and the weighted negative log-likelihood summed into loss[0].
 // Sites of a wavefront are consecutive and of one configuration (V % 64 == 0).
 template <int MODE>
@@ -377,13 +377,12 @@ __global__ __launch_bounds__(256) void k_conv3(const __bf16* __restrict__ A2,
       uint32_t ctr[4] = {(uint32_t)(s >> 2), t, config0 + (uint32_t)c, stream};
       philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
       const uint32_t r = ctr[s & 3];
-      const double p = 1.0 / (1.0 + exp(-(double)lg));
-      double th = ceil(p * 4294967296.0);
-      th = th < 1.0 ? 1.0 : (th > 4294967295.0 ? 4294967295.0 : th);
-      const bool up = (double)r < th;
+      const uint32_t th = proposal_threshold(lg);
+      const bool up = r < th;
       // log(q_uniform / q(phi)) = log(2^31 / T) or log(2^31 / (2^32 - T))
-      const double d = (th - 2147483648.0) / 2147483648.0;
-      const double lr = up ? -log1p(d) : -log1p(-d);
+      RatioAcc ra;
+      ra.add(up ? th : (uint32_t)(0u - th));
+      const double lr = ra.value();
       const uint64_t m = __ballot(up);
       const int lane = threadIdx.x & 63;
       if (lane == 0) words[c * nw + (s >> 5)] = (uint32_t)m;
@@ -453,7 +452,8 @@ __global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__
   __shared__ FusedLds<L> sm;
   char* const a1b = reinterpret_cast<char*>(&sm.a1[0][0][0]);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int row = lane & 15, grp = lane >> 4;
   const int half = grp & 1;
   const bool odd = (grp >> 1) != 0;
@@ -519,11 +519,13 @@ __global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__
       }
     }
     const float uc = u[c];
-    // logits of planes i - 3 and i - 2 in progress, of in-plane sites tid + j FUSED_NT
+    // logits of planes i - 3 and i - 2 in progress, of in-plane sites gsite(j): threads in
+    // reverse order from the second site on, so that the wavefronts with fewer layer tiles
+    // (the last ones) gather the remainder
     float acc_a[SPT], acc_b[SPT];
 #pragma unroll
     for (int j = 0; j < SPT; ++j) acc_a[j] = acc_b[j] = 0.f;
-    double lsum = 0.0;
+    RatioAcc ratio;
     __syncthreads();
 
     auto iteration = [&](const int i, auto ph_tag) {
@@ -625,8 +627,8 @@ __global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__
       if (i >= 2) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
-          const int yz = tid + j * FUSED_NT;
-          if (SPT * FUSED_NT != L2 && yz >= L2) break;
+          const int yz = j == 0 ? tid : j * FUSED_NT + (FUSED_NT - 1 - tid);
+          if (SPT * FUSED_NT != L2 && yz >= L2) continue;
           const int y = yz / L, z = yz % L;
           const int oy[3] = {((y + L - 1) % L) * L, y * L, ((y + 1) % L) * L};
           const int oz[3] = {(z + L - 1) % L, z, (z + 1) % L};
@@ -650,12 +652,9 @@ __global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__
               uint32_t ctr[4] = {(uint32_t)(s >> 2), t, config0 + (uint32_t)c, stream};
               philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
               const uint32_t r = ctr[s & 3];
-              const double p = 1.0 / (1.0 + exp(-(double)lg));
-              double th = ceil(p * 4294967296.0);
-              th = th < 1.0 ? 1.0 : (th > 4294967295.0 ? 4294967295.0 : th);
-              const bool up = (double)r < th;
-              const double d = (th - 2147483648.0) / 2147483648.0;
-              lsum += up ? -log1p(d) : -log1p(-d);
+              const uint32_t th = proposal_threshold(lg);
+              const bool up = r < th;
+              ratio.add(up ? th : (uint32_t)(0u - th));
               if (up) atomicOr(&sm.out[s >> 5], 1u << (s & 31));
             }
           }
@@ -668,7 +667,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_infer(const uint32_t* __restrict__
       if (i0 + 2 < L + 4) iteration(i0 + 2, std::integral_constant<int, 2>{});
     }
     if constexpr (MODE == 1) {
-      const double ws = wave_sum(lsum);
+      const double ws = wave_sum(ratio.value());
       if (lane == 0) sm.red[wave] = ws;
     }
     __syncthreads();

@@ -143,12 +143,29 @@ def loss_and_grad(p, prev_phi, target_phi, u, w, L, bf16_round=True):
     return loss, pack(g), lg
 
 
+def threshold(logit: np.ndarray) -> np.ndarray:
+    """T with q(+1) = T / 2^32 (proposal_threshold, nrcps_amd/csrc/nrc_device.h): T =
+    ceil(s 2^32) clamped to [1, 2^32 - 1], s = 1 / (1 + e), e = exp(-clamp(l, -30, 30)) by the
+    Cody-Waite reduction and the degree-6 Taylor polynomial, every step one FP32 operation
+    (numpy float32 arithmetic is IEEE, so this is bit-exact)."""
+    f = np.float32
+    l = np.asarray(logit, dtype=f)
+    x = -np.minimum(np.maximum(l, f(-30.0)), f(30.0))
+    n = np.rint(x * f(1.44269504))
+    r = (x - n * f(0.693145751953125)) - n * f(1.42860677e-6)
+    p = np.full_like(r, f(1.38888889e-3))
+    for c in (8.33333333e-3, 4.16666667e-2, 0.166666667, 0.5, 1.0, 1.0):
+        p = p * r + f(c)
+    e = np.ldexp(p, n.astype(np.int32))
+    s = f(1.0) / (f(1.0) + e)
+    T = np.ceil(s * f(4294967296.0)).astype(np.float64)
+    return np.clip(T, 1.0, 2.0 ** 32 - 1.0)
+
+
 def sample_ratio(logit: np.ndarray, up: np.ndarray) -> np.ndarray:
-    """Per-site log(0.5 / q(phi)) for the threshold draw: q(+1) = T / 2^32 with
-    T = ceil(sigmoid(logit) 2^32) clamped to [1, 2^32 - 1] (logit as the FP32 value)."""
-    lg = np.asarray(logit, dtype=np.float32).astype(np.float64)
-    p = 1.0 / (1.0 + np.exp(-lg))
-    T = np.clip(np.ceil(p * 2.0 ** 32), 1.0, 2.0 ** 32 - 1.0)
+    """Per-site log(0.5 / q(phi)) for the threshold draw: q(+1) = T / 2^32 with T =
+    threshold(logit) (logit as the FP32 value); returns it and T."""
+    T = threshold(logit)
     q = np.where(up, T, 2.0 ** 32 - T) / 2.0 ** 32
     return np.log(0.5 / q), T
 

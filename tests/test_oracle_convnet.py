@@ -93,3 +93,26 @@ def test_sample_ratio_is_normalised(cn):
     np.testing.assert_allclose(q_up + q_dn, 1.0, rtol=1e-15)
     assert T[2] == 2.0 ** 31 and r_up[2] == 0.0 and r_dn[2] == 0.0
     assert 1.0 <= T.min() and T.max() <= 2.0 ** 32 - 1
+
+
+def test_threshold_tracks_the_sigmoid(cn):
+    """The FP32 threshold recipe is monotone, exact at 0 and within 1e-7 of sigmoid * 2^32."""
+    lg = np.linspace(-35.0, 35.0, 200001).astype(np.float32)
+    T = cn.threshold(lg)
+    sig = 1.0 / (1.0 + np.exp(-lg.astype(np.float64)))
+    assert np.abs(T / 2.0 ** 32 - sig).max() < 1e-7
+    assert np.all(np.diff(T) >= 0)
+    assert cn.threshold(np.float32([0.0]))[0] == 2.0 ** 31
+    assert T.min() == 1.0 and T.max() == 2.0 ** 32 - 1
+    # a contraction of p * r + c into one fused multiply-add changes T somewhere: the recipe
+    # is the one with separately rounded products
+    f = np.float32
+    x = -np.clip(lg, f(-30), f(30))
+    n = np.rint(x * f(1.44269504))
+    r = (x - n * f(0.693145751953125)) - n * f(1.42860677e-6)
+    p = np.full_like(r, f(1.38888889e-3)).astype(np.float64)
+    for c in (8.33333333e-3, 4.16666667e-2, 0.166666667, 0.5, 1.0, 1.0):
+        p = (p * r.astype(np.float64) + np.float64(f(c))).astype(np.float32).astype(np.float64)
+    e = np.ldexp(p.astype(np.float32), n.astype(np.int32))
+    Tf = np.clip(np.ceil((f(1) / (f(1) + e)) * f(4294967296.0)).astype(np.float64), 1, 2 ** 32 - 1)
+    assert np.any(Tf != T)
