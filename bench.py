@@ -163,7 +163,7 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
 
 
 def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, n_plain=65536,
-                     n_learned=16384, steps=300, batch=64, lr=2e-3, seed=7012, optimizer=1,
+                     n_learned=16384, steps=300, batch=64, lr=4e-3, seed=7012, optimizer=0,
                      out_scale=0.0):
     """Effective samples per second of the learned proposal against the uniform sampler.
     The uniform sampler generates n_train configurations; the network is trained on them

@@ -112,9 +112,8 @@ __global__ void k_refresh(const float* p, __bf16* wb, __bf16* w2t) {
   }
 }
 
-// SGD with momentum 0.9: m = 0.9 m + g, p -= lr m.  Parameters whose gradient is pure noise
-// move in proportion to it (Adam would move them by ~lr every step), which keeps the
-// proposal's logits from picking up site-to-site noise.
+// SGD with momentum 0.9: m = 0.9 m + g, p -= lr m.  Unlike Adam, parameters whose gradient is
+// pure noise move only in proportion to it.
 __global__ void k_sgdm(float* p, const float* g, float* m, int n, float lr) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float mi = 0.9f * m[i] + g[i];

@@ -172,11 +172,12 @@ class LearnedSampler:
 
 def train(ctx: Context, prop: Proposal, la: np.ndarray, seed: int, config0: int, steps: int,
           batch: int, lr: float, ts_weight, rng: np.random.Generator, log_every: int = 0,
-          optimizer: int = 1):
+          optimizer: int = 0):
     """Weighted maximum likelihood on the uniform sampler's fields: configurations c are drawn
     in proportion to their det^2 weights (det2_weights over ts_weight), slices t uniformly;
     the sample is (phi_c(t-1), u_c(t)) -> phi_c(t), fields regenerated from Philox(seed).
-    The learning rate decays to zero on a cosine; SGD with momentum (optimizer 1) by default.
+    The learning rate decays to zero on a cosine; Adam (optimizer 0) by default, SGD with
+    momentum (optimizer 1) on request.
     Returns the loss history."""
     T, Cn = la.shape[0], la.shape[1]
     mu, sd = logw_stats(la)
