@@ -295,6 +295,7 @@ struct nrc_ensemble {
   void* tab_x = nullptr;     // k-space split: K(1/2) = 1/xi_half; MUL_TABLE format
   void* tab_k = nullptr;     // k-space split: K = 1/xi_half^2 (two folded halves)
   double* sink_k = nullptr;  // k-space split: K(1/2)(k_s) per plane-wave sink
+  double* lsrc = nullptr;    // FP32: log source_amplitude(c) per configuration
   void* sinks_k = nullptr;   // k-space split, dense sinks: K(1/2) phi_s
   std::vector<double> khalf; // k-space split: K(1/2) per site (host)
   bool folded = false;       // the stored orbitals are xi_half psi (k-space split, t >= 1)
@@ -328,7 +329,7 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->sinks_k, e->tab_yz, e->smear,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->sinks_k, e->tab_yz, e->smear,
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
@@ -476,6 +477,13 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   hipMemcpy(e->g_pos, gpos.data(), N * sizeof(int), hipMemcpyHostToDevice);
   hipMemcpy(e->g_sink, gsink.data(), N * sizeof(int), hipMemcpyHostToDevice);
   hipMemcpy(e->site_of_orb, site.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  if (d->prec == NRC_PREC_F32) {  // the amplitudes of the FP32 sources, divided out by the LU
+    std::vector<double> ls(C);
+    for (long c = 0; c < C; ++c) ls[c] = log(nrc::source_amplitude(c));
+    ENS_TRY(dev_alloc(&e->lsrc, C));
+    ENS_TRY(hipMemcpy(e->lsrc, ls.data(), C * sizeof(double), hipMemcpyHostToDevice) ==
+                    hipSuccess ? NRC_OK : NRC_ERR_HIP);
+  }
   if (d->split == NRC_SPLIT_KSPACE) {  // K(1/2) at the sink momenta, and K(1/2)-scaled sinks
     std::vector<double> sk(N);
     for (int s2 = 0; s2 < N; ++s2) sk[s2] = e->khalf[site[s2]];
@@ -715,7 +723,7 @@ static int ens_measure(nrc_ensemble* e, int t) {
   EvScope sc(e, 2);
   NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
                              e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                             nullptr, nullptr, nullptr, s));
+                             nullptr, nullptr, nullptr, s, e->lsrc));
   return NRC_OK;
 }
 
@@ -767,7 +775,7 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
     EvScope sc(e, 2);
     NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
                                e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                               nullptr, nullptr, nullptr, s));
+                               nullptr, nullptr, nullptr, s, e->lsrc));
   }
   return NRC_OK;
 }
@@ -810,7 +818,7 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
     const size_t row = (size_t)(t + k) * per_t;
     NRC_HIP(nrc::launch_lu_det(e->Mx + (size_t)(k - 1) * mstride, N, N, (int)C, e->d.minors,
                                e->o_la + row, e->o_ph + row, e->o_as + row, e->o_st + row,
-                               nullptr, nullptr, nullptr, s));
+                               nullptr, nullptr, nullptr, s, e->lsrc));
   }
   return NRC_OK;
 }
@@ -855,7 +863,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     EvScope sc(e, 2);
     NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
                                e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                               nullptr, nullptr, nullptr, s));
+                               nullptr, nullptr, nullptr, s, e->lsrc));
     return NRC_OK;
   };
   nrc::MulSpec fieldspec;
@@ -993,20 +1001,22 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
   NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * bytes, bytes, hipMemcpyDeviceToHost,
                          e->stream));
   NRC_HIP(hipStreamSynchronize(e->stream));
-  if (e->folded) {  // stored xi_half psi: apply the slice's trailing K(1/2)
-    const size_t n = (size_t)e->d.n_orb * e->V;
-    if (e->d.prec == NRC_PREC_F64) {
+  const size_t n = (size_t)e->d.n_orb * e->V;
+  if (e->d.prec == NRC_PREC_F64) {
+    if (e->folded) {  // stored xi_half psi: apply the slice's trailing K(1/2)
       double* o = (double*)out;
       for (size_t i = 0; i < n; ++i) {
         o[2 * i] *= e->khalf[i % e->V];
         o[2 * i + 1] *= e->khalf[i % e->V];
       }
-    } else {
-      float* o = (float*)out;
-      for (size_t i = 0; i < n; ++i) {
-        o[2 * i] = (float)((double)o[2 * i] * e->khalf[i % e->V]);
-        o[2 * i + 1] = (float)((double)o[2 * i + 1] * e->khalf[i % e->V]);
-      }
+    }
+  } else {  // and divide out the configuration's source amplitude
+    const double r = 1.0 / nrc::source_amplitude(c);
+    float* o = (float*)out;
+    for (size_t i = 0; i < n; ++i) {
+      const double f = e->folded ? e->khalf[i % e->V] * r : r;
+      o[2 * i] = (float)((double)o[2 * i] * f);
+      o[2 * i + 1] = (float)((double)o[2 * i + 1] * f);
     }
   }
   return NRC_OK;

@@ -261,7 +261,8 @@ __global__ void k_unit_sources(C2<T>* psi, const int* site_of_orb, long nvol, in
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
     const long v = i / vol, site = i - v * vol;
-    psi[i] = {site == site_of_orb[v % norb] ? T(1) : T(0), T(0)};
+    const T one = sizeof(T) == 4 ? (T)source_amplitude(v / norb) : T(1);
+    psi[i] = {site == site_of_orb[v % norb] ? one : T(0), T(0)};
   }
 }
 
@@ -292,7 +293,8 @@ __global__ void k_plane_waves(C2<T>* psi, const int* moms, long nvol, int norb, 
     const long ph = ((long)moms[3 * o] * x + (long)moms[3 * o + 1] * y + (long)moms[3 * o + 2] * z) % L;
     double sn, cs;
     sincospi(2.0 * (double)ph / (double)L, &sn, &cs);
-    psi[i] = {(T)(cs * nrm), (T)(sn * nrm)};
+    const double a = sizeof(T) == 4 ? nrm * source_amplitude(v / norb) : nrm;
+    psi[i] = {(T)(cs * a), (T)(sn * a)};
   }
 }
 

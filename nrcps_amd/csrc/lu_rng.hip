@@ -119,7 +119,8 @@ __device__ __forceinline__ double2 cmuld(double2 a, double2 b) {
 __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, int n, int ldm,
                                                int ncfg, int minors, double* logabs,
                                                double2* phase, double* argsum, int* status,
-                                               double2* lu_out, int* indx_out, double* d_out) {
+                                               double2* lu_out, int* indx_out, double* d_out,
+                                               const double* lsrc) {
   extern __shared__ double2 a[];
   const int nm = minors ? n : 1;
   const long blk = blockIdx.x;
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   }
   if (i == 0) {
     if (status) status[o] = bad ? 1 : 0;
-    if (logabs) logabs[o] = bad ? -INFINITY : la;
+    if (logabs) logabs[o] = bad ? -INFINITY : (lsrc ? la - sz * lsrc[cfg] : la);
     if (phase) phase[o] = bad ? make_double2(0.0, 0.0) : make_double2(dsign * ph.x, dsign * ph.y);
     if (argsum) argsum[o] = bad ? 0.0 : as + (dsign < 0.0 ? 3.14159265358979323846 : 0.0);
   }
@@ -227,12 +228,13 @@ hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s) 
 
 hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
                          double* logabs, double2* phase, double* argsum, int* status,
-                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s) {
+                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s,
+                         const double* lsrc) {
   if (n < 1 || n > 64) return hipErrorInvalidValue;
   const long blocks = (long)ncfg * (minors ? n : 1);
   const size_t shm = (size_t)n * n * sizeof(double2);
   hipLaunchKernelGGL(k_lu_det, dim3((unsigned)blocks), dim3(64), shm, s, M, n, ldm, ncfg,
-                     minors, logabs, phase, argsum, status, lu_out, indx_out, d_out);
+                     minors, logabs, phase, argsum, status, lu_out, indx_out, d_out, lsrc);
   return hipGetLastError();
 }
 

@@ -94,6 +94,8 @@ hipError_t launch_mul_complex(int prec, void* buf, const void* field, long batch
                               hipStream_t s);
 hipError_t launch_fill(void* p, uint8_t v, size_t bytes, hipStream_t s);
 hipError_t launch_plus_one_div_vol(int prec, void* buf, long vol, hipStream_t s);
+// sources of nvol = configurations x norb volumes; FP32 sources of configuration c carry the
+// amplitude source_amplitude(c)
 hipError_t launch_set_unit_sources(int prec, void* psi, const int* site_of_orb, long nvol,
                                    int norb, long vol, hipStream_t s);
 hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvol, int norb,
@@ -103,10 +105,28 @@ hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvo
 hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
                           uint32_t stream, int vol, uint32_t* words, hipStream_t s);
 
-// batched LU log-determinant of M[c][n][n] (leading minors when minors != 0)
+// batched LU log-determinant of M[c][n][n] (leading minors when minors != 0); with lsrc,
+// log|det| of a k x k minor of configuration c is reduced by k lsrc[c]
 hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
                          double* logabs, double2* phase, double* argsum, int* status,
-                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s);
+                         double2* lu_out, int* indx_out, double* d_out, hipStream_t s,
+                         const double* lsrc = nullptr);
+
+// Amplitude r_c in [1, 2) of configuration c's FP32 sources: 23 hashed mantissa bits, so r_c
+// is exact in FP32.  The first slice acts on sources that are the same in every
+// configuration, and FP32 rounds their products identically everywhere, which biases
+// log|det| by a constant (~4e-7 at L = 32, N = 33); scaling configuration c by r_c
+// decorrelates those roundings, and since the evolution is linear the measurements divide
+// out exactly (det of a k x k minor scales by r_c^k).
+__host__ __device__ inline double source_amplitude(long c) {
+  uint32_t h = (uint32_t)c * 0x9E3779B1u;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return 1.0 + (double)(h >> 9) * (1.0 / 8388608.0);
+}
 // |z| computed exactly as the host C library's hypot (the LU's pivot metric)
 hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s);
 

@@ -414,4 +414,5 @@ def test_ensemble_full_size_free_gas(ctx, orc, path):
     # coherently and reach ~3e-4).
     assert err < 5e-5
     assert np.abs(ph[:, :, 0] - 1).max() < 1e-5
-    assert np.all(la[0] == 0.0)
+    # t = 0 measures the sources; their FP32 amplitudes r_c divide out to rounding
+    assert np.abs(la[0]).max() < 1e-12
