@@ -256,7 +256,7 @@ def main():
                     help="skip the batched uneven-shape measurement")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-proposal measurement")
-    ap.add_argument("--precision-configs", type=int, default=64,
+    ap.add_argument("--precision-configs", type=int, default=1024,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")

@@ -20,6 +20,7 @@ CUTOFF_NONE, CUTOFF_HARD = 0, 1
 INTER_NONE, INTER_XISQMINUSONE, INTER_ONEMINUSXIINVSQ = 0, 1, 2
 SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 FIELD_PHILOX, FIELD_RANLXS, FIELD_ARRAY = 0, 1, 2
+WAVEFUNC_NONE, WAVEFUNC_UNIFORM, WAVEFUNC_GND, WAVEFUNC_TGND, WAVEFUNC_PAIR1, WAVEFUNC_PAIR2 = range(6)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -39,7 +40,7 @@ class RunDesc(C.Structure):
                 ("minors", C.c_int), ("moms", C.c_void_p), ("kin", C.c_void_p),
                 ("A", C.c_double), ("smear", C.c_void_p), ("field_kind", C.c_int),
                 ("seed", C.c_uint64), ("config", C.c_uint32), ("rng", C.c_void_p),
-                ("phi", C.c_void_p)]
+                ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p)]
 
 
 def build(force: bool = False) -> None:
@@ -97,6 +98,16 @@ def lib() -> C.CDLL:
         L.orc_run_config.argtypes = [C.POINTER(RunDesc), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]
         L.orc_exact_c1.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp]
+        L.orc_dawson.argtypes = [C.c_double]
+        L.orc_dawson.restype = C.c_double
+        L.orc_opposite_parity_index.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip]
+        L.orc_opposite_parity_index.restype = None
+        L.orc_two_body_wavefunc.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
+                                            C.c_int, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, C.c_double]
+        L.orc_two_body_wavefunc.restype = None
+        L.orc_pair_overlap.argtypes = [_dp, _dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_long]
+        L.orc_pair_overlap.restype = None
         L.orc_time_threads.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_int]
         L.orc_time_threads.restype = C.c_double
         _lib = L
@@ -258,14 +269,48 @@ def overlap(phi: np.ndarray, psi: np.ndarray) -> np.ndarray:
     return M.view(np.complex128).reshape(ns, nt)
 
 
+def dawson(x: float) -> float:
+    return lib().orc_dawson(float(x))
+
+
+def opposite_parity_index(L, bc=None) -> np.ndarray:
+    lx, ly, lz = (L, L, L) if np.isscalar(L) else L
+    out = np.empty(lx * ly * lz, dtype=np.int32)
+    lib().orc_opposite_parity_index(lx, ly, lz, _bc(bc), out)
+    return out
+
+
+def two_body_wavefunc(L, wtype, lam, disp1=0, disp2=0, mass1=1.0, mass2=1.0,
+                      cutoff_sq=np.pi ** 2, bc=None) -> np.ndarray:
+    lx, ly, lz = (L, L, L) if np.isscalar(L) else L
+    out = np.empty(lx * ly * lz)
+    lib().orc_two_body_wavefunc(out, lx, ly, lz, _bc(bc), wtype, disp1, disp2, mass1, mass2,
+                                lam, cutoff_sq)
+    return out
+
+
+def pair_overlap(psi1: np.ndarray, psi2: np.ndarray, w: np.ndarray, opp: np.ndarray):
+    """M[s][t] = sum_i w[i] psi1[s][i] psi2[t][opp[i]] (n1 x n2 complex)."""
+    n1, vol = psi1.shape
+    n2 = psi2.shape[0]
+    M = np.zeros(2 * n1 * n2)
+    lib().orc_pair_overlap(np.ascontiguousarray(psi1, np.complex128).reshape(-1).view(np.float64),
+                           np.ascontiguousarray(psi2, np.complex128).reshape(-1).view(np.float64),
+                           np.ascontiguousarray(w, np.float64),
+                           np.ascontiguousarray(opp, np.int32), M, n1, n2, vol)
+    return M.view(np.complex128).reshape(n1, n2)
+
+
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
-               want_psi=False):
+               want_psi=False, pair_w=None, pair_opp=None):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
-    (complex), argsum [T, nm], optional M [T, N, N] and psi [N, V]."""
+    (complex), argsum [T, nm], optional M [T, N, N] (the pair matrix with pair_w) and
+    psi [N, V].  With pair_w / pair_opp: nm = 2N, SlaterDet3's determinants (2n + 1: leading
+    (n+1) minor of the pair matrix; 2n: the same with column n from the one-body overlaps)."""
     moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)
     kin = np.ascontiguousarray(kin, dtype=np.float64)
-    nm = N if minors else 1
+    nm = 2 * N if pair_w is not None else (N if minors else 1)
     la = np.zeros(T * nm)
     ph = np.zeros(2 * T * nm)
     ag = np.zeros(T * nm)
@@ -281,6 +326,11 @@ def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smea
         sm = np.ascontiguousarray(smear, dtype=np.float64)
         d.smear = sm.ctypes.data
     keep = []
+    if pair_w is not None:
+        pw = np.ascontiguousarray(pair_w, dtype=np.float64)
+        po = np.ascontiguousarray(pair_opp, dtype=np.int32)
+        keep += [pw, po]
+        d.pair_w, d.pair_opp = pw.ctypes.data, po.ctypes.data
     if field == "philox":
         d.field_kind, d.seed, d.config = FIELD_PHILOX, seed, config
     elif field == "ranlxs":

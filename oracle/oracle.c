@@ -637,6 +637,104 @@ void orc_overlap(const double* phi, const double* psi, double* M, int ns, int nt
 }
 
 /* =========================================================================================
+ * The pair sink
+ * ========================================================================================= */
+/* Rybicki (1989): F(x) = lim_{h->0} pi^-1/2 sum over odd n of exp(-(x - n h)^2) / n.  With
+ * n0 the even integer nearest x/h and x' = x - n0 h, the odd n = n0 +- (2i - 1) give
+ * exp(-x'^2) c_i (e^{2 x' (2i-1) h} / (n0 + 2i - 1) + e^{-2 x' (2i-1) h} / (n0 - 2i + 1))
+ * with c_i = exp(-((2i - 1) h)^2); the exponentials are stepped by e^{4 x' h}.
+ * Below 0.2: x (1 - 2/3 x^2 (1 - 2/5 x^2 (1 - 2/7 x^2))), the Taylor series to x^7.
+ * Step, term count and evaluation order are special_functions.C:48-78's, so the values
+ * agree to rounding. */
+double orc_dawson(double x) {
+  const double h = 0.4;
+  if (fabs(x) < 0.2) {
+    const double x2 = x * x;
+    return x * (1.0 - (2.0 / 3.0) * x2 * (1.0 - 0.4 * x2 * (1.0 - (2.0 / 7.0) * x2)));
+  }
+  double c[7];
+  for (int i = 1; i <= 6; ++i) c[i] = exp(-((2.0 * i - 1.0) * h) * ((2.0 * i - 1.0) * h));
+  const double ax = fabs(x);
+  const int n0 = 2 * (int)(0.5 * ax / h + 0.5);
+  const double xp = ax - n0 * h;
+  double e1 = exp(2.0 * xp * h);
+  const double e2 = e1 * e1;
+  double d1 = n0 + 1, d2 = d1 - 2.0, sum = 0.0;
+  for (int i = 1; i <= 6; ++i, d1 += 2.0, d2 -= 2.0, e1 *= e2)
+    sum += c[i] * (e1 / d1 + 1.0 / (d2 * e1));
+  const double m = exp(-xp * xp);
+  return (x >= 0.0 ? m : -m) * sum / 1.7724538509055160273;
+}
+
+static int wrap_mod(int a, int n) { return ((a % n) + n) % n; }
+
+void orc_opposite_parity_index(int lx, int ly, int lz, const int bc[3], int* out) {
+  for (int x = 0; x < lx; ++x)
+    for (int y = 0; y < ly; ++y)
+      for (int z = 0; z < lz; ++z)
+        out[((long)x * ly + y) * lz + z] =
+            (wrap_mod(-(x + bc[0]), lx) * ly + wrap_mod(-(y + bc[1]), ly)) * lz +
+            wrap_mod(-(z + bc[2]), lz);
+}
+
+void orc_two_body_wavefunc(double* out, int lx, int ly, int lz, const int bc[3], int type,
+                           int disp1, int disp2, double mass1, double mass2, double lambda,
+                           double cutoff_sq) {
+  const long V = (long)lx * ly * lz;
+  double* e1 = (double*)malloc((size_t)V * sizeof(double));
+  double* e2 = (double*)malloc((size_t)V * sizeof(double));
+  if (type == ORC_WAVEFUNC_GND) {
+    orc_dispersion(e1, lx, ly, lz, bc, disp1, mass1, ORC_CUTOFF_HARD, cutoff_sq);
+    orc_dispersion(e2, lx, ly, lz, bc, disp2, mass2, ORC_CUTOFF_HARD, cutoff_sq);
+  } else if (type >= ORC_WAVEFUNC_TGND) {
+    orc_dispersion(e1, lx, ly, lz, bc, ORC_DISP_QUADRATIC, 1.0, ORC_CUTOFF_HARD, cutoff_sq);
+  }
+  for (long i = 0; i < V; ++i) {
+    double w = 0.0;
+    if (type == ORC_WAVEFUNC_UNIFORM) {
+      w = 1.0 / (double)V;
+    } else if (type == ORC_WAVEFUNC_GND) {
+      const double xi1 = 1.0 + e1[i], xi2 = 1.0 + e2[i];
+      w = 1.0;
+      w /= lambda - 1.0 / (xi1 * xi2);
+    } else if (type == ORC_WAVEFUNC_TGND) {
+      const double psq = 2.0 * e1[i];
+      if (psq > cutoff_sq) {
+        w = 0.0;
+      } else {
+        const double xi = sqrt(psq) / (2.0 * lambda);
+        w = xi < 1e-15 ? 1.0 : orc_dawson(xi) / xi;
+      }
+    } else if (type == ORC_WAVEFUNC_PAIR1) {
+      w = 1.0 / (2.0 * e1[i] + lambda);
+    } else if (type == ORC_WAVEFUNC_PAIR2) {
+      const double psq = 2.0 * e1[i];
+      w = psq < 1e-15 ? 0.0 : exp(-lambda * sqrt(psq)) / psq;
+    }
+    out[i] = w;
+  }
+  free(e1);
+  free(e2);
+}
+
+void orc_pair_overlap(const double* psi1, const double* psi2, const double* w, const int* opp,
+                      double* M, int n1, int n2, long vol) {
+  for (int s = 0; s < n1; ++s)
+    for (int t = 0; t < n2; ++t) {
+      const double* a = psi1 + 2 * (size_t)s * vol;
+      const double* b = psi2 + 2 * (size_t)t * vol;
+      double re = 0.0, im = 0.0;
+      for (long i = 0; i < vol; ++i) {
+        const long k = opp[i];
+        re += (a[2 * i] * b[2 * k] - a[2 * i + 1] * b[2 * k + 1]) * w[i];
+        im += (a[2 * i] * b[2 * k + 1] + a[2 * i + 1] * b[2 * k]) * w[i];
+      }
+      M[2 * ((size_t)s * n2 + t)] = re;
+      M[2 * ((size_t)s * n2 + t) + 1] = im;
+    }
+}
+
+/* =========================================================================================
  * The correlator run (production/slater_det/main.C:185-220 for one configuration)
  * ========================================================================================= */
 static void orc_plane_wave(double* out, int L, const int k[3], int xspace) {
@@ -674,19 +772,22 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
   const int L = d->L, N = d->N, T = d->T;
   const long V = (long)L * L * L;
   const int xspace = d->split == ORC_SPLIT_XSPACE;
-  const int nm = d->minors ? N : 1;
+  const int pair = d->pair_w != NULL;
+  const int nm = pair ? 2 * N : (d->minors ? N : 1);
+  if (pair && (xspace || !d->pair_opp)) return -1;
   double* psi = (double*)malloc(2 * (size_t)N * V * sizeof(double));
   double* sink = (double*)malloc(2 * (size_t)N * V * sizeof(double));
   double* fld = (double*)malloc(2 * (size_t)V * sizeof(double));
   double* lat = (double*)malloc(2 * (size_t)V * sizeof(double));
   double* kv = (double*)malloc((size_t)V * sizeof(double));
   double* M = (double*)malloc(2 * (size_t)N * N * sizeof(double));
+  double* Mp = (double*)malloc(2 * (size_t)N * N * sizeof(double));
   double* sub = (double*)malloc(2 * (size_t)N * N * sizeof(double));
   int* indx = (int*)malloc((size_t)N * sizeof(int));
   int8_t* phi = (int8_t*)malloc((size_t)V);
   uint32_t* words = (uint32_t*)malloc(((size_t)V / 32 + 4) * sizeof(uint32_t));
   int rc = 0;
-  if (!psi || !sink || !fld || !lat || !kv || !M || !sub || !indx || !phi || !words) {
+  if (!psi || !sink || !fld || !lat || !kv || !M || !Mp || !sub || !indx || !phi || !words) {
     rc = -3;
     goto done;
   }
@@ -700,13 +801,21 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
   for (int t = 0; t < T; ++t) {
     /* measure: SlaterDet ctor + Run(n) */
     orc_overlap(sink, psi, M, N, N, V);
-    if (M_out) memcpy(M_out + 2 * (size_t)t * N * N, M, 2 * (size_t)N * N * sizeof(double));
+    if (pair) orc_pair_overlap(psi, psi, d->pair_w, d->pair_opp, Mp, N, N, V);
+    if (M_out)
+      memcpy(M_out + 2 * (size_t)t * N * N, pair ? Mp : M, 2 * (size_t)N * N * sizeof(double));
     for (int m = 0; m < nm; ++m) {
-      const int n = d->minors ? m + 1 : N;
+      const int n = pair ? m / 2 + 1 : (d->minors ? m + 1 : N);
+      const double* src = pair ? Mp : M;
       for (int s = 0; s < n; ++s)
         for (int q = 0; q < n; ++q) {
-          sub[2 * (s * n + q)] = M[2 * (s * N + q)];
-          sub[2 * (s * n + q) + 1] = M[2 * (s * N + q) + 1];
+          sub[2 * (s * n + q)] = src[2 * (s * N + q)];
+          sub[2 * (s * n + q) + 1] = src[2 * (s * N + q) + 1];
+        }
+      if (pair && (m & 1) == 0) /* column n - 1: psi1[s + (n-1) N] = <sink_{n-1}|orbital_s> */
+        for (int s = 0; s < n; ++s) {
+          sub[2 * (s * n + n - 1)] = M[2 * ((n - 1) * N + s)];
+          sub[2 * (s * n + n - 1) + 1] = M[2 * ((n - 1) * N + s) + 1];
         }
       double dd, la = -INFINITY, ph[2] = {0.0, 0.0}, as = 0.0;
       if (orc_ludcmp(sub, n, indx, &dd) == 0) orc_lu_det(sub, n, dd, &la, ph, &as, NULL);
@@ -765,7 +874,7 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
   }
   if (psi_out) memcpy(psi_out, psi, 2 * (size_t)N * V * sizeof(double));
 done:
-  free(psi); free(sink); free(fld); free(lat); free(kv); free(M); free(sub); free(indx);
+  free(psi); free(sink); free(fld); free(lat); free(kv); free(M); free(Mp); free(sub); free(indx);
   free(phi); free(words);
   return rc;
 }

@@ -103,6 +103,28 @@ void orc_lu_det(const double* lu, int n, double d, double* logabs, double phase[
  * src/slater_det/slater_det.C:33-37).  phi: ns x vol, psi: nt x vol, M: ns x nt. */
 void orc_overlap(const double* phi, const double* psi, double* M, int ns, int nt, long vol);
 
+/* ---- the pair sink (src/two_body/two_body.C:21-143; noarch/two_body_run.C:16-43) ---- */
+enum { ORC_WAVEFUNC_NONE = 0, ORC_WAVEFUNC_UNIFORM = 1, ORC_WAVEFUNC_GND = 2,
+       ORC_WAVEFUNC_TGND = 3, ORC_WAVEFUNC_PAIR1 = 4, ORC_WAVEFUNC_PAIR2 = 5 };
+/* Dawson's integral F(x) = exp(-x^2) int_0^x exp(t^2) dt by Rybicki's sampling-theorem
+ * formula with step 0.4 and six odd terms each side, the three-term Taylor series below
+ * |x| = 0.2 (src/utils/special_functions.C:48-78; relative accuracy about 2e-7). */
+double orc_dawson(double x);
+/* out[i] = index of -(n(i) + bc) (src/two_body/two_body.C:31-33). */
+void orc_opposite_parity_index(int lx, int ly, int lz, const int bc[3], int* out);
+/* TwoBody's wave function w(p) for ORC_WAVEFUNC_* (src/two_body/two_body.C:37-128): GND
+ * 1/(lambda - 1/(xi1 xi2)) with xi = 1 + Dispersion(disp, mass, HARD); TGND F(x)/x with
+ * x = |p|/(2 lambda), |p|^2 = 2 Dispersion(QUADRATIC, 1, HARD), 0 beyond the cutoff; PAIR1
+ * 1/(|p|^2 + lambda); PAIR2 exp(-lambda |p|)/|p|^2 (0 at p = 0); UNIFORM 1/V; NONE 0. */
+void orc_two_body_wavefunc(double* out, int lx, int ly, int lz, const int bc[3], int type,
+                           int disp1, int disp2, double mass1, double mass2, double lambda,
+                           double cutoff_sq);
+/* M[s][t] = sum_i w[i] psi1_s[i] psi2_t[opp[i]], no conjugation (TwoBody::Run,
+ * src/two_body/noarch/two_body_run.C:16-43, for every pair (s, t) as
+ * src/slater_det/slater_det3.C:115-122 fills phi[t + N s]).  M: n1 x n2. */
+void orc_pair_overlap(const double* psi1, const double* psi2, const double* w, const int* opp,
+                      double* M, int n1, int n2, long vol);
+
 /* ---- the correlator path ---- */
 enum { ORC_SPLIT_KSPACE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
        ORC_SPLIT_XSPACE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */
@@ -124,11 +146,18 @@ typedef struct {
   uint32_t config;     /* PHILOX config index */
   orc_ranlxs* rng;     /* RANLXS: consumed sequentially, T*V draws */
   const int8_t* phi;   /* ARRAY: T x V values in {-1, +1} */
+  const double* pair_w; /* optional V real pair wave function (k-space split only): the
+                           measurement is SlaterDet3's (production/slater_det3_trapped/
+                           main-cum.C:217-238), 2N determinants per slice */
+  const int* pair_opp;  /* V opposite-parity indices for pair_w */
 } orc_run_desc;
 
 /* Runs one configuration through T slices, measuring before evolving at each t
  * (production/slater_det/main.C:185-220).  Outputs, for each t and each measured minor m
- * (m = 0 only, or m = 0..N-1 for minor size m+1), at [t*nm + m]:
+ * (m = 0 only, or m = 0..N-1 for minor size m+1; with pair_w m = 0..2N-1: at 2n + 1 the
+ * leading (n+1) x (n+1) minor of the pair matrix phi[s][t], at 2n the same minor with
+ * column n replaced by the one-body overlaps <sink_n | orbital_s>, SlaterDet3::Run(n+1, -1,
+ * -1) and Run(n+1, -1, n), src/slater_det/slater_det3.C:140-210), at [t*nm + m]:
  *   logabs, phase (2 doubles), argsum.  M_out (optional): T x N x N complex overlaps.
  * psi_out (optional): final N x V orbitals in storage space. */
 int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double* argsum,
