@@ -83,6 +83,17 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf_dev, long vol, int prec);
 int nrc_overlap(nrc_context* ctx, const void* phi_dev, const void* psi_dev, double* M_dev,
                 int ns, int nt, long vol, int ncfg, int mode);
 
+/* ---- the pair-sink matrix: TwoBody::Run(Float*, Float*) and Run(Float**, Float**,
+ *      complex<Float>*, int) (include/two_body.h:31-35; src/two_body/noarch/two_body_run.C:
+ *      16-43; src/two_body/cuda/two_body_run.cu) for ncfg batches:
+ *      M_dev[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]], no conjugation.
+ *      psi1: ncfg x n1 x vol, psi2: ncfg x n2 x vol complex (FP64 for NRC_OVERLAP_F64, FP32
+ *      otherwise); w: vol doubles (the wave function); opp: vol ints, the site of -(n + bc)
+ *      (src/two_body/two_body.C:31-33); all device.  Complex FP64 output. */
+int nrc_pair_overlap(nrc_context* ctx, const void* psi1_dev, const void* psi2_dev,
+                     const double* w_dev, const int* opp_dev, double* M_dev, int n1, int n2,
+                     long vol, int ncfg, int mode);
+
 /* ---- batched LU log-determinant: ludcmp (include/ludcmp.h:5; src/utils/ludcmp.C:9-65) and
  *      SlaterDet::Run's det / log det (src/slater_det/slater_det.C:59-115).  For each of
  *      ncfg row-major n x n complex matrices (row stride ldm) and, if minors, each leading
@@ -129,6 +140,14 @@ typedef struct nrc_ensemble_desc {
                           overlaps are dense products (split-BF16x3 on the matrix cores for
                           FP32, FP64 otherwise) instead of the plane-wave gather; single-pass
                           path only. */
+  const double* pair_w; /* optional V real pair wave function w(p) (k-space split, single-pass
+                           path).  When set, each measurement is SlaterDet3's
+                           (production/slater_det3_trapped/main-cum.C:217-238): the pair
+                           matrix phi[s][t] = sum_p w(p) psi_s(p) psi_t(opp(p)) (TwoBody::Run)
+                           and 2N results per configuration and slice: m = 2n + 1 the leading
+                           (n+1) x (n+1) minor of phi, m = 2n the same with column n replaced by
+                           the one-body overlaps <sink_n | orbital_s>. */
+  const int* pair_opp;  /* V opposite-parity sites for pair_w (src/two_body/two_body.C:31-33) */
 } nrc_ensemble_desc;
 
 int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* desc, nrc_ensemble** ens);

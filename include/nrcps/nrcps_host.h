@@ -108,6 +108,7 @@ class Arg {
   int ReadLine(DetType&);
   int ReadLine(InteractionType&);
   int ReadLine(IoType&);
+  int ReadLine(WavefuncType&);
 
  private:
   std::ifstream file;
@@ -187,6 +188,14 @@ class KineticArg : public Arg {
   NRC_ARG_CTORS(KineticArg)
   DispersionType dispersion_type;
   Float mass;
+};
+class TwoBodyArg : public Arg {
+ public:
+  NRC_ARG_CTORS(TwoBodyArg)
+  WavefuncType wavefunc_type;
+  DispersionType dispersion_type1, dispersion_type2;
+  Float mass1, mass2, lambda;
+  std::string file_stem;
 };
 
 // ------------------------------------------------------------------ globals
@@ -474,10 +483,39 @@ class OneBody {
   int vol;
 };
 
+// ------------------------------------------------------------------ special functions
+// Dawson's integral (src/utils/special_functions.C:48-78): Rybicki's sampling-theorem
+// formula, single-precision accurate (~2e-7), as the reference evaluates it.
+Float DawsonF(Float x);
+
+// ------------------------------------------------------------------ pair sink
+// The two-particle sink <pair| of SlaterDet / SlaterDet2 / SlaterDet3 (include/two_body.h:
+// 11-36): the wave function w(p) and its pair matrix element sum_p w(p) G1(p) G2(-(p + bc)).
+class TwoBody {
+ public:
+  explicit TwoBody(const TwoBodyArg&);
+  ~TwoBody();
+  void Deform(std::string);  // adds "k1 k2 k3 re" lines of a file to w
+  std::complex<Float> Run(Float*, Float*);  // device pointers
+  // phi[t + n s] = Run(props1[s], props2[t]) for s, t < n (device pointers)
+  void Run(Float** props1, Float** props2, std::complex<Float>* phi, int n);
+
+ private:
+  TwoBody(const TwoBody&);
+  TwoBody& operator=(const TwoBody&);
+  void Upload();
+  int vol;
+  std::vector<double> wavefunc;
+  std::vector<int> opposite_parity_index;
+  double* dev_w;
+  int* dev_opp;
+};
+
 // ------------------------------------------------------------------ Slater determinants
 class SlaterDet {
  public:
   explicit SlaterDet(Float** props, OneBody** one_body, int n);
+  explicit SlaterDet(Float** props1, Float** props2, TwoBody* two_body, int n);
   ~SlaterDet();
   std::complex<Float> Run(int n);
   std::complex<Float> SlaterMat(int s, int t) { return slater_mat[t + mat_size * s]; }
@@ -489,6 +527,45 @@ class SlaterDet {
   std::vector<std::complex<Float> > slater_mat;
   int mat_size;
   DetType det_type;
+};
+
+// A pair-sink Slater matrix phi[t + N s] = <pair | props1_s props2_t> with the optional
+// one-body matrices psi1[t + N s] = <one_body1_s | props1_t> and psi2 likewise, and the
+// determinant of the leading n x n minor with row s0 (from psi2) or column t0 (from psi1)
+// struck out and replaced (src/slater_det/slater_det2.C; slater_det3.C).  SlaterDet2 fills
+// the matrices in its constructor, SlaterDet3 in Initialize(two_body).
+class SlaterDet2 {
+ public:
+  explicit SlaterDet2(Float** props1, Float** props2, OneBody** one_body1, OneBody** one_body2,
+                      TwoBody* two_body, int n);
+  ~SlaterDet2();
+  std::complex<Float> Run(int n, int s0, int t0);
+  std::complex<Float> SlaterMat(int s, int t) { return phi[t + N * s]; }
+  void SetDetType(DetType t) { det_type = t; }
+
+ protected:
+  SlaterDet2(int n);
+  void Fill(Float** props1, Float** props2, OneBody** one_body1, OneBody** one_body2,
+            TwoBody* two_body);
+  int N;
+  DetType det_type;
+  std::vector<std::complex<Float> > phi, psi1, psi2;
+  bool has_psi1, has_psi2;
+
+ private:
+  SlaterDet2(const SlaterDet2&);
+  SlaterDet2& operator=(const SlaterDet2&);
+};
+
+class SlaterDet3 : public SlaterDet2 {
+ public:
+  explicit SlaterDet3(Float** props1, Float** props2, OneBody** one_body1, OneBody** one_body2,
+                      int n);
+  void Initialize(TwoBody* two_body);
+
+ private:
+  std::vector<Float*> props1, props2;
+  std::vector<OneBody*> one_body1, one_body2;
 };
 
 // ------------------------------------------------------------------ LU (ludcmp.h)

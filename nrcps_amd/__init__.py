@@ -28,7 +28,7 @@ EXPORTS = [
     "nrc_memcpy_h2d", "nrc_memcpy_d2h", "nrc_memcpy_d2d", "nrc_memset", "nrc_sync",
     "nrc_fft_plan", "nrc_fft_exec", "nrc_fft_destroy", "nrc_scale_real", "nrc_mul_complex",
     "nrc_plus_one_div_vol",
-    "nrc_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
+    "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -49,7 +49,7 @@ class EnsembleDesc(C.Structure):
     _fields_ = [("L", C.c_int), ("n_orb", C.c_int), ("n_configs", C.c_int), ("prec", C.c_int),
                 ("split", C.c_int), ("minors", C.c_int), ("moms", C.c_void_p),
                 ("kin", C.c_void_p), ("A", C.c_double), ("smear", C.c_void_p),
-                ("sinks", C.c_void_p)]
+                ("sinks", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p)]
 
 
 _lib = None
@@ -76,6 +76,7 @@ def load_library() -> C.CDLL:
         "nrc_mul_complex": [vp, vp, vp, i, C.c_long, i],
         "nrc_plus_one_div_vol": [vp, vp, C.c_long, i],
         "nrc_overlap": [vp, vp, vp, vp, i, i, C.c_long, i, i],
+        "nrc_pair_overlap": [vp, vp, vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_lu_det": [vp, vp, i, i, i, i, vp, vp, vp, vp],
         "nrc_ludcmp": [vp, vp, i, i, vp, vp, vp], "nrc_ref_abs": [vp, vp, vp, C.c_long],
         "nrc_z2_bits": [vp, u64, u32, i, u32, u32, i, vp],
@@ -198,6 +199,11 @@ class Context:
         _check(self.lib.nrc_overlap(self.h, phi.ptr, psi.ptr, M.ptr, ns, nt, vol, ncfg, mode),
                "overlap")
 
+    def pair_overlap(self, psi1, psi2, w, opp, M, n1, n2, vol, ncfg, mode):
+        """nrc_pair_overlap: M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]."""
+        _check(self.lib.nrc_pair_overlap(self.h, psi1.ptr, psi2.ptr, w.ptr, opp.ptr, M.ptr, n1,
+                                         n2, vol, ncfg, mode), "pair_overlap")
+
     def lu_det(self, M, n, ncfg, minors=False, ldm=None):
         nm = n if minors else 1
         la = self.array((ncfg, nm), np.float64)
@@ -241,10 +247,14 @@ class Ensemble:
     """The batched production slice loop (nrc_ensemble_*)."""
 
     def __init__(self, ctx: Context, L, moms, kin, A, n_configs, prec=PREC_F32,
-                 split=SPLIT_KSPACE, minors=False, smear=None, sinks=None):
+                 split=SPLIT_KSPACE, minors=False, smear=None, sinks=None, pair_w=None,
+                 pair_opp=None):
+        """pair_w / pair_opp: the pair-sink measurement (2N results per slice, SlaterDet3's:
+        m = 2n + 1 leading (n+1) minor of the pair matrix, m = 2n with column n from the
+        one-body overlaps)."""
         self.ctx, self.lib = ctx, ctx.lib
         self.L, self.N, self.C, self.prec = L, len(moms), n_configs, prec
-        self.nm = self.N if minors else 1
+        self.nm = 2 * self.N if pair_w is not None else (self.N if minors else 1)
         self._moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)
         self._kin = np.ascontiguousarray(kin, dtype=np.float64)
         d = EnsembleDesc()
@@ -259,6 +269,10 @@ class Ensemble:
         if sinks is not None:
             self._sinks = np.ascontiguousarray(sinks, dtype=np.complex128).reshape(-1)
             d.sinks = self._sinks.ctypes.data
+        if pair_w is not None:
+            self._pair = (np.ascontiguousarray(pair_w, dtype=np.float64),
+                          np.ascontiguousarray(pair_opp, dtype=np.int32))
+            d.pair_w, d.pair_opp = self._pair[0].ctypes.data, self._pair[1].ctypes.data
         h = C.c_void_p()
         _check(self.lib.nrc_ensemble_create(ctx.h, C.byref(d), C.byref(h)), "ensemble_create")
         self.h = h

@@ -246,6 +246,27 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
   return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream);
 }
 
+int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const double* w,
+                     const int* opp, double* M, int n1, int n2, long vol, int ncfg, int mode) {
+  if (!ctx || !psi1 || !psi2 || !w || !opp || !M || n1 < 1 || n2 < 1 || vol < 1 || ncfg < 0)
+    return NRC_ERR_ARG;
+  if (ncfg == 0) return NRC_OK;
+  if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {
+    const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
+    NRC_HIP(nrc::launch_pair_dense(prec, psi1, psi2, w, opp, (double2*)M, n1, n2, vol, ncfg,
+                                   ctx->stream));
+    return NRC_OK;
+  }
+  if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
+  if (n1 > 64 || n2 > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
+  const size_t bytes = nrc::pair_bf16x3_partial_floats(n1, n2, vol, ncfg) * sizeof(float);
+  void* work = ctx_workspace(ctx, bytes);
+  if (!work) return NRC_ERR_NOMEM;
+  NRC_HIP(nrc::launch_pair_bf16x3((const float2*)psi1, (const float2*)psi2, w, opp, (double2*)M,
+                                  n1, n2, vol, ncfg, (float*)work, ctx->stream));
+  return NRC_OK;
+}
+
 int nrc_lu_det(nrc_context* ctx, const double* M, int n, int ldm, int ncfg, int minors,
                double* logabs, double* phase, double* argsum, int* status) {
   if (!ctx || !M || !logabs || !phase || !argsum || n < 1 || ldm < n || ncfg < 0)
@@ -303,6 +324,11 @@ struct nrc_ensemble {
   void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
   void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
   void* ovl_work = nullptr;  // workspace of the split-BF16x3 overlaps of dense FP32 sinks
+  bool pair = false;         // pair-sink measurement (SlaterDet3)
+  double* pw = nullptr;      // pair: w(p) for the unfolded state, then w(p) K(1/2)(p) K(1/2)(opp p)
+  int* popp = nullptr;       // pair: opposite-parity sites
+  double2* Mp = nullptr;     // pair: [C][N][N] pair matrices
+  float* pair_work = nullptr;  // pair, FP32: split-BF16x3 partial sums
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
@@ -329,7 +355,8 @@ struct nrc_ensemble {
 };
 
 static void ens_free(nrc_ensemble* e) {
-  void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->sinks_k, e->tab_yz, e->smear,
+  void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->pw, e->popp, e->Mp,
+                  e->pair_work, e->sinks_k, e->tab_yz, e->smear,
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
@@ -378,6 +405,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   if (d->split != NRC_SPLIT_KSPACE && d->split != NRC_SPLIT_XSPACE) return NRC_ERR_ARG;
   if (!nrc::pass_supported(d->L) || d->n_orb > 64) return NRC_ERR_UNSUPPORTED;
   if (d->smear && d->split != NRC_SPLIT_KSPACE) return NRC_ERR_ARG;
+  if (d->pair_w && (d->split != NRC_SPLIT_KSPACE || !d->pair_opp)) return NRC_ERR_ARG;
   const int L = d->L, N = d->n_orb;
   for (int i = 0; i < 3 * N; ++i)
     if (d->moms[i] < 0 || d->moms[i] >= L) return NRC_ERR_ARG;
@@ -388,12 +416,15 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   e->d = *d;
   e->V = (long)L * L * L;
   e->nw = (int)((e->V + 31) / 32);
-  e->nm = d->minors ? N : 1;
+  e->pair = d->pair_w != nullptr;
+  e->nm = e->pair ? 2 * N : (d->minors ? N : 1);
   e->moms.assign(d->moms, d->moms + 3 * N);
   e->d.moms = e->moms.data();
   e->d.kin = nullptr;
   e->d.smear = nullptr;
   e->d.sinks = nullptr;
+  e->d.pair_w = nullptr;
+  e->d.pair_opp = nullptr;
   const long V = e->V;
   const long C = d->n_configs;
   int rc = NRC_OK;
@@ -513,6 +544,27 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
         ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 4, hipMemcpyHostToDevice) ==
                         hipSuccess ? NRC_OK : NRC_ERR_HIP);
       }
+    }
+  }
+  if (e->pair) {  // the pair wave function, unfolded (t = 0) and with K(1/2) on both momenta
+    for (long i = 0; i < V; ++i)
+      if (d->pair_opp[i] < 0 || d->pair_opp[i] >= V) ENS_TRY(NRC_ERR_ARG);
+    std::vector<double> w(2 * V);
+    for (long i = 0; i < V; ++i) {
+      w[i] = d->pair_w[i];
+      w[V + i] = e->khalf.empty() ? d->pair_w[i]
+                                  : d->pair_w[i] * e->khalf[i] * e->khalf[d->pair_opp[i]];
+    }
+    ENS_TRY(dev_alloc(&e->pw, 2 * V));
+    ENS_TRY(dev_alloc(&e->popp, V));
+    ENS_TRY(dev_alloc(&e->Mp, (size_t)C * N * N));
+    ENS_TRY(hipMemcpy(e->pw, w.data(), 2 * V * sizeof(double), hipMemcpyHostToDevice) ==
+                    hipSuccess ? NRC_OK : NRC_ERR_HIP);
+    ENS_TRY(hipMemcpy(e->popp, d->pair_opp, V * sizeof(int), hipMemcpyHostToDevice) ==
+                    hipSuccess ? NRC_OK : NRC_ERR_HIP);
+    if (d->prec == NRC_PREC_F32) {
+      if (V % 4) ENS_TRY(NRC_ERR_UNSUPPORTED);
+      ENS_TRY(dev_alloc(&e->pair_work, nrc::pair_bf16x3_partial_floats(N, N, V, (int)C)));
     }
   }
   if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
@@ -701,8 +753,40 @@ static int ens_set_sources(nrc_ensemble* e) {
 // ---- one slice of the single-pass path, in two halves (the loop of ens_enqueue, and the
 // step entry points that let a caller choose each slice's field after seeing the measurement)
 
+// LU log-dets of the measurement M (the one-body overlaps [C][N][N]) into the results row
+// `row`: the full matrix or its leading minors; in pair mode SlaterDet3's 2N determinants
+// (production/slater_det3_trapped/main-cum.C:217-238), m = 2n + 1 from the pair matrices
+// e->Mp, m = 2n the same minor with column n taken from M.  FP32 sources carry the
+// amplitude r_c: r_c^2 per pair column, r_c per one-body column.
+static int ens_lu(nrc_ensemble* e, const double2* M, size_t row) {
+  const int N = e->d.n_orb;
+  const int C = (int)e->d.n_configs;
+  nrc::LuOpts o;
+  o.lsrc = e->lsrc;
+  if (!e->pair) {
+    NRC_HIP(nrc::launch_lu_det(M, N, N, C, e->d.minors, e->o_la + row, e->o_ph + row,
+                               e->o_as + row, e->o_st + row, nullptr, nullptr, nullptr, e->stream,
+                               o));
+    return NRC_OK;
+  }
+  o.lsrc_cols = 2;
+  o.onm = 2 * N;
+  o.ostep = 2;
+  o.oofs = 1;
+  NRC_HIP(nrc::launch_lu_det(e->Mp, N, N, C, 1, e->o_la + row, e->o_ph + row, e->o_as + row,
+                             e->o_st + row, nullptr, nullptr, nullptr, e->stream, o));
+  o.oofs = 0;
+  o.repl = M;
+  o.ldr = N;
+  NRC_HIP(nrc::launch_lu_det(e->Mp, N, N, C, 1, e->o_la + row, e->o_ph + row, e->o_as + row,
+                             e->o_st + row, nullptr, nullptr, nullptr, e->stream, o));
+  return NRC_OK;
+}
+
 // measurement at t into the results row t: overlaps (gathered plane-wave sinks or dense
-// sinks) and the LU log-det (production/slater_det/main.C:206-215)
+// sinks) and the LU log-det (production/slater_det/main.C:206-215); in pair mode also the
+// pair matrices of the stored orbitals (TwoBody::Run for every (s, t),
+// src/slater_det/slater_det3.C:115-122)
 static int ens_measure(nrc_ensemble* e, int t) {
   hipStream_t s = e->stream;
   const int N = e->d.n_orb, prec = e->d.prec;
@@ -719,12 +803,18 @@ static int ens_measure(nrc_ensemble* e, int t) {
     else
       NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb,
                                        e->folded ? e->sink_k : nullptr, e->M, C, N, V, s));
+    if (e->pair) {
+      const double* w = e->pw + (e->folded ? V : 0);
+      if (prec == NRC_PREC_F64)
+        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, e->Mp, N, N, V,
+                                       (int)C, s));
+      else
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+                                        e->popp, e->Mp, N, N, V, (int)C, e->pair_work, s));
+    }
   }
   EvScope sc(e, 2);
-  NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
-                             e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                             nullptr, nullptr, nullptr, s, e->lsrc));
-  return NRC_OK;
+  return ens_lu(e, e->M, t * per_t);
 }
 
 // evolution of slice t with the fields words[c][nw] (device), single-pass kernel; for the
@@ -773,9 +863,7 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
   e->folded = ref;
   if (!ref && !e->sinks) {  // measured inside the slice
     EvScope sc(e, 2);
-    NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
-                               e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                               nullptr, nullptr, nullptr, s, e->lsrc));
+    NRC_TRY(ens_lu(e, e->M, t * per_t));
   }
   return NRC_OK;
 }
@@ -815,10 +903,7 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
   e->folded = true;
   for (int k = 1; k < S; ++k) {
     EvScope sc(e, 2);
-    const size_t row = (size_t)(t + k) * per_t;
-    NRC_HIP(nrc::launch_lu_det(e->Mx + (size_t)(k - 1) * mstride, N, N, (int)C, e->d.minors,
-                               e->o_la + row, e->o_ph + row, e->o_as + row, e->o_st + row,
-                               nullptr, nullptr, nullptr, s, e->lsrc));
+    NRC_TRY(ens_lu(e, e->Mx + (size_t)(k - 1) * mstride, (size_t)(t + k) * per_t));
   }
   return NRC_OK;
 }
@@ -861,10 +946,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   };
   auto lu = [&](int t) -> int {
     EvScope sc(e, 2);
-    NRC_HIP(nrc::launch_lu_det(e->M, N, N, (int)C, e->d.minors, e->o_la + t * per_t,
-                               e->o_ph + t * per_t, e->o_as + t * per_t, e->o_st + t * per_t,
-                               nullptr, nullptr, nullptr, s, e->lsrc));
-    return NRC_OK;
+    return ens_lu(e, e->M, t * per_t);
   };
   nrc::MulSpec fieldspec;
   fieldspec.kind = nrc::MUL_FIELD;
@@ -873,11 +955,12 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   const bool single = e->path == NRC_PATH_SINGLE_PASS ||
                       (e->path == NRC_PATH_AUTO && nrc::slice1_supported(prec, L));
   // single pass: every slice is one kernel over the orbitals (slice_fused.hip)
-  if (e->sinks && !single) return NRC_ERR_UNSUPPORTED;  // dense sinks read the stored state
+  // dense sinks and pair matrices read the stored state
+  if ((e->sinks || e->pair) && !single) return NRC_ERR_UNSUPPORTED;
   const bool measure_first = ref || e->sinks;
   // k-space split with the plain field: up to SLICE_FUSE_MAX slices per launch of the
   // persistent kernel, the measurements between them gathered inside it
-  const bool fuse = single && ref && !e->smear && !e->sinks;
+  const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair;
   for (int t = 0; single && t < T;) {
     const int S = fuse ? std::min(nrc::SLICE_FUSE_MAX, T - t) : 1;
     if (measure_first) NRC_TRY(ens_measure(e, t));

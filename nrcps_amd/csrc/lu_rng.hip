@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
                                                int ncfg, int minors, double* logabs,
                                                double2* phase, double* argsum, int* status,
                                                double2* lu_out, int* indx_out, double* d_out,
-                                               const double* lsrc) {
+                                               LuOpts opt) {
   extern __shared__ double2 a[];
   const int nm = minors ? n : 1;
   const long blk = blockIdx.x;
@@ -129,8 +129,10 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   const int sz = minors ? m + 1 : n;
   const int i = threadIdx.x;
   const double2* Mc = M + cfg * (long)n * ldm;
-  if (i < sz)
+  if (i < sz) {
     for (int j = 0; j < sz; ++j) a[j * sz + i] = Mc[(long)i * ldm + j];
+    if (opt.repl) a[(sz - 1) * sz + i] = opt.repl[cfg * (long)n * opt.ldr + (long)(sz - 1) * opt.ldr + i];
+  }
   __syncthreads();
   double vv = 0.0;
   int bad = 0;
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
       __syncthreads();
     }
   }
-  const long o = cfg * nm + m;
+  const long o = cfg * (opt.onm ? opt.onm : nm) + (long)m * opt.ostep + opt.oofs;
   if (lu_out && i < sz) {
     double2* out = lu_out + cfg * (long)n * ldm;
     for (int j = 0; j < sz; ++j) out[(long)i * ldm + j] = a[j * sz + i];
@@ -204,7 +206,8 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   }
   if (i == 0) {
     if (status) status[o] = bad ? 1 : 0;
-    if (logabs) logabs[o] = bad ? -INFINITY : (lsrc ? la - sz * lsrc[cfg] : la);
+    const double ncol = opt.lsrc_cols * sz - (opt.repl ? opt.lsrc_cols - 1 : 0);
+    if (logabs) logabs[o] = bad ? -INFINITY : (opt.lsrc ? la - ncol * opt.lsrc[cfg] : la);
     if (phase) phase[o] = bad ? make_double2(0.0, 0.0) : make_double2(dsign * ph.x, dsign * ph.y);
     if (argsum) argsum[o] = bad ? 0.0 : as + (dsign < 0.0 ? 3.14159265358979323846 : 0.0);
   }
@@ -229,12 +232,12 @@ hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s) 
 hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
                          double* logabs, double2* phase, double* argsum, int* status,
                          double2* lu_out, int* indx_out, double* d_out, hipStream_t s,
-                         const double* lsrc) {
+                         const LuOpts& opts) {
   if (n < 1 || n > 64) return hipErrorInvalidValue;
   const long blocks = (long)ncfg * (minors ? n : 1);
   const size_t shm = (size_t)n * n * sizeof(double2);
   hipLaunchKernelGGL(k_lu_det, dim3((unsigned)blocks), dim3(64), shm, s, M, n, ldm, ncfg,
-                     minors, logabs, phase, argsum, status, lu_out, indx_out, d_out, lsrc);
+                     minors, logabs, phase, argsum, status, lu_out, indx_out, d_out, opts);
   return hipGetLastError();
 }
 
@@ -285,6 +288,72 @@ __global__ __launch_bounds__(256) void k_overlap_dense(const C2<T>* __restrict__
     const double y = red[q][1][0] + red[q][1][1] + red[q][1][2] + red[q][1][3];
     M[(c * ns + s0 + q) * nt + t] = make_double2(x, y);
   }
+}
+
+// The pair-sink matrix of TwoBody::Run (src/two_body/noarch/two_body_run.C:16-43, for every
+// (s, t) as src/slater_det/slater_det3.C:115-122 fills phi[t + N s]):
+//   M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]   (no conjugation)
+// FP64 products and sums; one workgroup per (c, t) and 16 rows: column t's weighted mirror
+// image w[i] psi2_t[opp[i]] is formed once per element and meets the 16 rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pair_dense(const C2<T>* __restrict__ psi1,
+                                                    const C2<T>* __restrict__ psi2,
+                                                    const double* __restrict__ w,
+                                                    const int* __restrict__ opp, double2* M,
+                                                    int n1, int n2, long vol) {
+  constexpr int CH = 16;
+  __shared__ double red[CH][2][4];
+  const long ct = blockIdx.x;  // c * n2 + t
+  const long c = ct / n2;
+  const int t = (int)(ct - c * n2);
+  const int s0 = blockIdx.y * CH;
+  const C2<T>* b2 = psi2 + ct * vol;
+  const C2<T>* a1 = psi1 + c * n1 * vol;
+  double ar[CH], ai[CH];
+#pragma unroll
+  for (int q = 0; q < CH; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
+  for (long i = threadIdx.x; i < vol; i += 256) {
+    const C2<T> u = b2[opp[i]];
+    const double br = w[i] * (double)u.x, bi = w[i] * (double)u.y;
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      if (s0 + q < n1) {
+        const C2<T> a = a1[(long)(s0 + q) * vol + i];
+        ar[q] += (double)a.x * br - (double)a.y * bi;
+        ai[q] += (double)a.x * bi + (double)a.y * br;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    double x = ar[q], y = ai[q];
+    for (int off = 32; off >= 1; off >>= 1) {
+      x += __shfl_xor(x, off);
+      y += __shfl_xor(y, off);
+    }
+    if (lane == 0) { red[q][0][wv] = x; red[q][1][wv] = y; }
+  }
+  __syncthreads();
+  if (threadIdx.x < CH && s0 + (int)threadIdx.x < n1) {
+    const int q = threadIdx.x;
+    const double x = red[q][0][0] + red[q][0][1] + red[q][0][2] + red[q][0][3];
+    const double y = red[q][1][0] + red[q][1][1] + red[q][1][2] + red[q][1][3];
+    M[(c * n1 + s0 + q) * n2 + t] = make_double2(x, y);
+  }
+}
+
+hipError_t launch_pair_dense(int prec, const void* psi1, const void* psi2, const double* w,
+                             const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
+                             hipStream_t s) {
+  dim3 grid((unsigned)((long)ncfg * n2), (unsigned)((n1 + 15) / 16));
+  if (prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_pair_dense<double>, grid, dim3(256), 0, s, (const C2<double>*)psi1,
+                       (const C2<double>*)psi2, w, opp, M, n1, n2, vol);
+  else
+    hipLaunchKernelGGL(k_pair_dense<float>, grid, dim3(256), 0, s, (const C2<float>*)psi1,
+                       (const C2<float>*)psi2, w, opp, M, n1, n2, vol);
+  return hipGetLastError();
 }
 
 hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,

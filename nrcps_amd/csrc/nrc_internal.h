@@ -105,12 +105,28 @@ hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvo
 hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
                           uint32_t stream, int vol, uint32_t* words, hipStream_t s);
 
-// batched LU log-determinant of M[c][n][n] (leading minors when minors != 0); with lsrc,
-// log|det| of a k x k minor of configuration c is reduced by k lsrc[c]
+// Options of the batched LU (launch_lu_det), all off by default:
+//   repl, ldr: the last column of every minor (size k) is row k - 1 of repl[c] (a matrix of
+//     ldr columns per row, n rows per configuration) read as a column, i.e. SlaterDet3::
+//     Run(k, -1, k - 1) with psi1[s + (k-1) N] (src/slater_det/slater_det3.C:163-167);
+//   lsrc, lsrc_cols: log|det| of a k x k minor of configuration c is reduced by
+//     (lsrc_cols k - (repl ? lsrc_cols - 1 : 0)) lsrc[c]: lsrc_cols source amplitudes per
+//     column, one in a replaced column;
+//   onm, ostep, oofs: the result of minor m of configuration c goes to
+//     c onm + m ostep + oofs (onm 0: the number of minors).
+struct LuOpts {
+  const double2* repl = nullptr;
+  int ldr = 0;
+  const double* lsrc = nullptr;
+  int lsrc_cols = 1;
+  int onm = 0, ostep = 1, oofs = 0;
+};
+
+// batched LU log-determinant of M[c][n][n] (leading minors when minors != 0)
 hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
                          double* logabs, double2* phase, double* argsum, int* status,
                          double2* lu_out, int* indx_out, double* d_out, hipStream_t s,
-                         const double* lsrc = nullptr);
+                         const LuOpts& opts = LuOpts());
 
 // Amplitude r_c in [1, 2) of configuration c's FP32 sources: 23 hashed mantissa bits, so r_c
 // is exact in FP32.  The first slice acts on sources that are the same in every
@@ -138,6 +154,16 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
 // phi_split: bf16 hi[ns][2V] followed by lo[ns][2V]
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
+
+// pair-sink matrices M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]: FP64 sums
+// (dense), or split-BF16x3 on the matrix cores (FP32 orbitals, weights rounded to FP32)
+hipError_t launch_pair_dense(int prec, const void* psi1, const void* psi2, const double* w,
+                             const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
+                             hipStream_t s);
+hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const double* wf,
+                              const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
+                              float* partial, hipStream_t s);
+size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg);
 
 // the context's stream and device (api.hip), for entry points defined in other files
 hipStream_t ctx_stream(nrc_context* c);

@@ -164,6 +164,120 @@ __global__ __launch_bounds__(256) void k_overlap_bf16x3(const __bf16* __restrict
     }
 }
 
+// The pair-sink matrix M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]] (TwoBody::Run,
+// src/two_body/noarch/two_body_run.C:16-43) as the same split-BF16x3 GEMM: A rows are the
+// configuration's own psi1 orbitals, split as they are read; B column 2t holds
+// (b_re, -b_im) and column 2t + 1 (b_im, b_re) of b = w psi2_t(opp), so the real dot
+// products give Re and Im of the unconjugated complex product.  The weights are rounded to
+// FP32 as they are read.
+template <int MT, int NCT>
+__global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ psi1,
+                                                     const float2* __restrict__ psi2,
+                                                     const double* __restrict__ wf,
+                                                     const int* __restrict__ opp,
+                                                     float* __restrict__ partial, int n1,
+                                                     int n2, long vol, int mt, int nc_t,
+                                                     long kchunk) {
+  __shared__ float red[OV_WAVES - 1][4][64];
+  const int chunk = blockIdx.x, nchunks = gridDim.x;
+  const long c = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, grp = lane >> 4;
+  const long K = 2 * vol;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  f32x4 acc[MT][NCT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const float2* a1 = psi1 + c * n1 * vol;
+  const float2* b2 = psi2 + c * n2 * vol;
+  for (long kb = k0 + (long)wave * OV_KSTEP; kb < k1; kb += OV_WAVES * OV_KSTEP) {
+    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements ka/2 .. ka/2 + 3
+    const bool in = ka < K;
+    bf16x8 ah[MT], al[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+      if (a < mt) {
+        const int s = a * 16 + row;
+        float v[8];
+        if (s < n1 && in) {
+          const float4* p = reinterpret_cast<const float4*>(a1 + (long)s * vol + ka / 2);
+          const float4 u0 = p[0], u1 = p[1];
+          v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
+          v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = 0.f;
+        }
+        split8(v, ah[a], al[a]);
+      }
+    }
+    int site[4];
+    float wt[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      site[q] = in ? opp[ka / 2 + q] : 0;
+      wt[q] = in ? (float)wf[ka / 2 + q] : 0.f;
+    }
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) {
+      if (b < nc_t) {
+        const int col = b * 16 + row, t = col >> 1, part = col & 1;
+        float v[8];
+        if (t < n2 && in) {
+          const float2* p = b2 + (long)t * vol;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float2 u = p[site[q]];
+            const float br = wt[q] * u.x, bi = wt[q] * u.y;
+            v[2 * q] = part ? bi : br;
+            v[2 * q + 1] = part ? br : -bi;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = 0.f;
+        }
+        bf16x8 bh, bl;
+        split8(v, bh, bl);
+#pragma unroll
+        for (int a = 0; a < MT; ++a) {
+          if (a < mt) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  const int ldp = nc_t * 16;
+  float* out = partial + (c * nchunks + chunk) * (long)(mt * 16) * ldp;
+#pragma unroll
+  for (int a = 0; a < MT; ++a)
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) {
+      if (a < mt && b < nc_t) {  // uniform over the workgroup
+        if (wave > 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = acc[a][b][r];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float x = acc[a][b][r];
+#pragma unroll
+            for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
+            out[(long)(a * 16 + grp * 4 + r) * ldp + b * 16 + row] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+}
+
 // M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
 __global__ void k_overlap_reduce(const float* partial, double2* M, int ns, int nt, int nchunks,
                                  int mt, int nc_t, long ncfg) {
@@ -228,6 +342,41 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, ns,
                      nt, nchunks, mt, nc_t, (long)ncfg);
+  return hipGetLastError();
+}
+
+size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg) {
+  return overlap_bf16x3_partial_floats(n1, n2, vol, ncfg);
+}
+
+hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const double* wf,
+                              const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
+                              float* partial, hipStream_t s) {
+  if (n1 < 1 || n1 > 16 * OV_MAXT || n2 < 1 || 2 * n2 > 16 * OV_MAXN || (vol & 3))
+    return hipErrorInvalidValue;
+  const int mt = (n1 + 15) / 16, nc_t = (2 * n2 + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const int nchunks = (int)((K + kc - 1) / kc);
+  const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
+#define NRC_PR_CASE(A, B)                                                                     \
+  if (mt == A && nc_t == B) {                                                                 \
+    hipLaunchKernelGGL((k_pair_bf16x3<A, B>), grid, block, 0, s, psi1, psi2, wf, opp, partial, \
+                       n1, n2, vol, mt, nc_t, kc);                                            \
+  }
+#define NRC_PR_ROW(A) \
+  NRC_PR_CASE(A, 1) NRC_PR_CASE(A, 2) NRC_PR_CASE(A, 3) NRC_PR_CASE(A, 4) NRC_PR_CASE(A, 5) \
+  NRC_PR_CASE(A, 6) NRC_PR_CASE(A, 7) NRC_PR_CASE(A, 8)
+  NRC_PR_ROW(1) NRC_PR_ROW(2) NRC_PR_ROW(3) NRC_PR_ROW(4)
+#undef NRC_PR_ROW
+#undef NRC_PR_CASE
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long n = (long)ncfg * n1 * n2;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, n1,
+                     n2, nchunks, mt, nc_t, (long)ncfg);
   return hipGetLastError();
 }
 

@@ -573,20 +573,10 @@ SlaterDet::SlaterDet(Float** props, OneBody** one_body, int n)
 
 SlaterDet::~SlaterDet() {}
 
-// det (STANDARD) or log det (LOG) of the leading n x n minor (slater_det.C:59-115)
-std::complex<Float> SlaterDet::Run(int n) {
-  const char* fname = "complex<Float> SlaterDet::Run(int)";
-  if (n > mat_size) ERR.General(fname, "Submatrix exceeds size of matrix.");
-  if (n < 1) {
-    VRB.Warn(fname, "n < 1, returning unity.");
-    return 1.0;
-  }
-  std::vector<double> h(2 * (size_t)n * n);
-  for (int s = 0; s < n; ++s)
-    for (int t = 0; t < n; ++t) {
-      h[2 * ((size_t)s * n + t)] = slater_mat[t + mat_size * s].real();
-      h[2 * ((size_t)s * n + t) + 1] = slater_mat[t + mat_size * s].imag();
-    }
+// det (STANDARD) or log det (LOG) of the n x n matrix h[s][t] (interleaved re, im) by the
+// device LU; a singular matrix ends the program as ludcmp does (ludcmp.C:23)
+static std::complex<Float> det_of(const std::vector<double>& h, int n, DetType det_type,
+                                  const char* fname) {
   double* dev = dev_alloc<double>(h.size() + 4 + 1, fname);
   int* dev_st = dev_alloc<int>(1, fname);
   ERR.Check(fname, nrc_memcpy_h2d(ctx(), dev, h.data(), h.size() * sizeof(double)));
@@ -605,6 +595,232 @@ std::complex<Float> SlaterDet::Run(int n) {
   if (det_type == DET_TYPE_LOG) return std::complex<Float>((Float)r[0], (Float)r[3]);
   const double m = std::exp(r[0]);
   return std::complex<Float>((Float)(m * r[1]), (Float)(m * r[2]));
+}
+
+// det (STANDARD) or log det (LOG) of the leading n x n minor (slater_det.C:59-115)
+std::complex<Float> SlaterDet::Run(int n) {
+  const char* fname = "complex<Float> SlaterDet::Run(int)";
+  if (n > mat_size) ERR.General(fname, "Submatrix exceeds size of matrix.");
+  if (n < 1) {
+    VRB.Warn(fname, "n < 1, returning unity.");
+    return 1.0;
+  }
+  std::vector<double> h(2 * (size_t)n * n);
+  for (int s = 0; s < n; ++s)
+    for (int t = 0; t < n; ++t) {
+      h[2 * ((size_t)s * n + t)] = slater_mat[t + mat_size * s].real();
+      h[2 * ((size_t)s * n + t) + 1] = slater_mat[t + mat_size * s].imag();
+    }
+  return det_of(h, n, det_type, fname);
+}
+
+// M(s, t) = <pair| props1_s props2_t> (slater_det.C:41-57)
+SlaterDet::SlaterDet(Float** props1, Float** props2, TwoBody* two_body, int n)
+    : slater_mat((size_t)n * n), mat_size(n), det_type(DET_TYPE_STANDARD) {
+  two_body->Run(props1, props2, slater_mat.data(), n);
+}
+
+// ------------------------------------------------------------------ special functions
+// Rybicki (1989): F(x) = lim_{h->0} pi^-1/2 sum over odd n of exp(-(x - n h)^2) / n, summed
+// around the even n0 nearest x/h with h = 0.4 and six odd terms each side; below 0.2 the
+// Taylor series to x^7.  Step, term count and evaluation order are special_functions.C's
+// (48-78), so the wave functions built on it agree with the reference's to rounding.
+Float DawsonF(Float x) {
+  const double h = 0.4;
+  if (std::fabs(x) < 0.2) {
+    const double x2 = x * x;
+    return x * (1.0 - (2.0 / 3.0) * x2 * (1.0 - 0.4 * x2 * (1.0 - (2.0 / 7.0) * x2)));
+  }
+  double c[7];
+  for (int i = 1; i <= 6; ++i) c[i] = std::exp(-((2.0 * i - 1.0) * h) * ((2.0 * i - 1.0) * h));
+  const double ax = std::fabs(x);
+  const int n0 = 2 * (int)(0.5 * ax / h + 0.5);
+  const double xp = ax - n0 * h;
+  double e1 = std::exp(2.0 * xp * h);
+  const double e2 = e1 * e1;
+  double d1 = n0 + 1, d2 = d1 - 2.0, sum = 0.0;
+  for (int i = 1; i <= 6; ++i, d1 += 2.0, d2 -= 2.0, e1 *= e2)
+    sum += c[i] * (e1 / d1 + 1.0 / (d2 * e1));
+  const double m = std::exp(-xp * xp);
+  return (Float)((x >= 0.0 ? m : -m) * sum / 1.7724538509055160273);
+}
+
+// ------------------------------------------------------------------ TwoBody
+// w(p) of the two-body wave function types (two_body.C:21-128) and the site of -(p + bc)
+// (two_body.C:31-33); the pair matrix elements run on the device (nrc_pair_overlap).
+TwoBody::TwoBody(const TwoBodyArg& a)
+    : vol(GJP.Vol()), wavefunc(GJP.Vol(), 0.0), opposite_parity_index(GJP.Vol()) {
+  const char* fname = "TwoBody::TwoBody(const TwoBodyArg &)";
+  const int bx = GJP.Xbc(), by = GJP.Ybc(), bz = GJP.Zbc();
+  for (int i = 0; i < vol; ++i) {
+    const Vector3 v = IndexToVector3(i);
+    opposite_parity_index[i] =
+        Vector3ToIndex(Vector3(-(v.GetX() + bx), -(v.GetY() + by), -(v.GetZ() + bz)));
+  }
+  if (a.wavefunc_type == WAVEFUNC_TYPE_UNIFORM) {
+    for (int i = 0; i < vol; ++i) wavefunc[i] = 1.0 / vol;
+  } else if (a.wavefunc_type == WAVEFUNC_TYPE_GND) {
+    Dispersion d1(a.dispersion_type1, a.mass1, CUTOFF_TYPE_HARD);
+    Dispersion d2(a.dispersion_type2, a.mass2, CUTOFF_TYPE_HARD);
+    for (int i = 0; i < vol; ++i) {
+      const double xi1 = 1.0 + d1.Get(i), xi2 = 1.0 + d2.Get(i);
+      double w = 1.0;
+      w /= a.lambda - 1.0 / (xi1 * xi2);  // finite where xi is infinite (HARD cutoff)
+      wavefunc[i] = w;
+    }
+  } else if (a.wavefunc_type != WAVEFUNC_TYPE_NONE) {
+    Dispersion d(DISPERSION_TYPE_QUADRATIC, 1.0, CUTOFF_TYPE_HARD);
+    for (int i = 0; i < vol; ++i) {
+      const double psq = 2.0 * d.Get(i);
+      double w = 0.0;
+      if (a.wavefunc_type == WAVEFUNC_TYPE_TGND) {
+        if (psq <= GJP.CutoffSq()) {
+          const double xi = std::sqrt(psq) / (2.0 * a.lambda);
+          w = xi < 1e-15 ? 1.0 : DawsonF(xi) / xi;
+        }
+      } else if (a.wavefunc_type == WAVEFUNC_TYPE_PAIR1) {
+        w = 1.0 / (psq + a.lambda);
+      } else {
+        w = psq < 1e-15 ? 0.0 : std::exp(-a.lambda * std::sqrt(psq)) / psq;
+      }
+      wavefunc[i] = w;
+    }
+  }
+  dev_w = dev_alloc<double>(vol, fname);
+  dev_opp = dev_alloc<int>(vol, fname);
+  ERR.Check(fname, nrc_memcpy_h2d(ctx(), dev_opp, opposite_parity_index.data(),
+                                  vol * sizeof(int)));
+  Upload();
+}
+
+TwoBody::~TwoBody() {
+  nrc_free(ctx(), dev_w);
+  nrc_free(ctx(), dev_opp);
+}
+
+void TwoBody::Upload() {
+  ERR.Check("TwoBody::Upload", nrc_memcpy_h2d(ctx(), dev_w, wavefunc.data(), vol * sizeof(double)));
+}
+
+// two_body.C:145-196: each "k1 k2 k3 re" line adds re to w at that momentum
+void TwoBody::Deform(std::string name) {
+  const char* fname = "void TwoBody::Deform(string)";
+  std::ifstream file(name.c_str());
+  if (!file) ERR.FileR(fname, name.c_str());
+  int k1, k2, k3;
+  Float re;
+  while (file >> k1 >> k2 >> k3 >> re) {
+    VRB.Flow(fname, "Appending %f to %d component of wave function.", (double)re,
+             Vector3ToIndex(Vector3(k1, k2, k3)));
+    wavefunc[Vector3ToIndex(Vector3(k1, k2, k3))] += re;
+  }
+  Upload();
+}
+
+static int pair_mode() {
+  return NRC_FLOAT_PREC == NRC_PREC_F64 ? NRC_OVERLAP_F64 : NRC_OVERLAP_F32;
+}
+
+std::complex<Float> TwoBody::Run(Float* p1, Float* p2) {
+  const char* fname = "complex<Float> TwoBody::Run(Float*, Float*)";
+  double* dev_m = dev_alloc<double>(2, fname);
+  ERR.Check(fname, nrc_pair_overlap(ctx(), p1, p2, dev_w, dev_opp, dev_m, 1, 1, vol, 1,
+                                    pair_mode()));
+  double m[2];
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), m, dev_m, sizeof(m)));
+  nrc_free(ctx(), dev_m);
+  return std::complex<Float>((Float)m[0], (Float)m[1]);
+}
+
+void TwoBody::Run(Float** props1, Float** props2, std::complex<Float>* phi, int n) {
+  const char* fname = "void TwoBody::Run(Float**, Float**, complex<Float>*, int)";
+  bool contiguous = true;
+  for (int t = 1; t < n; ++t)
+    contiguous = contiguous && props1[t] == props1[0] + 2 * (long)vol * t &&
+                 props2[t] == props2[0] + 2 * (long)vol * t;
+  double* dev_M = dev_alloc<double>(2 * (size_t)n * n, fname);
+  if (contiguous) {
+    ERR.Check(fname, nrc_pair_overlap(ctx(), props1[0], props2[0], dev_w, dev_opp, dev_M, n, n,
+                                      vol, 1, pair_mode()));
+  } else {
+    for (int s = 0; s < n; ++s)
+      for (int t = 0; t < n; ++t)
+        ERR.Check(fname, nrc_pair_overlap(ctx(), props1[s], props2[t], dev_w, dev_opp,
+                                          dev_M + 2 * ((size_t)s * n + t), 1, 1, vol, 1,
+                                          pair_mode()));
+  }
+  std::vector<double> h(2 * (size_t)n * n);
+  ERR.Check(fname, nrc_memcpy_d2h(ctx(), h.data(), dev_M, h.size() * sizeof(double)));
+  nrc_free(ctx(), dev_M);
+  for (size_t i = 0; i < (size_t)n * n; ++i)
+    phi[i] = std::complex<Float>((Float)h[2 * i], (Float)h[2 * i + 1]);
+}
+
+// ------------------------------------------------------------------ SlaterDet2 / SlaterDet3
+// one-body matrix out[t + n s] = <one_body_s | props_t>
+static void one_body_matrix(Float** props, OneBody** one_body, int n,
+                            std::vector<std::complex<Float> >& out) {
+  SlaterDet m(props, one_body, n);
+  for (int s = 0; s < n; ++s)
+    for (int t = 0; t < n; ++t) out[t + n * s] = m.SlaterMat(s, t);
+}
+
+SlaterDet2::SlaterDet2(int n)
+    : N(n), det_type(DET_TYPE_STANDARD), phi((size_t)n * n), psi1((size_t)n * n),
+      psi2((size_t)n * n), has_psi1(false), has_psi2(false) {}
+
+// slater_det2.C:25-60
+SlaterDet2::SlaterDet2(Float** props1, Float** props2, OneBody** one_body1,
+                       OneBody** one_body2, TwoBody* two_body, int n)
+    : SlaterDet2(n) {
+  Fill(props1, props2, one_body1, one_body2, two_body);
+}
+
+SlaterDet2::~SlaterDet2() {}
+
+void SlaterDet2::Fill(Float** props1, Float** props2, OneBody** one_body1, OneBody** one_body2,
+                      TwoBody* two_body) {
+  has_psi1 = one_body1 != nullptr;
+  has_psi2 = one_body2 != nullptr;
+  if (has_psi1) one_body_matrix(props1, one_body1, N, psi1);
+  if (has_psi2) one_body_matrix(props2, one_body2, N, psi2);
+  two_body->Run(props1, props2, phi.data(), N);
+}
+
+// slater_det2.C:78-150: the leading n x n minor of phi with column t0 replaced by
+// psi1[s + t0 N] or row s0 by psi2[t + s0 N] (at most one of them non-negative)
+std::complex<Float> SlaterDet2::Run(int n, int s0, int t0) {
+  const char* fname = "complex<Float> SlaterDet2::Run(int, int, int)";
+  if (n > N) ERR.General(fname, "Sub-matrix exceeds size of matrix.");
+  if (n < 1) ERR.General(fname, "Specified size of sub-matrix is not positive.");
+  if (s0 >= 0 && t0 >= 0) ERR.General(fname, "s0 and t0 cannot both be non-negative.");
+  if (t0 >= 0 && t0 < n && !has_psi1)
+    ERR.General(fname, "t0 cannot be non-negative; psi1==NULL.");
+  if (s0 >= 0 && s0 < n && !has_psi2)
+    ERR.General(fname, "s0 cannot be non-negative; psi2==NULL.");
+  std::vector<double> h(2 * (size_t)n * n);
+  for (int s = 0; s < n; ++s)
+    for (int t = 0; t < n; ++t) {
+      std::complex<Float> v = phi[t + N * s];
+      if (t == t0) v = psi1[s + t0 * N];
+      if (s == s0) v = psi2[t + s0 * N];
+      h[2 * ((size_t)s * n + t)] = v.real();
+      h[2 * ((size_t)s * n + t) + 1] = v.imag();
+    }
+  return det_of(h, n, det_type, fname);
+}
+
+// slater_det3.C:18-75: the propagators and sinks are kept; Initialize(two_body) recomputes
+// the matrices (slater_det3.C:95-125), so one object serves every time slice
+SlaterDet3::SlaterDet3(Float** p1, Float** p2, OneBody** ob1, OneBody** ob2, int n)
+    : SlaterDet2(n), props1(p1, p1 + n), props2(p2, p2 + n) {
+  if (ob1) one_body1.assign(ob1, ob1 + n);
+  if (ob2) one_body2.assign(ob2, ob2 + n);
+}
+
+void SlaterDet3::Initialize(TwoBody* two_body) {
+  Fill(props1.data(), props2.data(), one_body1.empty() ? nullptr : one_body1.data(),
+       one_body2.empty() ? nullptr : one_body2.data(), two_body);
 }
 
 // ------------------------------------------------------------------ ludcmp

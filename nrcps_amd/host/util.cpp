@@ -264,6 +264,8 @@ NRC_READ_ENUM(DetType, "DET_TYPE_STANDARD", "DET_TYPE_LOG")
 NRC_READ_ENUM(InteractionType, "INTERACTION_TYPE_NONE", "INTERACTION_TYPE_XISQMINUSONE",
               "INTERACTION_TYPE_ONEMINUSXIINVSQ")
 NRC_READ_ENUM(IoType, "IO_TYPE_ROOT", "IO_TYPE_ALL")
+NRC_READ_ENUM(WavefuncType, "WAVEFUNC_TYPE_NONE", "WAVEFUNC_TYPE_UNIFORM", "WAVEFUNC_TYPE_GND",
+              "WAVEFUNC_TYPE_TGND", "WAVEFUNC_TYPE_PAIR1", "WAVEFUNC_TYPE_PAIR2")
 
 DoArg::DoArg()
     : t_sites(4), x_sites(4), y_sites(4), z_sites(4), x_boundary_type(BOUNDARY_TYPE_PRD),
@@ -375,6 +377,23 @@ void PotentialArg::Decode(std::string f) {
   ReadLine(spring_constant1);
   ReadLine(spring_constant2);
   ReadLine(spring_constant3);
+  CloseFile();
+}
+
+// arg.C:523-558
+TwoBodyArg::TwoBodyArg()
+    : wavefunc_type(WAVEFUNC_TYPE_UNIFORM), dispersion_type1(DISPERSION_TYPE_STANDARD),
+      dispersion_type2(DISPERSION_TYPE_STANDARD), mass1(1.0), mass2(1.0), lambda(1.0),
+      file_stem("two_body") {}
+void TwoBodyArg::Decode(std::string f) {
+  OpenFile(f);
+  ReadLine(wavefunc_type);
+  ReadLine(dispersion_type1);
+  ReadLine(dispersion_type2);
+  ReadLine(mass1);
+  ReadLine(mass2);
+  ReadLine(lambda);
+  ReadLine(file_stem);
   CloseFile();
 }
 

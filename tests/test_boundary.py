@@ -21,6 +21,13 @@ COMPATIBLE = [
     "tests/momenta/main.C",
     "tests/dispersion/main.C",
     "tests/lattice/main.C",
+    # pair sinks: TwoBody, SlaterDet2, SlaterDet3
+    "production/two_body/main.C",
+    "production/slater_det2_trapped/main-cum.C",
+    "production/slater_det2_trapped/main-no_cum.C",
+    "production/slater_det2_untrapped/main-cum.C",
+    "production/slater_det2_untrapped/main-no_cum.C",
+    "production/slater_det3_trapped/main-cum.C",
 ]
 
 
@@ -50,6 +57,8 @@ def test_class_layer_and_driver_are_built():
     for sym in ["Fourier::Forward()", "Propagator::Run(", "SlaterDet::Run(int)",
                 "OneBody::Project(double*)", "ludcmp(std::complex<double>*, int, int*, double*)",
                 "Interaction::Initialize()", "Kinetic::Evolve(Fourier*)", "Lattice::Refresh()",
-                "Momenta::Momenta(MomentaArg const&)", "Random::WriteState(char const*)"]:
+                "Momenta::Momenta(MomentaArg const&)", "Random::WriteState(char const*)",
+                "TwoBody::Run(double*, double*)", "SlaterDet3::Initialize(TwoBody*)",
+                "SlaterDet2::Run(int, int, int)", "DawsonF(double)"]:
         assert sym in out, sym
     assert os.path.exists(drv)
