@@ -40,7 +40,8 @@ class RunDesc(C.Structure):
                 ("minors", C.c_int), ("moms", C.c_void_p), ("kin", C.c_void_p),
                 ("A", C.c_double), ("smear", C.c_void_p), ("field_kind", C.c_int),
                 ("seed", C.c_uint64), ("config", C.c_uint32), ("rng", C.c_void_p),
-                ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p)]
+                ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p),
+                ("sources", C.c_void_p)]
 
 
 def build(force: bool = False) -> None:
@@ -303,7 +304,7 @@ def pair_overlap(psi1: np.ndarray, psi2: np.ndarray, w: np.ndarray, opp: np.ndar
 
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
-               want_psi=False, pair_w=None, pair_opp=None):
+               want_psi=False, pair_w=None, pair_opp=None, sources=None):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
     (complex), argsum [T, nm], optional M [T, N, N] (the pair matrix with pair_w) and
     psi [N, V].  With pair_w / pair_opp: nm = 2N, SlaterDet3's determinants (2n + 1: leading
@@ -326,6 +327,10 @@ def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smea
         sm = np.ascontiguousarray(smear, dtype=np.float64)
         d.smear = sm.ctypes.data
     keep = []
+    if sources is not None:  # N x V complex, replacing the plane waves of moms
+        src = np.ascontiguousarray(sources, dtype=np.complex128).reshape(-1).view(np.float64)
+        keep.append(src)
+        d.sources = src.ctypes.data
     if pair_w is not None:
         pw = np.ascontiguousarray(pair_w, dtype=np.float64)
         po = np.ascontiguousarray(pair_opp, dtype=np.int32)

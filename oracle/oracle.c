@@ -792,7 +792,11 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
     goto done;
   }
   for (int n = 0; n < N; ++n) {
-    orc_plane_wave(psi + 2 * (size_t)n * V, L, d->moms + 3 * n, xspace);
+    if (d->sources)
+      memcpy(psi + 2 * (size_t)n * V, d->sources + 2 * (size_t)n * V,
+             2 * (size_t)V * sizeof(double));
+    else
+      orc_plane_wave(psi + 2 * (size_t)n * V, L, d->moms + 3 * n, xspace);
     memcpy(sink + 2 * (size_t)n * V, psi + 2 * (size_t)n * V, 2 * (size_t)V * sizeof(double));
   }
   if (xspace)

@@ -150,6 +150,8 @@ typedef struct {
                            measurement is SlaterDet3's (production/slater_det3_trapped/
                            main-cum.C:217-238), 2N determinants per slice */
   const int* pair_opp;  /* V opposite-parity indices for pair_w */
+  const double* sources; /* optional N x V complex sources = sinks in storage space (k-space
+                            for the k-space split) instead of the plane waves of moms */
 } orc_run_desc;
 
 /* Runs one configuration through T slices, measuring before evolving at each t

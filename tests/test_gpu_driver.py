@@ -67,3 +67,75 @@ def test_slater_det_driver_matches_oracle(orc):
     assert len(state) == 4 + 8 * 105
     st = np.frombuffer(state[4:], dtype=np.int64)
     assert st[0] == 105 and st[101] == 397
+
+
+DRIVER3 = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det3")
+
+
+def test_slater_det3_driver_matches_oracle(orc):
+    """The pair-sink driver (SlaterDet3 + TwoBody on the class layer, main-cum.C's loop):
+    every determinant Run(n+1, -1, -1) and Run(n+1, -1, n) per slice and the six cumulant
+    powers against the oracle's pair run with the same sources, ranlxs stream, smeared
+    interaction and TGND wave function."""
+    assert os.path.exists(DRIVER3), "the slater_det3 driver is not built"
+    L, T, ncfg, N = 8, 6, 2, 5
+    V = L ** 3
+    rng = np.random.default_rng(8)
+    src = np.zeros((N, V), np.complex128)
+    lines = []
+    # random combinations of a shared set of momenta closed under k -> -k, so that the pair
+    # matrix sum_k w(k) psi_s(k) psi_t(-k) is regular from t = 0 on (as with SHO sources)
+    shared = [(0, 0, 0), (0, 0, 1), (0, 0, L - 1), (0, 1, 0), (0, L - 1, 0), (1, 0, 0),
+              (L - 1, 0, 0), (1, 1, 0), (L - 1, L - 1, 0)]
+    for n in range(N):
+        ks = [(a * L + b) * L + c for a, b, c in shared]
+        out = []
+        for k in ks:
+            c = complex(rng.standard_normal(), rng.standard_normal())
+            src[n, k] = c
+            out.append(f"{k // (L * L)} {(k // L) % L} {k % L} {c.real!r} {c.imag!r}\n")
+        lines.append(out)
+    args = dict(ARGS)
+    args["propagator.arg"] = f"{N}\nresults/prop\n"
+    args["two_body.arg"] = ("WAVEFUNC_TYPE_TGND\nDISPERSION_TYPE_STANDARD\n"
+                            "DISPERSION_TYPE_STANDARD\n1.0\n1.0\n0.176776695\nresults/two\n")
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs", "sources"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in args.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        for n in range(N):
+            with open(os.path.join(tmp, "sources", f"source.{n}"), "w") as f:
+                f.writelines(lines[n])
+        print("[gpu] running the slater_det3 driver (L=8, T=6, N=5, 2 configurations)",
+              flush=True)
+        r = subprocess.run([DRIVER3], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+        def rows(name):
+            return np.array([list(map(float, line.split()))
+                             for line in open(os.path.join(tmp, "results", name))
+                             if line.strip()])
+        det = np.stack([rows(f"slater.{m + 1}").reshape(ncfg, T, 2) for m in range(2 * N)])
+        z = np.stack([[rows(f"z{k + 1}.{m + 1}").reshape(ncfg, T) for m in range(2 * N)]
+                      for k in range(6)])
+    cut2 = PI * PI
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    assert rc == 0
+    w = orc.two_body_wavefunc(L, orc.WAVEFUNC_TGND, 0.176776695, cutoff_sq=cut2)
+    opp = orc.opposite_parity_index(L)
+    ranlxs = orc.ranlxs(2, 7012)
+    moms = np.zeros((N, 3), np.int32)  # unused: the sources are given
+    for j in range(ncfg):
+        ref = orc.run_config(L, N, T, moms, kin, smear=f, field="ranlxs", rng=ranlxs,
+                             pair_w=w, pair_opp=opp, sources=src)
+        for m in range(2 * N):
+            want = np.exp(ref["logabs"][:, m]) * ref["phase"][:, m]
+            got = det[m, j, :, 0] + 1j * det[m, j, :, 1]
+            np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-300)
+            for k in range(6):
+                np.testing.assert_allclose(z[k, m, j], ref["logabs"][:, m] ** (k + 1),
+                                           rtol=1e-9, atol=1e-12)
