@@ -240,6 +240,51 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                          "output_bias": float(ps[8240])}}
 
 
+def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012):
+    """The pair-sink measurement of production/slater_det3_trapped (L=32, N=35): every slice
+    the pair matrix phi[s][t] = sum_p w(p) psi_s(p) psi_t(-p) of every configuration (split-
+    BF16x3 on MFMA) and SlaterDet3's 2N determinants, plane-wave sources, the PAIR1 wave
+    function w = 1/(p^2 + 0.5).  Reports configs*slices/s and the pair-matrix kernel against
+    the MFMA peak (3 x 8 N^2 V flops per configuration-slice) and its algorithmic HBM bytes
+    (the orbitals read once: 8 N V)."""
+    n = np.arange(L)
+    f = np.where(n > L // 2, n - L, n) * (2 * np.pi / L)
+    p2 = (f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2).reshape(-1)
+    w = 1.0 / (p2 + 0.5)
+    m = (-n) % L
+    opp = ((m[:, None, None] * L + m[None, :, None]) * L + m[None, None, :]).reshape(-1)
+    ens = na.Ensemble(ctx, L, _lowest(L, N), _tables(L, tau, 1.0), A, C, prec=na.PREC_F32,
+                      split=na.SPLIT_KSPACE, pair_w=w, pair_opp=opp.astype(np.int32))
+    ens.run(seed, 0, 2)
+    ctx.sync()
+    t0 = time.perf_counter()
+    la, _, _ = ens.run(seed, C, T)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ens.profile(True)
+    ens.run(seed, 2 * C, T)
+    t_evo, n_evo = ens.kernel_time(0)
+    t_lu, n_lu = ens.kernel_time(2)
+    t_meas, n_meas = ens.kernel_time(4)
+    ens.profile(False)
+    ens.close()
+    V = L ** 3
+    meas_ms = t_meas / max(n_meas, 1)  # gather + pair matrices of one slice
+    flops = 3 * 8.0 * N * N * V * C
+    return {"workload": f"pair sinks (SlaterDet3): L={L}, N={N}, T={T}, {C} configurations, "
+                        f"FP32 evolution, split-BF16x3 pair matrices, 2N FP64 LU per slice",
+            "value": C * T / dt, "unit": "configs*slices/s",
+            "finite_fraction": float(np.isfinite(la).mean()),
+            "avg_ms_per_slice": {"evolution": t_evo / max(n_evo, 1),
+                                 "pair_matrices_and_gather": meas_ms,
+                                 "lu_2N": t_lu / max(n_meas, 1)},
+            "pair_matrix_mfma": {"achieved_tflops": flops / (meas_ms * 1e-3) / 1e12,
+                                 "peak_tflops": 2500.0,
+                                 "frac": flops / (meas_ms * 1e-3) / 1e12 / 2500.0},
+            "pair_matrix_hbm": {"algorithmic_gbs": 8.0 * N * V * C / (meas_ms * 1e-3) / 1e9,
+                                "peak_gbs": HBM_PEAK_GBS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +301,8 @@ def main():
                     help="skip the batched uneven-shape measurement")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-proposal measurement")
+    ap.add_argument("--no-pair", action="store_true",
+                    help="skip the pair-sink (SlaterDet3) measurement")
     ap.add_argument("--precision-configs", type=int, default=1024,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
     ap.add_argument("--path", type=int, default=0,
@@ -339,6 +386,12 @@ def main():
         print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
 
+    pair = None
+    if not args.no_pair:
+        ens.close()
+        pair = pair_sinks(ctx, na)
+        print(f"pair sinks: {pair['value']:.0f} configs*slices/s", file=sys.stderr, flush=True)
+
     precision = None
     if args.precision_configs > 0:
         precision = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed)
@@ -364,6 +417,7 @@ def main():
         "precision": precision,
         "batched_uneven": batched,
         "learned_proposal": learned,
+        "pair_sinks": pair,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]
