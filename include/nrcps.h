@@ -89,7 +89,8 @@ int nrc_overlap(nrc_context* ctx, const void* phi_dev, const void* psi_dev, doub
  *      M_dev[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]], no conjugation.
  *      psi1: ncfg x n1 x vol, psi2: ncfg x n2 x vol complex (FP64 for NRC_OVERLAP_F64, FP32
  *      otherwise); w: vol doubles (the wave function); opp: vol ints, the site of -(n + bc)
- *      (src/two_body/two_body.C:31-33); all device.  Complex FP64 output. */
+ *      (src/two_body/two_body.C:31-33); all device, w and opp 16-byte aligned for
+ *      NRC_OVERLAP_BF16X3.  Complex FP64 output. */
 int nrc_pair_overlap(nrc_context* ctx, const void* psi1_dev, const void* psi2_dev,
                      const double* w_dev, const int* opp_dev, double* M_dev, int n1, int n2,
                      long vol, int ncfg, int mode);

@@ -63,6 +63,7 @@ enum IoType { IO_TYPE_ROOT, IO_TYPE_ALL };
 const int FLUSHRATE = 10;
 const Float PI = 3.1415926535897932;
 const Float TWOPI = 6.2831853071795864;
+const Float PIM4 = 0.7511255444649425;  // pi^(-1/4)
 const Float PISQ = 9.8696044010893586;
 
 // ------------------------------------------------------------------ Vector3 (vector3.h)
@@ -477,6 +478,7 @@ class OneBody {
   OneBody& operator=(const OneBody&);
   void Upload();
   SourceType source_type;
+  Float lambda1, lambda2, lambda3;
   std::vector<Float> psi;
   Float* dev_psi;
   double* dev_m;
@@ -487,6 +489,10 @@ class OneBody {
 // Dawson's integral (src/utils/special_functions.C:48-78): Rybicki's sampling-theorem
 // formula, single-precision accurate (~2e-7), as the reference evaluates it.
 Float DawsonF(Float x);
+// Orthonormal Hermite polynomials h_n (int h_m h_n e^{-x^2} dx = delta_mn) by the three-term
+// recurrence from h_0 = pi^(-1/4) (special_functions.C:7-34): h_n(x), and h[0..n-1](x).
+Float HermiteH(int n, Float x);
+void HermiteH(Float* h, int n, Float x);
 
 // ------------------------------------------------------------------ pair sink
 // The two-particle sink <pair| of SlaterDet / SlaterDet2 / SlaterDet3 (include/two_body.h:

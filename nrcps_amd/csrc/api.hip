@@ -259,6 +259,7 @@ int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const
   }
   if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
   if (n1 > 64 || n2 > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
+  if (((uintptr_t)w | (uintptr_t)opp) & 15) return NRC_ERR_ARG;  // vector loads of w and opp
   const size_t bytes = nrc::pair_bf16x3_partial_floats(n1, n2, vol, ncfg) * sizeof(float);
   void* work = ctx_workspace(ctx, bytes);
   if (!work) return NRC_ERR_NOMEM;

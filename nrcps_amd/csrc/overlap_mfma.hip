@@ -166,17 +166,22 @@ __global__ __launch_bounds__(256) void k_overlap_bf16x3(const __bf16* __restrict
 
 // The pair-sink matrix M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]] (TwoBody::Run,
 // src/two_body/noarch/two_body_run.C:16-43) as the same split-BF16x3 GEMM: A rows are the
-// configuration's own psi1 orbitals, split as they are read; B column 2t holds
-// (b_re, -b_im) and column 2t + 1 (b_im, b_re) of b = w psi2_t(opp), so the real dot
-// products give Re and Im of the unconjugated complex product.  The weights are rounded to
-// FP32 as they are read.
-template <int MT, int NCT>
+// configuration's own psi1 orbitals, split as they are read; a lane of column tile b holds
+// orbital t = 16 b + l % 16 and builds both B columns from one load of b = w psi2_t(opp):
+// (b_re, -b_im) for the real part and (b_im, b_re) for the imaginary part, accumulated
+// separately.  The four mirrored sites of a lane's elements come as one int4; when they are
+// consecutive (opp reverses each axis, so every group but the one holding z = 0 under
+// periodic boundaries) the four values are two 16-byte loads (or 8 + 16 + 8 bytes from an
+// odd site), reversed in registers; opp and wf must be 16-byte aligned.  The
+// weights are rounded to FP32 as they are read.  Partial tiles keep re in column 2t and im
+// in 2t + 1, as the overlap's.
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ psi1,
                                                      const float2* __restrict__ psi2,
                                                      const double* __restrict__ wf,
                                                      const int* __restrict__ opp,
                                                      float* __restrict__ partial, int n1,
-                                                     int n2, long vol, int mt, int nc_t,
+                                                     int n2, long vol, int mt, int nt,
                                                      long kchunk) {
   __shared__ float red[OV_WAVES - 1][4][64];
   const int chunk = blockIdx.x, nchunks = gridDim.x;
@@ -186,15 +191,19 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
   const long K = 2 * vol;
   const long k0 = (long)chunk * kchunk;
   const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
-  f32x4 acc[MT][NCT];
+  f32x4 accr[MT][NT], acci[MT][NT];
 #pragma unroll
   for (int a = 0; a < MT; ++a)
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NT; ++b) {
+      accr[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      acci[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
   const float2* a1 = psi1 + c * n1 * vol;
   const float2* b2 = psi2 + c * n2 * vol;
   for (long kb = k0 + (long)wave * OV_KSTEP; kb < k1; kb += OV_WAVES * OV_KSTEP) {
-    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements ka/2 .. ka/2 + 3
+    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements j0 .. j0 + 3
+    const long j0 = ka / 2;
     const bool in = ka < K;
     bf16x8 ah[MT], al[MT];
 #pragma unroll
@@ -203,7 +212,7 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
         const int s = a * 16 + row;
         float v[8];
         if (s < n1 && in) {
-          const float4* p = reinterpret_cast<const float4*>(a1 + (long)s * vol + ka / 2);
+          const float4* p = reinterpret_cast<const float4*>(a1 + (long)s * vol + j0);
           const float4 u0 = p[0], u1 = p[1];
           v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
           v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
@@ -214,68 +223,101 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
         split8(v, ah[a], al[a]);
       }
     }
-    int site[4];
-    float wt[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      site[q] = in ? opp[ka / 2 + q] : 0;
-      wt[q] = in ? (float)wf[ka / 2 + q] : 0.f;
+    int4 st = make_int4(0, 0, 0, 0);
+    float wt[4] = {0.f, 0.f, 0.f, 0.f};
+    if (in) {
+      st = *reinterpret_cast<const int4*>(opp + j0);
+      const double2* wp = reinterpret_cast<const double2*>(wf + j0);
+      const double2 w01 = wp[0], w23 = wp[1];
+      wt[0] = (float)w01.x; wt[1] = (float)w01.y; wt[2] = (float)w23.x; wt[3] = (float)w23.y;
     }
+    // consecutive: the mirrored sites are st.w .. st.w + 3 for elements 3, 2, 1, 0
+    const bool run = st.x == st.w + 3 && st.y == st.w + 2 && st.z == st.w + 1;
+    const bool even = (st.w & 1) == 0;
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) {
-      if (b < nc_t) {
-        const int col = b * 16 + row, t = col >> 1, part = col & 1;
-        float v[8];
-        if (t < n2 && in) {
-          const float2* p = b2 + (long)t * vol;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const float2 u = p[site[q]];
-            const float br = wt[q] * u.x, bi = wt[q] * u.y;
-            v[2 * q] = part ? bi : br;
-            v[2 * q + 1] = part ? br : -bi;
-          }
+    for (int b = 0; b < NT; ++b) {
+      const int t = b * 16 + row;
+      float vr[8], vi[8];
+      if (b < nt && t < n2 && in) {
+        const float2* p = b2 + (long)t * vol;
+        float2 u[4];
+        if (run && even) {  // 16-byte aligned
+          const float4* q = reinterpret_cast<const float4*>(p + st.w);
+          const float4 u32 = q[0], u10 = q[1];
+          u[3] = make_float2(u32.x, u32.y);
+          u[2] = make_float2(u32.z, u32.w);
+          u[1] = make_float2(u10.x, u10.y);
+          u[0] = make_float2(u10.z, u10.w);
+        } else if (run) {  // odd start: 8 + 16 (aligned) + 8 bytes
+          const float4 u21 = *reinterpret_cast<const float4*>(p + st.w + 1);
+          u[3] = p[st.w];
+          u[2] = make_float2(u21.x, u21.y);
+          u[1] = make_float2(u21.z, u21.w);
+          u[0] = p[st.w + 3];
         } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) v[q] = 0.f;
+          u[0] = p[st.x];
+          u[1] = p[st.y];
+          u[2] = p[st.z];
+          u[3] = p[st.w];
         }
-        bf16x8 bh, bl;
-        split8(v, bh, bl);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float br = wt[q] * u[q].x, bi = wt[q] * u[q].y;
+          vr[2 * q] = br;
+          vr[2 * q + 1] = -bi;
+          vi[2 * q] = bi;
+          vi[2 * q + 1] = br;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) vr[q] = vi[q] = 0.f;
+      }
+      if (b < nt) {
+        bf16x8 rh, rl, ih, il;
+        split8(vr, rh, rl);
+        split8(vi, ih, il);
 #pragma unroll
         for (int a = 0; a < MT; ++a) {
           if (a < mt) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, accr[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, accr[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acci[a][b], 0, 0, 0);
           }
         }
       }
     }
   }
-  const int ldp = nc_t * 16;
+  // sum the waves tile by tile; column 2t holds re, 2t + 1 im
+  const int ldp = nt * 32;
   float* out = partial + (c * nchunks + chunk) * (long)(mt * 16) * ldp;
 #pragma unroll
   for (int a = 0; a < MT; ++a)
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) {
-      if (a < mt && b < nc_t) {  // uniform over the workgroup
-        if (wave > 0) {
+    for (int b = 0; b < NT; ++b)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = acc[a][b][r];
-        }
-        __syncthreads();
-        if (wave == 0) {
+      for (int part = 0; part < 2; ++part) {
+        if (a < mt && b < nt) {  // uniform over the workgroup
+          const f32x4 v = part ? acci[a][b] : accr[a][b];
+          if (wave > 0) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float x = acc[a][b][r];
-#pragma unroll
-            for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
-            out[(long)(a * 16 + grp * 4 + r) * ldp + b * 16 + row] = x;
+            for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = v[r];
           }
+          __syncthreads();
+          if (wave == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              float x = v[r];
+#pragma unroll
+              for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
+              out[(long)(a * 16 + grp * 4 + r) * ldp + 2 * (b * 16 + row) + part] = x;
+            }
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
-    }
 }
 
 // M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
@@ -346,26 +388,27 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
 }
 
 size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg) {
-  return overlap_bf16x3_partial_floats(n1, n2, vol, ncfg);
+  const int mt = (n1 + 15) / 16, nt = (n2 + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const long nchunks = (K + kc - 1) / kc;
+  return (size_t)ncfg * nchunks * (mt * 16) * (nt * 32);
 }
 
 hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const double* wf,
                               const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
                               float* partial, hipStream_t s) {
-  if (n1 < 1 || n1 > 16 * OV_MAXT || n2 < 1 || 2 * n2 > 16 * OV_MAXN || (vol & 3))
+  if (n1 < 1 || n1 > 16 * OV_MAXT || n2 < 1 || n2 > 64 || (vol & 3))
     return hipErrorInvalidValue;
-  const int mt = (n1 + 15) / 16, nc_t = (2 * n2 + 15) / 16;
+  const int mt = (n1 + 15) / 16, nt = (n2 + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const int nchunks = (int)((K + kc - 1) / kc);
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
 #define NRC_PR_CASE(A, B)                                                                     \
-  if (mt == A && nc_t == B) {                                                                 \
+  if (mt == A && nt == B) {                                                                   \
     hipLaunchKernelGGL((k_pair_bf16x3<A, B>), grid, block, 0, s, psi1, psi2, wf, opp, partial, \
-                       n1, n2, vol, mt, nc_t, kc);                                            \
+                       n1, n2, vol, mt, nt, kc);                                              \
   }
-#define NRC_PR_ROW(A) \
-  NRC_PR_CASE(A, 1) NRC_PR_CASE(A, 2) NRC_PR_CASE(A, 3) NRC_PR_CASE(A, 4) NRC_PR_CASE(A, 5) \
-  NRC_PR_CASE(A, 6) NRC_PR_CASE(A, 7) NRC_PR_CASE(A, 8)
+#define NRC_PR_ROW(A) NRC_PR_CASE(A, 1) NRC_PR_CASE(A, 2) NRC_PR_CASE(A, 3) NRC_PR_CASE(A, 4)
   NRC_PR_ROW(1) NRC_PR_ROW(2) NRC_PR_ROW(3) NRC_PR_ROW(4)
 #undef NRC_PR_ROW
 #undef NRC_PR_CASE
@@ -375,8 +418,9 @@ hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const doub
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
+  // column pairs (2t, 2t + 1) over 2 nt tiles of 16: the overlap's reduction applies
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, n1,
-                     n2, nchunks, mt, nc_t, (long)ncfg);
+                     n2, nchunks, mt, 2 * nt, (long)ncfg);
   return hipGetLastError();
 }
 

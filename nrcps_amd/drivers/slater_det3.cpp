@@ -2,8 +2,9 @@
 //
 // The measurement loop of production/slater_det3_trapped/main-cum.C:140-295 with the
 // reference's class API and file formats, untrapped: arguments from args/*.arg (two_body.arg
-// for the pair wave function), N = propagator.arg's n_fermions sources and one-body sinks
-// read from sources/source.<n> ("k1 k2 k3 re im" lines, as main-cum.C:139-143 reads them),
+// for the pair wave function), N = propagator.arg's n_fermions SHO sources and one-body sinks
+// whose coefficients are read from sources/source.<n> ("n1 n2 n3 re im" lines, as
+// main-cum.C:75-78 and 139-143 set them),
 // K(1/2) I K(1/2) per slice; before each slice SlaterDet3::Initialize(&two_body) and
 // for n = 0 .. N-1 the determinants Run(n+1, -1, -1) (n+1 pairs, into results/slater.<2n+2>)
 // and Run(n+1, -1, n) (n pairs and one particle, into results/slater.<2n+1>), with the
@@ -44,6 +45,10 @@ int main() {
   OneBodyArg one_body_arg("args/one_body.arg");
   TwoBodyArg two_body_arg("args/two_body.arg");
   if (evo_arg.unload_period < 1) ERR.General(fname, "evo_arg.unload_period must be positive.");
+  if (one_body_arg.source_type != SOURCE_TYPE_SHO) {  // main-cum.C:75-78
+    VRB.Warn(fname, "Setting one_body_arg.source_type to SOURCE_TYPE_SHO.");
+    one_body_arg.source_type = SOURCE_TYPE_SHO;
+  }
   const int N = prop_arg.n_fermions;
   if (N < 1 || N > 35) ERR.General(fname, "propagator_arg.n_fermions must be in 1 .. 35.");
 

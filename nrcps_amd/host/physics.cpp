@@ -476,7 +476,8 @@ void Propagator::Normalize(Float) {
 
 // ------------------------------------------------------------------ OneBody
 OneBody::OneBody(const OneBodyArg& a)
-    : source_type(a.source_type), psi(2 * GJP.Vol(), 0.0), vol(GJP.Vol()) {
+    : source_type(a.source_type), lambda1(a.lambda1), lambda2(a.lambda2), lambda3(a.lambda3),
+      psi(2 * GJP.Vol(), 0.0), vol(GJP.Vol()) {
   dev_psi = dev_alloc<Float>(psi.size(), "OneBody::OneBody");
   dev_m = dev_alloc<double>(2, "OneBody::OneBody");
 }
@@ -488,9 +489,55 @@ OneBody::~OneBody() {
 
 void OneBody::Upload() { upload(dev_psi, psi, "OneBody::Set"); }
 
+// SOURCE_TYPE_SHO (one_body.C:53-171): psi holds the coefficients c(n1, n2, n3) at momentum
+// index (n1, n2, n3), n < min(10, L); the source becomes, at each lattice momentum p_i =
+// 2 pi k_i / L (k centred on 0),
+//   sum_n c(n) h_n1(d1) h_n2(d2) h_n3(d3) exp(-(d1^2 + d2^2 + d3^2) / 2)
+//       sqrt(lambda1 lambda2 lambda3) sqrt((2 pi)^3 / V),
+// d1 = p1 lambda1, d2 = p2 lambda2, d3 = p3 lambda2 (the reference scales the third axis by
+// lambda2, kept so that sources agree with its files).
 void OneBody::Set() {
-  if (source_type == SOURCE_TYPE_SHO)
-    ERR.NotImplemented("void OneBody::Set()", "SOURCE_TYPE_SHO sources.");
+  const char* fname = "void OneBody::Set()";
+  if (source_type == SOURCE_TYPE_SHO) {
+    if (GJP.APBCQuery()) ERR.NotImplemented(fname, "Must use PBCs when a potential is present.");
+    const int X = GJP.Xsites(), Y = GJP.Ysites(), Z = GJP.Zsites();
+    const int n1m = X < 10 ? X : 10, n2m = Y < 10 ? Y : 10, n3m = Z < 10 ? Z : 10;
+    VRB.Warn(fname, "Maximum SHO levels (nx,ny,nz) are (%d,%d,%d)", n1m, n2m, n3m);
+    std::vector<double> c(2 * (size_t)n1m * n2m * n3m);
+    for (int a = 0; a < n1m; ++a)
+      for (int b = 0; b < n2m; ++b)
+        for (int d = 0; d < n3m; ++d) {
+          const int j = 2 * Vector3ToIndex(Vector3(a, b, d));
+          c[2 * ((a * n2m + b) * n3m + d)] = psi[j];
+          c[2 * ((a * n2m + b) * n3m + d) + 1] = psi[j + 1];
+        }
+    std::vector<Float> h1(n1m), h2(n2m), h3(n3m);
+    const double norm = std::sqrt(lambda1 * lambda2 * lambda3) *
+                        std::sqrt(TWOPI * TWOPI * TWOPI / vol);
+    for (int i = 0; i < vol; ++i) {
+      const Vector3 v = IndexToVector3(i);
+      int x = v.GetX(), y = v.GetY(), z = v.GetZ();
+      if (x >= X / 2) x -= X;
+      if (y >= Y / 2) y -= Y;
+      if (z >= Z / 2) z -= Z;
+      const double dx = TWOPI * x * lambda1 / X, dy = TWOPI * y * lambda2 / Y,
+                   dz = TWOPI * z * lambda2 / Z;
+      HermiteH(h1.data(), n1m, dx);
+      HermiteH(h2.data(), n2m, dy);
+      HermiteH(h3.data(), n3m, dz);
+      double re = 0.0, im = 0.0;
+      for (int a = 0; a < n1m; ++a)
+        for (int b = 0; b < n2m; ++b)
+          for (int d = 0; d < n3m; ++d) {
+            const double t = h1[a] * h2[b] * h3[d];
+            re += c[2 * ((a * n2m + b) * n3m + d)] * t;
+            im += c[2 * ((a * n2m + b) * n3m + d) + 1] * t;
+          }
+      const double g = std::exp(-(dx * dx + dy * dy + dz * dz) / 2.0) * norm;
+      psi[2 * i] = re * g;
+      psi[2 * i + 1] = im * g;
+    }
+  }
   Upload();
 }
 
@@ -643,6 +690,28 @@ Float DawsonF(Float x) {
     sum += c[i] * (e1 / d1 + 1.0 / (d2 * e1));
   const double m = std::exp(-xp * xp);
   return (Float)((x >= 0.0 ? m : -m) * sum / 1.7724538509055160273);
+}
+
+// orthonormal Hermite polynomials: h_0 = pi^(-1/4), h_j = x sqrt(2/j) h_{j-1} -
+// sqrt((j-1)/j) h_{j-2}
+Float HermiteH(int n, Float x) {
+  Float p1 = PIM4, p2 = 0.0;
+  for (int j = 1; j <= n; ++j) {
+    const Float p3 = p2;
+    p2 = p1;
+    p1 = x * std::sqrt(2.0 / j) * p2 - std::sqrt((Float)(j - 1) / j) * p3;
+  }
+  return p1;
+}
+
+void HermiteH(Float* h, int n, Float x) {
+  Float a = 0.0;
+  h[0] = PIM4;
+  for (int j = 1; j < n; ++j) {
+    const Float b = a;
+    a = h[j - 1];
+    h[j] = x * std::sqrt(2.0 / j) * a - std::sqrt((Float)(j - 1) / j) * b;
+  }
 }
 
 // ------------------------------------------------------------------ TwoBody

@@ -109,6 +109,10 @@ def lib() -> C.CDLL:
         L.orc_two_body_wavefunc.restype = None
         L.orc_pair_overlap.argtypes = [_dp, _dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_long]
         L.orc_pair_overlap.restype = None
+        L.orc_hermite.argtypes = [_dp, C.c_int, C.c_double]
+        L.orc_hermite.restype = None
+        L.orc_sho_source.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_sho_source.restype = None
         L.orc_time_threads.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_int]
         L.orc_time_threads.restype = C.c_double
         _lib = L
@@ -300,6 +304,23 @@ def pair_overlap(psi1: np.ndarray, psi2: np.ndarray, w: np.ndarray, opp: np.ndar
                            np.ascontiguousarray(w, np.float64),
                            np.ascontiguousarray(opp, np.int32), M, n1, n2, vol)
     return M.view(np.complex128).reshape(n1, n2)
+
+
+def hermite(n: int, x: float) -> np.ndarray:
+    h = np.zeros(n)
+    lib().orc_hermite(h, n, float(x))
+    return h
+
+
+def sho_source(coeffs: np.ndarray, L: int, l1: float, l2: float, l3: float) -> np.ndarray:
+    """OneBody::Set() for SHO sources: coeffs[n1, n2, n3] (complex, n < min(10, L)) -> the
+    k-space source vector (V complex)."""
+    c = np.zeros((L, L, L), np.complex128)
+    n = min(10, L)
+    c[:n, :n, :n] = np.asarray(coeffs)[:n, :n, :n]
+    out = np.zeros(2 * L ** 3)
+    lib().orc_sho_source(out, c.reshape(-1).view(np.float64), L, l1, l2, l3)
+    return out.view(np.complex128)
 
 
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,

@@ -735,6 +735,49 @@ void orc_pair_overlap(const double* psi1, const double* psi2, const double* w, c
 }
 
 /* =========================================================================================
+ * SHO sources
+ * ========================================================================================= */
+void orc_hermite(double* h, int n, double x) {
+  double a = 0.0;
+  h[0] = 0.7511255444649425; /* pi^(-1/4) */
+  for (int j = 1; j < n; ++j) {
+    const double b = a;
+    a = h[j - 1];
+    h[j] = x * sqrt(2.0 / j) * a - sqrt((double)(j - 1) / j) * b;
+  }
+}
+
+void orc_sho_source(double* out, const double* coeffs, int L, double l1, double l2, double l3) {
+  const long V = (long)L * L * L;
+  const int nm = L < 10 ? L : 10;
+  const double tp = 6.2831853071795864;
+  const double norm = sqrt(l1 * l2 * l3) * sqrt(tp * tp * tp / (double)V);
+  double h1[10], h2[10], h3[10];
+  for (long i = 0; i < V; ++i) {
+    int x = (int)(i / ((long)L * L)), y = (int)((i / L) % L), z = (int)(i % L);
+    if (x >= L / 2) x -= L;
+    if (y >= L / 2) y -= L;
+    if (z >= L / 2) z -= L;
+    const double dx = tp * x * l1 / L, dy = tp * y * l2 / L, dz = tp * z * l2 / L;
+    orc_hermite(h1, nm, dx);
+    orc_hermite(h2, nm, dy);
+    orc_hermite(h3, nm, dz);
+    double re = 0.0, im = 0.0;
+    for (int a = 0; a < nm; ++a)
+      for (int b = 0; b < nm; ++b)
+        for (int d = 0; d < nm; ++d) {
+          const long j = ((long)a * L + b) * L + d;
+          const double t = h1[a] * h2[b] * h3[d];
+          re += coeffs[2 * j] * t;
+          im += coeffs[2 * j + 1] * t;
+        }
+    const double g = exp(-(dx * dx + dy * dy + dz * dz) / 2.0) * norm;
+    out[2 * i] = re * g;
+    out[2 * i + 1] = im * g;
+  }
+}
+
+/* =========================================================================================
  * The correlator run (production/slater_det/main.C:185-220 for one configuration)
  * ========================================================================================= */
 static void orc_plane_wave(double* out, int L, const int k[3], int xspace) {

@@ -125,6 +125,15 @@ void orc_two_body_wavefunc(double* out, int lx, int ly, int lz, const int bc[3],
 void orc_pair_overlap(const double* psi1, const double* psi2, const double* w, const int* opp,
                       double* M, int n1, int n2, long vol);
 
+/* ---- SHO sources (src/one_body/one_body.C:53-171; src/utils/special_functions.C:7-34) ---- */
+/* h[0..n-1](x): orthonormal Hermite polynomials from h_0 = pi^(-1/4). */
+void orc_hermite(double* h, int n, double x);
+/* OneBody::Set() for SOURCE_TYPE_SHO on a cubic L^3 lattice: coeffs (V complex) holds c(n) at
+ * momentum index n = (n1, n2, n3), n_i < min(10, L); out (V complex) = sum_n c(n) h_n1(d1)
+ * h_n2(d2) h_n3(d3) exp(-|d|^2/2) sqrt(l1 l2 l3) sqrt((2 pi)^3 / V) with d = 2 pi k (l1, l2,
+ * l2) / L, k centred on 0 (the third axis scaled by l2, as the reference does). */
+void orc_sho_source(double* out, const double* coeffs, int L, double l1, double l2, double l3);
+
 /* ---- the correlator path ---- */
 enum { ORC_SPLIT_KSPACE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
        ORC_SPLIT_XSPACE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */

@@ -75,27 +75,28 @@ DRIVER3 = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det3")
 def test_slater_det3_driver_matches_oracle(orc):
     """The pair-sink driver (SlaterDet3 + TwoBody on the class layer, main-cum.C's loop):
     every determinant Run(n+1, -1, -1) and Run(n+1, -1, n) per slice and the six cumulant
-    powers against the oracle's pair run with the same sources, ranlxs stream, smeared
-    interaction and TGND wave function."""
+    powers against the oracle's pair run with the same SHO sources (OneBody::Set), ranlxs
+    stream, smeared interaction and TGND wave function."""
     assert os.path.exists(DRIVER3), "the slater_det3 driver is not built"
     L, T, ncfg, N = 8, 6, 2, 5
     V = L ** 3
     rng = np.random.default_rng(8)
+    lam = (1.0, 1.1, 0.9)
     src = np.zeros((N, V), np.complex128)
     lines = []
-    # random combinations of a shared set of momenta closed under k -> -k, so that the pair
-    # matrix sum_k w(k) psi_s(k) psi_t(-k) is regular from t = 0 on (as with SHO sources)
-    shared = [(0, 0, 0), (0, 0, 1), (0, 0, L - 1), (0, 1, 0), (0, L - 1, 0), (1, 0, 0),
-              (L - 1, 0, 0), (1, 1, 0), (L - 1, L - 1, 0)]
-    for n in range(N):
-        ks = [(a * L + b) * L + c for a, b, c in shared]
+    for n in range(N):  # SHO sources: random coefficients of the lowest levels
+        c = np.zeros((10, 10, 10), np.complex128)
         out = []
-        for k in ks:
-            c = complex(rng.standard_normal(), rng.standard_normal())
-            src[n, k] = c
-            out.append(f"{k // (L * L)} {(k // L) % L} {k % L} {c.real!r} {c.imag!r}\n")
+        for n1 in range(2):
+            for n2 in range(2):
+                for n3 in range(3):
+                    v = complex(rng.standard_normal(), rng.standard_normal())
+                    c[n1, n2, n3] = v
+                    out.append(f"{n1} {n2} {n3} {v.real!r} {v.imag!r}\n")
+        src[n] = orc.sho_source(c, L, *lam)
         lines.append(out)
     args = dict(ARGS)
+    args["one_body.arg"] = f"SOURCE_TYPE_SHO\n{lam[0]}\n{lam[1]}\n{lam[2]}\n"
     args["propagator.arg"] = f"{N}\nresults/prop\n"
     args["two_body.arg"] = ("WAVEFUNC_TYPE_TGND\nDISPERSION_TYPE_STANDARD\n"
                             "DISPERSION_TYPE_STANDARD\n1.0\n1.0\n0.176776695\nresults/two\n")

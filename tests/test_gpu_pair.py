@@ -47,13 +47,15 @@ def test_pair_overlap_dense(ctx, orc, mode, L, n1, n2, ncfg, bc):
         np.testing.assert_allclose(got[c], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("L,n1,n2,ncfg", [(32, 35, 35, 2), (16, 7, 7, 3), (8, 64, 64, 1),
-                                          (24, 14, 19, 2)])
-def test_pair_overlap_bf16x3_mfma(ctx, orc, L, n1, n2, ncfg):
+@pytest.mark.parametrize("L,n1,n2,ncfg,bc", [(32, 35, 35, 2, (0, 0, 0)), (16, 7, 7, 3, (0, 0, 0)),
+                                             (8, 64, 64, 1, (0, 0, 0)), (24, 14, 19, 2, (0, 0, 0)),
+                                             (16, 9, 5, 2, (1, 1, 1)), (8, 3, 17, 2, (0, 1, 1))])
+def test_pair_overlap_bf16x3_mfma(ctx, orc, L, n1, n2, ncfg, bc):
     """Each product carries ~2^-17 relative error; unit vectors times values with 16
-    significant bits are exact."""
-    print(f"[gpu] pair overlap bf16x3 L={L} n1={n1} n2={n2}", flush=True)
-    a, b, w, opp = _case(orc, L, n1, n2, ncfg, L + n1, (0, 0, 0))
+    significant bits are exact.  Periodic boundaries put the mirrored runs at odd sites,
+    antiperiodic ones at 16-byte aligned sites (both load paths)."""
+    print(f"[gpu] pair overlap bf16x3 L={L} n1={n1} n2={n2} bc={bc}", flush=True)
+    a, b, w, opp = _case(orc, L, n1, n2, ncfg, L + n1, bc)
     a, b = a.astype(np.complex64), b.astype(np.complex64)
     wf = w.astype(np.float32).astype(np.float64)
     M = ctx.array((ncfg, n1, n2), np.complex128)
