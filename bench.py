@@ -87,26 +87,34 @@ def correlator(la, ph):
     return m[:, 0], c
 
 
-def precision_report(ctx, na, L, N, T, tau, A, nconf, seed):
-    """Per-t relative error of C(t) from the FP32 path against the FP64 path on the same
-    configurations (same Philox fields), and the largest log|det| difference."""
+def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,)):
+    """Per-t relative error of C(t) from the FP32 path against the FP64 path (every orbital
+    evolved) on the same configurations (same Philox fields), and the largest log|det|
+    difference; one report per FP32 mode (conjugate partners off / on)."""
     moms = _lowest(L, N)
     kin = _tables(L, tau, 1.0)
-    out = {}
-    for prec in (na.PREC_F32, na.PREC_F64):
-        e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=prec, split=na.SPLIT_KSPACE)
-        out[prec] = e.run(seed, 0, T)
-        e.close()
-    (la32, ph32, _), (la64, ph64, _) = out[na.PREC_F32], out[na.PREC_F64]
-    m32, c32 = correlator(la32, ph32)
+    e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=na.PREC_F64, split=na.SPLIT_KSPACE)
+    la64, ph64, _ = e.run(seed, 0, T)
+    e.close()
     m64, c64 = correlator(la64, ph64)
-    rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
-    return {"configs": nconf, "T": T, "max_rel_err_C": float(rel.max()),
-            "median_rel_err_C": float(np.median(rel)),
+    reports = []
+    for conj in conj_modes:
+        e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+        e.set_symmetry(conj)
+        la32, ph32, _ = e.run(seed, 0, T)
+        e.close()
+        m32, c32 = correlator(la32, ph32)
+        rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+        reports.append({
+            "configs": nconf, "T": T, "conj_partners": bool(conj),
+            "max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
+            "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
             "rel_err_C_at_t": {str(t): float(rel[t]) for t in (0, T // 4, T // 2, 3 * T // 4,
                                                                 T - 1)},
             "max_abs_err_logdet": float(np.abs(la32 - la64).max()),
-            "reference": "FP64 evolution (FFT, multipliers, overlaps) on the same fields"}
+            "reference": "FP64 evolution of every orbital (FFT, multipliers, overlaps) on the "
+                         "same fields"})
+    return reports
 
 
 def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2):
@@ -301,10 +309,14 @@ def main():
                     help="skip the batched uneven-shape measurement")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-proposal measurement")
+    ap.add_argument("--no-other-mode", action="store_true",
+                    help="skip the run of the workload in the other conjugate-partner mode")
     ap.add_argument("--no-pair", action="store_true",
                     help="skip the pair-sink (SlaterDet3) measurement")
     ap.add_argument("--precision-configs", type=int, default=1024,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
+    ap.add_argument("--conj", type=int, default=0,
+                    help="1: conjugate partners (one orbital of each (k, -k) pair evolved)")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
@@ -324,6 +336,7 @@ def main():
     kin = _tables(L, tau, 1.0)
     ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
     ens.set_path(args.path)
+    ens.set_symmetry(bool(args.conj))
     seed = 7012
 
     for w in range(args.warmup):
@@ -353,7 +366,8 @@ def main():
     # the evolution kernels of one slice of every configuration (one single-pass launch, or an
     # X pass and a YZ pass); algorithmic bytes per configuration-slice: read + write the N FP32
     # orbitals once plus phi as one byte per site
-    b_evo = N * 16 * V + V
+    nvo = ens.volumes_per_config()
+    b_evo = nvo * 16 * V + V
     slice_ms = (tx + tyz) / T
     achieved = b_evo * C / (slice_ms * 1e-3) / 1e9
     single = nyz == 0
@@ -369,7 +383,7 @@ def main():
                 "avg_ms": {"evolution_per_slice": slice_ms, "evolution_launch": tx / max(nx, 1),
                            "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
                            "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
-                "bytes_per_cs": b_evo,
+                "bytes_per_cs": b_evo, "volumes_per_config": nvo,
                 "timing": "HIP events on the ensemble's stream around every evolution launch"}
 
     batched = None
@@ -392,9 +406,45 @@ def main():
         pair = pair_sinks(ctx, na)
         print(f"pair sinks: {pair['value']:.0f} configs*slices/s", file=sys.stderr, flush=True)
 
+    # the other mode of the same workload: conjugate partners on when the headline runs
+    # without them, and off when it runs with them
+    other = None
+    if not args.no_other_mode:
+        if ens is not None:
+            ens.close()
+        ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+        ens.set_path(args.path)
+        ens.set_symmetry(not args.conj)
+        ens.run(seed, (1 << 24), T)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ens.run(seed, k * C, T)
+        ctx.sync()
+        dt2 = time.perf_counter() - t0
+        ens.profile(True)
+        ens.run(seed, 1 << 23, T)
+        tx2, nx2 = ens.kernel_time(0)
+        ens.profile(False)
+        nvo2 = ens.volumes_per_config()
+        b2 = nvo2 * 16 * V + V
+        a2 = b2 * C / (tx2 / T * 1e-3) / 1e9
+        other = {"conj_partners": not args.conj, "value": C * T * args.steps / dt2,
+                 "unit": "configs*slices/s", "volumes_per_config": nvo2,
+                 "evolution_ms_per_slice": tx2 / T,
+                 "roofline": {"achieved": a2, "frac": a2 / HBM_PEAK_GBS, "bytes_per_cs": b2}}
+        print(f"other mode (conj={int(not args.conj)}): {other['value']:.0f} configs*slices/s",
+              file=sys.stderr, flush=True)
+        ens.close()
+        ens = None
+
     precision = None
     if args.precision_configs > 0:
-        precision = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed)
+        modes = (bool(args.conj),) if args.no_other_mode else (bool(args.conj), not args.conj)
+        reps = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed, modes)
+        precision = reps[0]
+        if len(reps) > 1 and other is not None:
+            other["precision"] = reps[1]
         print(f"precision: max rel err of C(t) {precision['max_rel_err_C']:.2e}",
               file=sys.stderr, flush=True)
 
@@ -418,11 +468,13 @@ def main():
         "batched_uneven": batched,
         "learned_proposal": learned,
         "pair_sinks": pair,
+        "other_mode": other,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]
     print(json.dumps(out))
-    ens.close()
+    if ens is not None:
+        ens.close()
     ctx.close()
 
 

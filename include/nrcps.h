@@ -169,6 +169,18 @@ int nrc_ensemble_orbitals(nrc_ensemble* ens, int c, void* out);
 #define NRC_PATH_TWO_PASS 1
 #define NRC_PATH_SINGLE_PASS 2
 int nrc_ensemble_set_path(nrc_ensemble* ens, int path);
+/* Conjugate partners (conj = 1; 0 turns them off).  Fields are real and the kinetic (and
+ * smear) tables even under p -> -p, so the evolution commutes with psi(p) -> conj psi(-p)
+ * (k-space; conj psi(x) in x-space) and the orbital of momentum -k stays the conjugate mirror
+ * of the one of momentum k.  One orbital of each (k, -k) pair of the set is then evolved and
+ * the other's sink values are read conjugated at the mirrored sinks: the same results (to
+ * rounding) from (N + self-conjugate orbitals) / 2 volumes per configuration.  The reference
+ * evolves every orbital (src/propagator/propagator.C:35-64); this has no counterpart there.
+ * Plane-wave sinks only (NRC_ERR_UNSUPPORTED with dense sinks, pair sinks or tables that are
+ * not even to 1e-12 relative); not while the ensemble is in a batch; call before a run. */
+int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
+/* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
+int nrc_ensemble_volumes(nrc_ensemble* ens, int* volumes_per_config);
 /* Per-kernel timing with HIP events on the ensemble's stream (enable before a run).
  * kind: 0 X pass or single-pass slice kernel, 1 YZ pass, 2 LU, 3 field generation,
  * 4 sink gather.  Returns total ms and launches. */

@@ -30,6 +30,7 @@ EXPORTS = [
     "nrc_plus_one_div_vol",
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
+    "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
     "nrc_ensemble_destroy", "nrc_batch_create", "nrc_batch_run", "nrc_batch_capture_ms",
@@ -84,6 +85,7 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_run": [vp, u64, u32, i, dp, dp, dp],
         "nrc_ensemble_run_fields": [vp, vp, i, dp, dp, dp],
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
+        "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
         "nrc_ensemble_profile": [vp, i],
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "nrc_ensemble_results": [vp, dp, dp, dp],
@@ -320,6 +322,16 @@ class Ensemble:
     def set_path(self, path: int):
         """PATH_AUTO, PATH_TWO_PASS or PATH_SINGLE_PASS (nrc_ensemble_set_path)."""
         _check(self.lib.nrc_ensemble_set_path(self.h, path), "ensemble_set_path")
+
+    def set_symmetry(self, conj: bool):
+        """Conjugate partners (nrc_ensemble_set_symmetry): evolve one orbital of each (k, -k)
+        pair and read the other's sink values as the conjugate mirror.  Call before a run."""
+        _check(self.lib.nrc_ensemble_set_symmetry(self.h, int(conj)), "ensemble_set_symmetry")
+
+    def volumes_per_config(self) -> int:
+        n = C.c_int()
+        _check(self.lib.nrc_ensemble_volumes(self.h, C.byref(n)), "ensemble_volumes")
+        return n.value
 
     def profile(self, enable: bool):
         _check(self.lib.nrc_ensemble_profile(self.h, int(enable)), "ensemble_profile")

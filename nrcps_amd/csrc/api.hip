@@ -346,6 +346,23 @@ struct nrc_ensemble {
   int* g_sink = nullptr;
   int* site_of_orb = nullptr;
   int* moms_dev = nullptr;
+  // conjugate partners (nrc_ensemble_set_symmetry): with real fields and tables even under
+  // p -> -p, the orbital of momentum -k is the conjugate mirror of the one of momentum k, so
+  // one volume per pair is evolved; nvo volumes per configuration (N without partners)
+  bool sym_ok = false;       // the kinetic (and smear) tables are even under p -> -p
+  int nvo = 0;
+  int* sym = nullptr;        // one device allocation holding the lists below
+  const int* vo_site = nullptr;   // [nvo] source site of each volume's orbital
+  const int* vo_moms = nullptr;   // [3 nvo] its momentum
+  const int* vo_col = nullptr;    // [nvo] its column of M
+  const int* vo_pcol = nullptr;   // [nvo] the column of its conjugate partner, or -1
+  const int* g2_ptr = nullptr;    // gather lists of the mirrored sink sites -k_s
+  const int* g2_pos = nullptr;
+  const int* g2_sink = nullptr;
+  const int* col_vol = nullptr;   // [N] volume of each column, conjugated when col_conj
+  const int* col_conj = nullptr;
+  const int* msite = nullptr;     // [N] site of -k_s
+  std::vector<int> h_col_vol, h_col_conj;
   int path = NRC_PATH_AUTO;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
   bool profile = false;
@@ -361,7 +378,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
-                  e->site_of_orb, e->moms_dev};
+                  e->site_of_orb, e->moms_dev, e->sym};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
@@ -397,6 +414,12 @@ template <typename X>
 static int dev_alloc(X** p, size_t n) {
   if (hipMalloc((void**)p, (n ? n : 1) * sizeof(X)) != hipSuccess) return NRC_ERR_NOMEM;
   return NRC_OK;
+}
+
+// site of -p for the site of p: (n0, n1, n2) -> ((L - n0) % L, ...), z fastest
+static long mirror_site(int L, long i) {
+  const long n0 = i / ((long)L * L), n1 = (i / L) % L, n2 = i % L;
+  return (((L - n0) % L) * L + (L - n1) % L) * L + (L - n2) % L;
 }
 
 extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
@@ -439,6 +462,18 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   } while (0)
   ENS_TRY(hipMalloc(&e->psi, (size_t)C * N * V * elem_bytes(d->prec)) == hipSuccess
               ? NRC_OK : NRC_ERR_NOMEM);
+  e->nvo = N;
+  // tables even under p -> -p, to 1e-12 relative (sin^2(pi n / L) and sin^2(pi (L - n) / L)
+  // differ in the last bit): the conjugate mirror then holds to rounding
+  e->sym_ok = true;
+  double smax = 0.0;
+  for (long i = 0; d->smear && i < V; ++i) smax = std::max(smax, std::fabs(d->smear[i]));
+  for (long i = 0; i < V && e->sym_ok; ++i) {
+    const long m = mirror_site(L, i);
+    const double a = d->kin[i], b = d->kin[m];
+    if (!(std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)))) e->sym_ok = false;
+    if (d->smear && !(std::fabs(d->smear[i] - d->smear[m]) <= 1e-12 * smax)) e->sym_ok = false;
+  }
   std::vector<double> h(V);
   if (d->split == NRC_SPLIT_KSPACE) {
     e->khalf.resize(V);
@@ -734,20 +769,28 @@ static nrc::GatherSpec ens_gather_spec(const nrc_ensemble* e) {
   g.M = e->M;
   g.nsink = e->d.n_orb;
   g.scale = e->d.split == NRC_SPLIT_KSPACE ? 1.0 : 1.0 / sqrt((double)e->V);
+  if (e->sym) {
+    g.col = e->vo_col;
+    g.pcol = e->vo_pcol;
+    g.ptr2 = e->g2_ptr;
+    g.pos2 = e->g2_pos;
+    g.sink2 = e->g2_sink;
+    g.ncol = e->d.n_orb;
+  }
   return g;
 }
 
 // Propagator::Set(one_bodies[n]->Get(), n): plane-wave sources in every configuration
 static int ens_set_sources(nrc_ensemble* e) {
-  const int N = e->d.n_orb;
   const long C = e->d.n_configs;
   e->folded = false;
+  const int nvo = e->nvo;
   if (e->d.split == NRC_SPLIT_KSPACE)
-    NRC_HIP(nrc::launch_set_unit_sources(e->d.prec, e->psi, e->site_of_orb, C * N, N, e->V,
-                                         e->stream));
+    NRC_HIP(nrc::launch_set_unit_sources(e->d.prec, e->psi, e->sym ? e->vo_site : e->site_of_orb,
+                                         C * nvo, nvo, e->V, e->stream));
   else
-    NRC_HIP(nrc::launch_set_plane_waves(e->d.prec, e->psi, e->moms_dev, C * N, N, e->d.L,
-                                        e->stream));
+    NRC_HIP(nrc::launch_set_plane_waves(e->d.prec, e->psi, e->sym ? e->vo_moms : e->moms_dev,
+                                        C * nvo, nvo, e->d.L, e->stream));
   return NRC_OK;
 }
 
@@ -801,9 +844,17 @@ static int ens_measure(nrc_ensemble* e, int t) {
       NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks)
       NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s));
-    else
+    else {
+      nrc::ConjCols cp;
+      if (e->sym) {
+        cp.vol = e->col_vol;
+        cp.conj = e->col_conj;
+        cp.msite = e->msite;
+        cp.nvo = e->nvo;
+      }
       NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb,
-                                       e->folded ? e->sink_k : nullptr, e->M, C, N, V, s));
+                                       e->folded ? e->sink_k : nullptr, e->M, C, N, V, s, cp));
+    }
     if (e->pair) {
       const double* w = e->pw + (e->folded ? V : 0);
       if (prec == NRC_PREC_F64)
@@ -833,8 +884,8 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
   fieldspec.bits = words;
   nrc::SliceArgs a;
   a.psi = e->psi;
-  a.nvol = C * N;
-  a.norb = N;
+  a.nvol = C * e->nvo;
+  a.norb = e->nvo;
   a.L = L;
   a.grid = e->ctx->n_cu;  // persistent kernel: workgroups per CU x CUs
   if (ref) {
@@ -880,8 +931,8 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
   const long mstride = C * N * N;
   nrc::SliceArgs a;
   a.psi = e->psi;
-  a.nvol = C * N;
-  a.norb = N;
+  a.nvol = C * e->nvo;
+  a.norb = e->nvo;
   a.L = e->d.L;
   a.grid = e->ctx->n_cu;
   a.pre = mul_table(e->folded ? e->tab_k : e->tab_x);
@@ -986,8 +1037,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     {
       nrc::PassArgs a;
       a.psi = e->psi;
-      a.nvol = C * N;
-      a.norb = N;
+      a.nvol = C * e->nvo;
+      a.norb = e->nvo;
       a.L = L;
       a.fft_a = t > 0;
       a.fft_b = !last;
@@ -1019,8 +1070,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     {
       nrc::PassArgs a;
       a.psi = e->psi;
-      a.nvol = C * N;
-      a.norb = N;
+      a.nvol = C * e->nvo;
+      a.norb = e->nvo;
       a.L = L;
       a.fft_a = 1;
       a.fft_b = 1;
@@ -1079,12 +1130,37 @@ extern "C" int nrc_ensemble_run_fields(nrc_ensemble* e, const uint32_t* words, i
   return ens_run(e, 0, 0, words, T, logabs, phase, argsum);
 }
 
+// the orbitals of configuration c: the stored volumes, with the conjugate mirrors of the
+// partners filled in (psi_{-k}(p) = conj psi_k(-p) in k-space, conj psi_k(x) in x-space)
+template <typename T>
+static void ens_expand_partners(const nrc_ensemble* e, T* o) {
+  const int N = e->d.n_orb, nvo = e->nvo;
+  const long V = e->V;
+  const bool kspace = e->d.split == NRC_SPLIT_KSPACE;
+  std::vector<T> vol((size_t)2 * nvo * V);
+  std::copy(o, o + vol.size(), vol.begin());
+  for (int n = 0; n < N; ++n) {
+    const T* src = vol.data() + (size_t)2 * e->h_col_vol[n] * V;
+    T* dst = o + (size_t)2 * n * V;
+    const bool cj = e->h_col_conj[n] != 0;
+    for (long i = 0; i < V; ++i) {
+      const long j = cj && kspace ? mirror_site(e->d.L, i) : i;
+      dst[2 * i] = src[2 * j];
+      dst[2 * i + 1] = cj ? -src[2 * j + 1] : src[2 * j + 1];
+    }
+  }
+}
+
 extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
   if (!e || !out || c < 0 || c >= e->d.n_configs) return NRC_ERR_ARG;
-  const size_t bytes = (size_t)e->d.n_orb * e->V * elem_bytes(e->d.prec);
-  NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * bytes, bytes, hipMemcpyDeviceToHost,
-                         e->stream));
+  const size_t vbytes = (size_t)e->V * elem_bytes(e->d.prec);
+  NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * e->nvo * vbytes, e->nvo * vbytes,
+                         hipMemcpyDeviceToHost, e->stream));
   NRC_HIP(hipStreamSynchronize(e->stream));
+  if (e->sym) {
+    if (e->d.prec == NRC_PREC_F64) ens_expand_partners(e, (double*)out);
+    else ens_expand_partners(e, (float*)out);
+  }
   const size_t n = (size_t)e->d.n_orb * e->V;
   if (e->d.prec == NRC_PREC_F64) {
     if (e->folded) {  // stored xi_half psi: apply the slice's trailing K(1/2)
@@ -1103,6 +1179,106 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
       o[2 * i + 1] = (float)((double)o[2 * i + 1] * f);
     }
   }
+  return NRC_OK;
+}
+
+// Conjugate partners: with a real field and kinetic (and smear) tables even under p -> -p,
+// the evolution commutes with psi(p) -> conj psi(-p), and the plane-wave source of momentum
+// -k is the image of the one of momentum k; so only one orbital of each (k, -k) pair is
+// evolved and the other's sink values are read conjugated at the mirrored sinks.  Orbital
+// n is a representative unless an earlier orbital already claimed it as its partner.
+extern "C" int nrc_ensemble_volumes(nrc_ensemble* e, int* nvo) {
+  if (!e || !nvo) return NRC_ERR_ARG;
+  *nvo = e->nvo;
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
+  if (!e || (conj != 0 && conj != 1)) return NRC_ERR_ARG;
+  if (e->stream != e->ctx->stream) return NRC_ERR_ARG;  // not while in a batch
+  if (conj && (!e->sym_ok || e->pair || e->sinks)) return NRC_ERR_UNSUPPORTED;
+  hipSetDevice(e->ctx->device);
+  NRC_HIP(hipStreamSynchronize(e->stream));
+  if (e->sym) hipFree(e->sym);
+  e->sym = nullptr;
+  e->nvo = e->d.n_orb;
+  e->h_col_vol.clear();
+  e->h_col_conj.clear();
+  if (!conj) return NRC_OK;
+  const int N = e->d.n_orb, L = e->d.L;
+  const int* k = e->moms.data();
+  auto site_of = [&](int a, int b, int c2) { return (a * L + b) * L + c2; };
+  auto neg = [&](int a) { return (L - a) % L; };
+  std::vector<int> col_vol(N, -1), col_conj(N, 0), vo_col, vo_pcol, vo_site, vo_moms;
+  for (int n = 0; n < N; ++n) {
+    if (col_vol[n] >= 0) continue;  // the partner of an earlier representative
+    const int r = (int)vo_col.size();
+    col_vol[n] = r;
+    const bool self = neg(k[3 * n]) == k[3 * n] && neg(k[3 * n + 1]) == k[3 * n + 1] &&
+                      neg(k[3 * n + 2]) == k[3 * n + 2];
+    int pc = -1;
+    for (int m = n + 1; m < N && !self && pc < 0; ++m)
+      if (col_vol[m] < 0 && k[3 * m] == neg(k[3 * n]) && k[3 * m + 1] == neg(k[3 * n + 1]) &&
+          k[3 * m + 2] == neg(k[3 * n + 2]))
+        pc = m;
+    if (pc >= 0) {
+      col_vol[pc] = r;
+      col_conj[pc] = 1;
+    }
+    vo_col.push_back(n);
+    vo_pcol.push_back(pc);
+    vo_site.push_back(site_of(k[3 * n], k[3 * n + 1], k[3 * n + 2]));
+    for (int j = 0; j < 3; ++j) vo_moms.push_back(k[3 * n + j]);
+  }
+  const int nvo = (int)vo_col.size();
+  // gather lists of the mirrored sinks, in the pencil layout of the plain ones: k-space split
+  // x-pencils (pencil (k1, k2), position k0), x-space split y-columns (pencil (k0, k2),
+  // position k1)
+  const int npen = L * L;
+  std::vector<int> cnt(npen + 1, 0), pen(N), pos(N), msite(N);
+  for (int s = 0; s < N; ++s) {
+    const int m0 = neg(k[3 * s]), m1 = neg(k[3 * s + 1]), m2 = neg(k[3 * s + 2]);
+    msite[s] = site_of(m0, m1, m2);
+    pen[s] = e->d.split == NRC_SPLIT_KSPACE ? m1 * L + m2 : m0 * L + m2;
+    pos[s] = e->d.split == NRC_SPLIT_KSPACE ? m0 : m1;
+    cnt[pen[s] + 1]++;
+  }
+  for (int p = 0; p < npen; ++p) cnt[p + 1] += cnt[p];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1), gpos(N), gsink(N);
+  for (int s = 0; s < N; ++s) {
+    const int slot = fill[pen[s]]++;
+    gpos[slot] = pos[s];
+    gsink[slot] = s;
+  }
+  std::vector<int> all;
+  auto put = [&](const std::vector<int>& v) {
+    const size_t o = all.size();
+    all.insert(all.end(), v.begin(), v.end());
+    return o;
+  };
+  const size_t o_site = put(vo_site), o_moms = put(vo_moms), o_col = put(vo_col),
+               o_pcol = put(vo_pcol), o_ptr = put(cnt), o_pos = put(gpos), o_sink = put(gsink),
+               o_cv = put(col_vol), o_cc = put(col_conj), o_ms = put(msite);
+  int* d = nullptr;
+  NRC_TRY(dev_alloc(&d, all.size()));
+  if (hipMemcpy(d, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(d);
+    return NRC_ERR_HIP;
+  }
+  e->sym = d;
+  e->vo_site = d + o_site;
+  e->vo_moms = d + o_moms;
+  e->vo_col = d + o_col;
+  e->vo_pcol = d + o_pcol;
+  e->g2_ptr = d + o_ptr;
+  e->g2_pos = d + o_pos;
+  e->g2_sink = d + o_sink;
+  e->col_vol = d + o_cv;
+  e->col_conj = d + o_cc;
+  e->msite = d + o_ms;
+  e->nvo = nvo;
+  e->h_col_vol = col_vol;
+  e->h_col_conj = col_conj;
   return NRC_OK;
 }
 

@@ -31,6 +31,12 @@ struct MulSpec {
 // Plane-wave sink measurement folded into a pass: for each pencil p of a volume, the
 // entries ptr[p] .. ptr[p+1] name (position within the pencil, sink index); the value at
 // that position (times `scale`) is written to M[(config * nsink + sink) * norb + orbital].
+//
+// Conjugate partners (col != null): volume `orbital` of a configuration holds orbital
+// col[orbital] and M has ncol columns.  When pcol[orbital] >= 0 the volume also stands for
+// the orbital of opposite momentum, whose state is the conjugate mirror of this one
+// (psi_{-k}(p) = conj psi_k(-p) for real fields and even kinetic tables); the entries
+// ptr2 / pos2 / sink2 name the sites -k_s, and conj(value) goes to column pcol[orbital].
 struct GatherSpec {
   const int* ptr = nullptr;
   const int* pos = nullptr;
@@ -39,6 +45,12 @@ struct GatherSpec {
   int nsink = 0;
   double scale = 1.0;
   const double* sink_scale = nullptr;  // per-sink factor (null: 1)
+  const int* col = nullptr;
+  const int* pcol = nullptr;
+  const int* ptr2 = nullptr;
+  const int* pos2 = nullptr;
+  const int* sink2 = nullptr;
+  int ncol = 0;
 };
 
 struct PassArgs {
@@ -82,10 +94,19 @@ struct SliceArgs {
 constexpr int SLICE_FUSE_MAX = 4;  // slices per launch of the persistent slice kernel
 bool slice1_supported(int prec, int L);
 hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);
+// Columns of conjugate partners for the gather from memory (all null: column n is volume n):
+// column n is volume vol[n] of its configuration (nvo volumes each), read at the mirrored
+// sink site msite[s] and conjugated when conj[n] != 0.
+struct ConjCols {
+  const int* vol = nullptr;
+  const int* conj = nullptr;
+  const int* msite = nullptr;
+  int nvo = 0;
+};
 // M[c][s][n] = psi[c][n][site_of_sink[s]] (plane-wave sinks on k-space orbitals)
 hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink,
                                const double* sink_scale, double2* M, long C, int N, long V,
-                               hipStream_t s);
+                               hipStream_t s, const ConjCols& cp = ConjCols());
 
 // element-wise helpers (the reference's separate kernels, used by the class layer)
 hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch, long vol,

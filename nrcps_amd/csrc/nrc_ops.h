@@ -163,12 +163,24 @@ __device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, 
 template <typename T, int L>
 __device__ __forceinline__ void gather_pencil(const C2<T>* v, const GatherSpec& g, int pencil,
                                               long cfg, int orb, int norb) {
+  const int nc = g.col ? g.ncol : norb;
+  const int c0 = g.col ? g.col[orb] : orb;
   const int e0 = g.ptr[pencil], e1 = g.ptr[pencil + 1];
   for (int e = e0; e < e1; ++e) {
     const C2<T> val = reg_select<T, L>(v, g.pos[e]);
     const double f = g.sink_scale ? g.scale * g.sink_scale[g.sink[e]] : g.scale;
-    g.M[(cfg * g.nsink + g.sink[e]) * norb + orb] =
+    g.M[(cfg * g.nsink + g.sink[e]) * nc + c0] =
         make_double2((double)val.x * f, (double)val.y * f);
+  }
+  if (!g.col) return;
+  const int pc = g.pcol[orb];
+  if (pc < 0) return;
+  const int m0 = g.ptr2[pencil], m1 = g.ptr2[pencil + 1];
+  for (int e = m0; e < m1; ++e) {
+    const C2<T> val = reg_select<T, L>(v, g.pos2[e]);
+    const double f = g.sink_scale ? g.scale * g.sink_scale[g.sink2[e]] : g.scale;
+    g.M[(cfg * g.nsink + g.sink2[e]) * nc + pc] =
+        make_double2((double)val.x * f, -(double)val.y * f);
   }
 }
 

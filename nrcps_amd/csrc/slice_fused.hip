@@ -120,34 +120,39 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1(SliceArgs a) {
 // M[c][s][n] = psi[c][n][site_of_sink[s]]: the measurement of plane-wave sinks on orbitals
 // stored in k-space (OneBody::Project of a unit vector, src/one_body/noarch/
 // one_body_project.C:14-27), read straight from memory before the slice overwrites them.
+// With conjugate partners (cp.vol != null) a configuration holds cp.nvo volumes and column n
+// is volume cp.vol[n], read at the mirrored sink site cp.msite[s] and conjugated when
+// cp.conj[n] != 0.
 template <typename T>
 __global__ void k_gather_sites(const C2<T>* psi, const int* site_of_sink, const double* scale,
-                               double2* M, long C, int N, long V) {
+                               double2* M, long C, int N, long V, ConjCols cp) {
   const long n = C * N * N;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
     const long c = i / ((long)N * N);
     const int r = (int)(i - c * N * N), s = r / N, o = r - (r / N) * N;
-    const C2<T> val = psi[(c * N + o) * V + site_of_sink[s]];
+    const bool cj = cp.vol && cp.conj[o];
+    const long vol = cp.vol ? c * cp.nvo + cp.vol[o] : c * N + o;
+    const C2<T> val = psi[vol * V + (cj ? cp.msite[s] : site_of_sink[s])];
     const double f = scale ? scale[s] : 1.0;
-    M[i] = make_double2((double)val.x * f, (double)val.y * f);
+    M[i] = make_double2((double)val.x * f, (cj ? -(double)val.y : (double)val.y) * f);
   }
 }
 
 // M[c][s][n] = sink_scale[s] psi[c][n][site_of_sink[s]] (sink_scale null: 1)
 hipError_t launch_gather_sites(int prec, const void* psi, const int* site_of_sink,
                                const double* sink_scale, double2* M, long C, int N, long V,
-                               hipStream_t s) {
+                               hipStream_t s, const ConjCols& cp) {
   const long n = C * N * N;
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
   if (prec == NRC_PREC_F64)
     hipLaunchKernelGGL(k_gather_sites<double>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const C2<double>*)psi, site_of_sink, sink_scale, M, C, N, V);
+                       (const C2<double>*)psi, site_of_sink, sink_scale, M, C, N, V, cp);
   else
     hipLaunchKernelGGL(k_gather_sites<float>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const C2<float>*)psi, site_of_sink, sink_scale, M, C, N, V);
+                       (const C2<float>*)psi, site_of_sink, sink_scale, M, C, N, V, cp);
   return hipGetLastError();
 }
 
