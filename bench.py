@@ -26,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+VALU_SIMDS = 256 * 4  # MI355X: 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9  # peak engine clock
 
 
 def _tables(L, tau, M):
@@ -62,20 +64,21 @@ def cpu_baseline(L, N, tau, A, slices, threads):
                       f"{secs:.2f} s wall"}
 
 
-def measured_traffic(L, N, C, per_launch):
-    """HBM bytes per launch of the slice kernel on this workload, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or None if it was taken on
-    another shape."""
+def measured_counters(L, N, C, per_launch, nvo):
+    """The committed rocprofv3 counter record of the slice kernel on this workload
+    (profiles/traffic.json: HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, and VALU wave
+    instructions per launch where measured), or None if none was taken on this shape."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            t = json.load(f)
+            recs = json.load(f)
     except (OSError, ValueError):
         return None
-    if (t.get("L"), t.get("N"), t.get("configs"), t.get("slices_per_launch")) != (L, N, C,
-                                                                                 per_launch):
-        return None
-    return t["traffic_bytes_per_launch"]
+    for t in recs if isinstance(recs, list) else [recs]:
+        if (t.get("L"), t.get("N"), t.get("configs"), t.get("slices_per_launch"),
+                t.get("volumes_per_config")) == (L, N, C, per_launch, nvo):
+            return t
+    return None
 
 
 def correlator(la, ph):
@@ -117,7 +120,7 @@ def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,))
     return reports
 
 
-def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2):
+def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2, conj=True):
     """4096 configurations of uneven shape: L uniform in {16, 24, 32}, N uniform in
     {1, 7, 19, 27, 33} (closed shells), drawn with a fixed seed and grouped by shape; every
     group is autotuned, then all groups run together, T slices captured into one HIP graph.
@@ -131,9 +134,10 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
         cnt = int(np.sum((Ls == L) & (Ns == N)))
         e = na.Ensemble(ctx, L, _lowest(L, N), _tables(L, tau, 1.0), A, cnt, prec=na.PREC_F32,
                         split=na.SPLIT_KSPACE)
+        e.set_symmetry(conj)
         path, ms = e.tune(3)
         groups.append(e)
-        per_shape.append({"L": L, "N": N, "configs": cnt,
+        per_shape.append({"L": L, "N": N, "configs": cnt, "volumes_per_config": e.volumes_per_config(),
                           "path": "single-pass" if path == na.PATH_SINGLE_PASS else "two-pass",
                           "tuned_ms_per_slice": ms})
     # each shape alone
@@ -161,7 +165,7 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
     (tg, cap), (tn, _) = out[True], out[False]
     return {"workload": f"{nconf} configurations, L uniform in {{16,24,32}}, N uniform in "
                         f"{{1,7,19,27,33}}, T={T}, tau={tau}, A={A}, FP32 evolution, FP64 LU",
-            "configs": nconf, "T": T, "shape_groups": len(groups),
+            "configs": nconf, "T": T, "shape_groups": len(groups), "conj_partners": bool(conj),
             "value": nconf * T / tg, "unit": "configs*slices/s",
             "ms_per_run": 1e3 * tg, "graph": True,
             "capture_and_instantiate_ms": cap,
@@ -172,7 +176,7 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
 
 def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, n_plain=65536,
                      n_learned=16384, steps=300, batch=64, lr=4e-3, seed=7012, optimizer=0,
-                     out_scale=0.0):
+                     out_scale=0.0, conj=True):
     """Effective samples per second of the learned proposal against the uniform sampler.
     The uniform sampler generates n_train configurations; the network is trained on them
     (weighted maximum likelihood, det^2 weights); then n_plain uniform and n_learned proposed
@@ -185,6 +189,7 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     ts = [T // 4, T // 2, 3 * T // 4]
     chunk = min(n_train, n_learned, 16384)
     ens = na.Ensemble(ctx, L, moms, kin, A, chunk, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+    ens.set_symmetry(conj)  # the same evolution engine for both samplers
     ens.run(seed, 0, 2)  # warm-up
     # uniform sampler: the training sample, then the comparison sample
     t0 = time.perf_counter()
@@ -231,12 +236,14 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "gain_with_training_amortised": (vp * tau_p) / (vl * tau_l_amort)}
     ps = prop.get_params()
     prop.close()
+    ens_nvo = ens.volumes_per_config()
     ens.close()
     w = np.exp(logr[ts[-1]] - logr[ts[-1]].max())  # the ratios r = prod 0.5 / q alone
     return {"workload": f"L={L}, N={N}, T={T}, tau={tau}, A={A}; {n_train} uniform "
                         f"configurations for training, {n_plain} uniform and {n_learned} "
                         f"proposed configurations for C(t)",
             "metric": "effective samples/s at equal statistical error of C(t)",
+            "conj_partners": bool(conj), "volumes_per_config": ens_nvo,
             "gain_min_over_t": min(v["gain"] for v in per_t.values()),
             "per_t": per_t,
             "proposal_weight_ess_fraction_at_3T/4": float(w.sum() ** 2 / (w ** 2).sum() / w.size),
@@ -315,8 +322,9 @@ def main():
                     help="skip the pair-sink (SlaterDet3) measurement")
     ap.add_argument("--precision-configs", type=int, default=1024,
                     help="configurations of the FP32-vs-FP64 correlator check (0: skip)")
-    ap.add_argument("--conj", type=int, default=0,
-                    help="1: conjugate partners (one orbital of each (k, -k) pair evolved)")
+    ap.add_argument("--conj", type=int, default=1,
+                    help="1: conjugate partners (one orbital of each (k, -k) pair evolved); "
+                         "0: every orbital evolved, as the reference does")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
@@ -375,28 +383,37 @@ def main():
     kernel = (f"k_slice1p ({per_launch:g} slices per launch)" if single and per_launch > 1
               else "k_slice1 (one launch per slice)" if single
               else "k_xpass + k_yzpass (one slice)")
+    cnt = measured_counters(L, N, C, per_launch, nvo) if single else None
+    valu = None
+    if cnt and cnt.get("valu_insts_per_launch"):
+        # wave64 FP32 VALU instructions (packed FP32 off) take 4 cycles of a 16-lane SIMD
+        peak = VALU_SIMDS * VALU_CLOCK_HZ / 4.0
+        rate = cnt["valu_insts_per_launch"] / (tx / max(nx, 1) * 1e-3)
+        valu = {"achieved_wave_insts_per_s": rate, "peak_wave_insts_per_s": peak,
+                "frac": rate / peak, "insts_per_launch": cnt["valu_insts_per_launch"],
+                "source": "SQ_INSTS_VALU per launch (profiles) / this run's launch time"}
     roofline = {"bound": "hbm", "kernel": kernel,
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(L, N, C, per_launch) if single else None,
+                "traffic": cnt["traffic_bytes_per_launch"] if cnt else None,
                 "algorithmic_bytes_per_launch": b_evo * C * per_launch,
                 "avg_ms": {"evolution_per_slice": slice_ms, "evolution_launch": tx / max(nx, 1),
                            "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
                            "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
-                "bytes_per_cs": b_evo, "volumes_per_config": nvo,
+                "bytes_per_cs": b_evo, "volumes_per_config": nvo, "valu": valu,
                 "timing": "HIP events on the ensemble's stream around every evolution launch"}
 
     batched = None
     if not args.no_batched:
         ens.close()
-        batched = batched_uneven(ctx, na)
+        batched = batched_uneven(ctx, na, conj=bool(args.conj))
         print(f"batched uneven shapes: {batched['value']:.0f} configs*slices/s",
               file=sys.stderr, flush=True)
 
     learned = None
     if not args.no_learned:
         ens.close()
-        learned = learned_proposal(ctx, na)
+        learned = learned_proposal(ctx, na, conj=bool(args.conj))
         print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
 
