@@ -255,7 +255,50 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                          "output_bias": float(ps[8240])}}
 
 
-def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012):
+def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012):
+    """The overlap matrix Phi^H Psi(t) as a batched complex GEMM on the matrix cores (dense
+    sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16x3 products
+    with FP32 accumulation, every orbital read once per measurement.  The sinks are the
+    plane waves as dense k-space vectors, so the gathered measurement is the check.  Reports
+    the overlap kernel time per slice against the MFMA peak (3 x 8 N^2 V bf16 flops per
+    configuration-slice) and the HBM peak (8 N V bytes read per configuration-slice)."""
+    moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
+    V = L ** 3
+    sinks = np.zeros((N, V), np.complex128)
+    sinks[np.arange(N), (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]] = 1.0
+    e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE,
+                    sinks=sinks)
+    e.set_path(na.PATH_SINGLE_PASS)
+    e.run(seed, 0, 2)
+    ctx.sync()
+    t0 = time.perf_counter()
+    la_d, _, _ = e.run(seed, 0, T)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    e.profile(True)
+    e.run(seed, 0, T)
+    t_ovl, n_ovl = e.kernel_time(4)
+    e.profile(False)
+    e.close()
+    g = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+    la_g, _, _ = g.run(seed, 0, T)
+    g.close()
+    ms = t_ovl / max(n_ovl, 1)
+    flops = 3 * 8.0 * N * N * V * C
+    byts = 8.0 * N * V * C
+    return {"workload": f"dense sinks: L={L}, N={N}, T={T}, {C} configurations, FP32 evolution, "
+                        f"split-BF16x3 overlaps on MFMA, FP64 LU",
+            "value": C * T / dt, "unit": "configs*slices/s",
+            "overlap_ms_per_slice": ms,
+            "mfma": {"achieved_tflops": flops / (ms * 1e-3) / 1e12, "peak_tflops": 2500.0,
+                     "frac": flops / (ms * 1e-3) / 2.5e15},
+            "hbm": {"achieved_gbs": byts / (ms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                    "frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "max_abs_diff_logdet_vs_gather": float(np.abs(la_d - la_g).max()),
+            "timing": "HIP events around the overlap kernels of each measurement"}
+
+
+def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, conj=True):
     """The pair-sink measurement of production/slater_det3_trapped (L=32, N=35): every slice
     the pair matrix phi[s][t] = sum_p w(p) psi_s(p) psi_t(-p) of every configuration (split-
     BF16x3 on MFMA) and SlaterDet3's 2N determinants, plane-wave sources, the PAIR1 wave
@@ -270,6 +313,8 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012):
     opp = ((m[:, None, None] * L + m[None, :, None]) * L + m[None, None, :]).reshape(-1)
     ens = na.Ensemble(ctx, L, _lowest(L, N), _tables(L, tau, 1.0), A, C, prec=na.PREC_F32,
                       split=na.SPLIT_KSPACE, pair_w=w, pair_opp=opp.astype(np.int32))
+    ens.set_symmetry(conj)
+    nvo = ens.volumes_per_config()
     ens.run(seed, 0, 2)
     ctx.sync()
     t0 = time.perf_counter()
@@ -285,10 +330,14 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012):
     ens.close()
     V = L ** 3
     meas_ms = t_meas / max(n_meas, 1)  # gather + pair matrices of one slice
-    flops = 3 * 8.0 * N * N * V * C
+    # products computed: N x N pair matrices, or with conjugate partners two nvo x nvo ones
+    # (mirrored sites, and the same sites conjugated) over the nvo stored volumes
+    prods = 2 * nvo * nvo if nvo < N else N * N
+    flops = 3 * 8.0 * prods * V * C
     return {"workload": f"pair sinks (SlaterDet3): L={L}, N={N}, T={T}, {C} configurations, "
                         f"FP32 evolution, split-BF16x3 pair matrices, 2N FP64 LU per slice",
             "value": C * T / dt, "unit": "configs*slices/s",
+            "conj_partners": bool(conj), "volumes_per_config": nvo,
             "finite_fraction": float(np.isfinite(la).mean()),
             "avg_ms_per_slice": {"evolution": t_evo / max(n_evo, 1),
                                  "pair_matrices_and_gather": meas_ms,
@@ -296,7 +345,7 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012):
             "pair_matrix_mfma": {"achieved_tflops": flops / (meas_ms * 1e-3) / 1e12,
                                  "peak_tflops": 2500.0,
                                  "frac": flops / (meas_ms * 1e-3) / 1e12 / 2500.0},
-            "pair_matrix_hbm": {"algorithmic_gbs": 8.0 * N * V * C / (meas_ms * 1e-3) / 1e9,
+            "pair_matrix_hbm": {"algorithmic_gbs": 8.0 * nvo * V * C / (meas_ms * 1e-3) / 1e9,
                                 "peak_gbs": HBM_PEAK_GBS}}
 
 
@@ -318,6 +367,8 @@ def main():
                     help="skip the learned-proposal measurement")
     ap.add_argument("--no-other-mode", action="store_true",
                     help="skip the run of the workload in the other conjugate-partner mode")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense-sink (MFMA overlap) measurement")
     ap.add_argument("--no-pair", action="store_true",
                     help="skip the pair-sink (SlaterDet3) measurement")
     ap.add_argument("--precision-configs", type=int, default=1024,
@@ -420,7 +471,7 @@ def main():
     pair = None
     if not args.no_pair:
         ens.close()
-        pair = pair_sinks(ctx, na)
+        pair = pair_sinks(ctx, na, conj=bool(args.conj))
         print(f"pair sinks: {pair['value']:.0f} configs*slices/s", file=sys.stderr, flush=True)
 
     # the other mode of the same workload: conjugate partners on when the headline runs
@@ -455,6 +506,15 @@ def main():
         ens.close()
         ens = None
 
+    dense = None
+    if not args.no_dense:
+        if ens is not None:
+            ens.close()
+            ens = None
+        dense = dense_overlap(ctx, na)
+        print(f"dense overlaps: {dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
+              f"{dense['mfma']['frac']:.3f}", file=sys.stderr, flush=True)
+
     precision = None
     if args.precision_configs > 0:
         modes = (bool(args.conj),) if args.no_other_mode else (bool(args.conj), not args.conj)
@@ -486,6 +546,7 @@ def main():
         "learned_proposal": learned,
         "pair_sinks": pair,
         "other_mode": other,
+        "dense_overlap": dense,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]

@@ -176,8 +176,10 @@ int nrc_ensemble_set_path(nrc_ensemble* ens, int path);
  * the other's sink values are read conjugated at the mirrored sinks: the same results (to
  * rounding) from (N + self-conjugate orbitals) / 2 volumes per configuration.  The reference
  * evolves every orbital (src/propagator/propagator.C:35-64); this has no counterpart there.
- * Plane-wave sinks only (NRC_ERR_UNSUPPORTED with dense sinks, pair sinks or tables that are
- * not even to 1e-12 relative); not while the ensemble is in a batch; call before a run. */
+ * Pair sinks: the pair matrices come from two products over the stored volumes (mirrored
+ * sites, and the same sites conjugated); they need pair_opp(p) = -p and pair_w even.
+ * NRC_ERR_UNSUPPORTED with dense sinks, or tables (kinetic, smear, pair_w) that are not even
+ * to 1e-12 relative; not while the ensemble is in a batch; call before a run. */
 int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
 int nrc_ensemble_volumes(nrc_ensemble* ens, int* volumes_per_config);

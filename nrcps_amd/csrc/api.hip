@@ -363,6 +363,8 @@ struct nrc_ensemble {
   const int* col_conj = nullptr;
   const int* msite = nullptr;     // [N] site of -k_s
   std::vector<int> h_col_vol, h_col_conj;
+  bool pair_sym_ok = false;  // pair sinks: opp is the mirror and w even (partners allowed)
+  double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
   bool profile = false;
@@ -378,7 +380,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
-                  e->site_of_orb, e->moms_dev, e->sym};
+                  e->site_of_orb, e->moms_dev, e->sym, e->pgh};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
@@ -585,6 +587,14 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   if (e->pair) {  // the pair wave function, unfolded (t = 0) and with K(1/2) on both momenta
     for (long i = 0; i < V; ++i)
       if (d->pair_opp[i] < 0 || d->pair_opp[i] >= V) ENS_TRY(NRC_ERR_ARG);
+    // partners need opp(p) = -p and w(p) = w(-p): psi_{-t}(opp p) is then conj psi_t(p)
+    double wmax = 0.0;
+    for (long i = 0; i < V; ++i) wmax = std::max(wmax, std::fabs(d->pair_w[i]));
+    e->pair_sym_ok = true;
+    for (long i = 0; i < V && e->pair_sym_ok; ++i)
+      if (d->pair_opp[i] != mirror_site(L, i) ||
+          !(std::fabs(d->pair_w[i] - d->pair_w[d->pair_opp[i]]) <= 1e-12 * wmax))
+        e->pair_sym_ok = false;
     std::vector<double> w(2 * V);
     for (long i = 0; i < V; ++i) {
       w[i] = d->pair_w[i];
@@ -855,7 +865,22 @@ static int ens_measure(nrc_ensemble* e, int t) {
       NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb,
                                        e->folded ? e->sink_k : nullptr, e->M, C, N, V, s, cp));
     }
-    if (e->pair) {
+    if (e->pair && e->sym) {  // G (mirrored sites) and H (conjugated) of the stored volumes
+      const double* w = e->pw + (e->folded ? V : 0);
+      const int nv = e->nvo;
+      double2* G = e->pgh;
+      double2* H = e->pgh + (size_t)C * nv * nv;
+      if (prec == NRC_PREC_F64) {
+        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, G, nv, nv, V, (int)C, s));
+        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, nullptr, H, nv, nv, V, (int)C, s));
+      } else {
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+                                        e->popp, G, nv, nv, V, (int)C, e->pair_work, s));
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+                                        nullptr, H, nv, nv, V, (int)C, e->pair_work, s));
+      }
+      NRC_HIP(nrc::launch_pair_expand(G, H, e->col_vol, e->col_conj, e->Mp, N, nv, C, s));
+    } else if (e->pair) {
       const double* w = e->pw + (e->folded ? V : 0);
       if (prec == NRC_PREC_F64)
         NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, e->Mp, N, N, V,
@@ -1196,11 +1221,14 @@ extern "C" int nrc_ensemble_volumes(nrc_ensemble* e, int* nvo) {
 extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   if (!e || (conj != 0 && conj != 1)) return NRC_ERR_ARG;
   if (e->stream != e->ctx->stream) return NRC_ERR_ARG;  // not while in a batch
-  if (conj && (!e->sym_ok || e->pair || e->sinks)) return NRC_ERR_UNSUPPORTED;
+  if (conj && (!e->sym_ok || (e->pair && !e->pair_sym_ok) || e->sinks))
+    return NRC_ERR_UNSUPPORTED;
   hipSetDevice(e->ctx->device);
   NRC_HIP(hipStreamSynchronize(e->stream));
   if (e->sym) hipFree(e->sym);
+  if (e->pgh) hipFree(e->pgh);
   e->sym = nullptr;
+  e->pgh = nullptr;
   e->nvo = e->d.n_orb;
   e->h_col_vol.clear();
   e->h_col_conj.clear();
@@ -1264,6 +1292,13 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   if (hipMemcpy(d, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     hipFree(d);
     return NRC_ERR_HIP;
+  }
+  if (e->pair) {
+    const int rc = dev_alloc(&e->pgh, (size_t)2 * e->d.n_configs * nvo * nvo);
+    if (rc != NRC_OK) {
+      hipFree(d);
+      return rc;
+    }
   }
   e->sym = d;
   e->vo_site = d + o_site;

@@ -313,7 +313,13 @@ __global__ __launch_bounds__(256) void k_pair_dense(const C2<T>* __restrict__ ps
 #pragma unroll
   for (int q = 0; q < CH; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
   for (long i = threadIdx.x; i < vol; i += 256) {
-    const C2<T> u = b2[opp[i]];
+    C2<T> u;
+    if (opp) {
+      u = b2[opp[i]];
+    } else {  // the conjugate partner's image: conj psi2_t at the same site
+      u = b2[i];
+      u.y = -u.y;
+    }
     const double br = w[i] * (double)u.x, bi = w[i] * (double)u.y;
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
@@ -353,6 +359,31 @@ hipError_t launch_pair_dense(int prec, const void* psi1, const void* psi2, const
   else
     hipLaunchKernelGGL(k_pair_dense<float>, grid, dim3(256), 0, s, (const C2<float>*)psi1,
                        (const C2<float>*)psi2, w, opp, M, n1, n2, vol);
+  return hipGetLastError();
+}
+
+__global__ void k_pair_expand(const double2* G, const double2* H, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, long ncfg) {
+  const long n = ncfg * N * N;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / ((long)N * N);
+    const int r = (int)(i - c * N * N), s = r / N, t = r - (r / N) * N;
+    const int js = col_conj[s], jt = col_conj[t];
+    const double2 g = (js == jt ? G : H)[(c * nvo + col_vol[s]) * nvo + col_vol[t]];
+    M[i] = js ? make_double2(g.x, -g.y) : g;
+  }
+}
+
+hipError_t launch_pair_expand(const double2* G, const double2* H, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, long ncfg,
+                              hipStream_t s) {
+  const long n = ncfg * N * N;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_pair_expand, dim3((unsigned)blocks), dim3(256), 0, s, G, H, col_vol,
+                     col_conj, M, N, nvo, ncfg);
   return hipGetLastError();
 }
 

@@ -177,7 +177,8 @@ hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hip
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 
 // pair-sink matrices M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]: FP64 sums
-// (dense), or split-BF16x3 on the matrix cores (FP32 orbitals, weights rounded to FP32)
+// (dense), or split-BF16x3 on the matrix cores (FP32 orbitals, weights rounded to FP32).
+// opp null: conj(psi2[c][t][i]) at the same site (a conjugate partner's mirror image).
 hipError_t launch_pair_dense(int prec, const void* psi1, const void* psi2, const double* w,
                              const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
                              hipStream_t s);
@@ -185,6 +186,13 @@ hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const doub
                               const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
                               float* partial, hipStream_t s);
 size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg);
+// the N x N pair matrices of configurations whose orbitals are stored as nvo volumes with
+// conjugate partners, from G[c][a][b] = sum w psi_a(p) psi_b(-p) and H[c][a][b] =
+// sum w psi_a(p) conj psi_b(p) over the stored volumes: orbital s is volume col_vol[s],
+// mirrored when col_conj[s] (M = G, H, conj H, conj G for the four cases)
+hipError_t launch_pair_expand(const double2* G, const double2* H, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, long ncfg,
+                              hipStream_t s);
 
 // the context's stream and device (api.hip), for entry points defined in other files
 hipStream_t ctx_stream(nrc_context* c);

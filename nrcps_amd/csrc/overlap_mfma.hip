@@ -174,8 +174,9 @@ __global__ __launch_bounds__(256) void k_overlap_bf16x3(const __bf16* __restrict
 // periodic boundaries) the four values are two 16-byte loads (or 8 + 16 + 8 bytes from an
 // odd site), reversed in registers; opp and wf must be 16-byte aligned.  The
 // weights are rounded to FP32 as they are read.  Partial tiles keep re in column 2t and im
-// in 2t + 1, as the overlap's.
-template <int MT, int NT>
+// in 2t + 1, as the overlap's.  CJ: B is w conj(psi2_t) at the same site (opp unused), the
+// conjugate partner's image psi_{-t}(-p) = conj psi_t(p).
+template <int MT, int NT, bool CJ>
 __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ psi1,
                                                      const float2* __restrict__ psi2,
                                                      const double* __restrict__ wf,
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
     int4 st = make_int4(0, 0, 0, 0);
     float wt[4] = {0.f, 0.f, 0.f, 0.f};
     if (in) {
-      st = *reinterpret_cast<const int4*>(opp + j0);
+      if constexpr (!CJ) st = *reinterpret_cast<const int4*>(opp + j0);
       const double2* wp = reinterpret_cast<const double2*>(wf + j0);
       const double2 w01 = wp[0], w23 = wp[1];
       wt[0] = (float)w01.x; wt[1] = (float)w01.y; wt[2] = (float)w23.x; wt[3] = (float)w23.y;
@@ -241,7 +242,14 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
       if (b < nt && t < n2 && in) {
         const float2* p = b2 + (long)t * vol;
         float2 u[4];
-        if (run && even) {  // 16-byte aligned
+        if constexpr (CJ) {  // the same four sites, conjugated
+          const float4* q = reinterpret_cast<const float4*>(p + j0);
+          const float4 u01 = q[0], u23 = q[1];
+          u[0] = make_float2(u01.x, -u01.y);
+          u[1] = make_float2(u01.z, -u01.w);
+          u[2] = make_float2(u23.x, -u23.y);
+          u[3] = make_float2(u23.z, -u23.w);
+        } else if (run && even) {  // 16-byte aligned
           const float4* q = reinterpret_cast<const float4*>(p + st.w);
           const float4 u32 = q[0], u10 = q[1];
           u[3] = make_float2(u32.x, u32.y);
@@ -397,6 +405,7 @@ size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg) {
 hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const double* wf,
                               const int* opp, double2* M, int n1, int n2, long vol, int ncfg,
                               float* partial, hipStream_t s) {
+  const bool cj = opp == nullptr;
   if (n1 < 1 || n1 > 16 * OV_MAXT || n2 < 1 || n2 > 64 || (vol & 3))
     return hipErrorInvalidValue;
   const int mt = (n1 + 15) / 16, nt = (n2 + 15) / 16;
@@ -404,9 +413,12 @@ hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const doub
   const int nchunks = (int)((K + kc - 1) / kc);
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
 #define NRC_PR_CASE(A, B)                                                                     \
-  if (mt == A && nt == B) {                                                                   \
-    hipLaunchKernelGGL((k_pair_bf16x3<A, B>), grid, block, 0, s, psi1, psi2, wf, opp, partial, \
-                       n1, n2, vol, mt, nt, kc);                                              \
+  if (mt == A && nt == B && !cj) {                                                            \
+    hipLaunchKernelGGL((k_pair_bf16x3<A, B, false>), grid, block, 0, s, psi1, psi2, wf, opp,   \
+                       partial, n1, n2, vol, mt, nt, kc);                                     \
+  } else if (mt == A && nt == B) {                                                            \
+    hipLaunchKernelGGL((k_pair_bf16x3<A, B, true>), grid, block, 0, s, psi1, psi2, wf, opp,    \
+                       partial, n1, n2, vol, mt, nt, kc);                                     \
   }
 #define NRC_PR_ROW(A) NRC_PR_CASE(A, 1) NRC_PR_CASE(A, 2) NRC_PR_CASE(A, 3) NRC_PR_CASE(A, 4)
   NRC_PR_ROW(1) NRC_PR_ROW(2) NRC_PR_ROW(3) NRC_PR_ROW(4)

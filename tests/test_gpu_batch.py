@@ -108,3 +108,36 @@ def test_tune_picks_a_path_that_keeps_results(ctx, orc, L, prec):
     la2, ph2, _ = ref.run(31, 0, T)
     tol = 1e-11 if prec == 1 else 2e-5
     assert np.abs(la - la2).max() < tol and np.abs(ph - ph2).max() < tol
+
+
+def test_batch_conj_partners_match_oracle_and_alone(ctx, orc):
+    """Shape groups with conjugate partners in a captured batch: each group gives what it gives
+    alone, and its FP64 determinants are the oracle's (which evolves every orbital)."""
+    import nrcps_amd as na
+
+    shapes = [(8, 7, 3, 1), (12, 19, 2, 1), (16, 27, 2, 0), (8, 1, 2, 1)]
+    T, seed, c0 = 7, 31, [3, 40, 7, 11]
+    alone = []
+    for g, c in zip(_groups(ctx, orc, shapes), c0):
+        g.set_symmetry(True)
+        alone.append(g.run(seed, c, T))
+    groups = _groups(ctx, orc, shapes)
+    for g in groups:
+        g.set_symmetry(True)
+    b = na.Batch(ctx, groups)
+    b.run(seed, c0, T, graph=True)
+    for g, ref, (L, N, C, prec), c_0 in zip(groups, alone, shapes, c0):
+        la, ph, _ = g.results(T)
+        assert np.array_equal(la, ref[0]) and np.array_equal(ph, ref[1])
+        if prec != 1:
+            continue
+        moms = orc.lowest_momenta(L, N)
+        kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+        for c in range(C):
+            o = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c_0 + c)
+            np.testing.assert_allclose(la[:, c, 0], o["logabs"][:, 0], rtol=0, atol=1e-11)
+            np.testing.assert_allclose(ph[:, c, 0], o["phase"][:, 0], rtol=0, atol=1e-11)
+    # set_symmetry is refused while the ensemble is in a batch
+    with pytest.raises(na.NrcError):
+        groups[0].set_symmetry(False)
+    b.close()

@@ -108,3 +108,51 @@ def test_ensemble_pair_matches_oracle(ctx, orc, prec):
         print(f"[gpu] config {c}: max |d log|det|| {err:.2e}, phase {perr:.2e}", flush=True)
         assert err < tol and perr < 50 * tol
     ens.close()
+
+
+@pytest.mark.parametrize("prec,N", [(1, 7), (1, 9), (0, 7)])
+def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N):
+    """Pair sinks with conjugate partners: the pair matrices come from two products over the
+    stored volumes (mirrored sites, and the same sites conjugated); N = 9 leaves two orbitals
+    of a partial shell without partner.  Against the oracle's run, which evolves every orbital:
+    FP64 1e-9, FP32 1e-4 in log|det|."""
+    import nrcps_amd as na
+
+    L, T, C, seed = 8, 6, 3, 29
+    print(f"[gpu] ensemble pair sinks, conjugate partners, prec={prec} N={N}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    w = orc.two_body_wavefunc(L, orc.WAVEFUNC_TGND, 0.176776695, cutoff_sq=np.pi ** 2)
+    opp = orc.opposite_parity_index(L)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=na.SPLIT_KSPACE,
+                      pair_w=w, pair_opp=opp)
+    ens.set_symmetry(True)
+    assert ens.volumes_per_config() == (4 if N == 7 else 6)
+    la, ph, _ = ens.run(seed, 0, T)
+    tol = 1e-9 if prec == 1 else 1e-4
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c, pair_w=w,
+                             pair_opp=opp)
+        both_inf = np.isneginf(la[:, c, :]) & np.isneginf(ref["logabs"])
+        assert np.all(np.isneginf(la[:, c, :]) == np.isneginf(ref["logabs"]))
+        err = np.abs(np.where(both_inf, 0.0, la[:, c, :] - ref["logabs"])).max()
+        perr = np.abs(ph[:, c, :] - ref["phase"]).max()
+        print(f"[gpu] config {c}: max |d log|det|| {err:.2e}, phase {perr:.2e}", flush=True)
+        assert err < tol and perr < 50 * tol
+    ens.close()
+
+
+def test_ensemble_pair_conj_partners_refused_for_twisted_mirror(ctx, orc):
+    """Antiperiodic boundaries shift the opposite-parity site off -p: no partners."""
+    import nrcps_amd as na
+
+    L, N = 8, 7
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    w = orc.two_body_wavefunc(L, orc.WAVEFUNC_PAIR1, 0.3, cutoff_sq=3 * np.pi ** 2, bc=(1, 0, 0))
+    opp = orc.opposite_parity_index(L, (1, 0, 0))
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, 2, prec=1, split=na.SPLIT_KSPACE, pair_w=w,
+                      pair_opp=opp)
+    with pytest.raises(na.NrcError):
+        ens.set_symmetry(True)
+    ens.close()
