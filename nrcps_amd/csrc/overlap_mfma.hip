@@ -19,8 +19,7 @@
 // Each workgroup (4 waves) takes one configuration and one chunk of K; every wave
 // accumulates all (sink tile, column tile) pairs for its K-steps, the waves are summed in
 // LDS and the chunk's partial tile written out; a second kernel sums the chunks in FP64.
-#include "nrc_device.h"
-#include "nrc_internal.h"
+#include "nrc_ops.h"
 
 namespace nrc {
 
@@ -28,7 +27,6 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int OV_MAXT = 4;    // sink tiles of 16 (ns <= 64)
-constexpr int OV_MAXN = 8;    // column tiles of 16 (2 nt <= 128)
 constexpr int OV_KSTEP = 32;  // reals per MFMA
 constexpr int OV_WAVES = 4;
 
@@ -68,100 +66,145 @@ hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hip
   return hipGetLastError();
 }
 
-// grid (kchunks, ncfg); partial[c][chunk][ns_pad][nc_pad] FP32
-template <int MT, int NCT>
-__global__ __launch_bounds__(256) void k_overlap_bf16x3(const __bf16* __restrict__ phi_hi,
-                                                        const __bf16* __restrict__ phi_lo,
-                                                        const float2* __restrict__ psi,
-                                                        float* __restrict__ partial, int ns,
-                                                        int nt, long vol, int mt, int nc_t,
-                                                        long kchunk) {
+// grid (kchunks, ncfg); partial[c][chunk][MT 16][NT 32] FP32, re in column 2t, im in 2t + 1.
+//
+// A lane of column tile b holds orbital t = 16 b + l % 16 and reads its four complex values
+// once; B's real-part column is (psi_re, psi_im) and its imaginary-part column
+// (psi_im, -psi_re), which in bf16 is the real-part column with the halves of every dword
+// swapped and the upper sign flipped (the split of -x is minus the split of x), so one split
+// serves both accumulators.  Sinks and orbitals go through buffer resources (uniform bases,
+// one lane offset per tile, the K position as the scalar offset; rows past ns / nt are past
+// the resource's end and read zero), and the next step's values are loaded before the
+// current step's products.
+template <int MT, int NT>
+__global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
+    const __bf16* __restrict__ phi_hi, const __bf16* __restrict__ phi_lo,
+    const float2* __restrict__ psi, float* __restrict__ partial, int ns, int nt, long vol,
+    long kchunk) {
   __shared__ float red[OV_WAVES - 1][4][64];
   const int chunk = blockIdx.x, nchunks = gridDim.x;
   const long c = blockIdx.y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index is uniform: in an SGPR, so the K position can be the scalar offset
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int row = lane & 15, grp = lane >> 4;
   const long K = 2 * vol;
   const long k0 = (long)chunk * kchunk;
   const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
-  f32x4 acc[MT][NCT];
+  const rsrc_t rh = make_rsrc(phi_hi, (uint32_t)(ns * K * 2));
+  const rsrc_t rl = make_rsrc(phi_lo, (uint32_t)(ns * K * 2));
+  const rsrc_t rb = make_rsrc(psi + c * nt * vol, (uint32_t)(nt * vol * 8));
+  constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
+  uint32_t oa[MT], ob[NT];
+#pragma unroll
+  for (int a = 0; a < MT; ++a) {
+    const int s = a * 16 + row;
+    oa[a] = s < ns ? (uint32_t)(s * K + 8 * grp) * 2u : kFar;
+  }
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+    const int t = b * 16 + row;
+    ob[b] = t < nt ? (uint32_t)(t * vol + 4 * grp) * 8u : kFar;
+  }
+  f32x4 accr[MT][NT], acci[MT][NT];
 #pragma unroll
   for (int a = 0; a < MT; ++a)
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  const float2* psic = psi + c * nt * vol;
-  for (long kb = k0 + (long)wave * OV_KSTEP; kb < k1; kb += OV_WAVES * OV_KSTEP) {
-    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements ka/2 .. ka/2 + 3
-    bf16x8 ah[MT], al[MT];
+    for (int b = 0; b < NT; ++b) {
+      accr[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      acci[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    }
+  using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+  // the sink (hi, lo) and orbital values of one step; lanes past the chunk end read zero (the
+  // rows are contiguous, so a read past K would be the next row's)
+  struct Step {
+    u32x4 ah[MT], al[MT];
+    float4 b0[NT], b1[NT];
+  };
+  auto load = [&](Step& st, long kb) {
+    const uint32_t sa = (uint32_t)kb * 2u, sb = (uint32_t)kb * 4u;
+    const bool in = kb + 8 * grp < k1;
 #pragma unroll
     for (int a = 0; a < MT; ++a) {
-      if (a < mt) {
-        const int s = a * 16 + row;
-        if (s < ns && ka < K) {
-          ah[a] = *reinterpret_cast<const bf16x8*>(phi_hi + (long)s * K + ka);
-          al[a] = *reinterpret_cast<const bf16x8*>(phi_lo + (long)s * K + ka);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) { ah[a][q] = to_bf16(0.f); al[a][q] = to_bf16(0.f); }
-        }
-      }
+      const uint32_t o = in ? oa[a] : kFar;
+      st.ah[a] = bload<u32x4>(rh, o, sa);
+      st.al[a] = bload<u32x4>(rl, o, sa);
     }
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) {
-      if (b < nc_t) {
-        const int col = b * 16 + row, t = col >> 1, part = col & 1;
-        float v[8];
-        if (t < nt && ka < K) {
-          const float4* p = reinterpret_cast<const float4*>(psic + (long)t * vol + ka / 2);
-          const float4 u0 = p[0], u1 = p[1];
-          const float raw[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+    for (int b = 0; b < NT; ++b) {
+      const uint32_t o = in ? ob[b] : kFar;
+      st.b0[b] = bload<float4>(rb, o, sb);
+      st.b1[b] = bload<float4>(rb, o + 16u, sb);
+    }
+  };
+  // tile by tile: the orbital values are split, then the products run term by term over the
+  // sink tiles, so consecutive MFMAs accumulate into different tiles
+  auto compute = [&](const Step& st) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            v[2 * q] = part ? raw[2 * q + 1] : raw[2 * q];
-            v[2 * q + 1] = part ? -raw[2 * q] : raw[2 * q + 1];
-          }
-        } else {
+    for (int b = 0; b < NT; ++b) {
+      const float v[8] = {st.b0[b].x, st.b0[b].y, st.b0[b].z, st.b0[b].w,
+                          st.b1[b].x, st.b1[b].y, st.b1[b].z, st.b1[b].w};
+      bf16x8 hr, lr;
+      split8(v, hr, lr);
+      // imaginary-part column: (im, -re) per element = dword halves swapped, upper sign flipped
+      const u32x4 uh = __builtin_bit_cast(u32x4, hr), ul = __builtin_bit_cast(u32x4, lr);
+      u32x4 vh, vl;
 #pragma unroll
-          for (int q = 0; q < 8; ++q) v[q] = 0.f;
-        }
-        bf16x8 bh, bl;
-        split8(v, bh, bl);
+      for (int q = 0; q < 4; ++q) {
+        vh[q] = ((uh[q] >> 16) | (uh[q] << 16)) ^ 0x80000000u;
+        vl[q] = ((ul[q] >> 16) | (ul[q] << 16)) ^ 0x80000000u;
+      }
+      const bf16x8 hi_ = __builtin_bit_cast(bf16x8, vh), li_ = __builtin_bit_cast(bf16x8, vl);
+#pragma unroll
+      for (int term = 0; term < 3; ++term)
 #pragma unroll
         for (int a = 0; a < MT; ++a) {
-          if (a < mt) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
-          }
+          const bf16x8 av = __builtin_bit_cast(bf16x8, term == 2 ? st.al[a] : st.ah[a]);
+          const bf16x8 br = term == 1 ? lr : hr, bi = term == 1 ? li_ : hi_;
+          accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, br, accr[a][b], 0, 0, 0);
+          acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bi, acci[a][b], 0, 0, 0);
         }
-      }
     }
+  };
+  // two buffers: the next step's loads go out before the current step's products
+  constexpr long S = OV_WAVES * OV_KSTEP;
+  long kb = k0 + (long)wave * OV_KSTEP;
+  Step r0, r1;
+  load(r0, kb);
+  while (kb < k1) {
+    load(r1, kb + S);
+    compute(r0);
+    kb += S;
+    if (kb >= k1) break;
+    load(r0, kb + S);
+    compute(r1);
+    kb += S;
   }
   // sum the waves tile by tile: waves 1..3 park a tile in LDS, wave 0 adds and stores it
-  const int ldp = nc_t * 16;
-  float* out = partial + (c * nchunks + chunk) * (long)(mt * 16) * ldp;
+  constexpr int ldp = NT * 32;
+  float* out = partial + (c * nchunks + chunk) * (long)(MT * 16) * ldp;
 #pragma unroll
   for (int a = 0; a < MT; ++a)
 #pragma unroll
-    for (int b = 0; b < NCT; ++b) {
-      if (a < mt && b < nc_t) {  // uniform over the workgroup
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+      for (int part = 0; part < 2; ++part) {
+        const f32x4 v = part ? acci[a][b] : accr[a][b];
         if (wave > 0) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = acc[a][b][r];
+          for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = v[r];
         }
         __syncthreads();
         if (wave == 0) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            float x = acc[a][b][r];
+            float x = v[r];
 #pragma unroll
             for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
-            out[(long)(a * 16 + grp * 4 + r) * ldp + b * 16 + row] = x;
+            out[(long)(a * 16 + grp * 4 + r) * ldp + 2 * (b * 16 + row) + part] = x;
           }
         }
         __syncthreads();
       }
-    }
 }
 
 // The pair-sink matrix M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]] (TwoBody::Run,
@@ -356,31 +399,29 @@ long overlap_bf16x3_kchunk(long vol) {
 }
 
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg) {
-  const int mt = (ns + 15) / 16, nc_t = (2 * nt + 15) / 16;
+  const int mt = (ns + 15) / 16, ntt = (nt + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const long nchunks = (K + kc - 1) / kc;
-  return (size_t)ncfg * nchunks * (mt * 16) * (nc_t * 16);
+  return (size_t)ncfg * nchunks * (mt * 16) * (ntt * 32);
 }
 
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
                                  int nt, long vol, int ncfg, float* partial, hipStream_t s) {
-  if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || 2 * nt > 16 * OV_MAXN || (vol & 3))
+  if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
     return hipErrorInvalidValue;
-  const int mt = (ns + 15) / 16, nc_t = (2 * nt + 15) / 16;
+  const int mt = (ns + 15) / 16, ntt = (nt + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const int nchunks = (int)((K + kc - 1) / kc);
   const __bf16* hi = (const __bf16*)phi_split;
   const __bf16* lo = hi + 2 * (long)ns * vol;
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
-  // one instantiation per (sink tiles, column tiles), so only the used accumulators exist
+  // one instantiation per (sink tiles, orbital tiles), so only the used accumulators exist
 #define NRC_OV_CASE(A, B)                                                                    \
-  if (mt == A && nc_t == B) {                                                                \
+  if (mt == A && ntt == B) {                                                                 \
     hipLaunchKernelGGL((k_overlap_bf16x3<A, B>), grid, block, 0, s, hi, lo, psi, partial, ns, \
-                       nt, vol, mt, nc_t, kc);                                               \
+                       nt, vol, kc);                                                         \
   }
-#define NRC_OV_ROW(A) \
-  NRC_OV_CASE(A, 1) NRC_OV_CASE(A, 2) NRC_OV_CASE(A, 3) NRC_OV_CASE(A, 4) NRC_OV_CASE(A, 5) \
-  NRC_OV_CASE(A, 6) NRC_OV_CASE(A, 7) NRC_OV_CASE(A, 8)
+#define NRC_OV_ROW(A) NRC_OV_CASE(A, 1) NRC_OV_CASE(A, 2) NRC_OV_CASE(A, 3) NRC_OV_CASE(A, 4)
   NRC_OV_ROW(1) NRC_OV_ROW(2) NRC_OV_ROW(3) NRC_OV_ROW(4)
 #undef NRC_OV_ROW
 #undef NRC_OV_CASE
@@ -390,8 +431,9 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
+  // column pairs (2t, 2t + 1) over 2 ntt tiles of 16
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, ns,
-                     nt, nchunks, mt, nc_t, (long)ncfg);
+                     nt, nchunks, mt, 2 * ntt, (long)ncfg);
   return hipGetLastError();
 }
 
