@@ -610,7 +610,9 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
                     hipSuccess ? NRC_OK : NRC_ERR_HIP);
     if (d->prec == NRC_PREC_F32) {
       if (V % 4) ENS_TRY(NRC_ERR_UNSUPPORTED);
-      ENS_TRY(dev_alloc(&e->pair_work, nrc::pair_bf16x3_partial_floats(N, N, V, (int)C)));
+      // the N x N products, or the two products of conjugate partners in one pass
+      ENS_TRY(dev_alloc(&e->pair_work, std::max(nrc::pair_bf16x3_partial_floats(N, N, V, (int)C),
+                                                nrc::pair_gh_bf16x3_partial_floats(N, V, (int)C))));
     }
   }
   if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
@@ -873,6 +875,9 @@ static int ens_measure(nrc_ensemble* e, int t) {
       if (prec == NRC_PREC_F64) {
         NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, G, nv, nv, V, (int)C, s));
         NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, nullptr, H, nv, nv, V, (int)C, s));
+      } else if (nv <= 48) {  // both products in one pass
+        NRC_HIP(nrc::launch_pair_gh_bf16x3((const float2*)e->psi, w, e->popp, G, H, nv, V,
+                                           (int)C, e->pair_work, s));
       } else {
         NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
                                         e->popp, G, nv, nv, V, (int)C, e->pair_work, s));

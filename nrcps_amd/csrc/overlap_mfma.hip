@@ -371,6 +371,163 @@ __global__ __launch_bounds__(256) void k_pair_bf16x3(const float2* __restrict__ 
       }
 }
 
+// The two pair products of conjugate partners in one pass over the stored volumes psi[c][n]:
+// G[c][s][t] = sum_i w[i] psi_s(i) psi_t(opp i) and H[c][s][t] = sum_i w[i] psi_s(i)
+// conj psi_t(i).  A lane of A tile a holds orbital s = 16 a + l % 16 at the step's four
+// sites, which are also H's B values for the orbital t = s of tile b = a, so every step reads
+// the sites once and their mirrors once.  The imaginary-part B column (bi, br) is the
+// real-part one (br, -bi) with the halves of every dword swapped and the lower sign flipped,
+// so one split serves both; the products run term by term over the sink tiles.  partial
+// holds G's chunk tiles, then H's, in k_pair_bf16x3's layout.
+template <int MT>
+__global__ __launch_bounds__(256) void k_pair_gh_bf16x3(const float2* __restrict__ psi,
+                                                        const double* __restrict__ wf,
+                                                        const int* __restrict__ opp,
+                                                        float* __restrict__ partial, int n,
+                                                        long vol, long kchunk) {
+  using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+  __shared__ float red[OV_WAVES - 1][4][64];
+  const int chunk = blockIdx.x, nchunks = gridDim.x;
+  const long c = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, grp = lane >> 4;
+  const long K = 2 * vol;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  f32x4 acc[4][MT][MT];  // G re, G im, H re, H im
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int b = 0; b < MT; ++b) acc[m][a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const float2* base = psi + c * n * vol;
+  for (long kb = k0 + (long)wave * OV_KSTEP; kb < k1; kb += OV_WAVES * OV_KSTEP) {
+    const long ka = kb + 8 * grp;  // this lane's 8 reals: complex elements j0 .. j0 + 3
+    const long j0 = ka / 2;
+    const bool in = ka < K;
+    float va[MT][8];
+    bf16x8 ah[MT], al[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+      const int s = a * 16 + row;
+      if (s < n && in) {
+        const float4* p = reinterpret_cast<const float4*>(base + (long)s * vol + j0);
+        const float4 u0 = p[0], u1 = p[1];
+        va[a][0] = u0.x; va[a][1] = u0.y; va[a][2] = u0.z; va[a][3] = u0.w;
+        va[a][4] = u1.x; va[a][5] = u1.y; va[a][6] = u1.z; va[a][7] = u1.w;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) va[a][q] = 0.f;
+      }
+      split8(va[a], ah[a], al[a]);
+    }
+    int4 st = make_int4(0, 0, 0, 0);
+    float wt[4] = {0.f, 0.f, 0.f, 0.f};
+    if (in) {
+      st = *reinterpret_cast<const int4*>(opp + j0);
+      const double2* wp = reinterpret_cast<const double2*>(wf + j0);
+      const double2 w01 = wp[0], w23 = wp[1];
+      wt[0] = (float)w01.x; wt[1] = (float)w01.y; wt[2] = (float)w23.x; wt[3] = (float)w23.y;
+    }
+    // consecutive: the mirrored sites are st.w .. st.w + 3 for elements 3, 2, 1, 0
+    const bool run = st.x == st.w + 3 && st.y == st.w + 2 && st.z == st.w + 1;
+    const bool even = (st.w & 1) == 0;
+#pragma unroll
+    for (int b = 0; b < MT; ++b) {
+      const int t = b * 16 + row;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {  // G: the mirrored sites; H: the same sites conjugated
+        float2 u[4];
+        if (m == 1) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) u[q] = make_float2(va[b][2 * q], -va[b][2 * q + 1]);
+        } else if (t < n && in) {
+          const float2* p = base + (long)t * vol;
+          if (run && even) {  // 16-byte aligned
+            const float4* q = reinterpret_cast<const float4*>(p + st.w);
+            const float4 u32 = q[0], u10 = q[1];
+            u[3] = make_float2(u32.x, u32.y);
+            u[2] = make_float2(u32.z, u32.w);
+            u[1] = make_float2(u10.x, u10.y);
+            u[0] = make_float2(u10.z, u10.w);
+          } else if (run) {  // odd start: 8 + 16 (aligned) + 8 bytes
+            const float4 u21 = *reinterpret_cast<const float4*>(p + st.w + 1);
+            u[3] = p[st.w];
+            u[2] = make_float2(u21.x, u21.y);
+            u[1] = make_float2(u21.z, u21.w);
+            u[0] = p[st.w + 3];
+          } else {
+            u[0] = p[st.x];
+            u[1] = p[st.y];
+            u[2] = p[st.z];
+            u[3] = p[st.w];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) u[q] = make_float2(0.f, 0.f);
+        }
+        float vr[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          vr[2 * q] = wt[q] * u[q].x;
+          vr[2 * q + 1] = -(wt[q] * u[q].y);
+        }
+        bf16x8 rh, rl;
+        split8(vr, rh, rl);
+        const u32x4 uh = __builtin_bit_cast(u32x4, rh), ul = __builtin_bit_cast(u32x4, rl);
+        u32x4 vh, vl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          vh[q] = ((uh[q] >> 16) | (uh[q] << 16)) ^ 0x00008000u;
+          vl[q] = ((ul[q] >> 16) | (ul[q] << 16)) ^ 0x00008000u;
+        }
+        const bf16x8 ih = __builtin_bit_cast(bf16x8, vh), il = __builtin_bit_cast(bf16x8, vl);
+#pragma unroll
+        for (int term = 0; term < 3; ++term)
+#pragma unroll
+          for (int a = 0; a < MT; ++a) {
+            const bf16x8 av = term == 2 ? al[a] : ah[a];
+            const bf16x8 br = term == 1 ? rl : rh, bi = term == 1 ? il : ih;
+            acc[2 * m][a][b] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, br, acc[2 * m][a][b], 0, 0, 0);
+            acc[2 * m + 1][a][b] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bi, acc[2 * m + 1][a][b], 0, 0, 0);
+          }
+      }
+    }
+  }
+  // sum the waves tile by tile; column 2t holds re, 2t + 1 im
+  constexpr int ldp = MT * 32;
+  const long half = (long)gridDim.y * nchunks * (MT * 16) * ldp;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int b = 0; b < MT; ++b)
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+          const f32x4 v = acc[2 * m + part][a][b];
+          if (wave > 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = v[r];
+          }
+          __syncthreads();
+          if (wave == 0) {
+            float* out = partial + m * half + (c * nchunks + chunk) * (long)(MT * 16) * ldp;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              float x = v[r];
+#pragma unroll
+              for (int w = 0; w < OV_WAVES - 1; ++w) x += red[w][r][lane];
+              out[(long)(a * 16 + grp * 4 + r) * ldp + 2 * (b * 16 + row) + part] = x;
+            }
+          }
+          __syncthreads();
+        }
+}
+
 // M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
 __global__ void k_overlap_reduce(const float* partial, double2* M, int ns, int nt, int nchunks,
                                  int mt, int nc_t, long ncfg) {
@@ -475,6 +632,38 @@ hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const doub
   // column pairs (2t, 2t + 1) over 2 nt tiles of 16: the overlap's reduction applies
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M, n1,
                      n2, nchunks, mt, 2 * nt, (long)ncfg);
+  return hipGetLastError();
+}
+
+size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg) {
+  return 2 * pair_bf16x3_partial_floats(n, n, vol, ncfg);
+}
+
+hipError_t launch_pair_gh_bf16x3(const float2* psi, const double* wf, const int* opp,
+                                 double2* G, double2* H, int n, long vol, int ncfg,
+                                 float* partial, hipStream_t s) {
+  if (n < 1 || n > 48 || (vol & 3)) return hipErrorInvalidValue;
+  const int mt = (n + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const int nchunks = (int)((K + kc - 1) / kc);
+  const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
+  if (mt == 1)
+    hipLaunchKernelGGL(k_pair_gh_bf16x3<1>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
+  else if (mt == 2)
+    hipLaunchKernelGGL(k_pair_gh_bf16x3<2>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
+  else
+    hipLaunchKernelGGL(k_pair_gh_bf16x3<3>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long nout = (long)ncfg * n * n;
+  long blocks = (nout + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  const size_t half = (size_t)ncfg * nchunks * (mt * 16) * (mt * 32);
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, G, n, n,
+                     nchunks, mt, 2 * mt, (long)ncfg);
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial + half, H,
+                     n, n, nchunks, mt, 2 * mt, (long)ncfg);
   return hipGetLastError();
 }
 

@@ -110,15 +110,18 @@ def test_ensemble_pair_matches_oracle(ctx, orc, prec):
     ens.close()
 
 
-@pytest.mark.parametrize("prec,N", [(1, 7), (1, 9), (0, 7)])
-def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N):
+@pytest.mark.parametrize("prec,N,L", [(1, 7, 8), (1, 9, 8), (0, 7, 8), (0, 35, 16)])
+def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N, L):
     """Pair sinks with conjugate partners: the pair matrices come from two products over the
-    stored volumes (mirrored sites, and the same sites conjugated); N = 9 leaves two orbitals
-    of a partial shell without partner.  Against the oracle's run, which evolves every orbital:
-    FP64 1e-9, FP32 1e-4 in log|det|."""
+    stored volumes (mirrored sites, and the same sites conjugated; in FP32 one pass computes
+    both); N = 9 leaves two orbitals of a partial shell without partner, N = 35 (19 volumes)
+    is the pair-sink benchmark's orbital set.  Against the oracle's run, which evolves every
+    orbital: FP64 1e-9, FP32 1e-4 in log|det| (5e-4 for the 35 x 35 minors, whose pivots
+    amplify the ~2^-17 relative error of the split products; the every-orbital FP32 run is
+    checked to the same bound beside it)."""
     import nrcps_amd as na
 
-    L, T, C, seed = 8, 6, 3, 29
+    T, C, seed = 6, 3, 29
     print(f"[gpu] ensemble pair sinks, conjugate partners, prec={prec} N={N}", flush=True)
     moms = orc.lowest_momenta(L, N)
     kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
@@ -127,12 +130,21 @@ def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N):
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=na.SPLIT_KSPACE,
                       pair_w=w, pair_opp=opp)
     ens.set_symmetry(True)
-    assert ens.volumes_per_config() == (4 if N == 7 else 6)
+    assert ens.volumes_per_config() == {7: 4, 9: 6, 35: 19}[N]
     la, ph, _ = ens.run(seed, 0, T)
-    tol = 1e-9 if prec == 1 else 1e-4
+    tol = 1e-9 if prec == 1 else (1e-4 if N < 20 else 5e-4)
+    full = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=na.SPLIT_KSPACE,
+                       pair_w=w, pair_opp=opp)
+    la_full, _, _ = full.run(seed, 0, T)
+    full.close()
     for c in range(C):
         ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c, pair_w=w,
                              pair_opp=opp)
+        fin = np.isfinite(ref["logabs"])
+        efull = np.abs(la_full[:, c, :][fin] - ref["logabs"][fin]).max()
+        print(f"[gpu] config {c}: every orbital evolved, max |d log|det|| {efull:.2e}",
+              flush=True)
+        assert efull < tol
         both_inf = np.isneginf(la[:, c, :]) & np.isneginf(ref["logabs"])
         assert np.all(np.isneginf(la[:, c, :]) == np.isneginf(ref["logabs"]))
         err = np.abs(np.where(both_inf, 0.0, la[:, c, :] - ref["logabs"])).max()
