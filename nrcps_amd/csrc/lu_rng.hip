@@ -149,6 +149,10 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   double dsign = 1.0, la = 0.0, as = 0.0;
   double2 ph = make_double2(1.0, 0.0);
   int my_indx = 0;
+  // lane k keeps pivot k; the logs, arguments and unit phases of all pivots are taken after
+  // the elimination, one pivot per lane, and summed across the wavefront
+  double2 my_piv = make_double2(1.0, 0.0);
+  double my_mag = 1.0;
   if (!bad) {
     for (int k = 0; k < sz; ++k) {
       double bv = (i >= k && i < sz) ? vv * cabs2(a[k * sz + i]) : -1.0;
@@ -179,9 +183,10 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
         mag = 1.0e-20;
         if (i == 0) a[k * sz + k] = piv;
       }
-      la += log(mag);
-      ph = cmuld(ph, make_double2(piv.x / mag, piv.y / mag));
-      as += atan2(piv.y, piv.x);
+      if (i == k) {
+        my_piv = piv;
+        my_mag = mag;
+      }
       const double2 r = crecip(piv);
       if (i > k && i < sz) {
         const double2 l = cmuld(a[k * sz + i], r);
@@ -195,6 +200,17 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
         }
       }
       __syncthreads();
+    }
+    if (i < sz) {
+      la = log(my_mag);
+      ph = make_double2(my_piv.x / my_mag, my_piv.y / my_mag);
+      as = atan2(my_piv.y, my_piv.x);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      la += __shfl_xor(la, off);
+      as += __shfl_xor(as, off);
+      ph = cmuld(ph, make_double2(__shfl_xor(ph.x, off), __shfl_xor(ph.y, off)));
     }
   }
   const long o = cfg * (opt.onm ? opt.onm : nm) + (long)m * opt.ostep + opt.oofs;
