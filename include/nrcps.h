@@ -160,7 +160,8 @@ int nrc_ensemble_run(nrc_ensemble* ens, uint64_t seed, uint32_t config0, int T, 
 int nrc_ensemble_run_fields(nrc_ensemble* ens, const uint32_t* words, int T, double* logabs,
                             double* phase, double* argsum);
 /* Orbitals of configuration c after the last run, N x V complex in the ensemble precision,
- * in storage space (k-space for the k-space split). */
+ * in storage space (k-space for the k-space split).  NRC_ERR_ARG before the first run and
+ * after nrc_ensemble_set_symmetry until the next one. */
 int nrc_ensemble_orbitals(nrc_ensemble* ens, int c, void* out);
 /* Kernel path for the evolution: NRC_PATH_SINGLE_PASS evolves each slice in one kernel that
  * reads and writes every orbital once (FP32 L <= 32, FP64 L <= 16); NRC_PATH_TWO_PASS uses an

@@ -321,6 +321,7 @@ struct nrc_ensemble {
   void* sinks_k = nullptr;   // k-space split, dense sinks: K(1/2) phi_s
   std::vector<double> khalf; // k-space split: K(1/2) per site (host)
   bool folded = false;       // the stored orbitals are xi_half psi (k-space split, t >= 1)
+  bool has_state = false;    // psi holds orbitals in the current layout (a run has started)
   void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass); MUL_TABLE format
   void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
   void* sinks = nullptr;     // dense sinks [N][V] complex (ensemble precision), or null
@@ -796,6 +797,7 @@ static nrc::GatherSpec ens_gather_spec(const nrc_ensemble* e) {
 static int ens_set_sources(nrc_ensemble* e) {
   const long C = e->d.n_configs;
   e->folded = false;
+  e->has_state = true;
   const int nvo = e->nvo;
   if (e->d.split == NRC_SPLIT_KSPACE)
     NRC_HIP(nrc::launch_set_unit_sources(e->d.prec, e->psi, e->sym ? e->vo_site : e->site_of_orb,
@@ -1183,6 +1185,7 @@ static void ens_expand_partners(const nrc_ensemble* e, T* o) {
 
 extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
   if (!e || !out || c < 0 || c >= e->d.n_configs) return NRC_ERR_ARG;
+  if (!e->has_state) return NRC_ERR_ARG;  // no run since creation or the last layout change
   const size_t vbytes = (size_t)e->V * elem_bytes(e->d.prec);
   NRC_HIP(hipMemcpyAsync(out, (char*)e->psi + (size_t)c * e->nvo * vbytes, e->nvo * vbytes,
                          hipMemcpyDeviceToHost, e->stream));
@@ -1234,6 +1237,7 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   if (e->pgh) hipFree(e->pgh);
   e->sym = nullptr;
   e->pgh = nullptr;
+  e->has_state = false;  // the stored volumes no longer match the layout
   e->nvo = e->d.n_orb;
   e->h_col_vol.clear();
   e->h_col_conj.clear();

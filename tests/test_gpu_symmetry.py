@@ -63,7 +63,12 @@ def test_conj_partners_orbitals_vs_oracle(ctx, orc, split):
     moms = orc.lowest_momenta(L, N)
     kin = _tables(orc, L, split)
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, 2, prec=na.PREC_F64, split=split)
+    with pytest.raises(na.NrcError):  # no run yet
+        ens.orbitals(0)
+    ens.run(3, 40, 2)
     ens.set_symmetry(True)
+    with pytest.raises(na.NrcError):  # stored in the layout before the change
+        ens.orbitals(0)
     ens.run(3, 40, T)
     ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, seed=3, config=41,
                          want_psi=True)["psi"]
