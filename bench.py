@@ -61,7 +61,39 @@ def cpu_baseline(L, N, tau, A, slices, threads):
     return {"value": threads * slices / secs, "unit": "configs*slices/s", "cores": threads,
             "kind": "port",
             "sample": f"{threads} threads x 1 config x {slices} slices at L={L}, N={N}, FP64; "
-                      f"{secs:.2f} s wall"}
+                      f"{secs:.2f} s wall",
+            "fft_calibration": fft_calibration(L, N)}
+
+
+def fft_calibration(L, N, reps=3):
+    """The oracle's FP64 3D FFT against numpy's (pocketfft) on one core, on the baseline's
+    batch (N volumes of L^3, one forward and one backward transform per slice): GFLOP/s at the
+    nominal 5 V log2 V per transform, and the baseline's cs/s if its transforms ran at numpy's
+    speed (the rest of a slice unchanged)."""
+    import oracle
+
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((N, L, L, L)) + 1j * rng.standard_normal((N, L, L, L))
+    oracle.fft3d(a, -1)
+    np.fft.fftn(a, axes=(1, 2, 3))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.fft3d(oracle.fft3d(a, -1), 1)
+    t_orc = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        np.fft.ifftn(np.fft.fftn(a, axes=(1, 2, 3)), axes=(1, 2, 3))
+    t_np = (time.perf_counter() - t0) / reps
+    V = L ** 3
+    flops = 2 * N * 5.0 * V * np.log2(V)
+    moms = _lowest(L, N)
+    kin = _tables(L, 0.05, 1.0)
+    t_slice = oracle.time_threads(L, N, 2, moms, kin, 0.4, nthreads=1, config=1 << 21) / 2
+    return {"oracle_fft_gflops": flops / t_orc / 1e9, "numpy_fft_gflops": flops / t_np / 1e9,
+            "ratio": t_np / t_orc, "slice_seconds_1_core": t_slice,
+            "fft_seconds_per_slice_1_core": t_orc,
+            "baseline_scale_with_numpy_fft": t_slice / max(t_slice - t_orc + t_np, 1e-9),
+            "note": "times include the copies of each call; one core"}
 
 
 def measured_counters(L, N, C, per_launch, nvo):
