@@ -30,7 +30,7 @@ EXPORTS = [
     "nrc_plus_one_div_vol",
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
-    "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes",
+    "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
     "nrc_ensemble_destroy", "nrc_batch_create", "nrc_batch_run", "nrc_batch_capture_ms",
@@ -86,6 +86,7 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_run_fields": [vp, vp, i, dp, dp, dp],
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
         "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
+        "nrc_ensemble_set_trap": [vp, vp],
         "nrc_ensemble_profile": [vp, i],
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "nrc_ensemble_results": [vp, dp, dp, dp],
@@ -327,6 +328,17 @@ class Ensemble:
         """Conjugate partners (nrc_ensemble_set_symmetry): evolve one orbital of each (k, -k)
         pair and read the other's sink values as the conjugate mirror.  Call before a run."""
         _check(self.lib.nrc_ensemble_set_symmetry(self.h, int(conj)), "ensemble_set_symmetry")
+
+    def set_trap(self, q):
+        """x-space multiplier q (V reals) applied with the field every slice, e.g. (V p)^2 for
+        the trapped order K, P, I, P, K with Potential's table p; None removes it
+        (nrc_ensemble_set_trap)."""
+        if q is None:
+            _check(self.lib.nrc_ensemble_set_trap(self.h, None), "ensemble_set_trap")
+            return
+        qa = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        assert qa.size == self.L ** 3
+        _check(self.lib.nrc_ensemble_set_trap(self.h, qa.ctypes.data), "ensemble_set_trap")
 
     def volumes_per_config(self) -> int:
         n = C.c_int()

@@ -365,6 +365,8 @@ struct nrc_ensemble {
   const int* msite = nullptr;     // [N] site of -k_s
   std::vector<int> h_col_vol, h_col_conj;
   bool pair_sym_ok = false;  // pair sinks: opp is the mirror and w even (partners allowed)
+  void* trap_tab = nullptr;  // trap (nrc_ensemble_set_trap): (1 +- A) q / V, MUL_TABLE format
+  double* trap_q = nullptr;  // the trap q per site (FP64), for the smeared field
   double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
@@ -381,7 +383,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
-                  e->site_of_orb, e->moms_dev, e->sym, e->pgh};
+                  e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
@@ -714,11 +716,15 @@ __global__ void k_bits_to_complex(nrc::C2<T>* out, const uint32_t* words, int nw
   }
 }
 
+// the real field (re + 1)/V, times the trap q(site) when one is set (q null: 1)
 template <typename T>
-__global__ void k_complex_to_field(const nrc::C2<T>* in, T* out, long n, double invV) {
+__global__ void k_complex_to_field(const nrc::C2<T>* in, T* out, long n, double invV,
+                                   const double* q, long V) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x)
-    out[i] = (T)(((double)in[i].x + 1.0) * invV);
+       i += (long)gridDim.x * blockDim.x) {
+    const double f = ((double)in[i].x + 1.0) * invV;
+    out[i] = (T)(q ? f * q[i % V] : f);
+  }
 }
 
 static int ens_smear_field(nrc_ensemble* e, const uint32_t* words) {
@@ -751,11 +757,11 @@ static int ens_smear_field(nrc_ensemble* e, const uint32_t* words) {
   if (e->d.prec == NRC_PREC_F64)
     hipLaunchKernelGGL(k_complex_to_field<double>, dim3(blocks), dim3(256), 0, s,
                        (const nrc::C2<double>*)e->work, (double*)e->rfield, n,
-                       1.0 / (double)e->V);
+                       1.0 / (double)e->V, (const double*)e->trap_q, e->V);
   else
     hipLaunchKernelGGL(k_complex_to_field<float>, dim3(blocks), dim3(256), 0, s,
                        (const nrc::C2<float>*)e->work, (float*)e->rfield, n,
-                       1.0 / (double)e->V);
+                       1.0 / (double)e->V, (const double*)e->trap_q, e->V);
   NRC_HIP(hipGetLastError());
   return NRC_OK;
 }
@@ -929,6 +935,10 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
       a.mid = fieldspec;
       a.mid.a0 = 1.0 / (double)V;
       a.mid.a1 = e->d.A / (double)V;
+      if (e->trap_tab) {  // the one-slice kernel reads the trapped factors per site
+        a.mid.table = e->trap_tab;
+        a.grid = 0;
+      }
     }
   } else {
     a.mid = mul_table(e->tab_yz);
@@ -1044,7 +1054,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   const bool measure_first = ref || e->sinks;
   // k-space split with the plain field: up to SLICE_FUSE_MAX slices per launch of the
   // persistent kernel, the measurements between them gathered inside it
-  const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair;
+  const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair && !e->trap_tab;
   for (int t = 0; single && t < T;) {
     const int S = fuse ? std::min(nrc::SLICE_FUSE_MAX, T - t) : 1;
     if (measure_first) NRC_TRY(ens_measure(e, t));
@@ -1116,6 +1126,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
           a.m1.bits = field_of(t);
           a.m1.a0 = 1.0 / (double)V;
           a.m1.a1 = e->d.A / (double)V;
+          a.m1.table = e->trap_tab;  // null without a trap
         }
       } else {
         a.gather = 1;
@@ -1220,6 +1231,37 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
 // -k is the image of the one of momentum k; so only one orbital of each (k, -k) pair is
 // evolved and the other's sink values are read conjugated at the mirrored sinks.  Orbital
 // n is a representative unless an earlier orbital already claimed it as its partner.
+// Trap: every slice the x-space multiplier of the field becomes (1 + A phi) q / V (or the
+// smeared field times q).  For the trapped production order K, P(1/2), I, P(1/2), K
+// (production/slater_det3_trapped/main-cum.C:97-106) q = (V p)^2 with p Potential's table
+// (src/hamiltonian/hamiltonian.C:163-240): the three x-space multiplies commute, and each
+// forward / backward pair with its 1/V is the multiply alone.
+extern "C" int nrc_ensemble_set_trap(nrc_ensemble* e, const double* q) {
+  if (!e) return NRC_ERR_ARG;
+  if (e->stream != e->ctx->stream) return NRC_ERR_ARG;  // not while in a batch
+  if (q && e->d.split != NRC_SPLIT_KSPACE) return NRC_ERR_UNSUPPORTED;
+  const long V = e->V;
+  for (long i = 0; q && i < V; ++i)
+    if (!std::isfinite(q[i])) return NRC_ERR_ARG;
+  hipSetDevice(e->ctx->device);
+  NRC_HIP(hipStreamSynchronize(e->stream));
+  if (e->trap_tab) hipFree(e->trap_tab);
+  if (e->trap_q) hipFree(e->trap_q);
+  e->trap_tab = nullptr;
+  e->trap_q = nullptr;
+  if (!q) return NRC_OK;
+  std::vector<double> h(2 * V);
+  const double a0 = 1.0 / (double)V, a1 = e->d.A / (double)V;
+  for (long i = 0; i < V; ++i) {
+    h[i] = (a0 + a1) * q[i];
+    h[V + i] = (a0 - a1) * q[i];
+  }
+  NRC_TRY(ens_upload_pair(e, &e->trap_tab, h));
+  NRC_TRY(dev_alloc(&e->trap_q, V));
+  NRC_HIP(hipMemcpy(e->trap_q, q, V * sizeof(double), hipMemcpyHostToDevice));
+  return NRC_OK;
+}
+
 extern "C" int nrc_ensemble_volumes(nrc_ensemble* e, int* nvo) {
   if (!e || !nvo) return NRC_ERR_ARG;
   *nvo = e->nvo;

@@ -12,7 +12,9 @@ enum MulKind { MUL_NONE = 0, MUL_TABLE = 1, MUL_FIELD = 2, MUL_RFIELD = 3 };
 
 // A real per-site multiplier applied inside an FFT pass.
 //   MUL_TABLE : table[site]                       (same for every volume)
-//   MUL_FIELD : a0 + a1 * phi(site), phi = +-1 from the configuration's packed bits
+//   MUL_FIELD : a0 + a1 * phi(site), phi = +-1 from the configuration's packed bits; with a
+//               table (a trap) table[site] for phi = +1 and table[V + site] for phi = -1,
+//               the factors (a0 +- a1) q(site) in MUL_TABLE format
 //   MUL_RFIELD: table[config * V + site]          (per-configuration real field)
 //
 // In FP32 passes a MUL_TABLE entry is an interleaved (hi, lo) float pair applied as

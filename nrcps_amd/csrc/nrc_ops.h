@@ -89,6 +89,19 @@ __device__ __forceinline__ void apply_mul_k(C2<T>* v, const MulSpec& m, long cfg
     const T ph = T(dp), mh = T(dm);
     const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
     const rsrc_t rb = make_rsrc(m.bits + cfg * m.nw, (uint32_t)m.nw * 4u);
+    if (m.table) {  // with a trap: the factors (a0 + a1 phi) q(site), phi = +1 table first
+      using E = TabEntry<T>;
+      const rsrc_t rt = make_rsrc(m.table, (uint32_t)(2 * V * sizeof(E)));
+#pragma unroll
+      for (int i = 0; i < L; ++i) {
+        const uint32_t site = (uint32_t)(i * STRIDE) + off;
+        const uint32_t w = bload<uint32_t>(rb, (site >> 5) << 2, 0u);
+        const bool up = (w >> (site & 31u)) & 1u;
+        const uint32_t idx = site + (up ? 0u : (uint32_t)V);
+        v[i] = scale_tab<T>(v[i], bload<E>(rt, idx * (uint32_t)sizeof(E), 0u));
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < L; ++i) {
       uint32_t w, bit;
@@ -141,11 +154,15 @@ __device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, 
     const T ph = T(dp), mh = T(dm);
     const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
     const uint32_t* bits = m.bits + cfg * m.nw;
+    const TabEntry<T>* trap = reinterpret_cast<const TabEntry<T>*>(m.table);
 #pragma unroll
     for (int i = 0; i < L; ++i) {
       const long site = i * stride + off;
       const bool up = (bits[site >> 5] >> (site & 31)) & 1u;
-      v[i] = scale2<T>(v[i], up ? ph : mh, up ? pl : ml);
+      if (trap)  // (a0 + a1 phi) q(site), phi = +1 table first
+        v[i] = scale_tab<T>(v[i], trap[site + (up ? 0 : V)]);
+      else
+        v[i] = scale2<T>(v[i], up ? ph : mh, up ? pl : ml);
     }
     return;
   }

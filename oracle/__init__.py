@@ -41,7 +41,7 @@ class RunDesc(C.Structure):
                 ("A", C.c_double), ("smear", C.c_void_p), ("field_kind", C.c_int),
                 ("seed", C.c_uint64), ("config", C.c_uint32), ("rng", C.c_void_p),
                 ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p),
-                ("sources", C.c_void_p)]
+                ("sources", C.c_void_p), ("pot", C.c_void_p)]
 
 
 def build(force: bool = False) -> None:
@@ -70,6 +70,9 @@ def lib() -> C.CDLL:
         L.orc_kinetic_xi.restype = None
         L.orc_kinetic_exp.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
         L.orc_kinetic_exp.restype = None
+        L.orc_potential_table.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_potential_table.restype = C.c_int
         L.orc_interaction_table.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, C.c_double,
                                             C.c_int, _dp, C.c_int, C.c_double, C.c_double]
         L.orc_interaction_field.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
@@ -163,6 +166,19 @@ def kinetic_xi(L, dtype, mass, dt, cutoff_sq=np.pi ** 2, bc=None):
 def kinetic_exp(L, tau, mass):
     out = np.empty(L ** 3)
     lib().orc_kinetic_exp(out, L, L, L, tau, mass)
+    return out
+
+
+POT_FORM_NONE, POT_FORM_HARMONIC = 0, 1
+POT_TYPE_LIN, POT_TYPE_EXP = 0, 1
+
+
+def potential_table(L, form, ptype, k, dt):
+    """Potential's table (with its 1/V) for spring constants k = (k1, k2, k3)."""
+    out = np.empty(L ** 3)
+    rc = lib().orc_potential_table(out, L, L, L, form, ptype, float(k[0]), float(k[1]),
+                                   float(k[2]), dt)
+    assert rc == 0
     return out
 
 
@@ -325,7 +341,7 @@ def sho_source(coeffs: np.ndarray, L: int, l1: float, l2: float, l3: float) -> n
 
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
-               want_psi=False, pair_w=None, pair_opp=None, sources=None):
+               want_psi=False, pair_w=None, pair_opp=None, sources=None, pot=None):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
     (complex), argsum [T, nm], optional M [T, N, N] (the pair matrix with pair_w) and
     psi [N, V].  With pair_w / pair_opp: nm = 2N, SlaterDet3's determinants (2n + 1: leading
@@ -352,6 +368,9 @@ def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smea
         src = np.ascontiguousarray(sources, dtype=np.complex128).reshape(-1).view(np.float64)
         keep.append(src)
         d.sources = src.ctypes.data
+    if pot is not None:  # Potential table: slices K, P, I, P, K (trapped production order)
+        pt = np.ascontiguousarray(pot, dtype=np.float64)
+        d.pot = pt.ctypes.data
     if pair_w is not None:
         pw = np.ascontiguousarray(pair_w, dtype=np.float64)
         po = np.ascontiguousarray(pair_opp, dtype=np.int32)

@@ -266,6 +266,43 @@ void orc_kinetic_exp(double* out, int lx, int ly, int lz, double tau, double mas
       }
 }
 
+/* src/hamiltonian/hamiltonian.C:163-240 (Potential::Potential); the site index is
+ * z + Z (y + Y x) and each coordinate n >= L/2 becomes n - L */
+int orc_potential_table(double* out, int lx, int ly, int lz, int form, int type, double k1,
+                        double k2, double k3, double dt) {
+  if (form != ORC_POT_FORM_NONE && form != ORC_POT_FORM_HARMONIC) return -1;
+  if (type != ORC_POT_TYPE_LIN && type != ORC_POT_TYPE_EXP) return -1;
+  const long V = (long)lx * ly * lz;
+  for (long i = 0; i < V; ++i) {
+    int z = (int)(i % lz), y = (int)((i / lz) % ly), x = (int)(i / ((long)lz * ly));
+    if (x >= lx / 2) x -= lx;
+    if (y >= ly / 2) y -= ly;
+    if (z >= lz / 2) z -= lz;
+    double v = 0.0;
+    if (form == ORC_POT_FORM_HARMONIC) {
+      v = k1 * x * x;
+      v += k2 * y * y;
+      v += k3 * z * z;
+      v /= 2.0;
+    }
+    const double p = type == ORC_POT_TYPE_LIN ? 1.0 - dt * v : exp(-dt * v);
+    out[i] = p / (double)V;
+  }
+  return 0;
+}
+
+/* Potential::Evolve (src/hamiltonian/noarch/hamiltonian_evolve.C:69-93) on N volumes:
+ * Forward, multiply by the table, Backward */
+static void orc_potential_evolve(double* psi, const double* pot, int L, int N) {
+  const long V = (long)L * L * L;
+  orc_fft3d(psi, L, L, L, N, -1);
+  for (int n = 0; n < N; ++n) {
+    double* a = psi + 2 * (size_t)n * V;
+    for (long i = 0; i < V; ++i) { a[2 * i] *= pot[i]; a[2 * i + 1] *= pot[i]; }
+  }
+  orc_fft3d(psi, L, L, L, N, +1);
+}
+
 /* src/hamiltonian/hamiltonian.C:77-147 */
 int orc_interaction_table(double* out, int lx, int ly, int lz, const int bc[3], double mass,
                           int itype, const double* couplings, int n_couplings, double dt,
@@ -818,6 +855,7 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
   const int pair = d->pair_w != NULL;
   const int nm = pair ? 2 * N : (d->minors ? N : 1);
   if (pair && (xspace || !d->pair_opp)) return -1;
+  if (d->pot && xspace) return -1;
   double* psi = (double*)malloc(2 * (size_t)N * V * sizeof(double));
   double* sink = (double*)malloc(2 * (size_t)N * V * sizeof(double));
   double* fld = (double*)malloc(2 * (size_t)V * sizeof(double));
@@ -887,6 +925,7 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
         double* a = psi + 2 * (size_t)n * V;
         for (long i = 0; i < V; ++i) { a[2 * i] /= d->kin[i]; a[2 * i + 1] /= d->kin[i]; }
       }
+      if (d->pot) orc_potential_evolve(psi, d->pot, L, N);
       orc_fft3d(psi, L, L, L, N, -1); /* Interaction::Evolve: Forward */
       for (int n = 0; n < N; ++n) {
         double* a = psi + 2 * (size_t)n * V;
@@ -898,6 +937,7 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
         }
       }
       orc_fft3d(psi, L, L, L, N, +1); /* Backward */
+      if (d->pot) orc_potential_evolve(psi, d->pot, L, N);
       for (int n = 0; n < N; ++n) {
         double* a = psi + 2 * (size_t)n * V;
         for (long i = 0; i < V; ++i) { a[2 * i] /= d->kin[i]; a[2 * i + 1] /= d->kin[i]; }

@@ -46,6 +46,14 @@ void orc_kinetic_xi(double* out, int lx, int ly, int lz, const int bc[3], int ty
 void orc_kinetic_exp(double* out, int lx, int ly, int lz, double tau, double mass);
 /* Interaction form factor f(k) (src/hamiltonian/hamiltonian.C:77-147).  Returns 0, or -1 if
  * the Taylor series is negative somewhere (the reference aborts there). */
+/* Potential's table (src/hamiltonian/hamiltonian.C:163-240): v = (k1 x^2 + k2 y^2 + k3 z^2)/2
+ * with each coordinate folded to [-L/2, L/2) (form 1 = HARMONIC; form 0 = NONE gives v = 0),
+ * then (1 - dt v)/V (type 0 = LIN) or exp(-dt v)/V (type 1 = EXP).  Returns -1 for another
+ * form or type. */
+enum { ORC_POT_FORM_NONE = 0, ORC_POT_FORM_HARMONIC = 1 };
+enum { ORC_POT_TYPE_LIN = 0, ORC_POT_TYPE_EXP = 1 };
+int orc_potential_table(double* out, int lx, int ly, int lz, int form, int type, double k1,
+                        double k2, double k3, double dt);
 int orc_interaction_table(double* out, int lx, int ly, int lz, const int bc[3], double mass,
                           int itype, const double* couplings, int n_couplings, double dt,
                           double cutoff_sq);
@@ -161,6 +169,10 @@ typedef struct {
   const int* pair_opp;  /* V opposite-parity indices for pair_w */
   const double* sources; /* optional N x V complex sources = sinks in storage space (k-space
                             for the k-space split) instead of the plane waves of moms */
+  const double* pot;     /* optional V real Potential table (orc_potential_table, with its
+                            1/V; k-space split): each slice is then the trapped production
+                            order K, P, I, P, K (production/slater_det3_trapped/main-cum.C:
+                            97-106), every P its own forward / backward transform pair */
 } orc_run_desc;
 
 /* Runs one configuration through T slices, measuring before evolving at each t
