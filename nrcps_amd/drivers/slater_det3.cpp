@@ -1,11 +1,11 @@
 // slater_det3.cpp -- the pair-sink correlator driver on the MI355X class layer.
 //
 // The measurement loop of production/slater_det3_trapped/main-cum.C:140-295 with the
-// reference's class API and file formats, untrapped: arguments from args/*.arg (two_body.arg
-// for the pair wave function), N = propagator.arg's n_fermions SHO sources and one-body sinks
-// whose coefficients are read from sources/source.<n> ("n1 n2 n3 re im" lines, as
-// main-cum.C:75-78 and 139-143 set them),
-// K(1/2) I K(1/2) per slice; before each slice SlaterDet3::Initialize(&two_body) and
+// reference's class API and file formats: arguments from args/*.arg (two_body.arg for the
+// pair wave function), N = propagator.arg's n_fermions SHO sources and one-body sinks whose
+// coefficients are read from sources/source.<n> ("n1 n2 n3 re im" lines, as main-cum.C:75-78
+// and 139-143 set them), K(1/2) I K(1/2) per slice, or K(1/2) P(1/2) I P(1/2) K(1/2) when
+// args/potential.arg is present (main-cum.C:97-106); before each slice SlaterDet3::Initialize(&two_body) and
 // for n = 0 .. N-1 the determinants Run(n+1, -1, -1) (n+1 pairs, into results/slater.<2n+2>)
 // and Run(n+1, -1, n) (n pairs and one particle, into results/slater.<2n+1>), with the
 // powers log|det|^(k+1), k < 6, accumulated into results/z<k+1>.<m>.  One line of
@@ -59,6 +59,15 @@ int main() {
   Interaction interaction(&lattice, interaction_arg, 1.0);
   Kinetic kinetic(kinetic_arg, 0.5);
   std::vector<Hamiltonian*> hamiltonians = {&kinetic, &interaction, &kinetic};
+  // with args/potential.arg the trapped improved order K, P(1/2), I, P(1/2), K
+  // (production/slater_det3_trapped/main-cum.C:97-106)
+  Potential* potential = nullptr;
+  if (FILE* pf = std::fopen("args/potential.arg", "r")) {
+    std::fclose(pf);
+    PotentialArg potential_arg("args/potential.arg");
+    potential = new Potential(potential_arg, 0.5);
+    hamiltonians = {&kinetic, potential, &interaction, potential, &kinetic};
+  }
 
   const int T = GJP.Tsites();
   std::vector<OneBody*> one_bodies(N);
@@ -131,6 +140,7 @@ int main() {
     }
   }
   for (int n = 0; n < N; ++n) delete one_bodies[n];
+  delete potential;
   for (FILE* f : file) Fclose(f);
   for (FILE* f : zfile) Fclose(f);
   Comms::Finalize();

@@ -72,11 +72,13 @@ def test_slater_det_driver_matches_oracle(orc):
 DRIVER3 = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det3")
 
 
-def test_slater_det3_driver_matches_oracle(orc):
+@pytest.mark.parametrize("trapped", [False, True])
+def test_slater_det3_driver_matches_oracle(orc, trapped):
     """The pair-sink driver (SlaterDet3 + TwoBody on the class layer, main-cum.C's loop):
     every determinant Run(n+1, -1, -1) and Run(n+1, -1, n) per slice and the six cumulant
     powers against the oracle's pair run with the same SHO sources (OneBody::Set), ranlxs
-    stream, smeared interaction and TGND wave function."""
+    stream, smeared interaction and TGND wave function; trapped: with the harmonic Potential
+    (Potential::Evolve on the class layer, the K P I P K order) against the oracle's."""
     assert os.path.exists(DRIVER3), "the slater_det3 driver is not built"
     L, T, ncfg, N = 8, 6, 2, 5
     V = L ** 3
@@ -100,6 +102,10 @@ def test_slater_det3_driver_matches_oracle(orc):
     args["propagator.arg"] = f"{N}\nresults/prop\n"
     args["two_body.arg"] = ("WAVEFUNC_TYPE_TGND\nDISPERSION_TYPE_STANDARD\n"
                             "DISPERSION_TYPE_STANDARD\n1.0\n1.0\n0.176776695\nresults/two\n")
+    spring = (0.04, 0.03, 0.05)
+    if trapped:
+        args["potential.arg"] = ("POTENTIAL_FORM_HARMONIC\nPOTENTIAL_TYPE_EXP\n"
+                                 f"{spring[0]}\n{spring[1]}\n{spring[2]}\n")
     with tempfile.TemporaryDirectory() as tmp:
         for d in ("args", "results", "rngs", "sources"):
             os.makedirs(os.path.join(tmp, d))
@@ -131,8 +137,10 @@ def test_slater_det3_driver_matches_oracle(orc):
     ranlxs = orc.ranlxs(2, 7012)
     moms = np.zeros((N, 3), np.int32)  # unused: the sources are given
     for j in range(ncfg):
+        pot = (orc.potential_table(L, orc.POT_FORM_HARMONIC, orc.POT_TYPE_EXP, spring, 0.5)
+               if trapped else None)
         ref = orc.run_config(L, N, T, moms, kin, smear=f, field="ranlxs", rng=ranlxs,
-                             pair_w=w, pair_opp=opp, sources=src)
+                             pair_w=w, pair_opp=opp, sources=src, pot=pot)
         for m in range(2 * N):
             want = np.exp(ref["logabs"][:, m]) * ref["phase"][:, m]
             got = det[m, j, :, 0] + 1j * det[m, j, :, 1]
