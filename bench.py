@@ -39,6 +39,16 @@ def _tables(L, tau, M):
     return (1.0 + 0.5 * tau * eps).reshape(-1)
 
 
+def _harmonic_trap(L, k, dt):
+    """q = (V p)^2 for the trapped slice K P I P K, p = exp(-dt v)/V Potential's EXP table
+    (hamiltonian.C:163-240): v = (k1 x^2 + k2 y^2 + k3 z^2)/2, coordinates folded to
+    [-L/2, L/2)."""
+    n = np.arange(L)
+    f = np.where(n >= L // 2, n - L, n).astype(float) ** 2
+    v = (k[0] * f[:, None, None] + k[1] * f[None, :, None] + k[2] * f[None, None, :]) / 2.0
+    return np.exp(-2.0 * dt * v).reshape(-1)
+
+
 def _lowest(L, N):
     f = np.where(np.arange(L) > L // 2, np.arange(L) - L, np.arange(L))
     pts = [(a, b, c) for a in range(L) for b in range(L) for c in range(L)]
@@ -330,7 +340,8 @@ def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012)
             "timing": "HIP events around the overlap kernels of each measurement"}
 
 
-def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, conj=True):
+def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, conj=True,
+               trap=True):
     """The pair-sink measurement of production/slater_det3_trapped (L=32, N=35): every slice
     the pair matrix phi[s][t] = sum_p w(p) psi_s(p) psi_t(-p) of every configuration (split-
     BF16x3 on MFMA) and SlaterDet3's 2N determinants, plane-wave sources, the PAIR1 wave
@@ -347,6 +358,20 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
                       split=na.SPLIT_KSPACE, pair_w=w, pair_opp=opp.astype(np.int32))
     ens.set_symmetry(conj)
     nvo = ens.volumes_per_config()
+    trapped = None
+    if trap:  # the production trap: K P I P K with potential.arg's harmonic EXP potential
+        ens.set_trap(_harmonic_trap(L, (0.000625, 0.000625, 0.000625), 0.5))
+        ens.run(seed, 0, 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        la_t, _, _ = ens.run(seed, C, T)
+        ctx.sync()
+        dtt = time.perf_counter() - t0
+        trapped = {"value": C * T / dtt, "unit": "configs*slices/s",
+                   "finite_fraction": float(np.isfinite(la_t).mean()),
+                   "trap": "harmonic, EXP, spring constants 0.000625 (slater_det3_trapped's "
+                           "potential.arg), K P I P K per slice, one slice per launch"}
+        ens.set_trap(None)
     ens.run(seed, 0, 2)
     ctx.sync()
     t0 = time.perf_counter()
@@ -369,7 +394,7 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
     return {"workload": f"pair sinks (SlaterDet3): L={L}, N={N}, T={T}, {C} configurations, "
                         f"FP32 evolution, split-BF16x3 pair matrices, 2N FP64 LU per slice",
             "value": C * T / dt, "unit": "configs*slices/s",
-            "conj_partners": bool(conj), "volumes_per_config": nvo,
+            "conj_partners": bool(conj), "volumes_per_config": nvo, "trapped": trapped,
             "finite_fraction": float(np.isfinite(la).mean()),
             "avg_ms_per_slice": {"evolution": t_evo / max(n_evo, 1),
                                  "pair_matrices_and_gather": meas_ms,
