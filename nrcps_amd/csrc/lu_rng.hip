@@ -191,7 +191,25 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
       if (i > k && i < sz) {
         const double2 l = cmuld(a[k * sz + i], r);
         a[k * sz + i] = l;
-        for (int j = k + 1; j < sz; ++j) {
+        // eight columns at a time, their LDS reads issued together (row k is a broadcast
+        // read, row i this lane's own), then the remaining columns one by one
+        int j = k + 1;
+        for (; j + 8 <= sz; j += 8) {
+          double2 u[8], x[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            u[q] = a[(j + q) * sz + k];
+            x[q] = a[(j + q) * sz + i];
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const double2 p = cmuld(l, u[q]);
+            x[q].x = x[q].x - p.x;
+            x[q].y = x[q].y - p.y;
+            a[(j + q) * sz + i] = x[q];
+          }
+        }
+        for (; j < sz; ++j) {
           const double2 p = cmuld(l, a[j * sz + k]);
           double2 x = a[j * sz + i];
           x.x = x.x - p.x;
