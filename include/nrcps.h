@@ -187,8 +187,7 @@ int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
  * trap).  The reference's trapped slice K, P(1/2), I, P(1/2), K (production/
  * slater_det3_trapped/main-cum.C:97-106; Potential, src/hamiltonian/hamiltonian.C:163-240,
  * noarch/hamiltonian_evolve.C:69-93) is q = (V p)^2 with p Potential's table, since the three
- * x-space multiplies commute.  Trapped slices run one per launch.  Not while the ensemble is
- * in a batch. */
+ * x-space multiplies commute.  Not while the ensemble is in a batch. */
 int nrc_ensemble_set_trap(nrc_ensemble* ens, const double* q);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
 int nrc_ensemble_volumes(nrc_ensemble* ens, int* volumes_per_config);

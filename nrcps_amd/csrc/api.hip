@@ -935,10 +935,7 @@ static int ens_evolve(nrc_ensemble* e, int t, const uint32_t* words) {
       a.mid = fieldspec;
       a.mid.a0 = 1.0 / (double)V;
       a.mid.a1 = e->d.A / (double)V;
-      if (e->trap_tab) {  // the one-slice kernel reads the trapped factors per site
-        a.mid.table = e->trap_tab;
-        a.grid = 0;
-      }
+      a.mid.table = e->trap_tab;  // with a trap, the factors per site (null without)
     }
   } else {
     a.mid = mul_table(e->tab_yz);
@@ -983,6 +980,7 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
   a.mid.bits = words;
   a.mid.a0 = 1.0 / (double)V;
   a.mid.a1 = e->d.A / (double)V;
+  a.mid.table = e->trap_tab;  // null without a trap
   a.nslices = S;
   a.pre_next = e->tab_k;
   a.bits_stride = (long)stride;
@@ -1054,7 +1052,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   const bool measure_first = ref || e->sinks;
   // k-space split with the plain field: up to SLICE_FUSE_MAX slices per launch of the
   // persistent kernel, the measurements between them gathered inside it
-  const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair && !e->trap_tab;
+  const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair &&
+                    (!e->trap_tab || nrc::slice1p_trap_ok(prec, L));
   for (int t = 0; single && t < T;) {
     const int S = fuse ? std::min(nrc::SLICE_FUSE_MAX, T - t) : 1;
     if (measure_first) NRC_TRY(ens_measure(e, t));

@@ -95,6 +95,7 @@ struct SliceArgs {
 };
 constexpr int SLICE_FUSE_MAX = 4;  // slices per launch of the persistent slice kernel
 bool slice1_supported(int prec, int L);
+bool slice1p_trap_ok(int prec, int L);  // the persistent kernel takes a trap at this shape
 hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);
 // Columns of conjugate partners for the gather from memory (all null: column n is volume n):
 // column n is volume vol[n] of its configuration (nvo volumes each), read at the mirrored

@@ -169,6 +169,28 @@ __device__ __forceinline__ void mul_field_lds(C2<T>* w, const uint32_t* bl, uint
   }
 }
 
+// the same with a trap: w[y] *= (a0 + a1 phi) q(site) read from the two factor tables
+// (phi = +1 first, MUL_TABLE format), four sites at a time so that few loads are in flight
+template <typename T, int L>
+__device__ __forceinline__ void mul_field_trap(C2<T>* w, const uint32_t* bl, uint32_t off,
+                                               rsrc_t rt, uint32_t V) {
+  using E = TabEntry<T>;
+  constexpr int G = 4;
+#pragma unroll
+  for (int y0 = 0; y0 < L; y0 += G) {
+    E e[G];
+#pragma unroll
+    for (int y = y0; y < y0 + G && y < L; ++y) {
+      const uint32_t site = off + (uint32_t)(y * L);
+      const bool up = (bl[site >> 5] >> (site & 31u)) & 1u;
+      e[y - y0] = bload<E>(rt, (site + (up ? 0u : V)) * (uint32_t)sizeof(E), 0u);
+    }
+#pragma unroll
+    for (int y = y0; y < y0 + G && y < L; ++y) w[y] = scale_tab<T>(w[y], e[y - y0]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Persistent, software-pipelined k_slice1 for the k-space split (kinetic factor, field, no
 // trailing factor): one workgroup per CU walks the volumes blockIdx.x, + gridDim.x, ...  A
 // volume is loaded, transformed and stored as in k_slice1, but
@@ -182,7 +204,7 @@ __device__ __forceinline__ void mul_field_lds(C2<T>* w, const uint32_t* bl, uint
 //    two of them the plane-wave sinks are gathered from registers into the next slice's
 //    overlap matrix (a.g, M advanced by a.m_stride per slice), and slice s > 0 starts with
 //    the factor a.pre_next.  HBM traffic per slice drops by that factor.
-template <typename T, int L, int PPT, int PF, int SMAX>
+template <typename T, int L, int PPT, int PF, int SMAX, bool TRAP>
 __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   using C = C2<T>;
   using E = TabEntry<T>;
@@ -199,6 +221,8 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   const T ph = T(dp), mh = T(dm);
   const T pl = T(dp - (double)ph), ml = T(dm - (double)mh);
   const rsrc_t rt = make_rsrc(a.pre.table, (uint32_t)(V * sizeof(E)));
+  // trap: the factor tables (a0 +- a1) q, phi = +1 first (unused without one)
+  const rsrc_t rq_trap = make_rsrc(a.mid.table, TRAP ? (uint32_t)(2 * V * sizeof(E)) : 0u);
   MulSpec knext = a.pre;
   knext.table = a.pre_next;
   C* psi = reinterpret_cast<C*>(a.psi);
@@ -314,7 +338,10 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
           for (int y = 0; y < L; ++y) w[y] = col[y * LP];
           fft_reg<T, L, -1>(w);
           const int x = h * H + rx;
-          mul_field_lds<T, L>(w, b, (uint32_t)(x * P + rq), ph, pl, mh, ml);
+          if constexpr (TRAP)
+            mul_field_trap<T, L>(w, b, (uint32_t)(x * P + rq), rq_trap, (uint32_t)V);
+          else
+            mul_field_lds<T, L>(w, b, (uint32_t)(x * P + rq), ph, pl, mh, ml);
           fft_reg<T, L, +1>(w);
 #pragma unroll
           for (int y = 0; y < L; ++y) col[y * LP] = w[y];
@@ -378,18 +405,25 @@ template <typename T, int L, int PPT, int PF>
 static hipError_t launch_s1p(const SliceArgs& a, hipStream_t s) {
   constexpr int SMAX = SLICE_FUSE_MAX;
   if (a.nslices < 1 || a.nslices > SMAX) return hipErrorInvalidValue;
-  static int per_cu = 0;
-  if (per_cu == 0) {
+  static int per_cu[2] = {0, 0};
+  const bool trap = a.mid.table != nullptr;
+  if (per_cu[trap] == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_slice1p<T, L, PPT, PF, SMAX>,
-                                                     L * L / PPT, 0) != hipSuccess || nb < 1)
-      nb = 1;
-    per_cu = nb;
+    const hipError_t e =
+        trap ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &nb, k_slice1p<T, L, PPT, PF, SMAX, true>, L * L / PPT, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &nb, k_slice1p<T, L, PPT, PF, SMAX, false>, L * L / PPT, 0);
+    per_cu[trap] = (e != hipSuccess || nb < 1) ? 1 : nb;
   }
-  long grid = (long)a.grid * per_cu;
+  long grid = (long)a.grid * per_cu[trap];
   if (grid > a.nvol) grid = a.nvol;
-  hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF, SMAX>), dim3((unsigned)grid), dim3(L * L / PPT),
-                     0, s, a);
+  if (trap)
+    hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF, SMAX, true>), dim3((unsigned)grid),
+                       dim3(L * L / PPT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_slice1p<T, L, PPT, PF, SMAX, false>), dim3((unsigned)grid),
+                       dim3(L * L / PPT), 0, s, a);
   return hipGetLastError();
 }
 
@@ -400,7 +434,8 @@ static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
   // gives a workgroup count (prefetch point: before the yz rounds, or after the first pencil's
   // store where the registers are short)
   if (a.grid > 0 && a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD &&
-      a.post.kind == MUL_NONE && !a.gather_mid)
+      a.post.kind == MUL_NONE && !a.gather_mid &&
+      (a.mid.table == nullptr || slice1p_trap_ok(sizeof(T) == 8 ? NRC_PREC_F64 : NRC_PREC_F32, L)))
     return launch_s1p<T, L, PPT, (sizeof(T) == 4 && L == 24) ? 2 : 1>(a, s);
   if (a.nslices != 1) return hipErrorInvalidValue;  // fused slices: persistent kernel only
   // k-space split: kinetic factor, field (or smeared field), measured beforehand; x-space
@@ -426,6 +461,10 @@ static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
+
+// the persistent kernel with a trap's factor tables: not for FP32 L = 32, whose registers
+// are full without them (that shape runs trapped slices one per launch in k_slice1)
+bool slice1p_trap_ok(int prec, int L) { return !(prec == NRC_PREC_F32 && L == 32); }
 
 bool slice1_supported(int prec, int L) {
   if (prec == NRC_PREC_F64) return L == 8 || L == 12 || L == 16;

@@ -105,3 +105,26 @@ def test_trap_refused_for_xspace_split(ctx, orc):
     with pytest.raises(na.NrcError):
         ens.set_trap(np.ones(L ** 3))
     ens.close()
+
+
+@pytest.mark.parametrize("L", [16, 24])
+def test_trap_f32_four_slices_per_launch(ctx, orc, L):
+    """FP32 trapped slices in the persistent kernel, four per launch (T = 9: launches of 4,
+    4 and 1), against the oracle's literal order; the measurements between fused slices are
+    gathered inside the kernel."""
+    import nrcps_amd as na
+
+    N, T, C, seed = 19, 9, 3, 47
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 20.0, 0.5)
+    pot, q = _trap(orc, L, 1, (0.01, 0.02, 0.015))
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32, minors=True)
+    ens.set_trap(q)
+    la, _, _ = ens.run(seed, 0, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, minors=True, seed=seed, config=c,
+                             pot=pot)
+        d = np.abs(la[:, c] - ref["logabs"]).max()
+        print(f"[gpu] FP32 trapped L={L}, fused slices: max |d log|det|| {d:.2e}", flush=True)
+        assert d < 1e-4
+    ens.close()
