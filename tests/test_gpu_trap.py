@@ -128,3 +128,31 @@ def test_trap_f32_four_slices_per_launch(ctx, orc, L):
         print(f"[gpu] FP32 trapped L={L}, fused slices: max |d log|det|| {d:.2e}", flush=True)
         assert d < 1e-4
     ens.close()
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_trap_with_pair_sinks_and_partners(ctx, orc, prec):
+    """SlaterDet3's measurement on trapped slices (the slater_det3_trapped loop) with conjugate
+    partners, against the oracle's pair run with the literal K P I P K order: FP64 1e-9, FP32
+    1e-4 + 1e-5 |log|det|| (the pair minors reach |log|det|| ~ 30 here)."""
+    import nrcps_amd as na
+
+    L, N, T, C, seed = 8, 7, 6, 2, 53
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    w = orc.two_body_wavefunc(L, orc.WAVEFUNC_TGND, 0.176776695, cutoff_sq=np.pi ** 2)
+    opp = orc.opposite_parity_index(L)
+    pot, q = _trap(orc, L, 1, (0.05, 0.04, 0.03))
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, pair_w=w, pair_opp=opp)
+    ens.set_symmetry(True)
+    ens.set_trap(q)
+    la, ph, _ = ens.run(seed, 0, T)
+    for c in range(C):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c, pair_w=w,
+                             pair_opp=opp, pot=pot)
+        fin = np.isfinite(ref["logabs"])
+        assert np.array_equal(np.isfinite(la[:, c, :]), fin)
+        r = ref["logabs"][fin]
+        tol = 1e-9 if prec == 1 else 1e-4 + 1e-5 * np.abs(r)
+        assert np.all(np.abs(la[:, c, :][fin] - r) < tol)
+    ens.close()
