@@ -993,9 +993,15 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
     NRC_HIP(nrc::launch_slice1(e->d.prec, a, s));
   }
   e->folded = true;
-  for (int k = 1; k < S; ++k) {
+  if (S > 1) {  // the S - 1 measurements gathered inside the launch: one LU launch for all
     EvScope sc(e, 2);
-    NRC_TRY(ens_lu(e, e->Mx + (size_t)(k - 1) * mstride, (size_t)(t + k) * per_t));
+    nrc::LuOpts o;
+    o.lsrc = e->lsrc;
+    o.lsrc_period = C;
+    const size_t row = (size_t)(t + 1) * per_t;  // result rows t + 1 .. t + S - 1, contiguous
+    NRC_HIP(nrc::launch_lu_det(e->Mx, N, N, (int)(C * (S - 1)), e->d.minors, e->o_la + row,
+                               e->o_ph + row, e->o_as + row, e->o_st + row, nullptr, nullptr,
+                               nullptr, s, o));
   }
   return NRC_OK;
 }

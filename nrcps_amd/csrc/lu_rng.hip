@@ -241,7 +241,8 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   if (i == 0) {
     if (status) status[o] = bad ? 1 : 0;
     const double ncol = opt.lsrc_cols * sz - (opt.repl ? opt.lsrc_cols - 1 : 0);
-    if (logabs) logabs[o] = bad ? -INFINITY : (opt.lsrc ? la - ncol * opt.lsrc[cfg] : la);
+    const long cs = opt.lsrc_period > 0 ? cfg % opt.lsrc_period : cfg;
+    if (logabs) logabs[o] = bad ? -INFINITY : (opt.lsrc ? la - ncol * opt.lsrc[cs] : la);
     if (phase) phase[o] = bad ? make_double2(0.0, 0.0) : make_double2(dsign * ph.x, dsign * ph.y);
     if (argsum) argsum[o] = bad ? 0.0 : as + (dsign < 0.0 ? 3.14159265358979323846 : 0.0);
   }

@@ -143,6 +143,7 @@ struct LuOpts {
   int ldr = 0;
   const double* lsrc = nullptr;
   int lsrc_cols = 1;
+  long lsrc_period = 0;  // > 0: matrix c uses lsrc[c % lsrc_period] (several slices at once)
   int onm = 0, ostep = 1, oofs = 0;
 };
 
