@@ -72,6 +72,14 @@ int nrc_scale_real(nrc_context* ctx, void* buf_dev, const void* table_dev, int b
                    int op, int prec);
 int nrc_mul_complex(nrc_context* ctx, void* buf_dev, const void* field_dev, int batch, long vol,
                     int prec);
+/* nrc_scale_const: buf[i] *= d (op 0) or /= d (op 1) for n complex values -- the loop of
+ *   Propagator::Normalize(Float) (src/alg/propagator.C:112-133, host-only in the reference,
+ *   NotImplemented in its GPU build).
+ * nrc_norms: norms[b] = sqrt(sum |buf[b][i]|^2) of each of batch volumes (FP64 sums), to the
+ *   host -- the norms of Propagator::Normalize() (propagator.C:78-110). */
+int nrc_scale_const(nrc_context* ctx, void* buf_dev, long n, double d, int op, int prec);
+int nrc_norms(nrc_context* ctx, const void* buf_dev, int batch, long vol, int prec,
+              double* norms);
 /* field = (field + 1)/V in place (re), field/V (im): the end of Interaction::Initialize
  * (src/hamiltonian/cuda/hamiltonian_initialize.cu:30-35, hamiltonian_kernels.cu:59-71). */
 int nrc_plus_one_div_vol(nrc_context* ctx, void* buf_dev, long vol, int prec);

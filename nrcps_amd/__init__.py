@@ -31,6 +31,7 @@ EXPORTS = [
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
+    "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
     "nrc_ensemble_destroy", "nrc_batch_create", "nrc_batch_run", "nrc_batch_capture_ms",
@@ -87,6 +88,8 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
         "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
         "nrc_ensemble_set_trap": [vp, vp],
+        "nrc_scale_const": [vp, vp, C.c_long, C.c_double, i, i],
+        "nrc_norms": [vp, vp, i, C.c_long, i, vp],
         "nrc_ensemble_profile": [vp, i],
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "nrc_ensemble_results": [vp, dp, dp, dp],
@@ -201,6 +204,16 @@ class Context:
     def overlap(self, phi, psi, M, ns, nt, vol, ncfg, mode):
         _check(self.lib.nrc_overlap(self.h, phi.ptr, psi.ptr, M.ptr, ns, nt, vol, ncfg, mode),
                "overlap")
+
+    def scale_const(self, buf, n, d, op, prec):
+        """nrc_scale_const: buf *= d (op 0) or /= d (op 1) over n complex values."""
+        _check(self.lib.nrc_scale_const(self.h, buf.ptr, n, d, op, prec), "scale_const")
+
+    def norms(self, buf, batch, vol, prec):
+        """nrc_norms: the L2 norm of each of `batch` volumes of `vol` complex values."""
+        out = np.zeros(batch)
+        _check(self.lib.nrc_norms(self.h, buf.ptr, batch, vol, prec, out.ctypes.data), "norms")
+        return out
 
     def pair_overlap(self, psi1, psi2, w, opp, M, n1, n2, vol, ncfg, mode):
         """nrc_pair_overlap: M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]."""

@@ -191,6 +191,31 @@ int nrc_scale_real(nrc_context* ctx, void* buf, const void* table, int batch, lo
   return NRC_OK;
 }
 
+int nrc_scale_const(nrc_context* ctx, void* buf, long n, double d, int op, int prec) {
+  if (!ctx || !buf || n < 0 || (op != 0 && op != 1) || !prec_ok(prec)) return NRC_ERR_ARG;
+  if (op == 1 && d == 0.0) return NRC_ERR_ARG;
+  if (n == 0) return NRC_OK;
+  NRC_HIP(nrc::launch_scale_const(prec, buf, n, d, op, ctx->stream));
+  return NRC_OK;
+}
+
+int nrc_norms(nrc_context* ctx, const void* buf, int batch, long vol, int prec, double* norms) {
+  if (!ctx || !buf || !norms || batch < 0 || vol < 1 || !prec_ok(prec)) return NRC_ERR_ARG;
+  if (batch == 0) return NRC_OK;
+  double* d = nullptr;
+  if (hipMalloc(&d, (size_t)batch * sizeof(double)) != hipSuccess) return NRC_ERR_NOMEM;
+  int rc = nrc::launch_norm2(prec, buf, batch, vol, d, ctx->stream) == hipSuccess ? NRC_OK
+                                                                                 : NRC_ERR_HIP;
+  if (rc == NRC_OK &&
+      hipMemcpyAsync(norms, d, (size_t)batch * sizeof(double), hipMemcpyDeviceToHost,
+                     ctx->stream) != hipSuccess)
+    rc = NRC_ERR_HIP;
+  if (rc == NRC_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = NRC_ERR_HIP;
+  hipFree(d);
+  for (int b = 0; rc == NRC_OK && b < batch; ++b) norms[b] = sqrt(norms[b]);
+  return rc;
+}
+
 int nrc_mul_complex(nrc_context* ctx, void* buf, const void* field, int batch, long vol,
                     int prec) {
   if (!ctx || !buf || !field || batch < 0 || vol < 1 || !prec_ok(prec)) return NRC_ERR_ARG;

@@ -168,6 +168,32 @@ __global__ void k_scale_real(C2<T>* buf, const T* table, long n, long vol, int o
 }
 
 template <typename T>
+__global__ void k_scale_const(C2<T>* buf, long n, double d, int op) {
+  const T t = (T)d;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const C2<T> v = buf[i];
+    buf[i] = op ? C2<T>{v.x / t, v.y / t} : C2<T>{v.x * t, v.y * t};
+  }
+}
+
+// sum of squares of the 2 vol reals of volume blockIdx.x, FP64 accumulation
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm2(const C2<T>* buf, long vol, double* out) {
+  __shared__ double red[4];
+  const C2<T>* b = buf + (long)blockIdx.x * vol;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < vol; i += 256) {
+    const C2<T> v = b[i];
+    acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+  }
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+template <typename T>
 __global__ void k_mul_complex(C2<T>* buf, const C2<T>* f, long n, long vol) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
@@ -190,6 +216,28 @@ hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch,
   else
     hipLaunchKernelGGL(k_scale_real<float>, dim3(grid_for(n)), dim3(256), 0, s,
                        (C2<float>*)buf, (const float*)table, n, vol, op);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_const(int prec, void* buf, long n, double d, int op, hipStream_t s) {
+  if (prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_scale_const<double>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (C2<double>*)buf, n, d, op);
+  else
+    hipLaunchKernelGGL(k_scale_const<float>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (C2<float>*)buf, n, d, op);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm2(int prec, const void* buf, long batch, long vol, double* out,
+                        hipStream_t s) {
+  if (batch < 1) return hipSuccess;
+  if (prec == NRC_PREC_F64)
+    hipLaunchKernelGGL(k_norm2<double>, dim3((unsigned)batch), dim3(256), 0, s,
+                       (const C2<double>*)buf, vol, out);
+  else
+    hipLaunchKernelGGL(k_norm2<float>, dim3((unsigned)batch), dim3(256), 0, s,
+                       (const C2<float>*)buf, vol, out);
   return hipGetLastError();
 }
 

@@ -116,6 +116,11 @@ hipError_t launch_scale_real(int prec, void* buf, const void* table, long batch,
                              int op, hipStream_t s);
 hipError_t launch_mul_complex(int prec, void* buf, const void* field, long batch, long vol,
                               hipStream_t s);
+// buf[i] *= d (op 0) or /= d (op 1) for n complex values
+hipError_t launch_scale_const(int prec, void* buf, long n, double d, int op, hipStream_t s);
+// out[b] = sum over volume b of |v|^2 (FP64), device output
+hipError_t launch_norm2(int prec, const void* buf, long batch, long vol, double* out,
+                        hipStream_t s);
 hipError_t launch_fill(void* p, uint8_t v, size_t bytes, hipStream_t s);
 hipError_t launch_plus_one_div_vol(int prec, void* buf, long vol, hipStream_t s);
 // sources of nvol = configurations x norb volumes; FP32 sources of configuration c carry the

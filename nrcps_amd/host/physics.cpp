@@ -469,9 +469,22 @@ void Propagator::Set(Float* psi, int n) {
 
 Float* Propagator::Get(int n) { return (Float*)b->Get(n); }
 
-void Propagator::Normalize() { ERR.NotImplemented("void Propagator::Normalize()", "Sorry."); }
-void Propagator::Normalize(Float) {
-  ERR.NotImplemented("void Propagator::Normalize(Float)", "Sorry.");
+// propagator.C:78-110: every orbital to unit norm (host-only in the reference; here on the
+// device: norms with FP64 sums, then one division per orbital)
+void Propagator::Normalize() {
+  const char* fname = "void Propagator::Normalize()";
+  const int n = b->BatchSize();
+  std::vector<double> norms(n);
+  ERR.Check(fname, nrc_norms(ctx(), b->Get(0), n, vol, NRC_FLOAT_PREC, norms.data()));
+  for (int k = 0; k < n; ++k)
+    ERR.Check(fname, nrc_scale_const(ctx(), b->Get(k), vol, norms[k], 1, NRC_FLOAT_PREC));
+}
+
+// propagator.C:112-133: every orbital divided by `norm`
+void Propagator::Normalize(Float norm) {
+  ERR.Check("void Propagator::Normalize(Float)",
+            nrc_scale_const(ctx(), b->Get(0), (long)b->BatchSize() * vol, norm, 1,
+                            NRC_FLOAT_PREC));
 }
 
 // ------------------------------------------------------------------ OneBody

@@ -28,6 +28,9 @@ COMPATIBLE = [
     "production/slater_det2_untrapped/main-cum.C",
     "production/slater_det2_untrapped/main-no_cum.C",
     "production/slater_det3_trapped/main-cum.C",
+    # reweighted fields: Propagator::Normalize(Float), Lattice::Mean
+    "production/one_body-reweight/main.C",
+    "production/few_body_untrapped-reweight/main.C",
 ]
 
 

@@ -416,3 +416,25 @@ def test_ensemble_full_size_free_gas(ctx, orc, path):
     assert np.abs(ph[:, :, 0] - 1).max() < 1e-5
     # t = 0 measures the sources; their FP32 amplitudes r_c divide out to rounding
     assert np.abs(la[0]).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_norms_and_scale_const(ctx, prec):
+    """Propagator::Normalize's two steps (src/alg/propagator.C:78-133): per-volume L2 norms and
+    the division by a constant, against numpy."""
+    import nrcps_amd as na
+
+    rng = np.random.default_rng(prec + 5)
+    B, V = 5, 12 ** 3
+    a = (rng.standard_normal((B, V)) + 1j * rng.standard_normal((B, V)))
+    dt = np.complex64 if prec == na.PREC_F32 else np.complex128
+    a = a.astype(dt)
+    d = ctx.to_device(a)
+    n = ctx.norms(d, B, V, prec)
+    ref = np.sqrt((np.abs(a.astype(np.complex128)) ** 2).sum(axis=1))
+    assert np.abs(n - ref).max() < 1e-13 * ref.max()
+    ctx.scale_const(d, B * V, 2.5, 1, prec)
+    tol = 1e-6 if prec == na.PREC_F32 else 1e-15
+    assert np.abs(d.download() - a / 2.5).max() <= tol * np.abs(a).max()
+    ctx.scale_const(d, B * V, 2.5, 0, prec)
+    assert np.abs(d.download() - a).max() <= 2 * tol * np.abs(a).max()
