@@ -195,8 +195,9 @@ __device__ __forceinline__ void mul_field_trap(C2<T>* w, const uint32_t* bl, uin
 // trailing factor): one workgroup per CU walks the volumes blockIdx.x, + gridDim.x, ...  A
 // volume is loaded, transformed and stored as in k_slice1, but
 //  * the loads of the next volume's first pencil are issued before this volume's last yz
-//    rounds (PF = 1), before its final x-transforms (PF = 0) or once its first pencil is stored
-//    (PF = 2), so they land while the CU computes; loads complete in order, so everything this
+//    rounds (PF = 1), before its final x-transforms (PF = 0), half at each of those points
+//    (PF = 3, L = 32: fewer registers live through the rounds) or once its first pencil is
+//    stored (PF = 2), so they land while the CU computes; loads complete in order, so everything this
 //    volume still waits for is issued first: the configurations' field words (staged in LDS)
 //    go out before the other pencils, and the prefetched pencil is multiplied by its first
 //    kinetic factor at the end of the previous volume;
@@ -232,13 +233,17 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
   // Loads of the first pencil of volume w; past the last volume the resource has no records
   // and the loads return zeros without touching memory, so the load is unconditional (a
   // conditional load would keep the previous values alive across the whole volume).
-  auto prefetch = [&](long w) {
+  auto prefetch = [&](long w, auto x0_tag, auto x1_tag) {  // elements [x0, x1) of the pencil
+    constexpr int X0 = decltype(x0_tag)::value, X1 = decltype(x1_tag)::value;
     const bool ok = w < a.nvol;
     const rsrc_t rn = make_rsrc(psi + (ok ? w : 0) * V, ok ? (uint32_t)(V * sizeof(C)) : 0u);
 #pragma unroll
-    for (int x = 0; x < L; ++x)
+    for (int x = X0; x < X1; ++x)
       nx[x] = bload<C>(rn, (uint32_t)tid * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
   };
+  using X0 = std::integral_constant<int, 0>;
+  using XH = std::integral_constant<int, L / 2>;
+  using XL = std::integral_constant<int, L>;
   // the first slice factor of that pencil (the table is the same for every volume); issued
   // once the pencil's loads are out, so that waiting for it waits for nothing later
   auto factor = [&]() {
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
 #pragma unroll
     for (int x = 0; x < L; ++x) nx[x] = scale_tab<T>(nx[x], tab0[x]);
   };
-  prefetch(vol);
+  prefetch(vol, X0{}, XL{});
   factor();
   for (; vol < a.nvol; vol += gridDim.x) {
     const long cfg = vol / a.norb;
@@ -307,8 +312,9 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
         fft_reg<T, L, -1>(v[q]);
         __builtin_amdgcn_sched_barrier(0);
       }
-      if constexpr (LAST && PF == 1) {
-        prefetch(nv);
+      if constexpr (LAST && (PF == 1 || PF == 3)) {  // PF 3: the first half only
+        if constexpr (PF == 1) prefetch(nv, X0{}, XL{});
+        else prefetch(nv, X0{}, XH{});
         __builtin_amdgcn_sched_barrier(0);
       }
       const uint32_t* b = bl[sl];
@@ -365,8 +371,9 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
         }
         __syncthreads();  // the planes are free for the next round, slice or volume
       }
-      if constexpr (LAST && PF == 0) {
-        prefetch(nv);
+      if constexpr (LAST && (PF == 0 || PF == 3)) {  // PF 3: the second half
+        if constexpr (PF == 0) prefetch(nv, X0{}, XL{});
+        else prefetch(nv, XH{}, XL{});
         __builtin_amdgcn_sched_barrier(0);
       }
       if constexpr (!LAST) {
@@ -391,7 +398,7 @@ __global__ __launch_bounds__(L * L / PPT) void k_slice1p(SliceArgs a) {
         bstore<C>(v[q][x], rv, yz * (uint32_t)sizeof(C), (uint32_t)(x * P * sizeof(C)));
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (PF == 2) {  // after the first pencil is stored, into its registers
-        if (q == 0) prefetch(nv);
+        if (q == 0) prefetch(nv, X0{}, XL{});
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -431,12 +438,12 @@ template <typename T, int L, int PPT>
 static hipError_t launch_s1(const SliceArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nvol), block(L * L / PPT);
   // the k-space split with the plain field: the persistent pipelined kernel when the caller
-  // gives a workgroup count (prefetch point: before the yz rounds, or after the first pencil's
-  // store where the registers are short)
+  // gives a workgroup count (prefetch point: before the yz rounds, split around them at
+  // L = 32, or after the first pencil's store where the registers are short)
   if (a.grid > 0 && a.pre.kind == MUL_TABLE && a.mid.kind == MUL_FIELD &&
       a.post.kind == MUL_NONE && !a.gather_mid &&
       (a.mid.table == nullptr || slice1p_trap_ok(sizeof(T) == 8 ? NRC_PREC_F64 : NRC_PREC_F32, L)))
-    return launch_s1p<T, L, PPT, (sizeof(T) == 4 && L == 24) ? 2 : 1>(a, s);
+    return launch_s1p<T, L, PPT, (sizeof(T) == 4 && L == 24) ? 2 : (sizeof(T) == 4 && L == 32) ? 3 : 1>(a, s);
   if (a.nslices != 1) return hipErrorInvalidValue;  // fused slices: persistent kernel only
   // k-space split: kinetic factor, field (or smeared field), measured beforehand; x-space
   // split: kappa/V in the middle with the measurement there, the field last
