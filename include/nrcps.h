@@ -28,6 +28,7 @@ extern "C" {
 #define NRC_ERR_HIP (-3)         /* HIP runtime failure */
 #define NRC_ERR_NOMEM (-4)       /* device allocation failed */
 #define NRC_ERR_SINGULAR (-5)    /* an all-zero matrix row (ludcmp.C:25-28 exits there) */
+#define NRC_ERR_BATCHED (-6)     /* the ensemble belongs to a batch (nrc_batch_create) */
 
 #define NRC_PREC_F32 0
 #define NRC_PREC_F64 1

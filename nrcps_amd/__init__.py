@@ -382,9 +382,10 @@ class Ensemble:
         return p.value, ms.value
 
     def close(self):
-        # an ensemble lives on its context's device; after nrc_finalize it is gone with it
+        # an ensemble lives on its context's device; after nrc_finalize it is gone with it.
+        # While it belongs to a batch the destroy is refused (NrcError) and the handle kept.
         if getattr(self, "h", None) and getattr(self.ctx, "h", None):
-            self.lib.nrc_ensemble_destroy(self.h)
+            _check(self.lib.nrc_ensemble_destroy(self.h), "ensemble_destroy")
         self.h = None
 
     def __del__(self):

@@ -66,6 +66,7 @@ const char* nrc_error_string(int code) {
     case NRC_ERR_HIP: return "HIP runtime error";
     case NRC_ERR_NOMEM: return "device allocation failed";
     case NRC_ERR_SINGULAR: return "singular matrix (all-zero row)";
+    case NRC_ERR_BATCHED: return "ensemble belongs to a batch";
     default: return "unknown error";
   }
 }
@@ -402,6 +403,11 @@ struct nrc_ensemble {
   std::vector<int> ev_kind;
 };
 
+// A batch has captured (or will capture) this ensemble's kernels with its buffers and path
+// baked in: while it belongs to a batch, nothing may reallocate its buffers, change its path,
+// layout or tables, or free it (nrc_batch_destroy hands it back).
+static bool ens_in_batch(const nrc_ensemble* e) { return e->stream != e->ctx->stream; }
+
 static void ens_free(nrc_ensemble* e) {
   void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->pw, e->popp, e->Mp,
                   e->pair_work, e->sinks_k, e->tab_yz, e->smear,
@@ -655,6 +661,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
 
 extern "C" int nrc_ensemble_destroy(nrc_ensemble* e) {
   if (!e) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   hipSetDevice(e->ctx->device);
   hipStreamSynchronize(e->stream);
   ens_free(e);
@@ -663,6 +670,7 @@ extern "C" int nrc_ensemble_destroy(nrc_ensemble* e) {
 
 extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
   if (!e || path < NRC_PATH_AUTO || path > NRC_PATH_SINGLE_PASS) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   if (path == NRC_PATH_SINGLE_PASS && !nrc::slice1_supported(e->d.prec, e->d.L))
     return NRC_ERR_UNSUPPORTED;
   e->path = path;
@@ -1194,12 +1202,14 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
 extern "C" int nrc_ensemble_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, int T,
                                 double* logabs, double* phase, double* argsum) {
   if (!e) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   return ens_run(e, seed, config0, nullptr, T, logabs, phase, argsum);
 }
 
 extern "C" int nrc_ensemble_run_fields(nrc_ensemble* e, const uint32_t* words, int T,
                                        double* logabs, double* phase, double* argsum) {
   if (!e || !words) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   return ens_run(e, 0, 0, words, T, logabs, phase, argsum);
 }
 
@@ -1268,7 +1278,7 @@ extern "C" int nrc_ensemble_orbitals(nrc_ensemble* e, int c, void* out) {
 // forward / backward pair with its 1/V is the multiply alone.
 extern "C" int nrc_ensemble_set_trap(nrc_ensemble* e, const double* q) {
   if (!e) return NRC_ERR_ARG;
-  if (e->stream != e->ctx->stream) return NRC_ERR_ARG;  // not while in a batch
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   if (q && e->d.split != NRC_SPLIT_KSPACE) return NRC_ERR_UNSUPPORTED;
   const long V = e->V;
   for (long i = 0; q && i < V; ++i)
@@ -1300,7 +1310,7 @@ extern "C" int nrc_ensemble_volumes(nrc_ensemble* e, int* nvo) {
 
 extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   if (!e || (conj != 0 && conj != 1)) return NRC_ERR_ARG;
-  if (e->stream != e->ctx->stream) return NRC_ERR_ARG;  // not while in a batch
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   if (conj && (!e->sym_ok || (e->pair && !e->pair_sym_ok) || e->sinks))
     return NRC_ERR_UNSUPPORTED;
   hipSetDevice(e->ctx->device);
@@ -1410,6 +1420,7 @@ extern "C" int nrc_ensemble_results(nrc_ensemble* e, double* logabs, double* pha
 extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
                                  double* ms_per_slice) {
   if (!e || T_probe < 1) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   std::vector<int> cand = {NRC_PATH_TWO_PASS};
   if (nrc::slice1_supported(e->d.prec, e->d.L)) cand.push_back(NRC_PATH_SINGLE_PASS);
   if (e->sinks) cand = {NRC_PATH_SINGLE_PASS};
@@ -1471,8 +1482,8 @@ extern "C" int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, i
                                 nrc_batch** out) {
   if (!ctx || !groups || n_groups < 1 || !out) return NRC_ERR_ARG;
   for (int i = 0; i < n_groups; ++i) {
-    if (!groups[i] || groups[i]->ctx != ctx || groups[i]->stream != ctx->stream)
-      return NRC_ERR_ARG;  // an ensemble belongs to one batch at a time
+    if (!groups[i] || groups[i]->ctx != ctx) return NRC_ERR_ARG;
+    if (ens_in_batch(groups[i])) return NRC_ERR_BATCHED;  // one batch at a time
     for (int j = 0; j < i; ++j)
       if (groups[j] == groups[i]) return NRC_ERR_ARG;
   }
@@ -1606,6 +1617,7 @@ static bool ens_step_ok(const nrc_ensemble* e) {
 
 extern "C" int nrc_ensemble_begin(nrc_ensemble* e, int T) {
   if (!e || T < 1) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
   NRC_TRY(ens_prepare(e, nullptr, T));
   e->last_T = T;

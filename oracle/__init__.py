@@ -118,6 +118,9 @@ def lib() -> C.CDLL:
         L.orc_sho_source.restype = None
         L.orc_time_threads.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_int]
         L.orc_time_threads.restype = C.c_double
+        L.orc_time_pool.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_int, C.c_int, C.c_int,
+                                    _dp]
+        L.orc_time_pool.restype = C.c_int
         _lib = L
     return _lib
 
@@ -418,3 +421,23 @@ def time_threads(L, N, slices, moms, kin, A=0.4, split=SPLIT_KSPACE, nthreads=1,
     d.A = A
     d.field_kind, d.seed, d.config = FIELD_PHILOX, seed, config
     return lib().orc_time_threads(C.byref(d), nthreads, slices)
+
+
+def time_pool(L, N, configs, slices, moms, kin, A=0.4, split=SPLIT_KSPACE, nthreads=1, reps=3,
+              seed=7012, config=0):
+    """The CPU-baseline protocol (orc_time_pool): `nthreads` streams, a one-slice warm-up
+    each, then `reps` barrier-synchronised repetitions of `configs` configurations x `slices`
+    slices per stream.  Returns the wall seconds of every repetition."""
+    moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)
+    kin = np.ascontiguousarray(kin, dtype=np.float64)
+    d = RunDesc()
+    d.L, d.N, d.T, d.split, d.minors = L, N, slices, split, 0
+    d.moms = moms.ctypes.data
+    d.kin = kin.ctypes.data
+    d.A = A
+    d.field_kind, d.seed, d.config = FIELD_PHILOX, seed, config
+    secs = np.zeros(reps)
+    rc = lib().orc_time_pool(C.byref(d), nthreads, configs, slices, reps, secs)
+    if rc != 0:
+        raise RuntimeError("orc_time_pool failed")
+    return secs

@@ -1004,6 +1004,73 @@ static void* orc_thread_main(void* p) {
   return NULL;
 }
 
+/* The CPU baseline's timing protocol: nthreads independent streams (one per core), each
+ * first runs a warm-up configuration of one slice, then `reps` repetitions of `configs`
+ * configurations x `slices` slices; every repetition starts on a barrier across the threads
+ * and its wall time is taken from the first thread's start to the last thread's end.
+ * secs[r] receives repetition r's wall seconds.  Returns 0, or -1 on failure. */
+typedef struct {
+  orc_run_desc d;
+  int configs, reps, rc;
+  pthread_barrier_t* bar;
+  struct timespec* t_start;  /* per repetition, written by thread 0 after the barrier */
+} orc_pool_arg;
+
+static void* orc_pool_main(void* p) {
+  orc_pool_arg* a = (orc_pool_arg*)p;
+  orc_run_desc w = a->d;
+  w.T = 1;
+  w.config = a->d.config + 0x40000000u;
+  a->rc |= orc_run_config(&w, NULL, NULL, NULL, NULL, NULL);
+  for (int r = 0; r < a->reps; ++r) {
+    pthread_barrier_wait(a->bar);
+    if (a->t_start) clock_gettime(CLOCK_MONOTONIC, &a->t_start[r]);
+    for (int c = 0; c < a->configs; ++c) {
+      orc_run_desc d = a->d;
+      d.config = a->d.config + (uint32_t)(r * a->configs + c) * 4096u;
+      a->rc |= orc_run_config(&d, NULL, NULL, NULL, NULL, NULL);
+    }
+    pthread_barrier_wait(a->bar);
+  }
+  return NULL;
+}
+
+int orc_time_pool(const orc_run_desc* d, int nthreads, int configs, int slices, int reps,
+                  double* secs) {
+  if (nthreads < 1 || configs < 1 || slices < 1 || reps < 1 || d->field_kind == ORC_FIELD_RANLXS)
+    return -1;
+  orc_pool_arg* args = (orc_pool_arg*)calloc((size_t)nthreads, sizeof(orc_pool_arg));
+  pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+  struct timespec* t0 = (struct timespec*)calloc((size_t)reps, sizeof(struct timespec));
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, (unsigned)nthreads + 1);
+  for (int i = 0; i < nthreads; ++i) {
+    args[i].d = *d;
+    args[i].d.T = slices;
+    args[i].d.config = d->config + (uint32_t)i;
+    args[i].configs = configs;
+    args[i].reps = reps;
+    args[i].bar = &bar;
+    args[i].t_start = i == 0 ? t0 : NULL;
+  }
+  for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, orc_pool_main, &args[i]);
+  for (int r = 0; r < reps; ++r) {
+    pthread_barrier_wait(&bar); /* every thread warmed up / done with repetition r - 1 */
+    pthread_barrier_wait(&bar); /* every thread done with repetition r */
+    struct timespec t1;
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    secs[r] = (double)(t1.tv_sec - t0[r].tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0[r].tv_nsec);
+  }
+  for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+  int bad = 0;
+  for (int i = 0; i < nthreads; ++i) bad |= args[i].rc;
+  pthread_barrier_destroy(&bar);
+  free(args);
+  free(th);
+  free(t0);
+  return bad ? -1 : 0;
+}
+
 double orc_time_threads(const orc_run_desc* d, int nthreads, int slices) {
   if (nthreads < 1 || d->field_kind == ORC_FIELD_RANLXS) return -1.0;
   orc_thread_arg* args = (orc_thread_arg*)calloc((size_t)nthreads, sizeof(orc_thread_arg));

@@ -195,6 +195,12 @@ int orc_exact_c1(int L, int T, const double* ka, const double* kb, double A, dou
  * of configuration `d` (config index offset per thread), in seconds. */
 double orc_time_threads(const orc_run_desc* d, int nthreads, int slices);
 
+/* The CPU-baseline protocol: `nthreads` streams, each warmed up by one slice of its own
+ * configuration, then `reps` barrier-synchronised repetitions of `configs` configurations x
+ * `slices` slices; secs[r] = wall seconds of repetition r.  Returns 0, -1 on failure. */
+int orc_time_pool(const orc_run_desc* d, int nthreads, int configs, int slices, int reps,
+                  double* secs);
+
 #ifdef __cplusplus
 }
 #endif

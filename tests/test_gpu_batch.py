@@ -141,3 +141,33 @@ def test_batch_conj_partners_match_oracle_and_alone(ctx, orc):
     with pytest.raises(na.NrcError):
         groups[0].set_symmetry(False)
     b.close()
+
+
+def test_batched_group_refuses_reallocating_calls(ctx, orc):
+    """While an ensemble belongs to a batch its captured kernels hold its buffers and path, so
+    every call that could reallocate them, change the path or layout, or free the ensemble is
+    refused; after the batch is destroyed the same calls work and the batch's graph is gone."""
+    import nrcps_amd as na
+
+    print("[gpu] refused calls on a batched group", flush=True)
+    g = _groups(ctx, orc, [(8, 7, 2, 1), (12, 1, 2, 0)])
+    T, seed = 4, 5
+    b = na.Batch(ctx, g)
+    b.run(seed, [0, 0], T, graph=True)
+    words = np.zeros((2 * T, 2, 16), np.uint32)
+    for call in (lambda: g[0].run(seed, 0, 2 * T),  # larger T: the outputs would be reallocated
+                 lambda: g[0].run_fields(words, 2 * T),
+                 lambda: g[0].tune(2),
+                 lambda: g[0].set_path(na.PATH_TWO_PASS),
+                 lambda: g[0].set_symmetry(True),
+                 lambda: g[0].set_trap(np.ones(8 ** 3)),
+                 lambda: g[0].close()):
+        with pytest.raises(na.NrcError, match="batch"):
+            call()
+    b.run(seed, [0, 0], T, graph=True)  # the kept graph is still valid
+    la, _, _ = g[0].results(T)
+    b.close()
+    la2, _, _ = g[0].run(seed, 0, T)
+    assert np.array_equal(la, la2)
+    g[0].set_path(na.PATH_TWO_PASS)
+    g[0].close()

@@ -146,9 +146,11 @@ def test_ref_abs_bit_identical_to_libm(ctx):
 
 
 @pytest.mark.parametrize("n", [2, 3, 10, 33, 64])
-def test_ludcmp_bit_exact_vs_reference(ctx, orc, n):
-    """Batched ludcmp reproduces the reference's factors, pivots and parity bit for bit,
-    including the row-maximum ties that are common for small matrices."""
+def test_ludcmp_bit_exact_vs_oracle(ctx, orc, n):
+    """Batched ludcmp reproduces the oracle's factors, pivots and parity bit for bit,
+    including the row-maximum ties that are common for small matrices.  The oracle's LU is
+    pinned bit for bit to the reference's own ludcmp.C on the CPU (test_oracle.py), so the
+    reference build never travels with the GPU tests."""
     rng = np.random.default_rng(100 + n)
     ncfg = 64
     Ms = _rand_c(rng, (ncfg, n, n))
@@ -162,9 +164,6 @@ def test_ludcmp_bit_exact_vs_reference(ctx, orc, n):
         assert np.array_equal(indx[c], rindx), c
         assert d[c] == rd
         assert np.array_equal(lu[c], rlu), (c, np.abs(lu[c] - rlu).max())
-        ref = orc.ref_ludcmp(Ms[c])
-        if ref is not None:
-            assert np.array_equal(ref[0], lu[c]) and np.array_equal(ref[1], indx[c])
 
 
 def test_lu_singular_row_status(ctx):

@@ -278,6 +278,24 @@ def test_ludcmp_reference_known_answer(orc):
         assert np.array_equal(rr[0], lu) and np.array_equal(rr[1], indx) and rr[2] == d
 
 
+@pytest.mark.parametrize("n", [2, 3, 10, 33, 64])
+def test_ludcmp_oracle_equals_reference_build_on_gpu_test_inputs(orc, n):
+    """The matrices of the GPU LU test (same seed, same tie cases): the oracle's factors,
+    pivots and parity are the reference ludcmp.C's bit for bit, so the GPU test compares with
+    the oracle alone and the reference build stays on this machine."""
+    if not os.path.exists(orc.REF_LUDCMP_PATH):
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    rng = np.random.default_rng(100 + n)
+    Ms = rng.standard_normal((64, n, n)) + 1j * rng.standard_normal((64, n, n))
+    if n == 10:
+        Ms[0] = np.array([[1.0 / (s + t + 2.0) for t in range(n)] for s in range(n)])
+        Ms[1] = np.eye(n)
+    for c in range(64):
+        lu, indx, d = orc.ludcmp(Ms[c])
+        rr = orc.ref_ludcmp(Ms[c])
+        assert np.array_equal(rr[0], lu) and np.array_equal(rr[1], indx) and rr[2] == d, c
+
+
 def test_ludcmp_random_vs_lapack_and_reference_build(orc):
     rng = np.random.default_rng(11)
     for n in [1, 2, 3, 7, 19, 33, 48]:
