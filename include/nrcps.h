@@ -198,6 +198,18 @@ int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
  * noarch/hamiltonian_evolve.C:69-93) is q = (V p)^2 with p Potential's table, since the three
  * x-space multiplies commute.  Not while the ensemble is in a batch. */
 int nrc_ensemble_set_trap(nrc_ensemble* ens, const double* q);
+/* Arithmetic of FP32 ensembles (k-space split, plain or trapped-free field, plane-wave sinks,
+ * single-pass path).  NRC_ACC_FP32: every transform and multiply in FP32 (compensated
+ * twiddles and factors).  NRC_ACC_X64 (the default): the state stays FP32 in HBM, registers
+ * and LDS, and so do the y and z transforms and the field multiply, but per x-pencil the
+ * chain x-DFT(+1) -> sink gather -> kinetic factor -> x-DFT(-1), where FP32 rounding lands
+ * on the sink momenta with full weight, runs on FP64 registers rounded once; between
+ * launches the state is held with its last x-DFT(+1) pending.  Other ensembles ignore the
+ * setting.  No counterpart in the reference (one Float type everywhere, config.h:19-48).
+ * Not while the ensemble is in a batch. */
+#define NRC_ACC_FP32 0
+#define NRC_ACC_X64 1
+int nrc_ensemble_set_accuracy(nrc_ensemble* ens, int mode);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
 int nrc_ensemble_volumes(nrc_ensemble* ens, int* volumes_per_config);
 /* Per-kernel timing with HIP events on the ensemble's stream (enable before a run).
@@ -235,6 +247,7 @@ int nrc_batch_destroy(nrc_batch* batch);
  *      measurement at t (Lattice::Refresh, lattice.C:60-109, replaced by the caller's
  *      words_dev[c][ceil(V/32)], device memory).  k-space split, single-pass path, on the
  *      context stream (not while the ensemble is in a batch).
+ * Stepping runs the plain FP32 kernels for FP32 ensembles (NRC_ACC_FP32 arithmetic).
  * nrc_ensemble_begin: plane-wave sources and result rows for T slices.
  * nrc_ensemble_measure: overlaps and LU at t into result row t.
  * nrc_ensemble_evolve: slice t with the given fields.

@@ -20,6 +20,7 @@ PREC_F32, PREC_F64 = 0, 1
 SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3 = 0, 1, 2
 PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
+ACC_FP32, ACC_X64 = 0, 1
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
 
 # every symbol include/nrcps.h declares
@@ -31,6 +32,7 @@ EXPORTS = [
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
+    "nrc_ensemble_set_accuracy",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -87,7 +89,7 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_run_fields": [vp, vp, i, dp, dp, dp],
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
         "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
-        "nrc_ensemble_set_trap": [vp, vp],
+        "nrc_ensemble_set_trap": [vp, vp], "nrc_ensemble_set_accuracy": [vp, i],
         "nrc_scale_const": [vp, vp, C.c_long, C.c_double, i, i],
         "nrc_norms": [vp, vp, i, C.c_long, i, vp],
         "nrc_ensemble_profile": [vp, i],
@@ -336,6 +338,10 @@ class Ensemble:
     def set_path(self, path: int):
         """PATH_AUTO, PATH_TWO_PASS or PATH_SINGLE_PASS (nrc_ensemble_set_path)."""
         _check(self.lib.nrc_ensemble_set_path(self.h, path), "ensemble_set_path")
+
+    def set_accuracy(self, mode: int):
+        """ACC_FP32 or ACC_X64 (the default) for FP32 ensembles (nrc_ensemble_set_accuracy)."""
+        _check(self.lib.nrc_ensemble_set_accuracy(self.h, mode), "ensemble_set_accuracy")
 
     def set_symmetry(self, conj: bool):
         """Conjugate partners (nrc_ensemble_set_symmetry): evolve one orbital of each (k, -k)

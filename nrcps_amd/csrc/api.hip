@@ -362,7 +362,7 @@ struct nrc_ensemble {
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
   size_t words_cap = 0;
   double2* M = nullptr;      // [C][N][N]
-  double2* Mx = nullptr;     // [SLICE_FUSE_MAX - 1][C][N][N]: sinks gathered between fused slices
+  double2* Mx = nullptr;     // [SLICE_FUSE_MAX][C][N][N]: sinks gathered inside fused launches
   double* o_la = nullptr;    // [T][C][nm]
   double2* o_ph = nullptr;
   double* o_as = nullptr;
@@ -395,6 +395,9 @@ struct nrc_ensemble {
   double* trap_q = nullptr;  // the trap q per site (FP64), for the smeared field
   double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
+  int acc = NRC_ACC_X64;         // FP32 ensembles: nrc_ensemble_set_accuracy
+  double* tab_x64 = nullptr;     // FP32 k-space split: K(1/2) and K as FP64 tables (x-chain)
+  double* tab_k64 = nullptr;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
   bool profile = false;
   double prof_ms[5] = {0, 0, 0, 0, 0};
@@ -414,7 +417,8 @@ static void ens_free(nrc_ensemble* e) {
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
-                  e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q};
+                  e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q,
+                  e->tab_x64, e->tab_k64};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
@@ -517,6 +521,10 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     ENS_TRY(ens_upload_pair(e, &e->tab_x, e->khalf));
     for (long i = 0; i < V; ++i) h[i] = e->khalf[i] * e->khalf[i];
     ENS_TRY(ens_upload_pair(e, &e->tab_k, h));
+    if (d->prec == NRC_PREC_F32) {  // the x-chain's FP64 factors
+      ENS_TRY(upload_table<double>(e, (void**)&e->tab_k64, h));
+      ENS_TRY(upload_table<double>(e, (void**)&e->tab_x64, e->khalf));
+    }
     if (d->smear) {
       h.assign(d->smear, d->smear + V);
       ENS_TRY(ens_upload_pair(e, &e->smear, h));
@@ -530,7 +538,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     ENS_TRY(ens_upload_pair(e, &e->tab_yz, h));
   }
   ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
-  ENS_TRY(dev_alloc(&e->Mx, (size_t)(nrc::SLICE_FUSE_MAX - 1) * C * N * N));
+  ENS_TRY(dev_alloc(&e->Mx, (size_t)nrc::SLICE_FUSE_MAX * C * N * N));
   if (d->sinks) {  // dense sinks in the ensemble precision
     if (d->prec == NRC_PREC_F64) {
       ENS_TRY(hipMalloc(&e->sinks, (size_t)N * V * 16) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
@@ -674,6 +682,13 @@ extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
   if (path == NRC_PATH_SINGLE_PASS && !nrc::slice1_supported(e->d.prec, e->d.L))
     return NRC_ERR_UNSUPPORTED;
   e->path = path;
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_set_accuracy(nrc_ensemble* e, int mode) {
+  if (!e || (mode != NRC_ACC_FP32 && mode != NRC_ACC_X64)) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
+  e->acc = mode;
   return NRC_OK;
 }
 
@@ -1039,6 +1054,57 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
   return NRC_OK;
 }
 
+// slices t .. t + S - 1 in one launch of the x-chain kernel (slice_ext.hip), with their
+// measurements gathered inside it into Mx and one LU launch for them: the launch starts from
+// the sources (t = 0) or the state the previous launch left with its x-DFT(+1) pending, and
+// the run's last launch stores the folded k-space state
+static bool ens_xchain(const nrc_ensemble* e, bool single) {
+  return single && e->d.prec == NRC_PREC_F32 && e->acc == NRC_ACC_X64 &&
+         e->d.split == NRC_SPLIT_KSPACE && !e->smear && !e->sinks && !e->pair &&
+         !e->trap_tab && nrc::slicex_supported(e->d.L);
+}
+
+static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, size_t stride,
+                        bool last) {
+  const int N = e->d.n_orb;
+  const long C = e->d.n_configs, V = e->V;
+  const size_t per_t = (size_t)C * e->nm;
+  nrc::SliceXArgs a;
+  a.psi = e->psi;
+  a.nvol = C * e->nvo;
+  a.norb = e->nvo;
+  a.L = e->d.L;
+  a.grid = e->ctx->n_cu;
+  a.nslices = S;
+  a.start_k = t == 0;
+  a.store_k = last;
+  a.tab_x = e->tab_x64;
+  a.tab_k = e->tab_k64;
+  a.bits = words;
+  a.nw = e->nw;
+  a.bits_stride = (long)stride;
+  a.a0 = 1.0 / (double)V;
+  a.a1 = e->d.A / (double)V;
+  a.g = ens_gather_spec(e);
+  a.g.M = e->Mx;
+  a.g.sink_scale = e->sink_k;  // not on the sources (t = 0): the kernel drops it there
+  a.m_stride = C * N * N;
+  {
+    EvScope sc(e, 0);
+    NRC_HIP(nrc::launch_slicex(a, e->stream));
+  }
+  e->folded = true;
+  EvScope sc(e, 2);
+  nrc::LuOpts o;
+  o.lsrc = e->lsrc;
+  o.lsrc_period = C;
+  const size_t row = (size_t)t * per_t;  // result rows t .. t + S - 1, contiguous
+  NRC_HIP(nrc::launch_lu_det(e->Mx, N, N, (int)(C * S), e->d.minors, e->o_la + row,
+                             e->o_ph + row, e->o_as + row, e->o_st + row, nullptr, nullptr,
+                             nullptr, e->stream, o));
+  return NRC_OK;
+}
+
 // allocations and host-to-device copies of a run of T slices (caller fields: words_host)
 static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (T < 1) return NRC_ERR_ARG;
@@ -1088,6 +1154,21 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   // single pass: every slice is one kernel over the orbitals (slice_fused.hip)
   // dense sinks and pair matrices read the stored state
   if ((e->sinks || e->pair) && !single) return NRC_ERR_UNSUPPORTED;
+  if (ens_xchain(e, single)) {  // FP32 state, FP64 x-chain, measurements inside the launches
+    for (int t = 0; t < T;) {
+      const int S = std::min(nrc::SLICE_FUSE_MAX, T - t);
+      if (!host_words) {
+        EvScope sc(e, 3);
+        for (int k = 0; k < S; ++k)
+          NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
+                                      e->words + (size_t)k * slice_words, s));
+      }
+      NRC_TRY(ens_evolve_x(e, t, S, host_words ? field_of(t) : e->words, slice_words,
+                           t + S == T));
+      t += S;
+    }
+    return NRC_OK;
+  }
   const bool measure_first = ref || e->sinks;
   // k-space split with the plain field: up to SLICE_FUSE_MAX slices per launch of the
   // persistent kernel, the measurements between them gathered inside it

@@ -94,6 +94,34 @@ struct SliceArgs {
   long m_stride = 0;
 };
 constexpr int SLICE_FUSE_MAX = 4;  // slices per launch of the persistent slice kernel
+
+// The k-space split slice loop with FP32 state and the x-chain in FP64 (slice_ext.hip):
+// `nslices` slices per launch of the persistent kernel.  The volumes hold the k-space
+// sources (start_k) or the state of the previous launch in (x, k_y, k_z) coordinates with its
+// last x-DFT(+1) pending; slice s of the launch gathers its measurement (the sinks of
+// g, g.M advanced by s * m_stride; sink_scale except on the sources), multiplies by the
+// FP64 table tab_x = K(1/2) (sources) or tab_k = K, and evolves with the field words at
+// bits + s * bits_stride: (a0 + a1 phi) in x-space.  store_k: finish with the x-DFT(+1) and
+// store the folded k-space state.
+struct SliceXArgs {
+  void* psi = nullptr;  // float2 volumes
+  long nvol = 0;
+  int norb = 1;
+  int L = 0;
+  int grid = 0;  // CUs
+  int nslices = 1;
+  int start_k = 0, store_k = 0;
+  const double* tab_x = nullptr;
+  const double* tab_k = nullptr;
+  const uint32_t* bits = nullptr;
+  int nw = 0;
+  long bits_stride = 0;
+  double a0 = 0.0, a1 = 0.0;
+  GatherSpec g;
+  long m_stride = 0;
+};
+bool slicex_supported(int L);
+hipError_t launch_slicex(const SliceXArgs& a, hipStream_t s);
 bool slice1_supported(int prec, int L);
 bool slice1p_trap_ok(int prec, int L);  // the persistent kernel takes a trap at this shape
 hipError_t launch_slice1(int prec, const SliceArgs& a, hipStream_t s);

@@ -389,7 +389,9 @@ def test_ensemble_f32_config3_shape(ctx, orc, split, path):
         r2 = np.exp(2 * ref["logabs"][:, 0]) * ref["phase"][:, 0] ** 2
         rel = np.abs(d2 - r2) / np.abs(r2)
         print(f"[gpu] FP32 det^2 rel err by t (config {c}): max {rel.max():.2e}", flush=True)
-        assert rel.max() < 1e-4
+        # single pass, k-space split: FP32 state with the FP64 x-chain (~7e-7 measured); the
+        # two-pass and x-space-split kernels are plain FP32 (~4e-6 and ~6e-6)
+        assert rel.max() < (2e-6 if split == 0 and path == 2 else 2e-5)
 
 
 @pytest.mark.parametrize("path", PATHS)

@@ -139,14 +139,18 @@ def test_training_learns_a_bias(ctx, orc):
 
 def test_stepping_reproduces_run(ctx, orc):
     """nrc_ensemble_begin / measure / evolve with the uniform sampler's Philox words gives
-    exactly nrc_ensemble_run."""
+    exactly nrc_ensemble_run in plain FP32 (the stepping kernels), and agrees with the
+    default FP64 x-chain run to FP32 evolution error."""
     import nrcps_amd as na
 
     L, N, C, T, seed = 8, 7, 3, 6, 21
     moms = orc.lowest_momenta(L, N)
     kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
     a = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    lx, _, _ = a.run(seed, 0, T)
+    a.set_accuracy(na.ACC_FP32)
     la, ph, _ = a.run(seed, 0, T)
+    assert np.abs(la - lx).max() < 2e-6
     b = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
     b.begin(T)
     words = ctx.array((C, L ** 3 // 32), np.uint32)
