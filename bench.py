@@ -57,22 +57,51 @@ def _lowest(L, N):
     return np.array([pts[i] for i in order[:N]], dtype=np.int32)
 
 
-def cpu_baseline(L, N, tau, A, slices, threads):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(L, N, tau, A, configs, slices, threads, reps=3):
     """The FP64 oracle (CPU restatement of the reference algorithm: batched FFTs, N^2
-    full-length overlaps, NR LU) on `threads` host cores, one configuration stream each."""
+    full-length overlaps, NR LU, every orbital evolved) on `threads` host cores, one
+    configuration stream per core as the reference's MPI ranks run (random.C:88): a one-slice
+    warm-up per stream, then `reps` barrier-synchronised repetitions of `configs`
+    configurations x `slices` slices per stream; the best repetition is the rate."""
     import oracle
 
     oracle.lib()
     moms = _lowest(L, N)
     kin = _tables(L, tau, 1.0)
-    secs = oracle.time_threads(L, N, slices, moms, kin, A, nthreads=threads, config=1 << 20)
-    if secs <= 0:
-        return None
-    return {"value": threads * slices / secs, "unit": "configs*slices/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{threads} threads x 1 config x {slices} slices at L={L}, N={N}, FP64; "
-                      f"{secs:.2f} s wall",
-            "fft_calibration": fft_calibration(L, N)}
+    secs = oracle.time_pool(L, N, configs, slices, moms, kin, A, nthreads=threads, reps=reps,
+                            config=1 << 20)
+    best = float(secs.min())
+    cal = fft_calibration(L, N)
+    value = threads * configs * slices / best
+    visible = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = visible
+    return {"value": value, "unit": "configs*slices/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} streams x {configs} configs x {slices} slices at L={L}, N={N}, "
+                      f"FP64, every orbital evolved; 1-slice warm-up, best of {reps} "
+                      f"({', '.join(f'{x:.1f}' for x in secs)} s)",
+            "cpu_model": _cpu_model(), "cpus_visible": visible, "cpus_in_affinity": affinity,
+            "cores_policy": ("one stream per CPU the process may use"
+                             if threads >= affinity else
+                             f"one stream per core of the GPU box's CPU share for one GPU "
+                             f"({threads} of {affinity} CPUs in the affinity mask)"),
+            "per_core_value": value / threads,
+            "value_if_fft_at_numpy_speed": value * cal["baseline_scale_with_numpy_fft"],
+            "value_all_visible_cpus_linear": value / threads * visible,
+            "fft_calibration": cal}
 
 
 def fft_calibration(L, N, reps=3):
@@ -106,10 +135,11 @@ def fft_calibration(L, N, reps=3):
             "note": "times include the copies of each call; one core"}
 
 
-def measured_counters(L, N, C, per_launch, nvo):
-    """The committed rocprofv3 counter record of the slice kernel on this workload
-    (profiles/traffic.json: HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, and VALU wave
-    instructions per launch where measured), or None if none was taken on this shape."""
+def measured_counters(kernel, L, N, C, per_launch, nvo):
+    """The committed rocprofv3 counter record of `kernel` on this workload (profiles/
+    traffic.json: HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, VALU wave instructions
+    per launch), or None if none was taken on this shape.  These are stored figures from the
+    profile named in the record, not measured by this run."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -117,8 +147,9 @@ def measured_counters(L, N, C, per_launch, nvo):
     except (OSError, ValueError):
         return None
     for t in recs if isinstance(recs, list) else [recs]:
-        if (t.get("L"), t.get("N"), t.get("configs"), t.get("slices_per_launch"),
-                t.get("volumes_per_config")) == (L, N, C, per_launch, nvo):
+        if (t.get("kernel_id"), t.get("L"), t.get("N"), t.get("configs"),
+                t.get("slices_per_launch"), t.get("volumes_per_config")) == (
+                kernel, L, N, C, per_launch, nvo):
             return t
     return None
 
@@ -132,26 +163,32 @@ def correlator(la, ph):
     return m[:, 0], c
 
 
-def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,)):
+def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,), acc=1,
+                     ref64=None):
     """Per-t relative error of C(t) from the FP32 path against the FP64 path (every orbital
     evolved) on the same configurations (same Philox fields), and the largest log|det|
-    difference; one report per FP32 mode (conjugate partners off / on)."""
+    difference; one report per FP32 mode (conjugate partners off / on) in the arithmetic
+    `acc` (1: FP64 x-chain, 0: plain FP32).  ref64: the FP64 run's (logabs, phase), if taken."""
     moms = _lowest(L, N)
     kin = _tables(L, tau, 1.0)
-    e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=na.PREC_F64, split=na.SPLIT_KSPACE)
-    la64, ph64, _ = e.run(seed, 0, T)
-    e.close()
+    if ref64 is None:
+        e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=na.PREC_F64, split=na.SPLIT_KSPACE)
+        ref64 = e.run(seed, 0, T)[:2]
+        e.close()
+    la64, ph64 = ref64
     m64, c64 = correlator(la64, ph64)
     reports = []
     for conj in conj_modes:
         e = na.Ensemble(ctx, L, moms, kin, A, nconf, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
         e.set_symmetry(conj)
+        e.set_accuracy(acc)
         la32, ph32, _ = e.run(seed, 0, T)
         e.close()
         m32, c32 = correlator(la32, ph32)
         rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
         reports.append({
             "configs": nconf, "T": T, "conj_partners": bool(conj),
+            "arithmetic": ("FP32 state, FP64 x-chain and z-rows" if acc == 1 else "plain FP32"),
             "max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
             "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
             "rel_err_C_at_t": {str(t): float(rel[t]) for t in (0, T // 4, T // 2, 3 * T // 4,
@@ -200,11 +237,11 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
             t0 = time.perf_counter()
             b.run(seed, c0, T, graph=graph)
             ts.append(time.perf_counter() - t0)
-        out[graph] = (min(ts), b.capture_ms() if graph else 0.0)
+        out[graph] = (min(ts), b.capture_ms() if graph else 0.0, b.captures())
     b.close()
     for e in groups:
         e.close()
-    (tg, cap), (tn, _) = out[True], out[False]
+    (tg, cap, ncap), (tn, _, _) = out[True], out[False]
     return {"workload": f"{nconf} configurations, L uniform in {{16,24,32}}, N uniform in "
                         f"{{1,7,19,27,33}}, T={T}, tau={tau}, A={A}, FP32 evolution, FP64 LU",
             "configs": nconf, "T": T, "shape_groups": len(groups), "conj_partners": bool(conj),
@@ -212,7 +249,9 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
             "ms_per_run": 1e3 * tg, "graph": True,
             "capture_and_instantiate_ms": cap,
             "value_without_graph": nconf * T / tn,
-            "timing": "one graph launch per run, capture included only in the warm-up run",
+            "graph_captures": ncap,
+            "timing": "one graph launch per run; the graph is captured once, in the warm-up "
+                      "run, and the timed runs (other configuration ranges) reuse it",
             "per_shape": per_shape}
 
 
@@ -297,7 +336,7 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                          "output_bias": float(ps[8240])}}
 
 
-def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012):
+def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012, ref64=None):
     """The overlap matrix Phi^H Psi(t) as a batched complex GEMM on the matrix cores (dense
     sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16x3 products
     with FP32 accumulation, every orbital read once per measurement.  The sinks are the
@@ -314,7 +353,7 @@ def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012)
     e.run(seed, 0, 2)
     ctx.sync()
     t0 = time.perf_counter()
-    la_d, _, _ = e.run(seed, 0, T)
+    la_d, ph_d, _ = e.run(seed, 0, T)
     ctx.sync()
     dt = time.perf_counter() - t0
     e.profile(True)
@@ -325,6 +364,16 @@ def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012)
     g = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
     la_g, _, _ = g.run(seed, 0, T)
     g.close()
+    prec = None
+    if ref64 is not None and ref64[0].shape[:2] == (T, C):
+        m64, c64 = correlator(*ref64)
+        m32, c32 = correlator(la_d, ph_d)
+        rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+        prec = {"max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
+                "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
+                "reference": "FP64 path on the same fields (the precision report's)",
+                "arithmetic": "plain FP32 evolution (single-slice kernel, every orbital), "
+                              "split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU"}
     ms = t_ovl / max(n_ovl, 1)
     flops = 3 * 8.0 * N * N * V * C
     byts = 8.0 * N * V * C
@@ -337,6 +386,7 @@ def dense_overlap(ctx, na, L=32, N=33, T=16, C=1024, tau=0.05, A=0.4, seed=7012)
             "hbm": {"achieved_gbs": byts / (ms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
                     "frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "max_abs_diff_logdet_vs_gather": float(np.abs(la_d - la_g).max()),
+            "precision": prec,
             "timing": "HIP events around the overlap kernels of each measurement"}
 
 
@@ -406,6 +456,58 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
                                 "peak_gbs": HBM_PEAK_GBS}}
 
 
+def evolution_roofline(L, N, T, C, V, nvo, args, tx, nx, tyz, nyz, tlu, nlu, tf, nf, tg, ng):
+    """The roofline record of the evolution kernel of the headline run.  Algorithmic bytes per
+    configuration-slice: read + write the evolved FP32 volumes once plus phi as one byte per
+    site (B = nvo 16 V + V); achieved = B x configs x slices per launch / launch time (HIP
+    events on the ensemble's stream).  The kernel is bound by its VALU issue: the committed
+    counter record of the same kernel and shape (SQ_INSTS_VALU per launch) over this run's
+    launch time gives the issue rate against 614 G wave instructions/s (256 CUs x 4 SIMDs x
+    2.4 GHz / 4 cycles per wave64 instruction, FP32 and FP64 alike), and its FETCH_SIZE /
+    WRITE_SIZE the HBM bytes actually moved."""
+    b_evo = nvo * 16 * V + V
+    single = nyz == 0
+    per_launch = T / max(nx, 1)  # slices per launch of the evolution kernel
+    launch_ms = tx / max(nx, 1)
+    if single and args.acc == 1 and args.path != 1:
+        kid, kernel = "k_slicex", f"k_slicex ({per_launch:g} slices per launch, FP64 x-chain)"
+    elif single and per_launch > 1:
+        kid, kernel = "k_slice1p", f"k_slice1p ({per_launch:g} slices per launch)"
+    elif single:
+        kid, kernel = "k_slice1", "k_slice1 (one launch per slice)"
+    else:
+        kid, kernel = "k_xpass+k_yzpass", "k_xpass + k_yzpass (one slice)"
+    alg_bytes = b_evo * C * per_launch
+    slice_ms = (tx + tyz) / T
+    hbm_alg = b_evo * C / (slice_ms * 1e-3) / 1e9
+    cnt = measured_counters(kid, L, N, C, per_launch, nvo) if single else None
+    valu_peak = VALU_SIMDS * VALU_CLOCK_HZ / 4.0
+    rec = {"kernel": kernel,
+           "hbm_algorithmic": {"achieved_gbs": hbm_alg, "peak_gbs": HBM_PEAK_GBS,
+                               "frac": hbm_alg / HBM_PEAK_GBS},
+           "traffic": cnt["traffic_bytes_per_launch"] if cnt else None,
+           "traffic_source": (cnt["source"] + " (stored counter record, not measured by "
+                              "this run)") if cnt else None,
+           "algorithmic_bytes_per_launch": alg_bytes,
+           "avg_ms": {"evolution_per_slice": slice_ms, "evolution_launch": launch_ms,
+                      "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
+                      "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
+           "bytes_per_cs": b_evo, "volumes_per_config": nvo,
+           "timing": "HIP events on the ensemble's stream around every evolution launch"}
+    if cnt:
+        hbm_cnt = cnt["traffic_bytes_per_launch"] / (launch_ms * 1e-3) / 1e9
+        rec["hbm_counters"] = {"achieved_gbs": hbm_cnt, "frac": hbm_cnt / HBM_PEAK_GBS}
+    if cnt and cnt.get("valu_insts_per_launch"):
+        rate = cnt["valu_insts_per_launch"] / (launch_ms * 1e-3)
+        rec.update({"bound": "valu", "achieved": rate, "peak": valu_peak,
+                    "unit": "wave64 VALU instructions/s", "frac": rate / valu_peak,
+                    "valu_insts_per_launch": cnt["valu_insts_per_launch"]})
+    else:
+        rec.update({"bound": "hbm", "achieved": hbm_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm_alg / HBM_PEAK_GBS})
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,8 +517,13 @@ def main():
     ap.add_argument("--T", type=int, default=128)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--N", type=int, default=33)
-    ap.add_argument("--cpu-slices", type=int, default=16)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-configs", type=int, default=2,
+                    help="CPU baseline: configurations per stream and repetition")
+    ap.add_argument("--cpu-slices", type=int, default=128,
+                    help="CPU baseline: slices per configuration")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline: best of this many")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline streams (0: the GPU box's CPU share, 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven-shape measurement")
@@ -433,6 +540,8 @@ def main():
     ap.add_argument("--conj", type=int, default=1,
                     help="1: conjugate partners (one orbital of each (k, -k) pair evolved); "
                          "0: every orbital evolved, as the reference does")
+    ap.add_argument("--acc", type=int, default=1,
+                    help="FP32 arithmetic: 1 FP64 x-chain and z-rows (default), 0 plain FP32")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
@@ -453,6 +562,7 @@ def main():
     ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
     ens.set_path(args.path)
     ens.set_symmetry(bool(args.conj))
+    ens.set_accuracy(args.acc)
     seed = 7012
 
     for w in range(args.warmup):
@@ -479,122 +589,110 @@ def main():
     tf, nf = ens.kernel_time(3)
     tg, ng = ens.kernel_time(4)
     ens.profile(False)
-    # the evolution kernels of one slice of every configuration (one single-pass launch, or an
-    # X pass and a YZ pass); algorithmic bytes per configuration-slice: read + write the N FP32
-    # orbitals once plus phi as one byte per site
-    nvo = ens.volumes_per_config()
-    b_evo = nvo * 16 * V + V
-    slice_ms = (tx + tyz) / T
-    achieved = b_evo * C / (slice_ms * 1e-3) / 1e9
-    single = nyz == 0
-    per_launch = T / max(nx, 1)  # slices per launch of the evolution kernel
-    kernel = (f"k_slice1p ({per_launch:g} slices per launch)" if single and per_launch > 1
-              else "k_slice1 (one launch per slice)" if single
-              else "k_xpass + k_yzpass (one slice)")
-    cnt = measured_counters(L, N, C, per_launch, nvo) if single else None
-    valu = None
-    if cnt and cnt.get("valu_insts_per_launch"):
-        # wave64 FP32 VALU instructions (packed FP32 off) take 4 cycles of a 16-lane SIMD
-        peak = VALU_SIMDS * VALU_CLOCK_HZ / 4.0
-        rate = cnt["valu_insts_per_launch"] / (tx / max(nx, 1) * 1e-3)
-        valu = {"achieved_wave_insts_per_s": rate, "peak_wave_insts_per_s": peak,
-                "frac": rate / peak, "insts_per_launch": cnt["valu_insts_per_launch"],
-                "source": "SQ_INSTS_VALU per launch (profiles) / this run's launch time"}
-    roofline = {"bound": "hbm", "kernel": kernel,
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": cnt["traffic_bytes_per_launch"] if cnt else None,
-                "algorithmic_bytes_per_launch": b_evo * C * per_launch,
-                "avg_ms": {"evolution_per_slice": slice_ms, "evolution_launch": tx / max(nx, 1),
-                           "yzpass": tyz / max(nyz, 1), "lu": tlu / max(nlu, 1),
-                           "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
-                "bytes_per_cs": b_evo, "volumes_per_config": nvo, "valu": valu,
-                "timing": "HIP events on the ensemble's stream around every evolution launch"}
+    roofline = evolution_roofline(L, N, T, C, V, ens.volumes_per_config(), args, tx, nx, tyz,
+                                  nyz, tlu, nlu, tf, nf, tg, ng)
+    ens.close()
+    ens = None
 
     batched = None
     if not args.no_batched:
-        ens.close()
         batched = batched_uneven(ctx, na, conj=bool(args.conj))
         print(f"batched uneven shapes: {batched['value']:.0f} configs*slices/s",
               file=sys.stderr, flush=True)
 
     learned = None
     if not args.no_learned:
-        ens.close()
         learned = learned_proposal(ctx, na, conj=bool(args.conj))
         print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
 
     pair = None
     if not args.no_pair:
-        ens.close()
         pair = pair_sinks(ctx, na, conj=bool(args.conj))
         print(f"pair sinks: {pair['value']:.0f} configs*slices/s", file=sys.stderr, flush=True)
 
-    # the other mode of the same workload: conjugate partners on when the headline runs
-    # without them, and off when it runs with them
+    # the same workload in the other conjugate-partner mode, and in the other arithmetic
     other = None
     if not args.no_other_mode:
-        if ens is not None:
-            ens.close()
-        ens = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
-        ens.set_path(args.path)
-        ens.set_symmetry(not args.conj)
-        ens.run(seed, (1 << 24), T)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            ens.run(seed, k * C, T)
-        ctx.sync()
-        dt2 = time.perf_counter() - t0
-        ens.profile(True)
-        ens.run(seed, 1 << 23, T)
-        tx2, nx2 = ens.kernel_time(0)
-        ens.profile(False)
-        nvo2 = ens.volumes_per_config()
-        b2 = nvo2 * 16 * V + V
-        a2 = b2 * C / (tx2 / T * 1e-3) / 1e9
-        other = {"conj_partners": not args.conj, "value": C * T * args.steps / dt2,
-                 "unit": "configs*slices/s", "volumes_per_config": nvo2,
-                 "evolution_ms_per_slice": tx2 / T,
-                 "roofline": {"achieved": a2, "frac": a2 / HBM_PEAK_GBS, "bytes_per_cs": b2}}
-        print(f"other mode (conj={int(not args.conj)}): {other['value']:.0f} configs*slices/s",
+        other = {}
+        for key, conj, acc in (("conj_partners_" + ("off" if args.conj else "on"),
+                                not args.conj, args.acc),
+                               ("plain_fp32" if args.acc else "fp64_x_chain", bool(args.conj),
+                                1 - args.acc)):
+            e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+            e.set_path(args.path)
+            e.set_symmetry(conj)
+            e.set_accuracy(acc)
+            e.run(seed, (1 << 24), T)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                e.run(seed, k * C, T)
+            ctx.sync()
+            dt2 = time.perf_counter() - t0
+            e.profile(True)
+            e.run(seed, 1 << 23, T)
+            tx2, nx2 = e.kernel_time(0)
+            e.profile(False)
+            nvo2 = e.volumes_per_config()
+            e.close()
+            b2 = nvo2 * 16 * V + V
+            a2 = b2 * C / (tx2 / T * 1e-3) / 1e9
+            other[key] = {"conj_partners": bool(conj), "acc": acc,
+                          "value": C * T * args.steps / dt2, "unit": "configs*slices/s",
+                          "volumes_per_config": nvo2, "evolution_ms_per_slice": tx2 / T,
+                          "hbm_algorithmic": {"achieved_gbs": a2, "frac": a2 / HBM_PEAK_GBS,
+                                              "bytes_per_cs": b2}}
+            print(f"other mode {key}: {other[key]['value']:.0f} configs*slices/s",
+                  file=sys.stderr, flush=True)
+
+    # the FP64 reference of the precision reports (every orbital, same fields)
+    ref64 = None
+    precision = None
+    if args.precision_configs > 0:
+        e = na.Ensemble(ctx, L, moms, kin, A, args.precision_configs, prec=na.PREC_F64,
+                        split=na.SPLIT_KSPACE)
+        ref64 = e.run(seed, 0, T)[:2]
+        e.close()
+        modes = (bool(args.conj),) if args.no_other_mode else (bool(args.conj), not args.conj)
+        reps = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed, modes,
+                                acc=args.acc, ref64=ref64)
+        precision = reps[0]
+        if len(reps) > 1 and other is not None:
+            other["conj_partners_" + ("off" if args.conj else "on")]["precision"] = reps[1]
+        if other is not None:
+            other["plain_fp32" if args.acc else "fp64_x_chain"]["precision"] = precision_report(
+                ctx, na, L, N, T, tau, A, args.precision_configs, seed, (bool(args.conj),),
+                acc=1 - args.acc, ref64=ref64)[0]
+        print(f"precision: max rel err of C(t) {precision['max_rel_err_C']:.2e}",
               file=sys.stderr, flush=True)
-        ens.close()
-        ens = None
 
     dense = None
     if not args.no_dense:
-        if ens is not None:
-            ens.close()
-            ens = None
-        dense = dense_overlap(ctx, na)
+        dense = dense_overlap(ctx, na, L=L, N=N, T=T, C=C, ref64=ref64 if (
+            args.precision_configs == C) else None)
         print(f"dense overlaps: {dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
               f"{dense['mfma']['frac']:.3f}", file=sys.stderr, flush=True)
-
-    precision = None
-    if args.precision_configs > 0:
-        modes = (bool(args.conj),) if args.no_other_mode else (bool(args.conj), not args.conj)
-        reps = precision_report(ctx, na, L, N, T, tau, A, args.precision_configs, seed, modes)
-        precision = reps[0]
-        if len(reps) > 1 and other is not None:
-            other["precision"] = reps[1]
-        print(f"precision: max rel err of C(t) {precision['max_rel_err_C']:.2e}",
-              file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(L, N, tau, A, args.cpu_slices, threads)
+        print(f"cpu baseline: {threads} streams x {args.cpu_configs} configs x "
+              f"{args.cpu_slices} slices, best of {args.cpu_reps}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(L, N, tau, A, args.cpu_configs, args.cpu_slices, threads,
+                           args.cpu_reps)
 
     out = {
         "metric": "aux-field configs*time-slices/sec at L=32,N=33",
         "value": cs_per_s, "unit": "configs*slices/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"Slater-determinant correlator over aux-field configurations: L={L}, N={N}, T={T}, "
-                               f"tau={tau}, A={A}, {C} configs per step, FP32 evolution, "
-                               f"FP64 LU", "L": L, "N": N, "T": T, "configs": C,
+        "config": {"workload": f"Slater-determinant correlator over aux-field configurations: "
+                               f"L={L}, N={N}, T={T}, tau={tau}, A={A}, {C} configs per step, "
+                               f"FP32 evolution ("
+                               + ("FP64 x-chain and z-rows" if args.acc else "plain FP32")
+                               + "), FP64 LU",
+                   "L": L, "N": N, "T": T, "configs": C,
                    "parallelism": "single GPU, batched configurations"},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -607,9 +705,9 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]
+        out["speedup_vs_cpu_all_visible_cpus_linear"] = (
+            cs_per_s / cpu["value_all_visible_cpus_linear"])
     print(json.dumps(out))
-    if ens is not None:
-        ens.close()
     ctx.close()
 
 

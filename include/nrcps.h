@@ -232,14 +232,17 @@ int nrc_ensemble_destroy(nrc_ensemble* ens);
  *   streams; destroy the batch before its ensembles.
  * nrc_batch_run: T slices of every group, group i on configurations config0[i] ..
  *   (Philox(seed) fields).  graph != 0 captures the kernels of all groups and slices into one
- *   HIP graph (fork/join over the group streams) and launches it as one unit; the graph is
- *   reused while T, seed and config0 are unchanged.  Results: nrc_ensemble_results per group.
- * nrc_batch_capture_ms: host time of the last capture and instantiation. */
+ *   HIP graph (fork/join over the group streams) and launches it as one unit; seed and
+ *   config0 reach the captured kernels through device parameter blocks, so the graph is
+ *   reused by every later run with the same T.  Results: nrc_ensemble_results per group.
+ * nrc_batch_capture_ms: host time of the last capture and instantiation.
+ * nrc_batch_captures: graphs captured so far (one per distinct T). */
 typedef struct nrc_batch nrc_batch;
 int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, int n_groups,
                      nrc_batch** batch);
 int nrc_batch_run(nrc_batch* batch, uint64_t seed, const uint32_t* config0, int T, int graph);
 int nrc_batch_capture_ms(nrc_batch* batch, double* ms);
+int nrc_batch_captures(nrc_batch* batch, long* n);
 int nrc_batch_destroy(nrc_batch* batch);
 
 /* ---- slice-by-slice stepping: the loop of nrc_ensemble_run (production/slater_det/

@@ -32,7 +32,7 @@ EXPORTS = [
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
-    "nrc_ensemble_set_accuracy",
+    "nrc_ensemble_set_accuracy", "nrc_batch_captures",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -100,6 +100,7 @@ def load_library() -> C.CDLL:
         "nrc_batch_create": [vp, vp, i, C.POINTER(vp)],
         "nrc_batch_run": [vp, u64, vp, i, i],
         "nrc_batch_capture_ms": [vp, C.POINTER(C.c_double)],
+        "nrc_batch_captures": [vp, C.POINTER(C.c_long)],
         "nrc_batch_destroy": [vp],
         "nrc_ensemble_begin": [vp, i], "nrc_ensemble_measure": [vp, i],
         "nrc_ensemble_evolve": [vp, i, vp],
@@ -421,6 +422,11 @@ class Batch:
         ms = C.c_double()
         _check(self.lib.nrc_batch_capture_ms(self.h, C.byref(ms)), "batch_capture_ms")
         return ms.value
+
+    def captures(self) -> int:
+        n = C.c_long()
+        _check(self.lib.nrc_batch_captures(self.h, C.byref(n)), "batch_captures")
+        return n.value
 
     def close(self):
         if getattr(self, "h", None) and getattr(self.ctx, "h", None):

@@ -396,6 +396,8 @@ struct nrc_ensemble {
   double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
   int acc = NRC_ACC_X64;         // FP32 ensembles: nrc_ensemble_set_accuracy
+  uint64_t* par_dev = nullptr;   // the run's (seed, config0), read by the field kernels
+  uint64_t* par_host = nullptr;  // pinned staging of par_dev
   double* tab_x64 = nullptr;     // FP32 k-space split: K(1/2) and K as FP64 tables (x-chain)
   double* tab_k64 = nullptr;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
@@ -421,6 +423,8 @@ static void ens_free(nrc_ensemble* e) {
                   e->tab_x64, e->tab_k64};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (e->par_dev) hipFree(e->par_dev);
+  if (e->par_host) hipHostFree(e->par_host);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
   delete e;
 }
@@ -502,6 +506,9 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   } while (0)
   ENS_TRY(hipMalloc(&e->psi, (size_t)C * N * V * elem_bytes(d->prec)) == hipSuccess
               ? NRC_OK : NRC_ERR_NOMEM);
+  ENS_TRY(dev_alloc(&e->par_dev, 2));
+  ENS_TRY(hipHostMalloc((void**)&e->par_host, 2 * sizeof(uint64_t), hipHostMallocDefault) ==
+                  hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
   e->nvo = N;
   // tables even under p -> -p, to 1e-12 relative (sin^2(pi n / L) and sin^2(pi (L - n) / L)
   // differ in the last bit): the conjugate mirror then holds to rounding
@@ -1125,6 +1132,17 @@ static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   return ens_reserve_outputs(e, T);
 }
 
+// the run's seed and first configuration into the device parameter block the field kernels
+// read, on `s` (the ensemble's stream, or the stream a batch launches its graph on); the
+// previous run has completed (runs synchronise), so the pinned staging words are free
+static int ens_set_params(nrc_ensemble* e, uint64_t seed, uint32_t config0, hipStream_t s) {
+  e->par_host[0] = seed;
+  e->par_host[1] = config0;
+  NRC_HIP(hipMemcpyAsync(e->par_dev, e->par_host, 2 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                         s));
+  return NRC_OK;
+}
+
 // the kernels of a run of T slices on the ensemble's stream; nothing here allocates, copies
 // to or from the host or synchronises, so a batch can capture it into a graph
 static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool host_words, int T) {
@@ -1161,7 +1179,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
         EvScope sc(e, 3);
         for (int k = 0; k < S; ++k)
           NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
-                                      e->words + (size_t)k * slice_words, s));
+                                      e->words + (size_t)k * slice_words, s, e->par_dev));
       }
       NRC_TRY(ens_evolve_x(e, t, S, host_words ? field_of(t) : e->words, slice_words,
                            t + S == T));
@@ -1181,7 +1199,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
       EvScope sc(e, 3);
       for (int k = 0; k < S; ++k)
         NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
-                                    e->words + (size_t)k * slice_words, s));
+                                    e->words + (size_t)k * slice_words, s, e->par_dev));
     }
     if (S > 1)
       NRC_TRY(ens_evolve_fused(e, t, S, host_words ? field_of(t) : e->words, slice_words));
@@ -1224,7 +1242,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     // Lattice::Refresh for slice t
     if (!host_words) {
       EvScope sc(e, 3);
-      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s));
+      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s,
+                                  e->par_dev));
     }
     if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
     // YZ pass: the field (k-space split) or kinetic (x-space split) multiply between the FFTs
@@ -1275,6 +1294,7 @@ static int ens_collect(nrc_ensemble* e, int T, double* logabs, double* phase, do
 static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
                    int T, double* logabs, double* phase, double* argsum) {
   NRC_TRY(ens_prepare(e, words_host, T));
+  NRC_TRY(ens_set_params(e, seed, config0, e->stream));
   NRC_TRY(ens_enqueue(e, seed, config0, words_host != nullptr, T));
   e->last_T = T;
   return ens_collect(e, T, logabs, phase, argsum);
@@ -1517,6 +1537,8 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
     if (rc != NRC_OK) break;
     rc = ens_prepare(e, nullptr, T_probe);
     if (rc != NRC_OK) break;
+    rc = ens_set_params(e, 0x5eedULL, 0xfffff000u, e->stream);
+    if (rc != NRC_OK) break;
     hipEventRecord(a, e->stream);
     rc = ens_enqueue(e, 0x5eedULL, 0xfffff000u, false, T_probe);
     hipEventRecord(b, e->stream);
@@ -1548,8 +1570,7 @@ struct nrc_batch {
   hipEvent_t fork = nullptr;
   hipGraphExec_t exec = nullptr;
   int exec_T = 0;
-  uint64_t exec_seed = 0;
-  std::vector<uint32_t> exec_cfg;
+  long captures = 0;  // graphs captured and instantiated so far
   double capture_ms = 0.0;
 };
 
@@ -1600,8 +1621,9 @@ extern "C" int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, i
 // Runs T slices of every group: configurations config0[i] .. of group i, fields from
 // Philox(seed).  graph != 0: the groups' kernels are captured once into one HIP graph (the
 // context stream forks to the group streams and joins them again) and launched as a whole;
-// the graph is kept while T, seed and config0 stay the same.  graph == 0: the same kernels
-// are launched directly on the group streams.
+// the field kernels read seed and config0 from the groups' device parameter blocks, written
+// before each launch, so the graph is kept for every later run with the same T.  graph == 0:
+// the same kernels are launched directly on the group streams.
 extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config0, int T,
                              int graph) {
   if (!b || !config0 || T < 1) return NRC_ERR_ARG;
@@ -1614,6 +1636,9 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
   }
   for (hipStream_t st : b->streams) NRC_HIP(hipStreamSynchronize(st));
   hipStream_t main = b->ctx->stream;
+  // seed and configuration ranges go to the groups' parameter blocks on the launching stream,
+  // ahead of the fork: the kernels (and a kept graph) read them from there
+  for (int i = 0; i < n; ++i) NRC_TRY(ens_set_params(b->g[i], seed, config0[i], main));
   if (!graph) {
     NRC_HIP(hipEventRecord(b->fork, main));
     for (int i = 0; i < n; ++i) {
@@ -1625,9 +1650,7 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     NRC_HIP(hipStreamSynchronize(main));
     return NRC_OK;
   }
-  const bool same = b->exec && b->exec_T == T && b->exec_seed == seed &&
-                    std::equal(b->exec_cfg.begin(), b->exec_cfg.end(), config0);
-  if (!same) {
+  if (!(b->exec && b->exec_T == T)) {  // one graph per T: seeds and ranges are parameters
     batch_drop_graph(b);
     hipEvent_t t0, t1;
     hipEventCreate(&t0);
@@ -1661,8 +1684,7 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     b->capture_ms = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - wall0).count();
     b->exec_T = T;
-    b->exec_seed = seed;
-    b->exec_cfg.assign(config0, config0 + n);
+    ++b->captures;
   }
   NRC_HIP(hipGraphLaunch(b->exec, main));
   NRC_HIP(hipStreamSynchronize(main));
@@ -1672,6 +1694,12 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
 extern "C" int nrc_batch_capture_ms(nrc_batch* b, double* ms) {
   if (!b || !ms) return NRC_ERR_ARG;
   *ms = b->capture_ms;
+  return NRC_OK;
+}
+
+extern "C" int nrc_batch_captures(nrc_batch* b, long* n) {
+  if (!b || !n) return NRC_ERR_ARG;
+  *n = b->captures;
   return NRC_OK;
 }
 

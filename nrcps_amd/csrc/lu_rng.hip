@@ -8,8 +8,14 @@ namespace nrc {
 // Word w of configuration c's field at slice t is output (w % 4) of the Philox4x32-10 block
 // with counter (w / 4, t, c, stream) and key (seed lo, seed hi); bit j of word w is site
 // 32 w + j, phi = +1 where set.
+// par != null: seed = par[0] and config0 = par[1] are read from device memory, so that a
+// captured graph of these launches serves any seed and configuration range
 __global__ void k_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
-                          uint32_t stream, int nw, uint32_t* words) {
+                          uint32_t stream, int nw, uint32_t* words, const uint64_t* par) {
+  if (par) {
+    seed = par[0];
+    config0 = (uint32_t)par[1];
+  }
   const int ng = (nw + 3) / 4;
   const long n = (long)nconf * ng;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -25,14 +31,15 @@ __global__ void k_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t
 }
 
 hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
-                          uint32_t stream, int vol, uint32_t* words, hipStream_t s) {
+                          uint32_t stream, int vol, uint32_t* words, hipStream_t s,
+                          const uint64_t* par) {
   const int nw = (vol + 31) / 32;
   const long n = (long)nconf * ((nw + 3) / 4);
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_z2_bits, dim3((unsigned)blocks), dim3(256), 0, s, seed, config0, nconf,
-                     t, stream, nw, words);
+                     t, stream, nw, words, par);
   return hipGetLastError();
 }
 

@@ -159,8 +159,10 @@ hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvo
                                   int L, hipStream_t s);
 
 // Philox Z2 field bits: words[c][w] for configs config0 .. config0 + nconf - 1
+// (par != null: seed and config0 read from par[0], par[1] in device memory at run time)
 hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
-                          uint32_t stream, int vol, uint32_t* words, hipStream_t s);
+                          uint32_t stream, int vol, uint32_t* words, hipStream_t s,
+                          const uint64_t* par = nullptr);
 
 // Options of the batched LU (launch_lu_det), all off by default:
 //   repl, ldr: the last column of every minor (size k) is row k - 1 of repl[c] (a matrix of

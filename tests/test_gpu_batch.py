@@ -47,13 +47,17 @@ def test_batch_matches_groups_alone(ctx, orc, graph):
             la, ph, ag = g.results(T)
             assert np.array_equal(la, ref[0]) and np.array_equal(ph, ref[1])
             assert np.array_equal(ag, ref[2])
-    # new configurations: a new capture
+    # new configurations and a new seed: the kept graph reads them from the parameter blocks
     c1 = [c + 100 for c in c0]
-    b.run(seed, c1, T, graph=bool(graph))
-    for g, (L, N, C, prec), c in zip(groups, SHAPES, c1):
-        la, ph, _ = g.results(T)
-        alone = _groups(ctx, orc, [(L, N, C, prec)])[0].run(seed, c, T)
-        assert np.array_equal(la, alone[0]) and np.array_equal(ph, alone[1])
+    for sd in (seed, seed + 1):
+        b.run(sd, c1, T, graph=bool(graph))
+        for g, (L, N, C, prec), c in zip(groups, SHAPES, c1):
+            la, ph, _ = g.results(T)
+            alone = _groups(ctx, orc, [(L, N, C, prec)])[0].run(sd, c, T)
+            assert np.array_equal(la, alone[0]) and np.array_equal(ph, alone[1])
+    assert b.captures() == (1 if graph else 0)
+    b.run(seed, c1, T + 1, graph=bool(graph))  # another T: another graph
+    assert b.captures() == (2 if graph else 0)
     b.close()
     # after the batch the ensembles run alone again
     la, _, _ = groups[0].run(seed, c0[0], T)
