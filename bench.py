@@ -336,13 +336,18 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                          "output_bias": float(ps[8240])}}
 
 
-def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012, ref64=None):
+def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012, ref64=None,
+                  acc=1):
     """The overlap matrix Phi^H Psi(t) as a batched complex GEMM on the matrix cores (dense
-    sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16x3 products
-    with FP32 accumulation, every orbital read once per measurement.  The sinks are the
-    plane waves as dense k-space vectors, so the gathered measurement is the check.  Reports
-    the overlap kernel time per slice against the MFMA peak (3 x 8 N^2 V bf16 flops per
-    configuration-slice) and the HBM peak (8 N V bytes read per configuration-slice)."""
+    sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16 products
+    with FP32 accumulation, every orbital read once per measurement; with the default
+    arithmetic (acc 1) the splits are three-way (BF16x6: six products, FP32-exact inputs) and
+    the evolution runs the FP64 x-chain one slice per launch, the GEMM reading the state
+    with its x-DFT pending against sinks transformed to match; acc 0: split-BF16x3 and plain
+    FP32 single-slice evolution.  The sinks are the plane waves as dense k-space vectors, so
+    the gathered measurement is the check.  Reports the overlap kernels' time per slice
+    against the MFMA peak (terms x 8 N^2 V bf16 flops per configuration-slice) and the HBM
+    peak (8 N V bytes read per configuration-slice)."""
     moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
     V = L ** 3
     sinks = np.zeros((N, V), np.complex128)
@@ -350,6 +355,8 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
     e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE,
                     sinks=sinks)
     e.set_path(na.PATH_SINGLE_PASS)
+    e.set_accuracy(acc)
+    terms = 6 if acc == 1 else 3
     e.run(seed, 0, 2)
     ctx.sync()
     t0 = time.perf_counter()
@@ -372,13 +379,18 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
         prec = {"max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
                 "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
                 "reference": "FP64 path on the same fields (the precision report's)",
-                "arithmetic": "plain FP32 evolution (single-slice kernel, every orbital), "
-                              "split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU"}
+                "arithmetic": ("FP32 evolution with the FP64 x-chain (one slice per launch, "
+                               "every orbital), split-BF16x6 overlaps on MFMA with FP32 "
+                               "accumulation, FP64 LU" if acc == 1 else
+                               "plain FP32 evolution (single-slice kernel, every orbital), "
+                               "split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU")}
     ms = t_ovl / max(n_ovl, 1)
-    flops = 3 * 8.0 * N * N * V * C
+    flops = terms * 8.0 * N * N * V * C
     byts = 8.0 * N * V * C
-    return {"workload": f"dense sinks: L={L}, N={N}, T={T}, {C} configurations, FP32 evolution, "
-                        f"split-BF16x3 overlaps on MFMA, FP64 LU",
+    return {"workload": f"dense sinks (the precision form as written): L={L}, N={N}, T={T}, "
+                        f"{C} configurations, FP32 evolution, split-BF16x{terms} overlaps on "
+                        f"MFMA, FP64 LU",
+            "terms": terms,
             "value": C * T / dt, "unit": "configs*slices/s",
             "overlap_ms_per_slice": ms,
             "mfma": {"achieved_tflops": flops / (ms * 1e-3) / 1e12, "peak_tflops": 2500.0,
@@ -670,7 +682,7 @@ def main():
     dense = None
     if not args.no_dense:
         dense = dense_overlap(ctx, na, L=L, N=N, T=T, C=C, ref64=ref64 if (
-            args.precision_configs == C) else None)
+            args.precision_configs == C) else None, acc=args.acc)
         print(f"dense overlaps: {dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
               f"{dense['mfma']['frac']:.3f}", file=sys.stderr, flush=True)
 

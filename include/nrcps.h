@@ -39,6 +39,8 @@ extern "C" {
 #define NRC_OVERLAP_F64 0    /* dense overlaps in FP64 */
 #define NRC_OVERLAP_F32 1    /* dense overlaps of FP32 data, FP64 accumulation */
 #define NRC_OVERLAP_BF16X3 2 /* split-BF16x3 MFMA products, FP32 accumulation */
+#define NRC_OVERLAP_BF16X6 3 /* three-way BF16 splits, six MFMA products (FP32-exact inputs),
+                                FP32 accumulation */
 
 typedef struct nrc_context nrc_context;
 typedef struct nrc_fft nrc_fft;

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG, "libnrcps_hip.so")
 OK = 0
 PREC_F32, PREC_F64 = 0, 1
 SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
-OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3 = 0, 1, 2
+OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3, OVERLAP_BF16X6 = 0, 1, 2, 3
 PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
 ACC_FP32, ACC_X64 = 0, 1
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)

@@ -235,20 +235,23 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf, long vol, int prec) {
 
 // workspace of one split-BF16x3 overlap call: the split sinks (hi and lo bf16) and the
 // per-chunk FP32 partial tiles
+// workspace of one split-BF16 overlap call: the split sinks (3 bf16 planes, room for the
+// BF16x6 split) and the per-chunk FP32 partial tiles
 static size_t overlap_split_bytes(int ns, long vol) {
-  return ((size_t)4 * ns * vol * 2 + 255) / 256 * 256;
+  return ((size_t)3 * 2 * ns * vol * 2 + 255) / 256 * 256;
 }
 static size_t overlap_bf16x3_workspace(int ns, int nt, long vol, int ncfg) {
   return overlap_split_bytes(ns, vol) +
          nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg) * sizeof(float);
 }
+// terms 3: split-BF16x3; 6: split-BF16x6 (three-way splits, six products)
 static int overlap_bf16x3(const void* phi, const void* psi, double* M, int ns, int nt, long vol,
-                          int ncfg, void* work, hipStream_t s) {
+                          int ncfg, void* work, hipStream_t s, int terms = 3) {
   char* w = (char*)work;
   const size_t split = overlap_split_bytes(ns, vol);
-  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s));
+  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s, terms == 6 ? 3 : 2));
   NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
-                                     (float*)(w + split), s));
+                                     (float*)(w + split), s, terms));
   return NRC_OK;
 }
 
@@ -263,13 +266,14 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
                                       ctx->stream));
     return NRC_OK;
   }
-  if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
+  if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
   if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
   if (ncfg == 0) return NRC_OK;
   const size_t bytes = overlap_bf16x3_workspace(ns, nt, vol, ncfg);
   void* w = ctx_workspace(ctx, bytes);
   if (!w) return NRC_ERR_NOMEM;
-  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream);
+  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
+                        mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
 }
 
 int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const double* w,
@@ -345,8 +349,10 @@ struct nrc_ensemble {
   double* sink_k = nullptr;  // k-space split: K(1/2)(k_s) per plane-wave sink
   double* lsrc = nullptr;    // FP32: log source_amplitude(c) per configuration
   void* sinks_k = nullptr;   // k-space split, dense sinks: K(1/2) phi_s
+  void* sinks_mx = nullptr;  // FP32 k-space split, dense sinks: X-[K(1/2) phi_s] (x-chain)
   std::vector<double> khalf; // k-space split: K(1/2) per site (host)
   bool folded = false;       // the stored orbitals are xi_half psi (k-space split, t >= 1)
+  bool pending_x = false;    // stored in (x, k_y, k_z) with the x-DFT(+1) pending (x-chain)
   bool has_state = false;    // psi holds orbitals in the current layout (a run has started)
   void* tab_yz = nullptr;    // x-space split: kappa/V (YZ pass); MUL_TABLE format
   void* smear = nullptr;     // k-space split + smear: f(k), MUL_TABLE format
@@ -415,6 +421,7 @@ static bool ens_in_batch(const nrc_ensemble* e) { return e->stream != e->ctx->st
 
 static void ens_free(nrc_ensemble* e) {
   void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->pw, e->popp, e->Mp,
+                  e->sinks_mx,
                   e->pair_work, e->sinks_k, e->tab_yz, e->smear,
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
@@ -458,6 +465,35 @@ template <typename X>
 static int dev_alloc(X** p, size_t n) {
   if (hipMalloc((void**)p, (n ? n : 1) * sizeof(X)) != hipSuccess) return NRC_ERR_NOMEM;
   return NRC_OK;
+}
+
+// s''(x, k1, k2) = sum_k0 s'(k0, k1, k2) e^{-2 pi i k0 x / L}, s' = K(1/2) s, for N complex
+// sink vectors (FP64 arithmetic, twiddles by exact reduction of k0 x mod L), rounded to FP32
+static std::vector<float> sinks_for_pending_x(const double* sinks, const std::vector<double>& kh,
+                                              int N, int L) {
+  const long P = (long)L * L, V = P * L;
+  std::vector<double> c(L), sn(L);
+  for (int m = 0; m < L; ++m) {
+    c[m] = cos(2.0 * M_PI * m / L);
+    sn[m] = -sin(2.0 * M_PI * m / L);
+  }
+  std::vector<float> out((size_t)2 * N * V);
+  for (int n = 0; n < N; ++n)
+    for (long yz = 0; yz < P; ++yz)
+      for (int x = 0; x < L; ++x) {
+        double re = 0.0, im = 0.0;
+        for (int k0 = 0; k0 < L; ++k0) {
+          const long i = (long)n * V + k0 * P + yz;
+          const double a = sinks[2 * i] * kh[k0 * P + yz], b = sinks[2 * i + 1] * kh[k0 * P + yz];
+          const int m = (int)(((long)k0 * x) % L);
+          re += a * c[m] - b * sn[m];
+          im += a * sn[m] + b * c[m];
+        }
+        const size_t o = (size_t)n * V + (size_t)x * P + yz;
+        out[2 * o] = (float)re;
+        out[2 * o + 1] = (float)im;
+      }
+  return out;
 }
 
 // site of -p for the site of p: (n0, n1, n2) -> ((L - n0) % L, ...), z fastest
@@ -629,6 +665,13 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
         }
         ENS_TRY(hipMalloc(&e->sinks_k, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
         ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 4, hipMemcpyHostToDevice) ==
+                        hipSuccess ? NRC_OK : NRC_ERR_HIP);
+        // the same sinks for the x-chain's stored state m, whose folded k-space state is
+        // X+ m (the x-DFT of sign +1 along the first index): <s|K(1/2) X+ m> =
+        // <X- (K(1/2) s)|m>, since the adjoint of X+ is X-; computed here in FP64
+        const std::vector<float> mx = sinks_for_pending_x(d->sinks, e->khalf, N, L);
+        ENS_TRY(hipMalloc(&e->sinks_mx, mx.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+        ENS_TRY(hipMemcpy(e->sinks_mx, mx.data(), mx.size() * 4, hipMemcpyHostToDevice) ==
                         hipSuccess ? NRC_OK : NRC_ERR_HIP);
       }
     }
@@ -858,6 +901,7 @@ static nrc::GatherSpec ens_gather_spec(const nrc_ensemble* e) {
 static int ens_set_sources(nrc_ensemble* e) {
   const long C = e->d.n_configs;
   e->folded = false;
+  e->pending_x = false;
   e->has_state = true;
   const int nvo = e->nvo;
   if (e->d.split == NRC_SPLIT_KSPACE)
@@ -913,12 +957,14 @@ static int ens_measure(nrc_ensemble* e, int t) {
   const size_t per_t = (size_t)C * e->nm;
   {
     EvScope sc(e, 4);
-    // folded k-space state xi_half psi: sinks carry the missing K(1/2)
-    const void* sk = e->folded ? e->sinks_k : e->sinks;
+    // folded k-space state xi_half psi: sinks carry the missing K(1/2); a state with its
+    // x-DFT(+1) pending: the sinks transformed by X- as well
+    const void* sk = e->pending_x ? e->sinks_mx : e->folded ? e->sinks_k : e->sinks;
     if (e->sinks && prec == NRC_PREC_F64)
       NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks)
-      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s));
+      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s,
+                             e->acc == NRC_ACC_X64 ? 6 : 3));
     else {
       nrc::ConjCols cp;
       if (e->sym) {
@@ -1067,12 +1113,15 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
 // the run's last launch stores the folded k-space state
 static bool ens_xchain(const nrc_ensemble* e, bool single) {
   return single && e->d.prec == NRC_PREC_F32 && e->acc == NRC_ACC_X64 &&
-         e->d.split == NRC_SPLIT_KSPACE && !e->smear && !e->sinks && !e->pair &&
-         !e->trap_tab && nrc::slicex_supported(e->d.L);
+         e->d.split == NRC_SPLIT_KSPACE && !e->smear && !e->pair && !e->trap_tab &&
+         nrc::slicex_supported(e->d.L);
 }
 
+// gathered plane-wave sinks: measured inside the launch (S slices); dense sinks: measured
+// beforehand by ens_measure, one slice per launch
 static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, size_t stride,
                         bool last) {
+  const bool gather = e->sinks == nullptr;
   const int N = e->d.n_orb;
   const long C = e->d.n_configs, V = e->V;
   const size_t per_t = (size_t)C * e->nm;
@@ -1092,6 +1141,7 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.bits_stride = (long)stride;
   a.a0 = 1.0 / (double)V;
   a.a1 = e->d.A / (double)V;
+  a.gather = gather;
   a.g = ens_gather_spec(e);
   a.g.M = e->Mx;
   a.g.sink_scale = e->sink_k;  // not on the sources (t = 0): the kernel drops it there
@@ -1100,7 +1150,9 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
     EvScope sc(e, 0);
     NRC_HIP(nrc::launch_slicex(a, e->stream));
   }
-  e->folded = true;
+  e->folded = last;
+  e->pending_x = !last;
+  if (!gather) return NRC_OK;
   EvScope sc(e, 2);
   nrc::LuOpts o;
   o.lsrc = e->lsrc;
@@ -1173,8 +1225,10 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   // dense sinks and pair matrices read the stored state
   if ((e->sinks || e->pair) && !single) return NRC_ERR_UNSUPPORTED;
   if (ens_xchain(e, single)) {  // FP32 state, FP64 x-chain, measurements inside the launches
+    e->pending_x = false;
     for (int t = 0; t < T;) {
-      const int S = std::min(nrc::SLICE_FUSE_MAX, T - t);
+      const int S = e->sinks ? 1 : std::min(nrc::SLICE_FUSE_MAX, T - t);
+      if (e->sinks) NRC_TRY(ens_measure(e, t));  // dense sinks: the GEMM before the slice
       if (!host_words) {
         EvScope sc(e, 3);
         for (int k = 0; k < S; ++k)

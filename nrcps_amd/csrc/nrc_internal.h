@@ -111,6 +111,7 @@ struct SliceXArgs {
   int grid = 0;  // CUs
   int nslices = 1;
   int start_k = 0, store_k = 0;
+  int gather = 1;  // gather the plane-wave sinks of g (0: measured elsewhere)
   const double* tab_x = nullptr;
   const double* tab_k = nullptr;
   const uint32_t* bits = nullptr;
@@ -209,10 +210,13 @@ hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s);
 // dense overlaps M[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i]
 hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,
                                 int ns, int nt, long vol, int ncfg, hipStream_t s);
+// terms 3: split-BF16x3 (phi_split: hi, lo planes); 6: split-BF16x6 (three planes)
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
-                                 int nt, long vol, int ncfg, float* partial, hipStream_t s);
-// phi_split: bf16 hi[ns][2V] followed by lo[ns][2V]
-hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s);
+                                 int nt, long vol, int ncfg, float* partial, hipStream_t s,
+                                 int terms = 3);
+// phi_split: `parts` bf16 planes [part][ns][2V] (hi, lo, and with parts = 3 the third part)
+hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s,
+                             int parts = 2);
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 
 // pair-sink matrices M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]: FP64 sums

@@ -42,27 +42,48 @@ __device__ __forceinline__ void split8(const float* v, bf16x8& hi, bf16x8& lo) {
   }
 }
 
-// sinks phi[s][V] complex FP32 -> hi[s][2V], lo[s][2V] bf16 (interleaved re, im)
-__global__ void k_split_bf16(const float2* phi, __bf16* hi, __bf16* lo, long n) {
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const float2 v = phi[i];
-    const __bf16 hr = to_bf16(v.x), hi_ = to_bf16(v.y);
-    hi[2 * i] = hr;
-    hi[2 * i + 1] = hi_;
-    lo[2 * i] = to_bf16(v.x - (float)hr);
-    lo[2 * i + 1] = to_bf16(v.y - (float)hi_);
+// split 8 floats into three bf16 parts, v = p0 + p1 + p2 to the last bit of FP32 (each part
+// holds the next 8 significant bits of the remainder; both subtractions are exact)
+__device__ __forceinline__ void split8_3(const float* v, bf16x8& p0, bf16x8& p1, bf16x8& p2) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const __bf16 h = to_bf16(v[q]);
+    const float r = v[q] - (float)h;
+    const __bf16 m = to_bf16(r);
+    p0[q] = h;
+    p1[q] = m;
+    p2[q] = to_bf16(r - (float)m);
   }
 }
 
-hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s) {
+// sinks phi[s][V] complex FP32 -> `parts` bf16 planes [part][s][2V] (interleaved re, im):
+// parts = 2: hi, lo; parts = 3: the three-way split of split8_3
+__global__ void k_split_bf16(const float2* phi, __bf16* out, long n, int parts) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 v = phi[i];
+    const float x[2] = {v.x, v.y};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const __bf16 h = to_bf16(x[j]);
+      const float r = x[j] - (float)h;
+      const __bf16 m = to_bf16(r);
+      out[2 * i + j] = h;
+      out[2 * n + 2 * i + j] = m;
+      if (parts == 3) out[4 * n + 2 * i + j] = to_bf16(r - (float)m);
+    }
+  }
+}
+
+hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s,
+                             int parts) {
+  if (parts != 2 && parts != 3) return hipErrorInvalidValue;
   const long n = (long)ns * vol;
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
-  __bf16* hi = (__bf16*)out;
-  __bf16* lo = hi + 2 * n;
-  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)blocks), dim3(256), 0, s, phi, hi, lo, n);
+  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)blocks), dim3(256), 0, s, phi, (__bf16*)out,
+                     n, parts);
   return hipGetLastError();
 }
 
@@ -76,11 +97,18 @@ hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hip
 // one lane offset per tile, the K position as the scalar offset; rows past ns / nt are past
 // the resource's end and read zero), and the next step's values are loaded before the
 // current step's products.
-template <int MT, int NT>
+//
+// TERMS = 6 (split-BF16x6): sinks and orbital values are split three ways, p = p0 + p1 + p2,
+// exact to the last bit of FP32, and a.b is taken as the six products of total order <= 2
+// (a0 b0, a0 b1, a1 b0, a0 b2, a1 b1, a2 b0; the dropped ones are below 2^-24 relative), so a
+// plane-wave sink is represented exactly and the overlap error is FP32 accumulation alone.
+template <int MT, int NT, int TERMS>
 __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
     const __bf16* __restrict__ phi_hi, const __bf16* __restrict__ phi_lo,
+    const __bf16* __restrict__ phi_lo2,
     const float2* __restrict__ psi, float* __restrict__ partial, int ns, int nt, long vol,
     long kchunk) {
+  static_assert(TERMS == 3 || TERMS == 6, "terms");
   __shared__ float red[OV_WAVES - 1][4][64];
   const int chunk = blockIdx.x, nchunks = gridDim.x;
   const long c = blockIdx.y;
@@ -92,6 +120,7 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
   const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
   const rsrc_t rh = make_rsrc(phi_hi, (uint32_t)(ns * K * 2));
   const rsrc_t rl = make_rsrc(phi_lo, (uint32_t)(ns * K * 2));
+  const rsrc_t rl2 = make_rsrc(phi_lo2, TERMS == 6 ? (uint32_t)(ns * K * 2) : 0u);
   const rsrc_t rb = make_rsrc(psi + c * nt * vol, (uint32_t)(nt * vol * 8));
   constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
   uint32_t oa[MT], ob[NT];
@@ -117,7 +146,7 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
   // the sink (hi, lo) and orbital values of one step; lanes past the chunk end read zero (the
   // rows are contiguous, so a read past K would be the next row's)
   struct Step {
-    u32x4 ah[MT], al[MT];
+    u32x4 ah[MT], al[MT], a2[TERMS == 6 ? MT : 1];
     float4 b0[NT], b1[NT];
   };
   auto load = [&](Step& st, long kb) {
@@ -128,6 +157,7 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
       const uint32_t o = in ? oa[a] : kFar;
       st.ah[a] = bload<u32x4>(rh, o, sa);
       st.al[a] = bload<u32x4>(rl, o, sa);
+      if constexpr (TERMS == 6) st.a2[a] = bload<u32x4>(rl2, o, sa);
     }
 #pragma unroll
     for (int b = 0; b < NT; ++b) {
@@ -143,23 +173,29 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
     for (int b = 0; b < NT; ++b) {
       const float v[8] = {st.b0[b].x, st.b0[b].y, st.b0[b].z, st.b0[b].w,
                           st.b1[b].x, st.b1[b].y, st.b1[b].z, st.b1[b].w};
-      bf16x8 hr, lr;
-      split8(v, hr, lr);
+      bf16x8 pr[3];  // orbital parts (the third only with TERMS = 6)
+      if constexpr (TERMS == 6) split8_3(v, pr[0], pr[1], pr[2]);
+      else split8(v, pr[0], pr[1]);
       // imaginary-part column: (im, -re) per element = dword halves swapped, upper sign flipped
-      const u32x4 uh = __builtin_bit_cast(u32x4, hr), ul = __builtin_bit_cast(u32x4, lr);
-      u32x4 vh, vl;
+      bf16x8 pi[3];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        vh[q] = ((uh[q] >> 16) | (uh[q] << 16)) ^ 0x80000000u;
-        vl[q] = ((ul[q] >> 16) | (ul[q] << 16)) ^ 0x80000000u;
+      for (int j = 0; j < (TERMS == 6 ? 3 : 2); ++j) {
+        const u32x4 u = __builtin_bit_cast(u32x4, pr[j]);
+        u32x4 w;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = ((u[q] >> 16) | (u[q] << 16)) ^ 0x80000000u;
+        pi[j] = __builtin_bit_cast(bf16x8, w);
       }
-      const bf16x8 hi_ = __builtin_bit_cast(bf16x8, vh), li_ = __builtin_bit_cast(bf16x8, vl);
+      // (sink part, orbital part) of each term: TERMS 3: (0,0) (0,1) (1,0); 6: also (0,2)
+      // (1,1) (2,0)
+      constexpr int TA[6] = {0, 0, 1, 0, 1, 2}, TB[6] = {0, 1, 0, 2, 1, 0};
 #pragma unroll
-      for (int term = 0; term < 3; ++term)
+      for (int term = 0; term < TERMS; ++term)
 #pragma unroll
         for (int a = 0; a < MT; ++a) {
-          const bf16x8 av = __builtin_bit_cast(bf16x8, term == 2 ? st.al[a] : st.ah[a]);
-          const bf16x8 br = term == 1 ? lr : hr, bi = term == 1 ? li_ : hi_;
+          const u32x4 araw = TA[term] == 0 ? st.ah[a] : TA[term] == 1 ? st.al[a] : st.a2[a % (TERMS == 6 ? MT : 1)];
+          const bf16x8 av = __builtin_bit_cast(bf16x8, araw);
+          const bf16x8 br = pr[TB[term]], bi = pi[TB[term]];
           accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, br, accr[a][b], 0, 0, 0);
           acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bi, acci[a][b], 0, 0, 0);
         }
@@ -563,7 +599,9 @@ size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg) {
 }
 
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
-                                 int nt, long vol, int ncfg, float* partial, hipStream_t s) {
+                                 int nt, long vol, int ncfg, float* partial, hipStream_t s,
+                                 int terms) {
+  if (terms != 3 && terms != 6) return hipErrorInvalidValue;
   if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
     return hipErrorInvalidValue;
   const int mt = (ns + 15) / 16, ntt = (nt + 15) / 16;
@@ -571,12 +609,16 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
   const int nchunks = (int)((K + kc - 1) / kc);
   const __bf16* hi = (const __bf16*)phi_split;
   const __bf16* lo = hi + 2 * (long)ns * vol;
+  const __bf16* lo2 = lo + 2 * (long)ns * vol;
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
   // one instantiation per (sink tiles, orbital tiles), so only the used accumulators exist
 #define NRC_OV_CASE(A, B)                                                                    \
-  if (mt == A && ntt == B) {                                                                 \
-    hipLaunchKernelGGL((k_overlap_bf16x3<A, B>), grid, block, 0, s, hi, lo, psi, partial, ns, \
-                       nt, vol, kc);                                                         \
+  if (mt == A && ntt == B && terms == 3) {                                                   \
+    hipLaunchKernelGGL((k_overlap_bf16x3<A, B, 3>), grid, block, 0, s, hi, lo, lo, psi,      \
+                       partial, ns, nt, vol, kc);                                            \
+  } else if (mt == A && ntt == B) {                                                          \
+    hipLaunchKernelGGL((k_overlap_bf16x3<A, B, 6>), grid, block, 0, s, hi, lo, lo2, psi,     \
+                       partial, ns, nt, vol, kc);                                            \
   }
 #define NRC_OV_ROW(A) NRC_OV_CASE(A, 1) NRC_OV_CASE(A, 2) NRC_OV_CASE(A, 3) NRC_OV_CASE(A, 4)
   NRC_OV_ROW(1) NRC_OV_ROW(2) NRC_OV_ROW(3) NRC_OV_ROW(4)

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
       // FP64 pencil is live at a time
       auto chain = [&](D* w, uint32_t yz) {
         if (!first) fft_reg<double, L, +1>(w);
-        gather_pencil<double, L>(w, g, (int)yz, cfg, orb, a.norb);
+        if (a.gather) gather_pencil<double, L>(w, g, (int)yz, cfg, orb, a.norb);
         const rsrc_t rk = first ? rkx : rkk;
 #pragma unroll
         for (int x0 = 0; x0 < L; x0 += 8) {

@@ -217,6 +217,36 @@ def test_overlap_bf16x3_mfma_vs_oracle(ctx, orc, ns, nt, V, ncfg):
     assert np.array_equal(got, q[:, :, sites].transpose(0, 2, 1).astype(np.complex128))
 
 
+@pytest.mark.parametrize("ns,nt,V,ncfg", [(33, 33, 4096, 3), (7, 7, 1728, 2), (19, 14, 13824, 2),
+                                          (1, 1, 216, 1), (64, 64, 512, 2)])
+def test_overlap_bf16x6_mfma_vs_oracle(ctx, orc, ns, nt, V, ncfg):
+    """Split-BF16x6 (three-way splits, six products): every FP32 value is represented exactly,
+    so products carry ~2^-24 relative error and the result is FP32 accumulation error alone;
+    unit-vector sinks pick any FP32 value exactly."""
+    import nrcps_amd as na
+
+    print(f"[gpu] overlap bf16x6 ns={ns} nt={nt} V={V}", flush=True)
+    rng = np.random.default_rng(ns * 100 + V + 1)
+    phi = _rand_c(rng, (ns, V)).astype(np.complex64)
+    psi = _rand_c(rng, (ncfg, nt, V)).astype(np.complex64)
+    M = ctx.array((ncfg, ns, nt), np.complex128)
+    ctx.overlap(ctx.to_device(phi), ctx.to_device(psi), M, ns, nt, V, ncfg, na.OVERLAP_BF16X6)
+    got = M.download()
+    for c in range(ncfg):
+        ref = orc.overlap(phi.astype(np.complex128), psi[c].astype(np.complex128))
+        # FP32 accumulation alone: a chunk's partial sum grows like sqrt(K) and every MFMA
+        # step rounds it, so the error grows like V (measured ~1.6e-8 V for these random
+        # values); split-BF16x3 on the same inputs is ~10x worse, limited by its splits
+        assert np.abs(got[c] - ref).max() < 4e-8 * V, np.abs(got[c] - ref).max()
+    e = np.zeros((ns, V), np.complex64)
+    sites = rng.choice(V, ns, replace=False)
+    e[np.arange(ns), sites] = 1
+    q = _rand_c(rng, (ncfg, nt, V)).astype(np.complex64)  # any FP32 values
+    ctx.overlap(ctx.to_device(e), ctx.to_device(q), M, ns, nt, V, ncfg, na.OVERLAP_BF16X6)
+    got = M.download()
+    assert np.array_equal(got, q[:, :, sites].transpose(0, 2, 1).astype(np.complex128))
+
+
 @pytest.mark.parametrize("split", [0, 1])
 def test_ensemble_dense_sinks_match_gather(ctx, orc, split):
     """Dense plane-wave sinks (split-BF16x3 overlaps) reproduce the gathered measurement."""
@@ -242,7 +272,11 @@ def test_ensemble_dense_sinks_match_gather(ctx, orc, split):
     lb, pb, _ = b.run(13, 0, T)
     d = np.abs(la - lb).max()
     print(f"[gpu] dense-sink vs gather log|det| max diff (split {split}): {d:.2e}", flush=True)
-    assert d < 2e-4 and np.abs(pa - pb).max() < 2e-4
+    # k-space split: both evolve with the FP64 x-chain (dense sinks: one slice per launch, the
+    # GEMM on the stored (x, k_y, k_z) state with transformed sinks); x-space split: the same
+    # plain FP32 evolution, the dense sinks spread over V sites (FP32 accumulation)
+    tol = 5e-6 if split == 0 else 5e-5
+    assert d < tol and np.abs(pa - pb).max() < tol
 
 
 # ------------------------------------------------------------------------------ the path

@@ -98,6 +98,28 @@ def test_config3_plain_fp32_error_is_bounded(ctx, fp64_reference):
     assert np.median(rel) <= 1e-6
 
 
+def test_config3_dense_sinks_on_mfma_within_1e6_of_fp64(ctx, fp64_reference):
+    """The precision form as written: the overlaps as a dense GEMM on the matrix cores
+    (split-BF16x6, FP32 accumulation) instead of the plane-wave gather, every orbital evolved
+    (dense sinks read every orbital's state), FP32 evolution with the FP64 x-chain, FP64 LU."""
+    import nrcps_amd as na
+
+    print("[gpu] FP32 with dense plane-wave sinks (BF16x6 GEMM)", flush=True)
+    moms = _lowest(L, N)
+    V = L ** 3
+    sinks = np.zeros((N, V), np.complex128)
+    sinks[np.arange(N), (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]] = 1.0
+    e = na.Ensemble(ctx, L, moms, _xi_half(L), 0.4, NCONF, prec=na.PREC_F32, sinks=sinks)
+    la, ph, _ = e.run(SEED, 0, T)
+    e.close()
+    m64, c64 = _corr(*fp64_reference)
+    m32, c32 = _corr(la, ph)
+    rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+    print(f"[gpu]   max rel err of C(t) {rel.max():.2e} at t={rel.argmax()}, median "
+          f"{np.median(rel):.2e}", flush=True)
+    assert rel.max() <= TOL_C
+
+
 def test_free_gas_t128_known_answer(ctx):
     """A = 0: plane waves are eigenvectors, det M(t) = prod_n K(1/2)(k_n)^(2t) exactly
     (log|det| = -2t sum log xi_half(k_n)); FP32 with the FP64 x-chain, conjugate partners."""
