@@ -29,6 +29,7 @@ extern "C" {
 #define NRC_ERR_NOMEM (-4)       /* device allocation failed */
 #define NRC_ERR_SINGULAR (-5)    /* an all-zero matrix row (ludcmp.C:25-28 exits there) */
 #define NRC_ERR_BATCHED (-6)     /* the ensemble belongs to a batch (nrc_batch_create) */
+#define NRC_ERR_TOO_LARGE (-7)   /* sizes beyond a no-allocation call's fixed scratch */
 
 #define NRC_PREC_F32 0
 #define NRC_PREC_F64 1
@@ -93,6 +94,16 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf_dev, long vol, int prec);
  *      for ncfg batches of nt orbitals (stride vol), complex FP64 output. */
 int nrc_overlap(nrc_context* ctx, const void* phi_dev, const void* psi_dev, double* M_dev,
                 int ns, int nt, long vol, int ncfg, int mode);
+/* nrc_overlap for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The same kernels and results as nrc_overlap, with the split-BF16 modes' workspace (split
+ * sinks and per-chunk partial tiles) in a fixed 64 MiB device array that each GPU holds from
+ * nrc_init until the process ends; it calls no allocator.  NRC_ERR_TOO_LARGE when the
+ * workspace of (ns, nt, vol, ncfg) exceeds 64 MiB (e.g. ns = nt = 33 at vol = 32768 fits
+ * about 170 configurations per call).  Calls on one device share the array: calls that
+ * overlap in time (other streams, other contexts of the device) race. */
+int nrc_overlap_noalloc(nrc_context* ctx, const void* phi_dev, const void* psi_dev,
+                        double* M_dev, int ns, int nt, long vol, int ncfg, int mode);
 
 /* ---- the pair-sink matrix: TwoBody::Run(Float*, Float*) and Run(Float**, Float**,
  *      complex<Float>*, int) (include/two_body.h:31-35; src/two_body/noarch/two_body_run.C:

@@ -32,7 +32,7 @@ EXPORTS = [
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
-    "nrc_ensemble_set_accuracy", "nrc_batch_captures",
+    "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -80,6 +80,7 @@ def load_library() -> C.CDLL:
         "nrc_mul_complex": [vp, vp, vp, i, C.c_long, i],
         "nrc_plus_one_div_vol": [vp, vp, C.c_long, i],
         "nrc_overlap": [vp, vp, vp, vp, i, i, C.c_long, i, i],
+        "nrc_overlap_noalloc": [vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_pair_overlap": [vp, vp, vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_lu_det": [vp, vp, i, i, i, i, vp, vp, vp, vp],
         "nrc_ludcmp": [vp, vp, i, i, vp, vp, vp], "nrc_ref_abs": [vp, vp, vp, C.c_long],

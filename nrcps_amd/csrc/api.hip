@@ -18,6 +18,16 @@ struct nrc_context {
   size_t work_bytes = 0;
 };
 
+// The scratch of the no-allocation overlap (nrc_overlap_noalloc): a fixed device array that
+// every GPU the library runs on holds until the process ends.  Host code never uses the
+// array's name as an address (on the host it is a stand-in symbol): nrc_init looks up each
+// device's own copy with hipGetSymbolAddress, with that device current, and keeps the address
+// in g_scratch_dev[device]; that address is what kernels and copies get.
+constexpr size_t NRC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NRC_SCRATCH_BYTES];
+constexpr int NRC_MAX_DEVICES = 64;
+static void* g_scratch_dev[NRC_MAX_DEVICES] = {nullptr};
+
 // the context workspace, grown to at least `bytes` (synchronises before a reallocation)
 static void* ctx_workspace(nrc_context* c, size_t bytes) {
   if (bytes <= c->work_bytes) return c->work;
@@ -67,6 +77,7 @@ const char* nrc_error_string(int code) {
     case NRC_ERR_NOMEM: return "device allocation failed";
     case NRC_ERR_SINGULAR: return "singular matrix (all-zero row)";
     case NRC_ERR_BATCHED: return "ensemble belongs to a batch";
+    case NRC_ERR_TOO_LARGE: return "sizes exceed the fixed scratch of the no-allocation call";
     default: return "unknown error";
   }
 }
@@ -86,6 +97,15 @@ int nrc_init(int device, nrc_context** ctx) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return NRC_ERR_HIP;
+  }
+  if (device < NRC_MAX_DEVICES && !g_scratch_dev[device]) {  // once per device, device current
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      hipStreamDestroy(c->stream);
+      delete c;
+      return NRC_ERR_HIP;
+    }
+    g_scratch_dev[device] = ptr;
   }
   *ctx = c;
   return NRC_OK;
@@ -272,6 +292,30 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
   const size_t bytes = overlap_bf16x3_workspace(ns, nt, vol, ncfg);
   void* w = ctx_workspace(ctx, bytes);
   if (!w) return NRC_ERR_NOMEM;
+  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
+                        mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
+}
+
+// nrc_overlap without allocating: the split sinks and the partial tiles of the split-BF16
+// modes live in the device's fixed scratch array (NRC_SCRATCH_BYTES), found at nrc_init.
+// Every kernel writes the scratch before reading it, so nothing is cleared; calls that
+// overlap in time on one device share the array and race.
+int nrc_overlap_noalloc(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns,
+                        int nt, long vol, int ncfg, int mode) {
+  if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
+  if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {  // these use no scratch
+    const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
+    NRC_HIP(nrc::launch_overlap_dense(prec, phi, psi, (double2*)M, ns, nt, vol, ncfg,
+                                      ctx->stream));
+    return NRC_OK;
+  }
+  if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
+  if (ncfg == 0) return NRC_OK;
+  if (overlap_bf16x3_workspace(ns, nt, vol, ncfg) > NRC_SCRATCH_BYTES)
+    return NRC_ERR_TOO_LARGE;
+  void* w = ctx->device < NRC_MAX_DEVICES ? g_scratch_dev[ctx->device] : nullptr;
+  if (!w) return NRC_ERR_HIP;
   return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
                         mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
 }
