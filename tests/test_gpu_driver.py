@@ -2,6 +2,7 @@
 against the reference's class API) on a small production-style argument set, checked
 against the oracle fed with the same ranlxs stream, kinetic table, smeared interaction and
 momenta.  FP64 build: agreement to 1e-10 relative per (n, t)."""
+import ctypes as C
 import os
 import subprocess
 import tempfile
@@ -67,6 +68,57 @@ def test_slater_det_driver_matches_oracle(orc):
     assert len(state) == 4 + 8 * 105
     st = np.frombuffer(state[4:], dtype=np.int64)
     assert st[0] == 105 and st[101] == 397
+
+
+def test_slater_det_driver_resumes_from_rng_state_file(orc):
+    """SEED_TYPE_FILE (src/random/random.C:78-87,157-181): a first run writes the ranlxs state
+    at the start of every block (RandomType, then long[105]); the state left after two blocks
+    is the oracle stream's after (T + 1) V draws (the first field is drawn by the Lattice
+    constructor).  A second run that reads it continues that stream: its block matches the
+    oracle's run from the same state."""
+    assert os.path.exists(DRIVER), "the slater_det driver is not built"
+    L, T = 8, 6
+    V = L ** 3
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in ARGS.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        print("[gpu] slater_det driver: two blocks, then a resumed block", flush=True)
+        r = subprocess.run([DRIVER], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        state = np.frombuffer(open(os.path.join(tmp, "rngs", "ranlxs2.0"), "rb").read()[4:],
+                              dtype=np.int64).copy()
+        with open(os.path.join(tmp, "args", "random.arg"), "w") as f:
+            f.write("RANDOM_TYPE_RANLXS2\nSEED_TYPE_FILE\n7012\nrngs/ranlxs2\n")
+        with open(os.path.join(tmp, "args", "evo.arg"), "w") as f:
+            f.write("0\n1\n1\n")
+        r = subprocess.run([DRIVER], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        cut2 = (1.0 * PI) * (1.0 * PI)
+        n, moms, par, om = orc.momenta(L, orc.DISP_PERFECT, 1.18343, 0.3, cutoff_sq=cut2)
+        got = []
+        for k in range(1, n + 1):
+            rows = [list(map(float, line.split()))
+                    for line in open(os.path.join(tmp, "results", f"slater.{k}")) if line.strip()]
+            assert len(rows) == 3  # two blocks, then the resumed one appended
+            got.append(np.array(rows[2]).reshape(T, 2))
+        got = np.stack(got)  # [n][t][re, im]
+    # the oracle stream: one configuration (T V draws), then the next block's first field
+    rng = orc.ranlxs(2, 7012)
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    orc.run_config(L, n, T, moms, kin, smear=f, minors=True, field="ranlxs", rng=rng)
+    orc.lib().orc_ranlxs_z2_field(C.byref(rng), V, np.zeros(V, np.int8))
+    st = np.zeros(105, np.int64)
+    orc.lib().orc_ranlxs_get_state(C.byref(rng), st)
+    assert np.array_equal(st, state)
+    ref = orc.run_config(L, n, T, moms, kin, smear=f, minors=True, field="ranlxs", rng=rng)
+    d2 = np.exp(2 * ref["logabs"])
+    for k in range(n):
+        np.testing.assert_allclose(got[k, :, 0], d2[:, k], rtol=1e-10)
 
 
 DRIVER3 = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det3")
@@ -148,3 +200,77 @@ def test_slater_det3_driver_matches_oracle(orc, trapped):
             for k in range(6):
                 np.testing.assert_allclose(z[k, m, j], ref["logabs"][:, m] ** (k + 1),
                                            rtol=1e-9, atol=1e-12)
+
+
+DRIVER2 = os.path.join(ROOT, "nrcps_amd", "drivers", "slater_det2")
+
+
+def test_slater_det2_driver_matches_oracle(orc):
+    """The two-species pair-sink driver (SlaterDet2 + TwoBody on the class layer, the loop of
+    production/slater_det2_untrapped/main-cum.C): momentum sources, species-2 orbitals
+    reindexed by the opposite-parity map, Run(n+1, -1, -1) and Run(n+1, -1, first index of
+    shell s) per slice, against determinants of the oracle's pair matrix and one-body
+    overlaps on the same ranlxs fields (src/slater_det/slater_det2.C:11-150)."""
+    assert os.path.exists(DRIVER2), "the slater_det2 driver is not built"
+    L, T, ncfg = 8, 6, 2
+    V = L ** 3
+    args = dict(ARGS)
+    args["two_body.arg"] = ("WAVEFUNC_TYPE_TGND\nDISPERSION_TYPE_STANDARD\n"
+                            "DISPERSION_TYPE_STANDARD\n1.0\n1.0\n0.176776695\nresults/two\n")
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in args.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        with open(os.path.join(tmp, "args", "two_body_def.arg"), "w") as f:
+            f.write("1 0 0 0.25\n0 2 7 -0.125\n")  # Deform: additions to the wave function
+        print("[gpu] running the slater_det2 driver (L=8, T=6, 2 configurations)", flush=True)
+        r = subprocess.run([DRIVER2], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+        def rows(name):
+            return np.array([list(map(float, line.split()))
+                             for line in open(os.path.join(tmp, "results", name))
+                             if line.strip()])
+        cut2 = PI * PI
+        n, moms, par, om = orc.momenta(L, orc.DISP_PERFECT, 1.18343, 0.3, cutoff_sq=cut2)
+        starts = [0] + [i for i in range(1, n) if om[i] != om[i - 1]]  # AccShellCount(s - 1)
+        even = np.stack([rows(f"slater2_even.{k + 1}").reshape(ncfg, T, 10) for k in range(n)])
+        odd = {(s, k): rows(f"slater2_odd_{s}.{k + 1}").reshape(ncfg, T, 2)
+               for s, t0 in enumerate(starts) for k in range(t0, n)}
+        srcs = open(os.path.join(tmp, "results", "sources")).read().splitlines()
+    assert len(srcs) == n and srcs[1].startswith(f"1 ({par[1]}): ")
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    assert rc == 0
+    w = orc.two_body_wavefunc(L, orc.WAVEFUNC_TGND, 0.176776695, cutoff_sq=cut2)
+    for k1, k2, k3, re in ((1, 0, 0, 0.25), (0, 2, 7, -0.125)):
+        w[(k1 * L + k2) * L + k3] += re
+    opp = orc.opposite_parity_index(L)
+    rng = orc.ranlxs(2, 7012)
+    for j in range(ncfg):
+        phi = np.zeros((T, V), np.int8)
+        for t in range(T):  # the Lattice's draws: site order, one field per slice
+            orc.lib().orc_ranlxs_z2_field(C.byref(rng), V, phi[t])
+        mp = orc.run_config(L, n, T, moms, kin, smear=f, field="array", phi=phi, pair_w=w,
+                            pair_opp=opp, want_M=True)["M"]
+        m1 = orc.run_config(L, n, T, moms, kin, smear=f, field="array", phi=phi,
+                            want_M=True)["M"]  # [t][sink][orbital]
+        for t in range(T):
+            P = mp[t][:, par]  # phi[s][t] = TwoBody::Run(up_s, down_t), down_t = up_par(t)
+            for k in range(n):
+                want = np.linalg.det(P[:k + 1, :k + 1])
+                got = even[k, j, t, 0] + 1j * even[k, j, t, 1]
+                assert abs(got - want) <= 1e-9 * abs(want) + 1e-300, (j, t, k)
+                z1 = np.log(want.real)
+                assert even[k, j, t, 2] == pytest.approx(z1, rel=1e-9, abs=1e-12)
+                for s, t0 in enumerate(starts):
+                    if k < t0:
+                        continue
+                    Q = P[:k + 1, :k + 1].copy()
+                    Q[:, t0] = m1[t][t0, :k + 1]  # psi1[s + t0 N] = <sink_t0 | up_s>
+                    want = np.linalg.det(Q)
+                    got = odd[s, k][j, t, 0] + 1j * odd[s, k][j, t, 1]
+                    assert abs(got - want) <= 1e-9 * abs(want) + 1e-300, (j, t, k, s)
