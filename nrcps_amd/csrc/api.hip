@@ -442,6 +442,7 @@ struct nrc_ensemble {
   std::vector<int> h_col_vol, h_col_conj;
   bool pair_sym_ok = false;  // pair sinks: opp is the mirror and w even (partners allowed)
   void* trap_tab = nullptr;  // trap (nrc_ensemble_set_trap): (1 +- A) q / V, MUL_TABLE format
+  double* trap_sep = nullptr;  // a separable trap q = qx(x) qy(y) qz(z): [qx, qy, qz] (FP64)
   double* trap_q = nullptr;  // the trap q per site (FP64), for the smeared field
   double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
@@ -471,7 +472,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q,
-                  e->tab_x64, e->tab_k64};
+                  e->tab_x64, e->tab_k64, e->trap_sep};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (e->par_dev) hipFree(e->par_dev);
@@ -1157,15 +1158,18 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
 // the run's last launch stores the folded k-space state
 static bool ens_xchain(const nrc_ensemble* e, bool single) {
   return single && e->d.prec == NRC_PREC_F32 && e->acc == NRC_ACC_X64 &&
-         e->d.split == NRC_SPLIT_KSPACE && !e->smear && !e->pair && !e->trap_tab &&
+         e->d.split == NRC_SPLIT_KSPACE && !e->smear && (!e->trap_tab || e->trap_sep) &&
          nrc::slicex_supported(e->d.L);
 }
 
-// gathered plane-wave sinks: measured inside the launch (S slices); dense sinks: measured
-// beforehand by ens_measure, one slice per launch
+// gathered plane-wave sinks: measured inside the launch (S slices), the state left with its
+// x-DFT(+1) pending between launches; dense sinks: measured beforehand by ens_measure on that
+// state, one slice per launch; pair sinks: measured beforehand from the folded k-space state,
+// which every launch (one slice) starts from and stores
 static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, size_t stride,
                         bool last) {
-  const bool gather = e->sinks == nullptr;
+  const bool gather = e->sinks == nullptr && !e->pair;
+  const bool kspace = e->pair;  // every launch starts and ends in k-space
   const int N = e->d.n_orb;
   const long C = e->d.n_configs, V = e->V;
   const size_t per_t = (size_t)C * e->nm;
@@ -1176,8 +1180,10 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.L = e->d.L;
   a.grid = e->ctx->n_cu;
   a.nslices = S;
-  a.start_k = t == 0;
-  a.store_k = last;
+  a.start_k = t == 0 || kspace;
+  a.half0 = t == 0;
+  a.store_k = last || kspace;
+  a.trap_sep = e->trap_sep;
   a.tab_x = e->tab_x64;
   a.tab_k = e->tab_k64;
   a.bits = words;
@@ -1194,8 +1200,8 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
     EvScope sc(e, 0);
     NRC_HIP(nrc::launch_slicex(a, e->stream));
   }
-  e->folded = last;
-  e->pending_x = !last;
+  e->folded = last || kspace;
+  e->pending_x = !(last || kspace);
   if (!gather) return NRC_OK;
   EvScope sc(e, 2);
   nrc::LuOpts o;
@@ -1271,8 +1277,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   if (ens_xchain(e, single)) {  // FP32 state, FP64 x-chain, measurements inside the launches
     e->pending_x = false;
     for (int t = 0; t < T;) {
-      const int S = e->sinks ? 1 : std::min(nrc::SLICE_FUSE_MAX, T - t);
-      if (e->sinks) NRC_TRY(ens_measure(e, t));  // dense sinks: the GEMM before the slice
+      const int S = (e->sinks || e->pair) ? 1 : std::min(nrc::SLICE_FUSE_MAX, T - t);
+      if (e->sinks || e->pair) NRC_TRY(ens_measure(e, t));  // dense or pair sinks: before
       if (!host_words) {
         EvScope sc(e, 3);
         for (int k = 0; k < S; ++k)
@@ -1486,9 +1492,33 @@ extern "C" int nrc_ensemble_set_trap(nrc_ensemble* e, const double* q) {
   NRC_HIP(hipStreamSynchronize(e->stream));
   if (e->trap_tab) hipFree(e->trap_tab);
   if (e->trap_q) hipFree(e->trap_q);
+  if (e->trap_sep) hipFree(e->trap_sep);
   e->trap_tab = nullptr;
   e->trap_q = nullptr;
+  e->trap_sep = nullptr;
   if (!q) return NRC_OK;
+  {  // separable, q(x,y,z) = qx(x) qy(y) qz(z) to 1e-14 relative (the harmonic EXP trap is):
+     // qx = q(x,0,0)/q(0,0,0), qy = q(0,y,0)/q(0,0,0), qz = q(0,0,z)
+    const int L = e->d.L;
+    const long P = (long)L * L;
+    const double q0 = q[0];
+    std::vector<double> sep(3 * (size_t)L);
+    for (int i = 0; i < L; ++i) {
+      sep[i] = q[i * P] / q0;
+      sep[L + i] = q[i * L] / q0;
+      sep[2 * L + i] = q[i];
+    }
+    bool ok = q0 != 0.0;
+    for (long i = 0; i < V && ok; ++i) {
+      const double r = sep[i / P] * sep[L + (i / L) % L] * sep[2 * L + i % L];
+      ok = std::fabs(r - q[i]) <= 1e-14 * std::fabs(q[i]);
+    }
+    if (ok) {
+      NRC_TRY(dev_alloc(&e->trap_sep, sep.size()));
+      NRC_HIP(hipMemcpy(e->trap_sep, sep.data(), sep.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+    }
+  }
   std::vector<double> h(2 * V);
   const double a0 = 1.0 / (double)V, a1 = e->d.A / (double)V;
   for (long i = 0; i < V; ++i) {

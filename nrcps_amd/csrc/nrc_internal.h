@@ -96,13 +96,14 @@ struct SliceArgs {
 constexpr int SLICE_FUSE_MAX = 4;  // slices per launch of the persistent slice kernel
 
 // The k-space split slice loop with FP32 state and the x-chain in FP64 (slice_ext.hip):
-// `nslices` slices per launch of the persistent kernel.  The volumes hold the k-space
-// sources (start_k) or the state of the previous launch in (x, k_y, k_z) coordinates with its
-// last x-DFT(+1) pending; slice s of the launch gathers its measurement (the sinks of
-// g, g.M advanced by s * m_stride; sink_scale except on the sources), multiplies by the
-// FP64 table tab_x = K(1/2) (sources) or tab_k = K, and evolves with the field words at
-// bits + s * bits_stride: (a0 + a1 phi) in x-space.  store_k: finish with the x-DFT(+1) and
-// store the folded k-space state.
+// `nslices` slices per launch of the persistent kernel.  The volumes hold a k-space state
+// (start_k: the sources when half0, else the folded state) or the state of the previous
+// launch in (x, k_y, k_z) coordinates with its last x-DFT(+1) pending; slice s of the launch
+// gathers its measurement (the sinks of g, g.M advanced by s * m_stride; sink_scale except
+// on the sources), multiplies by the FP64 table tab_x = K(1/2) (sources) or tab_k = K, and
+// evolves with the field words at bits + s * bits_stride: (a0 + a1 phi) in x-space, times
+// the separable trap when trap_sep is set.  store_k: finish with the x-DFT(+1) and store the
+// folded k-space state.
 struct SliceXArgs {
   void* psi = nullptr;  // float2 volumes
   long nvol = 0;
@@ -111,6 +112,7 @@ struct SliceXArgs {
   int grid = 0;  // CUs
   int nslices = 1;
   int start_k = 0, store_k = 0;
+  int half0 = 0;   // slice 0 acts on the sources: factor K(1/2), gather without sink_scale
   int gather = 1;  // gather the plane-wave sinks of g (0: measured elsewhere)
   const double* tab_x = nullptr;
   const double* tab_k = nullptr;
@@ -118,6 +120,7 @@ struct SliceXArgs {
   int nw = 0;
   long bits_stride = 0;
   double a0 = 0.0, a1 = 0.0;
+  const double* trap_sep = nullptr;  // separable trap q = qx qy qz: [qx(L), qy(L), qz(L)]
   GatherSpec g;
   long m_stride = 0;
 };

@@ -177,6 +177,28 @@ __device__ __forceinline__ void apply_mul(C2<T>* v, const MulSpec& m, long cfg, 
   for (int i = 0; i < L; ++i) v[i] = scale_tab<T>(v[i], tab[i * stride + off]);
 }
 
+// w[y] *= (a0 + a1 phi) q(site), site = off + y L, phi from the field words in LDS, with a trap: w[y] *= (a0 + a1 phi) q(site) read from the two factor tables
+// (phi = +1 first, MUL_TABLE format), four sites at a time so that few loads are in flight
+template <typename T, int L>
+__device__ __forceinline__ void mul_field_trap(C2<T>* w, const uint32_t* bl, uint32_t off,
+                                               rsrc_t rt, uint32_t V) {
+  using E = TabEntry<T>;
+  constexpr int G = 4;
+#pragma unroll
+  for (int y0 = 0; y0 < L; y0 += G) {
+    E e[G];
+#pragma unroll
+    for (int y = y0; y < y0 + G && y < L; ++y) {
+      const uint32_t site = off + (uint32_t)(y * L);
+      const bool up = (bl[site >> 5] >> (site & 31u)) & 1u;
+      e[y - y0] = bload<E>(rt, (site + (up ? 0u : V)) * (uint32_t)sizeof(E), 0u);
+    }
+#pragma unroll
+    for (int y = y0; y < y0 + G && y < L; ++y) w[y] = scale_tab<T>(w[y], e[y - y0]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <typename T, int L>
 __device__ __forceinline__ void gather_pencil(const C2<T>* v, const GatherSpec& g, int pencil,
                                               long cfg, int orb, int norb) {

@@ -39,27 +39,53 @@ __device__ __forceinline__ void mul_field_lds_x(C2<T>* w, const uint32_t* bl, ui
   }
 }
 
-// the z-DFT of one LDS row in place: FP64 registers rounded once (ZD), or FP32
-template <int L, int S, bool ZD>
-__device__ __forceinline__ void z_row(C2<float>* row) {
+// the z-DFT of one LDS row in place: FP64 registers rounded once (ZD), or FP32; Q: then
+// times q[z]
+template <int L, int S, bool ZD, bool Q>
+__device__ __forceinline__ void z_row(C2<float>* row, const double* q) {
   if constexpr (ZD) {
     C2<double> w[L];
 #pragma unroll
     for (int z = 0; z < L; ++z) w[z] = C2<double>{(double)row[z].x, (double)row[z].y};
     fft_reg<double, L, S>(w);
 #pragma unroll
-    for (int z = 0; z < L; ++z) row[z] = C2<float>{(float)w[z].x, (float)w[z].y};
+    for (int z = 0; z < L; ++z) {
+      if constexpr (Q) w[z] = C2<double>{w[z].x * q[z], w[z].y * q[z]};
+      row[z] = C2<float>{(float)w[z].x, (float)w[z].y};
+    }
   } else {
     C2<float> w[L];
 #pragma unroll
     for (int z = 0; z < L; ++z) w[z] = row[z];
     fft_reg<float, L, S>(w);
 #pragma unroll
-    for (int z = 0; z < L; ++z) row[z] = w[z];
+    for (int z = 0; z < L; ++z) {
+      if constexpr (Q) {
+        const double f = q[z];
+        const float h = (float)f, l = (float)(f - (double)h);
+        w[z] = scale2<float>(w[z], h, l);
+      }
+      row[z] = w[z];
+    }
   }
 }
 
-template <int L, int PPT, int SMAX, bool ZD>
+// w[y] *= (a0 + a1 phi(site)) qy[y]: the factor formed in FP64 and applied as a compensated
+// (hi, lo) pair
+template <int L>
+__device__ __forceinline__ void mul_field_lds_q(C2<float>* w, const uint32_t* bl, uint32_t off,
+                                                double dp, double dm, const double* qy) {
+#pragma unroll
+  for (int y = 0; y < L; ++y) {
+    const uint32_t site = off + (uint32_t)(y * L);
+    const bool up = (bl[site >> 5] >> (site & 31u)) & 1u;
+    const double f = (up ? dp : dm) * qy[y];
+    const float h = (float)f, l = (float)(f - (double)h);
+    w[y] = scale2<float>(w[y], h, l);
+  }
+}
+
+template <int L, int PPT, int SMAX, bool ZD, bool TRAP>
 __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
   using C = C2<float>;
   using D = C2<double>;
@@ -77,6 +103,12 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
   const float pl = (float)(dp - (double)ph), ml = (float)(dm - (double)mh);
   const rsrc_t rkx = make_rsrc(a.tab_x, (uint32_t)(V * sizeof(double)));
   const rsrc_t rkk = make_rsrc(a.tab_k, (uint32_t)(V * sizeof(double)));
+  // separable trap q = qx(x) qy(y) qz(z): qx after the x-DFT(-1) (x in position space), qz
+  // after the z-DFT(-1) rows, qy with the field; the index is the register index, the same in
+  // every lane
+  const double* qx = a.trap_sep;
+  const double* qy = a.trap_sep + L;
+  const double* qz = a.trap_sep + 2 * L;
   C* psi = reinterpret_cast<C*>(a.psi);
   for (long vol = blockIdx.x; vol < a.nvol; vol += gridDim.x) {
     const long cfg = vol / a.norb;
@@ -108,17 +140,18 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
       for (int k = 0; k < WPT; ++k)
         if (tid + k * NT < NW) bl[sl][tid + k * NT] = wb[sl][k];
     for (int sl = 0; sl < ns; ++sl) {
-      const bool first = a.start_k && sl == 0;
+      const bool first = a.start_k && sl == 0;  // no x-DFT(+1): the state is in k-space
+      const bool half = a.half0 && sl == 0;     // the sources: K(1/2), unscaled gather
       GatherSpec g = a.g;
       g.M = a.g.M + (long)sl * a.m_stride;
-      g.sink_scale = first ? nullptr : a.g.sink_scale;
+      g.sink_scale = half ? nullptr : a.g.sink_scale;
       // the x-chain in FP64, pencil by pencil; with two pencils per thread the idle one is
       // parked in LDS (free between rounds; thread-private slots [x][tid]) so that only one
       // FP64 pencil is live at a time
       auto chain = [&](D* w, uint32_t yz) {
         if (!first) fft_reg<double, L, +1>(w);
         if (a.gather) gather_pencil<double, L>(w, g, (int)yz, cfg, orb, a.norb);
-        const rsrc_t rk = first ? rkx : rkk;
+        const rsrc_t rk = half ? rkx : rkk;
 #pragma unroll
         for (int x0 = 0; x0 < L; x0 += 8) {
           double f[8];
@@ -130,6 +163,10 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
             w[x] = D{w[x].x * f[x - x0], w[x].y * f[x - x0]};
         }
         fft_reg<double, L, -1>(w);
+        if constexpr (TRAP) {
+#pragma unroll
+          for (int x = 0; x < L; ++x) w[x] = D{w[x].x * qx[x], w[x].y * qx[x]};
+        }
       };
       if constexpr (PPT == 1) {
         D w[L];
@@ -179,7 +216,7 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
           for (int xl = 0; xl < H; ++xl) sm[(xl * L + y) * LP + z] = v[q][h * H + xl];
         }
         __syncthreads();
-        z_row<L, -1, ZD>(sm + (rx * L + rq) * LP);
+        z_row<L, -1, ZD, TRAP>(sm + (rx * L + rq) * LP, qz);
         __syncthreads();
         {  // columns along y with the field
           C w[L];
@@ -188,13 +225,16 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
           for (int y = 0; y < L; ++y) w[y] = col[y * LP];
           fft_reg<float, L, -1>(w);
           const int x = h * H + rx;
-          mul_field_lds_x<float, L>(w, b, (uint32_t)(x * P + rq), ph, pl, mh, ml);
+          if constexpr (TRAP)
+            mul_field_lds_q<L>(w, b, (uint32_t)(x * P + rq), dp, dm, qy);
+          else
+            mul_field_lds_x<float, L>(w, b, (uint32_t)(x * P + rq), ph, pl, mh, ml);
           fft_reg<float, L, +1>(w);
 #pragma unroll
           for (int y = 0; y < L; ++y) col[y * LP] = w[y];
         }
         __syncthreads();
-        z_row<L, +1, ZD>(sm + (rx * L + rq) * LP);
+        z_row<L, +1, ZD, false>(sm + (rx * L + rq) * LP, qz);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -225,21 +265,27 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
   }
 }
 
-template <int L, int PPT, bool ZD = true>
-static hipError_t launch_sx(const SliceXArgs& a, hipStream_t s) {
+template <int L, int PPT, bool ZD, bool TRAP>
+static hipError_t launch_sx_t(const SliceXArgs& a, hipStream_t s) {
   constexpr int SMAX = SLICE_FUSE_MAX;
-  if (a.nslices < 1 || a.nslices > SMAX) return hipErrorInvalidValue;
   static int per_cu = 0;
   if (per_cu == 0) {
     int nb = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, k_slicex<L, PPT, SMAX, ZD>, L * L / PPT, 0);
+        &nb, k_slicex<L, PPT, SMAX, ZD, TRAP>, L * L / PPT, 0);
     per_cu = (e != hipSuccess || nb < 1) ? 1 : nb;
   }
   long grid = (long)a.grid * per_cu;
   if (grid > a.nvol) grid = a.nvol;
-  hipLaunchKernelGGL((k_slicex<L, PPT, SMAX, ZD>), dim3((unsigned)grid), dim3(L * L / PPT), 0, s, a);
+  hipLaunchKernelGGL((k_slicex<L, PPT, SMAX, ZD, TRAP>), dim3((unsigned)grid),
+                     dim3(L * L / PPT), 0, s, a);
   return hipGetLastError();
+}
+
+template <int L, int PPT, bool ZD = true>
+static hipError_t launch_sx(const SliceXArgs& a, hipStream_t s) {
+  if (a.nslices < 1 || a.nslices > SLICE_FUSE_MAX) return hipErrorInvalidValue;
+  return a.trap_sep ? launch_sx_t<L, PPT, ZD, true>(a, s) : launch_sx_t<L, PPT, ZD, false>(a, s);
 }
 
 bool slicex_supported(int L) { return L == 8 || L == 12 || L == 16 || L == 24 || L == 32; }

@@ -169,28 +169,6 @@ __device__ __forceinline__ void mul_field_lds(C2<T>* w, const uint32_t* bl, uint
   }
 }
 
-// the same with a trap: w[y] *= (a0 + a1 phi) q(site) read from the two factor tables
-// (phi = +1 first, MUL_TABLE format), four sites at a time so that few loads are in flight
-template <typename T, int L>
-__device__ __forceinline__ void mul_field_trap(C2<T>* w, const uint32_t* bl, uint32_t off,
-                                               rsrc_t rt, uint32_t V) {
-  using E = TabEntry<T>;
-  constexpr int G = 4;
-#pragma unroll
-  for (int y0 = 0; y0 < L; y0 += G) {
-    E e[G];
-#pragma unroll
-    for (int y = y0; y < y0 + G && y < L; ++y) {
-      const uint32_t site = off + (uint32_t)(y * L);
-      const bool up = (bl[site >> 5] >> (site & 31u)) & 1u;
-      e[y - y0] = bload<E>(rt, (site + (up ? 0u : V)) * (uint32_t)sizeof(E), 0u);
-    }
-#pragma unroll
-    for (int y = y0; y < y0 + G && y < L; ++y) w[y] = scale_tab<T>(w[y], e[y - y0]);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
 // Persistent, software-pipelined k_slice1 for the k-space split (kinetic factor, field, no
 // trailing factor): one workgroup per CU walks the volumes blockIdx.x, + gridDim.x, ...  A
 // volume is loaded, transformed and stored as in k_slice1, but

@@ -6,6 +6,7 @@
 // before each slice, results/slater.<n> holding one line of `%.15e %.15e ` pairs (re, im of
 // the accumulated |det|^2 per t, divided by the block size) per block, and the RNG state in
 // <file_stem>.<rank> before each block.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -39,10 +40,24 @@ int main() {
   PropagatorArg prop_arg("args/propagator.arg");
   OneBodyArg one_body_arg("args/one_body.arg");
   MomentaArg momenta_arg("args/momenta.arg");
-  // the sinks follow the kinetic operator's dispersion and mass
-  momenta_arg.dispersion_type = kinetic_arg.dispersion_type;
-  momenta_arg.mass = kinetic_arg.mass;
   if (evo_arg.unload_period < 1) ERR.General(fname, "evo_arg.unload_period must be positive.");
+  // production/slater_det/main.C:74-95: momentum sources, and the sinks follow the kinetic
+  // operator's dispersion and mass
+  if (one_body_arg.source_type != SOURCE_TYPE_MOM) {
+    VRB.Warn(fname, "one_body_arg.source_type not supported:");
+    VRB.Warn(fname, "\tusing SOURCE_TYPE_MOM instead.");
+    one_body_arg.source_type = SOURCE_TYPE_MOM;
+  }
+  if (momenta_arg.dispersion_type != kinetic_arg.dispersion_type) {
+    VRB.Warn(fname, "momenta_arg.dispersion_type and prop_arg.dispersion_type disagree:");
+    VRB.Warn(fname, "\tSetting momenta_arg.dispersion_type to prop_arg.dispersion_type.");
+    momenta_arg.dispersion_type = kinetic_arg.dispersion_type;
+  }
+  if (momenta_arg.mass != kinetic_arg.mass) {
+    VRB.Warn(fname, "momenta_arg.mass and prop_arg.mass disagree:");
+    VRB.Warn(fname, "\tSetting momenta_arg.mass to prop_arg.mass.");
+    momenta_arg.mass = kinetic_arg.mass;
+  }
 
   VRB.SetLevel(verbose_arg);
   GJP.Initialize(do_arg);
@@ -55,6 +70,15 @@ int main() {
   Momenta momenta(momenta_arg);
   const int N = momenta.MomentaCount();
   const int T = GJP.Tsites();
+  if (!Comms::Rank()) {  // results/sources (main.C:122-140): index (partner): k log(1 + eps)
+    Dispersion dispersion(kinetic_arg.dispersion_type, kinetic_arg.mass, CUTOFF_TYPE_HARD);
+    std::ofstream file("results/sources", std::ios_base::app);
+    for (int n = 0; n < N; ++n) {
+      const std::vector<int> k = momenta.GetMomentum(n);
+      file << n << " (" << momenta.OppositeParityIndex(n) << "): " << k[0] << " " << k[1]
+           << " " << k[2] << " " << std::log(1 + dispersion.Get(k[0], k[1], k[2])) << "\n";
+    }
+  }
   std::vector<OneBody*> sinks(N);
   for (int n = 0; n < N; ++n) {
     const std::vector<int> k = momenta.GetMomentum(n);

@@ -14,11 +14,12 @@
 // with the powers log(Re det)^k, k = 1 .. 8, of the accumulated values beside each; one line
 // of `%.15e` values per block, per t: corr re, corr im, z1 .. z8 (the odd files: the plain
 // corr re, im, then the positive-part corr re, im and z1 .. z8 over unload_period minus the
-// number of non-positive values).  results/sources lists n (opposite-parity index): k1 k2 k3
-// log(1 + eps(k)).
+// number of non-positive values).  results/sources lists "n (opposite-parity index): k1 k2 k3
+// log(1 + eps(k))" with the stream's default formatting.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <vector>
 
 #include "arg.h"
@@ -96,15 +97,14 @@ int main() {
   const int N = momenta.MomentaCount();
   const int T = GJP.Tsites();
   const int S = momenta.NumShells();
-  if (!Comms::Rank()) {
+  if (!Comms::Rank()) {  // the stream's default formatting, as main-cum.C:129-143 writes it
     Dispersion dispersion(kinetic_arg.dispersion_type, kinetic_arg.mass, CUTOFF_TYPE_HARD);
-    FILE* f = Fopen(IO_TYPE_ROOT, "results/sources", "a");
+    std::ofstream file("results/sources", std::ios_base::app);
     for (int n = 0; n < N; ++n) {
       const std::vector<int> k = momenta.GetMomentum(n);
-      Fprintf(f, "%d (%d): %d %d %d %.*e\n", n, momenta.OppositeParityIndex(n), k[0], k[1], k[2],
-              PREC, std::log(1 + dispersion.Get(k[0], k[1], k[2])));
+      file << n << " (" << momenta.OppositeParityIndex(n) << "): " << k[0] << " " << k[1]
+           << " " << k[2] << " " << std::log(1 + dispersion.Get(k[0], k[1], k[2])) << "\n";
     }
-    Fclose(f);
   }
 
   std::vector<OneBody*> one_bodies(N);

@@ -570,17 +570,19 @@ void OneBody::Set(std::string name) {
   Set();
 }
 
+// one_body.C:215-252: the unit vector (and its parity partner), then Set() as the reference
+// does, so that an SHO-typed source set by momentum is the SHO transform of those coefficients
 void OneBody::Set(int k1, int k2, int k3) {
   std::fill(psi.begin(), psi.end(), 0.0);
   psi[2 * Vector3ToIndex(Vector3(k1, k2, k3))] = 1.0;
-  Upload();
+  Set();
 }
 
 void OneBody::Set(int k1, int k2, int k3, int parity) {
   std::fill(psi.begin(), psi.end(), 0.0);
   psi[2 * Vector3ToIndex(Vector3(k1, k2, k3))] = 1.0;
   psi[2 * Vector3ToIndex(-1 * Vector3(k1, k2, k3))] = parity ? 1.0 : -1.0;
-  Upload();
+  Set();
 }
 
 void OneBody::Set(const Vector3& v) { Set(v.GetX(), v.GetY(), v.GetZ()); }

@@ -53,6 +53,11 @@ def test_slater_det_driver_matches_oracle(orc):
             got.append(np.array(rows).reshape(ncfg, T, 2))
         got = np.stack(got)  # [n][block][t][re, im]
         state = open(os.path.join(tmp, "rngs", "ranlxs2.0"), "rb").read()
+        sources = open(os.path.join(tmp, "results", "sources")).read().splitlines()
+    # results/sources (production/slater_det/main.C:122-140): "n (partner): k1 k2 k3 omega"
+    # with the stream's default formatting, omega = log(1 + eps(k)) of the kinetic dispersion
+    assert sources == [f"{i} ({par[i]}): {moms[i][0]} {moms[i][1]} {moms[i][2]} {om[i]:g}"
+                       for i in range(n)]
     kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
     cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
     f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)

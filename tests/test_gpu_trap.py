@@ -69,7 +69,8 @@ def test_trap_f32_l32_with_partners(ctx, orc):
         ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c, pot=pot)
         d = np.abs(la[:, c, 0] - ref["logabs"][:, 0]).max()
         print(f"[gpu] FP32 trapped, partners: max |d log|det|| {d:.2e}", flush=True)
-        assert d < 1e-4
+        # the separable trap inside the FP64 x-chain kernel, four slices per launch (~2e-7)
+        assert d < 1e-6
     ens.close()
 
 
@@ -126,7 +127,8 @@ def test_trap_f32_four_slices_per_launch(ctx, orc, L):
                              pot=pot)
         d = np.abs(la[:, c] - ref["logabs"]).max()
         print(f"[gpu] FP32 trapped L={L}, fused slices: max |d log|det|| {d:.2e}", flush=True)
-        assert d < 1e-4
+        # FP64 x-chain; L = 16 with FP64 z-rows (~4e-7), L = 24 with FP32 ones (~4e-6)
+        assert d < (2e-6 if L == 16 else 1e-5)
     ens.close()
 
 
