@@ -39,6 +39,7 @@ HOST_SOURCES = ["host/util.cpp", "host/physics.cpp"]
 DRIVER = os.path.join(PKG, "drivers", "slater_det")
 DRIVER3 = os.path.join(PKG, "drivers", "slater_det3")
 DRIVER2 = os.path.join(PKG, "drivers", "slater_det2")
+DRIVER_RW = os.path.join(PKG, "drivers", "one_body_reweight")
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
             f"-I{os.path.join(ROOT, 'include')}"]
@@ -75,7 +76,8 @@ def build_host() -> None:
           os.path.join(PKG, "drivers", "slater_det.cpp"), f"-L{PKG}", "-lnrcps", "-lnrcps_hip",
           "-Wl,-rpath,$ORIGIN/.."])
     os.replace(tmp, DRIVER)
-    for drv, src in ((DRIVER3, "slater_det3.cpp"), (DRIVER2, "slater_det2.cpp")):
+    for drv, src in ((DRIVER3, "slater_det3.cpp"), (DRIVER2, "slater_det2.cpp"),
+                     (DRIVER_RW, "one_body_reweight.cpp")):
         tmp = drv + ".tmp"
         _run([CXX, *CXXFLAGS, f"-I{os.path.join(ROOT, 'include', 'nrcps')}", "-o", tmp,
               os.path.join(PKG, "drivers", src), f"-L{PKG}", "-lnrcps", "-lnrcps_hip",

@@ -19,7 +19,7 @@ DISP_STANDARD, DISP_QUADRATIC, DISP_PERFECT = 0, 1, 2
 CUTOFF_NONE, CUTOFF_HARD = 0, 1
 INTER_NONE, INTER_XISQMINUSONE, INTER_ONEMINUSXIINVSQ = 0, 1, 2
 SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
-FIELD_PHILOX, FIELD_RANLXS, FIELD_ARRAY = 0, 1, 2
+FIELD_PHILOX, FIELD_RANLXS, FIELD_ARRAY, FIELD_REAL = 0, 1, 2, 3
 WAVEFUNC_NONE, WAVEFUNC_UNIFORM, WAVEFUNC_GND, WAVEFUNC_TGND, WAVEFUNC_PAIR1, WAVEFUNC_PAIR2 = range(6)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -41,7 +41,8 @@ class RunDesc(C.Structure):
                 ("A", C.c_double), ("smear", C.c_void_p), ("field_kind", C.c_int),
                 ("seed", C.c_uint64), ("config", C.c_uint32), ("rng", C.c_void_p),
                 ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p),
-                ("sources", C.c_void_p), ("pot", C.c_void_p)]
+                ("sources", C.c_void_p), ("pot", C.c_void_p), ("phi_real", C.c_void_p),
+                ("sinks", C.c_void_p), ("norms", C.c_void_p)]
 
 
 def build(force: bool = False) -> None:
@@ -87,6 +88,10 @@ def lib() -> C.CDLL:
         L.orc_ranlxs_set_state.argtypes = [C.POINTER(Ranlxs), _lp]
         L.orc_ranlxs_z2_field.argtypes = [C.POINTER(Ranlxs), C.c_int, _i8p]
         L.orc_ranlxs_z2_field.restype = None
+        L.orc_gauss.argtypes = [C.POINTER(Ranlxs), C.c_double]
+        L.orc_gauss.restype = C.c_double
+        L.orc_gauss_field.argtypes = [C.POINTER(Ranlxs), C.c_int, C.c_double, _dp]
+        L.orc_gauss_field.restype = None
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.orc_philox4x32_10.restype = None
         L.orc_z2_bits.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _u32p]
@@ -342,9 +347,17 @@ def sho_source(coeffs: np.ndarray, L: int, l1: float, l2: float, l3: float) -> n
     return out.view(np.complex128)
 
 
+def gauss_field(rng: Ranlxs, vol: int, shift: float = 0.0) -> np.ndarray:
+    """Lattice::Refresh(shift) for FIELD_TYPE_GAUSS: vol polar Box-Muller Gaussians + shift."""
+    out = np.zeros(vol)
+    lib().orc_gauss_field(C.byref(rng), vol, shift, out)
+    return out
+
+
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
-               want_psi=False, pair_w=None, pair_opp=None, sources=None, pot=None):
+               want_psi=False, pair_w=None, pair_opp=None, sources=None, pot=None,
+               phi_real=None, sinks=None, norms=None):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
     (complex), argsum [T, nm], optional M [T, N, N] (the pair matrix with pair_w) and
     psi [N, V].  With pair_w / pair_opp: nm = 2N, SlaterDet3's determinants (2n + 1: leading
@@ -379,7 +392,19 @@ def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smea
         po = np.ascontiguousarray(pair_opp, dtype=np.int32)
         keep += [pw, po]
         d.pair_w, d.pair_opp = pw.ctypes.data, po.ctypes.data
-    if field == "philox":
+    if sinks is not None:  # N x V complex, replacing the sources as measurement vectors
+        snk = np.ascontiguousarray(sinks, dtype=np.complex128).reshape(-1).view(np.float64)
+        keep.append(snk)
+        d.sinks = snk.ctypes.data
+    if norms is not None:  # T values: Propagator::Normalize(norms[t]) after slice t
+        nr = np.ascontiguousarray(norms, dtype=np.float64)
+        keep.append(nr)
+        d.norms = nr.ctypes.data
+    if field == "real":  # T x V real field values
+        fr = np.ascontiguousarray(phi_real, dtype=np.float64).reshape(-1)
+        keep.append(fr)
+        d.field_kind, d.phi_real = FIELD_REAL, fr.ctypes.data
+    elif field == "philox":
         d.field_kind, d.seed, d.config = FIELD_PHILOX, seed, config
     elif field == "ranlxs":
         d.field_kind = FIELD_RANLXS

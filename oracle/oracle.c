@@ -836,15 +836,34 @@ static void orc_plane_wave(double* out, int L, const int k[3], int xspace) {
       }
 }
 
+double orc_gauss(orc_ranlxs* r, double sigma) {
+  double x, y, r2;
+  do {
+    x = 2.0 * (double)orc_ranlxs_next(r) - 1.0;
+    y = 2.0 * (double)orc_ranlxs_next(r) - 1.0;
+    r2 = x * x + y * y;
+  } while (r2 > 1.0 || r2 == 0.0);
+  return sigma * y * sqrt(-2.0 * log(r2) / r2);
+}
+
+void orc_gauss_field(orc_ranlxs* r, int vol, double shift, double* out) {
+  for (int i = 0; i < vol; ++i) out[i] = orc_gauss(r, 1.0) + shift;
+}
+
 static void orc_get_field(const orc_run_desc* d, int t, long V, int8_t* phi, uint32_t* words) {
   if (d->field_kind == ORC_FIELD_PHILOX) {
     orc_z2_bits(d->seed, d->config, (uint32_t)t, 0u, (int)V, words);
     orc_bits_to_phi(words, (int)V, phi);
   } else if (d->field_kind == ORC_FIELD_RANLXS) {
     orc_ranlxs_z2_field(d->rng, (int)V, phi);
-  } else {
+  } else if (d->field_kind == ORC_FIELD_ARRAY) {
     memcpy(phi, d->phi + (size_t)t * V, (size_t)V);
   }
+}
+
+/* the real field value of site i at slice t: the +-1 of phi, or the REAL array's value */
+static double orc_field_value(const orc_run_desc* d, int t, long V, const int8_t* phi, long i) {
+  return d->field_kind == ORC_FIELD_REAL ? d->phi_real[(size_t)t * V + i] : (double)phi[i];
 }
 
 int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double* argsum,
@@ -878,7 +897,9 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
              2 * (size_t)V * sizeof(double));
     else
       orc_plane_wave(psi + 2 * (size_t)n * V, L, d->moms + 3 * n, xspace);
-    memcpy(sink + 2 * (size_t)n * V, psi + 2 * (size_t)n * V, 2 * (size_t)V * sizeof(double));
+    memcpy(sink + 2 * (size_t)n * V, d->sinks ? d->sinks + 2 * (size_t)n * V
+                                              : psi + 2 * (size_t)n * V,
+           2 * (size_t)V * sizeof(double));
   }
   if (xspace)
     for (long i = 0; i < V; ++i) kv[i] = d->kin[i] / (double)V;
@@ -913,11 +934,14 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
     orc_get_field(d, t, V, phi, words);
     if (!xspace) {
       if (d->smear) {
-        for (long i = 0; i < V; ++i) { lat[2 * i] = (double)phi[i]; lat[2 * i + 1] = 0.0; }
+        for (long i = 0; i < V; ++i) {
+          lat[2 * i] = orc_field_value(d, t, V, phi, i);
+          lat[2 * i + 1] = 0.0;
+        }
         orc_interaction_field(fld, lat, d->smear, L, L, L);
       } else {
         for (long i = 0; i < V; ++i) {
-          fld[2 * i] = (1.0 + d->A * (double)phi[i]) / (double)V;
+          fld[2 * i] = (1.0 + d->A * orc_field_value(d, t, V, phi, i)) / (double)V;
           fld[2 * i + 1] = 0.0;
         }
       }
@@ -952,12 +976,14 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
       for (int n = 0; n < N; ++n) {
         double* a = psi + 2 * (size_t)n * V;
         for (long i = 0; i < V; ++i) {
-          double f = 1.0 + d->A * (double)phi[i];
+          double f = 1.0 + d->A * orc_field_value(d, t, V, phi, i);
           a[2 * i] *= f;
           a[2 * i + 1] *= f;
         }
       }
     }
+    if (d->norms) /* Propagator::Normalize(norm): every value divided by norm */
+      for (size_t i = 0; i < 2 * (size_t)N * V; ++i) psi[i] /= d->norms[t];
   }
   if (psi_out) memcpy(psi_out, psi, 2 * (size_t)N * V * sizeof(double));
 done:

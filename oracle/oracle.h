@@ -145,7 +145,7 @@ void orc_sho_source(double* out, const double* coeffs, int L, double l1, double 
 /* ---- the correlator path ---- */
 enum { ORC_SPLIT_KSPACE = 0,  /* k-space storage, K(1/2) I K(1/2), field (1+A phi)/V in x */
        ORC_SPLIT_XSPACE = 1 }; /* x-space storage, psi <- F+[kappa F-[psi]]/V then (1+A phi) */
-enum { ORC_FIELD_PHILOX = 0, ORC_FIELD_RANLXS = 1, ORC_FIELD_ARRAY = 2 };
+enum { ORC_FIELD_PHILOX = 0, ORC_FIELD_RANLXS = 1, ORC_FIELD_ARRAY = 2, ORC_FIELD_REAL = 3 };
 
 typedef struct {
   int L;               /* cubic lattice side */
@@ -173,7 +173,21 @@ typedef struct {
                             1/V; k-space split): each slice is then the trapped production
                             order K, P, I, P, K (production/slater_det3_trapped/main-cum.C:
                             97-106), every P its own forward / backward transform pair */
+  const double* phi_real; /* REAL: T x V real field values (e.g. shifted Gaussians) */
+  const double* sinks;    /* optional N x V complex sinks in storage space (default: the
+                             sources) */
+  const double* norms;    /* optional T values: after slice t the orbitals are divided by
+                             norms[t] (Propagator::Normalize(Float), src/alg/propagator.C:
+                             116-133) */
 } orc_run_desc;
+
+/* Random::Gauss(sigma) (src/random/random.C:105-116): the polar Box-Muller method on the
+ * ranlxs uniforms (x, y from 2u - 1, rejected while r2 > 1 or r2 == 0), returning only
+ * sigma y sqrt(-2 log r2 / r2).  orc_gauss_field: Lattice::Refresh for FIELD_TYPE_GAUSS
+ * (src/alg/lattice.C:60-109) plus a real shift (Refresh(Float), lattice.C:111-165): out[i] =
+ * Gauss(1) + shift for the V sites in order (the imaginary parts draw nothing). */
+double orc_gauss(orc_ranlxs* r, double sigma);
+void orc_gauss_field(orc_ranlxs* r, int vol, double shift, double* out);
 
 /* Runs one configuration through T slices, measuring before evolving at each t
  * (production/slater_det/main.C:185-220).  Outputs, for each t and each measured minor m

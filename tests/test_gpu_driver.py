@@ -279,3 +279,71 @@ def test_slater_det2_driver_matches_oracle(orc):
                     want = np.linalg.det(Q)
                     got = odd[s, k][j, t, 0] + 1j * odd[s, k][j, t, 1]
                     assert abs(got - want) <= 1e-9 * abs(want) + 1e-300, (j, t, k, s)
+
+
+DRIVER_RW = os.path.join(ROOT, "nrcps_amd", "drivers", "one_body_reweight")
+
+
+def test_one_body_reweight_driver_matches_oracle(orc):
+    """Importance sampling with shifted Gaussian fields (production/one_body-reweight/
+    main.C): Lattice::Refresh(phi0) draws polar Box-Muller Gaussians from ranlxs plus the
+    shift (src/alg/lattice.C:111-165), the trapped slice K P I P K evolves the one propagator,
+    and Propagator::Normalize divides it by the density ratio exp(V phi0 (<phi> - phi0/2))
+    (src/alg/propagator.C:116-133, NotImplemented in the reference's GPU build).  The
+    overlaps with a separate sink against the oracle fed the same fields and ratios."""
+    assert os.path.exists(DRIVER_RW), "the one_body_reweight driver is not built"
+    L, T, ncfg, phi0 = 8, 6, 2, 0.0022
+    V = L ** 3
+    cpl = [0.48725928786895517, 0.29804274431821154, -0.21167497672068536,
+           0.040531132841112746, 0.0]
+    spring = (0.01, 0.02, 0.015)
+    args = {
+        "verbose.arg": "0\n0\n0\n0\n0\n",
+        "do.arg": f"{T}\n{L}\n{L}\n{L}\nBOUNDARY_TYPE_PRD\nBOUNDARY_TYPE_PRD\n"
+                  "BOUNDARY_TYPE_PRD\n1.0\n",
+        "evo.arg": f"0\n1\n{ncfg}\n",
+        "lattice.arg": "FIELD_TYPE_GAUSS\n",
+        "random.arg": "RANDOM_TYPE_RANLXS2\nSEED_TYPE_INPUT\n7012\nrngs/rng\n",
+        "interaction.arg": "5.0\nINTERACTION_TYPE_ONEMINUSXIINVSQ\n"
+                           + "".join(f"{c!r}\n" for c in cpl),
+        "kinetic.arg": "DISPERSION_TYPE_PERFECT\n5.0\n",
+        "potential.arg": "POTENTIAL_FORM_HARMONIC\nPOTENTIAL_TYPE_EXP\n"
+                         f"{spring[0]}\n{spring[1]}\n{spring[2]}\n",
+        "prop.arg": "1\nresults/prop\n",
+        "one_body.arg": "SOURCE_TYPE_MOM\n1.0\n1.0\n1.0\n",
+        "source.arg": "0 0 0 1.0 0.0\n",
+        "sink.arg": "0 0 0 1.0 0.0\n1 0 0 0.5 0.25\n",
+    }
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in args.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        print("[gpu] running the one_body_reweight driver (L=8, T=6, 2 configurations)",
+              flush=True)
+        r = subprocess.run([DRIVER_RW], cwd=tmp, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        got = np.array([list(map(float, line.split()))
+                        for line in open(os.path.join(tmp, "results", "prop")) if line.strip()])
+        got = got.reshape(ncfg, T, 2)
+    cut2 = PI * PI
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 5.0, 0.5, cutoff_sq=cut2)
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    assert rc == 0
+    pot = orc.potential_table(L, orc.POT_FORM_HARMONIC, orc.POT_TYPE_EXP, spring, 0.5)
+    src = np.zeros((1, V), np.complex128)
+    src[0, 0] = 1.0
+    snk = src.copy()
+    snk[0, (1 * L + 0) * L + 0] = 0.5 + 0.25j
+    rng = orc.ranlxs(2, 7012)
+    orc.gauss_field(rng, V, 0.0)  # the Lattice constructor's draw
+    moms = np.zeros((1, 3), np.int32)  # unused: the source is given
+    for j in range(ncfg):
+        fields = np.stack([orc.gauss_field(rng, V, phi0) for _ in range(T)])
+        norms = np.exp(V * phi0 * (fields.mean(axis=1) - 0.5 * phi0))
+        ref = orc.run_config(L, 1, T, moms, kin, smear=f, field="real", phi_real=fields,
+                             sources=src, sinks=snk, norms=norms, pot=pot)
+        want = np.exp(ref["logabs"][:, 0]) * ref["phase"][:, 0]
+        np.testing.assert_allclose(got[j, :, 0] + 1j * got[j, :, 1], want, rtol=1e-10)
+    assert np.all(np.abs(norms - 1) > 0)  # the ratios are not trivial

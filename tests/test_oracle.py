@@ -1,6 +1,7 @@
 """Pins the FP64 CPU oracle against the reference's known answers, published vectors,
 closed forms and brute force.  Every oracle function is covered here; a mistake such as a
 dropped term, a wrong sign or index or a transposed operand fails at least one test."""
+import ctypes as C
 import itertools
 import json
 import math
@@ -437,3 +438,61 @@ def test_exact_c1_vs_monte_carlo_config1(orc):
     s = np.array([np.exp(orc.run_config(L, 1, 8, moms, kin, A=A, split=1, seed=5,
                                         config=c)["logabs"][:, 0]) for c in range(100)])
     assert abs(s[:, -1].mean() - 1) < 4 * s[:, -1].std() / 10
+
+
+def test_gauss_field_is_standard_normal_and_shift_is_exact(orc):
+    """Random::Gauss's polar Box-Muller (src/random/random.C:105-116): Kolmogorov-Smirnov
+    against N(0, 1) (dropping the r2 division or taking x for y would fail it), acceptance
+    rate pi/4 (2 uniforms per attempt: 8/pi uniforms per value), and Refresh(shift) adds the
+    shift to the same draws."""
+    from scipy import stats
+
+    r = orc.ranlxs(2, 7012)
+    g = orc.gauss_field(r, 200000, 0.0)
+    assert stats.kstest(g, "norm").pvalue > 1e-3
+    r1, r2 = orc.ranlxs(2, 99), orc.ranlxs(2, 99)
+    assert np.array_equal(orc.gauss_field(r1, 1000, 0.0022),
+                          orc.gauss_field(r2, 1000, 0.0) + 0.0022)
+    # uniforms consumed: count draws until the two streams realign
+    a, b = orc.ranlxs(2, 5), orc.ranlxs(2, 5)
+    n_vals, n_unif = 20000, 0
+    orc.gauss_field(a, n_vals, 0.0)
+    sa = np.zeros(105, np.int64)
+    orc.lib().orc_ranlxs_get_state(C.byref(a), sa)
+    sb = np.zeros(105, np.int64)
+    while True:
+        orc.lib().orc_ranlxs_next(C.byref(b))
+        n_unif += 1
+        if n_unif % 2 == 0:
+            orc.lib().orc_ranlxs_get_state(C.byref(b), sb)
+            if np.array_equal(sa, sb):
+                break
+    assert abs(n_unif / n_vals - 8 / np.pi) < 0.03
+
+
+def test_run_config_real_field_sinks_and_norms(orc):
+    """The REAL field path equals the +-1 array path on +-1 values (with and without the
+    smeared interaction); sinks 2 x sources add N log 2 to every log|det|; dividing by
+    norms[t] after slice t subtracts N sum_{t' < t} log norms[t']."""
+    L, N, T = 6, 7, 5
+    V = L ** 3
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 20.0, 0.5)
+    rng = np.random.default_rng(3)
+    phi = np.where(rng.random((T, V)) < 0.5, 1, -1).astype(np.int8)
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, [0.5, 0.3], 1.0)
+    for smear in (None, f):
+        a = orc.run_config(L, N, T, moms, kin, smear=smear, field="array", phi=phi)
+        b = orc.run_config(L, N, T, moms, kin, smear=smear, field="real",
+                           phi_real=phi.astype(np.float64))
+        assert np.array_equal(a["logabs"], b["logabs"])
+    src = np.zeros((N, V), np.complex128)
+    src[np.arange(N), (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]] = 1.0
+    base = orc.run_config(L, N, T, moms, kin, field="array", phi=phi)
+    two = orc.run_config(L, N, T, moms, kin, field="array", phi=phi, sources=src,
+                         sinks=2 * src)
+    np.testing.assert_allclose(two["logabs"], base["logabs"] + N * np.log(2), atol=1e-12)
+    norms = np.array([1.5, 0.7, 2.0, 1.1, 0.9])
+    nrm = orc.run_config(L, N, T, moms, kin, field="array", phi=phi, norms=norms)
+    shift = -N * np.concatenate([[0.0], np.cumsum(np.log(norms))[:-1]])
+    np.testing.assert_allclose(nrm["logabs"][:, 0], base["logabs"][:, 0] + shift, atol=1e-12)
