@@ -280,14 +280,25 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     la_plain = np.concatenate([ens.run(seed, (1 << 22) + k, T)[0]
                                for k in range(0, n_plain, chunk)], axis=1)
     t_plain = time.perf_counter() - t0
-    # training
-    prop = pr.Proposal(ctx, L, max(batch, chunk),
-                       pr.init_params(np.random.default_rng(0), out_scale=out_scale))
+    # training: the output bias starts at the best constant tilt of the training sample (one
+    # parameter fitted to its fields and log-determinants), then the network trains
     t0 = time.perf_counter()
+    V = L ** 3
+    S_train = np.concatenate([pr.z2_sums(ctx, seed, k, chunk, T, V)
+                              for k in range(0, n_train, chunk)], axis=1)
+    params = pr.init_params(np.random.default_rng(0), out_scale=out_scale)
+    lam_fit = pr.best_constant_tilt(la_train, S_train, ts, V)
+    params[8240] = 2.0 * lam_fit
+    prop = pr.Proposal(ctx, L, max(batch, chunk), params)
     losses, mu, sd = pr.train(ctx, prop, la_train, seed, 0, steps, batch, lr, ts,
                               np.random.default_rng(1), optimizer=optimizer)
     ctx.sync()
     t_train = time.perf_counter() - t0
+    # the measured ceiling: the best non-adaptive per-site tilts on the uniform comparison
+    # sample (fitted on one half, evaluated on the other)
+    S_plain = np.concatenate([pr.z2_sums(ctx, seed, (1 << 22) + k, chunk, T, V)
+                              for k in range(0, n_plain, chunk)], axis=1)
+    ceiling = pr.tilt_ceiling(la_plain, S_plain, ts, V)
     # proposed configurations
     sampler = pr.LearnedSampler(ctx, prop, ens, mu, sd)
     sampler.run(seed + 1, 1 << 23, 2)  # warm-up
@@ -297,9 +308,32 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     t_learned = time.perf_counter() - t0
     la_l = np.concatenate([p[0] for p in parts], axis=1)
     logr = np.concatenate([p[2] for p in parts], axis=1)
+    # the fitted constant tilt alone: tilted Philox fields on the uniform sampler's engine
+    q = ens.set_tilt(0.5 * (1.0 + np.tanh(lam_fit)))
+    ens.run(seed + 2, 1 << 23, 2)  # warm-up
+    t0 = time.perf_counter()
+    tparts = []
+    for k in range(0, n_learned, chunk):
+        la_k = ens.run(seed + 2, (1 << 25) + k, T)[0]
+        tparts.append((la_k, ens.log_ratio(T)))
+    t_tilt = time.perf_counter() - t0
+    ens.set_tilt(0.5)
+    la_t = np.concatenate([a for a, _ in tparts], axis=1)
+    logr_t = np.concatenate([b for _, b in tparts], axis=1)
     sp = pr.estimator_stats(la_plain, None, ts)
     sl = pr.estimator_stats(la_l, logr, ts)
+    st = pr.estimator_stats(la_t, logr_t, ts)
     tau_p, tau_l = t_plain / n_plain, t_learned / n_learned
+    tau_t = t_tilt / n_learned
+    tilted = {}
+    for t in ts:
+        vp = sp[t]["rel_se"] ** 2 * n_plain
+        vt = st[t]["rel_se"] ** 2 * n_learned
+        sig = float(np.hypot(sp[t]["rel_se"], st[t]["rel_se"]))
+        tilted[str(t)] = {"log_C": st[t]["log_C"], "rel_se": st[t]["rel_se"],
+                          "bias_within_2sigma": bool(abs(st[t]["log_C"] - sp[t]["log_C"])
+                                                     <= 2 * sig),
+                          "variance_gain": vp / vt, "gain": (vp * tau_p) / (vt * tau_t)}
     tau_l_amort = (t_learned + t_train + t_train_gen) / n_learned
     per_t = {}
     for t in ts:
@@ -314,7 +348,9 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "eff_samples_per_s_plain": 1.0 / (vp * tau_p),
             "eff_samples_per_s_learned": 1.0 / (vl * tau_l),
             "gain": (vp * tau_p) / (vl * tau_l),
-            "gain_with_training_amortised": (vp * tau_p) / (vl * tau_l_amort)}
+            "gain_with_training_amortised": (vp * tau_p) / (vl * tau_l_amort),
+            "variance_gain": vp / vl,
+            "ceiling": ceiling[t]}
     ps = prop.get_params()
     prop.close()
     ens_nvo = ens.volumes_per_config()
@@ -326,10 +362,24 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "metric": "effective samples/s at equal statistical error of C(t)",
             "conj_partners": bool(conj), "volumes_per_config": ens_nvo,
             "gain_min_over_t": min(v["gain"] for v in per_t.values()),
+            "cost_ratio_proposed_to_uniform": tau_l / tau_p,
+            "ceiling_note": "variance gain of the best non-adaptive per-site tilt (constant, or "
+                            "one per slice) measured on the uniform sample: the gain in "
+                            "effective samples/s such a proposal reaches at the cost of a "
+                            "uniform configuration; the learned proposal's gain is its "
+                            "variance gain over its cost ratio",
             "per_t": per_t,
+            "fitted_tilt": {
+                "what": "the constant per-site tilt fitted to the training sample's field sums "
+                        "and log-determinants, drawn as tilted Philox fields on the uniform "
+                        "sampler's kernels (nrc_ensemble_set_tilt) and reweighted exactly",
+                "lambda": lam_fit, "p_plus": q, "cost_ratio_to_uniform": tau_t / tau_p,
+                "gain_min_over_t": min(v["gain"] for v in tilted.values()),
+                "per_t": tilted, "seconds": t_tilt},
             "proposal_weight_ess_fraction_at_3T/4": float(w.sum() ** 2 / (w ** 2).sum() / w.size),
             "seconds": {"uniform_training_sample": t_train_gen, "training": t_train,
-                        "uniform_comparison_sample": t_plain, "proposed_sample": t_learned},
+                        "uniform_comparison_sample": t_plain, "proposed_sample": t_learned,
+                        "tilted_sample": t_tilt},
             "training": {"steps": steps, "batch": batch, "lr": lr,
                          "optimizer": "sgd-momentum" if optimizer == 1 else "adam",
                          "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
@@ -615,6 +665,8 @@ def main():
     learned = None
     if not args.no_learned:
         learned = learned_proposal(ctx, na, conj=bool(args.conj))
+        print(f"fitted tilt: gain {learned['fitted_tilt']['gain_min_over_t']:.2f}x effective "
+              f"samples/s", file=sys.stderr, flush=True)
         print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
 

@@ -223,6 +223,22 @@ int nrc_ensemble_set_trap(nrc_ensemble* ens, const double* q);
 #define NRC_ACC_FP32 0
 #define NRC_ACC_X64 1
 int nrc_ensemble_set_accuracy(nrc_ensemble* ens, int mode);
+/* Importance-sampled Z2 fields (the Z2 form of production/one_body-reweight/main.C:73-76,
+ * which draws Lattice::Refresh(phi0), a Gaussian field shifted off zero, and divides the
+ * propagator by the density ratio): with p_plus != 0.5, nrc_ensemble_run draws each site +1
+ * with probability q = round(p_plus 2^32) / 2^32 (q exactly; site i of configuration c at
+ * slice t is +1 where output i % 4 of the Philox4x32-10 block with counter (i / 4, t, c, 2)
+ * and key (seed lo, seed hi) is below q 2^32), and keeps each slice's field sum.
+ * nrc_ensemble_log_ratio then gives log r[t][c] = sum over t' < t of log(p(phi_t') /
+ * q(phi_t')) (p the plain sampler's 2^-V), so that sum_c det(t)^2 r / C estimates C(t)
+ * without bias; the ratio is returned beside the results instead of being folded into the
+ * orbitals as Propagator::Normalize does.  p_plus = 0.5 restores the plain sampler; caller
+ * fields (nrc_ensemble_run_fields) are never tilted (their log ratio is zero).  Not while the
+ * ensemble is in a batch, and a batch refuses a tilted ensemble (NRC_ERR_UNSUPPORTED). */
+int nrc_ensemble_set_tilt(nrc_ensemble* ens, double p_plus);
+int nrc_ensemble_tilt(nrc_ensemble* ens, double* p_plus);  /* the q in use (0.5: plain) */
+/* log_ratio: [T][C], T the last run's slice count (NRC_ERR_ARG otherwise) */
+int nrc_ensemble_log_ratio(nrc_ensemble* ens, int T, double* log_ratio);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
 int nrc_ensemble_volumes(nrc_ensemble* ens, int* volumes_per_config);
 /* Per-kernel timing with HIP events on the ensemble's stream (enable before a run).
@@ -313,6 +329,11 @@ int nrc_norm_logw(nrc_context* ctx, const double* logabs_dev, int stride, int B,
 int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t* configs_dev,
                      const uint32_t* ts_dev, int n, uint32_t stream, int vol,
                      uint32_t* words_dev);
+/* sums_dev[t][c] = sum over the vol sites of phi(x, t) for configurations config0 .. config0 +
+ * nconf - 1 and t < T, the fields of nrc_z2_bits: the statistic of tilted proposals
+ * q ~ p exp(lambda S), whose exact weights are known from S alone. */
+int nrc_z2_sums(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
+                uint32_t stream, int vol, int* sums_dev);
 
 #ifdef __cplusplus
 }

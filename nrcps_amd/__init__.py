@@ -32,7 +32,8 @@ EXPORTS = [
     "nrc_overlap", "nrc_pair_overlap", "nrc_lu_det", "nrc_ludcmp", "nrc_ref_abs", "nrc_z2_bits", "nrc_ensemble_create", "nrc_ensemble_run",
     "nrc_ensemble_run_fields", "nrc_ensemble_orbitals", "nrc_ensemble_set_path",
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
-    "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc",
+    "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc", "nrc_z2_sums",
+    "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -91,6 +92,8 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_orbitals": [vp, i, vp], "nrc_ensemble_set_path": [vp, i],
         "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
         "nrc_ensemble_set_trap": [vp, vp], "nrc_ensemble_set_accuracy": [vp, i],
+        "nrc_ensemble_set_tilt": [vp, C.c_double],
+        "nrc_ensemble_tilt": [vp, C.POINTER(C.c_double)], "nrc_ensemble_log_ratio": [vp, i, dp],
         "nrc_scale_const": [vp, vp, C.c_long, C.c_double, i, i],
         "nrc_norms": [vp, vp, i, C.c_long, i, vp],
         "nrc_ensemble_profile": [vp, i],
@@ -114,6 +117,7 @@ def load_library() -> C.CDLL:
                                     C.POINTER(C.c_double)],
         "nrc_norm_logw": [vp, vp, i, i, C.c_double, C.c_double, vp],
         "nrc_z2_bits_list": [vp, u64, vp, vp, i, u32, i, vp],
+        "nrc_z2_sums": [vp, u64, u32, i, i, u32, i, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -344,6 +348,21 @@ class Ensemble:
     def set_accuracy(self, mode: int):
         """ACC_FP32 or ACC_X64 (the default) for FP32 ensembles (nrc_ensemble_set_accuracy)."""
         _check(self.lib.nrc_ensemble_set_accuracy(self.h, mode), "ensemble_set_accuracy")
+
+    def set_tilt(self, p_plus: float) -> float:
+        """Tilted fields (nrc_ensemble_set_tilt): each site +1 with probability p_plus (0.5:
+        the plain sampler); returns the exact probability in use (a multiple of 2^-32)."""
+        _check(self.lib.nrc_ensemble_set_tilt(self.h, float(p_plus)), "ensemble_set_tilt")
+        q = C.c_double()
+        _check(self.lib.nrc_ensemble_tilt(self.h, C.byref(q)), "ensemble_tilt")
+        return q.value
+
+    def log_ratio(self, T: int) -> np.ndarray:
+        """[T][C] log of the plain-to-tilted density ratio of the last run's fields before
+        each measured slice (zero for a plain run); T is the last run's slice count."""
+        out = np.zeros((T, self.C), dtype=np.float64)
+        _check(self.lib.nrc_ensemble_log_ratio(self.h, T, _ptr(out)), "ensemble_log_ratio")
+        return out
 
     def set_symmetry(self, conj: bool):
         """Conjugate partners (nrc_ensemble_set_symmetry): evolve one orbital of each (k, -k)

@@ -452,6 +452,11 @@ struct nrc_ensemble {
   double* tab_x64 = nullptr;     // FP32 k-space split: K(1/2) and K as FP64 tables (x-chain)
   double* tab_k64 = nullptr;
   int last_T = 0;                // slices of the last run (results are [last_T][C][nm])
+  // tilted Philox fields (nrc_ensemble_set_tilt): P(+1) = tilt_thr / 2^32 per site, 0 = plain
+  uint32_t tilt_thr = 0;
+  int* fsum = nullptr;           // [T][C] field sums of the last tilted run
+  size_t fsum_cap = 0;
+  bool last_tilted = false;      // the last run drew tilted fields (fsum is valid)
   bool profile = false;
   double prof_ms[5] = {0, 0, 0, 0, 0};
   long prof_n[5] = {0, 0, 0, 0, 0};
@@ -476,6 +481,7 @@ static void ens_free(nrc_ensemble* e) {
   for (void* b : bufs)
     if (b) hipFree(b);
   if (e->par_dev) hipFree(e->par_dev);
+  if (e->fsum) hipFree(e->fsum);
   if (e->par_host) hipHostFree(e->par_host);
   for (hipEvent_t x : e->ev) hipEventDestroy(x);
   delete e;
@@ -1231,6 +1237,15 @@ static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (words_host)
     NRC_HIP(hipMemcpyAsync(e->words, words_host, need_words * sizeof(uint32_t),
                            hipMemcpyHostToDevice, e->stream));
+  const size_t need_sums = e->tilt_thr && !words_host ? (size_t)T * e->d.n_configs : 0;
+  if (need_sums > e->fsum_cap) {
+    NRC_HIP(hipStreamSynchronize(e->stream));
+    hipFree(e->fsum);
+    e->fsum = nullptr;
+    e->fsum_cap = 0;
+    NRC_TRY(dev_alloc(&e->fsum, need_sums));
+    e->fsum_cap = need_sums;
+  }
   return ens_reserve_outputs(e, T);
 }
 
@@ -1257,6 +1272,15 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
 
   NRC_TRY(ens_set_sources(e));
   const nrc::GatherSpec g = ens_gather_spec(e);
+  // Lattice::Refresh for slice tt into dst: the plain sampler's Philox bits, or tilted ones
+  // with their field sums
+  auto refresh = [&](int tt, uint32_t* dst) -> hipError_t {
+    if (e->tilt_thr)
+      return nrc::launch_z2_tilt(seed, config0, (int)C, (uint32_t)tt, NRC_STREAM_TILT, (int)V,
+                                 e->tilt_thr, dst, e->fsum + (size_t)tt * C, s, e->par_dev);
+    return nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)tt, 0u, (int)V, dst, s,
+                               e->par_dev);
+  };
 
   auto field_of = [&](int t) -> const uint32_t* {
     return host_words ? e->words + (size_t)t * slice_words : e->words;
@@ -1281,9 +1305,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
       if (e->sinks || e->pair) NRC_TRY(ens_measure(e, t));  // dense or pair sinks: before
       if (!host_words) {
         EvScope sc(e, 3);
-        for (int k = 0; k < S; ++k)
-          NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
-                                      e->words + (size_t)k * slice_words, s, e->par_dev));
+        for (int k = 0; k < S; ++k) NRC_HIP(refresh(t + k, e->words + (size_t)k * slice_words));
       }
       NRC_TRY(ens_evolve_x(e, t, S, host_words ? field_of(t) : e->words, slice_words,
                            t + S == T));
@@ -1301,9 +1323,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     if (measure_first) NRC_TRY(ens_measure(e, t));
     if (!host_words) {
       EvScope sc(e, 3);
-      for (int k = 0; k < S; ++k)
-        NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)(t + k), 0u, (int)V,
-                                    e->words + (size_t)k * slice_words, s, e->par_dev));
+      for (int k = 0; k < S; ++k) NRC_HIP(refresh(t + k, e->words + (size_t)k * slice_words));
     }
     if (S > 1)
       NRC_TRY(ens_evolve_fused(e, t, S, host_words ? field_of(t) : e->words, slice_words));
@@ -1346,8 +1366,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
     // Lattice::Refresh for slice t
     if (!host_words) {
       EvScope sc(e, 3);
-      NRC_HIP(nrc::launch_z2_bits(seed, config0, (int)C, (uint32_t)t, 0u, (int)V, e->words, s,
-                                  e->par_dev));
+      NRC_HIP(refresh(t, e->words));
     }
     if (ref && e->smear) NRC_TRY(ens_smear_field(e, field_of(t)));
     // YZ pass: the field (k-space split) or kinetic (x-space split) multiply between the FFTs
@@ -1401,6 +1420,7 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   NRC_TRY(ens_set_params(e, seed, config0, e->stream));
   NRC_TRY(ens_enqueue(e, seed, config0, words_host != nullptr, T));
   e->last_T = T;
+  e->last_tilted = e->tilt_thr && !words_host;
   return ens_collect(e, T, logabs, phase, argsum);
 }
 
@@ -1637,6 +1657,48 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   return NRC_OK;
 }
 
+// ------------------------------------------------------------------ tilted fields
+extern "C" int nrc_ensemble_set_tilt(nrc_ensemble* e, double p_plus) {
+  if (!e || !(p_plus > 0.0 && p_plus < 1.0)) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
+  if (p_plus == 0.5) {
+    e->tilt_thr = 0;
+    return NRC_OK;
+  }
+  const double thr = std::nearbyint(std::ldexp(p_plus, 32));
+  e->tilt_thr = (uint32_t)std::min(std::max(thr, 1.0), 4294967295.0);
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_tilt(nrc_ensemble* e, double* p_plus) {
+  if (!e || !p_plus) return NRC_ERR_ARG;
+  *p_plus = e->tilt_thr ? std::ldexp((double)e->tilt_thr, -32) : 0.5;
+  return NRC_OK;
+}
+
+// log r_t[c] = sum over the slices t' < t of log(p(phi_t') / q(phi_t')): p the plain
+// sampler's 2^-V, q the tilted one's q^n(+1) (1 - q)^n(-1); zero for a plain run
+extern "C" int nrc_ensemble_log_ratio(nrc_ensemble* e, int T, double* out) {
+  if (!e || !out || e->last_T < 1 || T != e->last_T) return NRC_ERR_ARG;
+  const long C = e->d.n_configs, V = e->V;
+  if (!e->last_tilted) {
+    std::fill(out, out + (size_t)T * C, 0.0);
+    return NRC_OK;
+  }
+  std::vector<int> sums((size_t)T * C);
+  NRC_HIP(hipStreamSynchronize(e->stream));
+  NRC_HIP(hipMemcpy(sums.data(), e->fsum, sums.size() * sizeof(int), hipMemcpyDeviceToHost));
+  const double q = std::ldexp((double)e->tilt_thr, -32);
+  const double lp = std::log(0.5 / q), lm = std::log(0.5 / (1.0 - q));
+  for (long c = 0; c < C; ++c) out[c] = 0.0;
+  for (int t = 1; t < T; ++t)
+    for (long c = 0; c < C; ++c) {
+      const long np = (V + sums[(size_t)(t - 1) * C + c]) / 2;
+      out[(size_t)t * C + c] = out[(size_t)(t - 1) * C + c] + np * lp + (V - np) * lm;
+    }
+  return NRC_OK;
+}
+
 extern "C" int nrc_ensemble_results(nrc_ensemble* e, double* logabs, double* phase,
                                     double* argsum) {
   if (!e || e->last_T < 1) return NRC_ERR_ARG;
@@ -1714,6 +1776,7 @@ extern "C" int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, i
   for (int i = 0; i < n_groups; ++i) {
     if (!groups[i] || groups[i]->ctx != ctx) return NRC_ERR_ARG;
     if (ens_in_batch(groups[i])) return NRC_ERR_BATCHED;  // one batch at a time
+    if (groups[i]->tilt_thr) return NRC_ERR_UNSUPPORTED;  // tilted fields: single runs only
     for (int j = 0; j < i; ++j)
       if (groups[j] == groups[i]) return NRC_ERR_ARG;
   }
@@ -1761,6 +1824,7 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     e->profile = false;
     NRC_TRY(ens_prepare(e, nullptr, T));
     e->last_T = T;
+    e->last_tilted = false;
   }
   for (hipStream_t st : b->streams) NRC_HIP(hipStreamSynchronize(st));
   hipStream_t main = b->ctx->stream;
@@ -1858,6 +1922,7 @@ extern "C" int nrc_ensemble_begin(nrc_ensemble* e, int T) {
   if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
   NRC_TRY(ens_prepare(e, nullptr, T));
   e->last_T = T;
+  e->last_tilted = false;  // the caller's fields
   return ens_set_sources(e);
 }
 

@@ -43,6 +43,51 @@ hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t
   return hipGetLastError();
 }
 
+// Tilted Z2 fields: site i of configuration c at slice t is +1 where the Philox output
+// u_i < thr, u_i being output (i % 4) of the block with counter (i / 4, t, c, stream): P(+1) =
+// thr / 2^32 exactly.  One workgroup per configuration; each thread builds whole words (eight
+// Philox blocks per word) and fsum[c] receives the field sum 2 n(+1) - vol, from which the
+// sampling ratio of the slice follows.
+__global__ __launch_bounds__(256) void k_z2_tilt(uint64_t seed, uint32_t config0, int nconf,
+                                                 uint32_t t, uint32_t stream, int vol,
+                                                 uint32_t thr, uint32_t* words, int* fsum,
+                                                 const uint64_t* par) {
+  __shared__ int red[4];
+  if (par) {
+    seed = par[0];
+    config0 = (uint32_t)par[1];
+  }
+  const int c = blockIdx.x;
+  const int nw = (vol + 31) / 32;
+  int pop = 0;
+  for (int w = threadIdx.x; w < nw; w += 256) {
+    uint32_t bits = 0;
+#pragma unroll 2
+    for (int b = 0; b < 8; ++b) {
+      uint32_t ctr[4] = {(uint32_t)(8 * w + b), t, config0 + (uint32_t)c, stream};
+      philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) bits |= (uint32_t)(ctr[q] < thr) << (4 * b + q);
+    }
+    words[(long)c * nw + w] = bits;
+    const int valid = vol - 32 * w;
+    pop += __popc(valid >= 32 ? bits : bits & ((1u << valid) - 1u));
+  }
+  for (int off = 32; off >= 1; off >>= 1) pop += __shfl_xor(pop, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pop;
+  __syncthreads();
+  if (threadIdx.x == 0) fsum[c] = 2 * (red[0] + red[1] + red[2] + red[3]) - vol;
+}
+
+hipError_t launch_z2_tilt(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                          uint32_t stream, int vol, uint32_t thr, uint32_t* words, int* fsum,
+                          hipStream_t s, const uint64_t* par) {
+  if (nconf < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_z2_tilt, dim3((unsigned)nconf), dim3(256), 0, s, seed, config0, nconf,
+                     t, stream, vol, thr, words, fsum, par);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ batched LU
 // One wavefront per matrix, lane i owns row i, the matrix sits in LDS column-major so that
 // lanes touch consecutive addresses.  The factorisation is right-looking: at step k the

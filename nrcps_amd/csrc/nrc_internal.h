@@ -167,6 +167,14 @@ hipError_t launch_set_plane_waves(int prec, void* psi, const int* moms, long nvo
 hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
                           uint32_t stream, int vol, uint32_t* words, hipStream_t s,
                           const uint64_t* par = nullptr);
+// Philox stream of the tilted sampler's per-site uniforms (the plain sampler uses stream 0,
+// the learned proposal stream 1)
+#define NRC_STREAM_TILT 2u
+// Tilted Philox Z2 field bits: P(+1) = thr / 2^32 per site, one uniform per site; fsum[c]
+// receives each configuration's field sum
+hipError_t launch_z2_tilt(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                          uint32_t stream, int vol, uint32_t thr, uint32_t* words, int* fsum,
+                          hipStream_t s, const uint64_t* par = nullptr);
 
 // Options of the batched LU (launch_lu_det), all off by default:
 //   repl, ldr: the last column of every minor (size k) is row k - 1 of repl[c] (a matrix of

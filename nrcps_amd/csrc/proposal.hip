@@ -858,10 +858,38 @@ __global__ void k_z2_bits_list(uint64_t seed, const uint32_t* cfg, const uint32_
   }
 }
 
+// sums[t][c] = sum over sites of phi(x, t) of the plain sampler's field of configuration
+// config0 + c (2 popcount - vol): one workgroup of 256 per (c, t), the words regenerated
+__global__ __launch_bounds__(256) void k_z2_sums(uint64_t seed, uint32_t config0, int nconf,
+                                                 uint32_t stream, int vol, int* sums) {
+  __shared__ int red[4];
+  const int c = blockIdx.x;
+  const uint32_t t = blockIdx.y;
+  const int nw = (vol + 31) / 32, ng = (nw + 3) / 4;
+  int pop = 0;
+  for (int g = threadIdx.x; g < ng; g += 256) {
+    uint32_t ctr[4] = {(uint32_t)g, t, config0 + (uint32_t)c, stream};
+    philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int w = 4 * g + q;
+      if (w >= nw) break;
+      const int valid = vol - 32 * w;  // sites of this word inside the volume
+      const uint32_t m = valid >= 32 ? 0xFFFFFFFFu : ((1u << valid) - 1u);
+      pop += __popc(ctr[q] & m);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) pop += __shfl_xor(pop, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pop;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[(long)t * nconf + c] = 2 * (red[0] + red[1] + red[2] + red[3]) - vol;
+}
+
 }  // namespace nrc
 
 // ------------------------------------------------------------------ C ABI
 using namespace nrc;
+
 
 struct nrc_proposal {
   nrc_context* ctx = nullptr;
@@ -1140,6 +1168,17 @@ extern "C" int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t*
   const int nw = (vol + 31) / 32;
   hipLaunchKernelGGL(k_z2_bits_list, dim3(blocks_for((long)n * ((nw + 3) / 4))), dim3(256), 0,
                      ctx_stream(ctx), seed, configs_dev, ts_dev, n, stream, nw, words_dev);
+  PR_HIP(hipGetLastError());
+  return NRC_OK;
+}
+
+extern "C" int nrc_z2_sums(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
+                           uint32_t stream, int vol, int* sums_dev) {
+  if (!ctx || !sums_dev || nconf < 0 || T < 0 || vol < 1) return NRC_ERR_ARG;
+  if (nconf == 0 || T == 0) return NRC_OK;
+  if (nconf > 2147483647 / T) return NRC_ERR_ARG;
+  hipLaunchKernelGGL(k_z2_sums, dim3((unsigned)nconf, (unsigned)T), dim3(256), 0, ctx_stream(ctx),
+                     seed, config0, nconf, stream, vol, sums_dev);
   PR_HIP(hipGetLastError());
   return NRC_OK;
 }

@@ -213,6 +213,85 @@ def train(ctx: Context, prop: Proposal, la: np.ndarray, seed: int, config0: int,
     return np.array(losses), mu, sd
 
 
+def z2_sums(ctx: Context, seed: int, config0: int, nconf: int, T: int, vol: int,
+            stream=STREAM_UNIFORM) -> np.ndarray:
+    """S[t][c] = sum over sites of the plain sampler's phi(x, t) (nrc_z2_sums)."""
+    out = ctx.array((T, nconf), np.int32)
+    _check(ctx.lib.nrc_z2_sums(ctx.h, seed, config0, nconf, T, stream, vol, out.ptr), "z2_sums")
+    return out.download()
+
+
+def _tilt_log_ess(lam: np.ndarray, d2: np.ndarray, S: np.ndarray, V: int):
+    """log of the effective sample fraction of the tilted proposal q ~ p prod_t' exp(lam_t'
+    S_t') (per-site probability of +1: (1 + tanh lam_t') / 2) for the estimator of <d2>,
+    evaluated on a uniform sample: weights w = d2 prod cosh(lam)^V exp(-lam S), so
+    E_q[w] = E_p[d2] and E_q[w^2] = E_p[d2^2 prod cosh^V exp(-lam S)].  Returns it and its
+    gradient in lam."""
+    g = V * np.log(np.cosh(lam))[:, None] - lam[:, None] * S  # [t'][c]
+    e = 2.0 * np.log(d2) + g.sum(axis=0)
+    m = e.max()
+    w = np.exp(e - m)
+    log_e2 = np.log(w.mean()) + m
+    log_ess = 2.0 * np.log(d2.mean()) - log_e2
+    grad = -((V * np.tanh(lam))[:, None] - S) @ (w / w.sum())
+    return log_ess, grad
+
+
+def best_constant_tilt(la: np.ndarray, S: np.ndarray, ts, V: int) -> float:
+    """The per-site tilt lambda (q(+1) = (1 + tanh lambda) / 2, logit 2 lambda) that
+    maximises the mean over t in ts of the effective sample fraction on the uniform sample
+    (la [T][C][m], S [T][C] field sums)."""
+    from scipy.optimize import minimize_scalar
+
+    x = la[:, :, 0] if la.ndim == 3 else la
+
+    def f(l):
+        tot = 0.0
+        for t in ts:
+            d2 = np.exp(2.0 * (x[t] - x[t].max()))
+            tot += _tilt_log_ess(np.array([l]), d2, S[:t].sum(axis=0, dtype=np.float64)[None],
+                                 V * t)[0]
+        return -tot / len(ts)
+
+    return float(minimize_scalar(f, bounds=(-0.05, 0.05), method="bounded").x)
+
+
+def tilt_ceiling(la: np.ndarray, S: np.ndarray, ts, V: int, seed: int = 0) -> dict:
+    """The effective sample fraction of the best non-adaptive per-site tilts, measured on a
+    uniform sample (la [T][C][m], S [T][C] field sums): for each t in ts, a single tilt for
+    every slice before t and one tilt per slice, fitted on one half of the configurations and
+    evaluated on the other half (so the fit cannot flatter itself), against the uniform
+    sampler's fraction on that half.  The per-sample relative variance is 1 / ess - 1, so
+    the variance gain of a tilt at equal cost is (1/ess_uniform - 1) / (1/ess_tilt - 1)."""
+    from scipy.optimize import minimize, minimize_scalar
+
+    x = la[:, :, 0] if la.ndim == 3 else la
+    C = x.shape[1]
+    perm = np.random.default_rng(seed).permutation(C)
+    fit, ev = perm[: C // 2], perm[C // 2:]
+    out = {}
+    for t in ts:
+        d2 = np.exp(2.0 * (x[t] - x[t].max()))
+        St = S[:t].astype(np.float64)
+        Sc = St.sum(axis=0)
+
+        def one(l, idx):
+            return _tilt_log_ess(np.array([l]), d2[idx], Sc[None, idx], V * t)[0]
+
+        r1 = minimize_scalar(lambda l: -one(l, fit), bounds=(-0.05, 0.05), method="bounded")
+        lam_c = float(r1.x)
+        res = minimize(lambda l: tuple(-v for v in _tilt_log_ess(l, d2[fit], St[:, fit], V)),
+                       np.full(t, lam_c), jac=True, method="L-BFGS-B")
+        ess_u = float(np.exp(_tilt_log_ess(np.zeros(1), d2[ev], Sc[None, ev], V * t)[0]))
+        ess_c = float(np.exp(one(lam_c, ev)))
+        ess_s = float(np.exp(_tilt_log_ess(res.x, d2[ev], St[:, ev], V)[0]))
+        out[int(t)] = {"ess_fraction_uniform": ess_u, "ess_fraction_constant_tilt": ess_c,
+                       "ess_fraction_per_slice_tilt": ess_s, "constant_tilt": lam_c,
+                       "variance_gain_constant_tilt": (1 / ess_u - 1) / (1 / ess_c - 1),
+                       "variance_gain_per_slice_tilt": (1 / ess_u - 1) / (1 / ess_s - 1)}
+    return out
+
+
 def estimator_stats(la: np.ndarray, logr: np.ndarray | None, ts):
     """For each t in ts: log C(t) estimate, relative standard error and the effective sample
     size (sum x)^2 / sum x^2 of x = det(t)^2 r_t (r = 1 for the uniform sampler)."""

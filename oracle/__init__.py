@@ -246,6 +246,23 @@ def z2_bits(seed: int, config: int, t: int, vol: int, stream: int = 0) -> np.nda
     return w
 
 
+def z2_tilt_phi(seed: int, config: int, t: int, vol: int, thr: int) -> np.ndarray:
+    """The tilted Z2 field: site i is +1 where Philox output i (output i % 4 of the block with
+    counter (i / 4, t, config, 2)) is below thr, so P(+1) = thr / 2^32.  The importance-sampled
+    field of production/one_body-reweight/main.C:73-76 (a Gaussian shifted by phi0 there) in
+    Z2 form; the uniform words are the plain sampler's generator read one word per site."""
+    u = z2_bits(seed, config, t, 32 * vol, stream=2)
+    return np.where(u < np.uint64(thr), 1, -1).astype(np.int8)
+
+
+def z2_tilt_log_ratio(phis: np.ndarray, q: float) -> np.ndarray:
+    """log r_t = sum over t' < t of log(2^-V / (q^n+ (1 - q)^n-)) for fields phis [T][V]."""
+    V = phis.shape[1]
+    n_plus = (phis == 1).sum(axis=1).astype(np.float64)
+    per = n_plus * np.log(0.5 / q) + (V - n_plus) * np.log(0.5 / (1.0 - q))
+    return np.concatenate([[0.0], np.cumsum(per)[:-1]])
+
+
 def bits_to_phi(words: np.ndarray, vol: int) -> np.ndarray:
     phi = np.zeros(vol, dtype=np.int8)
     lib().orc_bits_to_phi(np.ascontiguousarray(words, np.uint32), vol, phi)

@@ -201,3 +201,69 @@ def test_learned_sampler_is_unbiased(ctx, orc):
             assert sl[11]["ess"] > 1.2 * sp[11]["ess"]
         prop.close()
         ens.close()
+
+
+def test_z2_sums_match_oracle(ctx, orc):
+    """nrc_z2_sums: the per-slice field sums of the uniform sampler's Philox fields."""
+    from nrcps_amd import proposal as pr
+
+    L, C, T, seed, c0 = 8, 4, 3, 7012, 5
+    V = L ** 3
+    S = pr.z2_sums(ctx, seed, c0, C, T, V)
+    for c in range(C):
+        for t in range(T):
+            bits = orc.z2_bits(seed, c0 + c, t, V)
+            ref = 2 * sum(bin(int(w)).count("1") for w in bits) - V
+            assert S[t, c] == ref
+
+
+def test_trained_proposal_gain_and_bias(ctx, orc):
+    """Training on the GPU end to end at L = 8, N = 7, T = 16: the output bias starts at the
+    best constant tilt of the training sample, the network trains on 16384 uniform
+    configurations, and 16384 proposed configurations estimate C(t).  The reweighted estimates
+    agree with 32768 uniform ones within 3.5 sigma, and the per-sample variance falls at least
+    1.5-fold at every t measured (about 2.3 - 2.6 in practice); the best non-adaptive tilt,
+    fitted on half of the uniform sample and evaluated on the other half, also beats the
+    uniform sampler."""
+    import nrcps_amd as na
+    from nrcps_amd import proposal as pr
+
+    L, N, T, C, seed = 8, 7, 16, 16384, 7012
+    V = L ** 3
+    ts = [4, 8, 12]
+    print(f"[gpu] proposal training L={L} N={N} T={T} C={C}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    ens.set_symmetry(True)
+    la_train, _, _ = ens.run(seed, 0, T)
+    S_train = pr.z2_sums(ctx, seed, 0, C, T, V)
+    params = pr.init_params(np.random.default_rng(0), out_scale=0.0)
+    lam = pr.best_constant_tilt(la_train, S_train, ts, V)
+    assert 0 < lam < 0.05  # the determinant favours positive field sums at A > 0
+    params[8240] = 2.0 * lam
+    prop = pr.Proposal(ctx, L, C, params)
+    losses, mu, sd = pr.train(ctx, prop, la_train, seed, 0, 300, 64, 4e-3, ts,
+                              np.random.default_rng(1))
+    assert np.all(np.isfinite(losses))
+    la_p = np.concatenate([ens.run(seed, (1 << 22) + k, T)[0] for k in (0, C)], axis=1)
+    S_p = np.concatenate([pr.z2_sums(ctx, seed, (1 << 22) + k, C, T, V) for k in (0, C)],
+                         axis=1)
+    la_l, _, logr = pr.LearnedSampler(ctx, prop, ens, mu, sd).run(seed + 1, 1 << 24, T)
+    assert np.all(np.isfinite(logr))
+    sp = pr.estimator_stats(la_p, None, ts)
+    sl = pr.estimator_stats(la_l, logr, ts)
+    ceil = pr.tilt_ceiling(la_p, S_p, ts, V)
+    for t in ts:
+        vp = sp[t]["rel_se"] ** 2 * la_p.shape[1]
+        vl = sl[t]["rel_se"] ** 2 * C
+        sig = np.hypot(sp[t]["rel_se"], sl[t]["rel_se"])
+        d = abs(sl[t]["log_C"] - sp[t]["log_C"])
+        print(f"[gpu]   t={t}: |d log C|/sigma {d / sig:.2f}, variance gain {vp / vl:.2f}, "
+              f"best constant tilt {ceil[t]['variance_gain_constant_tilt']:.2f}, per slice "
+              f"{ceil[t]['variance_gain_per_slice_tilt']:.2f}", flush=True)
+        assert d < 3.5 * sig
+        assert vp / vl >= 1.5
+        assert ceil[t]["variance_gain_constant_tilt"] > 1.0
+    prop.close()
+    ens.close()
