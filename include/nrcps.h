@@ -61,8 +61,10 @@ int nrc_memset(nrc_context* ctx, void* p_dev, int byte, size_t bytes);
 int nrc_sync(nrc_context* ctx);
 
 /* ---- FFT: Fourier(batch) / Forward / Backward (include/fourier.h:7-45;
- *      src/fourier/cuda/fourier.cu:9-79).  In-place batched 3D C2C on cubic lattices
- *      L in {4, 6, 8, 12, 16, 24, 32}. */
+ *      src/fourier/cuda/fourier.cu:9-79).  In-place batched unnormalised 3D C2C (sign -1 is
+ *      FFTW_FORWARD) on X x Y x Z lattices with every side in {2, 4, 6, 8, 12, 16, 24, 32,
+ *      48, 64}: cubic L <= 32 on two fused passes, other shapes one launch per axis.  Other
+ *      sides: NRC_ERR_UNSUPPORTED. */
 int nrc_fft_plan(nrc_context* ctx, int lx, int ly, int lz, int batch, int prec, nrc_fft** plan);
 int nrc_fft_exec(nrc_fft* plan, void* buf_dev, int sign);
 int nrc_fft_destroy(nrc_fft* plan);

@@ -193,9 +193,12 @@ class Context:
         _check(self.lib.nrc_sync(self.h), "nrc_sync")
 
     # ---------------------------------------------------------------- operators
-    def fft3d(self, buf: DeviceArray, L: int, batch: int, sign: int, prec: int):
+    def fft3d(self, buf: DeviceArray, L, batch: int, sign: int, prec: int):
+        """In-place 3D DFT of `batch` volumes; L is the side of a cubic lattice or (X, Y, Z)."""
+        lx, ly, lz = (L, L, L) if np.isscalar(L) else tuple(int(v) for v in L)
         plan = C.c_void_p()
-        _check(self.lib.nrc_fft_plan(self.h, L, L, L, batch, prec, C.byref(plan)), "fft_plan")
+        _check(self.lib.nrc_fft_plan(self.h, lx, ly, lz, batch, prec, C.byref(plan)),
+               "fft_plan")
         try:
             _check(self.lib.nrc_fft_exec(plan, buf.ptr, sign), "fft_exec")
         finally:

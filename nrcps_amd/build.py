@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libnrcps_hip.so")
-SOURCES = ["fft_passes.hip", "slice_fused.hip", "slice_ext.hip", "lu_rng.hip", "overlap_mfma.hip", "proposal.hip",
+SOURCES = ["fft_passes.hip", "fft_axis.hip", "slice_fused.hip", "slice_ext.hip", "lu_rng.hip", "overlap_mfma.hip", "proposal.hip",
            "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRC_OFFLOAD_ARCH", "gfx950")

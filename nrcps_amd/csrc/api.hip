@@ -49,6 +49,8 @@ int ctx_n_cu(nrc_context* c) { return c->n_cu; }
 struct nrc_fft {
   nrc_context* ctx;
   int L, batch, prec;
+  int lx, ly, lz;  // the lattice; fused = cubic L <= 32 on the fused passes, else per axis
+  bool fused;
 };
 
 #define NRC_HIP(expr)                                  \
@@ -179,13 +181,20 @@ int nrc_sync(nrc_context* ctx) {
 // ------------------------------------------------------------------ FFT
 int nrc_fft_plan(nrc_context* ctx, int lx, int ly, int lz, int batch, int prec, nrc_fft** plan) {
   if (!ctx || !plan || batch < 0 || !prec_ok(prec)) return NRC_ERR_ARG;
-  if (lx != ly || ly != lz || !nrc::pass_supported(lx)) return NRC_ERR_UNSUPPORTED;
-  *plan = new nrc_fft{ctx, lx, batch, prec};
+  const bool fused = lx == ly && ly == lz && nrc::pass_supported(lx);
+  if (!fused && !(nrc::axis_supported(lx) && nrc::axis_supported(ly) && nrc::axis_supported(lz)))
+    return NRC_ERR_UNSUPPORTED;
+  *plan = new nrc_fft{ctx, lx, batch, prec, lx, ly, lz, fused};
   return NRC_OK;
 }
 
 int nrc_fft_exec(nrc_fft* p, void* buf, int sign) {
   if (!p || !buf || (sign != -1 && sign != 1)) return NRC_ERR_ARG;
+  if (!p->fused) {
+    NRC_HIP(nrc::launch_fft3d_axes(p->prec, buf, p->lx, p->ly, p->lz, p->batch, sign,
+                                   p->ctx->stream));
+    return NRC_OK;
+  }
   // X pass: fft_a is the sign +1 x-transform, fft_b the sign -1 one; YZ pass: fft_a is the
   // sign -1 yz-transform, fft_b the sign +1 one.
   nrc::PassArgs x, yz;

@@ -33,14 +33,15 @@ __device__ __forceinline__ C2<T> cmul_i(C2<T> a) {
 }
 
 // ------------------------------------------------------------------ supported lengths
-// Radix plans: 4 = {4}, 6 = {3,2}, 8 = {8}, 12 = {4,3}, 16 = {4,4}, 24 = {8,3}, 32 = {8,4},
-// 48 = {4,4,3}, 64 = {8,8}.
+// Radix plans: 2 = {2}, 4 = {4}, 6 = {3,2}, 8 = {8}, 12 = {4,3}, 16 = {4,4}, 24 = {8,3},
+// 32 = {8,4}, 48 = {4,4,3}, 64 = {8,8}.
 __host__ __device__ constexpr int fft_nstages(int L) {
-  return L == 4 ? 1 : L == 6 ? 2 : L == 8 ? 1 : L == 12 ? 2 : L == 16 ? 2 : L == 24 ? 2
-       : L == 32 ? 2 : L == 48 ? 3 : L == 64 ? 2 : 0;
+  return L == 2 ? 1 : L == 4 ? 1 : L == 6 ? 2 : L == 8 ? 1 : L == 12 ? 2 : L == 16 ? 2
+       : L == 24 ? 2 : L == 32 ? 2 : L == 48 ? 3 : L == 64 ? 2 : 0;
 }
 __host__ __device__ constexpr int fft_radix(int L, int s) {
-  return L == 4 ? 4
+  return L == 2 ? 2
+       : L == 4 ? 4
        : L == 6 ? (s == 0 ? 3 : 2)
        : L == 8 ? 8
        : L == 12 ? (s == 0 ? 4 : 3)

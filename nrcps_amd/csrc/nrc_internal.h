@@ -71,6 +71,11 @@ struct PassArgs {
 hipError_t launch_xpass(int prec, const PassArgs& a, hipStream_t s);
 hipError_t launch_yzpass(int prec, const PassArgs& a, hipStream_t s);
 bool pass_supported(int L);
+// 3D DFTs of X x Y x Z lattices, one launch per axis (fft_axis.hip): sides in {2, 4, 6, 8, 12,
+// 16, 24, 32, 48, 64}
+bool axis_supported(int n);
+hipError_t launch_fft3d_axes(int prec, void* buf, int lx, int ly, int lz, long batch, int sign,
+                             hipStream_t st);
 
 // One whole slice per orbital volume in a single HBM pass (slice_fused.hip):
 // pre multiplier, x/y/z DFT (-1), mid multiplier, DFT (+1), post multiplier, with the sink

@@ -285,8 +285,6 @@ Float Lattice::Variance() {
 // ------------------------------------------------------------------ Fourier
 Fourier::Fourier(int batch) : batch_size(batch), b(nullptr), plan(nullptr) {
   const char* fname = "Fourier::Fourier(int)";
-  if (GJP.Xsites() != GJP.Ysites() || GJP.Ysites() != GJP.Zsites())
-    ERR.NotImplemented(fname, "non-cubic lattices.");
   b = dev_alloc<fftComplex>((size_t)batch * GJP.Vol(), fname);
   ERR.Check(fname, nrc_fft_plan(ctx(), GJP.Xsites(), GJP.Ysites(), GJP.Zsites(), batch,
                                 NRC_FLOAT_PREC, &plan));

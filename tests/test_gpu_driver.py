@@ -347,3 +347,75 @@ def test_one_body_reweight_driver_matches_oracle(orc):
         want = np.exp(ref["logabs"][:, 0]) * ref["phase"][:, 0]
         np.testing.assert_allclose(got[j, :, 0] + 1j * got[j, :, 1], want, rtol=1e-10)
     assert np.all(np.abs(norms - 1) > 0)  # the ratios are not trivial
+
+
+def _run_slater_det(args: dict, what: str):
+    """Runs the slater_det driver on `args` (ARGS with overrides) in a scratch directory and
+    returns {k: [block][t][re, im]} of results/slater.<k>."""
+    with tempfile.TemporaryDirectory() as tmp:
+        for d in ("args", "results", "rngs"):
+            os.makedirs(os.path.join(tmp, d))
+        for name, text in {**ARGS, **args}.items():
+            with open(os.path.join(tmp, "args", name), "w") as f:
+                f.write(text)
+        print(f"[gpu] running the slater_det driver ({what})", flush=True)
+        r = subprocess.run([DRIVER], cwd=tmp, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out = {}
+        for name in os.listdir(os.path.join(tmp, "results")):
+            if name.startswith("slater."):
+                rows = [list(map(float, line.split())) for line in
+                        open(os.path.join(tmp, "results", name)) if line.strip()]
+                out[int(name.split(".")[1])] = np.array(rows)
+        return out
+
+
+def test_slater_det_driver_l64_matches_oracle(orc):
+    """L = 64 (the side tests/hamiltonian_performance uses): the class layer's Fourier runs
+    one launch per axis with the two-step 64-point transform; 7 momenta (Fermi energy 0.006),
+    the smeared interaction, ranlxs fields, T = 2, one configuration, against the oracle."""
+    L, T = 64, 2
+    got = _run_slater_det({"do.arg": f"{T}\n{L}\n{L}\n{L}\nBOUNDARY_TYPE_PRD\nBOUNDARY_TYPE_PRD\n"
+                                     "BOUNDARY_TYPE_PRD\n1.0\n",
+                           "evo.arg": "0\n1\n1\n",
+                           "momenta.arg": "DISPERSION_TYPE_PERFECT\n0.006\n1.18343\n"},
+                          "L=64, T=2, 1 configuration")
+    cut2 = np.pi ** 2
+    n, moms, _, _ = orc.momenta(L, orc.DISP_PERFECT, 1.18343, 0.006, cutoff_sq=cut2)
+    assert n == 7 and sorted(got) == list(range(1, n + 1))
+    kin = orc.kinetic_xi(L, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    cpl = [9.82275, 2.06919, 0.45723, 0.00223528, 0.0451682, 0.00709158, 0.000244729]
+    f, rc = orc.interaction_table(L, 5.0, orc.INTER_ONEMINUSXIINVSQ, cpl, 1.0, cutoff_sq=cut2)
+    assert rc == 0
+    ref = orc.run_config(L, n, T, moms, kin, smear=f, minors=True, field="ranlxs",
+                         rng=orc.ranlxs(2, 7012))
+    d2 = np.exp(2 * ref["logabs"])
+    for k in range(1, n + 1):
+        row = got[k].reshape(T, 2)
+        np.testing.assert_allclose(row[:, 0], d2[:, k - 1], rtol=1e-10)
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 16), (16, 6, 48)])
+def test_slater_det_driver_non_cubic_free_gas(orc, shape):
+    """An X x Y x Z lattice (src/fourier/noarch/fourier.C:29-39 plans any shape) with the
+    interaction's couplings at zero: plane waves are eigenvectors of every factor, so
+    |det_k(t)|^2 = prod_{n < k} xi_half(p_n)^(-4 t) for the k lowest momenta of the X x Y x Z
+    dispersion (the form the oracle's cubic run gives to 1e-14).  A transposed axis, stride or
+    table index moves a wave onto another momentum and fails this."""
+    X, Y, Z = shape
+    T = 4
+    got = _run_slater_det({"do.arg": f"{T}\n{X}\n{Y}\n{Z}\nBOUNDARY_TYPE_PRD\nBOUNDARY_TYPE_PRD\n"
+                                     "BOUNDARY_TYPE_PRD\n1.0\n",
+                           "evo.arg": "0\n1\n1\n",
+                           "interaction.arg": "5.0\nINTERACTION_TYPE_ONEMINUSXIINVSQ\n0.0\n"},
+                          f"{X}x{Y}x{Z} free gas")
+    cut2 = np.pi ** 2
+    n, moms, _, _ = orc.momenta(shape, orc.DISP_PERFECT, 1.18343, 0.3, cutoff_sq=cut2)
+    assert n > 3 and sorted(got) == list(range(1, n + 1))
+    kin = orc.kinetic_xi(shape, orc.DISP_PERFECT, 1.18343, 0.5, cutoff_sq=cut2)
+    idx = (moms[:, 0] * Y + moms[:, 1]) * Z + moms[:, 2]
+    for k in range(1, n + 1):
+        row = got[k].reshape(T, 2)
+        want = np.array([np.prod(kin[idx[:k]] ** (-4.0 * t)) for t in range(T)])
+        np.testing.assert_allclose(row[:, 0], want, rtol=1e-11)
+        assert np.all(row[:, 1] == 0.0)

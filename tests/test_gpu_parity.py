@@ -44,6 +44,39 @@ def test_fft3d_vs_oracle(ctx, orc, L, prec):
         assert err < tol, (sign, err)
 
 
+@pytest.mark.parametrize("shape", [(4, 6, 8), (8, 12, 16), (2, 4, 64), (64, 64, 64),
+                                   (48, 8, 64), (32, 48, 12), (16, 32, 24), (48, 48, 48),
+                                   (6, 64, 2)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fft3d_general_shapes_vs_oracle(ctx, orc, shape, prec):
+    """X x Y x Z lattices and sides 48, 64: one launch per axis (fft_axis.hip), against the
+    oracle's mixed-radix DFT; every axis length takes the register path (<= 32) or the
+    two-step LDS path (48, 64), on strided axes and on the contiguous z axis."""
+    import nrcps_amd as na
+
+    print(f"[gpu] fft shape={shape} prec={prec}", flush=True)
+    rng = np.random.default_rng(sum(shape) * 10 + prec)
+    batch = 2 if np.prod(shape) <= 65536 else 1
+    a = _rand_c(rng, (batch,) + tuple(shape))
+    dt = np.complex64 if prec == na.PREC_F32 else np.complex128
+    tol = 3e-6 if prec == na.PREC_F32 else 2e-13
+    for sign in (-1, 1):
+        d = ctx.to_device(a.astype(dt))
+        ctx.fft3d(d, shape, batch, sign, prec)
+        got = d.download()
+        ref = orc.fft3d(a.astype(dt).astype(np.complex128), sign)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < tol, (sign, err)
+
+
+def test_fft3d_unsupported_side_is_refused(ctx):
+    import nrcps_amd as na
+
+    d = ctx.to_device(np.zeros((10, 8, 8), np.complex64))
+    with pytest.raises(na.NrcError):
+        ctx.fft3d(d, (10, 8, 8), 1, -1, na.PREC_F32)
+
+
 def test_fft3d_plane_wave_delta_f32(ctx):
     L = 32
     x = np.arange(L)

@@ -49,6 +49,20 @@ def test_fft3d_matches_numpy(orc, L):
                                atol=1e-12 * L ** 3)
 
 
+@pytest.mark.parametrize("shape", [(4, 6, 8), (2, 4, 64), (48, 8, 64), (32, 48, 12),
+                                   (6, 64, 2), (64, 64, 64)])
+def test_fft3d_non_cubic_matches_numpy(orc, shape):
+    """X x Y x Z lattices and sides 48, 64: the axes are transformed with their own lengths
+    and strides (a transposed stride or length fails this)."""
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal((1,) + shape) + 1j * rng.standard_normal((1,) + shape)
+    V = int(np.prod(shape))
+    np.testing.assert_allclose(orc.fft3d(a, -1), np.fft.fftn(a, axes=(1, 2, 3)), rtol=0,
+                               atol=1e-12 * V)
+    np.testing.assert_allclose(orc.fft3d(a, +1), np.fft.ifftn(a, axes=(1, 2, 3)) * V, rtol=0,
+                               atol=1e-12 * V)
+
+
 def test_fft3d_plane_wave_is_delta_and_roundtrip(orc):
     L = 8
     x = np.arange(L)
