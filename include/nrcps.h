@@ -106,6 +106,33 @@ int nrc_overlap(nrc_context* ctx, const void* phi_dev, const void* psi_dev, doub
  * overlap in time (other streams, other contexts of the device) race. */
 int nrc_overlap_noalloc(nrc_context* ctx, const void* phi_dev, const void* psi_dev,
                         double* M_dev, int ns, int nt, long vol, int ncfg, int mode);
+/* GEMM variants of the split-BF16 modes (a plan chosen by measurement, as
+ * src/fourier/noarch/fourier.C:30-39 plans its FFTs with FFTW_EXHAUSTIVE):
+ *   NRC_OVL_COLS        a real and an imaginary B column per orbital, built in registers,
+ *                       four waves per configuration;
+ *   NRC_OVL_SHARED      two A rows per sink ((re, im) and (-im, re)), split once per call
+ *                       and staged in LDS for the four configurations of a workgroup (one
+ *                       wave each): 2-step stages, two workgroups per CU;
+ *   NRC_OVL_SHARED_DEEP the same with 4-step stages and 4 steps of orbital loads in flight,
+ *                       one workgroup per CU;
+ *   NRC_OVL_SHARED_PIPE 2-step stages with 4 steps of orbital loads in flight, two
+ *                       workgroups per CU.
+ * A call uses the context's choice for (ns, nt, vol, mode), else NRC_OVL_SHARED (also for
+ * ensembles with dense FP32 sinks).  set_variant pins a choice (NRC_OVL_AUTO clears it);
+ * nrc_overlap_tune times every variant (reps timed calls each after an untimed one) on the
+ * caller's data, keeps the fastest for the shape and returns it, with each variant's best ms
+ * in ms[4] (COLS, SHARED, SHARED_DEEP, SHARED_PIPE order).  The variants differ in FP32 summation order
+ * only. */
+#define NRC_OVL_AUTO 0
+#define NRC_OVL_COLS 1
+#define NRC_OVL_SHARED 2
+#define NRC_OVL_SHARED_DEEP 3
+#define NRC_OVL_SHARED_PIPE 4
+int nrc_overlap_set_variant(nrc_context* ctx, int ns, int nt, long vol, int mode, int variant);
+int nrc_overlap_variant(nrc_context* ctx, int ns, int nt, long vol, int mode, int* variant);
+int nrc_overlap_tune(nrc_context* ctx, const void* phi_dev, const void* psi_dev, double* M_dev,
+                     int ns, int nt, long vol, int ncfg, int mode, int reps, int* best,
+                     double* ms);
 
 /* ---- the pair-sink matrix: TwoBody::Run(Float*, Float*) and Run(Float**, Float**,
  *      complex<Float>*, int) (include/two_body.h:31-35; src/two_body/noarch/two_body_run.C:
@@ -251,9 +278,26 @@ int nrc_ensemble_kernel_time(nrc_ensemble* ens, int kind, double* total_ms, long
 /* The per-slice results of the ensemble's last run (nrc_ensemble_run, _run_fields or a batch
  * run), [t][c][m] as nrc_ensemble_run returns them; any output may be NULL. */
 int nrc_ensemble_results(nrc_ensemble* ens, double* logabs, double* phase, double* argsum);
-/* Autotuner: times every evolution path the ensemble's shape supports (two-pass FFT passes,
- * single-pass slice kernel) over T_probe slices and keeps the fastest for later runs. */
+/* Autotuner (a plan chosen by measurement, as src/fourier/noarch/fourier.C:30-39 plans with
+ * FFTW_EXHAUSTIVE): times over T_probe slices every evolution path the ensemble's shape
+ * supports (two-pass FFT passes, single-pass slice kernel); with the path chosen, the slices
+ * per launch of the persistent kernels (4, 2, 1; where the path fuses slices, over at least 4
+ * slices); with dense FP32 sinks, the overlap GEMM variants (NRC_OVL_*) on the ensemble's own
+ * buffers.  Keeps the fastest of each for later runs and stores the choice in the context
+ * under the ensemble's shape key (L, N, precision, split, volumes per configuration,
+ * accuracy, measurement kind, configurations), where later ensembles of the same key find it
+ * unless nrc_ensemble_set_path chose a path.  Returns the chosen path and its ms per slice.
+ * tune_report: the i-th timing of the last tune (kind 0 path, 1 slices per launch, 2 GEMM
+ * variant; value; ms per slice, for kind 2 ms per measurement), NRC_ERR_ARG past the end.
+ * launch_slices: the slices per launch the ensemble's next run uses. */
 int nrc_ensemble_tune(nrc_ensemble* ens, int T_probe, int* path, double* ms_per_slice);
+int nrc_ensemble_tune_report(nrc_ensemble* ens, int i, int* kind, int* value, double* ms);
+int nrc_ensemble_launch_slices(nrc_ensemble* ens, int* slices);
+/* The context's tuned choices (GEMM variants, ensemble paths and slices per launch) to and
+ * from a text file, so that a later process starts with them.  A malformed file is refused
+ * (NRC_ERR_ARG) and changes nothing; loaded entries replace those of the same key. */
+int nrc_tune_save(nrc_context* ctx, const char* path);
+int nrc_tune_load(nrc_context* ctx, const char* path);
 int nrc_ensemble_destroy(nrc_ensemble* ens);
 
 /* ---- batches of uneven shapes: many configurations of different (L, N) evolved together,

@@ -21,6 +21,7 @@ SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3, OVERLAP_BF16X6 = 0, 1, 2, 3
 PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
 ACC_FP32, ACC_X64 = 0, 1
+OVL_AUTO, OVL_COLS, OVL_SHARED, OVL_SHARED_DEEP, OVL_SHARED_PIPE = 0, 1, 2, 3, 4
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
 
 # every symbol include/nrcps.h declares
@@ -34,6 +35,8 @@ EXPORTS = [
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
     "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc", "nrc_z2_sums",
     "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
+    "nrc_overlap_set_variant", "nrc_overlap_variant", "nrc_overlap_tune",
+    "nrc_ensemble_tune_report", "nrc_ensemble_launch_slices", "nrc_tune_save", "nrc_tune_load",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -82,6 +85,9 @@ def load_library() -> C.CDLL:
         "nrc_plus_one_div_vol": [vp, vp, C.c_long, i],
         "nrc_overlap": [vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_overlap_noalloc": [vp, vp, vp, vp, i, i, C.c_long, i, i],
+        "nrc_overlap_set_variant": [vp, i, i, C.c_long, i, i],
+        "nrc_overlap_variant": [vp, i, i, C.c_long, i, C.POINTER(i)],
+        "nrc_overlap_tune": [vp, vp, vp, vp, i, i, C.c_long, i, i, i, C.POINTER(i), dp],
         "nrc_pair_overlap": [vp, vp, vp, vp, vp, vp, i, i, C.c_long, i, i],
         "nrc_lu_det": [vp, vp, i, i, i, i, vp, vp, vp, vp],
         "nrc_ludcmp": [vp, vp, i, i, vp, vp, vp], "nrc_ref_abs": [vp, vp, vp, C.c_long],
@@ -100,6 +106,9 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_kernel_time": [vp, i, C.POINTER(C.c_double), C.POINTER(C.c_long)],
         "nrc_ensemble_results": [vp, dp, dp, dp],
         "nrc_ensemble_tune": [vp, i, C.POINTER(i), C.POINTER(C.c_double)],
+        "nrc_ensemble_tune_report": [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_double)],
+        "nrc_ensemble_launch_slices": [vp, C.POINTER(i)],
+        "nrc_tune_save": [vp, C.c_char_p], "nrc_tune_load": [vp, C.c_char_p],
         "nrc_ensemble_destroy": [vp],
         "nrc_batch_create": [vp, vp, i, C.POINTER(vp)],
         "nrc_batch_run": [vp, u64, vp, i, i],
@@ -215,6 +224,32 @@ class Context:
     def overlap(self, phi, psi, M, ns, nt, vol, ncfg, mode):
         _check(self.lib.nrc_overlap(self.h, phi.ptr, psi.ptr, M.ptr, ns, nt, vol, ncfg, mode),
                "overlap")
+
+    def tune_save(self, path: str):
+        """The context's tuned choices to a text file (nrc_tune_save)."""
+        _check(self.lib.nrc_tune_save(self.h, os.fsencode(path)), "tune_save")
+
+    def tune_load(self, path: str):
+        _check(self.lib.nrc_tune_load(self.h, os.fsencode(path)), "tune_load")
+
+    def set_overlap_variant(self, ns, nt, vol, mode, variant):
+        """Pins the split-BF16 GEMM variant (OVL_*) for a shape; OVL_AUTO clears the pin."""
+        _check(self.lib.nrc_overlap_set_variant(self.h, ns, nt, vol, mode, variant),
+               "overlap_set_variant")
+
+    def overlap_variant(self, ns, nt, vol, mode) -> int:
+        v = C.c_int()
+        _check(self.lib.nrc_overlap_variant(self.h, ns, nt, vol, mode, C.byref(v)),
+               "overlap_variant")
+        return v.value
+
+    def overlap_tune(self, phi, psi, M, ns, nt, vol, ncfg, mode, reps=3):
+        """Times every GEMM variant, keeps the fastest: (best, {variant: ms})."""
+        best, ms = C.c_int(), np.zeros(4)
+        _check(self.lib.nrc_overlap_tune(self.h, phi.ptr, psi.ptr, M.ptr, ns, nt, vol, ncfg,
+                                         mode, reps, C.byref(best), _ptr(ms)), "overlap_tune")
+        return best.value, dict(zip((OVL_COLS, OVL_SHARED, OVL_SHARED_DEEP, OVL_SHARED_PIPE),
+                                    ms.tolist()))
 
     def scale_const(self, buf, n, d, op, prec):
         """nrc_scale_const: buf *= d (op 0) or /= d (op 1) over n complex values."""
@@ -410,6 +445,22 @@ class Ensemble:
         _check(self.lib.nrc_ensemble_tune(self.h, T_probe, C.byref(p), C.byref(ms)),
                "ensemble_tune")
         return p.value, ms.value
+
+    def tune_report(self):
+        """The last tune's timings: [(kind, value, ms)], kind 'path', 'slices' or 'gemm'."""
+        out, i = [], 0
+        k, v, ms = C.c_int(), C.c_int(), C.c_double()
+        names = {0: "path", 1: "slices", 2: "gemm"}
+        while self.lib.nrc_ensemble_tune_report(self.h, i, C.byref(k), C.byref(v),
+                                                C.byref(ms)) == OK:
+            out.append((names[k.value], v.value, ms.value))
+            i += 1
+        return out
+
+    def launch_slices(self) -> int:
+        n = C.c_int()
+        _check(self.lib.nrc_ensemble_launch_slices(self.h, C.byref(n)), "ensemble_launch_slices")
+        return n.value
 
     def close(self):
         # an ensemble lives on its context's device; after nrc_finalize it is gone with it.

@@ -3,7 +3,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <cstdio>
+#include <map>
 #include <vector>
 
 #include "nrc_device.h"
@@ -16,6 +19,10 @@ struct nrc_context {
   int n_cu = 0;
   void* work = nullptr;  // grow-only workspace of the library's own calls
   size_t work_bytes = 0;
+  // tuned choices (nrc_overlap_tune, nrc_ensemble_tune; nrc_tune_save / nrc_tune_load):
+  // GEMM variant per (ns, nt, vol, terms); (path, slices per launch) per ensemble shape key
+  std::map<std::array<long, 4>, int> ovl_choice;
+  std::map<std::array<long, 8>, std::array<int, 2>> ens_choice;
 };
 
 // The scratch of the no-allocation overlap (nrc_overlap_noalloc): a fixed device array that
@@ -262,25 +269,98 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf, long vol, int prec) {
 // ------------------------------------------------------------------ overlaps, LU, fields
 }  // extern "C"
 
-// workspace of one split-BF16x3 overlap call: the split sinks (hi and lo bf16) and the
-// per-chunk FP32 partial tiles
-// workspace of one split-BF16 overlap call: the split sinks (3 bf16 planes, room for the
-// BF16x6 split) and the per-chunk FP32 partial tiles
-static size_t overlap_split_bytes(int ns, long vol) {
-  return ((size_t)3 * 2 * ns * vol * 2 + 255) / 256 * 256;
+// The split-BF16 overlap GEMM comes in variants (NRC_OVL_*): the column form (the orbital
+// side builds a real and an imaginary column per orbital, four waves per configuration), and
+// the sink-row form (two rows per sink, split once, staged in LDS for the four configurations
+// of a workgroup) with 2-step or 4-step stages.  A call's variant is the context's tuned
+// choice for (ns, nt, vol, terms), else the default below.
+static const int kOvlVariants[] = {NRC_OVL_COLS, NRC_OVL_SHARED, NRC_OVL_SHARED_DEEP,
+                                   NRC_OVL_SHARED_PIPE};
+constexpr int kNumOvlVariants = 4;
+
+static int overlap_variant(const nrc_context* c, int ns, int nt, long vol, int terms) {
+  if (c) {
+    const auto it = c->ovl_choice.find({ns, nt, vol, terms});
+    if (it != c->ovl_choice.end()) return it->second;
+  }
+  return NRC_OVL_SHARED;
 }
-static size_t overlap_bf16x3_workspace(int ns, int nt, long vol, int ncfg) {
-  return overlap_split_bytes(ns, vol) +
-         nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg) * sizeof(float);
+
+// workspace of one split-BF16 overlap call: the split sinks (up to 3 bf16 planes; 2 ns rows
+// in the row form) and the per-chunk FP32 partial tiles; variant NRC_OVL_AUTO sizes for any
+static size_t overlap_split_bytes(int ns, long vol, int variant) {
+  const size_t rows = variant == NRC_OVL_COLS ? (size_t)ns : (size_t)2 * ns;
+  return ((size_t)3 * rows * 2 * vol * 2 + 255) / 256 * 256;
+}
+static size_t overlap_bf16x3_workspace(int ns, int nt, long vol, int ncfg,
+                                       int variant = NRC_OVL_AUTO) {
+  if (variant == NRC_OVL_AUTO) {
+    size_t m = 0;
+    for (int v : kOvlVariants) m = std::max(m, overlap_bf16x3_workspace(ns, nt, vol, ncfg, v));
+    return m;
+  }
+  const size_t partial = variant == NRC_OVL_COLS
+                             ? nrc::overlap_bf16x3_partial_floats(ns, nt, vol, ncfg)
+                             : nrc::overlap_rows_partial_floats(ns, nt, vol, ncfg);
+  return overlap_split_bytes(ns, vol, variant) + partial * sizeof(float);
 }
 // terms 3: split-BF16x3; 6: split-BF16x6 (three-way splits, six products)
 static int overlap_bf16x3(const void* phi, const void* psi, double* M, int ns, int nt, long vol,
-                          int ncfg, void* work, hipStream_t s, int terms = 3) {
+                          int ncfg, void* work, hipStream_t s, int terms, int variant) {
   char* w = (char*)work;
-  const size_t split = overlap_split_bytes(ns, vol);
-  NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s, terms == 6 ? 3 : 2));
-  NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
-                                     (float*)(w + split), s, terms));
+  const size_t split = overlap_split_bytes(ns, vol, variant);
+  const int parts = terms == 6 ? 3 : 2;
+  if (variant == NRC_OVL_COLS) {
+    NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s, parts));
+    NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
+                                       (float*)(w + split), s, terms));
+    return NRC_OK;
+  }
+  const int form = variant == NRC_OVL_SHARED        ? 0
+                   : variant == NRC_OVL_SHARED_DEEP ? 1
+                   : variant == NRC_OVL_SHARED_PIPE ? 2
+                                                    : -1;
+  if (form < 0) return NRC_ERR_ARG;
+  NRC_HIP(nrc::launch_split_rows((const float2*)phi, w, ns, vol, s, parts));
+  NRC_HIP(nrc::launch_overlap_lds(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
+                                  (float*)(w + split), s, terms, form));
+  return NRC_OK;
+}
+
+// times every variant `reps` times on `s` (HIP events, after one untimed call each), keeps
+// the fastest in the context's table and returns it; ms[i] the best time of kOvlVariants[i]
+static int overlap_tune(nrc_context* c, const void* phi, const void* psi, double* M, int ns,
+                        int nt, long vol, int ncfg, void* work, hipStream_t s, int terms,
+                        int reps, int* best_out, double* ms_out) {
+  hipEvent_t a, b;
+  NRC_HIP(hipEventCreate(&a));
+  NRC_HIP(hipEventCreate(&b));
+  int best = NRC_OVL_SHARED, rc = NRC_OK;
+  float best_ms = -1.f;
+  for (int i = 0; i < kNumOvlVariants && rc == NRC_OK; ++i) {
+    const int v = kOvlVariants[i];
+    float vbest = -1.f;
+    for (int r = 0; r <= reps && rc == NRC_OK; ++r) {
+      hipEventRecord(a, s);
+      rc = overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, work, s, terms, v);
+      hipEventRecord(b, s);
+      if (rc != NRC_OK) break;
+      if (hipEventSynchronize(b) != hipSuccess) rc = NRC_ERR_HIP;
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, a, b);
+      if (r > 0 && (vbest < 0.f || ms < vbest)) vbest = ms;  // r = 0 is the untimed call
+    }
+    if (ms_out) ms_out[i] = vbest;
+    if (rc == NRC_OK && (best_ms < 0.f || vbest < best_ms)) {
+      best_ms = vbest;
+      best = v;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc != NRC_OK) return rc;
+  c->ovl_choice[{ns, nt, vol, terms}] = best;
+  if (best_out) *best_out = best;
   return NRC_OK;
 }
 
@@ -298,11 +378,43 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
   if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
   if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
   if (ncfg == 0) return NRC_OK;
-  const size_t bytes = overlap_bf16x3_workspace(ns, nt, vol, ncfg);
-  void* w = ctx_workspace(ctx, bytes);
+  const int terms = mode == NRC_OVERLAP_BF16X6 ? 6 : 3;
+  const int v = overlap_variant(ctx, ns, nt, vol, terms);
+  void* w = ctx_workspace(ctx, overlap_bf16x3_workspace(ns, nt, vol, ncfg, v));
   if (!w) return NRC_ERR_NOMEM;
-  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
-                        mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
+  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream, terms, v);
+}
+
+int nrc_overlap_set_variant(nrc_context* ctx, int ns, int nt, long vol, int mode, int variant) {
+  if (!ctx || ns < 1 || nt < 1 || vol < 1) return NRC_ERR_ARG;
+  if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  const int terms = mode == NRC_OVERLAP_BF16X6 ? 6 : 3;
+  if (variant == NRC_OVL_AUTO) {
+    ctx->ovl_choice.erase({ns, nt, vol, terms});
+    return NRC_OK;
+  }
+  if (variant < NRC_OVL_COLS || variant > NRC_OVL_SHARED_PIPE) return NRC_ERR_ARG;
+  ctx->ovl_choice[{ns, nt, vol, terms}] = variant;
+  return NRC_OK;
+}
+
+int nrc_overlap_variant(nrc_context* ctx, int ns, int nt, long vol, int mode, int* variant) {
+  if (!ctx || !variant) return NRC_ERR_ARG;
+  if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  *variant = overlap_variant(ctx, ns, nt, vol, mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
+  return NRC_OK;
+}
+
+int nrc_overlap_tune(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns,
+                     int nt, long vol, int ncfg, int mode, int reps, int* best, double* ms) {
+  if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 1 || reps < 1)
+    return NRC_ERR_ARG;
+  if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
+  void* w = ctx_workspace(ctx, overlap_bf16x3_workspace(ns, nt, vol, ncfg));
+  if (!w) return NRC_ERR_NOMEM;
+  return overlap_tune(ctx, phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
+                      mode == NRC_OVERLAP_BF16X6 ? 6 : 3, reps, best, ms);
 }
 
 // nrc_overlap without allocating: the split sinks and the partial tiles of the split-BF16
@@ -321,12 +433,13 @@ int nrc_overlap_noalloc(nrc_context* ctx, const void* phi, const void* psi, doub
   if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
   if (ns > 64 || nt > 64 || (vol & 3)) return NRC_ERR_UNSUPPORTED;
   if (ncfg == 0) return NRC_OK;
-  if (overlap_bf16x3_workspace(ns, nt, vol, ncfg) > NRC_SCRATCH_BYTES)
+  const int terms = mode == NRC_OVERLAP_BF16X6 ? 6 : 3;
+  const int v = overlap_variant(ctx, ns, nt, vol, terms);
+  if (overlap_bf16x3_workspace(ns, nt, vol, ncfg, v) > NRC_SCRATCH_BYTES)
     return NRC_ERR_TOO_LARGE;
   void* w = ctx->device < NRC_MAX_DEVICES ? g_scratch_dev[ctx->device] : nullptr;
   if (!w) return NRC_ERR_HIP;
-  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream,
-                        mode == NRC_OVERLAP_BF16X6 ? 6 : 3);
+  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream, terms, v);
 }
 
 int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const double* w,
@@ -455,6 +568,9 @@ struct nrc_ensemble {
   double* trap_q = nullptr;  // the trap q per site (FP64), for the smeared field
   double2* pgh = nullptr;    // pair sinks with partners: G then H, [C][nvo][nvo] each
   int path = NRC_PATH_AUTO;
+  bool user_path = false;        // nrc_ensemble_set_path chose the path (no tuned lookup)
+  int fuse = nrc::SLICE_FUSE_MAX;  // slices per launch of the persistent slice kernels
+  std::vector<std::array<double, 3>> tune_log;  // the last tune: (kind, value, ms per slice)
   int acc = NRC_ACC_X64;         // FP32 ensembles: nrc_ensemble_set_accuracy
   uint64_t* par_dev = nullptr;   // the run's (seed, config0), read by the field kernels
   uint64_t* par_host = nullptr;  // pinned staging of par_dev
@@ -792,6 +908,8 @@ extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
   if (path == NRC_PATH_SINGLE_PASS && !nrc::slice1_supported(e->d.prec, e->d.L))
     return NRC_ERR_UNSUPPORTED;
   e->path = path;
+  e->user_path = path != NRC_PATH_AUTO;
+  e->fuse = nrc::SLICE_FUSE_MAX;
   return NRC_OK;
 }
 
@@ -1022,9 +1140,11 @@ static int ens_measure(nrc_ensemble* e, int t) {
     const void* sk = e->pending_x ? e->sinks_mx : e->folded ? e->sinks_k : e->sinks;
     if (e->sinks && prec == NRC_PREC_F64)
       NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
-    else if (e->sinks)
-      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s,
-                             e->acc == NRC_ACC_X64 ? 6 : 3));
+    else if (e->sinks) {
+      const int terms = e->acc == NRC_ACC_X64 ? 6 : 3;
+      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s, terms,
+                             overlap_variant(e->ctx, N, N, V, terms)));
+    }
     else {
       nrc::ConjCols cp;
       if (e->sym) {
@@ -1310,7 +1430,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   if (ens_xchain(e, single)) {  // FP32 state, FP64 x-chain, measurements inside the launches
     e->pending_x = false;
     for (int t = 0; t < T;) {
-      const int S = (e->sinks || e->pair) ? 1 : std::min(nrc::SLICE_FUSE_MAX, T - t);
+      const int S = (e->sinks || e->pair) ? 1 : std::min(e->fuse, T - t);
       if (e->sinks || e->pair) NRC_TRY(ens_measure(e, t));  // dense or pair sinks: before
       if (!host_words) {
         EvScope sc(e, 3);
@@ -1328,7 +1448,7 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   const bool fuse = single && ref && !e->smear && !e->sinks && !e->pair &&
                     (!e->trap_tab || nrc::slice1p_trap_ok(prec, L));
   for (int t = 0; single && t < T;) {
-    const int S = fuse ? std::min(nrc::SLICE_FUSE_MAX, T - t) : 1;
+    const int S = fuse ? std::min(e->fuse, T - t) : 1;
     if (measure_first) NRC_TRY(ens_measure(e, t));
     if (!host_words) {
       EvScope sc(e, 3);
@@ -1423,8 +1543,25 @@ static int ens_collect(nrc_ensemble* e, int T, double* logabs, double* phase, do
   return ens_collect_profile(e);
 }
 
+// the key of an ensemble's tuned choice: shape, precision, layout and measurement kind
+static std::array<long, 8> ens_tune_key(const nrc_ensemble* e) {
+  const long flags = (e->sinks ? 1 : 0) | (e->pair ? 2 : 0) | (e->trap_tab ? 4 : 0) |
+                     (e->smear ? 8 : 0);
+  return {e->d.L, e->d.n_orb, e->d.prec, e->d.split, e->nvo, e->acc, flags, e->d.n_configs};
+}
+
+// a tuned (path, slices per launch) for the ensemble's key, unless the caller chose the path
+static void ens_apply_tuned(nrc_ensemble* e) {
+  if (e->user_path || ens_in_batch(e)) return;
+  const auto it = e->ctx->ens_choice.find(ens_tune_key(e));
+  if (it == e->ctx->ens_choice.end()) return;
+  e->path = it->second[0];
+  e->fuse = it->second[1];
+}
+
 static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
                    int T, double* logabs, double* phase, double* argsum) {
+  ens_apply_tuned(e);
   NRC_TRY(ens_prepare(e, words_host, T));
   NRC_TRY(ens_set_params(e, seed, config0, e->stream));
   NRC_TRY(ens_enqueue(e, seed, config0, words_host != nullptr, T));
@@ -1715,48 +1852,176 @@ extern "C" int nrc_ensemble_results(nrc_ensemble* e, double* logabs, double* pha
 }
 
 // ------------------------------------------------------------------ autotuner
-// Times every evolution path the ensemble's shape supports over a few slices (HIP events on
-// the ensemble's stream, after one untimed slice) and keeps the fastest.
+// Times the ensemble's variants over T_probe slices (HIP events on the ensemble's stream,
+// after one untimed slice): the evolution paths its shape supports; with the path chosen,
+// the slices per launch of the persistent kernels (1, 2 or 4, where the path fuses slices);
+// with dense FP32 sinks, the overlap GEMM variants on the ensemble's own buffers.  Keeps the
+// fastest of each, records every timing (nrc_ensemble_tune_report) and stores the choice in
+// the context under the ensemble's key (nrc_tune_save / nrc_tune_load), from where later
+// runs of ensembles with the same key take it.
+static int ens_time_slices(nrc_ensemble* e, int T_probe, hipEvent_t a, hipEvent_t b, float* ms) {
+  NRC_TRY(ens_run(e, 0x5eedULL, 0xfffff000u, nullptr, 1, nullptr, nullptr, nullptr));
+  NRC_TRY(ens_prepare(e, nullptr, T_probe));
+  NRC_TRY(ens_set_params(e, 0x5eedULL, 0xfffff000u, e->stream));
+  hipEventRecord(a, e->stream);
+  NRC_TRY(ens_enqueue(e, 0x5eedULL, 0xfffff000u, false, T_probe));
+  hipEventRecord(b, e->stream);
+  if (hipEventSynchronize(b) != hipSuccess) return NRC_ERR_HIP;
+  hipEventElapsedTime(ms, a, b);
+  return NRC_OK;
+}
+
 extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
                                  double* ms_per_slice) {
   if (!e || T_probe < 1) return NRC_ERR_ARG;
   if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   std::vector<int> cand = {NRC_PATH_TWO_PASS};
   if (nrc::slice1_supported(e->d.prec, e->d.L)) cand.push_back(NRC_PATH_SINGLE_PASS);
-  if (e->sinks) cand = {NRC_PATH_SINGLE_PASS};
+  if (e->sinks || e->pair) cand = {NRC_PATH_SINGLE_PASS};
   hipEvent_t a, b;
   NRC_HIP(hipEventCreate(&a));
   NRC_HIP(hipEventCreate(&b));
-  int best = cand[0];
+  e->tune_log.clear();
+  const bool user_path = e->user_path;
+  e->user_path = true;  // no tuned lookup while timing
+  int best = cand[0], best_fuse = nrc::SLICE_FUSE_MAX, rc = NRC_OK;
   float best_ms = -1.f;
-  int rc = NRC_OK;
+  e->fuse = nrc::SLICE_FUSE_MAX;
   for (int p : cand) {
     e->path = p;
-    rc = ens_run(e, 0x5eedULL, 0xfffff000u, nullptr, 1, nullptr, nullptr, nullptr);
-    if (rc != NRC_OK) break;
-    rc = ens_prepare(e, nullptr, T_probe);
-    if (rc != NRC_OK) break;
-    rc = ens_set_params(e, 0x5eedULL, 0xfffff000u, e->stream);
-    if (rc != NRC_OK) break;
-    hipEventRecord(a, e->stream);
-    rc = ens_enqueue(e, 0x5eedULL, 0xfffff000u, false, T_probe);
-    hipEventRecord(b, e->stream);
-    if (rc != NRC_OK) break;
-    if (hipEventSynchronize(b) != hipSuccess) { rc = NRC_ERR_HIP; break; }
     float ms = 0.f;
-    hipEventElapsedTime(&ms, a, b);
+    rc = ens_time_slices(e, T_probe, a, b, &ms);
+    if (rc != NRC_OK) break;
+    e->tune_log.push_back({0.0, (double)p, ms / T_probe});
     if (best_ms < 0.f || ms < best_ms) {
       best_ms = ms;
       best = p;
     }
   }
+  e->path = best;
+  // slices per launch: only where the chosen path fuses slices into one launch
+  const bool fuses = rc == NRC_OK && !e->sinks && !e->pair && best == NRC_PATH_SINGLE_PASS &&
+                     e->d.split == NRC_SPLIT_KSPACE && !e->smear;
+  if (fuses) {
+    const int Tf = std::max(T_probe, nrc::SLICE_FUSE_MAX);
+    float base = -1.f;
+    for (int f : {nrc::SLICE_FUSE_MAX, 2, 1}) {
+      e->fuse = f;
+      float ms = 0.f;
+      rc = ens_time_slices(e, Tf, a, b, &ms);
+      if (rc != NRC_OK) break;
+      e->tune_log.push_back({1.0, (double)f, ms / Tf});
+      if (base < 0.f || ms < base) {
+        base = ms;
+        best_fuse = f;
+      }
+    }
+    if (rc == NRC_OK) best_ms = base * T_probe / Tf;
+  }
+  e->fuse = best_fuse;
+  // dense FP32 sinks: the overlap GEMM variants on the ensemble's buffers
+  if (rc == NRC_OK && e->sinks && e->d.prec == NRC_PREC_F32) {
+    const int terms = e->acc == NRC_ACC_X64 ? 6 : 3;
+    const int N = e->d.n_orb;
+    double ms[kNumOvlVariants];
+    int v = 0;
+    rc = overlap_tune(e->ctx, e->sinks, e->psi, (double*)e->M, N, N, e->V, (int)e->d.n_configs,
+                      e->ovl_work, e->stream, terms, 2, &v, ms);
+    for (int i = 0; rc == NRC_OK && i < kNumOvlVariants; ++i)
+      e->tune_log.push_back({2.0, (double)kOvlVariants[i], ms[i]});
+    if (rc == NRC_OK) {  // the slice time with the chosen GEMM
+      float t = 0.f;
+      rc = ens_time_slices(e, T_probe, a, b, &t);
+      best_ms = t;
+    }
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
+  e->user_path = user_path;
   e->last_T = 0;
   if (rc != NRC_OK) return rc;
-  e->path = best;
+  e->ctx->ens_choice[ens_tune_key(e)] = {best, best_fuse};
   if (path_out) *path_out = best;
   if (ms_per_slice) *ms_per_slice = best_ms / T_probe;
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_tune_report(nrc_ensemble* e, int i, int* kind, int* value,
+                                        double* ms) {
+  if (!e || i < 0) return NRC_ERR_ARG;
+  if ((size_t)i >= e->tune_log.size()) return NRC_ERR_ARG;
+  if (kind) *kind = (int)e->tune_log[i][0];
+  if (value) *value = (int)e->tune_log[i][1];
+  if (ms) *ms = e->tune_log[i][2];
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_launch_slices(nrc_ensemble* e, int* slices) {
+  if (!e || !slices) return NRC_ERR_ARG;
+  ens_apply_tuned(e);
+  *slices = e->fuse;
+  return NRC_OK;
+}
+
+// ------------------------------------------------------------------ tuned choices on disk
+// A text file: a header line, then one line per choice,
+//   overlap <ns> <nt> <vol> <terms> <variant>
+//   ensemble <L> <N> <prec> <split> <volumes> <accuracy> <flags> <configs> <path> <slices>
+extern "C" int nrc_tune_save(nrc_context* ctx, const char* path) {
+  if (!ctx || !path) return NRC_ERR_ARG;
+  FILE* f = fopen(path, "w");
+  if (!f) return NRC_ERR_ARG;
+  fprintf(f, "nrcps-tune 1\n");
+  for (const auto& kv : ctx->ovl_choice)
+    fprintf(f, "overlap %ld %ld %ld %ld %d\n", kv.first[0], kv.first[1], kv.first[2],
+            kv.first[3], kv.second);
+  for (const auto& kv : ctx->ens_choice) {
+    fprintf(f, "ensemble");
+    for (long v : kv.first) fprintf(f, " %ld", v);
+    fprintf(f, " %d %d\n", kv.second[0], kv.second[1]);
+  }
+  return fclose(f) == 0 ? NRC_OK : NRC_ERR_ARG;
+}
+
+extern "C" int nrc_tune_load(nrc_context* ctx, const char* path) {
+  if (!ctx || !path) return NRC_ERR_ARG;
+  FILE* f = fopen(path, "r");
+  if (!f) return NRC_ERR_ARG;
+  char tag[32];
+  int version = 0;
+  int rc = (fscanf(f, "%31s %d", tag, &version) == 2 && !strcmp(tag, "nrcps-tune") &&
+            version == 1)
+               ? NRC_OK
+               : NRC_ERR_ARG;
+  std::map<std::array<long, 4>, int> ovl;
+  std::map<std::array<long, 8>, std::array<int, 2>> ens;
+  while (rc == NRC_OK && fscanf(f, "%31s", tag) == 1) {
+    if (!strcmp(tag, "overlap")) {
+      std::array<long, 4> k;
+      int v;
+      if (fscanf(f, "%ld %ld %ld %ld %d", &k[0], &k[1], &k[2], &k[3], &v) != 5 ||
+          v < NRC_OVL_COLS || v > NRC_OVL_SHARED_PIPE)
+        rc = NRC_ERR_ARG;
+      else
+        ovl[k] = v;
+    } else if (!strcmp(tag, "ensemble")) {
+      std::array<long, 8> k;
+      std::array<int, 2> v;
+      for (long& x : k)
+        if (fscanf(f, "%ld", &x) != 1) rc = NRC_ERR_ARG;
+      if (rc == NRC_OK && (fscanf(f, "%d %d", &v[0], &v[1]) != 2 || v[0] < NRC_PATH_AUTO ||
+                           v[0] > NRC_PATH_SINGLE_PASS || v[1] < 1 ||
+                           v[1] > nrc::SLICE_FUSE_MAX))
+        rc = NRC_ERR_ARG;
+      if (rc == NRC_OK) ens[k] = v;
+    } else {
+      rc = NRC_ERR_ARG;
+    }
+  }
+  fclose(f);
+  if (rc != NRC_OK) return rc;  // a malformed file changes nothing
+  for (const auto& kv : ovl) ctx->ovl_choice[kv.first] = kv.second;
+  for (const auto& kv : ens) ctx->ens_choice[kv.first] = kv.second;
   return NRC_OK;
 }
 

@@ -234,6 +234,17 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s,
                              int parts = 2);
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
+// the sink-row form: rows_split holds `parts` planes [part][2 ns][2V] (launch_split_rows),
+// row 2s = (re, im) and row 2s + 1 = (-im, re) of sink s, staged in LDS and shared by the four
+// configurations of a workgroup; form 0: 2-step stages, 2 steps of orbital loads in flight,
+// two workgroups per CU; 1: 4-step stages, 4 in flight, one per CU; 2: 2-step stages, 4 in
+// flight, two per CU
+hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hipStream_t s,
+                             int parts);
+hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2* M, int ns,
+                              int nt, long vol, int ncfg, float* partial, hipStream_t s,
+                              int terms, int form);
+size_t overlap_rows_partial_floats(int ns, int nt, long vol, int ncfg);
 
 // pair-sink matrices M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]: FP64 sums
 // (dense), or split-BF16x3 on the matrix cores (FP32 orbitals, weights rounded to FP32).

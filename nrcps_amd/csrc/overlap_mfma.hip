@@ -636,6 +636,319 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ overlaps, sink-row form
+// The same products with the real / imaginary split carried by the sinks instead of the
+// orbitals: A has two rows per sink, row 2s = (phi_re, phi_im) and row 2s + 1 = (-phi_im,
+// phi_re) interleaved along K, so that with B the plain orbital values (psi_re, psi_im),
+// D[2s][t] = Re conj(phi_s) . psi_t and D[2s + 1][t] = Im.  The sinks are split once per call
+// (they are shared by every configuration); the orbital side needs no per-element swap, and
+// 2 ns rows pack into fewer tiles of 16 than ns rows with two column sets (33 sinks x 33
+// orbitals: 5 x 3 tiles instead of 3 x 3 x 2).  Each thread splits its eight orbital values
+// with packed conversions: h = bf16x2(x0, x1) once, the floats of h by shifts, the remainders
+// in FP32, the next part packed again -- bit for bit the parts of split8 / split8_3.
+
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// x0, x1 -> packed bf16 (RNE) and the FP32 remainders x - float(bf16(x))
+__device__ __forceinline__ uint32_t split_pk(float x0, float x1, float& r0, float& r1) {
+  const uint32_t h =
+      __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2v){x0, x1}, bf16x2v));
+  r0 = x0 - __builtin_bit_cast(float, h << 16);
+  r1 = x1 - __builtin_bit_cast(float, h & 0xFFFF0000u);
+  return h;
+}
+
+template <int NP>
+__device__ __forceinline__ void split8_pk(const float* v, bf16x8* p) {
+  using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+  u32x4 w[NP];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float r0 = v[2 * q], r1 = v[2 * q + 1];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const float a0 = r0, a1 = r1;
+      w[j][q] = split_pk(a0, a1, r0, r1);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) p[j] = __builtin_bit_cast(bf16x8, w[j]);
+}
+
+// sinks phi[s][V] -> `parts` planes [part][2 ns][2V]: row 2s = (re, im), row 2s + 1 = (-im, re)
+__global__ void k_split_rows(const float2* phi, __bf16* out, long n, long vol, int parts) {
+  const long K = 2 * vol, plane = 2 * n * 2;  // n = ns V elements; a plane holds 2 ns rows of K
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long sidx = i / vol, site = i - sidx * vol;
+    const float2 v = phi[i];
+    const float x[4] = {v.x, v.y, -v.y, v.x};  // row 2s: (re, im); row 2s + 1: (-im, re)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long o = (2 * sidx + (j >> 1)) * K + 2 * site + (j & 1);
+      const __bf16 h = to_bf16(x[j]);
+      const float r = x[j] - (float)h;
+      const __bf16 m = to_bf16(r);
+      out[o] = h;
+      out[plane + o] = m;
+      if (parts == 3) out[2 * plane + o] = to_bf16(r - (float)m);
+    }
+  }
+}
+
+// M[c][s][t] = sum over chunks (FP64) of rows 2s (re) and 2s + 1 (im), column t
+__global__ void k_overlap_rows_reduce(const float* partial, double2* M, int ns, int nt,
+                                      int nchunks, int mr, int ntt, long ncfg) {
+  const long n = ncfg * ns * nt;
+  const int ldp = ntt * 16;
+  const long tile = (long)(mr * 16) * ldp;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / ((long)ns * nt);
+    const int r = (int)(i - c * ns * nt), s = r / nt, t = r - (r / nt) * nt;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nchunks; ++k) {
+      const float* p = partial + (c * nchunks + k) * tile + (long)(2 * s) * ldp + t;
+      re += (double)p[0];
+      im += (double)p[ldp];
+    }
+    M[i] = make_double2(re, im);
+  }
+}
+
+// partial tiles of the sink-row form: [c][chunk][2 ns rows padded to 16][nt padded to 16]
+size_t overlap_rows_partial_floats(int ns, int nt, long vol, int ncfg) {
+  const int mr = (2 * ns + 15) / 16, ntt = (nt + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const long nchunks = (K + kc - 1) / kc;
+  return (size_t)ncfg * nchunks * (mr * 16) * (ntt * 16);
+}
+
+hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hipStream_t s,
+                             int parts) {
+  if (parts != 2 && parts != 3) return hipErrorInvalidValue;
+  const long n = (long)ns * vol;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)blocks), dim3(256), 0, s, phi, (__bf16*)out,
+                     n, vol, parts);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ overlaps, shared sinks
+// The sink-row form with the sinks staged in LDS and shared by the four waves of a workgroup,
+// each wave one configuration over the same K chunk: the split sinks cross from L2 once per
+// four configurations instead of once per wave and step, which leaves the orbital stream
+// from HBM as the only large flow.  A stage is KS steps of 32 K values of every sink row and
+// part; the next stage's sink loads go out before the current stage's products and are
+// written to the other LDS buffer after them (one barrier per stage).  LDS layout per part
+// and 8-value K group q: R rows of 16 bytes, plane stride R 16 + 16 bytes, so the 16 rows a
+// quarter-wave reads are one contiguous 256-byte run (no bank conflicts) and the loaders'
+// 16-byte writes from four consecutive rows of 16 K groups land in distinct banks.
+// Steps past the chunk end read zero from both operands and add nothing.  Grid (group of four
+// configurations, K chunk) with the group fastest, so that the workgroups resident at one time
+// read one chunk of the split sinks, which stays in L2.
+// Three forms (FORM): 0, stages of 2 steps with 2 steps of orbital loads in flight, small
+// enough for two workgroups per CU (two waves per SIMD hide each other's load latency);
+// 1, stages of 4 steps with 4 steps in flight, one workgroup per CU; 2, stages of 2 steps with
+// 4 steps in flight, two workgroups per CU.  A shape whose 4-step stages do not fit the LDS
+// budget uses 2-step stages.
+template <int MR, int TERMS, int FORM>
+constexpr int ovl_lds_ks() {
+  constexpr int npa = TERMS == 6 ? 3 : 2, r = MR * 16;
+  return FORM == 1 && 2 * npa * 16 * (r * 16 + 16) <= 160 * 1024 ? 4 : 2;
+}
+
+template <int MR, int NT, int TERMS, int FORM>
+__global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
+    const __bf16* __restrict__ a0p, const __bf16* __restrict__ a1p,
+    const __bf16* __restrict__ a2p, const float2* __restrict__ psi, float* __restrict__ partial,
+    int nrow, int nt, long vol, int ncfg, long kchunk, int nchunks) {
+  constexpr int NPA = TERMS == 6 ? 3 : 2, R = MR * 16, KS = ovl_lds_ks<MR, TERMS, FORM>();
+  constexpr int PD = FORM == 0 ? 2 : 4, Q = KS * 4;
+  constexpr int SU = PD > KS ? PD / KS : 1;  // stages per loop iteration (static buffers)
+  constexpr int PL = R * 16 + 16;           // bytes per (part, K group) plane
+  constexpr int STAGE = NPA * Q * PL;       // bytes per stage
+  constexpr int NCH = R * Q;                // 16-byte chunks per part and stage
+  constexpr int NLD = (NCH + 255) / 256;    // per thread and part
+  static_assert((KS * SU) % PD == 0, "the orbital ring indexes by step within an iteration");
+  __shared__ __attribute__((aligned(16))) unsigned char lds[2 * STAGE];
+  using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int row = lane & 15, grp = lane >> 4;
+  const long c = (long)blockIdx.x * 4 + wave;
+  const bool live = c < ncfg;
+  const int chunk = blockIdx.y;
+  const long K = 2 * vol;
+  const long k0 = (long)chunk * kchunk;
+  const long k1 = k0 + kchunk < K ? k0 + kchunk : K;
+  const int nst = (int)((k1 - k0 + 32 * KS - 1) / (32 * KS));
+  const uint32_t abytes = (uint32_t)(nrow * K * 2);
+  const rsrc_t ra[3] = {make_rsrc(a0p, abytes), make_rsrc(a1p, abytes),
+                        make_rsrc(a2p, TERMS == 6 ? abytes : 0u)};
+  const rsrc_t rb = make_rsrc(psi + (live ? c : 0) * nt * vol, (uint32_t)(nt * vol * 8));
+  constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
+  // the stage loader, part by part (the part selects the buffer resource, so it stays
+  // uniform): chunk id = ((row / 4) Q + q) 4 + row % 4
+  u32x4 apf[NPA][NLD];
+  auto loadA = [&](long ks) {
+#pragma unroll
+    for (int j = 0; j < NPA; ++j)
+#pragma unroll
+      for (int i = 0; i < NLD; ++i) {
+        const int id = tid + 256 * i;
+        if (NCH % 256 != 0 && id >= NCH) break;
+        const int rlo = id & 3, q = (id >> 2) % Q, r = (id >> 2) / Q * 4 + rlo;
+        const long k = ks + 8 * q;
+        const uint32_t o = (r < nrow && k < k1) ? (uint32_t)(r * K + k) * 2u : kFar;
+        apf[j][i] = bload<u32x4>(ra[j], o, 0u);
+      }
+  };
+  auto storeA = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < NPA; ++j)
+#pragma unroll
+      for (int i = 0; i < NLD; ++i) {
+        const int id = tid + 256 * i;
+        if (NCH % 256 != 0 && id >= NCH) break;
+        const int rlo = id & 3, q = (id >> 2) % Q, r = (id >> 2) / Q * 4 + rlo;
+        *reinterpret_cast<u32x4*>(lds + buf * STAGE + (j * Q + q) * PL + r * 16) = apf[j][i];
+      }
+  };
+  uint32_t ob[NT];
+#pragma unroll
+  for (int b = 0; b < NT; ++b) {
+    const int t = b * 16 + row;
+    ob[b] = (live && t < nt) ? (uint32_t)(t * vol + 4 * grp) * 8u : kFar;
+  }
+  struct BStep {
+    float4 b0[NT], b1[NT];
+  };
+  auto loadB = [&](BStep& st, long kb) {
+    const uint32_t sb = (uint32_t)kb * 4u;
+    const bool in = kb + 8 * grp < k1;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const uint32_t o = in ? ob[b] : kFar;
+      st.b0[b] = bload<float4>(rb, o, sb);
+      st.b1[b] = bload<float4>(rb, o + 16u, sb);
+    }
+  };
+  f32x4 acc[MR][NT];
+#pragma unroll
+  for (int a = 0; a < MR; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // one step: the sink fragments of K group q = 4 u + grp from LDS, the orbital values split
+  auto compute = [&](int buf, int u, const BStep& B) {
+    const unsigned char* base = lds + buf * STAGE + (4 * u + grp) * PL + row * 16;
+    bf16x8 av[NPA][MR];
+#pragma unroll
+    for (int j = 0; j < NPA; ++j)
+#pragma unroll
+      for (int a = 0; a < MR; ++a)
+        av[j][a] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u32x4*>(base + j * Q * PL + a * 256));
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const float v[8] = {B.b0[b].x, B.b0[b].y, B.b0[b].z, B.b0[b].w,
+                          B.b1[b].x, B.b1[b].y, B.b1[b].z, B.b1[b].w};
+      bf16x8 pr[NPA];
+      split8_pk<NPA>(v, pr);
+      constexpr int TA[6] = {0, 0, 1, 0, 1, 2}, TB[6] = {0, 1, 0, 2, 1, 0};
+#pragma unroll
+      for (int term = 0; term < TERMS; ++term)
+#pragma unroll
+        for (int a = 0; a < MR; ++a)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[TA[term]][a], pr[TB[term]],
+                                                              acc[a][b], 0, 0, 0);
+    }
+  };
+  BStep B[PD];
+  loadA(k0);
+#pragma unroll
+  for (int j = 0; j < PD; ++j) loadB(B[j], k0 + 32 * j);
+  storeA(0);
+  __syncthreads();
+  for (int st0 = 0; st0 < nst; st0 += SU) {
+#pragma unroll
+    for (int h = 0; h < SU; ++h) {
+      const int st = st0 + h;
+      if (st >= nst) break;
+      const int buf = SU == 2 ? h : (st & 1);
+      const long ks = k0 + (long)st * 32 * KS;
+      const bool more = st + 1 < nst;
+      if (more) loadA(ks + 32 * KS);
+#pragma unroll
+      for (int u = 0; u < KS; ++u) {
+        const int i = (h * KS + u) % PD;
+        compute(buf, u, B[i]);
+        loadB(B[i], ks + 32 * (u + PD));
+      }
+      if (more) storeA(buf ^ 1);
+      __syncthreads();
+    }
+  }
+  if (!live) return;
+  constexpr int ldp = NT * 16;
+  float* out = partial + (c * nchunks + chunk) * (long)R * ldp;
+#pragma unroll
+  for (int a = 0; a < MR; ++a)
+#pragma unroll
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(long)(a * 16 + grp * 4 + r) * ldp + b * 16 + row] = acc[a][b][r];
+}
+
+hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2* M, int ns,
+                              int nt, long vol, int ncfg, float* partial, hipStream_t s,
+                              int terms, int form) {
+  if ((terms != 3 && terms != 6) || form < 0 || form > 2) return hipErrorInvalidValue;
+  if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
+    return hipErrorInvalidValue;
+  const int nrow = 2 * ns, mr = (nrow + 15) / 16, ntt = (nt + 15) / 16;
+  const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
+  const int nchunks = (int)((K + kc - 1) / kc);
+  const __bf16* a0 = (const __bf16*)rows_split;
+  const long plane = (long)nrow * K;
+  const __bf16* a1 = a0 + plane;
+  const __bf16* a2 = a1 + plane;
+  const dim3 grid((unsigned)((ncfg + 3) / 4), (unsigned)nchunks), block(256);
+  bool launched = false;
+#define NRC_OL_T(A, B, T, F)                                                                  \
+  hipLaunchKernelGGL((k_overlap_lds<A, B, T, F>), grid, block, 0, s, a0, a1, T == 6 ? a2 : a1,  \
+                     psi, partial, nrow, nt, vol, ncfg, kc, nchunks)
+#define NRC_OL_CASE(A, B)                                                                     \
+  if (!launched && mr == A && ntt == B) {                                                     \
+    launched = true;                                                                          \
+    if (terms == 3 && form == 0) NRC_OL_T(A, B, 3, 0);                                        \
+    else if (terms == 3 && form == 1) NRC_OL_T(A, B, 3, 1);                                   \
+    else if (terms == 3) NRC_OL_T(A, B, 3, 2);                                                \
+    else if (form == 0) NRC_OL_T(A, B, 6, 0);                                                 \
+    else if (form == 1) NRC_OL_T(A, B, 6, 1);                                                 \
+    else NRC_OL_T(A, B, 6, 2);                                                                \
+  }
+#define NRC_OL_ROW(A) NRC_OL_CASE(A, 1) NRC_OL_CASE(A, 2) NRC_OL_CASE(A, 3) NRC_OL_CASE(A, 4)
+  NRC_OL_ROW(1) NRC_OL_ROW(2) NRC_OL_ROW(3) NRC_OL_ROW(4) NRC_OL_ROW(5) NRC_OL_ROW(6)
+  NRC_OL_ROW(7) NRC_OL_ROW(8)
+#undef NRC_OL_ROW
+#undef NRC_OL_CASE
+#undef NRC_OL_T
+  if (!launched) return hipErrorInvalidValue;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long n = (long)ncfg * ns * nt;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_overlap_rows_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M,
+                     ns, nt, nchunks, mr, ntt, (long)ncfg);
+  return hipGetLastError();
+}
+
 size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg) {
   const int mt = (n1 + 15) / 16, nt = (n2 + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
