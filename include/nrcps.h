@@ -30,6 +30,7 @@ extern "C" {
 #define NRC_ERR_SINGULAR (-5)    /* an all-zero matrix row (ludcmp.C:25-28 exits there) */
 #define NRC_ERR_BATCHED (-6)     /* the ensemble belongs to a batch (nrc_batch_create) */
 #define NRC_ERR_TOO_LARGE (-7)   /* sizes beyond a no-allocation call's fixed scratch */
+#define NRC_ERR_CHECK (-8)       /* a check of the NRC_CHECK build failed (bad index, NaN) */
 
 #define NRC_PREC_F32 0
 #define NRC_PREC_F64 1
@@ -51,6 +52,16 @@ typedef struct nrc_ensemble nrc_ensemble;
 int nrc_init(int device, nrc_context** ctx);
 int nrc_finalize(nrc_context* ctx);
 const char* nrc_error_string(int code);
+/* "file:line: what" of the last failure on the calling thread ("" if none yet).  No
+ * counterpart in the reference, whose errors print and exit (error.C:36-46). */
+const char* nrc_last_error_detail(void);
+/* 1 in the check build (libnrcps_hip_check.so, compiled with -DNRC_CHECK), else 0.  The check
+ * build, the same ABI, synchronises after every HIP call outside graph capture (a kernel
+ * failure is reported by the entry point that launched it), checks index arrays a caller
+ * passes (pair_opp sites) before a kernel reads them, and scans FFT outputs, overlap and pair
+ * matrices, LU results and ensemble results for NaN (and infinities other than a singular
+ * log|det| of -inf), returning NRC_ERR_CHECK with the detail.  For debugging; slower. */
+int nrc_check_build(void);
 int nrc_device_name(nrc_context* ctx, char* buf, int len, int* n_cu);
 int nrc_malloc(nrc_context* ctx, void** p_dev, size_t bytes);
 int nrc_free(nrc_context* ctx, void* p_dev);

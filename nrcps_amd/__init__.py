@@ -13,7 +13,10 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libnrcps_hip.so")
+# NRC_CHECK=1 in the environment selects the check build (nrc_check_build): the same ABI with
+# device synchronisation after every launch, index checks and NaN scans of results
+LIB_PATH = os.path.join(PKG, "libnrcps_hip_check.so" if os.environ.get("NRC_CHECK") == "1"
+                        else "libnrcps_hip.so")
 
 OK = 0
 PREC_F32, PREC_F64 = 0, 1
@@ -37,6 +40,7 @@ EXPORTS = [
     "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
     "nrc_overlap_set_variant", "nrc_overlap_variant", "nrc_overlap_tune",
     "nrc_ensemble_tune_report", "nrc_ensemble_launch_slices", "nrc_tune_save", "nrc_tune_load",
+    "nrc_last_error_detail", "nrc_check_build",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -74,6 +78,7 @@ def load_library() -> C.CDLL:
     vp, sz, i, u32, u64, dp = C.c_void_p, C.c_size_t, C.c_int, C.c_uint32, C.c_uint64, C.c_void_p
     sig = {
         "nrc_init": [i, C.POINTER(vp)], "nrc_finalize": [vp], "nrc_error_string": [i],
+        "nrc_last_error_detail": [], "nrc_check_build": [],
         "nrc_device_name": [vp, C.c_char_p, i, C.POINTER(i)],
         "nrc_malloc": [vp, C.POINTER(vp), sz], "nrc_free": [vp, vp],
         "nrc_memcpy_h2d": [vp, vp, vp, sz], "nrc_memcpy_d2h": [vp, vp, vp, sz],
@@ -131,15 +136,18 @@ def load_library() -> C.CDLL:
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
-        f.restype = C.c_char_p if name == "nrc_error_string" else C.c_int
+        f.restype = (C.c_char_p if name in ("nrc_error_string", "nrc_last_error_detail")
+                     else C.c_int)
     _lib = L
     return L
 
 
 def _check(rc: int, what: str) -> None:
     if rc != OK:
-        msg = load_library().nrc_error_string(rc).decode()
-        raise NrcError(f"{what} failed: {msg} ({rc})")
+        lib = load_library()
+        msg = lib.nrc_error_string(rc).decode()
+        detail = lib.nrc_last_error_detail().decode()
+        raise NrcError(f"{what} failed: {msg} ({rc})" + (f" [{detail}]" if detail else ""))
 
 
 def _ptr(a: np.ndarray):

@@ -12,6 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libnrcps_hip.so")
+LIB_CHECK = os.path.join(PKG, "libnrcps_hip_check.so")  # the -DNRC_CHECK build of the ABI
 SOURCES = ["fft_passes.hip", "fft_axis.hip", "slice_fused.hip", "slice_ext.hip", "lu_rng.hip", "overlap_mfma.hip", "proposal.hip",
            "api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -57,10 +58,19 @@ def build(jobs: int = 4) -> str:
                                             text=True)))
         if len(procs) >= jobs:
             _wait(procs)
+    # the check build: api.hip again with -DNRC_CHECK (the kernels are the same objects)
+    api_check = os.path.join(CSRC, "api_check.o")
+    cmd = [HIPCC, *FLAGS, "-DNRC_CHECK", "-c", os.path.join(CSRC, "api.hip"), "-o", api_check]
+    procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                        text=True)))
     _wait(procs)
     tmp = LIB + ".tmp"
     _run([HIPCC, *FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    tmp = LIB_CHECK + ".tmp"
+    _run([HIPCC, *FLAGS, "-shared", "-o", tmp,
+          *[o for o in objs if not o.endswith("api.o")], api_check])
+    os.replace(tmp, LIB_CHECK)
     build_host()
     return LIB
 

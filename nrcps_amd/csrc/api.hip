@@ -60,10 +60,39 @@ struct nrc_fft {
   bool fused;
 };
 
-#define NRC_HIP(expr)                                  \
-  do {                                                 \
-    hipError_t e_ = (expr);                            \
-    if (e_ != hipSuccess) return NRC_ERR_HIP;          \
+// ------------------------------------------------------------------ failures and check mode
+// Every failure records "file:line: what" for its thread (nrc_last_error_detail).  A build
+// with -DNRC_CHECK (libnrcps_hip_check.so) also synchronises the device after every HIP call
+// outside graph capture, so that an asynchronous kernel failure is reported by the entry
+// point that launched it; checks the contents of index arrays a caller hands to a kernel
+// before the kernel runs; and scans results for NaN (and infinities where none may occur),
+// returning NRC_ERR_CHECK.  The default build does none of the extra work.
+static thread_local char g_detail[256] = "";
+static thread_local int g_capturing = 0;  // a graph capture is open on this thread
+
+static void set_detail(const char* file, int line, const char* what) {
+  const char* base = strrchr(file, '/');
+  snprintf(g_detail, sizeof(g_detail), "%s:%d: %s", base ? base + 1 : file, line, what);
+}
+
+#ifdef NRC_CHECK
+#define NRC_SYNC_CHECK(e_)                                       \
+  if (e_ == hipSuccess && !g_capturing) {                        \
+    e_ = hipDeviceSynchronize();                                 \
+    if (e_ == hipSuccess) e_ = hipGetLastError();                \
+  }
+#else
+#define NRC_SYNC_CHECK(e_)
+#endif
+
+#define NRC_HIP(expr)                                            \
+  do {                                                           \
+    hipError_t e_ = (expr);                                      \
+    NRC_SYNC_CHECK(e_)                                           \
+    if (e_ != hipSuccess) {                                      \
+      set_detail(__FILE__, __LINE__, hipGetErrorString(e_));     \
+      return NRC_ERR_HIP;                                        \
+    }                                                            \
   } while (0)
 
 #define NRC_TRY(expr)                                  \
@@ -71,6 +100,77 @@ struct nrc_fft {
     int r_ = (expr);                                   \
     if (r_ != NRC_OK) return r_;                       \
   } while (0)
+
+#ifdef NRC_CHECK
+// count of values that are NaN, or infinite unless allow_neg_inf and negative
+template <typename T>
+__global__ void k_check_finite(const T* p, long n, int allow_neg_inf, int* bad) {
+  int local = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double x = (double)p[i];
+    if (isnan(x) || (isinf(x) && !(allow_neg_inf && x < 0.0))) ++local;
+  }
+  if (local) atomicAdd(bad, local);
+}
+
+// count of indices outside [lo, hi)
+__global__ void k_check_range(const int* p, long n, int lo, int hi, int* bad) {
+  int local = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    if (p[i] < lo || p[i] >= hi) ++local;
+  if (local) atomicAdd(bad, local);
+}
+
+static int check_count(hipStream_t s, int* bad_dev, const char* what, const char* file,
+                       int line) {
+  int bad = 0;
+  NRC_HIP(hipMemcpyAsync(&bad, bad_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  NRC_HIP(hipStreamSynchronize(s));
+  hipFree(bad_dev);
+  if (!bad) return NRC_OK;
+  char msg[160];
+  snprintf(msg, sizeof(msg), "%s: %d bad values", what, bad);
+  set_detail(file, line, msg);
+  return NRC_ERR_CHECK;
+}
+
+template <typename T>
+static int check_finite(hipStream_t s, const T* p, long n, bool allow_neg_inf, const char* what,
+                        const char* file, int line) {
+  if (g_capturing || n <= 0) return NRC_OK;
+  int* bad = nullptr;
+  NRC_HIP(hipMalloc(&bad, sizeof(int)));
+  NRC_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const long blocks = std::min<long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_check_finite<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n,
+                     (int)allow_neg_inf, bad);
+  return check_count(s, bad, what, file, line);
+}
+
+static int check_range(hipStream_t s, const int* p, long n, int lo, int hi, const char* what,
+                       const char* file, int line) {
+  if (g_capturing || n <= 0) return NRC_OK;
+  int* bad = nullptr;
+  NRC_HIP(hipMalloc(&bad, sizeof(int)));
+  NRC_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const long blocks = std::min<long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_check_range, dim3((unsigned)blocks), dim3(256), 0, s, p, n, lo, hi, bad);
+  return check_count(s, bad, what, file, line);
+}
+#define NRC_CHECK_FINITE(s, p, n, allow_neg_inf, what) \
+  NRC_TRY(check_finite(s, p, n, allow_neg_inf, what, __FILE__, __LINE__))
+#define NRC_CHECK_RANGE(s, p, n, lo, hi, what) \
+  NRC_TRY(check_range(s, p, n, lo, hi, what, __FILE__, __LINE__))
+#else
+#define NRC_CHECK_FINITE(s, p, n, allow_neg_inf, what) \
+  do {                                                \
+  } while (0)
+#define NRC_CHECK_RANGE(s, p, n, lo, hi, what) \
+  do {                                        \
+  } while (0)
+#endif
 
 static size_t elem_bytes(int prec) { return prec == NRC_PREC_F64 ? 16 : 8; }
 static bool prec_ok(int prec) { return prec == NRC_PREC_F32 || prec == NRC_PREC_F64; }
@@ -87,8 +187,19 @@ const char* nrc_error_string(int code) {
     case NRC_ERR_SINGULAR: return "singular matrix (all-zero row)";
     case NRC_ERR_BATCHED: return "ensemble belongs to a batch";
     case NRC_ERR_TOO_LARGE: return "sizes exceed the fixed scratch of the no-allocation call";
+    case NRC_ERR_CHECK: return "a check of the NRC_CHECK build failed";
     default: return "unknown error";
   }
+}
+
+const char* nrc_last_error_detail(void) { return g_detail; }
+
+int nrc_check_build(void) {
+#ifdef NRC_CHECK
+  return 1;
+#else
+  return 0;
+#endif
 }
 
 int nrc_init(int device, nrc_context** ctx) {
@@ -197,9 +308,14 @@ int nrc_fft_plan(nrc_context* ctx, int lx, int ly, int lz, int batch, int prec, 
 
 int nrc_fft_exec(nrc_fft* p, void* buf, int sign) {
   if (!p || !buf || (sign != -1 && sign != 1)) return NRC_ERR_ARG;
+  const long nreal = 2L * p->batch * p->lx * p->ly * p->lz;
   if (!p->fused) {
     NRC_HIP(nrc::launch_fft3d_axes(p->prec, buf, p->lx, p->ly, p->lz, p->batch, sign,
                                    p->ctx->stream));
+    if (p->prec == NRC_PREC_F64)
+      NRC_CHECK_FINITE(p->ctx->stream, (const double*)buf, nreal, false, "fft output");
+    else
+      NRC_CHECK_FINITE(p->ctx->stream, (const float*)buf, nreal, false, "fft output");
     return NRC_OK;
   }
   // X pass: fft_a is the sign +1 x-transform, fft_b the sign -1 one; YZ pass: fft_a is the
@@ -211,6 +327,10 @@ int nrc_fft_exec(nrc_fft* p, void* buf, int sign) {
   if (sign < 0) { x.fft_b = 1; yz.fft_a = 1; } else { x.fft_a = 1; yz.fft_b = 1; }
   NRC_HIP(nrc::launch_xpass(p->prec, x, p->ctx->stream));
   NRC_HIP(nrc::launch_yzpass(p->prec, yz, p->ctx->stream));
+  if (p->prec == NRC_PREC_F64)
+    NRC_CHECK_FINITE(p->ctx->stream, (const double*)buf, nreal, false, "fft output");
+  else
+    NRC_CHECK_FINITE(p->ctx->stream, (const float*)buf, nreal, false, "fft output");
   return NRC_OK;
 }
 
@@ -369,10 +489,12 @@ extern "C" {
 int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns, int nt,
                 long vol, int ncfg, int mode) {
   if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
+  const long nM = 2L * ncfg * ns * nt;
   if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {
     const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
     NRC_HIP(nrc::launch_overlap_dense(prec, phi, psi, (double2*)M, ns, nt, vol, ncfg,
                                       ctx->stream));
+    NRC_CHECK_FINITE(ctx->stream, M, nM, false, "overlap matrix");
     return NRC_OK;
   }
   if (mode != NRC_OVERLAP_BF16X3 && mode != NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
@@ -382,7 +504,9 @@ int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, i
   const int v = overlap_variant(ctx, ns, nt, vol, terms);
   void* w = ctx_workspace(ctx, overlap_bf16x3_workspace(ns, nt, vol, ncfg, v));
   if (!w) return NRC_ERR_NOMEM;
-  return overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream, terms, v);
+  NRC_TRY(overlap_bf16x3(phi, psi, M, ns, nt, vol, ncfg, w, ctx->stream, terms, v));
+  NRC_CHECK_FINITE(ctx->stream, M, nM, false, "overlap matrix");
+  return NRC_OK;
 }
 
 int nrc_overlap_set_variant(nrc_context* ctx, int ns, int nt, long vol, int mode, int variant) {
@@ -447,10 +571,14 @@ int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const
   if (!ctx || !psi1 || !psi2 || !w || !opp || !M || n1 < 1 || n2 < 1 || vol < 1 || ncfg < 0)
     return NRC_ERR_ARG;
   if (ncfg == 0) return NRC_OK;
+  if (vol > INT32_MAX) return NRC_ERR_UNSUPPORTED;
+  NRC_CHECK_RANGE(ctx->stream, opp, vol, 0, (int)vol, "pair_opp sites");
+  const long nM = 2L * ncfg * n1 * n2;
   if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {
     const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
     NRC_HIP(nrc::launch_pair_dense(prec, psi1, psi2, w, opp, (double2*)M, n1, n2, vol, ncfg,
                                    ctx->stream));
+    NRC_CHECK_FINITE(ctx->stream, M, nM, false, "pair matrix");
     return NRC_OK;
   }
   if (mode != NRC_OVERLAP_BF16X3) return NRC_ERR_ARG;
@@ -461,6 +589,7 @@ int nrc_pair_overlap(nrc_context* ctx, const void* psi1, const void* psi2, const
   if (!work) return NRC_ERR_NOMEM;
   NRC_HIP(nrc::launch_pair_bf16x3((const float2*)psi1, (const float2*)psi2, w, opp, (double2*)M,
                                   n1, n2, vol, ncfg, (float*)work, ctx->stream));
+  NRC_CHECK_FINITE(ctx->stream, M, nM, false, "pair matrix");
   return NRC_OK;
 }
 
@@ -472,6 +601,9 @@ int nrc_lu_det(nrc_context* ctx, const double* M, int n, int ldm, int ncfg, int 
   if (ncfg == 0) return NRC_OK;
   NRC_HIP(nrc::launch_lu_det((const double2*)M, n, ldm, ncfg, minors, logabs, (double2*)phase,
                              argsum, status, nullptr, nullptr, nullptr, ctx->stream));
+  const long nout = (long)ncfg * (minors ? n : 1);
+  NRC_CHECK_FINITE(ctx->stream, logabs, nout, true, "log|det|");
+  NRC_CHECK_FINITE(ctx->stream, phase, 2 * nout, false, "phase");
   return NRC_OK;
 }
 
@@ -1536,6 +1668,8 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
 static int ens_collect(nrc_ensemble* e, int T, double* logabs, double* phase, double* argsum) {
   hipStream_t s = e->stream;
   const size_t tot = (size_t)e->d.n_configs * e->nm * T;
+  NRC_CHECK_FINITE(s, e->o_la, (long)tot, true, "ensemble log|det|");
+  NRC_CHECK_FINITE(s, (const double*)e->o_ph, 2 * (long)tot, false, "ensemble phase");
   if (logabs) NRC_HIP(hipMemcpyAsync(logabs, e->o_la, tot * sizeof(double), hipMemcpyDeviceToHost, s));
   if (phase) NRC_HIP(hipMemcpyAsync(phase, e->o_ph, tot * sizeof(double2), hipMemcpyDeviceToHost, s));
   if (argsum) NRC_HIP(hipMemcpyAsync(argsum, e->o_as, tot * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2124,7 +2258,12 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     hipEventRecord(t0, main);
     hipEventSynchronize(t0);
     const auto wall0 = std::chrono::steady_clock::now();
-    NRC_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+    g_capturing = 1;  // before the call: the check build must not synchronise during capture
+    if (hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      g_capturing = 0;
+      set_detail(__FILE__, __LINE__, "hipStreamBeginCapture failed");
+      return NRC_ERR_HIP;
+    }
     int rc = NRC_OK;
     if (hipEventRecord(b->fork, main) != hipSuccess) rc = NRC_ERR_HIP;
     for (int i = 0; rc == NRC_OK && i < n; ++i) {
@@ -2137,6 +2276,7 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     }
     hipGraph_t gr = nullptr;
     const hipError_t ce = hipStreamEndCapture(main, &gr);
+    g_capturing = 0;
     if (rc == NRC_OK && ce != hipSuccess) rc = NRC_ERR_HIP;
     if (rc == NRC_OK && hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0) != hipSuccess)
       rc = NRC_ERR_HIP;
