@@ -1012,7 +1012,8 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
       if (V % 4) ENS_TRY(NRC_ERR_UNSUPPORTED);
       // the N x N products, or the two products of conjugate partners in one pass
       ENS_TRY(dev_alloc(&e->pair_work, std::max(nrc::pair_bf16x3_partial_floats(N, N, V, (int)C),
-                                                nrc::pair_gh_bf16x3_partial_floats(N, V, (int)C))));
+                                                nrc::pair_gh_bf16x3_partial_floats(
+                                                    N, V, (int)C, d->L))));
     }
   }
   if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
@@ -1297,8 +1298,11 @@ static int ens_measure(nrc_ensemble* e, int t) {
         NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, G, nv, nv, V, (int)C, s));
         NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, nullptr, H, nv, nv, V, (int)C, s));
       } else if (nv <= 48) {  // both products in one pass
+        // the mirror form reads each site and its mirror together (pair_opp(p) = -p and an
+        // even w are conditions of partners; L a multiple of 4 and at least 8)
+        const int Lm = (e->d.L % 4 == 0 && e->d.L >= 8) ? e->d.L : 0;
         NRC_HIP(nrc::launch_pair_gh_bf16x3((const float2*)e->psi, w, e->popp, G, H, nv, V,
-                                           (int)C, e->pair_work, s));
+                                           (int)C, e->pair_work, s, Lm));
       } else {
         NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
                                         e->popp, G, nv, nv, V, (int)C, e->pair_work, s));

@@ -19,6 +19,8 @@
 // Each workgroup (4 waves) takes one configuration and one chunk of K; every wave
 // accumulates all (sink tile, column tile) pairs for its K-steps, the waves are summed in
 // LDS and the chunk's partial tile written out; a second kernel sums the chunks in FP64.
+#include <algorithm>
+
 #include "nrc_ops.h"
 
 namespace nrc {
@@ -564,6 +566,155 @@ __global__ __launch_bounds__(256) void k_pair_gh_bf16x3(const float2* __restrict
         }
 }
 
+// k_pair_gh_bf16x3 for opp(p) = -p on a cubic lattice (the conjugate partners' case), with
+// every orbital value read from HBM once: a site p and its mirror -p are loaded together and
+// serve both sites' terms.  The workgroups take units of the lattice: the plane pair (x, L -
+// x) for x = 1 .. L/2 - 1 -- the sites p of plane x and their mirrors in plane L - x, each
+// loaded pair giving the terms of p (A = psi(p), G's B = w psi(-p), H's B = w conj psi(p)) and
+// of -p (A = psi(-p), G's B = w psi(p), H's B = w conj psi(-p)) -- and the planes 0 and L/2,
+// mirrored onto themselves, whose sites each give their own terms (the mirror read hits the
+// cache: the same workgroup reads the plane).  w is even (w(-p) = w(p)), as conjugate
+// partners require.  A lane's four consecutive sites (y, z .. z + 3), z a multiple of 4, have
+// the mirrors (L - y, L - z .. L - z - 3): one reversed run, or for z = 0 the site (L - y, 0)
+// and a run of three.  Partial tiles per unit in k_pair_gh_bf16x3's layout.
+template <int MT>
+__global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict__ psi,
+                                                        const double* __restrict__ wf,
+                                                        float* __restrict__ partial, int n,
+                                                        int L) {
+  using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
+  __shared__ float red[OV_WAVES - 1][4][64];
+  const int unit = blockIdx.x, nunits = gridDim.x;
+  const long c = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, grp = lane >> 4;
+  const int P = L * L;
+  const long V = (long)L * P;
+  const int x = unit < L / 2 - 1 ? unit + 1 : (unit == L / 2 - 1 ? 0 : L / 2);
+  const int xm = (L - x) % L;
+  const bool pair = x != xm;
+  f32x4 acc[4][MT][MT];  // G re, G im, H re, H im
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int b = 0; b < MT; ++b) acc[m][a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const float2* base = psi + c * n * V;
+  for (int i0 = wave * 16; i0 < P; i0 += OV_WAVES * 16) {
+    const int i = i0 + 4 * grp;  // this lane's four sites of the plane
+    const int y = i / L, z = i - y * L;
+    const long p0 = (long)x * P + i;
+    const long mrow = (long)xm * P + (long)((L - y) % L) * L;
+    float va[MT][8], vm[MT][8];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+      const int s = a * 16 + row;
+      if (s < n) {
+        const float2* o = base + (long)s * V;
+        const float4* q = reinterpret_cast<const float4*>(o + p0);
+        const float4 u0 = q[0], u1 = q[1];
+        va[a][0] = u0.x; va[a][1] = u0.y; va[a][2] = u0.z; va[a][3] = u0.w;
+        va[a][4] = u1.x; va[a][5] = u1.y; va[a][6] = u1.z; va[a][7] = u1.w;
+        float2 m[4];
+        if (z > 0) {  // mirrors of z .. z + 3: m0 + 3 .. m0, m0 = L - z - 3 (odd)
+          const long m0 = mrow + (L - z - 3);
+          const float4 u21 = *reinterpret_cast<const float4*>(o + m0 + 1);
+          m[3] = o[m0];
+          m[2] = make_float2(u21.x, u21.y);
+          m[1] = make_float2(u21.z, u21.w);
+          m[0] = o[m0 + 3];
+        } else {
+          m[0] = o[mrow];
+          m[1] = o[mrow + L - 1];
+          m[2] = o[mrow + L - 2];
+          m[3] = o[mrow + L - 3];
+        }
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+          vm[a][2 * q2] = m[q2].x;
+          vm[a][2 * q2 + 1] = m[q2].y;
+        }
+      } else {
+#pragma unroll
+        for (int q2 = 0; q2 < 8; ++q2) va[a][q2] = vm[a][q2] = 0.f;
+      }
+    }
+    const double2* wp = reinterpret_cast<const double2*>(wf + p0);
+    const double2 w01 = wp[0], w23 = wp[1];
+    const float wt[4] = {(float)w01.x, (float)w01.y, (float)w23.x, (float)w23.y};
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      if (part == 1 && !pair) break;
+      bf16x8 ah[MT], al[MT];
+#pragma unroll
+      for (int a = 0; a < MT; ++a) split8(part ? vm[a] : va[a], ah[a], al[a]);
+#pragma unroll
+      for (int b = 0; b < MT; ++b)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {  // G: the mirrored values; H: the same values conjugated
+          const float* src = (m == 0) == (part == 0) ? vm[b] : va[b];
+          const float sg = m == 0 ? 1.f : -1.f;
+          float vr[8];
+#pragma unroll
+          for (int q2 = 0; q2 < 4; ++q2) {
+            vr[2 * q2] = wt[q2] * src[2 * q2];
+            vr[2 * q2 + 1] = -(wt[q2] * (sg * src[2 * q2 + 1]));
+          }
+          bf16x8 rh, rl;
+          split8(vr, rh, rl);
+          const u32x4 uh = __builtin_bit_cast(u32x4, rh), ul = __builtin_bit_cast(u32x4, rl);
+          u32x4 vh, vl;
+#pragma unroll
+          for (int q2 = 0; q2 < 4; ++q2) {
+            vh[q2] = ((uh[q2] >> 16) | (uh[q2] << 16)) ^ 0x00008000u;
+            vl[q2] = ((ul[q2] >> 16) | (ul[q2] << 16)) ^ 0x00008000u;
+          }
+          const bf16x8 ih = __builtin_bit_cast(bf16x8, vh), il = __builtin_bit_cast(bf16x8, vl);
+#pragma unroll
+          for (int term = 0; term < 3; ++term)
+#pragma unroll
+            for (int a = 0; a < MT; ++a) {
+              const bf16x8 av = term == 2 ? al[a] : ah[a];
+              const bf16x8 br = term == 1 ? rl : rh, bi = term == 1 ? il : ih;
+              acc[2 * m][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, br, acc[2 * m][a][b], 0, 0, 0);
+              acc[2 * m + 1][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bi, acc[2 * m + 1][a][b], 0, 0, 0);
+            }
+        }
+    }
+  }
+  constexpr int ldp = MT * 32;
+  const long half = (long)gridDim.y * nunits * (MT * 16) * ldp;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+      for (int b = 0; b < MT; ++b)
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+          const f32x4 v = acc[2 * m + part][a][b];
+          if (wave > 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave - 1][r][lane] = v[r];
+          }
+          __syncthreads();
+          if (wave == 0) {
+            float* out = partial + m * half + (c * nunits + unit) * (long)(MT * 16) * ldp;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              float xs = v[r];
+#pragma unroll
+              for (int w = 0; w < OV_WAVES - 1; ++w) xs += red[w][r][lane];
+              out[(long)(a * 16 + grp * 4 + r) * ldp + 2 * (b * 16 + row) + part] = xs;
+            }
+          }
+          __syncthreads();
+        }
+}
+
 // M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
 __global__ void k_overlap_reduce(const float* partial, double2* M, int ns, int nt, int nchunks,
                                  int mt, int nc_t, long ncfg) {
@@ -990,19 +1141,33 @@ hipError_t launch_pair_bf16x3(const float2* psi1, const float2* psi2, const doub
   return hipGetLastError();
 }
 
-size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg) {
-  return 2 * pair_bf16x3_partial_floats(n, n, vol, ncfg);
+// units of the mirror form: L/2 - 1 plane pairs and the planes 0 and L/2
+static int pair_gh_units(int L) { return L / 2 + 1; }
+
+size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg, int L) {
+  const size_t chunks = 2 * pair_bf16x3_partial_floats(n, n, vol, ncfg);
+  if (L <= 0) return chunks;
+  const int mt = (n + 15) / 16;
+  const size_t units = 2 * (size_t)ncfg * pair_gh_units(L) * (mt * 16) * (mt * 32);
+  return std::max(chunks, units);
 }
 
 hipError_t launch_pair_gh_bf16x3(const float2* psi, const double* wf, const int* opp,
                                  double2* G, double2* H, int n, long vol, int ncfg,
-                                 float* partial, hipStream_t s) {
+                                 float* partial, hipStream_t s, int L) {
   if (n < 1 || n > 48 || (vol & 3)) return hipErrorInvalidValue;
+  if (L > 0 && ((long)L * L * L != vol || L % 4 || L < 8)) return hipErrorInvalidValue;
   const int mt = (n + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
-  const int nchunks = (int)((K + kc - 1) / kc);
+  const int nchunks = L > 0 ? pair_gh_units(L) : (int)((K + kc - 1) / kc);
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
-  if (mt == 1)
+  if (L > 0 && mt == 1)
+    hipLaunchKernelGGL(k_pair_gh_mirror<1>, grid, block, 0, s, psi, wf, partial, n, L);
+  else if (L > 0 && mt == 2)
+    hipLaunchKernelGGL(k_pair_gh_mirror<2>, grid, block, 0, s, psi, wf, partial, n, L);
+  else if (L > 0)
+    hipLaunchKernelGGL(k_pair_gh_mirror<3>, grid, block, 0, s, psi, wf, partial, n, L);
+  else if (mt == 1)
     hipLaunchKernelGGL(k_pair_gh_bf16x3<1>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
   else if (mt == 2)
     hipLaunchKernelGGL(k_pair_gh_bf16x3<2>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
