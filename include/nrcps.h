@@ -293,13 +293,15 @@ int nrc_ensemble_results(nrc_ensemble* ens, double* logabs, double* phase, doubl
  * FFTW_EXHAUSTIVE): times over T_probe slices every evolution path the ensemble's shape
  * supports (two-pass FFT passes, single-pass slice kernel); with the path chosen, the slices
  * per launch of the persistent kernels (4, 2, 1; where the path fuses slices, over at least 4
- * slices); with dense FP32 sinks, the overlap GEMM variants (NRC_OVL_*) on the ensemble's own
- * buffers.  Keeps the fastest of each for later runs and stores the choice in the context
+ * slices); for the FP64 x-chain kernel, its tiles (x-pencils per thread, 1 or 2, where the
+ * shape offers both); with dense FP32 sinks, the overlap GEMM variants (NRC_OVL_*) on the
+ * ensemble's own buffers.  Keeps the fastest of each for later runs and stores the choice in the context
  * under the ensemble's shape key (L, N, precision, split, volumes per configuration,
  * accuracy, measurement kind, configurations), where later ensembles of the same key find it
  * unless nrc_ensemble_set_path chose a path.  Returns the chosen path and its ms per slice.
  * tune_report: the i-th timing of the last tune (kind 0 path, 1 slices per launch, 2 GEMM
- * variant; value; ms per slice, for kind 2 ms per measurement), NRC_ERR_ARG past the end.
+ * variant, 3 tile; value; ms per slice, for kind 2 ms per measurement), NRC_ERR_ARG past the
+ * end.
  * launch_slices: the slices per launch the ensemble's next run uses. */
 int nrc_ensemble_tune(nrc_ensemble* ens, int T_probe, int* path, double* ms_per_slice);
 int nrc_ensemble_tune_report(nrc_ensemble* ens, int i, int* kind, int* value, double* ms);

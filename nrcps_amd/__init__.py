@@ -455,10 +455,11 @@ class Ensemble:
         return p.value, ms.value
 
     def tune_report(self):
-        """The last tune's timings: [(kind, value, ms)], kind 'path', 'slices' or 'gemm'."""
+        """The last tune's timings: [(kind, value, ms)], kind 'path', 'slices', 'gemm' or
+        'tile'."""
         out, i = [], 0
         k, v, ms = C.c_int(), C.c_int(), C.c_double()
-        names = {0: "path", 1: "slices", 2: "gemm"}
+        names = {0: "path", 1: "slices", 2: "gemm", 3: "tile"}
         while self.lib.nrc_ensemble_tune_report(self.h, i, C.byref(k), C.byref(v),
                                                 C.byref(ms)) == OK:
             out.append((names[k.value], v.value, ms.value))

@@ -22,7 +22,7 @@ struct nrc_context {
   // tuned choices (nrc_overlap_tune, nrc_ensemble_tune; nrc_tune_save / nrc_tune_load):
   // GEMM variant per (ns, nt, vol, terms); (path, slices per launch) per ensemble shape key
   std::map<std::array<long, 4>, int> ovl_choice;
-  std::map<std::array<long, 8>, std::array<int, 2>> ens_choice;
+  std::map<std::array<long, 8>, std::array<int, 3>> ens_choice;  // path, slices, tile
 };
 
 // The scratch of the no-allocation overlap (nrc_overlap_noalloc): a fixed device array that
@@ -702,6 +702,7 @@ struct nrc_ensemble {
   int path = NRC_PATH_AUTO;
   bool user_path = false;        // nrc_ensemble_set_path chose the path (no tuned lookup)
   int fuse = nrc::SLICE_FUSE_MAX;  // slices per launch of the persistent slice kernels
+  int tile = 0;                  // x-pencils per thread of the FP64 x-chain kernel (0: default)
   std::vector<std::array<double, 3>> tune_log;  // the last tune: (kind, value, ms per slice)
   int acc = NRC_ACC_X64;         // FP32 ensembles: nrc_ensemble_set_accuracy
   uint64_t* par_dev = nullptr;   // the run's (seed, config0), read by the field kernels
@@ -1043,6 +1044,7 @@ extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
   e->path = path;
   e->user_path = path != NRC_PATH_AUTO;
   e->fuse = nrc::SLICE_FUSE_MAX;
+  e->tile = 0;
   return NRC_OK;
 }
 
@@ -1448,6 +1450,7 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.psi = e->psi;
   a.nvol = C * e->nvo;
   a.norb = e->nvo;
+  a.tile = e->tile;
   a.L = e->d.L;
   a.grid = e->ctx->n_cu;
   a.nslices = S;
@@ -1695,6 +1698,7 @@ static void ens_apply_tuned(nrc_ensemble* e) {
   if (it == e->ctx->ens_choice.end()) return;
   e->path = it->second[0];
   e->fuse = it->second[1];
+  e->tile = it->second[2];
 }
 
 static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint32_t* words_host,
@@ -2057,6 +2061,31 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
     if (rc == NRC_OK) best_ms = base * T_probe / Tf;
   }
   e->fuse = best_fuse;
+  // x-pencils per thread of the FP64 x-chain kernel, where the shape offers more than one
+  int best_tile = 0;
+  e->tile = 0;
+  int tiles[4];
+  const int ntiles = nrc::slicex_tiles(e->d.L, tiles);
+  if (rc == NRC_OK && ntiles > 1 && ens_xchain(e, best == NRC_PATH_SINGLE_PASS ||
+                                                     (best == NRC_PATH_AUTO &&
+                                                      nrc::slice1_supported(e->d.prec,
+                                                                            e->d.L)))) {
+    const int Tt = std::max(T_probe, e->fuse);
+    float tb = -1.f;
+    for (int i = 0; i < ntiles && rc == NRC_OK; ++i) {
+      e->tile = tiles[i];
+      float ms = 0.f;
+      rc = ens_time_slices(e, Tt, a, b, &ms);
+      if (rc != NRC_OK) break;
+      e->tune_log.push_back({3.0, (double)tiles[i], ms / Tt});
+      if (tb < 0.f || ms < tb) {
+        tb = ms;
+        best_tile = tiles[i];
+      }
+    }
+    if (rc == NRC_OK) best_ms = tb * T_probe / Tt;
+  }
+  e->tile = best_tile;
   // dense FP32 sinks: the overlap GEMM variants on the ensemble's buffers
   if (rc == NRC_OK && e->sinks && e->d.prec == NRC_PREC_F32) {
     const int terms = e->acc == NRC_ACC_X64 ? 6 : 3;
@@ -2078,7 +2107,7 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
   e->user_path = user_path;
   e->last_T = 0;
   if (rc != NRC_OK) return rc;
-  e->ctx->ens_choice[ens_tune_key(e)] = {best, best_fuse};
+  e->ctx->ens_choice[ens_tune_key(e)] = {best, best_fuse, best_tile};
   if (path_out) *path_out = best;
   if (ms_per_slice) *ms_per_slice = best_ms / T_probe;
   return NRC_OK;
@@ -2105,18 +2134,20 @@ extern "C" int nrc_ensemble_launch_slices(nrc_ensemble* e, int* slices) {
 // A text file: a header line, then one line per choice,
 //   overlap <ns> <nt> <vol> <terms> <variant>
 //   ensemble <L> <N> <prec> <split> <volumes> <accuracy> <flags> <configs> <path> <slices>
+//            <tile>
+// (version 1 files, without the tile, load with the default tile)
 extern "C" int nrc_tune_save(nrc_context* ctx, const char* path) {
   if (!ctx || !path) return NRC_ERR_ARG;
   FILE* f = fopen(path, "w");
   if (!f) return NRC_ERR_ARG;
-  fprintf(f, "nrcps-tune 1\n");
+  fprintf(f, "nrcps-tune 2\n");
   for (const auto& kv : ctx->ovl_choice)
     fprintf(f, "overlap %ld %ld %ld %ld %d\n", kv.first[0], kv.first[1], kv.first[2],
             kv.first[3], kv.second);
   for (const auto& kv : ctx->ens_choice) {
     fprintf(f, "ensemble");
     for (long v : kv.first) fprintf(f, " %ld", v);
-    fprintf(f, " %d %d\n", kv.second[0], kv.second[1]);
+    fprintf(f, " %d %d %d\n", kv.second[0], kv.second[1], kv.second[2]);
   }
   return fclose(f) == 0 ? NRC_OK : NRC_ERR_ARG;
 }
@@ -2128,11 +2159,11 @@ extern "C" int nrc_tune_load(nrc_context* ctx, const char* path) {
   char tag[32];
   int version = 0;
   int rc = (fscanf(f, "%31s %d", tag, &version) == 2 && !strcmp(tag, "nrcps-tune") &&
-            version == 1)
+            (version == 1 || version == 2))
                ? NRC_OK
                : NRC_ERR_ARG;
   std::map<std::array<long, 4>, int> ovl;
-  std::map<std::array<long, 8>, std::array<int, 2>> ens;
+  std::map<std::array<long, 8>, std::array<int, 3>> ens;
   while (rc == NRC_OK && fscanf(f, "%31s", tag) == 1) {
     if (!strcmp(tag, "overlap")) {
       std::array<long, 4> k;
@@ -2144,12 +2175,14 @@ extern "C" int nrc_tune_load(nrc_context* ctx, const char* path) {
         ovl[k] = v;
     } else if (!strcmp(tag, "ensemble")) {
       std::array<long, 8> k;
-      std::array<int, 2> v;
+      std::array<int, 3> v = {0, 0, 0};
       for (long& x : k)
         if (fscanf(f, "%ld", &x) != 1) rc = NRC_ERR_ARG;
       if (rc == NRC_OK && (fscanf(f, "%d %d", &v[0], &v[1]) != 2 || v[0] < NRC_PATH_AUTO ||
                            v[0] > NRC_PATH_SINGLE_PASS || v[1] < 1 ||
                            v[1] > nrc::SLICE_FUSE_MAX))
+        rc = NRC_ERR_ARG;
+      if (rc == NRC_OK && version == 2 && (fscanf(f, "%d", &v[2]) != 1 || v[2] < 0 || v[2] > 2))
         rc = NRC_ERR_ARG;
       if (rc == NRC_OK) ens[k] = v;
     } else {

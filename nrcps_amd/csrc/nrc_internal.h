@@ -128,8 +128,10 @@ struct SliceXArgs {
   const double* trap_sep = nullptr;  // separable trap q = qx qy qz: [qx(L), qy(L), qz(L)]
   GatherSpec g;
   long m_stride = 0;
+  int tile = 0;  // x-pencils per thread (0: the shape's default; slicex_tiles)
 };
 bool slicex_supported(int L);
+int slicex_tiles(int L, int* tiles);  // the tiles a shape offers, default first; returns count
 hipError_t launch_slicex(const SliceXArgs& a, hipStream_t s);
 bool slice1_supported(int prec, int L);
 bool slice1p_trap_ok(int prec, int L);  // the persistent kernel takes a trap at this shape

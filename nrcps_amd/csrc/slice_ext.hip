@@ -290,15 +290,30 @@ static hipError_t launch_sx(const SliceXArgs& a, hipStream_t s) {
 
 bool slicex_supported(int L) { return L == 8 || L == 12 || L == 16 || L == 24 || L == 32; }
 
+// tiles (x-pencils per thread) a shape offers: the first is the default
+int slicex_tiles(int L, int* tiles) {
+  if (L == 32) {  // one pencil per thread would need twice the registers
+    tiles[0] = 2;
+    return 1;
+  }
+  if (!slicex_supported(L)) return 0;
+  tiles[0] = 1;
+  tiles[1] = 2;
+  return 2;
+}
+
 hipError_t launch_slicex(const SliceXArgs& a, hipStream_t s) {
   if (a.nvol <= 0) return hipSuccess;
   if (a.grid < 1 || !a.tab_x || !a.tab_k || !a.bits) return hipErrorInvalidValue;
+  const bool two = a.tile == 2;
+  if (a.tile != 0 && a.tile != 1 && a.tile != 2) return hipErrorInvalidValue;
   switch (a.L) {
-    case 8: return launch_sx<8, 1>(a, s);
-    case 12: return launch_sx<12, 1>(a, s);
-    case 16: return launch_sx<16, 1>(a, s);
-    case 24: return launch_sx<24, 1, false>(a, s);  // FP64 rows spill at this shape
-    case 32: return launch_sx<32, 2>(a, s);
+    case 8: return two ? launch_sx<8, 2>(a, s) : launch_sx<8, 1>(a, s);
+    case 12: return two ? launch_sx<12, 2>(a, s) : launch_sx<12, 1>(a, s);
+    case 16: return two ? launch_sx<16, 2>(a, s) : launch_sx<16, 1>(a, s);
+    // FP64 rows spill at this shape
+    case 24: return two ? launch_sx<24, 2, false>(a, s) : launch_sx<24, 1, false>(a, s);
+    case 32: return a.tile == 1 ? hipErrorInvalidValue : launch_sx<32, 2>(a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -34,6 +34,7 @@ def test_tune_slices_per_launch_and_persist(orc):
     kinds = [k for k, _, _ in rep]
     assert kinds.count("path") == 2 and kinds.count("slices") == 3
     assert sorted(v for k, v, _ in rep if k == "slices") == [1, 2, 4]
+    assert sorted(v for k, v, _ in rep if k == "tile") == [1, 2]  # x-pencils per thread
     assert all(t > 0 for _, _, t in rep) and ms > 0
     best = min((t, v) for k, v, t in rep if k == "slices")[1]
     assert e.launch_slices() == best
@@ -48,7 +49,7 @@ def test_tune_slices_per_launch_and_persist(orc):
         path_f = os.path.join(tmp, "tune.txt")
         ctx.tune_save(path_f)
         text = open(path_f).read()
-        assert text.startswith("nrcps-tune 1\n") and "ensemble 16 7 0 0" in text
+        assert text.startswith("nrcps-tune 2\n") and "ensemble 16 7 0 0" in text
         ctx2 = na.Context(0)
         h = _ens(ctx2, orc, L, N, C)
         assert h.launch_slices() == 4
@@ -56,7 +57,7 @@ def test_tune_slices_per_launch_and_persist(orc):
         assert h.launch_slices() == best
         bad = os.path.join(tmp, "bad.txt")
         with open(bad, "w") as fh:
-            fh.write("nrcps-tune 1\nensemble 16 7 0 0 4 1 0 64 2 9\n")  # 9 slices: refused
+            fh.write("nrcps-tune 2\nensemble 16 7 0 0 4 1 0 64 2 9 1\n")  # 9 slices: refused
         with pytest.raises(na.NrcError):
             ctx2.tune_load(bad)
         h.close()
