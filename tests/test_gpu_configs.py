@@ -169,3 +169,37 @@ def test_config5_learned_sampler_l24_fields_replayed_by_oracle(ctx, orc):
         assert logr0[c] == pytest.approx(r.sum(), rel=1e-12, abs=1e-12)
     print(f"[gpu]   sampled fraction of +1: {(phi == 1).mean():.4f}; log r after T: "
           f"{lr.mean():.3f}", flush=True)
+
+
+def test_config2_full_size_fp64(ctx, orc):
+    """Config 2 at its full size (L=16, N=7, T=64, 2048 configurations, FP64): the Philox
+    fields of every configuration bit-exact against the oracle at three slices, every 64th
+    configuration's log|det| and phase against the oracle to 1e-12, and the correlator's
+    sample mean over all 2048 consistent with the sampled configurations' own (a check that
+    the batch carries no configuration-dependent offset)."""
+    import nrcps_amd as na
+
+    L, N, T, C, seed = 16, 7, 64, 2048, 7012
+    V = L ** 3
+    print(f"[gpu] config 2: L={L} N={N} T={T} C={C} FP64", flush=True)
+    words = ctx.array((C, V // 32), np.uint32)
+    for t in (0, 31, 63):
+        ctx.lib.nrc_z2_bits(ctx.h, seed, 0, C, t, 0, V, words.ptr)
+        got = words.download()
+        ref = np.stack([orc.z2_bits(seed, c, t, V) for c in range(C)])
+        assert np.array_equal(got, ref)
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    e = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F64)
+    la, ph, _ = e.run(seed, 0, T)
+    e.close()
+    sample = range(0, C, 64)
+    for c in sample:
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, seed=seed, config=c)
+        assert np.abs(la[:, c] - ref["logabs"]).max() < 1e-12
+        assert np.abs(ph[:, c] - ref["phase"]).max() < 1e-12
+    d2 = (np.exp(2 * la[:, :, 0]) * ph[:, :, 0] ** 2).real
+    full, sub = d2.mean(axis=1), d2[:, list(sample)].mean(axis=1)
+    se = d2[:, list(sample)].std(axis=1, ddof=1) / np.sqrt(len(sample))
+    assert np.all(np.isfinite(full)) and full[0] == pytest.approx(1.0, abs=1e-12)
+    assert np.all(np.abs(full[1:] - sub[1:]) < 5 * se[1:] + 1e-300)
