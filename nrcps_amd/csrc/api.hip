@@ -489,6 +489,8 @@ extern "C" {
 int nrc_overlap(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns, int nt,
                 long vol, int ncfg, int mode) {
   if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
+  if (mode < NRC_OVERLAP_F64 || mode > NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  if (ncfg == 0) return NRC_OK;  // nothing to launch
   const long nM = 2L * ncfg * ns * nt;
   if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {
     const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
@@ -548,6 +550,8 @@ int nrc_overlap_tune(nrc_context* ctx, const void* phi, const void* psi, double*
 int nrc_overlap_noalloc(nrc_context* ctx, const void* phi, const void* psi, double* M, int ns,
                         int nt, long vol, int ncfg, int mode) {
   if (!ctx || !phi || !psi || !M || ns < 1 || nt < 1 || vol < 1 || ncfg < 0) return NRC_ERR_ARG;
+  if (mode < NRC_OVERLAP_F64 || mode > NRC_OVERLAP_BF16X6) return NRC_ERR_ARG;
+  if (ncfg == 0) return NRC_OK;  // nothing to launch
   if (mode == NRC_OVERLAP_F64 || mode == NRC_OVERLAP_F32) {  // these use no scratch
     const int prec = mode == NRC_OVERLAP_F64 ? NRC_PREC_F64 : NRC_PREC_F32;
     NRC_HIP(nrc::launch_overlap_dense(prec, phi, psi, (double2*)M, ns, nt, vol, ncfg,
