@@ -308,6 +308,14 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     t_learned = time.perf_counter() - t0
     la_l = np.concatenate([p[0] for p in parts], axis=1)
     logr = np.concatenate([p[2] for p in parts], axis=1)
+    # the same network drawing four slices' fields per group, evolved in four-slice launches
+    sampler.run(seed + 3, 1 << 23, 4, group=4)  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    gparts = [sampler.run(seed + 3, (1 << 26) + k, T, group=4) for k in range(0, n_learned, chunk)]
+    t_grouped = time.perf_counter() - t0
+    la_g = np.concatenate([q[0] for q in gparts], axis=1)
+    logr_g = np.concatenate([q[2] for q in gparts], axis=1)
     # the fitted constant tilt alone: tilted Philox fields on the uniform sampler's engine
     q = ens.set_tilt(0.5 * (1.0 + np.tanh(lam_fit)))
     ens.run(seed + 2, 1 << 23, 2)  # warm-up
@@ -323,8 +331,19 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     sp = pr.estimator_stats(la_plain, None, ts)
     sl = pr.estimator_stats(la_l, logr, ts)
     st = pr.estimator_stats(la_t, logr_t, ts)
+    sg = pr.estimator_stats(la_g, logr_g, ts)
     tau_p, tau_l = t_plain / n_plain, t_learned / n_learned
     tau_t = t_tilt / n_learned
+    tau_g = t_grouped / n_learned
+    grouped = {}
+    for t in ts:
+        vp = sp[t]["rel_se"] ** 2 * n_plain
+        vg = sg[t]["rel_se"] ** 2 * n_learned
+        sig = float(np.hypot(sp[t]["rel_se"], sg[t]["rel_se"]))
+        grouped[str(t)] = {"log_C": sg[t]["log_C"], "rel_se": sg[t]["rel_se"],
+                           "bias_within_2sigma": bool(abs(sg[t]["log_C"] - sp[t]["log_C"])
+                                                      <= 2 * sig),
+                           "variance_gain": vp / vg, "gain": (vp * tau_p) / (vg * tau_g)}
     tilted = {}
     for t in ts:
         vp = sp[t]["rel_se"] ** 2 * n_plain
@@ -369,6 +388,13 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                             "uniform configuration; the learned proposal's gain is its "
                             "variance gain over its cost ratio",
             "per_t": per_t,
+            "network_grouped": {
+                "what": "the trained network drawing four slices' fields in a row (each on the "
+                        "previous field and the log-weight of the last row measured before "
+                        "the group), evolved in four-slice launches (nrc_ensemble_evolve_fused)",
+                "cost_ratio_to_uniform": tau_g / tau_p,
+                "gain_min_over_t": min(v["gain"] for v in grouped.values()),
+                "per_t": grouped, "seconds": t_grouped},
             "fitted_tilt": {
                 "what": "the constant per-site tilt fitted to the training sample's field sums "
                         "and log-determinants, drawn as tilted Philox fields on the uniform "
@@ -379,7 +405,7 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "proposal_weight_ess_fraction_at_3T/4": float(w.sum() ** 2 / (w ** 2).sum() / w.size),
             "seconds": {"uniform_training_sample": t_train_gen, "training": t_train,
                         "uniform_comparison_sample": t_plain, "proposed_sample": t_learned,
-                        "tilted_sample": t_tilt},
+                        "tilted_sample": t_tilt, "grouped_proposed_sample": t_grouped},
             "training": {"steps": steps, "batch": batch, "lr": lr,
                          "optimizer": "sgd-momentum" if optimizer == 1 else "adam",
                          "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
@@ -675,6 +701,9 @@ def main():
         learned = learned_proposal(ctx, na, conj=bool(args.conj))
         print(f"fitted tilt: gain {learned['fitted_tilt']['gain_min_over_t']:.2f}x effective "
               f"samples/s", file=sys.stderr, flush=True)
+        print(f"learned proposal, 4-slice groups: gain "
+              f"{learned['network_grouped']['gain_min_over_t']:.2f}x effective samples/s",
+              file=sys.stderr, flush=True)
         print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
 

@@ -346,6 +346,14 @@ int nrc_batch_destroy(nrc_batch* batch);
 int nrc_ensemble_begin(nrc_ensemble* ens, int T);
 int nrc_ensemble_measure(nrc_ensemble* ens, int t);
 int nrc_ensemble_evolve(nrc_ensemble* ens, int t, const uint32_t* words_dev);
+/* Slices t .. t + S - 1 (S <= 4) in one launch of the persistent kernels with the caller's
+ * fields words_dev[S][n_configs][ceil(V/32)], measuring rows t .. t + S - 1 inside (plane-wave
+ * sinks only).  FP32 ensembles at the default accuracy run the FP64 x-chain kernel, whose
+ * state keeps an x-transform pending between calls: such a run goes through evolve_fused
+ * from t = 0 to the end (measure and evolve are refused in between).  Other ensembles measure
+ * row t and run the fused single-pass kernel.  For samplers that draw several slices' fields
+ * before evolving them (nrcps_amd/proposal.py LearnedSampler with group > 1). */
+int nrc_ensemble_evolve_fused(nrc_ensemble* ens, int t, int S, const uint32_t* words_dev);
 int nrc_ensemble_results_dev(nrc_ensemble* ens, const double** logabs_dev,
                              const double** phase_dev);
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
     "nrc_overlap_set_variant", "nrc_overlap_variant", "nrc_overlap_tune",
     "nrc_ensemble_tune_report", "nrc_ensemble_launch_slices", "nrc_tune_save", "nrc_tune_load",
-    "nrc_last_error_detail", "nrc_check_build",
+    "nrc_last_error_detail", "nrc_check_build", "nrc_ensemble_evolve_fused",
     "nrc_scale_const", "nrc_norms",
     "nrc_ensemble_profile",
     "nrc_ensemble_kernel_time", "nrc_ensemble_results", "nrc_ensemble_tune",
@@ -79,6 +79,7 @@ def load_library() -> C.CDLL:
     sig = {
         "nrc_init": [i, C.POINTER(vp)], "nrc_finalize": [vp], "nrc_error_string": [i],
         "nrc_last_error_detail": [], "nrc_check_build": [],
+        "nrc_ensemble_evolve_fused": [vp, i, i, vp],
         "nrc_device_name": [vp, C.c_char_p, i, C.POINTER(i)],
         "nrc_malloc": [vp, C.POINTER(vp), sz], "nrc_free": [vp, vp],
         "nrc_memcpy_h2d": [vp, vp, vp, sz], "nrc_memcpy_d2h": [vp, vp, vp, sz],
@@ -366,6 +367,12 @@ class Ensemble:
 
     def evolve(self, t: int, words: "DeviceArray"):
         _check(self.lib.nrc_ensemble_evolve(self.h, t, words.ptr), "ensemble_evolve")
+
+    def evolve_fused(self, t: int, S: int, words: "DeviceArray"):
+        """Slices t .. t+S-1 in one launch with fields words [S][C][nw], rows t .. t+S-1
+        measured inside (nrc_ensemble_evolve_fused)."""
+        _check(self.lib.nrc_ensemble_evolve_fused(self.h, t, S, words.ptr),
+               "ensemble_evolve_fused")
 
     def results_dev(self):
         """Device addresses (ints) of the [T][C][m] log|det| and phase result rows."""

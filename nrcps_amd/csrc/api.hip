@@ -2384,12 +2384,39 @@ extern "C" int nrc_ensemble_begin(nrc_ensemble* e, int T) {
 extern "C" int nrc_ensemble_measure(nrc_ensemble* e, int t) {
   if (!e || t < 0 || t >= e->last_T) return NRC_ERR_ARG;
   if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  if (e->pending_x) return NRC_ERR_ARG;  // mid-run of evolve_fused: it measures inside
   return ens_measure(e, t);
 }
 
 extern "C" int nrc_ensemble_evolve(nrc_ensemble* e, int t, const uint32_t* words_dev) {
   if (!e || !words_dev || t < 0 || t >= e->last_T) return NRC_ERR_ARG;
   if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  if (e->pending_x) return NRC_ERR_ARG;
+  return ens_evolve(e, t, words_dev);
+}
+
+// S slices t .. t + S - 1 in one launch of the persistent kernels with the caller's device
+// fields words_dev[S][C][nw], measuring rows t .. t + S - 1 inside (plane-wave sinks): the FP64
+// x-chain kernel where the ensemble runs it (between calls the state keeps its x-DFT(+1)
+// pending, so measure / evolve are refused until the run ends), else the fused single-pass
+// kernel after measuring row t.
+extern "C" int nrc_ensemble_evolve_fused(nrc_ensemble* e, int t, int S,
+                                         const uint32_t* words_dev) {
+  if (!e || !words_dev || S < 1 || S > nrc::SLICE_FUSE_MAX || t < 0 || t + S > e->last_T)
+    return NRC_ERR_ARG;
+  if (!ens_step_ok(e)) return NRC_ERR_UNSUPPORTED;
+  if (e->sinks || e->pair) return NRC_ERR_UNSUPPORTED;  // measured inside: plane waves only
+  const size_t slice_words = (size_t)e->d.n_configs * e->nw;
+  if (ens_xchain(e, true)) {
+    if (t == 0) e->pending_x = false;
+    else if (!e->pending_x) return NRC_ERR_ARG;  // a run of the x-chain starts at t = 0
+    return ens_evolve_x(e, t, S, words_dev, slice_words, t + S == e->last_T);
+  }
+  if (e->pending_x) return NRC_ERR_ARG;
+  const bool fuse = !e->smear && (!e->trap_tab || nrc::slice1p_trap_ok(e->d.prec, e->d.L));
+  if (S > 1 && !fuse) return NRC_ERR_UNSUPPORTED;
+  NRC_TRY(ens_measure(e, t));
+  if (S > 1) return ens_evolve_fused(e, t, S, words_dev, slice_words);
   return ens_evolve(e, t, words_dev);
 }
 

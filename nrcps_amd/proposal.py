@@ -134,6 +134,13 @@ def det2_weights(la: np.ndarray, ts) -> np.ndarray:
     return w / len(ts)
 
 
+class _Slot:
+    """A device address inside a larger allocation (one slice of a [S][B][nw] field array)."""
+
+    def __init__(self, addr: int):
+        self.ptr = C.c_void_p(addr)
+
+
 class LearnedSampler:
     """Configurations sampled slice by slice from the proposal: measure at t, normalise the
     log-weight, draw phi(t), evolve slice t (nrc_ensemble_begin / measure / evolve)."""
@@ -146,10 +153,16 @@ class LearnedSampler:
         self.words = [ctx.array((B, prop.nw), np.uint32), ctx.array((B, prop.nw), np.uint32)]
         self.u = ctx.array((B,), np.float32)
         self.logr = ctx.array((B,), np.float64)
+        self.group_words = None
 
-    def run(self, seed: int, config0: int, T: int):
+    def run(self, seed: int, config0: int, T: int, group: int = 1):
         """Returns log|det| [T][C][m], phase [T][C][m] and log r_t [T][C] (the ratio of the
-        slices before t)."""
+        slices before t).  group > 1: the fields of `group` slices are drawn in a row, each
+        conditioned on the previous slice's field and on the log-weight of the last row
+        measured before the group (0 before any), and evolved in one launch of the persistent
+        kernels (nrc_ensemble_evolve_fused); the ratios stay exact for the fields drawn."""
+        if group > 1:
+            return self._run_grouped(seed, config0, T, group)
         e, B = self.ens, self.ens.C
         e.begin(T)
         la_dev, _ = e.results_dev()
@@ -166,6 +179,35 @@ class LearnedSampler:
             self.prop.sample(prev, self.u, B, seed, config0, t, cur, self.logr)
             e.evolve(t, cur)
             prev = cur
+        la, ph, _ = e.results(T)
+        return la, ph, logr_hist.download()
+
+    def _run_grouped(self, seed: int, config0: int, T: int, group: int):
+        e, B, nw = self.ens, self.ens.C, self.prop.nw
+        if self.group_words is None or self.group_words.shape[0] < group:
+            self.group_words = self.ctx.array((group, B, nw), np.uint32)
+        gw = self.group_words
+        e.begin(T)
+        la_dev, _ = e.results_dev()
+        lib, h = self.ctx.lib, self.ctx.h
+        _check(lib.nrc_memset(h, self.logr.ptr, 0, self.logr.nbytes), "memset")
+        logr_hist = self.ctx.array((T, B), np.float64)
+        prev = None
+        for t0 in range(0, T, group):
+            S = min(group, T - t0)
+            if t0 == 0:
+                _check(lib.nrc_memset(h, self.u.ptr, 0, self.u.nbytes), "memset")
+            else:
+                norm_logw(self.ctx, la_dev + (t0 - 1) * B * e.nm * 8, e.nm, B,
+                          float(self.mu[t0 - 1]), float(self.sd[t0 - 1]), self.u)
+            for k in range(S):
+                t = t0 + k
+                _check(lib.nrc_memcpy_d2d(h, C.c_void_p(logr_hist.ptr.value + t * B * 8),
+                                          self.logr.ptr, B * 8), "d2d")
+                cur = _Slot(gw.ptr.value + k * B * nw * 4)
+                self.prop.sample(prev, self.u, B, seed, config0, t, cur, self.logr)
+                prev = cur
+            e.evolve_fused(t0, S, gw)
         la, ph, _ = e.results(T)
         return la, ph, logr_hist.download()
 

@@ -267,3 +267,104 @@ def test_trained_proposal_gain_and_bias(ctx, orc):
         assert ceil[t]["variance_gain_constant_tilt"] > 1.0
     prop.close()
     ens.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_evolve_fused_reproduces_run(ctx, orc, prec):
+    """nrc_ensemble_evolve_fused over groups of 4, 4 and 2 slices with the uniform sampler's
+    Philox words gives exactly nrc_ensemble_run (the same kernels: the FP64 x-chain for FP32,
+    the fused single pass for FP64); single-slice measure / evolve are refused while the
+    x-chain state is mid-run."""
+    import nrcps_amd as na
+
+    L, N, C, T, seed = 8, 7, 3, 10, 21
+    V = L ** 3
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    p = na.PREC_F32 if prec == "f32" else na.PREC_F64
+    a = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=p)
+    a.set_symmetry(True)
+    la, ph, _ = a.run(seed, 0, T)
+    a.begin(T)
+    words = ctx.array((4, C, V // 32), np.uint32)
+    for t0, S in ((0, 4), (4, 4), (8, 2)):
+        w = np.stack([[orc.z2_bits(seed, c, t0 + k, V) for c in range(C)] for k in range(S)])
+        words.upload(np.concatenate([w, np.zeros((4 - S, C, V // 32), np.uint32)]))
+        a.evolve_fused(t0, S, words)
+        if prec == "f32" and t0 == 0:
+            with pytest.raises(na.NrcError):
+                a.measure(4)
+    lb, pb, _ = a.results(T)
+    a.close()
+    assert np.array_equal(la, lb) and np.array_equal(ph, pb)
+
+
+def test_grouped_learned_sampler_fields_replayed_and_unbiased(ctx, orc):
+    """The learned sampler drawing 4 slices' fields in a row (each conditioned on the previous
+    field and on the log-weight of the last row measured before the group) and evolving them
+    in one launch: the fields it draws, replayed through the oracle, give the same
+    determinants (a random network); and with a uniformly biased proposal (output bias 0.03)
+    its reweighted C(t) agrees with the uniform sampler's."""
+    import nrcps_amd as na
+    from nrcps_amd import proposal as pr
+
+    L, N = 8, 7
+    V = L ** 3
+    moms = orc.lowest_momenta(L, N)
+    kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
+    p = pr.init_params(np.random.default_rng(2), out_scale=0.05)
+    p[8240] = 0.03
+    # replay at B = 3, T = 10: the grouped loop with every group's fields kept
+    B, T, seed, G = 3, 10, 12, 4
+    prop = pr.Proposal(ctx, L, B, p)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, B, prec=na.PREC_F32)
+    ens.begin(T)
+    la_dev, _ = ens.results_dev()
+    u = ctx.to_device(np.zeros(B, np.float32))
+    logr = ctx.to_device(np.zeros(B, np.float64))
+    gw = ctx.array((G, B, V // 32), np.uint32)
+    slots = [ctx.array((B, V // 32), np.uint32) for _ in range(G)]
+    words, prev = [], None
+    for t0 in range(0, T, G):
+        S = min(G, T - t0)
+        if t0 > 0:
+            pr.norm_logw(ctx, la_dev + (t0 - 1) * B * ens.nm * 8, ens.nm, B, 0.0, 1.0, u)
+        for k in range(S):
+            prop.sample(prev, u, B, seed, 0, t0 + k, slots[k], logr)
+            prev = slots[k]
+        drawn = np.stack([slots[k].download() for k in range(S)])
+        words.extend(drawn)
+        gw.upload(np.concatenate([drawn, np.zeros((G - S, B, V // 32), np.uint32)]))
+        ens.evolve_fused(t0, S, gw)
+    la, ph, _ = ens.results(T)
+    prop.close()
+    ens.close()
+    w = np.stack(words)  # [T][B][nw]
+    bits = (w[..., :, None] >> np.arange(32, dtype=np.uint32)) & 1
+    phi = (2 * bits.reshape(T, B, -1)[..., :V] - 1).astype(np.int8)
+    for c in range(B):
+        ref = orc.run_config(L, N, T, moms, kin, A=0.4, field="array", phi=phi[:, c])
+        assert np.abs(la[:, c] - ref["logabs"]).max() < 5e-6
+        assert np.abs(ph[:, c] - ref["phase"]).max() < 5e-6
+    # unbiasedness at C = 8192, T = 12
+    C, T = 8192, 12
+    print(f"[gpu] grouped learned sampler L={L} N={N} T={T} C={C}", flush=True)
+    plain = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    la_p, _, _ = plain.run(11, 0, T)
+    plain.close()
+    mu, sd = pr.logw_stats(la_p)
+    p2 = np.zeros(8241, np.float32)  # a uniform tilt: ratios stay light-tailed
+    p2[8240] = 0.03
+    prop = pr.Proposal(ctx, L, C, p2)
+    ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=na.PREC_F32)
+    la_l, _, lr = pr.LearnedSampler(ctx, prop, ens, mu, sd).run(12, 0, T, group=4)
+    assert np.all(np.isfinite(lr)) and np.all(lr[0] == 0)
+    sp = pr.estimator_stats(la_p, None, [3, 6, 9, 11])
+    sl = pr.estimator_stats(la_l, lr, [3, 6, 9, 11])
+    for t in (3, 6, 9, 11):
+        d = abs(sl[t]["log_C"] - sp[t]["log_C"])
+        sig = np.hypot(sl[t]["rel_se"], sp[t]["rel_se"])
+        print(f"[gpu]   t={t}: |d log C|/sigma {d / sig:.2f}", flush=True)
+        assert d < 3.5 * sig
+    prop.close()
+    ens.close()
