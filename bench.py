@@ -163,12 +163,18 @@ def correlator(la, ph):
     return m[:, 0], c
 
 
+ACC_NAMES = {0: "plain FP32", 1: "FP32 state, FP64 x-chain and z-rows",
+             2: "FP32 state, FP64 x-chain, FP32 z-rows"}
+ACC_KEYS = {0: "plain_fp32", 1: "fp64_x_chain_and_z_rows", 2: "fp64_x_chain"}
+
+
 def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,), acc=1,
                      ref64=None):
     """Per-t relative error of C(t) from the FP32 path against the FP64 path (every orbital
     evolved) on the same configurations (same Philox fields), and the largest log|det|
     difference; one report per FP32 mode (conjugate partners off / on) in the arithmetic
-    `acc` (1: FP64 x-chain, 0: plain FP32).  ref64: the FP64 run's (logabs, phase), if taken."""
+    `acc` (1: FP64 x-chain and z-rows, 2: FP64 x-chain only, 0: plain FP32).  ref64: the FP64
+    run's (logabs, phase), if taken."""
     moms = _lowest(L, N)
     kin = _tables(L, tau, 1.0)
     if ref64 is None:
@@ -188,7 +194,7 @@ def precision_report(ctx, na, L, N, T, tau, A, nconf, seed, conj_modes=(False,),
         rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
         reports.append({
             "configs": nconf, "T": T, "conj_partners": bool(conj),
-            "arithmetic": ("FP32 state, FP64 x-chain and z-rows" if acc == 1 else "plain FP32"),
+            "arithmetic": ACC_NAMES[acc],
             "max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
             "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
             "rel_err_C_at_t": {str(t): float(rel[t]) for t in (0, T // 4, T // 2, 3 * T // 4,
@@ -432,7 +438,7 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
                     sinks=sinks)
     e.set_path(na.PATH_SINGLE_PASS)
     e.set_accuracy(acc)
-    terms = 6 if acc == 1 else 3
+    terms = 6 if acc != 0 else 3
     e.run(seed, 0, 2)
     ctx.sync()
     t0 = time.perf_counter()
@@ -457,7 +463,7 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
                 "reference": "FP64 path on the same fields (the precision report's)",
                 "arithmetic": ("FP32 evolution with the FP64 x-chain (one slice per launch, "
                                "every orbital), split-BF16x6 overlaps on MFMA with FP32 "
-                               "accumulation, FP64 LU" if acc == 1 else
+                               "accumulation, FP64 LU" if acc != 0 else
                                "plain FP32 evolution (single-slice kernel, every orbital), "
                                "split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU")}
     ms = t_ovl / max(n_ovl, 1)
@@ -565,7 +571,7 @@ def evolution_roofline(L, N, T, C, V, nvo, args, tx, nx, tyz, nyz, tlu, nlu, tf,
     single = nyz == 0
     per_launch = T / max(nx, 1)  # slices per launch of the evolution kernel
     launch_ms = tx / max(nx, 1)
-    if single and args.acc == 1 and args.path != 1:
+    if single and args.acc != 0 and args.path != 1:
         kid, kernel = "k_slicex", f"k_slicex ({per_launch:g} slices per launch, FP64 x-chain)"
     elif single and per_launch > 1:
         kid, kernel = "k_slice1p", f"k_slice1p ({per_launch:g} slices per launch)"
@@ -637,7 +643,8 @@ def main():
                     help="1: conjugate partners (one orbital of each (k, -k) pair evolved); "
                          "0: every orbital evolved, as the reference does")
     ap.add_argument("--acc", type=int, default=1,
-                    help="FP32 arithmetic: 1 FP64 x-chain and z-rows (default), 0 plain FP32")
+                    help="FP32 arithmetic: 1 FP64 x-chain and z-rows (default), 2 FP64 x-chain "
+                         "with FP32 z-rows, 0 plain FP32")
     ap.add_argument("--path", type=int, default=0,
                     help="evolution kernels: 0 auto, 1 two passes per slice, 2 single pass")
     args = ap.parse_args()
@@ -716,10 +723,9 @@ def main():
     other = None
     if not args.no_other_mode:
         other = {}
-        for key, conj, acc in (("conj_partners_" + ("off" if args.conj else "on"),
-                                not args.conj, args.acc),
-                               ("plain_fp32" if args.acc else "fp64_x_chain", bool(args.conj),
-                                1 - args.acc)):
+        modes = [("conj_partners_" + ("off" if args.conj else "on"), not args.conj, args.acc)]
+        modes += [(ACC_KEYS[a], bool(args.conj), a) for a in (0, 1, 2) if a != args.acc]
+        for key, conj, acc in modes:
             e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
             e.set_path(args.path)
             e.set_symmetry(conj)
@@ -762,9 +768,11 @@ def main():
         if len(reps) > 1 and other is not None:
             other["conj_partners_" + ("off" if args.conj else "on")]["precision"] = reps[1]
         if other is not None:
-            other["plain_fp32" if args.acc else "fp64_x_chain"]["precision"] = precision_report(
-                ctx, na, L, N, T, tau, A, args.precision_configs, seed, (bool(args.conj),),
-                acc=1 - args.acc, ref64=ref64)[0]
+            for a in (0, 1, 2):
+                if a != args.acc:
+                    other[ACC_KEYS[a]]["precision"] = precision_report(
+                        ctx, na, L, N, T, tau, A, args.precision_configs, seed,
+                        (bool(args.conj),), acc=a, ref64=ref64)[0]
         print(f"precision: max rel err of C(t) {precision['max_rel_err_C']:.2e}",
               file=sys.stderr, flush=True)
 
@@ -790,9 +798,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"Slater-determinant correlator over aux-field configurations: "
                                f"L={L}, N={N}, T={T}, tau={tau}, A={A}, {C} configs per step, "
-                               f"FP32 evolution ("
-                               + ("FP64 x-chain and z-rows" if args.acc else "plain FP32")
-                               + "), FP64 LU",
+                               f"evolution in {ACC_NAMES[args.acc]}, FP64 LU",
                    "L": L, "N": N, "T": T, "configs": C,
                    "parallelism": "single GPU, batched configurations"},
         "roofline": roofline,

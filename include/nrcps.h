@@ -253,15 +253,19 @@ int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
 int nrc_ensemble_set_trap(nrc_ensemble* ens, const double* q);
 /* Arithmetic of FP32 ensembles (k-space split, plain or trapped-free field, plane-wave sinks,
  * single-pass path).  NRC_ACC_FP32: every transform and multiply in FP32 (compensated
- * twiddles and factors).  NRC_ACC_X64 (the default): the state stays FP32 in HBM, registers
- * and LDS, and so do the y and z transforms and the field multiply, but per x-pencil the
- * chain x-DFT(+1) -> sink gather -> kinetic factor -> x-DFT(-1), where FP32 rounding lands
- * on the sink momenta with full weight, runs on FP64 registers rounded once; between
- * launches the state is held with its last x-DFT(+1) pending.  Other ensembles ignore the
- * setting.  No counterpart in the reference (one Float type everywhere, config.h:19-48).
- * Not while the ensemble is in a batch. */
+ * twiddles and factors).  NRC_ACC_XCHAIN: the state stays FP32 in HBM, registers and LDS,
+ * and so do the y and z transforms and the field multiply, but per x-pencil the chain
+ * x-DFT(+1) -> sink gather -> kinetic factor -> x-DFT(-1), where FP32 rounding lands on the
+ * sink momenta with full weight, runs on FP64 registers rounded once; between launches the
+ * state is held with its last x-DFT(+1) pending.  NRC_ACC_X64 (the default): the same with
+ * the z transforms of each LDS row also on FP64 registers (except at L = 24, where they
+ * would spill).  At L=32, N=33, T=128 the correlator is within ~2e-6 (FP32), ~8e-7 (XCHAIN)
+ * and ~2.5e-7 (X64) of FP64; XCHAIN runs ~9% faster than X64, FP32 ~13%.  Other ensembles
+ * ignore the setting.  No counterpart in the reference (one Float type everywhere,
+ * config.h:19-48).  Not while the ensemble is in a batch. */
 #define NRC_ACC_FP32 0
 #define NRC_ACC_X64 1
+#define NRC_ACC_XCHAIN 2
 int nrc_ensemble_set_accuracy(nrc_ensemble* ens, int mode);
 /* Importance-sampled Z2 fields (the Z2 form of production/one_body-reweight/main.C:73-76,
  * which draws Lattice::Refresh(phi0), a Gaussian field shifted off zero, and divides the

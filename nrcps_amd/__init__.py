@@ -23,7 +23,7 @@ PREC_F32, PREC_F64 = 0, 1
 SPLIT_KSPACE, SPLIT_XSPACE = 0, 1
 OVERLAP_F64, OVERLAP_F32, OVERLAP_BF16X3, OVERLAP_BF16X6 = 0, 1, 2, 3
 PATH_AUTO, PATH_TWO_PASS, PATH_SINGLE_PASS = 0, 1, 2
-ACC_FP32, ACC_X64 = 0, 1
+ACC_FP32, ACC_X64, ACC_XCHAIN = 0, 1, 2
 OVL_AUTO, OVL_COLS, OVL_SHARED, OVL_SHARED_DEEP, OVL_SHARED_PIPE = 0, 1, 2, 3, 4
 SUPPORTED_L = (4, 6, 8, 12, 16, 24, 32)
 
@@ -399,7 +399,8 @@ class Ensemble:
         _check(self.lib.nrc_ensemble_set_path(self.h, path), "ensemble_set_path")
 
     def set_accuracy(self, mode: int):
-        """ACC_FP32 or ACC_X64 (the default) for FP32 ensembles (nrc_ensemble_set_accuracy)."""
+        """ACC_FP32, ACC_X64 (the default) or ACC_XCHAIN for FP32 ensembles
+        (nrc_ensemble_set_accuracy)."""
         _check(self.lib.nrc_ensemble_set_accuracy(self.h, mode), "ensemble_set_accuracy")
 
     def set_tilt(self, p_plus: float) -> float:

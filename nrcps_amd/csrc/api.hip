@@ -1053,7 +1053,8 @@ extern "C" int nrc_ensemble_set_path(nrc_ensemble* e, int path) {
 }
 
 extern "C" int nrc_ensemble_set_accuracy(nrc_ensemble* e, int mode) {
-  if (!e || (mode != NRC_ACC_FP32 && mode != NRC_ACC_X64)) return NRC_ERR_ARG;
+  if (!e || (mode != NRC_ACC_FP32 && mode != NRC_ACC_X64 && mode != NRC_ACC_XCHAIN))
+    return NRC_ERR_ARG;
   if (ens_in_batch(e)) return NRC_ERR_BATCHED;
   e->acc = mode;
   return NRC_OK;
@@ -1280,7 +1281,7 @@ static int ens_measure(nrc_ensemble* e, int t) {
     if (e->sinks && prec == NRC_PREC_F64)
       NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks) {
-      const int terms = e->acc == NRC_ACC_X64 ? 6 : 3;
+      const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
       NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s, terms,
                              overlap_variant(e->ctx, N, N, V, terms)));
     }
@@ -1434,7 +1435,7 @@ static int ens_evolve_fused(nrc_ensemble* e, int t, int S, const uint32_t* words
 // the sources (t = 0) or the state the previous launch left with its x-DFT(+1) pending, and
 // the run's last launch stores the folded k-space state
 static bool ens_xchain(const nrc_ensemble* e, bool single) {
-  return single && e->d.prec == NRC_PREC_F32 && e->acc == NRC_ACC_X64 &&
+  return single && e->d.prec == NRC_PREC_F32 && e->acc != NRC_ACC_FP32 &&
          e->d.split == NRC_SPLIT_KSPACE && !e->smear && (!e->trap_tab || e->trap_sep) &&
          nrc::slicex_supported(e->d.L);
 }
@@ -1455,6 +1456,7 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.nvol = C * e->nvo;
   a.norb = e->nvo;
   a.tile = e->tile;
+  a.zd = e->acc == NRC_ACC_X64;
   a.L = e->d.L;
   a.grid = e->ctx->n_cu;
   a.nslices = S;
@@ -2092,7 +2094,7 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
   e->tile = best_tile;
   // dense FP32 sinks: the overlap GEMM variants on the ensemble's buffers
   if (rc == NRC_OK && e->sinks && e->d.prec == NRC_PREC_F32) {
-    const int terms = e->acc == NRC_ACC_X64 ? 6 : 3;
+    const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
     const int N = e->d.n_orb;
     double ms[kNumOvlVariants];
     int v = 0;

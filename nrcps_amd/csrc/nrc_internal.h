@@ -129,6 +129,7 @@ struct SliceXArgs {
   GatherSpec g;
   long m_stride = 0;
   int tile = 0;  // x-pencils per thread (0: the shape's default; slicex_tiles)
+  int zd = 1;    // z rows in FP64 where the shape allows it (0: FP32 z rows)
 };
 bool slicex_supported(int L);
 int slicex_tiles(int L, int* tiles);  // the tiles a shape offers, default first; returns count

@@ -282,10 +282,15 @@ static hipError_t launch_sx_t(const SliceXArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ZD: the shape may run its z rows in FP64 (a.zd chooses); false: FP32 z rows only
 template <int L, int PPT, bool ZD = true>
 static hipError_t launch_sx(const SliceXArgs& a, hipStream_t s) {
   if (a.nslices < 1 || a.nslices > SLICE_FUSE_MAX) return hipErrorInvalidValue;
-  return a.trap_sep ? launch_sx_t<L, PPT, ZD, true>(a, s) : launch_sx_t<L, PPT, ZD, false>(a, s);
+  if (ZD && a.zd)
+    return a.trap_sep ? launch_sx_t<L, PPT, true, true>(a, s)
+                      : launch_sx_t<L, PPT, true, false>(a, s);
+  return a.trap_sep ? launch_sx_t<L, PPT, false, true>(a, s)
+                    : launch_sx_t<L, PPT, false, false>(a, s);
 }
 
 bool slicex_supported(int L) { return L == 8 || L == 12 || L == 16 || L == 24 || L == 32; }

@@ -83,6 +83,23 @@ def test_config3_correlator_within_1e6_of_fp64(ctx, fp64_reference, conj):
     assert np.abs(la - fp64_reference[0]).max() < 1e-5
 
 
+def test_config3_xchain_only_within_1e6_of_fp64(ctx, fp64_reference):
+    """ACC_XCHAIN keeps the FP64 x-chain but runs the y/z rows in FP32: about 8e-7 at the
+    largest t at this shape, still inside 1e-6 at every t (and the z rows' rounding is the
+    whole difference from ACC_X64, so it must stay well below the plain-FP32 figure)."""
+    import nrcps_amd as na
+
+    print("[gpu] FP32 (x-chain in FP64, z rows in FP32)", flush=True)
+    la, ph = _fp32_run(ctx, True, na.ACC_XCHAIN)
+    m64, c64 = _corr(*fp64_reference)
+    m32, c32 = _corr(la, ph)
+    rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+    print(f"[gpu]   max rel err of C(t) {rel.max():.2e} at t={rel.argmax()}, median "
+          f"{np.median(rel):.2e}", flush=True)
+    assert rel.max() <= TOL_C
+    assert np.abs(la - fp64_reference[0]).max() < 1e-5
+
+
 def test_config3_plain_fp32_error_is_bounded(ctx, fp64_reference):
     """Every transform and multiply in FP32 (ACC_FP32) misses 1e-6 at some t at this shape
     (about 2e-6 at large t); this guards that figure so that a regression shows."""
