@@ -334,6 +334,25 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     ens.set_tilt(0.5)
     la_t = np.concatenate([a for a, _ in tparts], axis=1)
     logr_t = np.concatenate([b for _, b in tparts], axis=1)
+    # fields with memory: (lambda, theta, rho) fitted to the training sample (its fields
+    # regenerated on the device, the proposal's exact ratio evaluated on them), then drawn on
+    # the uniform sampler's engine and reweighted exactly
+    t0 = time.perf_counter()
+    mem = pr.fit_memory(ctx, la_train, seed, 0, T, V, ts, lam0=float(np.tanh(lam_fit)))
+    t_mem_fit = time.perf_counter() - t0
+    mem = ens.set_memory(*mem)
+    ens.run(seed + 4, 1 << 23, 2)  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    mparts = []
+    for k in range(0, n_learned, chunk):
+        la_k = ens.run(seed + 4, (1 << 27) + k, T)[0]
+        mparts.append((la_k, ens.log_ratio(T)))
+    t_mem = time.perf_counter() - t0
+    ens.set_memory(0.0, 0.0, 0.5)
+    la_m = np.concatenate([a for a, _ in mparts], axis=1)
+    logr_m = np.concatenate([b for _, b in mparts], axis=1)
+    sm = pr.estimator_stats(la_m, logr_m, ts)
     sp = pr.estimator_stats(la_plain, None, ts)
     sl = pr.estimator_stats(la_l, logr, ts)
     st = pr.estimator_stats(la_t, logr_t, ts)
@@ -341,6 +360,23 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     tau_p, tau_l = t_plain / n_plain, t_learned / n_learned
     tau_t = t_tilt / n_learned
     tau_g = t_grouped / n_learned
+    tau_m = t_mem / n_learned
+    tau_m_amort = (t_mem + t_mem_fit + t_train_gen) / n_learned
+    memory = {}
+    for t in ts:
+        vp = sp[t]["rel_se"] ** 2 * n_plain
+        vm = sm[t]["rel_se"] ** 2 * n_learned
+        sig = float(np.hypot(sp[t]["rel_se"], sm[t]["rel_se"]))
+        memory[str(t)] = {"log_C": sm[t]["log_C"], "rel_se": sm[t]["rel_se"],
+                          "bias_within_2sigma": bool(abs(sm[t]["log_C"] - sp[t]["log_C"])
+                                                     <= 2 * sig),
+                          "variance_gain": vp / vm, "gain": (vp * tau_p) / (vm * tau_m),
+                          "gain_with_fit_amortised": (vp * tau_p) / (vm * tau_m_amort)}
+    # configurations of the memory sampler after which its gain with the one-time cost (the
+    # uniform training sample and the fit) amortised reaches 3x: g n tau / (n tau + F) = 3
+    g_min = min(v["gain"] for v in memory.values())
+    mem_3x = (3.0 * (t_mem_fit + t_train_gen) / (tau_m * (g_min - 3.0))
+              if g_min > 3.0 else None)
     grouped = {}
     for t in ts:
         vp = sp[t]["rel_se"] ** 2 * n_plain
@@ -401,6 +437,19 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                 "cost_ratio_to_uniform": tau_g / tau_p,
                 "gain_min_over_t": min(v["gain"] for v in grouped.values()),
                 "per_t": grouped, "seconds": t_grouped},
+            "memory_fields": {
+                "what": "fields with memory: each site +1 with probability (1 + lambda + theta m)"
+                        " / 2, m = sum_k rho^(k-1) phi(t-k) its own decayed past field "
+                        "(nrc_ensemble_set_memory), fitted to the training sample through the "
+                        "exact ratio evaluated on its fields (nrc_z2_memory_eval), drawn on the "
+                        "uniform sampler's kernels and reweighted exactly",
+                "lambda": mem[0], "theta": mem[1], "rho": mem[2],
+                "cost_ratio_to_uniform": tau_m / tau_p,
+                "gain_min_over_t": min(v["gain"] for v in memory.values()),
+                "gain_with_fit_amortised_min_over_t": min(v["gain_with_fit_amortised"]
+                                                          for v in memory.values()),
+                "configs_for_3x_with_fit_amortised": mem_3x,
+                "per_t": memory, "seconds": t_mem, "fit_seconds": t_mem_fit},
             "fitted_tilt": {
                 "what": "the constant per-site tilt fitted to the training sample's field sums "
                         "and log-determinants, drawn as tilted Philox fields on the uniform "
@@ -411,7 +460,8 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "proposal_weight_ess_fraction_at_3T/4": float(w.sum() ** 2 / (w ** 2).sum() / w.size),
             "seconds": {"uniform_training_sample": t_train_gen, "training": t_train,
                         "uniform_comparison_sample": t_plain, "proposed_sample": t_learned,
-                        "tilted_sample": t_tilt, "grouped_proposed_sample": t_grouped},
+                        "tilted_sample": t_tilt, "grouped_proposed_sample": t_grouped,
+                        "memory_fit": t_mem_fit, "memory_sample": t_mem},
             "training": {"steps": steps, "batch": batch, "lr": lr,
                          "optimizer": "sgd-momentum" if optimizer == 1 else "adam",
                          "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
@@ -706,6 +756,8 @@ def main():
     learned = None
     if not args.no_learned:
         learned = learned_proposal(ctx, na, conj=bool(args.conj))
+        print(f"fields with memory: gain {learned['memory_fields']['gain_min_over_t']:.2f}x "
+              f"effective samples/s", file=sys.stderr, flush=True)
         print(f"fitted tilt: gain {learned['fitted_tilt']['gain_min_over_t']:.2f}x effective "
               f"samples/s", file=sys.stderr, flush=True)
         print(f"learned proposal, 4-slice groups: gain "

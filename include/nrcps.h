@@ -281,6 +281,22 @@ int nrc_ensemble_set_accuracy(nrc_ensemble* ens, int mode);
  * ensemble is in a batch, and a batch refuses a tilted ensemble (NRC_ERR_UNSUPPORTED). */
 int nrc_ensemble_set_tilt(nrc_ensemble* ens, double p_plus);
 int nrc_ensemble_tilt(nrc_ensemble* ens, double* p_plus);  /* the q in use (0.5: plain) */
+/* Fields with memory (no counterpart in the reference; an importance sampler for the Z2
+ * field of src/lattice/lattice.C:81-86 in the manner of production/one_body-reweight's shifted
+ * field): every site remembers its own past field, m(t) = sum_{k >= 1} rho^(k-1) phi(t - k),
+ * and at slice t is +1 with probability q = (1 + lambda + theta m) / 2.  Realised exactly in
+ * integers: M = 1024 m in int16 with M(t + 1) = floor((r M + 2^14) / 2^15) + 1024 phi(t),
+ * r = round(2^15 rho); q = (2^31 + e) / 2^32 with e = clamp(round(2^31 lambda) +
+ * round(2^21 theta) M, -2^25, 2^25); site i is +1 where output i % 4 of the Philox4x32-10
+ * block with counter (i / 4, t, c, 3) is below 2^31 + e.  |lambda|, |theta| <= 2^-6,
+ * 0 <= rho <= 0.96 (NRC_ERR_ARG otherwise); lambda = theta = 0 restores the plain sampler.
+ * nrc_ensemble_log_ratio then gives log r[t][c] = sum over slices t' < t and sites of
+ * -log1p(phi eps) (eps = e / 2^31), so that sum_c det(t)^2 r / C estimates C(t) without bias.
+ * Setting memory turns the tilt off and a tilt turns memory off; not while in a batch, and a
+ * batch refuses an ensemble with memory (NRC_ERR_UNSUPPORTED).  nrc_ensemble_memory returns
+ * the realised (lambda, theta, rho). */
+int nrc_ensemble_set_memory(nrc_ensemble* ens, double lambda, double theta, double rho);
+int nrc_ensemble_memory(nrc_ensemble* ens, double* lambda, double* theta, double* rho);
 /* log_ratio: [T][C], T the last run's slice count (NRC_ERR_ARG otherwise) */
 int nrc_ensemble_log_ratio(nrc_ensemble* ens, int T, double* log_ratio);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
@@ -405,6 +421,14 @@ int nrc_z2_bits_list(nrc_context* ctx, uint64_t seed, const uint32_t* configs_de
  * q ~ p exp(lambda S), whose exact weights are known from S alone. */
 int nrc_z2_sums(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
                 uint32_t stream, int vol, int* sums_dev);
+/* The log ratio log(p / q) that fields with memory (lambda, theta, rho) would carry, evaluated
+ * on the plain sampler's fields (stream 0) of configurations config0 .. config0 + nconf - 1:
+ * out_dev[t][c] = the sum over slices t' < t (out_dev[0] = 0), for t < T.  With det(t)^2 of
+ * the same configurations, E_p[det^2 p / q] is the second moment the memory sampler's estimate
+ * would have, which is how its parameters are fitted to a plain sample.  vol <= 32768
+ * (NRC_ERR_UNSUPPORTED otherwise). */
+int nrc_z2_memory_eval(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
+                       int vol, double lambda, double theta, double rho, double* out_dev);
 
 #ifdef __cplusplus
 }

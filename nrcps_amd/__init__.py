@@ -38,6 +38,7 @@ EXPORTS = [
     "nrc_ensemble_set_symmetry", "nrc_ensemble_volumes", "nrc_ensemble_set_trap",
     "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc", "nrc_z2_sums",
     "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
+    "nrc_ensemble_set_memory", "nrc_ensemble_memory", "nrc_z2_memory_eval",
     "nrc_overlap_set_variant", "nrc_overlap_variant", "nrc_overlap_tune",
     "nrc_ensemble_tune_report", "nrc_ensemble_launch_slices", "nrc_tune_save", "nrc_tune_load",
     "nrc_last_error_detail", "nrc_check_build", "nrc_ensemble_evolve_fused",
@@ -105,6 +106,10 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_set_symmetry": [vp, i], "nrc_ensemble_volumes": [vp, C.POINTER(i)],
         "nrc_ensemble_set_trap": [vp, vp], "nrc_ensemble_set_accuracy": [vp, i],
         "nrc_ensemble_set_tilt": [vp, C.c_double],
+        "nrc_ensemble_set_memory": [vp, C.c_double, C.c_double, C.c_double],
+        "nrc_ensemble_memory": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)],
+        "nrc_z2_memory_eval": [vp, u64, u32, i, i, i, C.c_double, C.c_double, C.c_double, vp],
         "nrc_ensemble_tilt": [vp, C.POINTER(C.c_double)], "nrc_ensemble_log_ratio": [vp, i, dp],
         "nrc_scale_const": [vp, vp, C.c_long, C.c_double, i, i],
         "nrc_norms": [vp, vp, i, C.c_long, i, vp],
@@ -411,9 +416,24 @@ class Ensemble:
         _check(self.lib.nrc_ensemble_tilt(self.h, C.byref(q)), "ensemble_tilt")
         return q.value
 
+    def set_memory(self, lam: float, theta: float, rho: float):
+        """Fields with memory (nrc_ensemble_set_memory): each site +1 with probability
+        (1 + lam + theta m) / 2, m = sum_k rho^(k-1) phi(t - k) its own decayed past field
+        (lam = theta = 0: the plain sampler); returns the realised (lam, theta, rho)."""
+        _check(self.lib.nrc_ensemble_set_memory(self.h, float(lam), float(theta), float(rho)),
+               "ensemble_set_memory")
+        return self.memory()
+
+    def memory(self):
+        out = [C.c_double() for _ in range(3)]
+        _check(self.lib.nrc_ensemble_memory(self.h, *[C.byref(x) for x in out]),
+               "ensemble_memory")
+        return tuple(x.value for x in out)
+
     def log_ratio(self, T: int) -> np.ndarray:
-        """[T][C] log of the plain-to-tilted density ratio of the last run's fields before
-        each measured slice (zero for a plain run); T is the last run's slice count."""
+        """[T][C] log of the plain-to-proposal density ratio of the last run's fields before
+        each measured slice (tilted fields or fields with memory; zero for a plain run); T is
+        the last run's slice count."""
         out = np.zeros((T, self.C), dtype=np.float64)
         _check(self.lib.nrc_ensemble_log_ratio(self.h, T, _ptr(out)), "ensemble_log_ratio")
         return out

@@ -719,6 +719,14 @@ struct nrc_ensemble {
   int* fsum = nullptr;           // [T][C] field sums of the last tilted run
   size_t fsum_cap = 0;
   bool last_tilted = false;      // the last run drew tilted fields (fsum is valid)
+  // fields with memory (nrc_ensemble_set_memory): integer parameters of launch_z2_memory
+  bool mem_on = false;
+  int mem_a = 0, mem_b = 0, mem_r = 0;
+  int16_t* mem = nullptr;        // [C][nw * 32] per-site memory
+  size_t mem_cap = 0;
+  double* lrinc = nullptr;       // [T][C] log(p / q) of each slice of the last memory run
+  size_t lrinc_cap = 0;
+  bool last_memory = false;      // the last run drew fields with memory (lrinc is valid)
   bool profile = false;
   double prof_ms[5] = {0, 0, 0, 0, 0};
   long prof_n[5] = {0, 0, 0, 0, 0};
@@ -739,7 +747,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q,
-                  e->tab_x64, e->tab_k64, e->trap_sep};
+                  e->tab_x64, e->tab_k64, e->trap_sep, e->mem, e->lrinc};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (e->par_dev) hipFree(e->par_dev);
@@ -1511,6 +1519,26 @@ static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (words_host)
     NRC_HIP(hipMemcpyAsync(e->words, words_host, need_words * sizeof(uint32_t),
                            hipMemcpyHostToDevice, e->stream));
+  if (e->mem_on && !words_host) {
+    const size_t need_mem = (size_t)e->d.n_configs * e->nw * 32;
+    if (need_mem > e->mem_cap) {
+      NRC_HIP(hipStreamSynchronize(e->stream));
+      hipFree(e->mem);
+      e->mem = nullptr;
+      e->mem_cap = 0;
+      NRC_TRY(dev_alloc(&e->mem, need_mem));
+      e->mem_cap = need_mem;
+    }
+    const size_t need_lr = (size_t)T * e->d.n_configs;
+    if (need_lr > e->lrinc_cap) {
+      NRC_HIP(hipStreamSynchronize(e->stream));
+      hipFree(e->lrinc);
+      e->lrinc = nullptr;
+      e->lrinc_cap = 0;
+      NRC_TRY(dev_alloc(&e->lrinc, need_lr));
+      e->lrinc_cap = need_lr;
+    }
+  }
   const size_t need_sums = e->tilt_thr && !words_host ? (size_t)T * e->d.n_configs : 0;
   if (need_sums > e->fsum_cap) {
     NRC_HIP(hipStreamSynchronize(e->stream));
@@ -1546,9 +1574,13 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
 
   NRC_TRY(ens_set_sources(e));
   const nrc::GatherSpec g = ens_gather_spec(e);
-  // Lattice::Refresh for slice tt into dst: the plain sampler's Philox bits, or tilted ones
-  // with their field sums
+  // Lattice::Refresh for slice tt into dst: the plain sampler's Philox bits, tilted ones with
+  // their field sums, or fields with memory with their sampling ratios
   auto refresh = [&](int tt, uint32_t* dst) -> hipError_t {
+    if (e->mem_on)
+      return nrc::launch_z2_memory(seed, config0, (int)C, (uint32_t)tt, NRC_STREAM_MEMORY,
+                                   (int)V, e->mem_a, e->mem_b, e->mem_r, dst, e->mem,
+                                   e->lrinc + (size_t)tt * C, s, e->par_dev);
     if (e->tilt_thr)
       return nrc::launch_z2_tilt(seed, config0, (int)C, (uint32_t)tt, NRC_STREAM_TILT, (int)V,
                                  e->tilt_thr, dst, e->fsum + (size_t)tt * C, s, e->par_dev);
@@ -1714,7 +1746,8 @@ static int ens_run(nrc_ensemble* e, uint64_t seed, uint32_t config0, const uint3
   NRC_TRY(ens_set_params(e, seed, config0, e->stream));
   NRC_TRY(ens_enqueue(e, seed, config0, words_host != nullptr, T));
   e->last_T = T;
-  e->last_tilted = e->tilt_thr && !words_host;
+  e->last_tilted = e->tilt_thr && !words_host && !e->mem_on;
+  e->last_memory = e->mem_on && !words_host;
   return ens_collect(e, T, logabs, phase, argsum);
 }
 
@@ -1955,6 +1988,7 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
 extern "C" int nrc_ensemble_set_tilt(nrc_ensemble* e, double p_plus) {
   if (!e || !(p_plus > 0.0 && p_plus < 1.0)) return NRC_ERR_ARG;
   if (ens_in_batch(e)) return NRC_ERR_BATCHED;
+  e->mem_on = false;  // one importance sampler at a time
   if (p_plus == 0.5) {
     e->tilt_thr = 0;
     return NRC_OK;
@@ -1970,11 +2004,72 @@ extern "C" int nrc_ensemble_tilt(nrc_ensemble* e, double* p_plus) {
   return NRC_OK;
 }
 
+// ------------------------------------------------------------------ fields with memory
+// (lambda, theta, rho) to the integer parameters of launch_z2_memory: a = 2^31 lambda,
+// b = 2^21 theta (M = 1024 m), r = 2^15 rho, each rounded to nearest
+static int mem_params(double lambda, double theta, double rho, int* a, int* b, int* r) {
+  if (!(std::fabs(lambda) <= 0x1p-6) || !(std::fabs(theta) <= 0x1p-6) ||
+      !(rho >= 0.0 && rho <= 0.96))
+    return NRC_ERR_ARG;
+  *a = (int)std::nearbyint(std::ldexp(lambda, 31));
+  *b = (int)std::nearbyint(std::ldexp(theta, 21));
+  *r = (int)std::nearbyint(std::ldexp(rho, 15));
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_set_memory(nrc_ensemble* e, double lambda, double theta,
+                                       double rho) {
+  if (!e) return NRC_ERR_ARG;
+  int a, b, r;
+  NRC_TRY(mem_params(lambda, theta, rho, &a, &b, &r));
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
+  e->mem_a = a;
+  e->mem_b = b;
+  e->mem_r = r;
+  e->mem_on = a != 0 || b != 0;
+  if (e->mem_on) e->tilt_thr = 0;  // one importance sampler at a time
+  return NRC_OK;
+}
+
+extern "C" int nrc_ensemble_memory(nrc_ensemble* e, double* lambda, double* theta,
+                                   double* rho) {
+  if (!e || !lambda || !theta || !rho) return NRC_ERR_ARG;
+  *lambda = e->mem_on ? std::ldexp((double)e->mem_a, -31) : 0.0;
+  *theta = e->mem_on ? std::ldexp((double)e->mem_b, -21) : 0.0;
+  *rho = std::ldexp((double)e->mem_r, -15);
+  return NRC_OK;
+}
+
+extern "C" int nrc_z2_memory_eval(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf,
+                                  int T, int vol, double lambda, double theta, double rho,
+                                  double* out_dev) {
+  if (!ctx || !out_dev || nconf < 0 || T < 0 || vol < 1) return NRC_ERR_ARG;
+  if (((long)vol + 31) / 32 * 32 > 32768) return NRC_ERR_UNSUPPORTED;
+  int a, b, r;
+  NRC_TRY(mem_params(lambda, theta, rho, &a, &b, &r));
+  if (nconf == 0 || T == 0) return NRC_OK;
+  hipSetDevice(ctx->device);
+  NRC_HIP(nrc::launch_z2_memory_eval(seed, config0, nconf, T, vol, a, b, r, out_dev,
+                                     ctx->stream));
+  return NRC_OK;
+}
+
 // log r_t[c] = sum over the slices t' < t of log(p(phi_t') / q(phi_t')): p the plain
 // sampler's 2^-V, q the tilted one's q^n(+1) (1 - q)^n(-1); zero for a plain run
 extern "C" int nrc_ensemble_log_ratio(nrc_ensemble* e, int T, double* out) {
   if (!e || !out || e->last_T < 1 || T != e->last_T) return NRC_ERR_ARG;
   const long C = e->d.n_configs, V = e->V;
+  if (e->last_memory) {
+    std::vector<double> inc((size_t)T * C);
+    NRC_HIP(hipStreamSynchronize(e->stream));
+    NRC_HIP(hipMemcpy(inc.data(), e->lrinc, inc.size() * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    for (long c = 0; c < C; ++c) out[c] = 0.0;
+    for (int t = 1; t < T; ++t)
+      for (long c = 0; c < C; ++c)
+        out[(size_t)t * C + c] = out[(size_t)(t - 1) * C + c] + inc[(size_t)(t - 1) * C + c];
+    return NRC_OK;
+  }
   if (!e->last_tilted) {
     std::fill(out, out + (size_t)T * C, 0.0);
     return NRC_OK;
@@ -2227,7 +2322,8 @@ extern "C" int nrc_batch_create(nrc_context* ctx, nrc_ensemble* const* groups, i
   for (int i = 0; i < n_groups; ++i) {
     if (!groups[i] || groups[i]->ctx != ctx) return NRC_ERR_ARG;
     if (ens_in_batch(groups[i])) return NRC_ERR_BATCHED;  // one batch at a time
-    if (groups[i]->tilt_thr) return NRC_ERR_UNSUPPORTED;  // tilted fields: single runs only
+    if (groups[i]->tilt_thr || groups[i]->mem_on)
+      return NRC_ERR_UNSUPPORTED;  // tilted fields and fields with memory: single runs only
     for (int j = 0; j < i; ++j)
       if (groups[j] == groups[i]) return NRC_ERR_ARG;
   }
@@ -2276,6 +2372,7 @@ extern "C" int nrc_batch_run(nrc_batch* b, uint64_t seed, const uint32_t* config
     NRC_TRY(ens_prepare(e, nullptr, T));
     e->last_T = T;
     e->last_tilted = false;
+    e->last_memory = false;
   }
   for (hipStream_t st : b->streams) NRC_HIP(hipStreamSynchronize(st));
   hipStream_t main = b->ctx->stream;
@@ -2380,6 +2477,7 @@ extern "C" int nrc_ensemble_begin(nrc_ensemble* e, int T) {
   NRC_TRY(ens_prepare(e, nullptr, T));
   e->last_T = T;
   e->last_tilted = false;  // the caller's fields
+  e->last_memory = false;
   return ens_set_sources(e);
 }
 

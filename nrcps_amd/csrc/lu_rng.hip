@@ -88,6 +88,185 @@ hipError_t launch_z2_tilt(uint64_t seed, uint32_t config0, int nconf, uint32_t t
   return hipGetLastError();
 }
 
+// Z2 fields with memory.  Every site carries a memory m, the decayed sum of its own past
+// field, m(t) = sum_{k >= 1} rho^(k-1) phi(t - k), held in fixed point M = 1024 m (int16):
+//   M(0) = 0,   M(t + 1) = floor((r M(t) + 2^14) / 2^15) + 1024 phi(t),   rho = r / 2^15.
+// At slice t the site is +1 with probability q = thr / 2^32, thr = 2^31 + e,
+// e = clamp(a + b M(t), -2^25, 2^25): q = (1 + eps) / 2 with eps = e / 2^31 = lambda + theta m
+// (lambda = a / 2^31, theta = b / 2^21) within |eps| <= 2^-6.  All of it is integer
+// arithmetic, so the draw is reproducible bit for bit, and the slice's sampling ratio
+// log(p / q) = sum over sites of log((1/2) / q(phi)) = -sum log1p(phi eps) is exact up to
+// its FP64 evaluation: log1p by its Taylor series to x^10 (|x| <= 2^-6, truncation below
+// 2^-60 |x|).
+constexpr int kMemEMax = 1 << 25;
+
+__device__ __forceinline__ int mem_drive(int a, int b, int M) {
+  long long e = (long long)a + (long long)b * M;
+  e = e > kMemEMax ? kMemEMax : (e < -kMemEMax ? -kMemEMax : e);
+  return (int)e;
+}
+
+__device__ __forceinline__ int mem_next(int r, int M, bool up) {
+  return ((r * M + (1 << 14)) >> 15) + (up ? 1024 : -1024);
+}
+
+// -log1p(x) for |x| <= 2^-6: x (-1 + x (1/2 + x (-1/3 + ...)))
+__device__ __forceinline__ double neg_log1p_small(double x) {
+  double s = 1.0 / 10.0;
+  s = fma(x, s, -1.0 / 9.0);
+  s = fma(x, s, 1.0 / 8.0);
+  s = fma(x, s, -1.0 / 7.0);
+  s = fma(x, s, 1.0 / 6.0);
+  s = fma(x, s, -1.0 / 5.0);
+  s = fma(x, s, 1.0 / 4.0);
+  s = fma(x, s, -1.0 / 3.0);
+  s = fma(x, s, 1.0 / 2.0);
+  s = fma(x, s, -1.0);
+  return x * s;
+}
+
+// sum over the workgroup (256 threads) of one FP64 value per thread, in a fixed order
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double s = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return s;
+}
+
+// The sampler: slice t of configurations config0 .. config0 + nconf - 1, one workgroup per
+// configuration, each thread whole words (eight Philox blocks of counter (i / 4, t, c,
+// stream) per word, output i % 4 for site i, as the tilted sampler).  mem[c] holds the
+// configuration's M, 32 per word (t = 0: not read, taken as 0); lr[c] receives the slice's
+// log(p / q).
+__global__ __launch_bounds__(256) void k_z2_memory(uint64_t seed, uint32_t config0, uint32_t t,
+                                                   uint32_t stream, int vol, int a, int b,
+                                                   int r, uint32_t* words, int16_t* mem,
+                                                   double* lr, const uint64_t* par) {
+  __shared__ double red[4];
+  if (par) {
+    seed = par[0];
+    config0 = (uint32_t)par[1];
+  }
+  const int c = blockIdx.x;
+  const int nw = (vol + 31) / 32;
+  double acc = 0.0;
+  for (int w = threadIdx.x; w < nw; w += 256) {
+    uint4* m = reinterpret_cast<uint4*>(mem + ((long)c * nw + w) * 32);
+    uint32_t mp[16];  // M of sites 2j (low half) and 2j + 1 (high half)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = t == 0 ? make_uint4(0u, 0u, 0u, 0u) : m[k];
+      mp[4 * k] = v.x;
+      mp[4 * k + 1] = v.y;
+      mp[4 * k + 2] = v.z;
+      mp[4 * k + 3] = v.w;
+    }
+    const int valid = vol - 32 * w;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int blk = 0; blk < 8; ++blk) {
+      uint32_t ctr[4] = {(uint32_t)(8 * w + blk), t, config0 + (uint32_t)c, stream};
+      philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 4 * blk + q, sh = 16 * (i & 1);
+        const int M = (int16_t)(mp[i >> 1] >> sh);
+        const int e = mem_drive(a, b, M);
+        const bool up = ctr[q] < (uint32_t)((1u << 31) + (uint32_t)e);
+        bits |= (uint32_t)up << i;
+        const double x = (up ? 0x1p-31 : -0x1p-31) * (double)e;
+        if (i < valid) acc += neg_log1p_small(x);
+        const uint32_t nm = (uint32_t)(uint16_t)(int16_t)mem_next(r, M, up);
+        mp[i >> 1] = (mp[i >> 1] & (0xffff0000u >> sh)) | (nm << sh);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      m[k] = make_uint4(mp[4 * k], mp[4 * k + 1], mp[4 * k + 2], mp[4 * k + 3]);
+    words[(long)c * nw + w] = bits;
+  }
+  const double s = block_sum256(acc, red);
+  if (threadIdx.x == 0) lr[c] = s;
+}
+
+hipError_t launch_z2_memory(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                            uint32_t stream, int vol, int a, int b, int r, uint32_t* words,
+                            int16_t* mem, double* lr, hipStream_t s, const uint64_t* par) {
+  if (nconf < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_z2_memory, dim3((unsigned)nconf), dim3(256), 0, s, seed, config0, t,
+                     stream, vol, a, b, r, words, mem, lr, par);
+  return hipGetLastError();
+}
+
+// The same proposal evaluated on the plain sampler's fields (stream 0, word w of slice t =
+// output w % 4 of the block (w / 4, t, c, 0)): out[t][c] = the cumulative log(p / q) of
+// slices 0 .. t - 1 (out[0] = 0).  One workgroup per configuration; thread j owns words j,
+// j + 256, .. (WPT of them) and walks their memory over t in registers (two sites per
+// register).
+constexpr int kMemEvalMaxVol = 32768;
+
+template <int WPT>
+__global__ __launch_bounds__(256) void k_z2_memory_eval(uint64_t seed, uint32_t config0, int T,
+                                                        int vol, int a, int b, int r,
+                                                        double* out, int nconf) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int nw = (vol + 31) / 32;
+  uint32_t mp[WPT][16];
+#pragma unroll
+  for (int k = 0; k < WPT; ++k)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) mp[k][j] = 0u;
+  double run = 0.0;
+  for (int t = 0; t < T; ++t) {
+    if (threadIdx.x == 0) out[(long)t * nconf + c] = run;
+    if (t == T - 1) break;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+      const int w = threadIdx.x + 256 * k;
+      if (w < nw) {
+        uint32_t ctr[4] = {(uint32_t)(w >> 2), (uint32_t)t, config0 + (uint32_t)c, 0u};
+        philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const int q = w & 3;
+        const uint32_t word = q == 0 ? ctr[0] : q == 1 ? ctr[1] : q == 2 ? ctr[2] : ctr[3];
+        const int valid = vol - 32 * w;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+          const int sh = 16 * (i & 1);
+          const int M = (int16_t)(mp[k][i >> 1] >> sh);
+          const int e = mem_drive(a, b, M);
+          const bool up = (word >> i) & 1u;
+          const double x = (up ? 0x1p-31 : -0x1p-31) * (double)e;
+          if (i < valid) acc += neg_log1p_small(x);
+          const uint32_t nm = (uint32_t)(uint16_t)(int16_t)mem_next(r, M, up);
+          mp[k][i >> 1] = (mp[k][i >> 1] & (0xffff0000u >> sh)) | (nm << sh);
+        }
+      }
+    }
+    run += block_sum256(acc, red);
+  }
+}
+
+hipError_t launch_z2_memory_eval(uint64_t seed, uint32_t config0, int nconf, int T, int vol,
+                                 int a, int b, int r, double* out, hipStream_t s) {
+  if (nconf < 1 || T < 1) return hipSuccess;
+  if (vol < 1 || ((vol + 31) / 32) * 32 > kMemEvalMaxVol) return hipErrorInvalidValue;
+  const int nw = (vol + 31) / 32;
+  if (nw <= 256)
+    hipLaunchKernelGGL(k_z2_memory_eval<1>, dim3((unsigned)nconf), dim3(256), 0, s, seed,
+                       config0, T, vol, a, b, r, out, nconf);
+  else if (nw <= 512)
+    hipLaunchKernelGGL(k_z2_memory_eval<2>, dim3((unsigned)nconf), dim3(256), 0, s, seed,
+                       config0, T, vol, a, b, r, out, nconf);
+  else
+    hipLaunchKernelGGL(k_z2_memory_eval<4>, dim3((unsigned)nconf), dim3(256), 0, s, seed,
+                       config0, T, vol, a, b, r, out, nconf);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ batched LU
 // One wavefront per matrix, lane i owns row i, the matrix sits in LDS column-major so that
 // lanes touch consecutive addresses.  The factorisation is right-looking: at step k the

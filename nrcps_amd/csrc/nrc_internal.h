@@ -183,6 +183,19 @@ hipError_t launch_z2_bits(uint64_t seed, uint32_t config0, int nconf, uint32_t t
 hipError_t launch_z2_tilt(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
                           uint32_t stream, int vol, uint32_t thr, uint32_t* words, int* fsum,
                           hipStream_t s, const uint64_t* par = nullptr);
+// Philox stream of the memory sampler's per-site uniforms
+#define NRC_STREAM_MEMORY 3u
+// Z2 fields with memory (integer parameters a, b, r; see lu_rng.hip): the words of slice t,
+// the per-site memory mem (int16, 32 per word, updated in place; not read at t = 0) and
+// lr[c], the slice's log(p / q)
+hipError_t launch_z2_memory(uint64_t seed, uint32_t config0, int nconf, uint32_t t,
+                            uint32_t stream, int vol, int a, int b, int r, uint32_t* words,
+                            int16_t* mem, double* lr, hipStream_t s,
+                            const uint64_t* par = nullptr);
+// The memory proposal's cumulative log(p / q) over the plain sampler's fields: out[t][c] for
+// t < T (vol <= 32768)
+hipError_t launch_z2_memory_eval(uint64_t seed, uint32_t config0, int nconf, int T, int vol,
+                                 int a, int b, int r, double* out, hipStream_t s);
 
 // Options of the batched LU (launch_lu_det), all off by default:
 //   repl, ldr: the last column of every minor (size k) is row k - 1 of repl[c] (a matrix of

@@ -263,6 +263,60 @@ def z2_sums(ctx: Context, seed: int, config0: int, nconf: int, T: int, vol: int,
     return out.download()
 
 
+def memory_log_ratio(ctx: Context, seed: int, config0: int, nconf: int, T: int, vol: int,
+                     lam: float, theta: float, rho: float) -> np.ndarray:
+    """[T][nconf] cumulative log(p / q) that fields with memory (lam, theta, rho) would carry,
+    evaluated on the plain sampler's fields of configurations config0 .. config0 + nconf - 1
+    (nrc_z2_memory_eval)."""
+    out = ctx.array((T, nconf), np.float64)
+    _check(ctx.lib.nrc_z2_memory_eval(ctx.h, seed, config0, nconf, T, vol, float(lam),
+                                      float(theta), float(rho), out.ptr), "z2_memory_eval")
+    return out.download()
+
+
+def _lme(x: np.ndarray) -> float:
+    """log of the mean of exp(x)."""
+    m = x.max()
+    return float(np.log(np.exp(x - m).mean()) + m)
+
+
+def memory_log_ess(la: np.ndarray, lr: np.ndarray, ts) -> float:
+    """Mean over t in ts of the log effective-sample fraction that a proposal q with
+    log(p / q) = lr [T][C] (on a plain sample) gives the estimator of <det(t)^2>:
+    E_p[d2]^2 / E_p[d2^2 p / q] (la [T][C][m] of the same configurations)."""
+    x = la[:, :, 0] if la.ndim == 3 else la
+    return float(np.mean([2 * _lme(2 * x[t]) - _lme(4 * x[t] + lr[t]) for t in ts]))
+
+
+def fit_memory(ctx: Context, la: np.ndarray, seed: int, config0: int, T: int, vol: int, ts,
+               lam0: float, theta0: float = 2.5e-4, rho0: float = 0.88, maxiter: int = 100):
+    """(lam, theta, rho) of fields with memory that maximise memory_log_ess on a plain sample
+    (la of configurations config0 ..; their fields regenerated on the device), Nelder-Mead in
+    scaled coordinates from (lam0, theta0, rho0)."""
+    from scipy.optimize import minimize
+
+    C = la.shape[1]
+    lim = 2.0 ** -6
+
+    def unpack(u):
+        lam = float(np.clip(u[0] * 1e-3, -lim, lim))
+        theta = float(np.clip(u[1] * 1e-4, -lim, lim))
+        rho = float(np.clip(0.96 / (1.0 + np.exp(-u[2])), 0.0, 0.96))
+        return lam, theta, rho
+
+    def f(u):
+        lr = memory_log_ratio(ctx, seed, config0, C, T, vol, *unpack(u))
+        return -memory_log_ess(la, lr, ts)
+
+    r0 = min(max(rho0 / 0.96, 1e-6), 1 - 1e-6)
+    u0 = np.array([lam0 * 1e3, theta0 * 1e4, np.log(r0 / (1 - r0))])
+    res = minimize(f, u0, method="Nelder-Mead",
+                   options={"maxiter": maxiter, "xatol": 1e-2, "fatol": 1e-4,
+                            "initial_simplex": [u0, u0 + [0.2, 0, 0], u0 + [0, 0.5, 0],
+                                                u0 + [0, 0, 0.3]]})
+    return unpack(res.x)
+
+
 def _tilt_log_ess(lam: np.ndarray, d2: np.ndarray, S: np.ndarray, V: int):
     """log of the effective sample fraction of the tilted proposal q ~ p prod_t' exp(lam_t'
     S_t') (per-site probability of +1: (1 + tanh lam_t') / 2) for the estimator of <d2>,

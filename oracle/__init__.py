@@ -263,6 +263,43 @@ def z2_tilt_log_ratio(phis: np.ndarray, q: float) -> np.ndarray:
     return np.concatenate([[0.0], np.cumsum(per)[:-1]])
 
 
+def z2_memory_phi(seed: int, config: int, T: int, vol: int, a: int, b: int, r: int,
+                  stream: int = 3):
+    """Z2 fields with memory, slices 0 .. T-1 of one configuration: every site carries
+    M = 1024 m, m(t) = sum_{k >= 1} rho^(k-1) phi(t - k) in fixed point (M(0) = 0, M(t + 1) =
+    floor((r M + 2^14) / 2^15) + 1024 phi(t)); at slice t site i is +1 where Philox output i
+    (output i % 4 of the block with counter (i / 4, t, config, stream)) is below 2^31 + e,
+    e = clip(a + b M, -2^25, 2^25).  No counterpart in the reference: an importance sampler
+    for lattice.C:81-86's Z2 field, exact like one_body-reweight/main.C:73-76's shifted one.
+    Returns phis [T][vol] (int8) and the cumulative log(p / q) before each slice, [T], from
+    numpy's log1p."""
+    M = np.zeros(vol, dtype=np.int64)
+    phis = np.empty((T, vol), dtype=np.int8)
+    inc = np.zeros(T)
+    for t in range(T):
+        u = z2_bits(seed, config, t, 32 * vol, stream=stream).astype(np.int64)
+        e = np.clip(a + b * M, -(1 << 25), 1 << 25)
+        phi = np.where(u < (1 << 31) + e, 1, -1)
+        inc[t] = -np.log1p(phi * e / 2.0 ** 31).sum()
+        M = ((r * M + (1 << 14)) >> 15) + 1024 * phi
+        phis[t] = phi
+    return phis, np.concatenate([[0.0], np.cumsum(inc)[:-1]])
+
+
+def z2_memory_log_ratio(phis: np.ndarray, a: int, b: int, r: int) -> np.ndarray:
+    """The cumulative log(p / q) of given fields phis [T][vol] under fields with memory (a, b,
+    r), as z2_memory_phi computes it for its own draws."""
+    T, vol = phis.shape
+    M = np.zeros(vol, dtype=np.int64)
+    inc = np.zeros(T)
+    for t in range(T):
+        e = np.clip(a + b * M, -(1 << 25), 1 << 25)
+        phi = phis[t].astype(np.int64)
+        inc[t] = -np.log1p(phi * e / 2.0 ** 31).sum()
+        M = ((r * M + (1 << 14)) >> 15) + 1024 * phi
+    return np.concatenate([[0.0], np.cumsum(inc)[:-1]])
+
+
 def bits_to_phi(words: np.ndarray, vol: int) -> np.ndarray:
     phi = np.zeros(vol, dtype=np.int8)
     lib().orc_bits_to_phi(np.ascontiguousarray(words, np.uint32), vol, phi)
