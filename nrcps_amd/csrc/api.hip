@@ -665,6 +665,8 @@ struct nrc_ensemble {
   int* popp = nullptr;       // pair: opposite-parity sites
   double2* Mp = nullptr;     // pair: [C][N][N] pair matrices
   float* pair_work = nullptr;  // pair, FP32: split-BF16x3 partial sums
+  float* pws = nullptr;      // pair, FP32 partners: sg(w) sqrt|w| of both pw tables
+  bool pws_neg = false;      // some pair weight is negative
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
   uint32_t* words = nullptr; // [C][nw] (Philox) or [T][C][nw] (caller fields)
@@ -742,7 +744,7 @@ static bool ens_in_batch(const nrc_ensemble* e) { return e->stream != e->ctx->st
 static void ens_free(nrc_ensemble* e) {
   void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->pw, e->popp, e->Mp,
                   e->sinks_mx,
-                  e->pair_work, e->sinks_k, e->tab_yz, e->smear,
+                  e->pair_work, e->pws, e->sinks_k, e->tab_yz, e->smear,
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
@@ -1027,6 +1029,16 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
       ENS_TRY(dev_alloc(&e->pair_work, std::max(nrc::pair_bf16x3_partial_floats(N, N, V, (int)C),
                                                 nrc::pair_gh_bf16x3_partial_floats(
                                                     N, V, (int)C, d->L))));
+      if (e->pair_sym_ok) {  // the mirror form's scaled weights, sg(w) sqrt|w|
+        std::vector<float> ws(2 * V);
+        for (long i = 0; i < 2 * V; ++i) {
+          ws[i] = (float)std::copysign(std::sqrt(std::fabs(w[i])), w[i]);
+          if (w[i] < 0.0) e->pws_neg = true;
+        }
+        ENS_TRY(dev_alloc(&e->pws, 2 * V));
+        ENS_TRY(hipMemcpy(e->pws, ws.data(), 2 * V * sizeof(float), hipMemcpyHostToDevice) ==
+                        hipSuccess ? NRC_OK : NRC_ERR_HIP);
+      }
     }
   }
   if (hipMemcpy(e->moms_dev, e->moms.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice) !=
@@ -1317,7 +1329,9 @@ static int ens_measure(nrc_ensemble* e, int t) {
         // even w are conditions of partners; L a multiple of 4 and at least 8)
         const int Lm = (e->d.L % 4 == 0 && e->d.L >= 8) ? e->d.L : 0;
         NRC_HIP(nrc::launch_pair_gh_bf16x3((const float2*)e->psi, w, e->popp, G, H, nv, V,
-                                           (int)C, e->pair_work, s, Lm));
+                                           (int)C, e->pair_work, s, e->pws ? Lm : 0,
+                                           e->pws ? e->pws + (e->folded ? V : 0) : nullptr,
+                                           e->pws_neg));
       } else {
         NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
                                         e->popp, G, nv, nv, V, (int)C, e->pair_work, s));

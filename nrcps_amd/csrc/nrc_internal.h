@@ -275,10 +275,12 @@ size_t pair_bf16x3_partial_floats(int n1, int n2, long vol, int ncfg);
 // G[c][s][t] = sum_i w[i] psi[c][s][i] psi[c][t][opp[i]] and H[c][s][t] = sum_i w[i]
 // psi[c][s][i] conj psi[c][t][i] in one pass (split-BF16x3 on the matrix cores; n <= 48).
 // L > 0: opp is the mirror p -> -p of a cubic lattice of side L (a multiple of 4, >= 8) and w
-// is even; each site and its mirror are then loaded together (opp unused)
+// is even; each site and its mirror are then loaded together (opp and wf unused) and ws[V]
+// holds sg(w) sqrt|w| in FP32 (ws_neg: some w is negative)
 hipError_t launch_pair_gh_bf16x3(const float2* psi, const double* wf, const int* opp,
                                  double2* G, double2* H, int n, long vol, int ncfg,
-                                 float* partial, hipStream_t s, int L = 0);
+                                 float* partial, hipStream_t s, int L = 0,
+                                 const float* ws = nullptr, bool ws_neg = false);
 size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg, int L = 0);
 // the N x N pair matrices of configurations whose orbitals are stored as nvo volumes with
 // conjugate partners, from G[c][a][b] = sum w psi_a(p) psi_b(-p) and H[c][a][b] =

@@ -567,19 +567,35 @@ __global__ __launch_bounds__(256) void k_pair_gh_bf16x3(const float2* __restrict
 }
 
 // k_pair_gh_bf16x3 for opp(p) = -p on a cubic lattice (the conjugate partners' case), with
-// every orbital value read from HBM once: a site p and its mirror -p are loaded together and
-// serve both sites' terms.  The workgroups take units of the lattice: the plane pair (x, L -
-// x) for x = 1 .. L/2 - 1 -- the sites p of plane x and their mirrors in plane L - x, each
-// loaded pair giving the terms of p (A = psi(p), G's B = w psi(-p), H's B = w conj psi(p)) and
-// of -p (A = psi(-p), G's B = w psi(p), H's B = w conj psi(-p)) -- and the planes 0 and L/2,
-// mirrored onto themselves, whose sites each give their own terms (the mirror read hits the
-// cache: the same workgroup reads the plane).  w is even (w(-p) = w(p)), as conjugate
-// partners require.  A lane's four consecutive sites (y, z .. z + 3), z a multiple of 4, have
-// the mirrors (L - y, L - z .. L - z - 3): one reversed run, or for z = 0 the site (L - y, 0)
-// and a run of three.  Partial tiles per unit in k_pair_gh_bf16x3's layout.
-template <int MT>
-__global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict__ psi,
-                                                        const double* __restrict__ wf,
+// every orbital value read from HBM once and split once.  The workgroups take units of the
+// lattice: the plane pair (x, L - x) for x = 1 .. L/2 - 1 -- the sites p of plane x and their
+// mirrors in plane L - x, each loaded pair giving the terms of p and of -p -- and the planes 0
+// and L/2, mirrored onto themselves, whose sites each give their own terms (the mirror read
+// hits the cache: the same workgroup reads the plane).  A lane's four consecutive sites (y,
+// z .. z + 3), z a multiple of 4, have the mirrors (L - y, L - z .. L - z - 3): one reversed
+// run, or for z = 0 the site (L - y, 0) and a run of three.
+//
+// w is even (w(-p) = w(p)), so with u(p) = sqrt|w(p)| psi(p) and sg(p) the sign of w(p)
+//   G[s][t] = sum_p sg(p) u_s(p) u_t(-p),   H[s][t] = sum_p sg(p) u_s(p) conj u_t(p):
+// the scaled values of every stored volume, split once into bf16 (hi, lo), are both the A and
+// the B operands.  In the interleaved (re, im) layout, with A = (a_r, a_i) and B = (b_r, b_i):
+//   Re a b      = A . (b_r, -b_i)        Im a b      = swap(A) . B
+//   Re a conj b = A . B                  Im a conj b = swap(A) . (b_r, -b_i)
+// (swap exchanges the halves of every dword; the split of -x is minus the split of x), so a
+// step's operands take one dword rotate per A dword and one sign xor per B dword, the sign of
+// w riding on the B side (ws holds sg(p) sqrt|w(p)| in FP32).
+//
+// G is symmetric and H Hermitian.  So over a plane pair the terms of the mirrored sites add
+// the transpose of the sites' own G terms, and only H's tiles on and above the diagonal are
+// formed: the sites of plane x give G (every tile) and H (upper tiles), their mirrors H (upper
+// tiles) -- 10 tile products a step at MT = 2 instead of 16.  k_pair_gh_reduce assembles G
+// (adding the transpose for plane pairs) and H (conjugating the upper tiles into the lower).
+// Partial tiles per unit in k_pair_gh_bf16x3's layout; H's tiles below the diagonal are not
+// written.
+template <int MT, bool NEG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MT == 2 ? 3 : 1)))
+void k_pair_gh_mirror(const float2* __restrict__ psi,
+                                                        const float* __restrict__ ws,
                                                         float* __restrict__ partial, int n,
                                                         int L) {
   using u32x4 = __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t;
@@ -601,12 +617,14 @@ __global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict
 #pragma unroll
       for (int b = 0; b < MT; ++b) acc[m][a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const float2* base = psi + c * n * V;
-  for (int i0 = wave * 16; i0 < P; i0 += OV_WAVES * 16) {
+  // a lane's values of step i0: its four sites (va), their mirrors (vm) and ws there (loading
+  // the next step's values ahead of the products needs registers that cost the kernel its
+  // third wave per SIMD, and was slower)
+  auto load = [&](int i0, float (&va)[MT][8], float (&vm)[MT][8], float4& wv) {
     const int i = i0 + 4 * grp;  // this lane's four sites of the plane
     const int y = i / L, z = i - y * L;
     const long p0 = (long)x * P + i;
     const long mrow = (long)xm * P + (long)((L - y) % L) * L;
-    float va[MT][8], vm[MT][8];
 #pragma unroll
     for (int a = 0; a < MT; ++a) {
       const int s = a * 16 + row;
@@ -640,49 +658,105 @@ __global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict
         for (int q2 = 0; q2 < 8; ++q2) va[a][q2] = vm[a][q2] = 0.f;
       }
     }
-    const double2* wp = reinterpret_cast<const double2*>(wf + p0);
-    const double2 w01 = wp[0], w23 = wp[1];
-    const float wt[4] = {(float)w01.x, (float)w01.y, (float)w23.x, (float)w23.y};
+    wv = *reinterpret_cast<const float4*>(ws + p0);
+  };
+  for (int i0 = wave * 16; i0 < P; i0 += OV_WAVES * 16) {
+    float va[MT][8], vm[MT][8];
+    float4 wv;
+    load(i0, va, vm, wv);
+    const float wa[4] = {fabsf(wv.x), fabsf(wv.y), fabsf(wv.z), fabsf(wv.w)};
+    // per site dword: m1 = the sign of w on both halves, m2 = m1 with the imaginary sign flipped
+    u32x4 m1 = {0u, 0u, 0u, 0u}, m2 = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+    if constexpr (NEG) {
+      const float wq[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+      for (int q2 = 0; q2 < 4; ++q2) {
+        const uint32_t sb = __float_as_uint(wq[q2]) & 0x80000000u;
+        m1[q2] = sb | (sb >> 16);
+        m2[q2] = m1[q2] ^ 0x80000000u;
+      }
+    }
+    // scaled values of the sites (p) and of their mirrors (m), split into hi and lo
+    u32x4 ph[MT], pl[MT], mh[MT], ml[MT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+      float t[8];
+      bf16x8 h, l;
+#pragma unroll
+      for (int q2 = 0; q2 < 8; ++q2) t[q2] = va[a][q2] * wa[q2 >> 1];
+      split8(t, h, l);
+      ph[a] = __builtin_bit_cast(u32x4, h);
+      pl[a] = __builtin_bit_cast(u32x4, l);
+#pragma unroll
+      for (int q2 = 0; q2 < 8; ++q2) t[q2] = vm[a][q2] * wa[q2 >> 1];
+      split8(t, h, l);
+      mh[a] = __builtin_bit_cast(u32x4, h);
+      ml[a] = __builtin_bit_cast(u32x4, l);
+    }
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
       if (part == 1 && !pair) break;
-      bf16x8 ah[MT], al[MT];
+      // part 0, the terms of the sites p: A = u(p), G's B = u(-p), H's B = u(p);
+      // part 1, the mirrored sites' H terms: A = B = u(-p) (their G terms are the transpose)
+      const u32x4* Ah = part ? mh : ph;
+      const u32x4* Al = part ? ml : pl;
+      bf16x8 a_h[MT], a_l[MT], s_h[MT], s_l[MT];  // A and swap(A)
 #pragma unroll
-      for (int a = 0; a < MT; ++a) split8(part ? vm[a] : va[a], ah[a], al[a]);
+      for (int a = 0; a < MT; ++a) {
+        u32x4 xh, xl;
 #pragma unroll
-      for (int b = 0; b < MT; ++b)
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {  // G: the mirrored values; H: the same values conjugated
-          const float* src = (m == 0) == (part == 0) ? vm[b] : va[b];
-          const float sg = m == 0 ? 1.f : -1.f;
-          float vr[8];
-#pragma unroll
-          for (int q2 = 0; q2 < 4; ++q2) {
-            vr[2 * q2] = wt[q2] * src[2 * q2];
-            vr[2 * q2 + 1] = -(wt[q2] * (sg * src[2 * q2 + 1]));
-          }
-          bf16x8 rh, rl;
-          split8(vr, rh, rl);
-          const u32x4 uh = __builtin_bit_cast(u32x4, rh), ul = __builtin_bit_cast(u32x4, rl);
-          u32x4 vh, vl;
-#pragma unroll
-          for (int q2 = 0; q2 < 4; ++q2) {
-            vh[q2] = ((uh[q2] >> 16) | (uh[q2] << 16)) ^ 0x00008000u;
-            vl[q2] = ((ul[q2] >> 16) | (ul[q2] << 16)) ^ 0x00008000u;
-          }
-          const bf16x8 ih = __builtin_bit_cast(bf16x8, vh), il = __builtin_bit_cast(bf16x8, vl);
-#pragma unroll
-          for (int term = 0; term < 3; ++term)
-#pragma unroll
-            for (int a = 0; a < MT; ++a) {
-              const bf16x8 av = term == 2 ? al[a] : ah[a];
-              const bf16x8 br = term == 1 ? rl : rh, bi = term == 1 ? il : ih;
-              acc[2 * m][a][b] =
-                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, br, acc[2 * m][a][b], 0, 0, 0);
-              acc[2 * m + 1][a][b] =
-                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bi, acc[2 * m + 1][a][b], 0, 0, 0);
-            }
+        for (int q2 = 0; q2 < 4; ++q2) {
+          xh[q2] = (Ah[a][q2] >> 16) | (Ah[a][q2] << 16);
+          xl[q2] = (Al[a][q2] >> 16) | (Al[a][q2] << 16);
         }
+        a_h[a] = __builtin_bit_cast(bf16x8, Ah[a]);
+        a_l[a] = __builtin_bit_cast(bf16x8, Al[a]);
+        s_h[a] = __builtin_bit_cast(bf16x8, xh);
+        s_l[a] = __builtin_bit_cast(bf16x8, xl);
+      }
+#pragma unroll
+      for (int b = 0; b < MT; ++b) {
+        // G: Re = A . (B ^ m2), Im = swap(A) . (B ^ m1), B = u(-p);
+        // H: Re = A . (B ^ m1), Im = swap(A) . (B ^ m2), B = A's values
+        bf16x8 gr[2], gi[2], hr[2], hi[2];
+#pragma unroll
+        for (int lo = 0; lo < 2; ++lo) {
+          const u32x4 gb = lo ? ml[b] : mh[b], hb = lo ? Al[b] : Ah[b];
+          u32x4 t0, t1, t2, t3;
+#pragma unroll
+          for (int q2 = 0; q2 < 4; ++q2) {
+            t0[q2] = gb[q2] ^ m2[q2];
+            t1[q2] = NEG ? gb[q2] ^ m1[q2] : gb[q2];
+            t2[q2] = NEG ? hb[q2] ^ m1[q2] : hb[q2];
+            t3[q2] = hb[q2] ^ m2[q2];
+          }
+          gr[lo] = __builtin_bit_cast(bf16x8, t0);
+          gi[lo] = __builtin_bit_cast(bf16x8, t1);
+          hr[lo] = __builtin_bit_cast(bf16x8, t2);
+          hi[lo] = __builtin_bit_cast(bf16x8, t3);
+        }
+#pragma unroll
+        for (int term = 0; term < 3; ++term) {  // (hi, hi), (hi, lo), (lo, hi)
+          const int tb = term == 1 ? 1 : 0;
+#pragma unroll
+          for (int a = 0; a < MT; ++a) {
+            const bf16x8 av = term == 2 ? a_l[a] : a_h[a];
+            const bf16x8 sv = term == 2 ? s_l[a] : s_h[a];
+            if (part == 0) {
+              acc[0][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, gr[tb], acc[0][a][b], 0, 0, 0);
+              acc[1][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(sv, gi[tb], acc[1][a][b], 0, 0, 0);
+            }
+            if (a <= b) {
+              acc[2][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, hr[tb], acc[2][a][b], 0, 0, 0);
+              acc[3][a][b] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(sv, hi[tb], acc[3][a][b], 0, 0, 0);
+            }
+          }
+        }
+      }
     }
   }
   constexpr int ldp = MT * 32;
@@ -695,6 +769,7 @@ __global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict
       for (int b = 0; b < MT; ++b)
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
+          if (m == 1 && a > b) continue;  // H below the diagonal: the reduce conjugates
           const f32x4 v = acc[2 * m + part][a][b];
           if (wave > 0) {
 #pragma unroll
@@ -713,6 +788,38 @@ __global__ __launch_bounds__(256) void k_pair_gh_mirror(const float2* __restrict
           }
           __syncthreads();
         }
+}
+
+// G and H of the mirror form from its unit partials (FP64 sums): G[s][t] over every unit, plus
+// G[t][s] over the plane pairs (units below npair); H[s][t] from the tiles on and above the
+// diagonal, conj H[t][s] below it
+__global__ void k_pair_gh_reduce(const float* partial, double2* G, double2* H, int n, int nunits,
+                                 int npair, int mt, long ncfg) {
+  const long total = ncfg * n * n;
+  const int ldp = mt * 32;
+  const long tile = (long)(mt * 16) * ldp;
+  const long half = ncfg * nunits * tile;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / ((long)n * n);
+    const int r = (int)(i - c * n * n), s = r / n, t = r - (r / n) * n;
+    double gr = 0.0, gi = 0.0, hr = 0.0, hi = 0.0;
+    const bool up = s / 16 <= t / 16;
+    const int hs = up ? s : t, ht = up ? t : s;
+    for (int u = 0; u < nunits; ++u) {
+      const float* p = partial + (c * nunits + u) * tile;
+      gr += (double)p[(long)s * ldp + 2 * t];
+      gi += (double)p[(long)s * ldp + 2 * t + 1];
+      if (u < npair) {
+        gr += (double)p[(long)t * ldp + 2 * s];
+        gi += (double)p[(long)t * ldp + 2 * s + 1];
+      }
+      hr += (double)p[half + (long)hs * ldp + 2 * ht];
+      hi += (double)p[half + (long)hs * ldp + 2 * ht + 1];
+    }
+    G[i] = make_double2(gr, gi);
+    H[i] = make_double2(hr, up ? hi : -hi);
+  }
 }
 
 // M[c][s][t] = sum over chunks (FP64): re from column 2t, im from column 2t + 1
@@ -1154,19 +1261,28 @@ size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg, int L) {
 
 hipError_t launch_pair_gh_bf16x3(const float2* psi, const double* wf, const int* opp,
                                  double2* G, double2* H, int n, long vol, int ncfg,
-                                 float* partial, hipStream_t s, int L) {
+                                 float* partial, hipStream_t s, int L, const float* ws,
+                                 bool ws_neg) {
   if (n < 1 || n > 48 || (vol & 3)) return hipErrorInvalidValue;
-  if (L > 0 && ((long)L * L * L != vol || L % 4 || L < 8)) return hipErrorInvalidValue;
+  if (L > 0 && ((long)L * L * L != vol || L % 4 || L < 8 || ws == nullptr))
+    return hipErrorInvalidValue;
   const int mt = (n + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const int nchunks = L > 0 ? pair_gh_units(L) : (int)((K + kc - 1) / kc);
   const dim3 grid(nchunks, ncfg), block(64 * OV_WAVES);
-  if (L > 0 && mt == 1)
-    hipLaunchKernelGGL(k_pair_gh_mirror<1>, grid, block, 0, s, psi, wf, partial, n, L);
-  else if (L > 0 && mt == 2)
-    hipLaunchKernelGGL(k_pair_gh_mirror<2>, grid, block, 0, s, psi, wf, partial, n, L);
-  else if (L > 0)
-    hipLaunchKernelGGL(k_pair_gh_mirror<3>, grid, block, 0, s, psi, wf, partial, n, L);
+#define NRC_PGM(A)                                                                            \
+  if (ws_neg)                                                                                 \
+    hipLaunchKernelGGL((k_pair_gh_mirror<A, true>), grid, block, 0, s, psi, ws, partial, n, L); \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_pair_gh_mirror<A, false>), grid, block, 0, s, psi, ws, partial, n, L);
+  if (L > 0 && mt == 1) {
+    NRC_PGM(1)
+  } else if (L > 0 && mt == 2) {
+    NRC_PGM(2)
+  } else if (L > 0) {
+    NRC_PGM(3)
+  }
+#undef NRC_PGM
   else if (mt == 1)
     hipLaunchKernelGGL(k_pair_gh_bf16x3<1>, grid, block, 0, s, psi, wf, opp, partial, n, vol, kc);
   else if (mt == 2)
@@ -1179,6 +1295,11 @@ hipError_t launch_pair_gh_bf16x3(const float2* psi, const double* wf, const int*
   long blocks = (nout + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
+  if (L > 0) {
+    hipLaunchKernelGGL(k_pair_gh_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, G, H,
+                       n, nchunks, L / 2 - 1, mt, (long)ncfg);
+    return hipGetLastError();
+  }
   const size_t half = (size_t)ncfg * nchunks * (mt * 16) * (mt * 32);
   hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, G, n, n,
                      nchunks, mt, 2 * mt, (long)ncfg);

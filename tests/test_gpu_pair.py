@@ -110,22 +110,31 @@ def test_ensemble_pair_matches_oracle(ctx, orc, prec):
     ens.close()
 
 
-@pytest.mark.parametrize("prec,N,L", [(1, 7, 8), (1, 9, 8), (0, 7, 8), (0, 35, 16)])
-def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N, L):
+@pytest.mark.parametrize("prec,N,L,signed", [(1, 7, 8, False), (1, 9, 8, False),
+                                             (0, 7, 8, False), (0, 35, 16, False),
+                                             (0, 9, 8, True), (0, 35, 16, True)])
+def test_ensemble_pair_conj_partners_match_oracle(ctx, orc, prec, N, L, signed):
     """Pair sinks with conjugate partners: the pair matrices come from two products over the
     stored volumes (mirrored sites, and the same sites conjugated; in FP32 one pass computes
     both); N = 9 leaves two orbitals of a partial shell without partner, N = 35 (19 volumes)
-    is the pair-sink benchmark's orbital set.  Against the oracle's run, which evolves every
+    is the pair-sink benchmark's orbital set.  `signed` multiplies the wave function by
+    cos(2 pi n_x / L), even under p -> -p and negative on half the lattice, so the FP32 pass
+    carries the sign of w on its products.  Against the oracle's run, which evolves every
     orbital: FP64 1e-9, FP32 1e-4 in log|det| (5e-4 for the 35 x 35 minors, whose pivots
     amplify the ~2^-17 relative error of the split products; the every-orbital FP32 run is
     checked to the same bound beside it)."""
     import nrcps_amd as na
 
     T, C, seed = 6, 3, 29
-    print(f"[gpu] ensemble pair sinks, conjugate partners, prec={prec} N={N}", flush=True)
+    print(f"[gpu] ensemble pair sinks, conjugate partners, prec={prec} N={N} L={L} "
+          f"signed={signed}", flush=True)
     moms = orc.lowest_momenta(L, N)
     kin = orc.kinetic_xi(L, orc.DISP_STANDARD, 1 / 0.05, 0.5)
     w = orc.two_body_wavefunc(L, orc.WAVEFUNC_TGND, 0.176776695, cutoff_sq=np.pi ** 2)
+    if signed:
+        nx = np.arange(L ** 3) // (L * L)
+        w = w * np.cos(2 * np.pi * nx / L)
+        assert (w < 0).any()
     opp = orc.opposite_parity_index(L)
     ens = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=na.SPLIT_KSPACE,
                       pair_w=w, pair_opp=opp)
