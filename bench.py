@@ -263,13 +263,15 @@ def batched_uneven(ctx, na, nconf=4096, T=96, tau=0.05, A=0.4, seed=7012, runs=2
 
 def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, n_plain=65536,
                      n_learned=16384, steps=300, batch=64, lr=4e-3, seed=7012, optimizer=0,
-                     out_scale=0.0, conj=True):
+                     out_scale=0.0, conj=True, n_fit=4096, n_memory=65536):
     """Effective samples per second of the learned proposal against the uniform sampler.
     The uniform sampler generates n_train configurations; the network is trained on them
     (weighted maximum likelihood, det^2 weights); then n_plain uniform and n_learned proposed
-    configurations estimate C(t) = <det(t)^2>.  Per t in {T/4, T/2, 3T/4}: the relative
-    standard error of each estimate, the per-sample variance v = n rel_se^2 and the cost per
-    sample tau give the effective samples per second 1 / (v tau) at equal error."""
+    configurations estimate C(t) = <det(t)^2> (n_memory with fields with memory, whose three
+    parameters are fitted on the first n_fit training configurations).  Per t in
+    {T/4, T/2, 3T/4}: the relative standard error of each estimate, the per-sample variance
+    v = n rel_se^2 and the cost per sample tau give the effective samples per second
+    1 / (v tau) at equal error."""
     from nrcps_amd import proposal as pr
 
     moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
@@ -338,14 +340,15 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     # regenerated on the device, the proposal's exact ratio evaluated on them), then drawn on
     # the uniform sampler's engine and reweighted exactly
     t0 = time.perf_counter()
-    mem = pr.fit_memory(ctx, la_train, seed, 0, T, V, ts, lam0=float(np.tanh(lam_fit)))
+    mem = pr.fit_memory(ctx, la_train, seed, 0, T, V, ts, lam0=float(np.tanh(lam_fit)),
+                        nfit=n_fit)
     t_mem_fit = time.perf_counter() - t0
     mem = ens.set_memory(*mem)
     ens.run(seed + 4, 1 << 23, 2)  # warm-up
     ctx.sync()
     t0 = time.perf_counter()
     mparts = []
-    for k in range(0, n_learned, chunk):
+    for k in range(0, n_memory, chunk):
         la_k = ens.run(seed + 4, (1 << 27) + k, T)[0]
         mparts.append((la_k, ens.log_ratio(T)))
     t_mem = time.perf_counter() - t0
@@ -360,23 +363,38 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     tau_p, tau_l = t_plain / n_plain, t_learned / n_learned
     tau_t = t_tilt / n_learned
     tau_g = t_grouped / n_learned
-    tau_m = t_mem / n_learned
-    tau_m_amort = (t_mem + t_mem_fit + t_train_gen) / n_learned
+    tau_m = t_mem / n_memory
+    tau_m_amort = (t_mem + t_mem_fit + t_train_gen) / n_memory
+    # the uniform training sample is itself a plain sample of C(t): reused, the run's estimate
+    # is the inverse-variance combination of it and the memory sample, bought with the
+    # training sample, the fit and the memory sample together
+    st_train = pr.estimator_stats(la_train, None, ts)
+    t_all = t_train_gen + t_mem_fit + t_mem
     memory = {}
     for t in ts:
         vp = sp[t]["rel_se"] ** 2 * n_plain
-        vm = sm[t]["rel_se"] ** 2 * n_learned
+        vm = sm[t]["rel_se"] ** 2 * n_memory
+        vtr = st_train[t]["rel_se"] ** 2 * n_train
         sig = float(np.hypot(sp[t]["rel_se"], sm[t]["rel_se"]))
+        info = n_train / vtr + n_memory / vm  # inverse variance of the combined estimate
         memory[str(t)] = {"log_C": sm[t]["log_C"], "rel_se": sm[t]["rel_se"],
                           "bias_within_2sigma": bool(abs(sm[t]["log_C"] - sp[t]["log_C"])
                                                      <= 2 * sig),
                           "variance_gain": vp / vm, "gain": (vp * tau_p) / (vm * tau_m),
-                          "gain_with_fit_amortised": (vp * tau_p) / (vm * tau_m_amort)}
+                          "gain_with_fit_amortised": (vp * tau_p) / (vm * tau_m_amort),
+                          "gain_with_fit_amortised_training_sample_reused":
+                              info * vp * tau_p / t_all}
     # configurations of the memory sampler after which its gain with the one-time cost (the
     # uniform training sample and the fit) amortised reaches 3x: g n tau / (n tau + F) = 3
     g_min = min(v["gain"] for v in memory.values())
     mem_3x = (3.0 * (t_mem_fit + t_train_gen) / (tau_m * (g_min - 3.0))
               if g_min > 3.0 else None)
+    # the same with the training sample reused: (n_train + n g_v) tau_p = 3 (F + n tau_m) at
+    # the smallest variance gain g_v over t
+    gv_min = min(v["variance_gain"] for v in memory.values())
+    den = gv_min * tau_p - 3.0 * tau_m
+    mem_3x_reused = (max(0.0, (3.0 * (t_train_gen + t_mem_fit) - n_train * tau_p) / den)
+                     if den > 0 else None)
     grouped = {}
     for t in ts:
         vp = sp[t]["rel_se"] ** 2 * n_plain
@@ -418,18 +436,26 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
     ens.close()
     w = np.exp(logr[ts[-1]] - logr[ts[-1]].max())  # the ratios r = prod 0.5 / q alone
     return {"workload": f"L={L}, N={N}, T={T}, tau={tau}, A={A}; {n_train} uniform "
-                        f"configurations for training, {n_plain} uniform and {n_learned} "
-                        f"proposed configurations for C(t)",
+                        f"configurations for training, {n_plain} uniform, {n_memory} with "
+                        f"memory and {n_learned} from each other proposal for C(t)",
             "metric": "effective samples/s at equal statistical error of C(t)",
             "conj_partners": bool(conj), "volumes_per_config": ens_nvo,
-            "gain_min_over_t": min(v["gain"] for v in per_t.values()),
-            "cost_ratio_proposed_to_uniform": tau_l / tau_p,
+            "best_learned_proposal": {
+                "proposal": "memory_fields",
+                "gain_min_over_t": min(v["gain"] for v in memory.values()),
+                "gain_with_fit_amortised_training_sample_reused_min_over_t": min(
+                    v["gain_with_fit_amortised_training_sample_reused"]
+                    for v in memory.values()),
+                "bias_within_2sigma_every_t": all(v["bias_within_2sigma"]
+                                                  for v in memory.values())},
+            "network_gain_min_over_t": min(v["gain"] for v in per_t.values()),
+            "network_cost_ratio_to_uniform": tau_l / tau_p,
             "ceiling_note": "variance gain of the best non-adaptive per-site tilt (constant, or "
                             "one per slice) measured on the uniform sample: the gain in "
                             "effective samples/s such a proposal reaches at the cost of a "
                             "uniform configuration; the learned proposal's gain is its "
                             "variance gain over its cost ratio",
-            "per_t": per_t,
+            "network_per_t": per_t,
             "network_grouped": {
                 "what": "the trained network drawing four slices' fields in a row (each on the "
                         "previous field and the log-weight of the last row measured before "
@@ -448,7 +474,13 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
                 "gain_min_over_t": min(v["gain"] for v in memory.values()),
                 "gain_with_fit_amortised_min_over_t": min(v["gain_with_fit_amortised"]
                                                           for v in memory.values()),
+                "gain_with_fit_amortised_training_sample_reused_min_over_t": min(
+                    v["gain_with_fit_amortised_training_sample_reused"]
+                    for v in memory.values()),
                 "configs_for_3x_with_fit_amortised": mem_3x,
+                "configs_for_3x_with_fit_amortised_training_sample_reused": mem_3x_reused,
+                "fit_configs": min(n_fit, n_train) if n_fit > 0 else n_train,
+                "configs": n_memory,
                 "per_t": memory, "seconds": t_mem, "fit_seconds": t_mem_fit},
             "fitted_tilt": {
                 "what": "the constant per-site tilt fitted to the training sample's field sums "
@@ -763,7 +795,8 @@ def main():
         print(f"learned proposal, 4-slice groups: gain "
               f"{learned['network_grouped']['gain_min_over_t']:.2f}x effective samples/s",
               file=sys.stderr, flush=True)
-        print(f"learned proposal: gain {learned['gain_min_over_t']:.2f}x effective samples/s",
+        print(f"learned proposal (network): gain {learned['network_gain_min_over_t']:.2f}x "
+              "effective samples/s",
               file=sys.stderr, flush=True)
 
     pair = None

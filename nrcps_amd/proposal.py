@@ -289,13 +289,18 @@ def memory_log_ess(la: np.ndarray, lr: np.ndarray, ts) -> float:
 
 
 def fit_memory(ctx: Context, la: np.ndarray, seed: int, config0: int, T: int, vol: int, ts,
-               lam0: float, theta0: float = 2.5e-4, rho0: float = 0.88, maxiter: int = 100):
+               lam0: float, theta0: float = 2.5e-4, rho0: float = 0.88, maxiter: int = 100,
+               nfit: int = 0):
     """(lam, theta, rho) of fields with memory that maximise memory_log_ess on a plain sample
     (la of configurations config0 ..; their fields regenerated on the device), Nelder-Mead in
-    scaled coordinates from (lam0, theta0, rho0)."""
+    scaled coordinates from (lam0, theta0, rho0).  nfit > 0 fits on the first nfit
+    configurations only; the ratios are evaluated up to the last slice in ts."""
     from scipy.optimize import minimize
 
+    if nfit > 0:
+        la = la[:, :nfit]
     C = la.shape[1]
+    T = min(T, max(ts) + 1)
     lim = 2.0 ** -6
 
     def unpack(u):
