@@ -325,6 +325,18 @@ __device__ double ref_abs(double x, double y) {
 
 __device__ __forceinline__ double cabs2(double2 a) { return ref_abs(a.x, a.y); }
 
+// the maximum over the active lanes of a wavefront (the ROCm device library's DPP reduction)
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_max_f64(double);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_max_i32(int);
+
+// lane l's value of x (l uniform)
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const unsigned long long b = __double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // (1 + 0i) / z by Smith's method
 __device__ __forceinline__ double2 crecip(double2 z) {
   const double a = 1.0, b = 0.0, c = z.x, d = z.y;
@@ -360,19 +372,67 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   const int sz = minors ? m + 1 : n;
   const int i = threadIdx.x;
   const double2* Mc = M + cfg * (long)n * ldm;
-  if (i < sz) {
-    for (int j = 0; j < sz; ++j) a[j * sz + i] = Mc[(long)i * ldm + j];
-    if (opt.repl) a[(sz - 1) * sz + i] = opt.repl[cfg * (long)n * opt.ldr + (long)(sz - 1) * opt.ldr + i];
+  {  // the minor, coalesced: element e = r sz + c, four loads in flight per lane
+    const int tot = sz * sz;
+    const bool rp = opt.repl != nullptr;
+    for (int e0 = 0; e0 < tot; e0 += 256) {
+      double2 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + 64 * q + i;
+        if (e < tot) {
+          const int r = e / sz, c = e - r * sz;
+          if (!(rp && c == sz - 1)) v[q] = Mc[(long)r * ldm + c];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + 64 * q + i;
+        if (e < tot) {
+          const int r = e / sz, c = e - r * sz;
+          if (!(rp && c == sz - 1)) a[c * sz + r] = v[q];
+        }
+      }
+    }
+    if (rp && i < sz)
+      a[(sz - 1) * sz + i] = opt.repl[cfg * (long)n * opt.ldr + (long)(sz - 1) * opt.ldr + i];
   }
   __syncthreads();
+  // Row scale vv_i = 1 / max_j |a_ij| with the exact |.| (ref_abs).  The maximum is found
+  // among candidates picked by the squared magnitudes, computed to within 2^-51 relative: an
+  // entry whose square is below (1 - 2^-40) of the row's largest square has |z|, and so
+  // ref_abs(z) (within an ulp of |z|), below the largest entry's ref_abs.  A row with one
+  // candidate takes a single ref_abs; a row with several, or whose largest square lies outside
+  // [2^-900, 2^900] (underflow, overflow, infinities, an all-zero row), takes every entry's.
   double vv = 0.0;
   int bad = 0;
+  bool slow = false;
+  double big = 0.0;
   if (i < sz) {
-    double big = 0.0;
+    double b2 = 0.0, s2 = 0.0;
+    int jb = 0;
+    for (int j = 0; j < sz; ++j) {
+      const double2 z = a[j * sz + i];
+      const double m2 = z.x * z.x + z.y * z.y;
+      if (m2 > b2) {
+        s2 = b2;
+        b2 = m2;
+        jb = j;
+      } else if (m2 > s2) {
+        s2 = m2;
+      }
+    }
+    if (!(b2 >= 0x1p-900 && b2 <= 0x1p+900) || s2 >= b2 * (1.0 - 0x1p-40)) slow = true;
+    else big = cabs2(a[jb * sz + i]);
+  }
+  if (__any(slow) && slow) {
+    big = 0.0;
     for (int j = 0; j < sz; ++j) {
       const double t = cabs2(a[j * sz + i]);
       if (t > big) big = t;
     }
+  }
+  if (i < sz) {
     if (big == 0.0) bad = 1;
     else vv = 1.0 / big;
   }
@@ -384,67 +444,74 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   // the elimination, one pivot per lane, and summed across the wavefront
   double2 my_piv = make_double2(1.0, 0.0);
   double my_mag = 1.0;
+  // Rows stay in their lanes; pos is the row's position in the reference's row order (a row
+  // swap exchanges two positions), and the rows already pivoted take no further part.
+  int pos = i;
+  bool done = i >= sz;
   if (!bad) {
     for (int k = 0; k < sz; ++k) {
-      double bv = (i >= k && i < sz) ? vv * cabs2(a[k * sz + i]) : -1.0;
-      int bi = i;
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi > bi)) { bv = ov; bi = oi; }
+      // the pivot: the largest vv |a_ik| over the rows at positions >= k, the last position
+      // among equals (the reference's >=): the wavefront maximum, then among the lanes holding
+      // it the highest position
+      const double2 z = done ? make_double2(0.0, 0.0) : a[k * sz + i];
+      const double h = done ? 0.0 : cabs2(z);
+      const double bv = done ? -1.0 : vv * h;
+      const double mx = __ockl_wfred_max_f64(bv);
+      const bool hit = !done && bv == mx;
+      const unsigned long long tie = __ballot(hit);
+      int p = tie ? __ffsll((long long)tie) - 1 : 0;
+      if (__popcll(tie) > 1) {
+        const int best = __ockl_wfred_max_i32(hit ? pos : -1);
+        p = __ffsll((long long)__ballot(hit && pos == best)) - 1;
       }
-      const int imax = bi;
+      const int imax = __builtin_amdgcn_readlane(pos, p);
       if (i == k) my_indx = imax;
       if (imax != k) {
-        if (i < sz) {
-          const double2 t = a[i * sz + k];
-          a[i * sz + k] = a[i * sz + imax];
-          a[i * sz + imax] = t;
-        }
-        const double vk = __shfl(vv, k);
-        if (i == imax) vv = vk;
+        if (pos == k) pos = imax;
         dsign = -dsign;
       }
-      __syncthreads();
-      double2 piv = a[k * sz + k];
-      double mag = cabs2(piv);
+      if (i == p) {
+        pos = k;
+        done = true;
+      }
+      double2 piv = make_double2(readlane_d(z.x, p), readlane_d(z.y, p));
+      double mag = readlane_d(h, p);
       if (mag == 0.0) {
         piv = make_double2(1.0e-20, 0.0);
         mag = 1.0e-20;
-        if (i == 0) a[k * sz + k] = piv;
+        if (i == p) a[k * sz + p] = piv;
       }
       if (i == k) {
         my_piv = piv;
         my_mag = mag;
       }
       const double2 r = crecip(piv);
-      if (i > k && i < sz) {
-        const double2 l = cmuld(a[k * sz + i], r);
+      if (!done) {
+        const double2 l = cmuld(z, r);
         a[k * sz + i] = l;
-        // eight columns at a time, their LDS reads issued together (row k is a broadcast
-        // read, row i this lane's own), then the remaining columns one by one
+        // eight columns at a time, their LDS reads issued together (the pivot row's a
+        // broadcast read, row i this lane's own), then the remaining columns one by one
         int j = k + 1;
         for (; j + 8 <= sz; j += 8) {
           double2 u[8], x[8];
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
-            u[q] = a[(j + q) * sz + k];
+            u[q] = a[(j + q) * sz + p];
             x[q] = a[(j + q) * sz + i];
           }
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
-            const double2 p = cmuld(l, u[q]);
-            x[q].x = x[q].x - p.x;
-            x[q].y = x[q].y - p.y;
+            const double2 pr = cmuld(l, u[q]);
+            x[q].x = x[q].x - pr.x;
+            x[q].y = x[q].y - pr.y;
             a[(j + q) * sz + i] = x[q];
           }
         }
         for (; j < sz; ++j) {
-          const double2 p = cmuld(l, a[j * sz + k]);
+          const double2 pr = cmuld(l, a[j * sz + p]);
           double2 x = a[j * sz + i];
-          x.x = x.x - p.x;
-          x.y = x.y - p.y;
+          x.x = x.x - pr.x;
+          x.y = x.y - pr.y;
           a[j * sz + i] = x;
         }
       }
@@ -463,9 +530,9 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
     }
   }
   const long o = cfg * (opt.onm ? opt.onm : nm) + (long)m * opt.ostep + opt.oofs;
-  if (lu_out && i < sz) {
+  if (lu_out && i < sz) {  // row i of the factors is at position pos
     double2* out = lu_out + cfg * (long)n * ldm;
-    for (int j = 0; j < sz; ++j) out[(long)i * ldm + j] = a[j * sz + i];
+    for (int j = 0; j < sz; ++j) out[(long)pos * ldm + j] = a[j * sz + i];
     indx_out[cfg * (long)n + i] = my_indx;
     if (i == 0) d_out[cfg] = dsign;
   }

@@ -617,14 +617,14 @@ void k_pair_gh_mirror(const float2* __restrict__ psi,
 #pragma unroll
       for (int b = 0; b < MT; ++b) acc[m][a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const float2* base = psi + c * n * V;
-  // a lane's values of step i0: its four sites (va), their mirrors (vm) and ws there (loading
-  // the next step's values ahead of the products needs registers that cost the kernel its
-  // third wave per SIMD, and was slower)
-  auto load = [&](int i0, float (&va)[MT][8], float (&vm)[MT][8], float4& wv) {
+  // (loading the next step's values ahead of the products needs registers that cost the
+  // kernel its third wave per SIMD, and was slower)
+  for (int i0 = wave * 16; i0 < P; i0 += OV_WAVES * 16) {
     const int i = i0 + 4 * grp;  // this lane's four sites of the plane
     const int y = i / L, z = i - y * L;
     const long p0 = (long)x * P + i;
     const long mrow = (long)xm * P + (long)((L - y) % L) * L;
+    float va[MT][8], vm[MT][8];
 #pragma unroll
     for (int a = 0; a < MT; ++a) {
       const int s = a * 16 + row;
@@ -658,12 +658,7 @@ void k_pair_gh_mirror(const float2* __restrict__ psi,
         for (int q2 = 0; q2 < 8; ++q2) va[a][q2] = vm[a][q2] = 0.f;
       }
     }
-    wv = *reinterpret_cast<const float4*>(ws + p0);
-  };
-  for (int i0 = wave * 16; i0 < P; i0 += OV_WAVES * 16) {
-    float va[MT][8], vm[MT][8];
-    float4 wv;
-    load(i0, va, vm, wv);
+    const float4 wv = *reinterpret_cast<const float4*>(ws + p0);
     const float wa[4] = {fabsf(wv.x), fabsf(wv.y), fabsf(wv.z), fabsf(wv.w)};
     // per site dword: m1 = the sign of w on both halves, m2 = m1 with the imaginary sign flipped
     u32x4 m1 = {0u, 0u, 0u, 0u}, m2 = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
