@@ -199,6 +199,52 @@ def test_ludcmp_bit_exact_vs_oracle(ctx, orc, n):
         assert np.array_equal(lu[c], rlu), (c, np.abs(lu[c] - rlu).max())
 
 
+@pytest.mark.parametrize("n", [12, 33])
+def test_ludcmp_bit_exact_ties_and_scales(ctx, orc, n):
+    """The batched LU's exact paths, bit for bit against the oracle: rows of unit-modulus
+    entries (many candidates for the row maximum, so every entry's |.| is taken), small-integer
+    matrices (exact ties vv |a| in the pivot search at every step, after earlier row swaps, so
+    the last position among equals must win), rows scaled by 2^500 and 2^-500 (squared
+    magnitudes overflow or underflow), and a zero column (the 1e-20 pivot)."""
+    print(f"[gpu] ludcmp ties and scales n={n}", flush=True)
+    rng = np.random.default_rng(300 + n)
+    ncfg = 32
+    Ms = np.empty((ncfg, n, n), complex)
+    vals = np.array([0, 1, -1, 1j, -1j, 1 + 1j, 2, -2 + 1j])
+    for c in range(ncfg):
+        kind = c % 4
+        if kind == 0:
+            Ms[c] = np.exp(2j * np.pi * rng.random((n, n)))
+        elif kind == 1:
+            Ms[c] = vals[rng.integers(0, len(vals), (n, n))]
+            Ms[c][np.arange(n), rng.permutation(n)] += 3  # nonsingular, still tie-rich
+        elif kind == 2:
+            Ms[c] = _rand_c(rng, (n, n))
+            Ms[c, 0::3] *= 2.0 ** 500
+            Ms[c, 1::3] *= 2.0 ** -500
+        else:
+            Ms[c] = vals[rng.integers(1, len(vals), (n, n))]
+            Ms[c][:, n // 2] = 0
+    lu, indx, d, st = ctx.ludcmp(Ms)
+    assert np.all(st == 0)
+    for c in range(ncfg):
+        rlu, rindx, rd = orc.ludcmp(Ms[c])
+        assert np.array_equal(indx[c], rindx), c
+        assert d[c] == rd, c
+        assert np.array_equal(lu[c], rlu, equal_nan=True), (c, np.nanmax(np.abs(lu[c] - rlu)))
+    la, ph, ag, st = ctx.lu_det(ctx.to_device(Ms), n, ncfg, True)
+    for c in range(ncfg):
+        for m in range(n):
+            zero_row = bool((np.abs(Ms[c, :m + 1, :m + 1]).max(axis=1) == 0).any())
+            assert st[c, m] == int(zero_row), (c, m)  # an all-zero row is a status
+            if zero_row:
+                assert np.isneginf(la[c, m])
+                continue
+            rlu, _, rd = orc.ludcmp(Ms[c, :m + 1, :m + 1])
+            rla, rph, rag, _ = orc.lu_det(rlu, rd)
+            assert la[c, m] == rla or abs(la[c, m] - rla) < 1e-12 * max(1.0, abs(rla)), (c, m)
+
+
 def test_lu_singular_row_status(ctx):
     M = np.eye(4, dtype=complex)[None].repeat(2, 0)
     M[1, 2, :] = 0
