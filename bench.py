@@ -637,7 +637,11 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
                                  "peak_tflops": 2500.0,
                                  "frac": flops / (meas_ms * 1e-3) / 1e12 / 2500.0},
             "pair_matrix_hbm": {"algorithmic_gbs": 8.0 * nvo * V * C / (meas_ms * 1e-3) / 1e9,
-                                "peak_gbs": HBM_PEAK_GBS}}
+                                "peak_gbs": HBM_PEAK_GBS,
+                                "frac": 8.0 * nvo * V * C / (meas_ms * 1e-3) / 1e9
+                                        / HBM_PEAK_GBS,
+                                "bound": "hbm: the stored orbitals are read once per slice, "
+                                         "114 useful bf16 flops per byte"}}
 
 
 def evolution_roofline(L, N, T, C, V, nvo, args, tx, nx, tyz, nyz, tlu, nlu, tf, nf, tg, ng):
