@@ -504,65 +504,82 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
                   acc=1):
     """The overlap matrix Phi^H Psi(t) as a batched complex GEMM on the matrix cores (dense
     sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16 products
-    with FP32 accumulation, every orbital read once per measurement; with the default
-    arithmetic (acc 1) the splits are three-way (BF16x6: six products, FP32-exact inputs) and
-    the evolution runs the FP64 x-chain one slice per launch, the GEMM reading the state
-    with its x-DFT pending against sinks transformed to match; acc 0: split-BF16x3 and plain
-    FP32 single-slice evolution.  The sinks are the plane waves as dense k-space vectors, so
-    the gathered measurement is the check.  Reports the overlap kernels' time per slice
-    against the MFMA peak (terms x 8 N^2 V bf16 flops per configuration-slice) and the HBM
-    peak (8 N V bytes read per configuration-slice)."""
+    with FP32 accumulation; with the default arithmetic (acc 1) the splits are three-way
+    (BF16x6: six products, FP32-exact inputs) and the evolution runs the FP64 x-chain one
+    slice per launch, the GEMM reading the state with its x-DFT pending against sinks
+    transformed to match; acc 0: split-BF16x3 and plain FP32 single-slice evolution.  With
+    conjugate partners the stored volumes (one per (k, -k) pair) are projected on the sinks
+    and on their conjugate mirrors, two GEMMs of N sink rows each; every orbital evolved (the
+    reference's layout) is reported beside it.  The sinks are the plane waves as dense k-space
+    vectors, so the gathered measurement is the check.  Reports the overlap kernels' time per
+    slice against the MFMA peak (the executed terms x 8 N n_cols V bf16 flops per
+    configuration-slice, n_cols = 2 n_vol with partners, N without) and the HBM peak (8 n_cols
+    V bytes read per configuration-slice)."""
     moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
     V = L ** 3
     sinks = np.zeros((N, V), np.complex128)
     sinks[np.arange(N), (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]] = 1.0
-    e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE,
-                    sinks=sinks)
-    e.set_path(na.PATH_SINGLE_PASS)
-    e.set_accuracy(acc)
     terms = 6 if acc != 0 else 3
-    e.run(seed, 0, 2)
-    ctx.sync()
-    t0 = time.perf_counter()
-    la_d, ph_d, _ = e.run(seed, 0, T)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    e.profile(True)
-    e.run(seed, 0, T)
-    t_ovl, n_ovl = e.kernel_time(4)
-    e.profile(False)
-    e.close()
-    g = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
-    la_g, _, _ = g.run(seed, 0, T)
-    g.close()
-    prec = None
-    if ref64 is not None and ref64[0].shape[:2] == (T, C):
-        m64, c64 = correlator(*ref64)
-        m32, c32 = correlator(la_d, ph_d)
-        rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
-        prec = {"max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
-                "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
-                "reference": "FP64 path on the same fields (the precision report's)",
-                "arithmetic": ("FP32 evolution with the FP64 x-chain (one slice per launch, "
-                               "every orbital), split-BF16x6 overlaps on MFMA with FP32 "
-                               "accumulation, FP64 LU" if acc != 0 else
-                               "plain FP32 evolution (single-slice kernel, every orbital), "
-                               "split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU")}
-    ms = t_ovl / max(n_ovl, 1)
-    flops = terms * 8.0 * N * N * V * C
-    byts = 8.0 * N * V * C
-    return {"workload": f"dense sinks (the precision form as written): L={L}, N={N}, T={T}, "
-                        f"{C} configurations, FP32 evolution, split-BF16x{terms} overlaps on "
-                        f"MFMA, FP64 LU",
-            "terms": terms,
-            "value": C * T / dt, "unit": "configs*slices/s",
-            "overlap_ms_per_slice": ms,
+
+    def run(conj):
+        e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE,
+                        sinks=sinks)
+        e.set_path(na.PATH_SINGLE_PASS)
+        e.set_accuracy(acc)
+        e.set_symmetry(conj)
+        nvo = e.volumes_per_config()
+        e.run(seed, 0, 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        la, ph, _ = e.run(seed, 0, T)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        e.profile(True)
+        e.run(seed, 0, T)
+        t_ovl, n_ovl = e.kernel_time(4)
+        e.profile(False)
+        e.close()
+        ms = t_ovl / max(n_ovl, 1)
+        cols = 2 * nvo if conj else N
+        flops = terms * 8.0 * N * cols * V * C
+        byts = 8.0 * cols * V * C
+        return la, ph, {
+            "conj_partners": conj, "volumes_per_config": nvo,
+            "value": C * T / dt, "unit": "configs*slices/s", "overlap_ms_per_slice": ms,
             "mfma": {"achieved_tflops": flops / (ms * 1e-3) / 1e12, "peak_tflops": 2500.0,
                      "frac": flops / (ms * 1e-3) / 2.5e15},
             "hbm": {"achieved_gbs": byts / (ms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
-                    "frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "max_abs_diff_logdet_vs_gather": float(np.abs(la_d - la_g).max()),
-            "precision": prec,
+                    "frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+    la_d, ph_d, rec = run(True)
+    la_f, ph_f, rec_full = run(False)
+    g = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F32, split=na.SPLIT_KSPACE)
+    la_g, _, _ = g.run(seed, 0, T)
+    g.close()
+
+    def prec_of(la, ph, what):
+        if ref64 is None or ref64[0].shape[:2] != (T, C):
+            return None
+        m64, c64 = correlator(*ref64)
+        m32, c32 = correlator(la, ph)
+        rel = np.abs(c32 * np.exp(m32 - m64) - c64) / np.abs(c64)
+        return {"max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
+                "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
+                "reference": "FP64 path on the same fields (the precision report's)",
+                "arithmetic": (f"FP32 evolution with the FP64 x-chain (one slice per launch, "
+                               f"{what}), split-BF16x6 overlaps on MFMA with FP32 "
+                               f"accumulation, FP64 LU" if acc != 0 else
+                               f"plain FP32 evolution (single-slice kernel, {what}), "
+                               f"split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU")}
+
+    rec["precision"] = prec_of(la_d, ph_d, "conjugate partners")
+    rec_full["precision"] = prec_of(la_f, ph_f, "every orbital")
+    rec["max_abs_diff_logdet_vs_gather"] = float(np.abs(la_d - la_g).max())
+    rec_full["max_abs_diff_logdet_vs_gather"] = float(np.abs(la_f - la_g).max())
+    return {"workload": f"dense sinks (the precision form as written): L={L}, N={N}, T={T}, "
+                        f"{C} configurations, FP32 evolution, split-BF16x{terms} overlaps on "
+                        f"MFMA, FP64 LU",
+            "terms": terms, **rec, "every_orbital": rec_full,
             "timing": "HIP events around the overlap kernels of each measurement"}
 
 
@@ -869,7 +886,9 @@ def main():
     if not args.no_dense:
         dense = dense_overlap(ctx, na, L=L, N=N, T=T, C=C, ref64=ref64 if (
             args.precision_configs == C) else None, acc=args.acc)
-        print(f"dense overlaps: {dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
+        print(f"dense overlaps: {dense['value']:.0f} configs*slices/s with partners "
+              f"({dense['every_orbital']['value']:.0f} every orbital), "
+              f"{dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
               f"{dense['mfma']['frac']:.3f}", file=sys.stderr, flush=True)
 
     cpu = None

@@ -426,14 +426,15 @@ static size_t overlap_bf16x3_workspace(int ns, int nt, long vol, int ncfg,
 }
 // terms 3: split-BF16x3; 6: split-BF16x6 (three-way splits, six products)
 static int overlap_bf16x3(const void* phi, const void* psi, double* M, int ns, int nt, long vol,
-                          int ncfg, void* work, hipStream_t s, int terms, int variant) {
+                          int ncfg, void* work, hipStream_t s, int terms, int variant,
+                          long cstride = 0) {
   char* w = (char*)work;
   const size_t split = overlap_split_bytes(ns, vol, variant);
   const int parts = terms == 6 ? 3 : 2;
   if (variant == NRC_OVL_COLS) {
     NRC_HIP(nrc::launch_split_bf16((const float2*)phi, w, ns, vol, s, parts));
     NRC_HIP(nrc::launch_overlap_bf16x3(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
-                                       (float*)(w + split), s, terms));
+                                       (float*)(w + split), s, terms, cstride));
     return NRC_OK;
   }
   const int form = variant == NRC_OVL_SHARED        ? 0
@@ -443,7 +444,7 @@ static int overlap_bf16x3(const void* phi, const void* psi, double* M, int ns, i
   if (form < 0) return NRC_ERR_ARG;
   NRC_HIP(nrc::launch_split_rows((const float2*)phi, w, ns, vol, s, parts));
   NRC_HIP(nrc::launch_overlap_lds(w, (const float2*)psi, (double2*)M, ns, nt, vol, ncfg,
-                                  (float*)(w + split), s, terms, form));
+                                  (float*)(w + split), s, terms, form, cstride));
   return NRC_OK;
 }
 
@@ -700,6 +701,7 @@ struct nrc_ensemble {
   const int* col_conj = nullptr;
   const int* msite = nullptr;     // [N] site of -k_s
   std::vector<int> h_col_vol, h_col_conj;
+  int mir_v0 = 0, mir_v1 = 0;  // dense sinks: the stored volumes projected on the mirrored sinks
   bool pair_sym_ok = false;  // pair sinks: opp is the mirror and w even (partners allowed)
   void* trap_tab = nullptr;  // trap (nrc_ensemble_set_trap): (1 +- A) q / V, MUL_TABLE format
   double* trap_sep = nullptr;  // a separable trap q = qx(x) qy(y) qz(z): [qx, qy, qz] (FP64)
@@ -825,6 +827,23 @@ static long mirror_site(int L, long i) {
   return (((L - n0) % L) * L + (L - n1) % L) * L + (L - n2) % L;
 }
 
+// the N dense sinks followed by their conjugate mirrors J phi_s, [2N][V] complex: J phi(p) =
+// conj phi(-p) in k-space (conj phi(x) in x-space).  With conjugate partners the column of the
+// partner J psi_r of a stored volume r is <phi_s | J psi_r> = conj <J phi_s | psi_r>.
+static std::vector<double> sinks_with_mirrors(const double* sinks, int N, int L, bool kspace) {
+  const long V = (long)L * L * L;
+  std::vector<double> out((size_t)4 * N * V);
+  std::copy(sinks, sinks + (size_t)2 * N * V, out.begin());
+  for (int n = 0; n < N; ++n)
+    for (long i = 0; i < V; ++i) {
+      const long j = kspace ? mirror_site(L, i) : i;
+      const size_t o = ((size_t)(N + n) * V + i) * 2, src = ((size_t)n * V + j) * 2;
+      out[o] = sinks[src];
+      out[o + 1] = -sinks[src + 1];
+    }
+  return out;
+}
+
 extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
                                    nrc_ensemble** out) {
   if (!ctx || !d || !out || !d->moms || !d->kin) return NRC_ERR_ARG;
@@ -905,18 +924,23 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
   }
   ENS_TRY(dev_alloc(&e->M, (size_t)C * N * N));
   ENS_TRY(dev_alloc(&e->Mx, (size_t)nrc::SLICE_FUSE_MAX * C * N * N));
-  if (d->sinks) {  // dense sinks in the ensemble precision
+  // dense sinks and their conjugate mirrors (2N sink vectors; the mirrors serve conjugate
+  // partners), in the ensemble precision
+  const std::vector<double> sk2 =
+      d->sinks ? sinks_with_mirrors(d->sinks, N, L, d->split == NRC_SPLIT_KSPACE)
+               : std::vector<double>();
+  if (d->sinks) {
     if (d->prec == NRC_PREC_F64) {
-      ENS_TRY(hipMalloc(&e->sinks, (size_t)N * V * 16) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
-      ENS_TRY(hipMemcpy(e->sinks, d->sinks, (size_t)N * V * 16, hipMemcpyHostToDevice) ==
+      ENS_TRY(hipMalloc(&e->sinks, sk2.size() * 8) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
+      ENS_TRY(hipMemcpy(e->sinks, sk2.data(), sk2.size() * 8, hipMemcpyHostToDevice) ==
                       hipSuccess ? NRC_OK : NRC_ERR_HIP);
     } else {
       if ((V & 3) != 0) {
         ens_free(e);
         return NRC_ERR_UNSUPPORTED;
       }
-      std::vector<float> f((size_t)2 * N * V);
-      for (size_t i = 0; i < f.size(); ++i) f[i] = (float)d->sinks[i];
+      std::vector<float> f(sk2.size());
+      for (size_t i = 0; i < f.size(); ++i) f[i] = (float)sk2[i];
       ENS_TRY(hipMalloc(&e->sinks, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
       ENS_TRY(hipMemcpy(e->sinks, f.data(), f.size() * 4, hipMemcpyHostToDevice) == hipSuccess
                   ? NRC_OK : NRC_ERR_HIP);
@@ -968,13 +992,13 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
     ENS_TRY(hipMemcpy(e->sink_k, sk.data(), N * sizeof(double), hipMemcpyHostToDevice) ==
                     hipSuccess ? NRC_OK : NRC_ERR_HIP);
     if (d->sinks) {
-      const size_t n = (size_t)N * V;
+      const size_t n = (size_t)2 * N * V;
       if (d->prec == NRC_PREC_F64) {
         std::vector<double> f(2 * n);
         for (size_t i = 0; i < n; ++i) {
           const double k = e->khalf[i % V];
-          f[2 * i] = d->sinks[2 * i] * k;
-          f[2 * i + 1] = d->sinks[2 * i + 1] * k;
+          f[2 * i] = sk2[2 * i] * k;
+          f[2 * i + 1] = sk2[2 * i + 1] * k;
         }
         ENS_TRY(hipMalloc(&e->sinks_k, f.size() * 8) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
         ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 8, hipMemcpyHostToDevice) ==
@@ -983,8 +1007,8 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
         std::vector<float> f(2 * n);
         for (size_t i = 0; i < n; ++i) {
           const double k = e->khalf[i % V];
-          f[2 * i] = (float)(d->sinks[2 * i] * k);
-          f[2 * i + 1] = (float)(d->sinks[2 * i + 1] * k);
+          f[2 * i] = (float)(sk2[2 * i] * k);
+          f[2 * i + 1] = (float)(sk2[2 * i + 1] * k);
         }
         ENS_TRY(hipMalloc(&e->sinks_k, f.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
         ENS_TRY(hipMemcpy(e->sinks_k, f.data(), f.size() * 4, hipMemcpyHostToDevice) ==
@@ -992,7 +1016,7 @@ extern "C" int nrc_ensemble_create(nrc_context* ctx, const nrc_ensemble_desc* d,
         // the same sinks for the x-chain's stored state m, whose folded k-space state is
         // X+ m (the x-DFT of sign +1 along the first index): <s|K(1/2) X+ m> =
         // <X- (K(1/2) s)|m>, since the adjoint of X+ is X-; computed here in FP64
-        const std::vector<float> mx = sinks_for_pending_x(d->sinks, e->khalf, N, L);
+        const std::vector<float> mx = sinks_for_pending_x(sk2.data(), e->khalf, 2 * N, L);
         ENS_TRY(hipMalloc(&e->sinks_mx, mx.size() * 4) == hipSuccess ? NRC_OK : NRC_ERR_NOMEM);
         ENS_TRY(hipMemcpy(e->sinks_mx, mx.data(), mx.size() * 4, hipMemcpyHostToDevice) ==
                         hipSuccess ? NRC_OK : NRC_ERR_HIP);
@@ -1298,7 +1322,30 @@ static int ens_measure(nrc_ensemble* e, int t) {
     // folded k-space state xi_half psi: sinks carry the missing K(1/2); a state with its
     // x-DFT(+1) pending: the sinks transformed by X- as well
     const void* sk = e->pending_x ? e->sinks_mx : e->folded ? e->sinks_k : e->sinks;
-    if (e->sinks && prec == NRC_PREC_F64)
+    if (e->sinks && e->sym) {
+      // conjugate partners: the sinks and their mirrors against the stored volumes, then the
+      // N x N matrix (a partner's column is the conjugate of its mirrored-sink overlaps)
+      // (the mirrored sinks only against the volumes that have a partner: a contiguous range
+      // for shells filled in the reference's momentum order, so k = 0 drops out)
+      const int nv = e->nvo, v0 = e->mir_v0, nm = e->mir_v1 - e->mir_v0;
+      const size_t esz = prec == NRC_PREC_F64 ? 16 : 8;
+      double2* Mp = e->Mx;
+      double2* Mm = e->Mx + (size_t)C * N * nv;
+      const void* skm = (const char*)sk + (size_t)N * V * esz;
+      const void* pm = (const char*)e->psi + (size_t)v0 * V * esz;
+      if (prec == NRC_PREC_F64) {
+        NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, Mp, N, nv, V, (int)C, s));
+        NRC_HIP(nrc::launch_overlap_dense(prec, skm, pm, Mm, N, nm, V, (int)C, s, nv * V));
+      } else {
+        const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
+        NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)Mp, N, nv, V, (int)C, e->ovl_work, s, terms,
+                               overlap_variant(e->ctx, N, nv, V, terms)));
+        NRC_TRY(overlap_bf16x3(skm, pm, (double*)Mm, N, nm, V, (int)C, e->ovl_work, s, terms,
+                               overlap_variant(e->ctx, N, nm, V, terms), nv * V));
+      }
+      NRC_HIP(nrc::launch_sink_expand(Mp, Mm, e->col_vol, e->col_conj, e->M, N, nv, nm, v0, C,
+                                      s));
+    } else if (e->sinks && prec == NRC_PREC_F64)
       NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks) {
       const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
@@ -1901,8 +1948,7 @@ extern "C" int nrc_ensemble_volumes(nrc_ensemble* e, int* nvo) {
 extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   if (!e || (conj != 0 && conj != 1)) return NRC_ERR_ARG;
   if (ens_in_batch(e)) return NRC_ERR_BATCHED;
-  if (conj && (!e->sym_ok || (e->pair && !e->pair_sym_ok) || e->sinks))
-    return NRC_ERR_UNSUPPORTED;
+  if (conj && (!e->sym_ok || (e->pair && !e->pair_sym_ok))) return NRC_ERR_UNSUPPORTED;
   hipSetDevice(e->ctx->device);
   NRC_HIP(hipStreamSynchronize(e->stream));
   if (e->sym) hipFree(e->sym);
@@ -1940,6 +1986,19 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
     for (int j = 0; j < 3; ++j) vo_moms.push_back(k[3 * n + j]);
   }
   const int nvo = (int)vo_col.size();
+  // the volumes with a partner: [v0, v1) when they are contiguous, else every volume
+  int v0 = nvo, v1 = 0;
+  for (int r = 0; r < nvo; ++r)
+    if (vo_pcol[r] >= 0) {
+      v0 = std::min(v0, r);
+      v1 = std::max(v1, r + 1);
+    }
+  bool contiguous = v1 > v0;
+  for (int r = v0; r < v1 && contiguous; ++r) contiguous = vo_pcol[r] >= 0;
+  if (!contiguous) {
+    v0 = 0;
+    v1 = nvo;
+  }
   // gather lists of the mirrored sinks, in the pencil layout of the plain ones: k-space split
   // x-pencils (pencil (k1, k2), position k0), x-space split y-columns (pencil (k0, k2),
   // position k1)
@@ -1995,6 +2054,8 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
   e->nvo = nvo;
   e->h_col_vol = col_vol;
   e->h_col_conj = col_conj;
+  e->mir_v0 = v0;
+  e->mir_v1 = v1;
   return NRC_OK;
 }
 
@@ -2207,10 +2268,17 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
     const int N = e->d.n_orb;
     double ms[kNumOvlVariants];
     int v = 0;
-    rc = overlap_tune(e->ctx, e->sinks, e->psi, (double*)e->M, N, N, e->V, (int)e->d.n_configs,
-                      e->ovl_work, e->stream, terms, 2, &v, ms);
+    rc = overlap_tune(e->ctx, e->sinks, e->psi, (double*)e->Mx, N, e->sym ? e->nvo : N, e->V,
+                      (int)e->d.n_configs, e->ovl_work, e->stream, terms, 2, &v, ms);
     for (int i = 0; rc == NRC_OK && i < kNumOvlVariants; ++i)
       e->tune_log.push_back({2.0, (double)kOvlVariants[i], ms[i]});
+    const int nm = e->mir_v1 - e->mir_v0;
+    if (rc == NRC_OK && e->sym && nm != e->nvo) {  // the mirrored sinks' shape as well
+      rc = overlap_tune(e->ctx, e->sinks, e->psi, (double*)e->Mx, N, nm, e->V,
+                        (int)e->d.n_configs, e->ovl_work, e->stream, terms, 2, &v, ms);
+      for (int i = 0; rc == NRC_OK && i < kNumOvlVariants; ++i)
+        e->tune_log.push_back({2.0, (double)kOvlVariants[i], ms[i]});
+    }
     if (rc == NRC_OK) {  // the slice time with the chosen GEMM
       float t = 0.f;
       rc = ens_time_slices(e, T_probe, a, b, &t);

@@ -582,14 +582,15 @@ hipError_t launch_lu_det(const double2* M, int n, int ldm, int ncfg, int minors,
 template <typename T>
 __global__ __launch_bounds__(256) void k_overlap_dense(const C2<T>* __restrict__ phi,
                                                        const C2<T>* __restrict__ psi,
-                                                       double2* M, int ns, int nt, long vol) {
+                                                       double2* M, int ns, int nt, long vol,
+                                                       long cstride) {
   constexpr int CH = 16;
   __shared__ double red[CH][2][4];
   const long ct = blockIdx.x;  // c * nt + t
   const long c = ct / nt;
   const int t = (int)(ct - c * nt);
   const int s0 = blockIdx.y * CH;
-  const C2<T>* p = psi + ct * vol;
+  const C2<T>* p = psi + c * cstride + (long)t * vol;
   double ar[CH], ai[CH];
 #pragma unroll
   for (int q = 0; q < CH; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
@@ -720,15 +721,50 @@ hipError_t launch_pair_expand(const double2* G, const double2* H, const int* col
   return hipGetLastError();
 }
 
+// M[c][s][t] of orbitals stored as nvo volumes with conjugate partners, from the overlaps of
+// the sinks with the stored volumes (Mp, [c][s][nvo]) and of their mirrors J phi_s with the
+// volumes v0 .. v0 + nm - 1 (Mm, [c][s][nm]): column t is volume col_vol[t], or the conjugate
+// of its mirrored-sink overlap when col_conj[t]
+__global__ void k_sink_expand(const double2* Mp, const double2* Mm, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, int nm, int v0,
+                              long ncfg) {
+  const long n = ncfg * N * N;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const long c = i / ((long)N * N);
+    const int r = (int)(i - c * N * N), s = r / N, t = r - (r / N) * N;
+    if (col_conj[t]) {
+      const double2 g = Mm[(c * N + s) * nm + col_vol[t] - v0];
+      M[i] = make_double2(g.x, -g.y);
+    } else {
+      M[i] = Mp[(c * N + s) * nvo + col_vol[t]];
+    }
+  }
+}
+
+hipError_t launch_sink_expand(const double2* Mp, const double2* Mm, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, int nm, int v0,
+                              long ncfg, hipStream_t s) {
+  const long n = ncfg * N * N;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_sink_expand, dim3((unsigned)blocks), dim3(256), 0, s, Mp, Mm, col_vol,
+                     col_conj, M, N, nvo, nm, v0, ncfg);
+  return hipGetLastError();
+}
+
 hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,
-                                int ns, int nt, long vol, int ncfg, hipStream_t s) {
+                                int ns, int nt, long vol, int ncfg, hipStream_t s,
+                                long cstride) {
+  if (cstride == 0) cstride = nt * vol;
   dim3 grid((unsigned)((long)ncfg * nt), (unsigned)((ns + 15) / 16));
   if (prec == NRC_PREC_F64)
     hipLaunchKernelGGL(k_overlap_dense<double>, grid, dim3(256), 0, s,
-                       (const C2<double>*)phi, (const C2<double>*)psi, M, ns, nt, vol);
+                       (const C2<double>*)phi, (const C2<double>*)psi, M, ns, nt, vol, cstride);
   else
     hipLaunchKernelGGL(k_overlap_dense<float>, grid, dim3(256), 0, s, (const C2<float>*)phi,
-                       (const C2<float>*)psi, M, ns, nt, vol);
+                       (const C2<float>*)psi, M, ns, nt, vol, cstride);
   return hipGetLastError();
 }
 

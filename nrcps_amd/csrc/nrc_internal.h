@@ -239,13 +239,16 @@ __host__ __device__ inline double source_amplitude(long c) {
 // |z| computed exactly as the host C library's hypot (the LU's pivot metric)
 hipError_t launch_ref_abs(const double2* z, double* out, long n, hipStream_t s);
 
-// dense overlaps M[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i]
+// dense overlaps M[c][s][t] = sum_i conj(phi[s][i]) psi[c][t][i]; the overlap entry points
+// take configuration c's orbitals at psi + c cstride (complex elements; 0: nt vol), so a call
+// can cover a range of the volumes a configuration stores
 hipError_t launch_overlap_dense(int prec, const void* phi, const void* psi, double2* M,
-                                int ns, int nt, long vol, int ncfg, hipStream_t s);
+                                int ns, int nt, long vol, int ncfg, hipStream_t s,
+                                long cstride = 0);
 // terms 3: split-BF16x3 (phi_split: hi, lo planes); 6: split-BF16x6 (three planes)
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
                                  int nt, long vol, int ncfg, float* partial, hipStream_t s,
-                                 int terms = 3);
+                                 int terms = 3, long cstride = 0);
 // phi_split: `parts` bf16 planes [part][ns][2V] (hi, lo, and with parts = 3 the third part)
 hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hipStream_t s,
                              int parts = 2);
@@ -259,7 +262,7 @@ hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hip
                              int parts);
 hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2* M, int ns,
                               int nt, long vol, int ncfg, float* partial, hipStream_t s,
-                              int terms, int form);
+                              int terms, int form, long cstride = 0);
 size_t overlap_rows_partial_floats(int ns, int nt, long vol, int ncfg);
 
 // pair-sink matrices M[c][s][t] = sum_i w[i] psi1[c][s][i] psi2[c][t][opp[i]]: FP64 sums
@@ -286,6 +289,12 @@ size_t pair_gh_bf16x3_partial_floats(int n, long vol, int ncfg, int L = 0);
 // conjugate partners, from G[c][a][b] = sum w psi_a(p) psi_b(-p) and H[c][a][b] =
 // sum w psi_a(p) conj psi_b(p) over the stored volumes: orbital s is volume col_vol[s],
 // mirrored when col_conj[s] (M = G, H, conj H, conj G for the four cases)
+// the N x N overlaps of configurations stored as nvo volumes with conjugate partners, from
+// the overlaps of the sinks with the stored volumes (Mp) and of their conjugate mirrors with
+// the volumes v0 .. v0 + nm - 1 (Mm)
+hipError_t launch_sink_expand(const double2* Mp, const double2* Mm, const int* col_vol,
+                              const int* col_conj, double2* M, int N, int nvo, int nm, int v0,
+                              long ncfg, hipStream_t s);
 hipError_t launch_pair_expand(const double2* G, const double2* H, const int* col_vol,
                               const int* col_conj, double2* M, int N, int nvo, long ncfg,
                               hipStream_t s);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
     const __bf16* __restrict__ phi_hi, const __bf16* __restrict__ phi_lo,
     const __bf16* __restrict__ phi_lo2,
     const float2* __restrict__ psi, float* __restrict__ partial, int ns, int nt, long vol,
-    long kchunk) {
+    long kchunk, long cstride) {
   static_assert(TERMS == 3 || TERMS == 6, "terms");
   __shared__ float red[OV_WAVES - 1][4][64];
   const int chunk = blockIdx.x, nchunks = gridDim.x;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256, (MT * NT <= 9 ? 2 : 1)) void k_overlap_bf16x3(
   const rsrc_t rh = make_rsrc(phi_hi, (uint32_t)(ns * K * 2));
   const rsrc_t rl = make_rsrc(phi_lo, (uint32_t)(ns * K * 2));
   const rsrc_t rl2 = make_rsrc(phi_lo2, TERMS == 6 ? (uint32_t)(ns * K * 2) : 0u);
-  const rsrc_t rb = make_rsrc(psi + c * nt * vol, (uint32_t)(nt * vol * 8));
+  const rsrc_t rb = make_rsrc(psi + c * cstride, (uint32_t)(nt * vol * 8));
   constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
   uint32_t oa[MT], ob[NT];
 #pragma unroll
@@ -853,8 +853,9 @@ size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg) {
 
 hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, double2* M, int ns,
                                  int nt, long vol, int ncfg, float* partial, hipStream_t s,
-                                 int terms) {
+                                 int terms, long cstride) {
   if (terms != 3 && terms != 6) return hipErrorInvalidValue;
+  if (cstride == 0) cstride = nt * vol;
   if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
     return hipErrorInvalidValue;
   const int mt = (ns + 15) / 16, ntt = (nt + 15) / 16;
@@ -868,10 +869,10 @@ hipError_t launch_overlap_bf16x3(const void* phi_split, const float2* psi, doubl
 #define NRC_OV_CASE(A, B)                                                                    \
   if (mt == A && ntt == B && terms == 3) {                                                   \
     hipLaunchKernelGGL((k_overlap_bf16x3<A, B, 3>), grid, block, 0, s, hi, lo, lo, psi,      \
-                       partial, ns, nt, vol, kc);                                            \
+                       partial, ns, nt, vol, kc, cstride);                                   \
   } else if (mt == A && ntt == B) {                                                          \
     hipLaunchKernelGGL((k_overlap_bf16x3<A, B, 6>), grid, block, 0, s, hi, lo, lo2, psi,     \
-                       partial, ns, nt, vol, kc);                                            \
+                       partial, ns, nt, vol, kc, cstride);                                   \
   }
 #define NRC_OV_ROW(A) NRC_OV_CASE(A, 1) NRC_OV_CASE(A, 2) NRC_OV_CASE(A, 3) NRC_OV_CASE(A, 4)
   NRC_OV_ROW(1) NRC_OV_ROW(2) NRC_OV_ROW(3) NRC_OV_ROW(4)
@@ -1018,7 +1019,7 @@ template <int MR, int NT, int TERMS, int FORM>
 __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
     const __bf16* __restrict__ a0p, const __bf16* __restrict__ a1p,
     const __bf16* __restrict__ a2p, const float2* __restrict__ psi, float* __restrict__ partial,
-    int nrow, int nt, long vol, int ncfg, long kchunk, int nchunks) {
+    int nrow, int nt, long vol, int ncfg, long kchunk, int nchunks, long cstride) {
   constexpr int NPA = TERMS == 6 ? 3 : 2, R = MR * 16, KS = ovl_lds_ks<MR, TERMS, FORM>();
   constexpr int PD = FORM == 0 ? 2 : 4, Q = KS * 4;
   constexpr int SU = PD > KS ? PD / KS : 1;  // stages per loop iteration (static buffers)
@@ -1042,7 +1043,7 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
   const uint32_t abytes = (uint32_t)(nrow * K * 2);
   const rsrc_t ra[3] = {make_rsrc(a0p, abytes), make_rsrc(a1p, abytes),
                         make_rsrc(a2p, TERMS == 6 ? abytes : 0u)};
-  const rsrc_t rb = make_rsrc(psi + (live ? c : 0) * nt * vol, (uint32_t)(nt * vol * 8));
+  const rsrc_t rb = make_rsrc(psi + (live ? c : 0) * cstride, (uint32_t)(nt * vol * 8));
   constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
   // the stage loader, part by part (the part selects the buffer resource, so it stays
   // uniform): chunk id = ((row / 4) Q + q) 4 + row % 4
@@ -1158,8 +1159,9 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
 
 hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2* M, int ns,
                               int nt, long vol, int ncfg, float* partial, hipStream_t s,
-                              int terms, int form) {
+                              int terms, int form, long cstride) {
   if ((terms != 3 && terms != 6) || form < 0 || form > 2) return hipErrorInvalidValue;
+  if (cstride == 0) cstride = nt * vol;
   if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
     return hipErrorInvalidValue;
   const int nrow = 2 * ns, mr = (nrow + 15) / 16, ntt = (nt + 15) / 16;
@@ -1173,7 +1175,7 @@ hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2
   bool launched = false;
 #define NRC_OL_T(A, B, T, F)                                                                  \
   hipLaunchKernelGGL((k_overlap_lds<A, B, T, F>), grid, block, 0, s, a0, a1, T == 6 ? a2 : a1,  \
-                     psi, partial, nrow, nt, vol, ncfg, kc, nchunks)
+                     psi, partial, nrow, nt, vol, ncfg, kc, nchunks, cstride)
 #define NRC_OL_CASE(A, B)                                                                     \
   if (!launched && mr == A && ntt == B) {                                                     \
     launched = true;                                                                          \

@@ -151,11 +151,42 @@ def test_conj_partners_rejected_where_not_exact(ctx, orc):
     with pytest.raises(na.NrcError):
         e.set_symmetry(True)
     e.close()
-    # dense sinks
-    sinks = np.zeros((N, V), complex)
-    for s, k in enumerate(moms):
-        sinks[s, (k[0] * L + k[1]) * L + k[2]] = 1
-    e = na.Ensemble(ctx, L, moms, kin, 0.4, 2, prec=na.PREC_F64, sinks=sinks)
-    with pytest.raises(na.NrcError):
-        e.set_symmetry(True)
+
+
+@pytest.mark.parametrize("prec,split", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_dense_sinks_with_conj_partners(ctx, orc, prec, split):
+    """Dense sinks with conjugate partners: the sinks and their conjugate mirrors J phi(p) =
+    conj phi(-p) are projected on the stored volumes and a partner's column is the conjugate
+    of its mirrored-sink overlaps.  Random dense sinks (no symmetry of their own), against the
+    oracle, which evolves every orbital and projects on the sinks as given: FP64 1e-11; FP32
+    against its own every-orbital run (the same GEMM, separately rounded partner volumes)."""
+    import nrcps_amd as na
+
+    L, N, T, C, seed = 8, 7, 6, 3, 41
+    print(f"[gpu] dense sinks with conjugate partners prec={prec} split={split}", flush=True)
+    moms = orc.lowest_momenta(L, N)
+    kin = _tables(orc, L, split)
+    V = L ** 3
+    rng = np.random.default_rng(17)
+    sinks = rng.standard_normal((N, V)) + 1j * rng.standard_normal((N, V))
+    sinks /= np.linalg.norm(sinks, axis=1, keepdims=True)
+    e = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=split, sinks=sinks)
+    e.set_symmetry(True)
+    assert e.volumes_per_config() == 4
+    la, ph, _ = e.run(seed, 0, T)
     e.close()
+    if prec == na.PREC_F64:
+        for c in range(C):
+            ref = orc.run_config(L, N, T, moms, kin, A=0.4, split=split, seed=seed, config=c,
+                                 sinks=sinks)
+            err = np.abs(la[:, c, 0] - ref["logabs"][:, 0]).max()
+            perr = np.abs(ph[:, c, 0] - ref["phase"][:, 0]).max()
+            print(f"[gpu] config {c}: max |d log|det|| {err:.2e}, phase {perr:.2e}", flush=True)
+            assert err < 1e-11 and perr < 1e-10
+    else:
+        f = na.Ensemble(ctx, L, moms, kin, 0.4, C, prec=prec, split=split, sinks=sinks)
+        lf, pf, _ = f.run(seed, 0, T)
+        f.close()
+        err, perr = np.abs(la - lf).max(), np.abs(ph - pf).max()
+        print(f"[gpu] FP32 partners vs every orbital: {err:.2e}, phase {perr:.2e}", flush=True)
+        assert err < 2e-5 and perr < 2e-5
