@@ -505,9 +505,10 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
     """The overlap matrix Phi^H Psi(t) as a batched complex GEMM on the matrix cores (dense
     sinks, OneBody::Project / Cuda::InnerProduct for any sink vectors): split-BF16 products
     with FP32 accumulation; with the default arithmetic (acc 1) the splits are three-way
-    (BF16x6: six products, FP32-exact inputs) and the evolution runs the FP64 x-chain one
-    slice per launch, the GEMM reading the state with its x-DFT pending against sinks
-    transformed to match; acc 0: split-BF16x3 and plain FP32 single-slice evolution.  With
+    (BF16x6: six products, FP32-exact inputs) and the evolution runs the FP64 x-chain four
+    slices per launch, writing the folded states inside a launch for the GEMM (which reads the
+    launch's first state with its x-DFT pending, against sinks transformed to match); acc 0:
+    split-BF16x3 and plain FP32 single-slice evolution.  With
     conjugate partners the stored volumes (one per (k, -k) pair) are projected on the sinks
     and on their conjugate mirrors, two GEMMs of N sink rows each; every orbital evolved (the
     reference's layout) is reported beside it.  The sinks are the plane waves as dense k-space
@@ -566,9 +567,10 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
         return {"max_rel_err_C": float(rel.max()), "median_rel_err_C": float(np.median(rel)),
                 "frac_t_within_1e-6": float((rel <= 1e-6).mean()),
                 "reference": "FP64 path on the same fields (the precision report's)",
-                "arithmetic": (f"FP32 evolution with the FP64 x-chain (one slice per launch, "
-                               f"{what}), split-BF16x6 overlaps on MFMA with FP32 "
-                               f"accumulation, FP64 LU" if acc != 0 else
+                "arithmetic": (f"FP32 evolution with the FP64 x-chain (four slices per launch, "
+                               f"the states inside a launch written for the GEMM, {what}), "
+                               f"split-BF16x6 overlaps on MFMA with FP32 accumulation, FP64 LU"
+                               if acc != 0 else
                                f"plain FP32 evolution (single-slice kernel, {what}), "
                                f"split-BF16x3 overlaps on MFMA with FP32 accumulation, FP64 LU")}
 
@@ -613,7 +615,8 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
         trapped = {"value": C * T / dtt, "unit": "configs*slices/s",
                    "finite_fraction": float(np.isfinite(la_t).mean()),
                    "trap": "harmonic, EXP, spring constants 0.000625 (slater_det3_trapped's "
-                           "potential.arg), K P I P K per slice, one slice per launch"}
+                           "potential.arg), K P I P K per slice, the trap applied inside the "
+                           "FP64 x-chain kernel"}
         ens.set_trap(None)
     ens.run(seed, 0, 2)
     ctx.sync()
@@ -647,9 +650,10 @@ def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, con
                            "(src/utils/ludcmp.C:25-28 stops the program on such a row; the "
                            "batched LU reports it as a status and -inf instead); from t = 1 "
                            "on every value is finite",
-            "avg_ms_per_slice": {"evolution": t_evo / max(n_evo, 1),
+            "avg_ms_per_slice": {"evolution": t_evo / T,
                                  "pair_matrices_and_gather": meas_ms,
                                  "lu_2N": t_lu / max(n_meas, 1)},
+            "evolution_launches_per_run": n_evo,
             "pair_matrix_mfma": {"achieved_tflops": flops / (meas_ms * 1e-3) / 1e12,
                                  "peak_tflops": 2500.0,
                                  "frac": flops / (meas_ms * 1e-3) / 1e12 / 2500.0},

@@ -667,6 +667,10 @@ struct nrc_ensemble {
   double2* Mp = nullptr;     // pair: [C][N][N] pair matrices
   float* pair_work = nullptr;  // pair, FP32: split-BF16x3 partial sums
   float* pws = nullptr;      // pair, FP32 partners: sg(w) sqrt|w| of both pw tables
+  // dense or pair sinks on the FP32 x-chain: the folded k-space states after the slices of a
+  // launch but its last, [SLICE_FUSE_MAX - 1][C][nvo][V] FP32 complex, measured after it
+  void* snap = nullptr;
+  size_t snap_cap = 0;       // bytes
   bool pws_neg = false;      // some pair weight is negative
   void* work = nullptr;      // reference + smear: [C][V] complex scratch
   void* rfield = nullptr;    // reference + smear: [C][V] real field
@@ -746,7 +750,7 @@ static bool ens_in_batch(const nrc_ensemble* e) { return e->stream != e->ctx->st
 static void ens_free(nrc_ensemble* e) {
   void* bufs[] = {e->psi, e->tab_x, e->tab_k, e->sink_k, e->lsrc, e->pw, e->popp, e->Mp,
                   e->sinks_mx,
-                  e->pair_work, e->pws, e->sinks_k, e->tab_yz, e->smear,
+                  e->pair_work, e->pws, e->snap, e->sinks_k, e->tab_yz, e->smear,
                   e->sinks, e->ovl_work, e->work, e->Mx,
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
@@ -1312,16 +1316,21 @@ static int ens_lu(nrc_ensemble* e, const double2* M, size_t row) {
 // sinks) and the LU log-det (production/slater_det/main.C:206-215); in pair mode also the
 // pair matrices of the stored orbitals (TwoBody::Run for every (s, t),
 // src/slater_det/slater_det3.C:115-122)
-static int ens_measure(nrc_ensemble* e, int t) {
+// psi_in: measure that folded k-space state instead of the stored one (the states a launch
+// of several slices wrote for the slices inside it)
+static int ens_measure(nrc_ensemble* e, int t, const void* psi_in = nullptr) {
   hipStream_t s = e->stream;
   const int N = e->d.n_orb, prec = e->d.prec;
   const long C = e->d.n_configs, V = e->V;
   const size_t per_t = (size_t)C * e->nm;
+  void* const psi = psi_in ? const_cast<void*>(psi_in) : e->psi;
+  const bool folded = psi_in ? true : e->folded;
+  const bool pending_x = psi_in ? false : e->pending_x;
   {
     EvScope sc(e, 4);
     // folded k-space state xi_half psi: sinks carry the missing K(1/2); a state with its
     // x-DFT(+1) pending: the sinks transformed by X- as well
-    const void* sk = e->pending_x ? e->sinks_mx : e->folded ? e->sinks_k : e->sinks;
+    const void* sk = pending_x ? e->sinks_mx : folded ? e->sinks_k : e->sinks;
     if (e->sinks && e->sym) {
       // conjugate partners: the sinks and their mirrors against the stored volumes, then the
       // N x N matrix (a partner's column is the conjugate of its mirrored-sink overlaps)
@@ -1332,13 +1341,13 @@ static int ens_measure(nrc_ensemble* e, int t) {
       double2* Mp = e->Mx;
       double2* Mm = e->Mx + (size_t)C * N * nv;
       const void* skm = (const char*)sk + (size_t)N * V * esz;
-      const void* pm = (const char*)e->psi + (size_t)v0 * V * esz;
+      const void* pm = (const char*)psi + (size_t)v0 * V * esz;
       if (prec == NRC_PREC_F64) {
-        NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, Mp, N, nv, V, (int)C, s));
+        NRC_HIP(nrc::launch_overlap_dense(prec, sk, psi, Mp, N, nv, V, (int)C, s));
         NRC_HIP(nrc::launch_overlap_dense(prec, skm, pm, Mm, N, nm, V, (int)C, s, nv * V));
       } else {
         const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
-        NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)Mp, N, nv, V, (int)C, e->ovl_work, s, terms,
+        NRC_TRY(overlap_bf16x3(sk, psi, (double*)Mp, N, nv, V, (int)C, e->ovl_work, s, terms,
                                overlap_variant(e->ctx, N, nv, V, terms)));
         NRC_TRY(overlap_bf16x3(skm, pm, (double*)Mm, N, nm, V, (int)C, e->ovl_work, s, terms,
                                overlap_variant(e->ctx, N, nm, V, terms), nv * V));
@@ -1346,10 +1355,10 @@ static int ens_measure(nrc_ensemble* e, int t) {
       NRC_HIP(nrc::launch_sink_expand(Mp, Mm, e->col_vol, e->col_conj, e->M, N, nv, nm, v0, C,
                                       s));
     } else if (e->sinks && prec == NRC_PREC_F64)
-      NRC_HIP(nrc::launch_overlap_dense(prec, sk, e->psi, e->M, N, N, V, (int)C, s));
+      NRC_HIP(nrc::launch_overlap_dense(prec, sk, psi, e->M, N, N, V, (int)C, s));
     else if (e->sinks) {
       const int terms = e->acc != NRC_ACC_FP32 ? 6 : 3;
-      NRC_TRY(overlap_bf16x3(sk, e->psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s, terms,
+      NRC_TRY(overlap_bf16x3(sk, psi, (double*)e->M, N, N, V, (int)C, e->ovl_work, s, terms,
                              overlap_variant(e->ctx, N, N, V, terms)));
     }
     else {
@@ -1360,39 +1369,39 @@ static int ens_measure(nrc_ensemble* e, int t) {
         cp.msite = e->msite;
         cp.nvo = e->nvo;
       }
-      NRC_HIP(nrc::launch_gather_sites(prec, e->psi, e->site_of_orb,
-                                       e->folded ? e->sink_k : nullptr, e->M, C, N, V, s, cp));
+      NRC_HIP(nrc::launch_gather_sites(prec, psi, e->site_of_orb,
+                                       folded ? e->sink_k : nullptr, e->M, C, N, V, s, cp));
     }
     if (e->pair && e->sym) {  // G (mirrored sites) and H (conjugated) of the stored volumes
-      const double* w = e->pw + (e->folded ? V : 0);
+      const double* w = e->pw + (folded ? V : 0);
       const int nv = e->nvo;
       double2* G = e->pgh;
       double2* H = e->pgh + (size_t)C * nv * nv;
       if (prec == NRC_PREC_F64) {
-        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, G, nv, nv, V, (int)C, s));
-        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, nullptr, H, nv, nv, V, (int)C, s));
+        NRC_HIP(nrc::launch_pair_dense(prec, psi, psi, w, e->popp, G, nv, nv, V, (int)C, s));
+        NRC_HIP(nrc::launch_pair_dense(prec, psi, psi, w, nullptr, H, nv, nv, V, (int)C, s));
       } else if (nv <= 48) {  // both products in one pass
         // the mirror form reads each site and its mirror together (pair_opp(p) = -p and an
         // even w are conditions of partners; L a multiple of 4 and at least 8)
         const int Lm = (e->d.L % 4 == 0 && e->d.L >= 8) ? e->d.L : 0;
-        NRC_HIP(nrc::launch_pair_gh_bf16x3((const float2*)e->psi, w, e->popp, G, H, nv, V,
+        NRC_HIP(nrc::launch_pair_gh_bf16x3((const float2*)psi, w, e->popp, G, H, nv, V,
                                            (int)C, e->pair_work, s, e->pws ? Lm : 0,
-                                           e->pws ? e->pws + (e->folded ? V : 0) : nullptr,
+                                           e->pws ? e->pws + (folded ? V : 0) : nullptr,
                                            e->pws_neg));
       } else {
-        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)psi, (const float2*)psi, w,
                                         e->popp, G, nv, nv, V, (int)C, e->pair_work, s));
-        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)psi, (const float2*)psi, w,
                                         nullptr, H, nv, nv, V, (int)C, e->pair_work, s));
       }
       NRC_HIP(nrc::launch_pair_expand(G, H, e->col_vol, e->col_conj, e->Mp, N, nv, C, s));
     } else if (e->pair) {
-      const double* w = e->pw + (e->folded ? V : 0);
+      const double* w = e->pw + (folded ? V : 0);
       if (prec == NRC_PREC_F64)
-        NRC_HIP(nrc::launch_pair_dense(prec, e->psi, e->psi, w, e->popp, e->Mp, N, N, V,
+        NRC_HIP(nrc::launch_pair_dense(prec, psi, psi, w, e->popp, e->Mp, N, N, V,
                                        (int)C, s));
       else
-        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)e->psi, (const float2*)e->psi, w,
+        NRC_HIP(nrc::launch_pair_bf16x3((const float2*)psi, (const float2*)psi, w,
                                         e->popp, e->Mp, N, N, V, (int)C, e->pair_work, s));
     }
   }
@@ -1545,6 +1554,11 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.g.M = e->Mx;
   a.g.sink_scale = e->sink_k;  // not on the sources (t = 0): the kernel drops it there
   a.m_stride = C * N * N;
+  if (!gather && S > 1) {  // the states inside the launch, for ens_measure
+    if (!e->snap) return NRC_ERR_ARG;
+    a.snap = e->snap;
+    a.snap_stride = C * e->nvo * V;
+  }
   {
     EvScope sc(e, 0);
     NRC_HIP(nrc::launch_slicex(a, e->stream));
@@ -1567,6 +1581,21 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
 static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
   if (T < 1) return NRC_ERR_ARG;
   hipSetDevice(e->ctx->device);
+  // dense or pair sinks on the FP32 x-chain: room for the states inside a launch of several
+  // slices (without it, or where it does not fit, such runs take one slice per launch)
+  if ((e->sinks || e->pair) && e->d.prec == NRC_PREC_F32 && e->acc != NRC_ACC_FP32 &&
+      e->d.split == NRC_SPLIT_KSPACE && !e->smear && nrc::slicex_supported(e->d.L) && T > 1 &&
+      e->fuse > 1) {
+    const size_t need = (size_t)(nrc::SLICE_FUSE_MAX - 1) * e->d.n_configs * e->nvo * e->V * 8;
+    if (need > e->snap_cap) {
+      NRC_HIP(hipStreamSynchronize(e->stream));
+      hipFree(e->snap);
+      e->snap = nullptr;
+      e->snap_cap = 0;
+      if (hipMalloc(&e->snap, need) == hipSuccess) e->snap_cap = need;
+      else (void)hipGetLastError();
+    }
+  }
   const size_t slice_words = (size_t)e->d.n_configs * e->nw;
   const size_t need_words = words_host ? slice_words * T : slice_words * nrc::SLICE_FUSE_MAX;
   if (need_words > e->words_cap) {
@@ -1667,15 +1696,20 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   if ((e->sinks || e->pair) && !single) return NRC_ERR_UNSUPPORTED;
   if (ens_xchain(e, single)) {  // FP32 state, FP64 x-chain, measurements inside the launches
     e->pending_x = false;
+    const bool outside = e->sinks || e->pair;  // dense or pair sinks: measured outside
+    const size_t snap_need = (size_t)(nrc::SLICE_FUSE_MAX - 1) * C * e->nvo * V * 8;
+    const int fo = outside && (e->snap == nullptr || e->snap_cap < snap_need) ? 1 : e->fuse;
     for (int t = 0; t < T;) {
-      const int S = (e->sinks || e->pair) ? 1 : std::min(e->fuse, T - t);
-      if (e->sinks || e->pair) NRC_TRY(ens_measure(e, t));  // dense or pair sinks: before
+      const int S = std::min(fo, T - t);
+      if (outside) NRC_TRY(ens_measure(e, t));  // the state the launch starts from
       if (!host_words) {
         EvScope sc(e, 3);
         for (int k = 0; k < S; ++k) NRC_HIP(refresh(t + k, e->words + (size_t)k * slice_words));
       }
       NRC_TRY(ens_evolve_x(e, t, S, host_words ? field_of(t) : e->words, slice_words,
                            t + S == T));
+      for (int k = 1; outside && k < S; ++k)  // the states the launch wrote for t + 1 ..
+        NRC_TRY(ens_measure(e, t + k, (const char*)e->snap + (size_t)(k - 1) * C * e->nvo * V * 8));
       t += S;
     }
     return NRC_OK;

@@ -130,6 +130,10 @@ struct SliceXArgs {
   long m_stride = 0;
   int tile = 0;  // x-pencils per thread (0: the shape's default; slicex_tiles)
   int zd = 1;    // z rows in FP64 where the shape allows it (0: FP32 z rows)
+  // with nslices > 1: the folded k-space state after each slice but the last, FP32 volumes
+  // at snap + k snap_stride (k = 0 .. nslices - 2, snap_stride complex elements apart)
+  void* snap = nullptr;
+  long snap_stride = 0;
 };
 bool slicex_supported(int L);
 int slicex_tiles(int L, int* tiles);  // the tiles a shape offers, default first; returns count

@@ -85,7 +85,10 @@ __device__ __forceinline__ void mul_field_lds_q(C2<float>* w, const uint32_t* bl
   }
 }
 
-template <int L, int PPT, int SMAX, bool ZD, bool TRAP>
+// SNAP: after the x-DFT(+1) that finishes slice sl - 1 of the launch (sl >= 1), the folded
+// k-space state is also written to snap + (sl - 1) snap_stride (FP32, the volume layout), for
+// measurements made outside the launch (dense sinks, pair sinks)
+template <int L, int PPT, int SMAX, bool ZD, bool TRAP, bool SNAP>
 __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
   using C = C2<float>;
   using D = C2<double>;
@@ -150,6 +153,17 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
       // FP64 pencil is live at a time
       auto chain = [&](D* w, uint32_t yz) {
         if (!first) fft_reg<double, L, +1>(w);
+        if constexpr (SNAP) {
+          if (sl > 0) {
+            const rsrc_t rs = make_rsrc(reinterpret_cast<C*>(a.snap) + (sl - 1) * a.snap_stride +
+                                            vol * V,
+                                        (uint32_t)(V * sizeof(C)));
+#pragma unroll
+            for (int x = 0; x < L; ++x)
+              bstore<C>(C{(float)w[x].x, (float)w[x].y}, rs, yz * (uint32_t)sizeof(C),
+                        (uint32_t)(x * P * sizeof(C)));
+          }
+        }
         if (a.gather) gather_pencil<double, L>(w, g, (int)yz, cfg, orb, a.norb);
         const rsrc_t rk = half ? rkx : rkk;
 #pragma unroll
@@ -265,21 +279,27 @@ __global__ __launch_bounds__(L * L / PPT) void k_slicex(SliceXArgs a) {
   }
 }
 
-template <int L, int PPT, bool ZD, bool TRAP>
-static hipError_t launch_sx_t(const SliceXArgs& a, hipStream_t s) {
+template <int L, int PPT, bool ZD, bool TRAP, bool SNAP>
+static hipError_t launch_sx_s(const SliceXArgs& a, hipStream_t s) {
   constexpr int SMAX = SLICE_FUSE_MAX;
   static int per_cu = 0;
   if (per_cu == 0) {
     int nb = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nb, k_slicex<L, PPT, SMAX, ZD, TRAP>, L * L / PPT, 0);
+        &nb, k_slicex<L, PPT, SMAX, ZD, TRAP, SNAP>, L * L / PPT, 0);
     per_cu = (e != hipSuccess || nb < 1) ? 1 : nb;
   }
   long grid = (long)a.grid * per_cu;
   if (grid > a.nvol) grid = a.nvol;
-  hipLaunchKernelGGL((k_slicex<L, PPT, SMAX, ZD, TRAP>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((k_slicex<L, PPT, SMAX, ZD, TRAP, SNAP>), dim3((unsigned)grid),
                      dim3(L * L / PPT), 0, s, a);
   return hipGetLastError();
+}
+
+template <int L, int PPT, bool ZD, bool TRAP>
+static hipError_t launch_sx_t(const SliceXArgs& a, hipStream_t s) {
+  return a.snap && a.nslices > 1 ? launch_sx_s<L, PPT, ZD, TRAP, true>(a, s)
+                                 : launch_sx_s<L, PPT, ZD, TRAP, false>(a, s);
 }
 
 // ZD: the shape may run its z rows in FP64 (a.zd chooses); false: FP32 z rows only

@@ -115,18 +115,22 @@ def test_config3_plain_fp32_error_is_bounded(ctx, fp64_reference):
     assert np.median(rel) <= 1e-6
 
 
-def test_config3_dense_sinks_on_mfma_within_1e6_of_fp64(ctx, fp64_reference):
+@pytest.mark.parametrize("conj", [False, True])
+def test_config3_dense_sinks_on_mfma_within_1e6_of_fp64(ctx, fp64_reference, conj):
     """The precision form as written: the overlaps as a dense GEMM on the matrix cores
-    (split-BF16x6, FP32 accumulation) instead of the plane-wave gather, every orbital evolved
-    (dense sinks read every orbital's state), FP32 evolution with the FP64 x-chain, FP64 LU."""
+    (split-BF16x6, FP32 accumulation) instead of the plane-wave gather, FP32 evolution with the
+    FP64 x-chain four slices per launch (the states inside a launch written for the GEMM),
+    FP64 LU; every orbital evolved, or conjugate partners (the stored volumes projected on the
+    sinks and on their conjugate mirrors)."""
     import nrcps_amd as na
 
-    print("[gpu] FP32 with dense plane-wave sinks (BF16x6 GEMM)", flush=True)
+    print(f"[gpu] FP32 with dense plane-wave sinks (BF16x6 GEMM), partners={conj}", flush=True)
     moms = _lowest(L, N)
     V = L ** 3
     sinks = np.zeros((N, V), np.complex128)
     sinks[np.arange(N), (moms[:, 0] * L + moms[:, 1]) * L + moms[:, 2]] = 1.0
     e = na.Ensemble(ctx, L, moms, _xi_half(L), 0.4, NCONF, prec=na.PREC_F32, sinks=sinks)
+    e.set_symmetry(conj)
     la, ph, _ = e.run(SEED, 0, T)
     e.close()
     m64, c64 = _corr(*fp64_reference)
