@@ -2251,9 +2251,11 @@ extern "C" int nrc_ensemble_tune(nrc_ensemble* e, int T_probe, int* path_out,
     }
   }
   e->path = best;
-  // slices per launch: only where the chosen path fuses slices into one launch
-  const bool fuses = rc == NRC_OK && !e->sinks && !e->pair && best == NRC_PATH_SINGLE_PASS &&
-                     e->d.split == NRC_SPLIT_KSPACE && !e->smear;
+  // slices per launch: only where the chosen path fuses slices into one launch (dense and
+  // pair sinks: on the FP32 x-chain, which writes the states inside a launch for them)
+  const bool fuses = rc == NRC_OK && best == NRC_PATH_SINGLE_PASS &&
+                     e->d.split == NRC_SPLIT_KSPACE && !e->smear &&
+                     (!(e->sinks || e->pair) || ens_xchain(e, true));
   if (fuses) {
     const int Tf = std::max(T_probe, nrc::SLICE_FUSE_MAX);
     float base = -1.f;
