@@ -202,7 +202,12 @@ typedef struct nrc_ensemble_desc {
                           for the k-space split, x-space for the x-space split).  When set,
                           overlaps are dense products (split-BF16x3 on the matrix cores for
                           FP32, FP64 otherwise) instead of the plane-wave gather; single-pass
-                          path only. */
+                          path only.  The ensemble keeps the sinks and their conjugate mirrors
+                          (conj phi(-p) in k-space), 2 n_orb vectors, for conjugate partners.
+                          Dense and pair sinks on the FP32 x-chain run several slices per launch
+                          and allocate, at the first run, room for the states inside a launch:
+                          3 x n_configs x volumes x V complex FP32 (without it, one slice per
+                          launch). */
   const double* pair_w; /* optional V real pair wave function w(p) (k-space split, single-pass
                            path).  When set, each measurement is SlaterDet3's
                            (production/slater_det3_trapped/main-cum.C:217-238): the pair
@@ -240,9 +245,11 @@ int nrc_ensemble_set_path(nrc_ensemble* ens, int path);
  * rounding) from (N + self-conjugate orbitals) / 2 volumes per configuration.  The reference
  * evolves every orbital (src/propagator/propagator.C:35-64); this has no counterpart there.
  * Pair sinks: the pair matrices come from two products over the stored volumes (mirrored
- * sites, and the same sites conjugated); they need pair_opp(p) = -p and pair_w even.
- * NRC_ERR_UNSUPPORTED with dense sinks, or tables (kinetic, smear, pair_w) that are not even
- * to 1e-12 relative; not while the ensemble is in a batch; call before a run. */
+ * sites, and the same sites conjugated); they need pair_opp(p) = -p and pair_w even.  Dense
+ * sinks: the stored volumes are projected on the sinks and on their conjugate mirrors, since
+ * <phi | J psi> = conj <J phi | psi>.  NRC_ERR_UNSUPPORTED with tables (kinetic, smear,
+ * pair_w) that are not even to 1e-12 relative; not while the ensemble is in a batch; call
+ * before a run. */
 int nrc_ensemble_set_symmetry(nrc_ensemble* ens, int conj);
 /* Trap (k-space split): every slice the x-space multiplier of the field becomes
  * (1 + A phi) q / V (or the smeared field times q), q = V real values (q = NULL removes the
