@@ -587,6 +587,48 @@ def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012
             "timing": "HIP events around the overlap kernels of each measurement"}
 
 
+def fp64_config2(ctx, na, L=16, N=7, T=64, C=2048, tau=0.05, A=0.4, seed=7012, threads=0):
+    """The plain FP64 GPU path at the FP64 parity workload (L=16, N=7 plane waves, T=64, 2048
+    configurations, Philox Z2 fields): configs*slices/s with every orbital evolved (the
+    reference's layout) and with conjugate partners, the largest |d log|det|| of the first
+    configurations against the FP64 oracle on the same fields, and (threads > 0) the oracle's
+    own rate at this shape on that many host cores, one configuration stream per core."""
+    import oracle
+
+    moms, kin = _lowest(L, N), _tables(L, tau, 1.0)
+    rec = {"workload": f"L={L}, N={N}, T={T}, tau={tau}, A={A}, {C} configurations, FP64 "
+                       f"evolution, plane-wave sinks gathered, FP64 LU",
+           "unit": "configs*slices/s"}
+    la0 = None
+    for conj in (False, True):
+        e = na.Ensemble(ctx, L, moms, kin, A, C, prec=na.PREC_F64, split=na.SPLIT_KSPACE)
+        e.set_symmetry(conj)
+        e.run(seed, 1 << 20, 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        la, _, _ = e.run(seed, 0, T)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        e.close()
+        rec["value_conj_partners" if conj else "value"] = C * T / dt
+        if not conj:
+            la0 = la
+    oracle.lib()
+    err = 0.0
+    for c in range(2):
+        ref = oracle.run_config(L, N, T, moms, kin, A=A, seed=seed, config=c)
+        err = max(err, float(np.abs(la0[:, c, 0] - ref["logabs"][:, 0]).max()))
+    rec["max_abs_diff_logdet_vs_oracle_first_2_configs"] = err
+    if threads > 0:
+        secs = oracle.time_pool(L, N, 2, T, moms, kin, A, nthreads=threads, reps=1,
+                                config=1 << 20)
+        rec["cpu_oracle"] = {"value": threads * 2 * T / float(secs.min()), "cores": threads,
+                             "sample": f"{threads} streams x 2 configs x {T} slices, FP64, "
+                                       f"every orbital"}
+        rec["speedup_vs_cpu_oracle"] = rec["value"] / rec["cpu_oracle"]["value"]
+    return rec
+
+
 def pair_sinks(ctx, na, L=32, N=35, T=32, C=256, tau=0.05, A=0.4, seed=7012, conj=True,
                trap=True):
     """The pair-sink measurement of production/slater_det3_trapped (L=32, N=35): every slice
@@ -744,6 +786,8 @@ def main():
                     help="skip the run of the workload in the other conjugate-partner mode")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense-sink (MFMA overlap) measurement")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 record at L=16, N=7, T=64, 2048 configurations")
     ap.add_argument("--no-pair", action="store_true",
                     help="skip the pair-sink (SlaterDet3) measurement")
     ap.add_argument("--precision-configs", type=int, default=1024,
@@ -897,6 +941,15 @@ def main():
               f"{dense['overlap_ms_per_slice']:.3f} ms per slice, MFMA frac "
               f"{dense['mfma']['frac']:.3f}", file=sys.stderr, flush=True)
 
+    fp64c2 = None
+    if not args.no_fp64:
+        fp64c2 = fp64_config2(ctx, na, threads=0 if args.no_cpu else (
+            args.cpu_threads or min(16, os.cpu_count() or 1)))
+        print(f"FP64 L=16 N=7: {fp64c2['value']:.0f} configs*slices/s "
+              f"({fp64c2['value_conj_partners']:.0f} with partners), log|det| within "
+              f"{fp64c2['max_abs_diff_logdet_vs_oracle_first_2_configs']:.1e} of the oracle",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -923,6 +976,7 @@ def main():
         "pair_sinks": pair,
         "other_mode": other,
         "dense_overlap": dense,
+        "fp64_config2": fp64c2,
     }
     if cpu:
         out["speedup_vs_cpu"] = cs_per_s / cpu["value"]
