@@ -366,9 +366,11 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
                                                LuOpts opt) {
   extern __shared__ double2 a[];
   const int nm = minors ? n : 1;
+  // minors: the largest of every configuration dispatched first, the smallest last, so that
+  // the longest factorizations do not start at the end of the launch
   const long blk = blockIdx.x;
-  const long cfg = blk / nm;
-  const int m = (int)(blk - cfg * nm);
+  const int m = nm - 1 - (int)(blk / ncfg);
+  const long cfg = blk - (long)(nm - 1 - m) * ncfg;
   const int sz = minors ? m + 1 : n;
   const int i = threadIdx.x;
   const double2* Mc = M + cfg * (long)n * ldm;
