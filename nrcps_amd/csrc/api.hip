@@ -1555,7 +1555,7 @@ static int ens_evolve_x(nrc_ensemble* e, int t, int S, const uint32_t* words, si
   a.g.sink_scale = e->sink_k;  // not on the sources (t = 0): the kernel drops it there
   a.m_stride = C * N * N;
   if (!gather && S > 1) {  // the states inside the launch, for ens_measure
-    if (!e->snap) return NRC_ERR_ARG;
+    if (!e->snap || e->snap_cap < (size_t)(S - 1) * C * e->nvo * V * 8) return NRC_ERR_ARG;
     a.snap = e->snap;
     a.snap_stride = C * e->nvo * V;
   }
