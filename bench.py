@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+LDS_PEAK_GBS = 256 * 128 * 2.4  # LDS: 256 CUs x 128 bytes per clock x 2.4 GHz, GB/s
 VALU_SIMDS = 256 * 4  # MI355X: 256 CUs x 4 SIMDs
 VALU_CLOCK_HZ = 2.4e9  # peak engine clock
 
@@ -747,6 +748,16 @@ def evolution_roofline(L, N, T, C, V, nvo, args, tx, nx, tyz, nyz, tlu, nlu, tf,
                       "fields": tf / max(nf, 1), "gather": tg / max(ng, 1)},
            "bytes_per_cs": b_evo, "volumes_per_config": nvo,
            "timing": "HIP events on the ensemble's stream around every evolution launch"}
+    if kid == "k_slicex" and L == 32:
+        # LDS bytes per volume-slice of k_slicex at L = 32 (two x-pencils per thread, two
+        # rounds of 16 planes): the parked pencil written, swapped and restored (4 x V/2 x 8),
+        # per round the planes written and read back and the z rows, y columns and z rows read
+        # and written (8 x V/2 x 8 per round, two rounds), the field word of every site (4 V)
+        lds = (16 + 64 + 4) * V * C * nvo * per_launch
+        lds_gbs = lds / (launch_ms * 1e-3) / 1e9
+        rec["lds_algorithmic"] = {"bytes_per_launch": lds, "achieved_gbs": lds_gbs,
+                                  "peak_gbs": LDS_PEAK_GBS, "frac": lds_gbs / LDS_PEAK_GBS,
+                                  "peak_note": "128 bytes per clock per CU, 256 CUs, 2.4 GHz"}
     if cnt:
         hbm_cnt = cnt["traffic_bytes_per_launch"] / (launch_ms * 1e-3) / 1e9
         rec["hbm_counters"] = {"achieved_gbs": hbm_cnt, "frac": hbm_cnt / HBM_PEAK_GBS}
