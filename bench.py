@@ -498,7 +498,15 @@ def learned_proposal(ctx, na, L=24, N=14, T=96, tau=0.05, A=0.4, n_train=16384, 
             "training": {"steps": steps, "batch": batch, "lr": lr,
                          "optimizer": "sgd-momentum" if optimizer == 1 else "adam",
                          "loss_first": float(losses[0]), "loss_last10": float(losses[-10:].mean()),
-                         "output_bias": float(ps[8240])}}
+                         "output_bias": float(ps[8240]),
+                         # the output bias starts at the fitted constant tilt, so the first
+                         # loss already holds what a constant output gains over q = 1/2:
+                         # KL(1/2 + d || 1/2) ~ 2 d^2 nats per site, d = tanh(lambda) / 2, a
+                         # drop in the seventh decimal for a 2x variance gain; training adds
+                         # only what the fields' spatial structure and u carry beyond it
+                         "loss_gain_of_the_fitted_tilt_over_q_half_per_site":
+                             2.0 * (0.5 * float(np.tanh(lam_fit))) ** 2,
+                         "loss_drop_in_training": float(losses[0] - losses[-10:].mean())}}
 
 
 def dense_overlap(ctx, na, L=32, N=33, T=128, C=1024, tau=0.05, A=0.4, seed=7012, ref64=None,
