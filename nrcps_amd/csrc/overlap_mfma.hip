@@ -951,9 +951,10 @@ __global__ void k_split_rows(const float2* phi, __bf16* out, long n, long vol, i
   }
 }
 
-// M[c][s][t] = sum over chunks (FP64) of rows 2s (re) and 2s + 1 (im), column t
+// M[c][s][t] = sum over chunks (FP64) of rows 2s (re) and 2s + 1 (im); column (c, t) is
+// column c ntp + t of the tiles of ntt 16 columns
 __global__ void k_overlap_rows_reduce(const float* partial, double2* M, int ns, int nt,
-                                      int nchunks, int mr, int ntt, long ncfg) {
+                                      long ntp, int nchunks, int mr, int ntt, long ncfg) {
   const long n = ncfg * ns * nt;
   const int ldp = ntt * 16;
   const long tile = (long)(mr * 16) * ldp;
@@ -961,9 +962,11 @@ __global__ void k_overlap_rows_reduce(const float* partial, double2* M, int ns, 
        i += (long)gridDim.x * blockDim.x) {
     const long c = i / ((long)ns * nt);
     const int r = (int)(i - c * ns * nt), s = r / nt, t = r - (r / nt) * nt;
+    const long col = c * ntp + t, tl = col / ldp;
+    const int j = (int)(col - tl * ldp);
     double re = 0.0, im = 0.0;
     for (int k = 0; k < nchunks; ++k) {
-      const float* p = partial + (c * nchunks + k) * tile + (long)(2 * s) * ldp + t;
+      const float* p = partial + (tl * nchunks + k) * tile + (long)(2 * s) * ldp + j;
       re += (double)p[0];
       im += (double)p[ldp];
     }
@@ -971,12 +974,42 @@ __global__ void k_overlap_rows_reduce(const float* partial, double2* M, int ns, 
   }
 }
 
-// partial tiles of the sink-row form: [c][chunk][2 ns rows padded to 16][nt padded to 16]
+// Column tiling of the sink-row form.  The columns (c, t) of all configurations form one
+// sequence, column c nt + t, cut into tiles of 16 NT columns, so that no tile carries padded
+// columns when nt is not a multiple of 16 (17 stored volumes per configuration of the
+// partnered 33-orbital benchmark would otherwise fill 17 of 32 columns).  A tile's orbitals
+// are addressed from one buffer resource at its first configuration; where that range could
+// pass 2^31 bytes (or for one configuration) every configuration starts a tile of its own
+// (ntp = 16 NT columns per configuration).
+struct OvlCols {
+  int nt_tiles;  // NT: 16-column tiles per wave
+  long ntp;      // column stride of a configuration
+  long ntiles;   // tiles of 16 NT columns
+};
+// shared tiles of 48 columns (three 16-column tiles per wave) where the row tiles leave the
+// registers for them (up to 7 row tiles, 112 sink rows), else 32; wider for nt > 48
+static int ovl_flat_nt(int nt, int mr) {
+  const int ntt = (nt + 15) / 16;
+  return ntt >= 3 ? ntt : (mr <= 7 ? 3 : 2);
+}
+static OvlCols ovl_cols(int nt, int mr, long vol, long ncfg, long cstride) {
+  const int ntt = (nt + 15) / 16, ntf = ovl_flat_nt(nt, mr);
+  const long span = (16L * ntf - 1) / nt + 2;  // configurations a tile can reach
+  if (ncfg > 1 && ((span - 1) * cstride + (long)nt * vol) * 8 <= 0x7FFF0000L) {
+    const long ncol = ncfg * nt, w = 16L * ntf;
+    return {ntf, (long)nt, (ncol + w - 1) / w};
+  }
+  return {ntt, 16L * ntt, ncfg};
+}
+
+// partial tiles of the sink-row form: [tile][chunk][2 ns rows padded to 16][16 NT], sized
+// for either tiling (the shared tiles hold at most ncfg nt + 63 columns) and non-decreasing in
+// nt, so a workspace sized for nt columns serves any smaller nt
 size_t overlap_rows_partial_floats(int ns, int nt, long vol, int ncfg) {
   const int mr = (2 * ns + 15) / 16, ntt = (nt + 15) / 16;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const long nchunks = (K + kc - 1) / kc;
-  return (size_t)ncfg * nchunks * (mr * 16) * (ntt * 16);
+  return (size_t)nchunks * (mr * 16) * ((long)ncfg * 16 * ntt + 64);
 }
 
 hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hipStream_t s,
@@ -993,16 +1026,16 @@ hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hip
 
 // ------------------------------------------------------------------ overlaps, shared sinks
 // The sink-row form with the sinks staged in LDS and shared by the four waves of a workgroup,
-// each wave one configuration over the same K chunk: the split sinks cross from L2 once per
-// four configurations instead of once per wave and step, which leaves the orbital stream
-// from HBM as the only large flow.  A stage is KS steps of 32 K values of every sink row and
+// each wave one tile of 16 NT orbital columns over the same K chunk: the split sinks cross
+// from L2 once per four tiles instead of once per wave and step, which leaves the orbital
+// stream from HBM as the only large flow.  A stage is KS steps of 32 K values of every sink row and
 // part; the next stage's sink loads go out before the current stage's products and are
 // written to the other LDS buffer after them (one barrier per stage).  LDS layout per part
 // and 8-value K group q: R rows of 16 bytes, plane stride R 16 + 16 bytes, so the 16 rows a
 // quarter-wave reads are one contiguous 256-byte run (no bank conflicts) and the loaders'
 // 16-byte writes from four consecutive rows of 16 K groups land in distinct banks.
 // Steps past the chunk end read zero from both operands and add nothing.  Grid (group of four
-// configurations, K chunk) with the group fastest, so that the workgroups resident at one time
+// column tiles, K chunk) with the group fastest, so that the workgroups resident at one time
 // read one chunk of the split sinks, which stays in L2.
 // Three forms (FORM): 0, stages of 2 steps with 2 steps of orbital loads in flight, small
 // enough for two workgroups per CU (two waves per SIMD hide each other's load latency);
@@ -1019,7 +1052,7 @@ template <int MR, int NT, int TERMS, int FORM>
 __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
     const __bf16* __restrict__ a0p, const __bf16* __restrict__ a1p,
     const __bf16* __restrict__ a2p, const float2* __restrict__ psi, float* __restrict__ partial,
-    int nrow, int nt, long vol, int ncfg, long kchunk, int nchunks, long cstride) {
+    int nrow, int nt, long ntp, long vol, int ncfg, long kchunk, int nchunks, long cstride) {
   constexpr int NPA = TERMS == 6 ? 3 : 2, R = MR * 16, KS = ovl_lds_ks<MR, TERMS, FORM>();
   constexpr int PD = FORM == 0 ? 2 : 4, Q = KS * 4;
   constexpr int SU = PD > KS ? PD / KS : 1;  // stages per loop iteration (static buffers)
@@ -1033,8 +1066,12 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int row = lane & 15, grp = lane >> 4;
-  const long c = (long)blockIdx.x * 4 + wave;
-  const bool live = c < ncfg;
+  // the wave's tile: columns col0 ... col0 + 16 NT - 1, column (c, t) = c ntp + t (t < nt)
+  const long tile = (long)blockIdx.x * 4 + wave;
+  const long ncol = (long)ncfg * ntp, col0 = tile * 16 * NT;
+  const bool live = col0 < ncol;
+  const long cf = live ? col0 / ntp : 0;  // the tile's first and last configurations
+  const long cl = live ? ((col0 + 16 * NT < ncol ? col0 + 16 * NT : ncol) - 1) / ntp : 0;
   const int chunk = blockIdx.y;
   const long K = 2 * vol;
   const long k0 = (long)chunk * kchunk;
@@ -1043,7 +1080,8 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
   const uint32_t abytes = (uint32_t)(nrow * K * 2);
   const rsrc_t ra[3] = {make_rsrc(a0p, abytes), make_rsrc(a1p, abytes),
                         make_rsrc(a2p, TERMS == 6 ? abytes : 0u)};
-  const rsrc_t rb = make_rsrc(psi + (live ? c : 0) * cstride, (uint32_t)(nt * vol * 8));
+  const rsrc_t rb = make_rsrc(psi + cf * cstride,
+                              live ? (uint32_t)(((cl - cf) * cstride + (long)nt * vol) * 8) : 0u);
   constexpr uint32_t kFar = 0x7FFFFFF0u;  // past any resource: loads return zero
   // the stage loader, part by part (the part selects the buffer resource, so it stays
   // uniform): chunk id = ((row / 4) Q + q) 4 + row % 4
@@ -1075,8 +1113,10 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
   uint32_t ob[NT];
 #pragma unroll
   for (int b = 0; b < NT; ++b) {
-    const int t = b * 16 + row;
-    ob[b] = (live && t < nt) ? (uint32_t)(t * vol + 4 * grp) * 8u : kFar;
+    const long col = col0 + b * 16 + row, cc = col / ntp, t = col - cc * ntp;
+    ob[b] = (live && col < ncol && t < nt)
+                ? (uint32_t)(((cc - cf) * cstride + t * vol + 4 * grp) * 8)
+                : kFar;
   }
   struct BStep {
     float4 b0[NT], b1[NT];
@@ -1148,7 +1188,7 @@ __global__ __launch_bounds__(256, FORM == 1 ? 1 : 2) void k_overlap_lds(
   }
   if (!live) return;
   constexpr int ldp = NT * 16;
-  float* out = partial + (c * nchunks + chunk) * (long)R * ldp;
+  float* out = partial + (tile * nchunks + chunk) * (long)R * ldp;
 #pragma unroll
   for (int a = 0; a < MR; ++a)
 #pragma unroll
@@ -1164,18 +1204,20 @@ hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2
   if (cstride == 0) cstride = nt * vol;
   if (ns < 1 || ns > 16 * OV_MAXT || nt < 1 || nt > 16 * OV_MAXT || (vol & 3))
     return hipErrorInvalidValue;
-  const int nrow = 2 * ns, mr = (nrow + 15) / 16, ntt = (nt + 15) / 16;
+  const int nrow = 2 * ns, mr = (nrow + 15) / 16;
+  const OvlCols cols = ovl_cols(nt, mr, vol, ncfg, cstride);
+  const int ntt = cols.nt_tiles;
   const long K = 2 * vol, kc = overlap_bf16x3_kchunk(vol);
   const int nchunks = (int)((K + kc - 1) / kc);
   const __bf16* a0 = (const __bf16*)rows_split;
   const long plane = (long)nrow * K;
   const __bf16* a1 = a0 + plane;
   const __bf16* a2 = a1 + plane;
-  const dim3 grid((unsigned)((ncfg + 3) / 4), (unsigned)nchunks), block(256);
+  const dim3 grid((unsigned)((cols.ntiles + 3) / 4), (unsigned)nchunks), block(256);
   bool launched = false;
 #define NRC_OL_T(A, B, T, F)                                                                  \
   hipLaunchKernelGGL((k_overlap_lds<A, B, T, F>), grid, block, 0, s, a0, a1, T == 6 ? a2 : a1,  \
-                     psi, partial, nrow, nt, vol, ncfg, kc, nchunks, cstride)
+                     psi, partial, nrow, nt, cols.ntp, vol, ncfg, kc, nchunks, cstride)
 #define NRC_OL_CASE(A, B)                                                                     \
   if (!launched && mr == A && ntt == B) {                                                     \
     launched = true;                                                                          \
@@ -1200,7 +1242,7 @@ hipError_t launch_overlap_lds(const void* rows_split, const float2* psi, double2
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_overlap_rows_reduce, dim3((unsigned)blocks), dim3(256), 0, s, partial, M,
-                     ns, nt, nchunks, mr, ntt, (long)ncfg);
+                     ns, nt, cols.ntp, nchunks, mr, ntt, (long)ncfg);
   return hipGetLastError();
 }
 
