@@ -122,8 +122,9 @@ int nrc_overlap_noalloc(nrc_context* ctx, const void* phi_dev, const void* psi_d
  *   NRC_OVL_COLS        a real and an imaginary B column per orbital, built in registers,
  *                       four waves per configuration;
  *   NRC_OVL_SHARED      two A rows per sink ((re, im) and (-im, re)), split once per call
- *                       and staged in LDS for the four configurations of a workgroup (one
- *                       wave each): 2-step stages, two workgroups per CU;
+ *                       and staged in LDS for the four waves of a workgroup, each wave a
+ *                       tile of 48 orbital columns (32 above 56 sinks) taken across
+ *                       configurations: 2-step stages, two workgroups per CU;
  *   NRC_OVL_SHARED_DEEP the same with 4-step stages and 4 steps of orbital loads in flight,
  *                       one workgroup per CU;
  *   NRC_OVL_SHARED_PIPE 2-step stages with 4 steps of orbital loads in flight, two
