@@ -85,6 +85,16 @@ def cpu_baseline(L, N, tau, A, configs, slices, threads, reps=3):
     best = float(secs.min())
     cal = fft_calibration(L, N)
     value = threads * configs * slices / best
+    # the plane-wave-gather variant (overlaps read at the sink momenta), reported beside the
+    # baseline, not as it: one repetition of the same sample
+    gsecs = oracle.time_pool(L, N, configs, slices, moms, kin, A, nthreads=threads, reps=1,
+                             config=1 << 20, gather=True)
+    gather = {"value": threads * configs * slices / float(gsecs.min()),
+              "unit": "configs*slices/s",
+              "sample": f"the same {threads} streams x {configs} configs x {slices} slices, one "
+                        f"repetition ({float(gsecs.min()):.1f} s)",
+              "note": "plane-wave sinks measured by reading psi_t at the sink momenta instead of "
+                      "N^2 full-length products (the same overlaps); not the reference's path"}
     visible = os.cpu_count() or 1
     try:
         affinity = len(os.sched_getaffinity(0))
@@ -102,7 +112,8 @@ def cpu_baseline(L, N, tau, A, configs, slices, threads, reps=3):
             "per_core_value": value / threads,
             "value_if_fft_at_numpy_speed": value * cal["baseline_scale_with_numpy_fft"],
             "value_all_visible_cpus_linear": value / threads * visible,
-            "fft_calibration": cal}
+            "fft_calibration": cal,
+            "gather_variant": gather}
 
 
 def fft_calibration(L, N, reps=3):

@@ -42,7 +42,7 @@ class RunDesc(C.Structure):
                 ("seed", C.c_uint64), ("config", C.c_uint32), ("rng", C.c_void_p),
                 ("phi", C.c_void_p), ("pair_w", C.c_void_p), ("pair_opp", C.c_void_p),
                 ("sources", C.c_void_p), ("pot", C.c_void_p), ("phi_real", C.c_void_p),
-                ("sinks", C.c_void_p), ("norms", C.c_void_p)]
+                ("sinks", C.c_void_p), ("norms", C.c_void_p), ("gather", C.c_int)]
 
 
 def build(force: bool = False) -> None:
@@ -411,7 +411,7 @@ def gauss_field(rng: Ranlxs, vol: int, shift: float = 0.0) -> np.ndarray:
 def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smear=None,
                field="philox", seed=7012, config=0, rng=None, phi=None, want_M=False,
                want_psi=False, pair_w=None, pair_opp=None, sources=None, pot=None,
-               phi_real=None, sinks=None, norms=None):
+               phi_real=None, sinks=None, norms=None, gather=False):
     """One configuration through T slices.  Returns dict of logabs [T, nm], phase [T, nm]
     (complex), argsum [T, nm], optional M [T, N, N] (the pair matrix with pair_w) and
     psi [N, V].  With pair_w / pair_opp: nm = 2N, SlaterDet3's determinants (2n + 1: leading
@@ -429,6 +429,7 @@ def run_config(L, N, T, moms, kin, A=0.4, split=SPLIT_KSPACE, minors=False, smea
     d.moms = moms.ctypes.data
     d.kin = kin.ctypes.data
     d.A = A
+    d.gather = int(gather)
     sm = None
     if smear is not None:
         sm = np.ascontiguousarray(smear, dtype=np.float64)
@@ -503,14 +504,16 @@ def time_threads(L, N, slices, moms, kin, A=0.4, split=SPLIT_KSPACE, nthreads=1,
 
 
 def time_pool(L, N, configs, slices, moms, kin, A=0.4, split=SPLIT_KSPACE, nthreads=1, reps=3,
-              seed=7012, config=0):
+              seed=7012, config=0, gather=False):
     """The CPU-baseline protocol (orc_time_pool): `nthreads` streams, a one-slice warm-up
     each, then `reps` barrier-synchronised repetitions of `configs` configurations x `slices`
-    slices per stream.  Returns the wall seconds of every repetition."""
+    slices per stream.  Returns the wall seconds of every repetition.  gather: the overlaps
+    read at the sink momenta instead of the reference's N^2 full-length products."""
     moms = np.ascontiguousarray(moms, dtype=np.int32).reshape(-1)
     kin = np.ascontiguousarray(kin, dtype=np.float64)
     d = RunDesc()
     d.L, d.N, d.T, d.split, d.minors = L, N, slices, split, 0
+    d.gather = int(gather)
     d.moms = moms.ctypes.data
     d.kin = kin.ctypes.data
     d.A = A

@@ -906,7 +906,17 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
 
   for (int t = 0; t < T; ++t) {
     /* measure: SlaterDet ctor + Run(n) */
-    orc_overlap(sink, psi, M, N, N, V);
+    if (d->gather && !xspace && !d->sources && !d->sinks) {
+      for (int s = 0; s < N; ++s) {
+        const size_t js = ((size_t)d->moms[3 * s] * L + d->moms[3 * s + 1]) * L + d->moms[3 * s + 2];
+        for (int q = 0; q < N; ++q) {
+          M[2 * (s * N + q)] = psi[2 * ((size_t)q * V + js)];
+          M[2 * (s * N + q) + 1] = psi[2 * ((size_t)q * V + js) + 1];
+        }
+      }
+    } else {
+      orc_overlap(sink, psi, M, N, N, V);
+    }
     if (pair) orc_pair_overlap(psi, psi, d->pair_w, d->pair_opp, Mp, N, N, V);
     if (M_out)
       memcpy(M_out + 2 * (size_t)t * N * N, pair ? Mp : M, 2 * (size_t)N * N * sizeof(double));

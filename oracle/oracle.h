@@ -179,6 +179,11 @@ typedef struct {
   const double* norms;    /* optional T values: after slice t the orbitals are divided by
                              norms[t] (Propagator::Normalize(Float), src/alg/propagator.C:
                              116-133) */
+  int gather;             /* 1, with plane-wave sinks and sources in the k-space split: each
+                             overlap <k_s|psi_t> is read from psi_t at k_s instead of the
+                             N^2 full-length products of the reference (one_body_project.C:
+                             20-23); the values are the same, the products adding exact
+                             zeros.  Not the baseline: a timing variant */
 } orc_run_desc;
 
 /* Random::Gauss(sigma) (src/random/random.C:105-116): the polar Box-Muller method on the
