@@ -450,21 +450,41 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
   // swap exchanges two positions), and the rows already pivoted take no further part.
   int pos = i;
   bool done = i >= sz;
+  // The pivot search by squared magnitudes: b2 = vv^2 (x^2 + y^2) is within a few ulps of
+  // (vv |z|)^2, so a row whose b2 lies below (1 - 2^-40) of the largest has vv ref_abs(z)
+  // below that row's, rounding included.  When exactly one row reaches that band, and every
+  // square and product lies in [2^-900, 2^900] (no underflow or overflow can reorder them),
+  // that row is the reference's pivot; otherwise the step compares vv ref_abs(z) exactly.
+  // |pivot| itself is only needed for log|det|, after the elimination.
+  const double vv2 = vv * vv;
+  const bool fast_rows = !__any(i < sz && !(vv2 >= 0x1p-900 && vv2 <= 0x1p+900));
   if (!bad) {
     for (int k = 0; k < sz; ++k) {
       // the pivot: the largest vv |a_ik| over the rows at positions >= k, the last position
       // among equals (the reference's >=): the wavefront maximum, then among the lanes holding
       // it the highest position
       const double2 z = done ? make_double2(0.0, 0.0) : a[k * sz + i];
-      const double h = done ? 0.0 : cabs2(z);
-      const double bv = done ? -1.0 : vv * h;
-      const double mx = __ockl_wfred_max_f64(bv);
-      const bool hit = !done && bv == mx;
-      const unsigned long long tie = __ballot(hit);
-      int p = tie ? __ffsll((long long)tie) - 1 : 0;
-      if (__popcll(tie) > 1) {
-        const int best = __ockl_wfred_max_i32(hit ? pos : -1);
-        p = __ffsll((long long)__ballot(hit && pos == best)) - 1;
+      const double m2 = z.x * z.x + z.y * z.y;
+      const bool zero = z.x == 0.0 && z.y == 0.0;
+      const bool in_range = done || zero || (m2 >= 0x1p-900 && m2 <= 0x1p+900);
+      const double b2 = done ? -1.0 : vv2 * m2;
+      const double mx2 = __ockl_wfred_max_f64(b2);
+      const unsigned long long cand = __ballot(!done && b2 >= mx2 * (1.0 - 0x1p-40));
+      int p;
+      if (fast_rows && mx2 >= 0x1p-900 && mx2 <= 0x1p+900 && __popcll(cand) == 1 &&
+          !__any(!in_range)) {
+        p = __ffsll((long long)cand) - 1;
+      } else {
+        const double h = done ? 0.0 : cabs2(z);
+        const double bv = done ? -1.0 : vv * h;
+        const double mx = __ockl_wfred_max_f64(bv);
+        const bool hit = !done && bv == mx;
+        const unsigned long long tie = __ballot(hit);
+        p = tie ? __ffsll((long long)tie) - 1 : 0;
+        if (__popcll(tie) > 1) {
+          const int best = __ockl_wfred_max_i32(hit ? pos : -1);
+          p = __ffsll((long long)__ballot(hit && pos == best)) - 1;
+        }
       }
       const int imax = __builtin_amdgcn_readlane(pos, p);
       if (i == k) my_indx = imax;
@@ -477,16 +497,11 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
         done = true;
       }
       double2 piv = make_double2(readlane_d(z.x, p), readlane_d(z.y, p));
-      double mag = readlane_d(h, p);
-      if (mag == 0.0) {
+      if (piv.x == 0.0 && piv.y == 0.0) {  // |pivot| = 0 exactly
         piv = make_double2(1.0e-20, 0.0);
-        mag = 1.0e-20;
         if (i == p) a[k * sz + p] = piv;
       }
-      if (i == k) {
-        my_piv = piv;
-        my_mag = mag;
-      }
+      if (i == k) my_piv = piv;
       const double2 r = crecip(piv);
       if (!done) {
         const double2 l = cmuld(z, r);
@@ -520,6 +535,7 @@ __global__ __launch_bounds__(64) void k_lu_det(const double2* __restrict__ M, in
       __syncthreads();
     }
     if (i < sz) {
+      my_mag = cabs2(my_piv);
       la = log(my_mag);
       ph = make_double2(my_piv.x / my_mag, my_piv.y / my_mag);
       as = atan2(my_piv.y, my_piv.x);

@@ -245,6 +245,34 @@ def test_ludcmp_bit_exact_ties_and_scales(ctx, orc, n):
             assert la[c, m] == rla or abs(la[c, m] - rla) < 1e-12 * max(1.0, abs(rla)), (c, m)
 
 
+@pytest.mark.parametrize("n", [5, 33])
+def test_ludcmp_bit_exact_near_ties(ctx, orc, n):
+    """Pivot candidates a few ulps apart: every row has the same largest entry (so the same
+    scale vv), and the pivot columns hold values that differ in the last bits of |z| (real,
+    imaginary and mixed phases).  The squared-magnitude pivot search must hand such steps to
+    the exact comparison; factors and pivots bit for bit against the oracle."""
+    print(f"[gpu] ludcmp near ties n={n}", flush=True)
+    rng = np.random.default_rng(500 + n)
+    ncfg = 24
+    Ms = np.empty((ncfg, n, n), complex)
+    ulp = 2.0 ** -52
+    for c in range(ncfg):
+        m = 0.3 * _rand_c(rng, (n, n))
+        m[:, n - 1] = 4.0  # the row maximum of every row: vv = 1/4 for all rows
+        for col in range(min(3, n - 1)):
+            k = rng.permutation(n).astype(float)
+            ph = np.exp(1j * rng.uniform(0, 2 * np.pi)) if c % 3 == 2 else (1j if c % 3 else 1)
+            m[:, col] = (1.0 + k * ulp * (c % 4 + 1)) * ph
+        Ms[c] = m
+    lu, indx, d, st = ctx.ludcmp(Ms)
+    assert np.all(st == 0)
+    for c in range(ncfg):
+        rlu, rindx, rd = orc.ludcmp(Ms[c])
+        assert np.array_equal(indx[c], rindx), c
+        assert d[c] == rd, c
+        assert np.array_equal(lu[c], rlu), (c, np.abs(lu[c] - rlu).max())
+
+
 def test_lu_singular_row_status(ctx):
     M = np.eye(4, dtype=complex)[None].repeat(2, 0)
     M[1, 2, :] = 0
