@@ -391,8 +391,8 @@ int nrc_plus_one_div_vol(nrc_context* ctx, void* buf, long vol, int prec) {
 
 // The split-BF16 overlap GEMM comes in variants (NRC_OVL_*): the column form (the orbital
 // side builds a real and an imaginary column per orbital, four waves per configuration), and
-// the sink-row form (two rows per sink, split once, staged in LDS for the four configurations
-// of a workgroup) with 2-step or 4-step stages.  A call's variant is the context's tuned
+// the sink-row form (two rows per sink, split once, staged in LDS for the four column tiles of
+// a workgroup) with 2-step or 4-step stages.  A call's variant is the context's tuned
 // choice for (ns, nt, vol, terms), else the default below.
 static const int kOvlVariants[] = {NRC_OVL_COLS, NRC_OVL_SHARED, NRC_OVL_SHARED_DEEP,
                                    NRC_OVL_SHARED_PIPE};

@@ -259,7 +259,8 @@ hipError_t launch_split_bf16(const float2* phi, void* out, int ns, long vol, hip
 size_t overlap_bf16x3_partial_floats(int ns, int nt, long vol, int ncfg);
 // the sink-row form: rows_split holds `parts` planes [part][2 ns][2V] (launch_split_rows),
 // row 2s = (re, im) and row 2s + 1 = (-im, re) of sink s, staged in LDS and shared by the four
-// configurations of a workgroup; form 0: 2-step stages, 2 steps of orbital loads in flight,
+// waves of a workgroup, each a tile of orbital columns (c, t) taken across configurations;
+// form 0: 2-step stages, 2 steps of orbital loads in flight,
 // two workgroups per CU; 1: 4-step stages, 4 in flight, one per CU; 2: 2-step stages, 4 in
 // flight, two per CU
 hipError_t launch_split_rows(const float2* phi, void* out, int ns, long vol, hipStream_t s,
