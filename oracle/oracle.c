@@ -908,7 +908,8 @@ int orc_run_config(const orc_run_desc* d, double* logabs, double* phase, double*
     /* measure: SlaterDet ctor + Run(n) */
     if (d->gather && !xspace && !d->sources && !d->sinks) {
       for (int s = 0; s < N; ++s) {
-        const size_t js = ((size_t)d->moms[3 * s] * L + d->moms[3 * s + 1]) * L + d->moms[3 * s + 2];
+        const int* k = d->moms + 3 * s;
+        const size_t js = ((size_t)k[0] * L + k[1]) * L + k[2];
         for (int q = 0; q < N; ++q) {
           M[2 * (s * N + q)] = psi[2 * ((size_t)q * V + js)];
           M[2 * (s * N + q) + 1] = psi[2 * ((size_t)q * V + js) + 1];
