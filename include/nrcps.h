@@ -305,6 +305,21 @@ int nrc_ensemble_tilt(nrc_ensemble* ens, double* p_plus);  /* the q in use (0.5:
  * the realised (lambda, theta, rho). */
 int nrc_ensemble_set_memory(nrc_ensemble* ens, double lambda, double theta, double rho);
 int nrc_ensemble_memory(nrc_ensemble* ens, double* lambda, double* theta, double* rho);
+/* The memory network (no counterpart in the reference; the same kind of exact importance
+ * sampler): every site carries two memories of its own past field, M_j = 1024 m_j in int16
+ * with the fields-with-memory recurrence for rho_0 = 0.75 and rho_1 = 0.94 (r = 24576, 30802),
+ * and a pointwise network of 8 ReLU units maps s_j = M_j / 1024 to a logit
+ *   a_k = max((W1[k][0] s_0 + W1[k][1] s_1) + b1[k], 0),
+ *   l = ((c_0 s_0 + c_1 s_1) + b2 + sum_k W2[k] a_k) 2^-10   (k in order),
+ * every operation one FP32 rounding without contraction.  Site i of slice t is +1 where output
+ * i % 4 of the Philox4x32-10 block with counter (i / 4, t, c, 4) is below T(l), the learned
+ * proposal's FP32 threshold (q(+1) = T / 2^32 exactly); nrc_ensemble_log_ratio then gives
+ * log r[t][c] = sum over slices t' < t and sites of log(2^31 / f), f = T or 2^32 - T.
+ * params: 35 host floats [W1 (8 x 2), b1 (8), W2 (8), c (2), b2], finite (NRC_ERR_ARG
+ * otherwise), or NULL to return to the plain sampler.  Setting it turns the tilt and the
+ * fields with memory off (and nrc_ensemble_set_memory / _set_tilt turn it off); not while in
+ * a batch, and a batch refuses it like the fields with memory. */
+int nrc_ensemble_set_memnet(nrc_ensemble* ens, const float* params);
 /* log_ratio: [T][C], T the last run's slice count (NRC_ERR_ARG otherwise) */
 int nrc_ensemble_log_ratio(nrc_ensemble* ens, int T, double* log_ratio);
 /* Orbital volumes evolved per configuration: n_orb, or fewer with conjugate partners. */
@@ -437,6 +452,18 @@ int nrc_z2_sums(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, in
  * (NRC_ERR_UNSUPPORTED otherwise). */
 int nrc_z2_memory_eval(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
                        int vol, double lambda, double theta, double rho, double* out_dev);
+/* The memory network (params, 35 host floats) on the plain sampler's fields of
+ * configurations config0 .. config0 + nconf - 1: nrc_z2_memnet_eval writes out_dev[t][c], the
+ * cumulative log(p / q) of slices t' < t, as nrc_z2_memory_eval; nrc_z2_memnet_grad writes
+ * grad_dev[c][35] = sum_t G_dev[t][c] sum_sites d log q(phi) / d params, with q(+1) =
+ * sigmoid(l) in place of T / 2^32 (they agree to ~2e-7), for t < T.  With G[t][c] the
+ * weight of slice t of configuration c in the gradient of the effective-sample fraction
+ * E_p[det^2]^2 / E_p[det^4 p / q], this trains the network on a plain sample.  Both
+ * synchronise; vol <= 32768 (NRC_ERR_UNSUPPORTED otherwise). */
+int nrc_z2_memnet_eval(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
+                       int vol, const float* params, double* out_dev);
+int nrc_z2_memnet_grad(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T,
+                       int vol, const float* params, const double* G_dev, double* grad_dev);
 
 #ifdef __cplusplus
 }

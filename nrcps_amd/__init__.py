@@ -39,6 +39,7 @@ EXPORTS = [
     "nrc_ensemble_set_accuracy", "nrc_batch_captures", "nrc_overlap_noalloc", "nrc_z2_sums",
     "nrc_ensemble_set_tilt", "nrc_ensemble_tilt", "nrc_ensemble_log_ratio",
     "nrc_ensemble_set_memory", "nrc_ensemble_memory", "nrc_z2_memory_eval",
+    "nrc_ensemble_set_memnet", "nrc_z2_memnet_eval", "nrc_z2_memnet_grad",
     "nrc_overlap_set_variant", "nrc_overlap_variant", "nrc_overlap_tune",
     "nrc_ensemble_tune_report", "nrc_ensemble_launch_slices", "nrc_tune_save", "nrc_tune_load",
     "nrc_last_error_detail", "nrc_check_build", "nrc_ensemble_evolve_fused",
@@ -110,6 +111,9 @@ def load_library() -> C.CDLL:
         "nrc_ensemble_memory": [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)],
         "nrc_z2_memory_eval": [vp, u64, u32, i, i, i, C.c_double, C.c_double, C.c_double, vp],
+        "nrc_ensemble_set_memnet": [vp, vp],
+        "nrc_z2_memnet_eval": [vp, u64, u32, i, i, i, vp, vp],
+        "nrc_z2_memnet_grad": [vp, u64, u32, i, i, i, vp, vp, vp],
         "nrc_ensemble_tilt": [vp, C.POINTER(C.c_double)], "nrc_ensemble_log_ratio": [vp, i, dp],
         "nrc_scale_const": [vp, vp, C.c_long, C.c_double, i, i],
         "nrc_norms": [vp, vp, i, C.c_long, i, vp],
@@ -423,6 +427,17 @@ class Ensemble:
         _check(self.lib.nrc_ensemble_set_memory(self.h, float(lam), float(theta), float(rho)),
                "ensemble_set_memory")
         return self.memory()
+
+    def set_memnet(self, params):
+        """The memory network (nrc_ensemble_set_memnet): 35 FP32 parameters, or None for the
+        plain sampler."""
+        if params is None:
+            _check(self.lib.nrc_ensemble_set_memnet(self.h, None), "ensemble_set_memnet")
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (35,):
+            raise ValueError("35 parameters")
+        _check(self.lib.nrc_ensemble_set_memnet(self.h, p.ctypes.data), "ensemble_set_memnet")
 
     def memory(self):
         out = [C.c_double() for _ in range(3)]

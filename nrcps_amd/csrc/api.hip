@@ -730,8 +730,11 @@ struct nrc_ensemble {
   // fields with memory (nrc_ensemble_set_memory): integer parameters of launch_z2_memory
   bool mem_on = false;
   int mem_a = 0, mem_b = 0, mem_r = 0;
-  int16_t* mem = nullptr;        // [C][nw * 32] per-site memory
+  int16_t* mem = nullptr;        // [C][nw * 32] per-site memory (two per site for the network)
   size_t mem_cap = 0;
+  // the memory network (nrc_ensemble_set_memnet): 35 FP32 parameters; mem_on is set with it
+  bool mn_on = false;
+  float* mn_prm = nullptr;
   double* lrinc = nullptr;       // [T][C] log(p / q) of each slice of the last memory run
   size_t lrinc_cap = 0;
   bool last_memory = false;      // the last run drew fields with memory (lrinc is valid)
@@ -755,7 +758,7 @@ static void ens_free(nrc_ensemble* e) {
                   e->rfield, e->words, e->M,
                   e->o_la, e->o_ph, e->o_as, e->o_st, e->g_ptr, e->g_pos, e->g_sink,
                   e->site_of_orb, e->moms_dev, e->sym, e->pgh, e->trap_tab, e->trap_q,
-                  e->tab_x64, e->tab_k64, e->trap_sep, e->mem, e->lrinc};
+                  e->tab_x64, e->tab_k64, e->trap_sep, e->mem, e->lrinc, e->mn_prm};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (e->par_dev) hipFree(e->par_dev);
@@ -1610,7 +1613,7 @@ static int ens_prepare(nrc_ensemble* e, const uint32_t* words_host, int T) {
     NRC_HIP(hipMemcpyAsync(e->words, words_host, need_words * sizeof(uint32_t),
                            hipMemcpyHostToDevice, e->stream));
   if (e->mem_on && !words_host) {
-    const size_t need_mem = (size_t)e->d.n_configs * e->nw * 32;
+    const size_t need_mem = (size_t)e->d.n_configs * e->nw * 32 * (e->mn_on ? 2 : 1);
     if (need_mem > e->mem_cap) {
       NRC_HIP(hipStreamSynchronize(e->stream));
       hipFree(e->mem);
@@ -1667,6 +1670,10 @@ static int ens_enqueue(nrc_ensemble* e, uint64_t seed, uint32_t config0, bool ho
   // Lattice::Refresh for slice tt into dst: the plain sampler's Philox bits, tilted ones with
   // their field sums, or fields with memory with their sampling ratios
   auto refresh = [&](int tt, uint32_t* dst) -> hipError_t {
+    if (e->mn_on)
+      return nrc::launch_z2_memnet(seed, config0, (int)C, (uint32_t)tt, (int)V, e->mn_prm, dst,
+                                   reinterpret_cast<uint32_t*>(e->mem),
+                                   e->lrinc + (size_t)tt * C, s, e->par_dev);
     if (e->mem_on)
       return nrc::launch_z2_memory(seed, config0, (int)C, (uint32_t)tt, NRC_STREAM_MEMORY,
                                    (int)V, e->mem_a, e->mem_b, e->mem_r, dst, e->mem,
@@ -2097,7 +2104,7 @@ extern "C" int nrc_ensemble_set_symmetry(nrc_ensemble* e, int conj) {
 extern "C" int nrc_ensemble_set_tilt(nrc_ensemble* e, double p_plus) {
   if (!e || !(p_plus > 0.0 && p_plus < 1.0)) return NRC_ERR_ARG;
   if (ens_in_batch(e)) return NRC_ERR_BATCHED;
-  e->mem_on = false;  // one importance sampler at a time
+  e->mem_on = e->mn_on = false;  // one importance sampler at a time
   if (p_plus == 0.5) {
     e->tilt_thr = 0;
     return NRC_OK;
@@ -2136,8 +2143,70 @@ extern "C" int nrc_ensemble_set_memory(nrc_ensemble* e, double lambda, double th
   e->mem_b = b;
   e->mem_r = r;
   e->mem_on = a != 0 || b != 0;
+  e->mn_on = false;
   if (e->mem_on) e->tilt_thr = 0;  // one importance sampler at a time
   return NRC_OK;
+}
+
+// ------------------------------------------------------------------ the memory network
+static constexpr int kMnParams = 35;
+
+static bool mn_params_ok(const float* p) {
+  for (int i = 0; i < kMnParams; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+extern "C" int nrc_ensemble_set_memnet(nrc_ensemble* e, const float* params) {
+  if (!e) return NRC_ERR_ARG;
+  if (params && !mn_params_ok(params)) return NRC_ERR_ARG;
+  if (ens_in_batch(e)) return NRC_ERR_BATCHED;
+  if (!params) {
+    if (e->mn_on) e->mem_on = false;
+    e->mn_on = false;
+    return NRC_OK;
+  }
+  if (!e->mn_prm && hipMalloc(&e->mn_prm, kMnParams * sizeof(float)) != hipSuccess) {
+    e->mn_prm = nullptr;
+    return NRC_ERR_NOMEM;
+  }
+  NRC_HIP(hipStreamSynchronize(e->stream));
+  NRC_HIP(hipMemcpy(e->mn_prm, params, kMnParams * sizeof(float), hipMemcpyHostToDevice));
+  e->mn_on = e->mem_on = true;
+  e->mem_a = e->mem_b = 0;
+  e->tilt_thr = 0;  // one importance sampler at a time
+  return NRC_OK;
+}
+
+// the memory network on the plain sampler's fields: grad false, out_dev[t][c] = cumulative
+// log(p / q); grad true, out_dev[c][35] = the gradient of sum_t G[t][c] sum log q
+static int mn_walk(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf, int T, int vol,
+                   const float* params, const double* G_dev, double* out_dev, bool grad) {
+  if (!ctx || !params || !out_dev || (grad && !G_dev) || nconf < 0 || T < 0 || vol < 1)
+    return NRC_ERR_ARG;
+  if (!mn_params_ok(params)) return NRC_ERR_ARG;
+  if (((long)vol + 31) / 32 * 32 > 32768) return NRC_ERR_UNSUPPORTED;
+  if (nconf == 0 || T == 0) return NRC_OK;
+  hipSetDevice(ctx->device);
+  float* prm = (float*)ctx_workspace(ctx, kMnParams * sizeof(float));
+  if (!prm) return NRC_ERR_NOMEM;
+  NRC_HIP(hipMemcpyAsync(prm, params, kMnParams * sizeof(float), hipMemcpyHostToDevice,
+                         ctx->stream));
+  NRC_HIP(nrc::launch_z2_memnet_walk(seed, config0, nconf, T, vol, prm, G_dev, out_dev, grad,
+                                     ctx->stream));
+  NRC_HIP(hipStreamSynchronize(ctx->stream));  // the parameters are read before returning
+  return NRC_OK;
+}
+
+extern "C" int nrc_z2_memnet_eval(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf,
+                                  int T, int vol, const float* params, double* out_dev) {
+  return mn_walk(ctx, seed, config0, nconf, T, vol, params, nullptr, out_dev, false);
+}
+
+extern "C" int nrc_z2_memnet_grad(nrc_context* ctx, uint64_t seed, uint32_t config0, int nconf,
+                                  int T, int vol, const float* params, const double* G_dev,
+                                  double* grad_dev) {
+  return mn_walk(ctx, seed, config0, nconf, T, vol, params, G_dev, grad_dev, true);
 }
 
 extern "C" int nrc_ensemble_memory(nrc_ensemble* e, double* lambda, double* theta,

@@ -189,6 +189,20 @@ hipError_t launch_z2_tilt(uint64_t seed, uint32_t config0, int nconf, uint32_t t
                           hipStream_t s, const uint64_t* par = nullptr);
 // Philox stream of the memory sampler's per-site uniforms
 #define NRC_STREAM_MEMORY 3u
+// Philox stream of the memory network's per-site uniforms
+#define NRC_STREAM_MEMNET 4u
+// The memory network (proposal.hip): 35 FP32 parameters prm (device); the words of slice t,
+// the per-site memories state (one word per site, updated in place; not read at t = 0) and
+// lr[c], the slice's log(p / q)
+hipError_t launch_z2_memnet(uint64_t seed, uint32_t config0, int nconf, uint32_t t, int vol,
+                            const float* prm, uint32_t* words, uint32_t* state, double* lr,
+                            hipStream_t s, const uint64_t* par = nullptr);
+// The memory network on the plain sampler's fields: grad false, out[t][c] = the cumulative
+// log(p / q) for t < T; grad true, out[c][35] = the gradient of sum_t G[t][c] sum log q
+// (vol <= 32768)
+hipError_t launch_z2_memnet_walk(uint64_t seed, uint32_t config0, int nconf, int T, int vol,
+                                 const float* prm, const double* G, double* out, bool grad,
+                                 hipStream_t s);
 // Z2 fields with memory (integer parameters a, b, r; see lu_rng.hip): the words of slice t,
 // the per-site memory mem (int16, 32 per word, updated in place; not read at t = 0) and
 // lr[c], the slice's log(p / q)

@@ -885,6 +885,271 @@ __global__ __launch_bounds__(256) void k_z2_sums(uint64_t seed, uint32_t config0
   if (threadIdx.x == 0) sums[(long)t * nconf + c] = 2 * (red[0] + red[1] + red[2] + red[3]) - vol;
 }
 
+// ------------------------------------------------------------------ memory network
+// A pointwise proposal network over each site's own past field.  Every site carries two
+// decayed memories M_j = 1024 m_j, m_j(t) = sum_{k >= 1} rho_j^(k-1) phi(t - k), in int16
+// fixed point (M_j(0) = 0, M_j(t + 1) = floor((r_j M_j + 2^14) / 2^15) + 1024 phi(t), r_j =
+// round(2^15 rho_j), rho = 0.75 and 0.94), packed in one word per site (M_0 low, M_1 high).
+// With s_j = M_j / 1024 the logit is
+//   a_k = max((W1[k][0] s_0 + W1[k][1] s_1) + b1[k], 0)                    (k < 8)
+//   l   = ((c_0 s_0 + c_1 s_1) + b2 + sum_k W2[k] a_k) 2^-10               (k in order)
+// every operation one FP32 rounding, no contraction; parameters [W1 (16), b1 (8), W2 (8),
+// c (2), b2].  Site i of slice t is +1 where output i % 4 of the Philox block (i / 4, t, c,
+// stream 4) is below T = proposal_threshold(l): q(+1) = T / 2^32 exactly and the site's
+// log(p / q) is log(2^31 / f), f = T or 2^32 - T.  The linear part alone is the fields with
+// memory with two decays; the hidden units let the tilt bend with the memory.
+constexpr int MN_H = 8;
+constexpr int MN_NP = 35;
+constexpr int MN_R0 = 24576;  // round(2^15 0.75)
+constexpr int MN_R1 = 30802;  // round(2^15 0.94)
+constexpr int kMnMaxVol = 32768;
+
+__device__ __forceinline__ float memnet_logit(const float* P, float s0, float s1) {
+#pragma clang fp contract(off)
+  float acc = P[32] * s0;
+  acc = acc + P[33] * s1;
+  acc = acc + P[34];
+#pragma unroll
+  for (int k = 0; k < MN_H; ++k) {
+    float z = P[2 * k] * s0;
+    z = z + P[2 * k + 1] * s1;
+    z = z + P[16 + k];
+    acc = acc + P[24 + k] * fmaxf(z, 0.f);
+  }
+  return acc * 0x1p-10f;
+}
+
+// the logit and the hidden units' inputs, in memnet_logit's order and rounding
+__device__ __forceinline__ float memnet_logit_pre(const float* P, float s0, float s1,
+                                                  float* pre) {
+#pragma clang fp contract(off)
+  float acc = P[32] * s0;
+  acc = acc + P[33] * s1;
+  acc = acc + P[34];
+#pragma unroll
+  for (int k = 0; k < MN_H; ++k) {
+    float z = P[2 * k] * s0;
+    z = z + P[2 * k + 1] * s1;
+    z = z + P[16 + k];
+    pre[k] = z;
+    acc = acc + P[24 + k] * fmaxf(z, 0.f);
+  }
+  return acc * 0x1p-10f;
+}
+
+// the site's two memories after a draw
+__device__ __forceinline__ uint32_t memnet_next(uint32_t st, bool up) {
+  const int m0 = (int16_t)(st & 0xFFFFu), m1 = (int16_t)(st >> 16);
+  const int d = up ? 1024 : -1024;
+  const int n0 = ((MN_R0 * m0 + (1 << 14)) >> 15) + d;
+  const int n1 = ((MN_R1 * m1 + (1 << 14)) >> 15) + d;
+  return (uint32_t)(uint16_t)(int16_t)n0 | ((uint32_t)(uint16_t)(int16_t)n1 << 16);
+}
+
+__device__ __forceinline__ float mn_s0(uint32_t st) {
+  return (float)(int16_t)(st & 0xFFFFu) * (1.f / 1024.f);
+}
+__device__ __forceinline__ float mn_s1(uint32_t st) {
+  return (float)(int16_t)(st >> 16) * (1.f / 1024.f);
+}
+
+// sum over the workgroup (NTH threads) in a fixed order
+template <int NTH>
+__device__ __forceinline__ double mn_block_sum(double v, double* red) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = (red[0] + red[1]) + (red[2] + red[3]);
+  if constexpr (NTH == 512) s += (red[4] + red[5]) + (red[6] + red[7]);
+  __syncthreads();
+  return s;
+}
+
+// The sampler: slice t of configurations config0 .. config0 + nconf - 1, one workgroup per
+// configuration, each thread whole words; state[c] holds the memories, 32 words per field
+// word (not read at t = 0); lr[c] receives the slice's log(p / q).
+__global__ __launch_bounds__(256) void k_z2_memnet(uint64_t seed, uint32_t config0, uint32_t t,
+                                                   int vol, const float* __restrict__ prm,
+                                                   uint32_t* __restrict__ words,
+                                                   uint32_t* __restrict__ state, double* lr,
+                                                   const uint64_t* par) {
+  __shared__ double red[4];
+  if (par) {
+    seed = par[0];
+    config0 = (uint32_t)par[1];
+  }
+  float P[MN_NP];
+#pragma unroll
+  for (int i = 0; i < MN_NP; ++i) P[i] = prm[i];
+  const int c = blockIdx.x;
+  const int nw = (vol + 31) / 32;
+  RatioAcc ra;
+  for (int w = threadIdx.x; w < nw; w += 256) {
+    uint4* sp = reinterpret_cast<uint4*>(state + ((long)c * nw + w) * 32);
+    uint32_t ms[32];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint4 v = t == 0 ? make_uint4(0u, 0u, 0u, 0u) : sp[k];
+      ms[4 * k] = v.x;
+      ms[4 * k + 1] = v.y;
+      ms[4 * k + 2] = v.z;
+      ms[4 * k + 3] = v.w;
+    }
+    const int valid = vol - 32 * w;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int blk = 0; blk < 8; ++blk) {
+      uint32_t ctr[4] = {(uint32_t)(8 * w + blk), t, config0 + (uint32_t)c, NRC_STREAM_MEMNET};
+      philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 4 * blk + q;
+        const uint32_t th = proposal_threshold(memnet_logit(P, mn_s0(ms[i]), mn_s1(ms[i])));
+        const bool up = ctr[q] < th;
+        bits |= (uint32_t)up << i;
+        if (i < valid) ra.add(up ? th : 0u - th);
+        ms[i] = memnet_next(ms[i], up);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      sp[k] = make_uint4(ms[4 * k], ms[4 * k + 1], ms[4 * k + 2], ms[4 * k + 3]);
+    words[(long)c * nw + w] = bits;
+  }
+  const double s = mn_block_sum<256>(ra.value(), red);
+  if (threadIdx.x == 0) lr[c] = s;
+}
+
+// The proposal on the plain sampler's fields (stream 0, word w of slice t = output w % 4 of the
+// block (w / 4, t, c, 0)), one workgroup of NTH threads per configuration walking its memories
+// over t in registers.  GRAD false: out[t][c] = the cumulative log(p / q) of slices 0 .. t - 1.
+// GRAD true: g[c][j] = 2^-10 sum_t G[t][c] sum_sites (up - sigmoid(l)) dl / dP_j, with
+// l / 2^-10 differentiated (the gradient of sum log q with q(+1) = sigmoid(l), the smooth form
+// of T / 2^32); each slice's per-thread FP32 sums go through LDS to thread j, which adds their
+// FP64 total times G[t][c] to parameter j's; G = 0 skips a slice's products.
+template <int WPT, bool GRAD, int NTH>
+__global__ __launch_bounds__(NTH) void k_z2_memnet_walk(uint64_t seed, uint32_t config0, int T,
+                                                        int vol, const float* __restrict__ prm,
+                                                        const double* __restrict__ G,
+                                                        double* __restrict__ out, int nconf) {
+  __shared__ double red[NTH / 64];
+  __shared__ float gs[GRAD ? MN_NP : 1][GRAD ? NTH : 1];
+  float P[MN_NP];
+#pragma unroll
+  for (int i = 0; i < MN_NP; ++i) P[i] = prm[i];
+  const int c = blockIdx.x;
+  const int nw = (vol + 31) / 32;
+  uint32_t ms[WPT][32];
+#pragma unroll
+  for (int k = 0; k < WPT; ++k)
+#pragma unroll
+    for (int j = 0; j < 32; ++j) ms[k][j] = 0u;
+  double run = 0.0, gacc = 0.0;
+  for (int t = 0; t < T; ++t) {
+    if constexpr (!GRAD) {
+      if (threadIdx.x == 0) out[(long)t * nconf + c] = run;
+      if (t == T - 1) break;
+    }
+    const double gt = GRAD ? G[(long)t * nconf + c] : 0.0;
+    RatioAcc ra;
+    float g32[GRAD ? MN_NP : 1];
+#pragma unroll
+    for (int j = 0; j < (GRAD ? MN_NP : 1); ++j) g32[j] = 0.f;
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+      const int w = threadIdx.x + NTH * k;
+      if (w < nw) {
+        uint32_t ctr[4] = {(uint32_t)(w >> 2), (uint32_t)t, config0 + (uint32_t)c, 0u};
+        philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const int q = w & 3;
+        const uint32_t word = q == 0 ? ctr[0] : q == 1 ? ctr[1] : q == 2 ? ctr[2] : ctr[3];
+        const int valid = vol - 32 * w;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+          const bool up = (word >> i) & 1u;
+          const float s0 = mn_s0(ms[k][i]), s1 = mn_s1(ms[k][i]);
+          if constexpr (GRAD) {
+            if (gt != 0.0 && i < valid) {
+              float pre[MN_H];
+              const float l = memnet_logit_pre(P, s0, s1, pre);
+              const float dl = (up ? 1.f : 0.f) - 1.f / (1.f + __expf(-l));
+#pragma unroll
+              for (int h = 0; h < MN_H; ++h) {
+                const float dz = pre[h] > 0.f ? dl * P[24 + h] : 0.f;
+                g32[2 * h] += dz * s0;
+                g32[2 * h + 1] += dz * s1;
+                g32[16 + h] += dz;
+                g32[24 + h] += dl * fmaxf(pre[h], 0.f);
+              }
+              g32[32] += dl * s0;
+              g32[33] += dl * s1;
+              g32[34] += dl;
+            }
+          } else {
+            if (i < valid) {
+              const uint32_t th = proposal_threshold(memnet_logit(P, s0, s1));
+              ra.add(up ? th : 0u - th);
+            }
+          }
+          ms[k][i] = memnet_next(ms[k][i], up);
+        }
+      }
+    }
+    if constexpr (GRAD) {
+      if (gt != 0.0) {  // uniform: one configuration per workgroup
+#pragma unroll
+        for (int j = 0; j < MN_NP; ++j) gs[j][threadIdx.x] = g32[j];
+        __syncthreads();
+        if (threadIdx.x < MN_NP) {
+          double s = 0.0;
+          for (int i = 0; i < NTH; ++i) s += (double)gs[threadIdx.x][i];
+          gacc += gt * s;
+        }
+        __syncthreads();
+      }
+    } else {
+      run += mn_block_sum<NTH>(ra.value(), red);
+    }
+  }
+  if constexpr (GRAD) {
+    if (threadIdx.x < MN_NP) out[(long)c * MN_NP + threadIdx.x] = gacc * 0x1p-10;
+  }
+}
+
+hipError_t launch_z2_memnet(uint64_t seed, uint32_t config0, int nconf, uint32_t t, int vol,
+                            const float* prm, uint32_t* words, uint32_t* state, double* lr,
+                            hipStream_t s, const uint64_t* par) {
+  if (nconf < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_z2_memnet, dim3((unsigned)nconf), dim3(256), 0, s, seed, config0, t, vol,
+                     prm, words, state, lr, par);
+  return hipGetLastError();
+}
+
+hipError_t launch_z2_memnet_walk(uint64_t seed, uint32_t config0, int nconf, int T, int vol,
+                                 const float* prm, const double* G, double* out, bool grad,
+                                 hipStream_t s) {
+  if (nconf < 1 || T < 1) return hipSuccess;
+  if (vol < 1 || ((vol + 31) / 32) * 32 > kMnMaxVol) return hipErrorInvalidValue;
+  const int nw = (vol + 31) / 32;
+  const dim3 g((unsigned)nconf);
+#define NRC_MN_WALK(W, NT)                                                                    \
+  if (grad)                                                                                   \
+    hipLaunchKernelGGL((k_z2_memnet_walk<W, true, NT>), g, dim3(NT), 0, s, seed, config0, T,   \
+                       vol, prm, G, out, nconf);                                              \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_z2_memnet_walk<W, false, NT>), g, dim3(NT), 0, s, seed, config0, T,  \
+                       vol, prm, G, out, nconf)
+  if (nw <= 256) {
+    NRC_MN_WALK(1, 256);
+  } else if (nw <= 512) {
+    NRC_MN_WALK(2, 256);
+  } else {
+    NRC_MN_WALK(2, 512);
+  }
+#undef NRC_MN_WALK
+  return hipGetLastError();
+}
+
 }  // namespace nrc
 
 // ------------------------------------------------------------------ C ABI

@@ -322,6 +322,108 @@ def fit_memory(ctx: Context, la: np.ndarray, seed: int, config0: int, T: int, vo
     return unpack(res.x)
 
 
+# ------------------------------------------------------------------ the memory network
+MEMNET_NP = 35
+
+
+def memnet_init(lam: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Starting parameters of the memory network [W1 (8 x 2), b1 (8), W2 (8), c (2), b2]: the
+    constant tilt lam (q(+1) = (1 + tanh lam) / 2, logit 2 lam = b2 2^-10), zero memory
+    couplings and small random hidden units (their output weights small, so the network
+    starts at the tilt)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MEMNET_NP, np.float32)
+    p[:16] = rng.standard_normal(16) * 0.5
+    p[16:24] = rng.standard_normal(8) * 0.5
+    p[24:32] = rng.standard_normal(8) * 0.05
+    p[34] = 2.0 * lam * 1024.0
+    return p
+
+
+def memnet_log_ratio(ctx: Context, seed: int, config0: int, nconf: int, T: int, vol: int,
+                     params: np.ndarray) -> np.ndarray:
+    """[T][nconf] cumulative log(p / q) of the memory network on the plain sampler's fields
+    of configurations config0 .. (nrc_z2_memnet_eval)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    out = ctx.array((T, nconf), np.float64)
+    _check(ctx.lib.nrc_z2_memnet_eval(ctx.h, seed, config0, nconf, T, vol, p.ctypes.data,
+                                      out.ptr), "z2_memnet_eval")
+    return out.download()
+
+
+def memnet_objective(la: np.ndarray, lr: np.ndarray, ts):
+    """The effective-sample objective of fit_memory (mean over t in ts of
+    log E_p[d2]^2 / E_p[d2^2 p / q]) and its weights G[t'][c] = d objective / d sum log q of
+    slice t' of configuration c: the mean over t in ts, t > t', of softmax_c(4 x_t + lr_t)."""
+    x = la[:, :, 0] if la.ndim == 3 else la
+    T = lr.shape[0]
+    G = np.zeros((T, x.shape[1]))
+    J = 0.0
+    for t in ts:
+        e = 4 * x[t] + lr[t]
+        w = np.exp(e - e.max())
+        J += 2 * _lme(2 * x[t]) - _lme(e)
+        G[:t] += (w / w.sum())[None, :] / len(ts)
+    return J / len(ts), G
+
+
+def memnet_gradient(ctx: Context, seed: int, config0: int, nconf: int, T: int, vol: int,
+                    params: np.ndarray, G: np.ndarray) -> np.ndarray:
+    """sum_c grad[c]: the gradient of sum_t G[t][c] sum log q (nrc_z2_memnet_grad)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    Gd = ctx.to_device(np.ascontiguousarray(G[:T], dtype=np.float64))
+    out = ctx.array((nconf, MEMNET_NP), np.float64)
+    _check(ctx.lib.nrc_z2_memnet_grad(ctx.h, seed, config0, nconf, T, vol, p.ctypes.data,
+                                      Gd.ptr, out.ptr), "z2_memnet_grad")
+    return out.download().sum(axis=0)
+
+
+def fit_memnet(ctx: Context, la: np.ndarray, seed: int, config0: int, T: int, vol: int, ts,
+               p0: np.ndarray, iters: int = 120, step: float = 0.05, nfit: int = 0):
+    """Trains the memory network on a plain sample (la [T][C] of configurations config0 ..;
+    their fields regenerated on the device): Adam ascent of memnet_objective on the first half
+    of the configurations (or of the first nfit), the gradient from nrc_z2_memnet_grad; the
+    parameters kept are those with the best objective on the other half.  Returns them and a
+    record of the training."""
+    if nfit > 0:
+        la = la[:, :nfit]
+    C = la.shape[1]
+    T = min(T, max(ts) + 1)
+    h = C // 2
+    la_tr, la_va = la[:, :h], la[:, h:]
+    p = np.asarray(p0, dtype=np.float64).copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    best = (-np.inf, p.astype(np.float32), -1)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(iters + 1):
+        pf = p.astype(np.float32)
+        lr_va = memnet_log_ratio(ctx, seed, config0 + h, C - h, T, vol, pf)
+        J_va = memnet_objective(la_va, lr_va, ts)[0]
+        if J_va > best[0]:
+            best = (J_va, pf.copy(), it)
+        if it == iters:
+            break
+        lr_tr = memnet_log_ratio(ctx, seed, config0, h, T, vol, pf)
+        J_tr, G = memnet_objective(la_tr, lr_tr, ts)
+        g = memnet_gradient(ctx, seed, config0, h, T, vol, pf, G)
+        hist.append((it, float(J_tr), float(J_va)))
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        mh = m / (1 - b1 ** (it + 1))
+        vh = v / (1 - b2 ** (it + 1))
+        p = p + step * mh / (np.sqrt(vh) + eps)
+    plain = memnet_objective(la_va, np.zeros((T, C - h)), ts)[0]
+    rec = {"iterations": iters, "train_configs": h, "validation_configs": C - h,
+           "best_iteration": best[2], "validation_objective": float(best[0]),
+           "validation_objective_plain": float(plain),
+           "seconds": time.perf_counter() - t0,
+           "history_every_10": [x for x in hist if x[0] % 10 == 0]}
+    return best[1], rec
+
+
 def _tilt_log_ess(lam: np.ndarray, d2: np.ndarray, S: np.ndarray, V: int):
     """log of the effective sample fraction of the tilted proposal q ~ p prod_t' exp(lam_t'
     S_t') (per-site probability of +1: (1 + tanh lam_t') / 2) for the estimator of <d2>,
