@@ -107,7 +107,7 @@ def cpu_baseline(L, N, tau, A, configs, slices, threads, reps=3):
             "cpu_model": _cpu_model(), "cpus_visible": visible, "cpus_in_affinity": affinity,
             "cores_policy": ("one stream per CPU the process may use"
                              if threads >= affinity else
-                             f"one stream per core of the GPU box's CPU share for one GPU "
+                             f"one stream per core of the host's CPU share for one GPU "
                              f"({threads} of {affinity} CPUs in the affinity mask)"),
             "per_core_value": value / threads,
             "value_if_fft_at_numpy_speed": value * cal["baseline_scale_with_numpy_fft"],
@@ -866,7 +866,7 @@ def main():
                     help="CPU baseline: slices per configuration")
     ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline: best of this many")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="CPU baseline streams (0: the GPU box's CPU share, 16)")
+                    help="CPU baseline streams (0: the host's CPU share for one GPU, 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven-shape measurement")

@@ -1,5 +1,6 @@
-"""Oracle of the memory-network proposal (test infrastructure: only tests/, the smoke check and
-bench.py's CPU leg use it, as the checker).
+"""Oracle of the memory-network proposal.
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP memory-network kernels; only tests/ import it.
 
 No counterpart in the reference: its fields are uniform Z2 draws (src/alg/lattice.C:81-86);
 the proposal is an importance sampler for them, exact like the shifted Gaussian of
