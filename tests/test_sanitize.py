@@ -1,5 +1,5 @@
-"""AddressSanitizer and UndefinedBehaviorSanitizer builds on the CPU (the sanitizer build of
-SURVEY §5): the FP64 oracle with a driver that runs every one of its entries
+"""AddressSanitizer and UndefinedBehaviorSanitizer builds on the CPU: the FP64 oracle with a
+driver that runs every one of its entries
 (tests/sanitize/oracle_drive.c), and the host-only half of the class layer -- argument files,
 job parameters, Dispersion, Momenta, ranlxs with its state files, Gaussian draws, special
 functions -- with tests/sanitize/host_drive.cpp.  Any out-of-bounds access, use after free,
